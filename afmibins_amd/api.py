@@ -1,0 +1,298 @@
+"""Python mirror of the reference's ib / mg / ns interfaces over libafm.
+
+Device memory is plain torch tensors (float64, contiguous, on the context's
+GPU); every call goes through the C ABI of include/afm.h and is ordered on
+torch's current stream.  Names follow the reference:
+
+  Context.interp / Context.spread   ib::grid::interpolate (ib/grid.h:68-84)
+  MGSolver(ctx, tol, nu1, nu2).solve(rhs)
+                                    mg::solver<G>::solve (mg/solver.h:218-237)
+  NSSolver(ctx, mu, rho, dt, tol, u0, v0).step(fx, fy)
+                                    ns::solver<G>::step (ns/solver.h:90-140)
+  IBSolver(ns, X, R, k, ds).step(n) the build's IB time step
+"""
+import ctypes as C
+
+import torch
+
+from . import _lib
+from ._lib import check
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """A 2-D doubly periodic unit-square MAC grid with n x n cells."""
+
+    def __init__(self, n, device=0):
+        L = _lib.load()
+        self.device = torch.device("cuda", device)
+        g = _lib.Geom(2, n, 1, 1)
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            check(L.afm_ctx_create(C.byref(g), device, C.byref(h)), "afm_ctx_create")
+        self._h = h
+        self.n = n
+        nx, ny, lv, lc = C.c_uint(), C.c_uint(), C.c_uint(), C.c_uint()
+        check(L.afm_ctx_info(h, C.byref(nx), C.byref(ny), C.byref(lv), C.byref(lc)), "afm_ctx_info")
+        self.nx, self.ny, self.levels, self.coarse_level = nx.value, ny.value, lv.value, lc.value
+        self._bind_stream()
+
+    # -- plumbing ---------------------------------------------------------
+    def _bind_stream(self):
+        s = torch.cuda.current_stream(self.device)
+        check(_lib.load().afm_set_stream(self._h, C.c_void_p(s.cuda_stream)), "afm_set_stream")
+        self._stream = s.cuda_stream
+
+    def _call(self, name, *args):
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        if s != self._stream:
+            self._bind_stream()
+        return check(getattr(_lib.load(), name)(self._h, *args), name)
+
+    def _f(self, t, shape=None, name="tensor"):
+        if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or t.device != self.device \
+                or not t.is_contiguous():
+            raise TypeError(f"{name} must be a contiguous float64 tensor on {self.device}")
+        if shape is not None and tuple(t.shape) != tuple(shape):
+            raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+        return t
+
+    def field(self, level=None):
+        n = self.n if level is None else (1 << level)
+        return torch.zeros((n, n), dtype=torch.float64, device=self.device)
+
+    def sync(self):
+        torch.cuda.synchronize(self.device)
+        check(_lib.load().afm_sync(self._h), "afm_sync")
+
+    def close(self):
+        if self._h:
+            torch.cuda.synchronize(self.device)
+            _lib.load().afm_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- ib ---------------------------------------------------------------
+    def delta(self, r):
+        r = self._f(r, name="r")
+        out = torch.empty_like(r)
+        self._call("afm_delta", _ptr(r), r.numel(), _ptr(out))
+        return out
+
+    def interp(self, comp, field, X):
+        """U_k = J(X) field for MAC component comp (0 = u, 1 = v)."""
+        self._f(field, (self.ny, self.nx), "field")
+        X = self._f(X, name="X")
+        m = X.shape[0]
+        U = torch.empty(m, dtype=torch.float64, device=self.device)
+        self._call("afm_interp", comp, _ptr(field), _ptr(X), m, _ptr(U))
+        return U
+
+    def spread(self, comp, F, ds, X, out=None):
+        """f = S(X) (F ds) on MAC component comp, ascending marker order per cell."""
+        X = self._f(X, name="X")
+        m = X.shape[0]
+        self._f(F, (m,), "F")
+        self._f(ds, (m,), "ds")
+        if out is None:
+            out = torch.empty((self.ny, self.nx), dtype=torch.float64, device=self.device)
+        self._f(out, (self.ny, self.nx), "out")
+        self._call("afm_spread", comp, _ptr(F), _ptr(ds), _ptr(X), m, _ptr(out))
+        return out
+
+    # -- mg ---------------------------------------------------------------
+    def laplacian(self, x, level=None):
+        level = self.levels if level is None else level
+        n = 1 << level
+        self._f(x, (n, n), "x")
+        y = torch.empty_like(x)
+        self._call("afm_laplacian_apply", level, _ptr(x), _ptr(y))
+        return y
+
+    def mg_smooth(self, b, x=None, sweeps=2, level=None):
+        level = self.levels if level is None else level
+        out = torch.empty_like(b)
+        self._call("afm_mg_smooth", level, _ptr(x) if x is not None else None, _ptr(b), _ptr(out), sweeps,
+                   1 if x is None else 0)
+        return out
+
+    def mg_restrict(self, fine, level=None):
+        level = self.levels if level is None else level
+        n = 1 << level
+        out = torch.empty((n // 2, n // 2), dtype=torch.float64, device=self.device)
+        self._call("afm_mg_restrict", level, _ptr(fine), _ptr(out))
+        return out
+
+    def mg_prolong(self, coarse, level=None):
+        level = self.levels if level is None else level
+        n = 1 << level
+        out = torch.empty((n, n), dtype=torch.float64, device=self.device)
+        self._call("afm_mg_prolong", level, _ptr(coarse), _ptr(out))
+        return out
+
+    def mg_vcycle(self, rhs, nu1=2, nu2=2):
+        self._f(rhs, (self.ny, self.nx), "rhs")
+        self._call("afm_mg_vcycle", _ptr(rhs), nu1, nu2)
+        return rhs
+
+    def mg_solve(self, rhs, tol=1e-10, tol_mode=1, nu1=2, nu2=2, precision=0, max_cycles=200):
+        """In place: rhs <- solution.  Returns (cycles, final RMS residual)."""
+        self._f(rhs, (self.ny, self.nx), "rhs")
+        cyc, rms = C.c_int(), C.c_double()
+        self._call("afm_mg_solve", _ptr(rhs), tol, tol_mode, nu1, nu2, precision, max_cycles, C.byref(cyc),
+                   C.byref(rms))
+        return cyc.value, rms.value
+
+    # -- ns operators -----------------------------------------------------
+    def helmholtz(self, scale, x):
+        self._f(x, (self.ny, self.nx), "x")
+        y = torch.empty_like(x)
+        self._call("afm_helmholtz_apply", scale, _ptr(x), _ptr(y))
+        return y
+
+    def helmholtz_cg(self, scale, b, x, tol=1e-10, tol_mode=1, max_iters=10000):
+        self._f(b, (self.ny, self.nx), "b")
+        self._f(x, (self.ny, self.nx), "x")
+        it, res = C.c_int(), C.c_double()
+        self._call("afm_helmholtz_cg", scale, _ptr(b), _ptr(x), tol, tol_mode, max_iters, C.byref(it), C.byref(res))
+        return it.value, res.value
+
+    def div(self, u, v):
+        d = torch.empty_like(u)
+        self._call("afm_div", _ptr(self._f(u, (self.ny, self.nx), "u")), _ptr(self._f(v, (self.ny, self.nx), "v")),
+                   _ptr(d))
+        return d
+
+    def grad_sub(self, p, u, v):
+        self._call("afm_grad_sub", _ptr(p), _ptr(u), _ptr(v))
+        return u, v
+
+    def advect(self, u, v):
+        ua, va = torch.empty_like(u), torch.empty_like(v)
+        self._call("afm_advect", _ptr(u), _ptr(v), _ptr(ua), _ptr(va))
+        return ua, va
+
+
+class MGSolver:
+    """mg::solver<G>(tol, nu1, nu2) (mg/solver.h:60-62); tol is the
+    reference's absolute RMS threshold unless tol_mode=1 (relative)."""
+
+    def __init__(self, ctx, tol, nu1=2, nu2=2, tol_mode=0, precision=0, max_cycles=1000):
+        self.ctx, self.tol, self.nu1, self.nu2 = ctx, tol, nu1, nu2
+        self.tol_mode, self.precision, self.max_cycles = tol_mode, precision, max_cycles
+        self.last_rms = None
+
+    def solve(self, rhs):
+        cyc, rms = self.ctx.mg_solve(rhs, self.tol, self.tol_mode, self.nu1, self.nu2, self.precision,
+                                     self.max_cycles)
+        self.last_rms = rms
+        return cyc
+
+
+class NSSolver:
+    """ns::solver<G>(mu, rho, dt, tol, u0, v0, w0) (ns/solver.h:61-72) on the
+    periodic plane: u0 and v0 are aliased and updated in place by step()."""
+
+    def __init__(self, ctx, mu, rho, dt, tol, u0, v0, tol_mode=1, precision=0):
+        self.ctx = ctx
+        ctx._f(u0, (ctx.ny, ctx.nx), "u0")
+        ctx._f(v0, (ctx.ny, ctx.nx), "v0")
+        self.u, self.v = u0, v0
+        h = C.c_void_p()
+        ctx._call("afm_ns_create", mu, rho, dt, tol, tol_mode, precision, _ptr(u0), _ptr(v0), C.byref(h))
+        self._h = h
+        self.mu, self.rho, self.dt, self.tol = mu, rho, dt, tol
+
+    def _lib(self, name, *args):
+        self.ctx._call  # keep the stream binding fresh
+        s = torch.cuda.current_stream(self.ctx.device).cuda_stream
+        if s != self.ctx._stream:
+            self.ctx._bind_stream()
+        return check(getattr(_lib.load(), name)(self._h, *args), name)
+
+    def step(self, fx, fy):
+        self.ctx._f(fx, (self.ctx.ny, self.ctx.nx), "fx")
+        self.ctx._f(fy, (self.ctx.ny, self.ctx.nx), "fy")
+        self._lib("afm_ns_step", _ptr(fx), _ptr(fy))
+
+    def project(self, u, v):
+        self._lib("afm_ns_project", _ptr(u), _ptr(v))
+
+    def get_p(self):
+        p = torch.empty_like(self.u)
+        self._lib("afm_ns_get_p", _ptr(p))
+        return p
+
+    def stats(self):
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self._lib("afm_ns_stats", C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        return {"cycles": a.value, "cg_u": b.value, "cg_v": c.value, "have_half_stepped": bool(d.value)}
+
+    def close(self):
+        if self._h:
+            torch.cuda.synchronize(self.ctx.device)
+            _lib.load().afm_ns_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class IBSolver:
+    """Tethered markers coupled to an NSSolver (see include/afm.h)."""
+
+    def __init__(self, ns, X, R, k, ds):
+        ctx = ns.ctx
+        X = ctx._f(X, name="X")
+        m = X.shape[0]
+        ctx._f(R, (m, 2), "R")
+        ctx._f(k, (m,), "k")
+        ctx._f(ds, (m,), "ds")
+        self.ns, self.m = ns, m
+        h = C.c_void_p()
+        check(_lib.load().afm_ib_create(ns._h, m, _ptr(X), _ptr(R), _ptr(k), _ptr(ds), C.byref(h)), "afm_ib_create")
+        self._h = h
+
+    def step(self, steps=1):
+        ctx = self.ns.ctx
+        if torch.cuda.current_stream(ctx.device).cuda_stream != ctx._stream:
+            ctx._bind_stream()
+        check(_lib.load().afm_ib_step(self._h, steps), "afm_ib_step")
+
+    def markers(self):
+        X = torch.empty((self.m, 2), dtype=torch.float64, device=self.ns.ctx.device)
+        check(_lib.load().afm_ib_markers(self._h, _ptr(X)), "afm_ib_markers")
+        return X
+
+    def close(self):
+        if self._h:
+            torch.cuda.synchronize(self.ns.ctx.device)
+            _lib.load().afm_ib_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def delta_host(r, n):
+    """The build's delta evaluated by libafm on the CPU (numpy float64 in/out)."""
+    import numpy as np
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.empty_like(r)
+    check(_lib.load().afm_delta_host(r.ctypes.data_as(C.c_void_p), r.size, n, out.ctypes.data_as(C.c_void_p)),
+          "afm_delta_host")
+    return out

@@ -1,0 +1,356 @@
+// afm_capi.hip -- the extern "C" boundary of libafm (declared in include/afm.h).
+#include "afm_device.h"
+#include "../../include/afm.h"
+
+namespace afm {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+Ctx::~Ctx()
+{
+	if (device >= 0) (void) hipSetDevice(device);
+	for (void* p : owned) (void) hipFree(p);
+	if (rec) (void) hipFree(rec);
+	if (bin_list) (void) hipFree(bin_list);
+	if (hctl) (void) hipHostFree(hctl);
+	if (own_stream) (void) hipStreamDestroy(own_stream);
+}
+
+int Ctx::alloc(void** p, size_t bytes)
+{
+	AFM_HIP(hipMalloc(p, bytes));
+	owned.push_back(*p);
+	return 0;
+}
+
+// defined in afm_ns.hip
+struct NS;
+struct IB;
+int ns_create(Ctx*, double, double, double, double, int, int, double*, double*, NS**);
+int ns_step(NS*, const double*, const double*);
+int ns_project(NS*, double*, double*);
+int ns_get_p(NS*, double*);
+int ib_create(NS*, long, const double*, const double*, const double*, const double*, IB**);
+int ib_step(IB*);
+int op_div(Ctx*, const double*, const double*, double*);
+int op_grad_sub(Ctx*, const double*, double*, double*);
+int op_advect(Ctx*, const double*, const double*, double*, double*);
+int op_helm_apply(Ctx*, double, const double*, double*);
+struct CGWork;
+int op_cg_single(Ctx*, double, const double*, double*, double, int, int, int*, double*);
+int ns_stats(NS*, int*, int*, int*, int*);
+int ns_old(NS*, const double**, const double**);
+Ctx* ns_ctx(NS*);
+void ns_free(NS*);
+Ctx* ib_ctx(IB*);
+int ib_markers(IB*, double*);
+void ib_free(IB*);
+
+}  // namespace afm
+
+using namespace afm;
+
+struct afm_ns { NS* s; };
+struct afm_ib { IB* b; };
+
+#define CHECK_ARG(cond, msg) do { if (!(cond)) { set_error(msg); return -1; } } while (0)
+
+static int use(Ctx* c)
+{
+	AFM_HIP(hipSetDevice(c->device));
+	return 0;
+}
+
+extern "C" {
+
+const char* afm_last_error(void) { return last_error(); }
+
+int afm_ctx_create(const afm_geom* g, int device, afm_ctx** out)
+{
+	CHECK_ARG(g && out, "null argument");
+	CHECK_ARG(g->dim == 2, "only dim == 2 (doubly periodic plane) is built");
+	CHECK_ARG(g->width == 1 && g->depth == 1, "only the unit square (width = depth = 1) is built");
+	CHECK_ARG(g->n >= 8 && (g->n & (g->n - 1)) == 0, "n must be a power of two >= 8");
+	CHECK_ARG(g->n <= 16384, "n must be <= 16384");
+	int ndev = 0;
+	AFM_HIP(hipGetDeviceCount(&ndev));
+	CHECK_ARG(device >= 0 && device < ndev, "device index out of range");
+	AFM_HIP(hipSetDevice(device));
+	afm_ctx* c = new afm_ctx();
+	c->device = device;
+	c->n = g->n; c->W = g->width; c->D = g->depth;
+	c->nx = g->width * g->n; c->ny = g->depth * g->n;
+	c->N = (size_t) c->nx * c->ny;
+	int rc = 0;
+	auto fail = [&](int r) { delete c; return r; };
+	if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) { set_error("hipStreamCreate failed"); return fail(-3); }
+	c->stream = c->own_stream;
+	if ((rc = c->alloc((void**) &c->ctl, sizeof(SolveCtl) * NCTL))) return fail(rc);
+	if (hipHostMalloc((void**) &c->hctl, sizeof(SolveCtl) * NCTL, hipHostMallocDefault) != hipSuccess) { set_error("hipHostMalloc failed"); return fail(-3); }
+	if ((rc = c->alloc((void**) &c->partials, sizeof(double) * MAX_RED_BLOCKS * 4))) return fail(rc);
+	if (hipMemsetAsync(c->ctl, 0, sizeof(SolveCtl) * NCTL, c->stream) != hipSuccess) { set_error("memset failed"); return fail(-3); }
+	c->ntx = (int) ((c->nx + 15) / 16); c->nty = (int) ((c->ny + 15) / 16);
+	if ((rc = c->alloc((void**) &c->bins, sizeof(int) * (6 * c->ntx * c->nty + 1)))) return fail(rc);
+	if ((rc = mg_setup(c))) return fail(rc);
+	if (hipStreamSynchronize(c->stream) != hipSuccess) { set_error("sync failed"); return fail(-3); }
+	*out = c;
+	return 0;
+}
+
+int afm_ctx_destroy(afm_ctx* c)
+{
+	if (!c) return 0;
+	hipSetDevice(c->device);
+	hipStreamSynchronize(c->stream);
+	delete c;
+	return 0;
+}
+
+int afm_ctx_info(afm_ctx* c, unsigned* nx, unsigned* ny, unsigned* levels, unsigned* coarse_level)
+{
+	CHECK_ARG(c, "null context");
+	if (nx) *nx = c->nx;
+	if (ny) *ny = c->ny;
+	if (levels) *levels = c->levels;
+	if (coarse_level) *coarse_level = c->lc;
+	return 0;
+}
+
+int afm_set_stream(afm_ctx* c, void* stream)
+{
+	CHECK_ARG(c, "null context");
+	AFM_TRY(use(c));
+	AFM_HIP(hipStreamSynchronize(c->stream));
+	c->stream = stream ? (hipStream_t) stream : c->own_stream;
+	return 0;
+}
+
+int afm_sync(afm_ctx* c)
+{
+	CHECK_ARG(c, "null context");
+	AFM_TRY(use(c));
+	AFM_HIP(hipStreamSynchronize(c->stream));
+	hipDeviceSynchronize();
+	int err = 0;
+	AFM_HIP(hipMemcpy(&err, &c->ctl[3].err, sizeof(int), hipMemcpyDeviceToHost));
+	if (err) {
+		int zero = 0;
+		hipMemcpy(&c->ctl[3].err, &zero, sizeof(int), hipMemcpyHostToDevice);
+		set_error("spreading: more than 4096 candidate markers around one 16x16 tile");
+		return -1;
+	}
+	return 0;
+}
+
+int afm_delta(afm_ctx* c, const double* r, long count, double* out)
+{
+	CHECK_ARG(c && (count == 0 || (r && out)), "null argument");
+	AFM_TRY(use(c));
+	return launch_delta(c, r, count, c->n, out);
+}
+
+int afm_delta_host(const double* r, long count, unsigned n, double* out)
+{
+	CHECK_ARG(count == 0 || (r && out), "null argument");
+	CHECK_ARG(n >= 1 && (n & (n - 1)) == 0, "n must be a power of two");
+	for (long i = 0; i < count; ++i) out[i] = delta(r[i], n);
+	return 0;
+}
+
+int afm_interp(afm_ctx* c, int comp, const double* field, const double* X, long m, double* U)
+{
+	CHECK_ARG(c && (m == 0 || (field && X && U)), "null argument");
+	CHECK_ARG(comp == 0 || comp == 1, "comp must be 0 (u) or 1 (v)");
+	CHECK_ARG(m >= 0, "m must be >= 0");
+	AFM_TRY(use(c));
+	return launch_interp(c, comp, field, X, m, U);
+}
+
+int afm_spread(afm_ctx* c, int comp, const double* F, const double* ds, const double* X, long m, double* f)
+{
+	CHECK_ARG(c && f && (m == 0 || (F && ds && X)), "null argument");
+	CHECK_ARG(comp == 0 || comp == 1, "comp must be 0 (u) or 1 (v)");
+	CHECK_ARG(m >= 0 && m < (1l << 30), "m out of range");
+	AFM_TRY(use(c));
+	return launch_spread(c, 1 << comp, comp == 0 ? F : nullptr, comp == 1 ? F : nullptr, ds, X, nullptr, nullptr, m,
+	                     comp == 0 ? f : nullptr, comp == 1 ? f : nullptr);
+}
+
+int afm_laplacian_apply(afm_ctx* c, unsigned level, const double* x, double* y)
+{
+	CHECK_ARG(c && x && y, "null argument");
+	AFM_TRY(use(c));
+	return mg_apply_laplacian(c, level, x, y);
+}
+
+int afm_mg_smooth(afm_ctx* c, unsigned level, const double* x_in, const double* b, double* x_out, int sweeps,
+                  int zero_init)
+{
+	CHECK_ARG(c && b && x_out && (zero_init || x_in), "null argument");
+	CHECK_ARG(x_in != x_out, "x_out must differ from x_in");
+	AFM_TRY(use(c));
+	return mg_smooth(c, level, x_in, b, x_out, sweeps, zero_init);
+}
+
+int afm_mg_restrict(afm_ctx* c, unsigned level, const double* fine, double* coarse)
+{
+	CHECK_ARG(c && fine && coarse, "null argument");
+	AFM_TRY(use(c));
+	return mg_restrict(c, level, fine, coarse);
+}
+
+int afm_mg_prolong(afm_ctx* c, unsigned level, const double* coarse, double* fine)
+{
+	CHECK_ARG(c && fine && coarse, "null argument");
+	AFM_TRY(use(c));
+	return mg_prolong(c, level, coarse, fine);
+}
+
+int afm_mg_vcycle(afm_ctx* c, double* rhs, int nu1, int nu2)
+{
+	CHECK_ARG(c && rhs, "null argument");
+	CHECK_ARG(nu1 >= 1 && nu1 <= 2 && nu2 >= 1 && nu2 <= 2, "nu1, nu2 must be 1 or 2");
+	AFM_TRY(use(c));
+	return mg_vcycle(c, rhs, nu1, nu2);
+}
+
+int afm_mg_solve(afm_ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, int precision, int max_cycles,
+                 int* cycles, double* final_rms)
+{
+	CHECK_ARG(c && rhs, "null argument");
+	CHECK_ARG(tol_mode == 0 || tol_mode == 1, "tol_mode must be 0 or 1");
+	CHECK_ARG(tol >= 0 && max_cycles >= 0, "tol and max_cycles must be >= 0");
+	AFM_TRY(use(c));
+	return mg_solve(c, rhs, tol, tol_mode, nu1, nu2, precision, max_cycles, cycles, final_rms);
+}
+
+int afm_helmholtz_apply(afm_ctx* c, double scale, const double* x, double* y)
+{
+	CHECK_ARG(c && x && y, "null argument");
+	AFM_TRY(use(c));
+	return op_helm_apply(c, scale, x, y);
+}
+
+int afm_helmholtz_cg(afm_ctx* c, double scale, const double* b, double* x, double tol, int tol_mode, int max_iters,
+                     int* iters, double* final_norm)
+{
+	CHECK_ARG(c && b && x, "null argument");
+	CHECK_ARG(tol_mode == 0 || tol_mode == 1, "tol_mode must be 0 or 1");
+	AFM_TRY(use(c));
+	return op_cg_single(c, scale, b, x, tol, tol_mode, max_iters, iters, final_norm);
+}
+
+int afm_div(afm_ctx* c, const double* u, const double* v, double* d)
+{
+	CHECK_ARG(c && u && v && d, "null argument");
+	AFM_TRY(use(c));
+	return op_div(c, u, v, d);
+}
+
+int afm_grad_sub(afm_ctx* c, const double* p, double* u, double* v)
+{
+	CHECK_ARG(c && p && u && v, "null argument");
+	AFM_TRY(use(c));
+	return op_grad_sub(c, p, u, v);
+}
+
+int afm_advect(afm_ctx* c, const double* u, const double* v, double* ua, double* va)
+{
+	CHECK_ARG(c && u && v && ua && va, "null argument");
+	AFM_TRY(use(c));
+	return op_advect(c, u, v, ua, va);
+}
+
+int afm_ns_create(afm_ctx* c, double mu, double rho, double dt, double tol, int tol_mode, int precision, double* u0,
+                  double* v0, afm_ns** out)
+{
+	CHECK_ARG(c && u0 && v0 && out, "null argument");
+	CHECK_ARG(rho > 0 && dt > 0 && mu >= 0 && tol >= 0, "need rho > 0, dt > 0, mu >= 0, tol >= 0");
+	CHECK_ARG(tol_mode == 0 || tol_mode == 1, "tol_mode must be 0 or 1");
+	CHECK_ARG(precision == 0 || precision == 1, "precision must be 0 or 1");
+	AFM_TRY(use(c));
+	NS* s = nullptr;
+	AFM_TRY(ns_create(c, mu, rho, dt, tol, tol_mode, precision, u0, v0, &s));
+	*out = new afm_ns{s};
+	return 0;
+}
+
+int afm_ns_step(afm_ns* s, const double* fx, const double* fy)
+{
+	CHECK_ARG(s && fx && fy, "null argument");
+	AFM_TRY(use(ns_ctx(s->s)));
+	return ns_step(s->s, fx, fy);
+}
+
+int afm_ns_project(afm_ns* s, double* u, double* v)
+{
+	CHECK_ARG(s && u && v, "null argument");
+	AFM_TRY(use(ns_ctx(s->s)));
+	return ns_project(s->s, u, v);
+}
+
+int afm_ns_get_p(afm_ns* s, double* p)
+{
+	CHECK_ARG(s && p, "null argument");
+	AFM_TRY(use(ns_ctx(s->s)));
+	return ns_get_p(s->s, p);
+}
+
+int afm_ns_stats(afm_ns* s, int* cyc, int* cgu, int* cgv, int* half)
+{
+	CHECK_ARG(s, "null argument");
+	return ns_stats(s->s, cyc, cgu, cgv, half);
+}
+
+int afm_ns_old(afm_ns* s, const double** uo, const double** vo)
+{
+	CHECK_ARG(s && uo && vo, "null argument");
+	return ns_old(s->s, uo, vo);
+}
+
+int afm_ns_destroy(afm_ns* s)
+{
+	if (!s) return 0;
+	ns_free(s->s);
+	delete s;
+	return 0;
+}
+
+int afm_ib_create(afm_ns* s, long m, const double* X, const double* R, const double* k, const double* ds,
+                  afm_ib** out)
+{
+	CHECK_ARG(s && out && (m == 0 || (X && R && k && ds)), "null argument");
+	CHECK_ARG(m >= 0 && m < (1l << 30), "m out of range");
+	AFM_TRY(use(ns_ctx(s->s)));
+	IB* b = nullptr;
+	AFM_TRY(ib_create(s->s, m, X, R, k, ds, &b));
+	*out = new afm_ib{b};
+	return 0;
+}
+
+int afm_ib_step(afm_ib* b, int steps)
+{
+	CHECK_ARG(b && steps >= 0, "bad argument");
+	AFM_TRY(use(ib_ctx(b->b)));
+	for (int i = 0; i < steps; ++i) AFM_TRY(ib_step(b->b));
+	return 0;
+}
+
+int afm_ib_markers(afm_ib* b, double* X_out)
+{
+	CHECK_ARG(b && X_out, "null argument");
+	AFM_TRY(use(ib_ctx(b->b)));
+	return ib_markers(b->b, X_out);
+}
+
+int afm_ib_destroy(afm_ib* b)
+{
+	if (!b) return 0;
+	ib_free(b->b);
+	delete b;
+	return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,188 @@
+// afm_device.h -- device helpers: deterministic reductions, the periodic
+// 5-point stencil in the reference's CSR summation order, and the 4x4 IB
+// window.  Included by the .hip translation units only.
+#pragma once
+#include "afm_internal.h"
+
+namespace afm {
+
+// ------------------------------------------------------------ reduce ----
+// Fixed-tree block sum: wave shuffle tree, then wave partials summed in
+// order by thread 0.  The result is valid in thread 0 only.
+template<int NV>
+__device__ inline void block_sum(double (&v)[NV])
+{
+	__shared__ double wsum[NV][16];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+	const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+	const int lane2 = tid & 63, wave2 = tid >> 6;
+	(void) lane; (void) wave;
+#pragma unroll
+	for (int j = 0; j < NV; ++j) {
+		double s = v[j];
+		for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+		if (lane2 == 0) wsum[j][wave2] = s;
+	}
+	__syncthreads();
+	if (tid == 0) {
+#pragma unroll
+		for (int j = 0; j < NV; ++j) {
+			double s = 0.0;
+			for (int w = 0; w < nw; ++w) s += wsum[j][w];
+			v[j] = s;
+		}
+	}
+	__syncthreads();
+}
+
+// Writes this block's NV partial sums, then returns true in the one block
+// that arrives last; that block receives the NV totals (summed over blocks in
+// index order, so the value does not depend on arrival order) in tot[].
+// The reduction group is the blocks sharing blockIdx.y (so one launch can
+// carry independent reductions, one per y slice, each with its own ticket and
+// partials slice).
+template<int NV>
+__device__ inline bool reduce_last(double (&v)[NV], double* partials, unsigned* ticket, double (&tot)[NV])
+{
+	__shared__ int am_last;
+	const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+	const int nthr = blockDim.x * blockDim.y;
+	const unsigned nblk = gridDim.x;
+	const unsigned bid = blockIdx.x;
+	block_sum<NV>(v);
+	if (tid == 0) {
+#pragma unroll
+		for (int j = 0; j < NV; ++j) partials[(size_t) j * MAX_RED_BLOCKS + bid] = v[j];
+		__threadfence();
+		unsigned t = atomicAdd(ticket, 1u);
+		am_last = (t == nblk - 1);
+	}
+	__syncthreads();
+	if (!am_last) return false;
+	__threadfence();
+	double s[NV];
+#pragma unroll
+	for (int j = 0; j < NV; ++j) {
+		s[j] = 0.0;
+		for (unsigned b = tid; b < nblk; b += nthr) s[j] += __builtin_nontemporal_load(&partials[(size_t) j * MAX_RED_BLOCKS + b]);
+	}
+	block_sum<NV>(s);
+	if (tid == 0) {
+#pragma unroll
+		for (int j = 0; j < NV; ++j) tot[j] = s[j];
+		*ticket = 0u;
+	}
+	return true;
+}
+
+// ------------------------------------------------------------ stencil ---
+// Periodic 5-point stencil (nx, ny >= 3), terms in ascending column order as
+// stored by mg/laplacian.h:97-150 (and ns/helmholtz.h:119-172) with ny == 1:
+//   N(wrap), S, E(wrap), W, [diag], E, W(wrap), N, S(wrap)
+// sum starts from 0 and adds a*x per stored entry (dt/matrix.h:87-91).
+template<typename T, bool DIAG>
+__device__ inline T stencil5(T off, T dg, T c, T e, T w, T nn, T s, bool we, bool ww, bool wn, bool ws)
+{
+	T v = T(0);
+	if (wn) v += off * nn;
+	if (!ws) v += off * s;
+	if (we) v += off * e;
+	if (!ww) v += off * w;
+	if (DIAG) v += dg * c;
+	if (!we) v += off * e;
+	if (ww) v += off * w;
+	if (!wn) v += off * nn;
+	if (ws) v += off * s;
+	return v;
+}
+
+// General row of the 2-D laplacian (any nx, ny >= 1), mg/laplacian.h:50-150
+// with ny(reference) == 1 and the reference's nx/nz == 1, 2 folding.
+struct LapRow {
+	int col[5];
+	double a[5];
+	int cnt;
+	double diag;
+};
+
+__device__ inline void lap_row(int x, int y, int nx, int ny, double off, LapRow& r)
+{
+	double diagonal = -6.0 * off, left = off, right = off, front = off, back = off;
+	if (nx == 1) { diagonal += left + right; left = right = 0; }
+	else if (nx == 2) { if (x == 0) { right += left; left = 0; } else { left += right; right = 0; } }
+	diagonal += off;   // the reference's y axis folded (ny == 1: bottom and top)
+	diagonal += off;
+	if (ny == 1) { diagonal += back + front; back = front = 0; }
+	else if (ny == 2) { if (y == 0) { front += back; back = 0; } else { back += front; front = 0; } }
+	r.cnt = 0;
+	r.diag = diagonal;
+	const int i = x + nx * y;
+	auto put = [&](double v, int c) { r.a[r.cnt] = v; r.col[r.cnt] = c; r.cnt++; };
+	if (front != 0 && y == ny - 1) put(front, x);
+	if (back != 0 && y > 0) put(back, i - nx);
+	if (right != 0 && x == nx - 1) put(right, nx * y);
+	if (left != 0 && x > 0) put(left, i - 1);
+	if (diagonal != 0) put(diagonal, i);
+	if (right != 0 && x < nx - 1) put(right, i + 1);
+	if (left != 0 && x == 0) put(left, nx - 1 + nx * y);
+	if (front != 0 && y < ny - 1) put(front, i + nx);
+	if (back != 0 && y == 0) put(back, x + nx * (ny - 1));
+}
+
+// ------------------------------------------------------------- IB window --
+// The 4x4 window of one MAC component around marker (cx, cy):
+// ib/eulerian_grid.h:151-180 through ib/mac_grid.h:244-257, restricted to the
+// periodic (x, z) plane.  comp 0 (u, x-faces) is shifted by -h/2 in y before
+// the search, comp 1 (v, y-faces) in x.  Tap (a, b) is the cell
+// ((ix0 + a) % nx, (iy0 + b) % ny) with weight dx[a] * dy[b].
+__device__ inline void ib_window(int comp, double cx, double cy, unsigned n, unsigned W, unsigned D,
+                                 unsigned nx, unsigned ny, int& ix0, int& iy0, double dx[4], double dy[4])
+{
+	double q0 = cx, q1 = cy;
+	if (comp == 0) q1 -= 0.5 / n; else q0 -= 0.5 / n;
+	q0 = fmod(fmod(q0, (double) W) + W, (double) W);
+	q1 = fmod(fmod(q1, (double) D) + D, (double) D);
+	double fx = floor(q0 * n), fy = floor(q1 * n);
+	// finite inputs give fx in [0, nx]; keep non-finite ones in bounds
+	if (!(fx >= 0.0 && fx <= (double) nx)) fx = 0.0;
+	if (!(fy >= 0.0 && fy <= (double) ny)) fy = 0.0;
+	ix0 = (int) (((size_t) fx + nx - 1) % nx);
+	iy0 = (int) (((size_t) fy + ny - 1) % ny);
+#pragma unroll
+	for (int d = 0; d < 4; ++d) {
+		double x = (fx + d - 1.0) / n, y = (fy + d - 1.0) / n;
+		if (comp == 0) y += 0.5 / n; else x += 0.5 / n;
+		dx[d] = delta(x - cx, n);
+		dy[d] = delta(y - cy, n);
+	}
+}
+
+// sum_taps (w * vol) * field[idx], taps in the reference order (x fastest).
+__device__ inline double ib_gather(const double* __restrict__ f, unsigned nx, unsigned ny, int ix0, int iy0,
+                                   const double dx[4], const double dy[4], double vol)
+{
+	double value = 0.0;
+#pragma unroll
+	for (int b = 0; b < 4; ++b) {
+		int iy = iy0 + b; if (iy >= (int) ny) iy -= ny;
+#pragma unroll
+		for (int a = 0; a < 4; ++a) {
+			int ix = ix0 + a; if (ix >= (int) nx) ix -= nx;
+			double w = dx[a] * dy[b];
+			value += w * vol * f[ix + (size_t) nx * iy];
+		}
+	}
+	return value;
+}
+
+// XCD-aware block order: consecutive block ids go to different XCDs (round
+// robin over 8), so map them back to spatially consecutive tiles per XCD.
+__device__ inline unsigned xcd_tile(unsigned bid, unsigned nblk)
+{
+	const unsigned X = 8;
+	if (nblk % X) return bid;
+	return (bid % X) * (nblk / X) + bid / X;
+}
+
+}  // namespace afm

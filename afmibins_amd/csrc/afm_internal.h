@@ -1,0 +1,130 @@
+// afm_internal.h -- device-side data structures and launchers shared by the
+// HIP translation units of libafm.  Not part of the public C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+#include "afm_delta.h"
+
+namespace afm {
+
+// ----------------------------------------------------------- errors ----
+void set_error(const std::string& msg);
+const char* last_error();
+
+#define AFM_HIP(call)                                                                    \
+	do {                                                                                 \
+		hipError_t e_ = (call);                                                          \
+		if (e_ != hipSuccess) {                                                          \
+			::afm::set_error(std::string(#call) + ": " + hipGetErrorString(e_));         \
+			return -3;                                                                   \
+		}                                                                                \
+	} while (0)
+
+#define AFM_LAUNCH_CHECK()                                                               \
+	do {                                                                                 \
+		hipError_t e_ = hipGetLastError();                                               \
+		if (e_ != hipSuccess) {                                                          \
+			::afm::set_error(std::string("kernel launch: ") + hipGetErrorString(e_));    \
+			return -3;                                                                   \
+		}                                                                                \
+	} while (0)
+
+#define AFM_TRY(expr)                                                                    \
+	do {                                                                                 \
+		int r_ = (expr);                                                                 \
+		if (r_ != 0) return r_;                                                          \
+	} while (0)
+
+// ---------------------------------------------------- solve control ----
+// One iterative solve's scalars, resident in device memory so that the
+// stop test (mg/solver.h:228, alg/cg.h:42) runs on the GPU and the host only
+// synchronises once per batch of iterations.
+struct SolveCtl {
+	double tol;       // absolute threshold (RMS for MG, 2-norm for CG)
+	double res;       // current residual norm
+	double n0, nu, mu;
+	double mean;      // mean removed from the MG right-hand side
+	double sum2;      // scratch second sum
+	int count;        // completed iterations
+	int done;         // stop test has fired
+	int max_iter;
+	int err;          // kernel-detected error (e.g. spread bin overflow)
+	unsigned ticket;  // last-block counter of the fused reductions
+	int pad[3];
+};
+
+constexpr int NCTL = 8;          // ctl[0] MG, ctl[1..2] CG fields, ctl[3] misc
+constexpr int RED_THREADS = 256;
+constexpr int MAX_RED_BLOCKS = 4096;
+
+// Per-marker, per-component transfer record used by spreading.
+struct MarkerRec {
+	int ix0, iy0;           // first tap cell
+	int tile, pad;
+	double dx[4], dy[4];    // delta factors of the 4x4 window
+	double ds, F;           // source volume and value
+};
+
+// ------------------------------------------------------------- context --
+struct LevelBuf {
+	unsigned n = 0, nx = 0, ny = 0;
+	double *Xa = nullptr, *Xb = nullptr, *B = nullptr, *CP = nullptr;
+	float *Xa32 = nullptr, *Xb32 = nullptr, *B32 = nullptr;
+};
+
+struct Ctx {
+	int device = 0;
+	hipStream_t stream = nullptr;          // stream all work is ordered on
+	hipStream_t own_stream = nullptr;      // created with the context
+	unsigned n = 0, nx = 0, ny = 0, W = 1, D = 1, levels = 0, lc = 0;
+	size_t N = 0;
+	std::vector<LevelBuf> lv;          // by level, 0 = coarsest
+	double *XSa = nullptr, *XSb = nullptr, *RES = nullptr;
+	float* RES32 = nullptr;
+	SolveCtl* ctl = nullptr;           // device [NCTL]
+	SolveCtl* hctl = nullptr;          // pinned host mirror
+	double* partials = nullptr;        // [MAX_RED_BLOCKS * 4]
+	// spreading
+	MarkerRec* rec = nullptr;          // [2 * mcap]
+	int* bins = nullptr;               // counts | offsets | cursors, 2 comps
+	int* bin_list = nullptr;           // [2 * mcap]
+	long mcap = 0;
+	int ntx = 0, nty = 0;
+	int last_mg_cycles = 0;            // predicted batch size for the next solve
+	int last_cg_iters = 0;
+	std::vector<void*> owned;
+	~Ctx();
+	int alloc(void** p, size_t bytes);
+};
+
+int ensure_markers(Ctx* c, long m);
+
+// -------------------------------------------------------- launchers ----
+// IB transfer (afm_ib.hip)
+int launch_delta(Ctx* c, const double* r, long count, unsigned n, double* out);
+int launch_interp(Ctx* c, int comp, const double* field, const double* X, long m, double* U);
+int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, const double* ds,
+                  const double* X, const double* R, const double* k, long m, double* fx, double* fy);
+int launch_ib_move_half(Ctx* c, const double* u, const double* v, const double* X, long m, double half, double* Xh);
+int launch_ib_move_full(Ctx* c, const double* u_old, const double* v_old, const double* u, const double* v,
+                        const double* Xh, long m, double dt, double* X);
+
+// Multigrid (afm_mg.hip)
+int mg_setup(Ctx* c);
+int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, int precision, int max_cycles,
+             int* cycles, double* final_rms);
+int mg_apply_laplacian(Ctx* c, unsigned level, const double* x, double* y);
+int mg_smooth(Ctx* c, unsigned level, const double* x_in, const double* b, double* x_out, int sweeps, int zero_init);
+int mg_restrict(Ctx* c, unsigned level, const double* fine, double* coarse);
+int mg_prolong(Ctx* c, unsigned level, const double* coarse, double* fine);
+int mg_vcycle(Ctx* c, double* rhs_inout, int nu1, int nu2);
+
+// Reductions / helpers (afm_ns.hip)
+int launch_sum(Ctx* c, const double* x, size_t N, double* out_host);
+
+}  // namespace afm
+
+struct afm_ctx : afm::Ctx {};

@@ -1,0 +1,679 @@
+// afm_mg.hip -- geometric multigrid for the doubly periodic cell-centred
+// Poisson problem on gfx950 (mg/solver.h:35-271 restated).
+//
+// The algorithm is the reference's: nested-iteration start, mean removal,
+// then V(nu1, nu2) cycles on the residual equation until RMS(r) <= tol, with
+// 16-tap full-weighting restriction, bilinear prolongation and the coarsest
+// level solved by mean removal + CG.  The smoother is red-black Gauss-Seidel
+// (the reference's lexicographic sweep is inherently serial); each row update
+// uses the reference's arithmetic (alg/smooth.h:39-50) and stencil order.
+//
+// Levels with n >= 64 run as LDS-tiled global kernels:
+//   k_smooth        nu sweeps per launch, temporally blocked (halo 2 nu),
+//                   optionally fused with "x += P e" or "x = P e";
+//   k_resid_restr   residual b - L x on a fine tile in LDS, restricted;
+//   k_update_resid  x += e, r = rhs - L x, deterministic RMS, stop test.
+// Levels with n <= 32 run inside one workgroup (k_coarse), all in LDS.
+// The stop test runs on the GPU (SolveCtl); every kernel of a cycle checks
+// ctl->done first, so the host enqueues cycles in batches and synchronises
+// once per batch.
+#include "afm_device.h"
+
+namespace afm {
+
+constexpr int STX = 64, STY = 16;        // smoother tile
+constexpr int RTX = 32, RTY = 8;         // restriction tile (coarse cells)
+constexpr int CMAXL = 5;                 // finest level handled in LDS (32x32)
+constexpr int CTHREADS = 1024;
+
+enum { M_NORMAL = 0, M_ZERO = 1, M_PADD = 2, M_PSET = 3 };
+
+template<typename T>
+__device__ inline T prolong_at(const T* __restrict__ xc, int gx, int gy, int hx, int hy)
+{
+	// mg/solver.h:141-148 with the periodic (x, z) plane: fine cell 2a+1 has
+	// primary coarse a, fine cell 2a+2 has primary a+1.
+	int px, ox, py, oy;
+	if (gx & 1) { px = (gx - 1) >> 1; ox = px + 1; if (ox == hx) ox = 0; }
+	else { px = gx >> 1; if (px == hx) px = 0; ox = px - 1; if (ox < 0) ox += hx; }
+	if (gy & 1) { py = (gy - 1) >> 1; oy = py + 1; if (oy == hy) oy = 0; }
+	else { py = gy >> 1; if (py == hy) py = 0; oy = py - 1; if (oy < 0) oy += hy; }
+	T P = xc[px + (size_t) hx * py], Sx = xc[ox + (size_t) hx * py], Sy = xc[px + (size_t) hx * oy],
+	  Dg = xc[ox + (size_t) hx * oy];
+	return T(9. / 16.) * P + T(3. / 16.) * (Sx + Sy) + T(1. / 16.) * Dg;
+}
+
+// ---------------------------------------------------------------- smooth --
+template<typename T, int S, int MODE>
+__global__ __launch_bounds__(256) void k_smooth(const T* __restrict__ xin, const T* __restrict__ b,
+                                                const T* __restrict__ xc, T* __restrict__ xout, int nx, int ny,
+                                                T off, T dg, const int* __restrict__ done)
+{
+	constexpr int H = 2 * S;
+	constexpr int LX = STX + 2 * H, LY = STY + 2 * H;
+	constexpr int BX = STX + 2 * (H - 1), BY = STY + 2 * (H - 1);
+	__shared__ T xs[LX * LY];
+	__shared__ T bs[BX * BY];
+	if (done && *done) return;
+	const unsigned ntx = nx / STX;
+	const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+	const int X0 = (tile % ntx) * STX, Y0 = (tile / ntx) * STY;
+	const int tid = threadIdx.x;
+	const int hx = nx >> 1, hy = ny >> 1;
+	for (int p = tid; p < LX * LY; p += 256) {
+		int lx = p % LX, ly = p / LX;
+		int gx = X0 - H + lx; gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
+		int gy = Y0 - H + ly; gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
+		T v;
+		if (MODE == M_ZERO) v = T(0);
+		else if (MODE == M_NORMAL) v = xin[gx + (size_t) nx * gy];
+		else if (MODE == M_PADD) v = xin[gx + (size_t) nx * gy] + prolong_at(xc, gx, gy, hx, hy);
+		else v = prolong_at(xc, gx, gy, hx, hy);
+		xs[p] = v;
+	}
+	for (int p = tid; p < BX * BY; p += 256) {
+		int lx = p % BX, ly = p / BX;
+		int gx = X0 - (H - 1) + lx; gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
+		int gy = Y0 - (H - 1) + ly; gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
+		bs[p] = b[gx + (size_t) nx * gy];
+	}
+	__syncthreads();
+#pragma unroll
+	for (int ph = 0; ph < 2 * S; ++ph) {
+		const int ring = H - 1 - ph;            // points within `ring` of the tile are valid after this phase
+		const int color = ph & 1;
+		const int RX = STX + 2 * ring, RY = STY + 2 * ring;
+		// only points of `color` are updated: walk the half of the region
+		for (int q = tid; q < (RX * RY + 1) / 2; q += 256) {
+			int ry = (2 * q) / RX;
+			int rx0 = (2 * q) % RX;
+			int lx = H - ring + rx0, ly = H - ring + ry;
+			int gx = X0 - H + lx, gy = Y0 - H + ly;
+			// parity of (gx + gy); shift rx by one when the row starts on the other colour
+			int gxn = gx + (gx < 0 ? nx : 0) - (gx >= nx ? nx : 0);
+			int gyn = gy + (gy < 0 ? ny : 0) - (gy >= ny ? ny : 0);
+			if (((gxn + gyn) & 1) != color) {
+				lx += 1; gx += 1; gxn += 1; if (gxn >= nx) gxn -= nx;
+				if (lx >= H - ring + RX) continue;
+			}
+			const int li = lx + LX * ly;
+			const int bi = (lx - 1) + BX * (ly - 1);
+			T e = xs[li + 1], w = xs[li - 1], nn = xs[li + LX], s = xs[li - LX];
+			T value = stencil5<T, false>(off, dg, T(0), e, w, nn, s, gxn == nx - 1, gxn == 0, gyn == ny - 1, gyn == 0);
+			xs[li] = (bs[bi] - value) / dg;
+		}
+		__syncthreads();
+	}
+	for (int p = tid; p < STX * STY; p += 256) {
+		int lx = p % STX, ly = p / STX;
+		xout[(X0 + lx) + (size_t) nx * (Y0 + ly)] = xs[(lx + H) + LX * (ly + H)];
+	}
+}
+
+// ------------------------------------------------------ residual+restrict --
+// RESID: coarse[c] = sum_taps (w/64) * (b - L x)[fine]; else sum of x itself.
+// With cp != nullptr the fine input is also copied to cp (nested iteration).
+template<typename T, bool RESID>
+__global__ __launch_bounds__(256) void k_resid_restr(const T* __restrict__ x, const T* __restrict__ b,
+                                                     T* __restrict__ rc, T* __restrict__ cp, int nx, int ny,
+                                                     T off, T dg, const int* __restrict__ done)
+{
+	constexpr int FX = 2 * RTX + 2, FY = 2 * RTY + 2;   // fine cells feeding the tile
+	constexpr int LX = FX + 2, LY = FY + 2;             // plus the stencil halo
+	__shared__ T xs[RESID ? LX * LY : 1];
+	__shared__ T rs[FX * FY];
+	if (done && *done) return;
+	const int cnx = nx >> 1;
+	const unsigned ntx = cnx / RTX;
+	const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+	const int C0x = (tile % ntx) * RTX, C0y = (tile / ntx) * RTY;
+	const int F0x = 2 * C0x - 1, F0y = 2 * C0y - 1;     // first fine cell of the taps
+	const int tid = threadIdx.x;
+	if (RESID) {
+		for (int p = tid; p < LX * LY; p += 256) {
+			int gx = F0x - 1 + p % LX, gy = F0y - 1 + p / LX;
+			gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
+			gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
+			xs[p] = x[gx + (size_t) nx * gy];
+		}
+		__syncthreads();
+	}
+	for (int p = tid; p < FX * FY; p += 256) {
+		int lx = p % FX, ly = p / FX;
+		int gx = F0x + lx, gy = F0y + ly;
+		gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
+		gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
+		if (RESID) {
+			const int li = (lx + 1) + LX * (ly + 1);
+			T Lx = stencil5<T, true>(off, dg, xs[li], xs[li + 1], xs[li - 1], xs[li + LX], xs[li - LX],
+			                         gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
+			rs[p] = b[gx + (size_t) nx * gy] - Lx;
+		} else {
+			rs[p] = x[gx + (size_t) nx * gy];
+		}
+	}
+	__syncthreads();
+	const int cxl = tid % RTX, cyl = tid / RTX;
+	T o = T(0);
+#pragma unroll
+	for (int i = 0; i < 16; ++i) {
+		const int dx = (i % 4) - 2, dy = i / 4 - 2;
+		const int wgt = (4 - abs(2 * dx + 1)) * (4 - abs(2 * dy + 1));
+		o += T(wgt / 64.) * rs[(2 * cxl + dx + 2) + FX * (2 * cyl + dy + 2)];
+	}
+	rc[(C0x + cxl) + (size_t) cnx * (C0y + cyl)] = o;
+	if (cp) {
+		// each fine cell of the tile's own 2x2 blocks, copied once
+		for (int p = tid; p < 4 * RTX * RTY; p += 256) {
+			int fx = 2 * C0x + p % (2 * RTX), fy = 2 * C0y + p / (2 * RTX);
+			cp[fx + (size_t) nx * fy] = x[fx + (size_t) nx * fy];
+		}
+	}
+}
+
+// ------------------------------------------------------- update + residual --
+// xs_new = xs + e (e may be null), r = rhs' - L xs_new, where rhs' = rhs -
+// mean when `sub_mean` (written back to rhs).  Deterministic RMS of r and the
+// stop test (mg/solver.h:226-233) in the last block.
+template<typename TE, typename TR>
+__global__ __launch_bounds__(256) void k_update_resid(const double* __restrict__ xs, const TE* __restrict__ e,
+                                                      double* __restrict__ rhs, double* __restrict__ xs_out,
+                                                      TR* __restrict__ res, int nx, int ny, double off, double dg,
+                                                      int sub_mean, int first, SolveCtl* __restrict__ ctl,
+                                                      double* __restrict__ partials)
+{
+	if (!first && ctl->done) return;
+	const size_t N = (size_t) nx * ny;
+	const double mean = ctl->mean;
+	double acc[1] = {0.0};
+	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
+		const int gx = (int) (i % nx), gy = (int) (i / nx);
+		const size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1;
+		const size_t iw = gx == 0 ? i + (nx - 1) : i - 1;
+		const size_t in_ = gy == ny - 1 ? gx : i + nx;
+		const size_t is = gy == 0 ? i + N - nx : i - nx;
+		double c = xs[i], E = xs[ie], Wv = xs[iw], Nv = xs[in_], Sv = xs[is];
+		if (e) {
+			c = c + (double) e[i]; E = E + (double) e[ie]; Wv = Wv + (double) e[iw];
+			Nv = Nv + (double) e[in_]; Sv = Sv + (double) e[is];
+			xs_out[i] = c;
+		}
+		double r0 = rhs[i];
+		if (sub_mean) { r0 = r0 - mean * 1.0; rhs[i] = r0; }
+		double Lx = stencil5<double, true>(off, dg, c, E, Wv, Nv, Sv, gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
+		double r = r0 - Lx;
+		res[i] = (TR) r;
+		acc[0] += r * r;
+	}
+	double tot[1];
+	if (reduce_last<1>(acc, partials, &ctl->ticket, tot) && threadIdx.x == 0) {
+		double rms = sqrt(tot[0]) / sqrt((double) N);
+		int count = first ? 0 : ctl->count + 1;
+		ctl->count = count;
+		ctl->res = rms;
+		ctl->done = !(rms > ctl->tol && count < ctl->max_iter);
+	}
+}
+
+// sum(rhs * 1) -> ctl->mean = sum / N ; second pass: tol = rel * RMS(rhs - mean)
+__global__ __launch_bounds__(256) void k_mean(const double* __restrict__ rhs, size_t N, int pass, double rel,
+                                              SolveCtl* __restrict__ ctl, double* __restrict__ partials)
+{
+	double acc[1] = {0.0};
+	const double mean = pass ? ctl->mean : 0.0;
+	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
+		if (pass == 0) acc[0] += rhs[i] * 1.0;
+		else { double t = rhs[i] - mean * 1.0; acc[0] += t * t; }
+	}
+	double tot[1];
+	if (reduce_last<1>(acc, partials, &ctl->ticket, tot) && threadIdx.x == 0) {
+		if (pass == 0) ctl->mean = tot[0] / (double) N;
+		else ctl->tol = rel * (sqrt(tot[0]) / sqrt((double) N));
+	}
+}
+
+template<typename T>
+__global__ void k_copy_cvt(const T* __restrict__ src, double* __restrict__ dst, size_t N)
+{
+	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+	if (i < N) dst[i] = (double) src[i];
+}
+
+// ------------------------------------------------------------- coarse -----
+// Levels 0..Lt (n <= 32) of the unit square, one workgroup, all in LDS.
+struct CoarseLDS {
+	double X[1365], B[1365], CP[1365], TMP[1024];
+};
+
+__device__ inline int lvl_off(int l) { return ((1 << (2 * l)) - 1) / 3; }   // sum_{j<l} 4^j
+
+__device__ void c_smooth(CoarseLDS& S, int l, double* x, const double* b, bool zero, int sweeps)
+{
+	const int nx = 1 << l, N = nx * nx;
+	const double off = (double) ((1u << l) * (1u << l));
+	if (zero) {
+		for (int i = threadIdx.x; i < N; i += CTHREADS) x[i] = 0.0;
+		__syncthreads();
+	}
+	for (int s = 0; s < sweeps; ++s)
+		for (int color = 0; color < 2; ++color) {
+			for (int i = threadIdx.x; i < N; i += CTHREADS) {
+				int gx = i % nx, gy = i / nx;
+				if (((gx + gy) & 1) != color) continue;
+				LapRow r;
+				lap_row(gx, gy, nx, nx, off, r);
+				double value = 0.0, diagonal = 0.0;
+				for (int k = 0; k < r.cnt; ++k) {
+					if (r.col[k] == i) { diagonal = r.a[k]; continue; }
+					value += r.a[k] * x[r.col[k]];
+				}
+				x[i] = (b[i] - value) / diagonal;
+			}
+			__syncthreads();
+		}
+}
+
+// dst(level l-1) = R src  (src = b - L x when x != nullptr)
+__device__ void c_restrict(CoarseLDS& S, int l, const double* x, const double* src, double* dst)
+{
+	const int nx = 1 << l, N = nx * nx, cn = nx >> 1;
+	const double off = (double) ((1u << l) * (1u << l));
+	const double* in = src;
+	if (x) {
+		for (int i = threadIdx.x; i < N; i += CTHREADS) {
+			LapRow r;
+			lap_row(i % nx, i / nx, nx, nx, off, r);
+			double y = 0.0;
+			for (int k = 0; k < r.cnt; ++k) y += r.a[k] * x[r.col[k]];
+			S.TMP[i] = src[i] - y;
+		}
+		__syncthreads();
+		in = S.TMP;
+	}
+	for (int c = threadIdx.x; c < cn * cn; c += CTHREADS) {
+		int xcc = c % cn, zcc = c / cn;
+		int xcf = 2 * xcc + 1, zcf = 2 * zcc + 1;
+		double o = 0.0;
+		for (int i = 0; i < 16; ++i) {
+			int dx = (i % 4) - 2, dz = i / 4 - 2;
+			int xx = (xcf + dx + nx) % nx, zz = (zcf + dz + nx) % nx;
+			int w = (4 - abs(2 * dx + 1)) * (4 - abs(2 * dz + 1));
+			o += w / 64. * in[xx + nx * zz];
+		}
+		dst[c] = o;
+	}
+	__syncthreads();
+}
+
+// dst(level l) (+)= P xc(level l-1)
+__device__ void c_prolong(int l, const double* xc, double* dst, bool add)
+{
+	const int nx = 1 << l, N = nx * nx, h = nx >> 1;
+	for (int i = threadIdx.x; i < N; i += CTHREADS) {
+		double v = prolong_at(xc, i % nx, i / nx, h, h);
+		dst[i] = add ? dst[i] + v : v;
+	}
+	__syncthreads();
+}
+
+// mg/solver.h:264-271 on the 1x1 coarsest grid: mean removal leaves 0 and
+// CG stops at once (the 1x1 laplacian has no stored entries).
+__device__ void c_solve0(CoarseLDS& S)
+{
+	if (threadIdx.x == 0) {
+		double c = (1.0 * S.B[0]) / 1u;
+		S.B[0] -= 1.0 * c;
+		S.X[0] = S.B[0];
+	}
+	__syncthreads();
+}
+
+__device__ void c_recurse(CoarseLDS& S, int l, int nu1, int nu2)
+{
+	for (int k = l; k >= 1; --k) {
+		c_smooth(S, k, S.X + lvl_off(k), S.B + lvl_off(k), true, nu1);
+		c_restrict(S, k, S.X + lvl_off(k), S.B + lvl_off(k), S.B + lvl_off(k - 1));
+	}
+	c_solve0(S);
+	for (int k = 1; k <= l; ++k) {
+		c_prolong(k, S.X + lvl_off(k - 1), S.X + lvl_off(k), true);
+		c_smooth(S, k, S.X + lvl_off(k), S.B + lvl_off(k), false, nu2);
+	}
+}
+
+__device__ void c_nested(CoarseLDS& S, int l, int nu1, int nu2)
+{
+	for (int k = l; k >= 1; --k) {
+		for (int i = threadIdx.x; i < (1 << (2 * k)); i += CTHREADS) S.CP[lvl_off(k) + i] = S.B[lvl_off(k) + i];
+		__syncthreads();
+		c_restrict(S, k, nullptr, S.B + lvl_off(k), S.B + lvl_off(k - 1));
+	}
+	for (int k = 1; k <= l; ++k) {
+		c_recurse(S, k - 1, nu1, nu2);
+		c_prolong(k, S.X + lvl_off(k - 1), S.B + lvl_off(k), false);
+		c_smooth(S, k, S.B + lvl_off(k), S.CP + lvl_off(k), false, nu2);
+	}
+}
+
+// mode 0: recurse(Lt, in) -> out;  1: nested(Lt, in); recurse -> out;
+// mode 2: nested(Lt, in) -> out (the approximate solution itself).
+template<typename TI>
+__global__ __launch_bounds__(CTHREADS) void k_coarse(int mode, int Lt, const TI* __restrict__ in, TI* __restrict__ out,
+                                                     int nu1, int nu2, const int* __restrict__ done)
+{
+	__shared__ CoarseLDS S;
+	if (done && *done) return;
+	const int N = 1 << (2 * Lt), o = lvl_off(Lt);
+	for (int i = threadIdx.x; i < N; i += CTHREADS) S.B[o + i] = (double) in[i];
+	__syncthreads();
+	if (mode >= 1) c_nested(S, Lt, nu1, nu2);
+	if (mode <= 1) {
+		c_recurse(S, Lt, nu1, nu2);
+		for (int i = threadIdx.x; i < N; i += CTHREADS) out[i] = (TI) S.X[o + i];
+	} else {
+		for (int i = threadIdx.x; i < N; i += CTHREADS) out[i] = (TI) S.B[o + i];
+	}
+}
+
+// ------------------------------------------------------------- host side --
+static inline unsigned red_blocks(size_t N)
+{
+	size_t b = (N + 255) / 256;
+	return (unsigned) (b < (size_t) MAX_RED_BLOCKS ? b : (size_t) MAX_RED_BLOCKS);
+}
+
+template<typename T>
+static int smooth_launch(Ctx* c, unsigned l, int mode, int sweeps, const T* xin, const T* b, const T* xc, T* xout,
+                         const int* done)
+{
+	const LevelBuf& L = c->lv[l];
+	const T off = (T) (double) (L.n * L.n), dg = (T) (-6.0 * (double) (L.n * L.n) + (double) (L.n * L.n) + (double) (L.n * L.n));
+	dim3 g((L.nx / STX) * (L.ny / STY));
+#define AFM_SM(S_, M_) hipLaunchKernelGGL((k_smooth<T, S_, M_>), g, dim3(256), 0, c->stream, xin, b, xc, xout, (int) L.nx, (int) L.ny, off, dg, done)
+	if (sweeps == 1) {
+		switch (mode) { case M_NORMAL: AFM_SM(1, M_NORMAL); break; case M_ZERO: AFM_SM(1, M_ZERO); break;
+		                case M_PADD: AFM_SM(1, M_PADD); break; default: AFM_SM(1, M_PSET); }
+	} else if (sweeps == 2) {
+		switch (mode) { case M_NORMAL: AFM_SM(2, M_NORMAL); break; case M_ZERO: AFM_SM(2, M_ZERO); break;
+		                case M_PADD: AFM_SM(2, M_PADD); break; default: AFM_SM(2, M_PSET); }
+	} else {
+		set_error("smoother supports 1 or 2 sweeps per launch");
+		return -1;
+	}
+#undef AFM_SM
+	AFM_LAUNCH_CHECK();
+	return 0;
+}
+
+// nu sweeps as launches of at most 2 sweeps; the first launch applies `mode`.
+template<typename T>
+static int smooth_n(Ctx* c, unsigned l, int mode, int nu, const T* xin, const T* b, const T* xc, T* xout, T* tmp,
+                    const int* done)
+{
+	if (nu == 0) {
+		// x = xin (+ P xc) with no sweep: a 0-sweep launch is not provided, so
+		// express it through the restriction-free path of one tile copy.
+		set_error("nu == 0 is not supported by the GPU smoother");
+		return -1;
+	}
+	const T* src = xin;
+	int left = nu;
+	int m = mode;
+	T* bufs[2] = {xout, tmp};
+	// arrange for the final launch to land in xout
+	int launches = (nu + 1) / 2;
+	int cur = (launches % 2 == 1) ? 0 : 1;
+	while (left > 0) {
+		int s = left >= 2 ? 2 : 1;
+		T* dst = bufs[cur];
+		AFM_TRY(smooth_launch<T>(c, l, m, s, src, b, xc, dst, done));
+		src = dst;
+		cur ^= 1;
+		left -= s;
+		m = M_NORMAL;
+	}
+	return 0;
+}
+
+template<typename T>
+static int resid_restrict_launch(Ctx* c, unsigned l, bool resid, const T* x, const T* b, T* rc, T* cp, const int* done)
+{
+	const LevelBuf& L = c->lv[l];
+	const T off = (T) (double) (L.n * L.n), dg = (T) (-4.0 * (double) (L.n * L.n));
+	dim3 g(((L.nx / 2) / RTX) * ((L.ny / 2) / RTY));
+	if (resid)
+		hipLaunchKernelGGL((k_resid_restr<T, true>), g, dim3(256), 0, c->stream, x, b, rc, cp, (int) L.nx, (int) L.ny, off, dg, done);
+	else
+		hipLaunchKernelGGL((k_resid_restr<T, false>), g, dim3(256), 0, c->stream, x, b, rc, cp, (int) L.nx, (int) L.ny, off, dg, done);
+	AFM_LAUNCH_CHECK();
+	return 0;
+}
+
+template<typename T> static T* lvX(LevelBuf& L, int ab);
+template<> double* lvX<double>(LevelBuf& L, int ab) { return ab ? L.Xb : L.Xa; }
+template<> float* lvX<float>(LevelBuf& L, int ab) { return ab ? L.Xb32 : L.Xa32; }
+template<typename T> static T* lvB(LevelBuf& L);
+template<> double* lvB<double>(LevelBuf& L) { return L.B; }
+template<> float* lvB<float>(LevelBuf& L) { return L.B32; }
+
+template<typename T>
+static int coarse_launch(Ctx* c, int mode, int Lt, const T* in, T* out, int nu1, int nu2, const int* done)
+{
+	hipLaunchKernelGGL((k_coarse<T>), dim3(1), dim3(CTHREADS), 0, c->stream, mode, Lt, in, out, nu1, nu2, done);
+	AFM_LAUNCH_CHECK();
+	return 0;
+}
+
+// recurse(l, B_l) -> X_l (final in Xb); rhs of level l given explicitly.
+template<typename T>
+static int vcycle(Ctx* c, unsigned l, const T* rhs_l, int nu1, int nu2, const int* done)
+{
+	const unsigned lc = c->lc;
+	if (l <= lc) return coarse_launch<T>(c, 0, (int) l, rhs_l, lvX<T>(c->lv[l], 1), nu1, nu2, done);
+	for (unsigned k = l; k > lc; --k) {
+		const T* bk = (k == l) ? rhs_l : lvB<T>(c->lv[k]);
+		AFM_TRY(smooth_n<T>(c, k, M_ZERO, nu1, nullptr, bk, nullptr, lvX<T>(c->lv[k], 0), lvX<T>(c->lv[k], 1), done));
+		AFM_TRY(resid_restrict_launch<T>(c, k, true, lvX<T>(c->lv[k], 0), bk, lvB<T>(c->lv[k - 1]), nullptr, done));
+	}
+	AFM_TRY(coarse_launch<T>(c, 0, (int) lc, lvB<T>(c->lv[lc]), lvX<T>(c->lv[lc], 1), nu1, nu2, done));
+	for (unsigned k = lc + 1; k <= l; ++k) {
+		const T* bk = (k == l) ? rhs_l : lvB<T>(c->lv[k]);
+		AFM_TRY(smooth_n<T>(c, k, M_PADD, nu2, lvX<T>(c->lv[k], 0), bk, lvX<T>(c->lv[k - 1], 1), lvX<T>(c->lv[k], 1),
+		                    lvX<T>(c->lv[k], 0), done));
+	}
+	return 0;
+}
+
+// nested(T, rhs) -> XSa (mg/solver.h:183-197 unrolled over the levels).
+static int nested(Ctx* c, double* rhs, int nu1, int nu2)
+{
+	const unsigned T = c->levels, lc = c->lc;
+	if (T <= lc) return coarse_launch<double>(c, 2, (int) T, rhs, c->XSa, nu1, nu2, nullptr);
+	// down: CP_l = B_l, B_{l-1} = R B_l  (CP_T is rhs itself)
+	for (unsigned k = T; k > lc; --k) {
+		const double* src = (k == T) ? rhs : c->lv[k].B;
+		double* cp = (k == T) ? nullptr : c->lv[k].CP;
+		AFM_TRY(resid_restrict_launch<double>(c, k, false, src, nullptr, c->lv[k - 1].B, cp, nullptr));
+	}
+	AFM_TRY(coarse_launch<double>(c, 1, (int) lc, c->lv[lc].B, c->lv[lc].Xb, nu1, nu2, nullptr));
+	for (unsigned k = lc + 1; k <= T; ++k) {
+		double* dst = (k == T) ? c->XSa : c->lv[k].B;
+		const double* cpk = (k == T) ? rhs : c->lv[k].CP;
+		AFM_TRY(smooth_n<double>(c, k, M_PSET, nu2, nullptr, cpk, c->lv[k - 1].Xb, dst, c->lv[k].Xa, nullptr));
+		if (k < T) AFM_TRY(vcycle<double>(c, k, c->lv[k].B, nu1, nu2, nullptr));
+	}
+	return 0;
+}
+
+int mg_setup(Ctx* c)
+{
+	unsigned T = 0;
+	while ((1u << T) < c->n) ++T;
+	c->levels = T;
+	c->lc = T < (unsigned) CMAXL ? T : (unsigned) CMAXL;
+	c->lv.assign(T + 1, LevelBuf());
+	for (unsigned l = 0; l <= T; ++l) {
+		LevelBuf& L = c->lv[l];
+		L.n = 1u << l; L.nx = L.n; L.ny = L.n;
+		size_t N = (size_t) L.nx * L.ny;
+		if (l < c->lc) continue;                         // LDS only
+		AFM_TRY(c->alloc((void**) &L.Xb, N * 8));
+		AFM_TRY(c->alloc((void**) &L.B, N * 8));
+		AFM_TRY(c->alloc((void**) &L.Xb32, N * 4));
+		AFM_TRY(c->alloc((void**) &L.B32, N * 4));
+		if (l > c->lc) {
+			AFM_TRY(c->alloc((void**) &L.Xa, N * 8));
+			AFM_TRY(c->alloc((void**) &L.CP, N * 8));
+			AFM_TRY(c->alloc((void**) &L.Xa32, N * 4));
+		}
+	}
+	AFM_TRY(c->alloc((void**) &c->XSa, c->N * 8));
+	AFM_TRY(c->alloc((void**) &c->XSb, c->N * 8));
+	AFM_TRY(c->alloc((void**) &c->RES, c->N * 8));
+	AFM_TRY(c->alloc((void**) &c->RES32, c->N * 4));
+	return 0;
+}
+
+static int read_ctl(Ctx* c, int i)
+{
+	AFM_HIP(hipMemcpyAsync(&c->hctl[i], &c->ctl[i], sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
+	AFM_HIP(hipStreamSynchronize(c->stream));
+	return 0;
+}
+
+int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, int precision, int max_cycles,
+             int* cycles, double* final_rms)
+{
+	if (nu1 < 1 || nu1 > 2 || nu2 < 1 || nu2 > 2) { set_error("nu1, nu2 must be 1 or 2 (one fused launch per smoothing step)"); return -1; }
+	if (precision != 0 && precision != 1) { set_error("precision must be 0 (fp64) or 1 (fp32 V-cycle in fp64 refinement)"); return -1; }
+	SolveCtl* dctl = &c->ctl[0];
+	SolveCtl h{};
+	h.tol = tol; h.max_iter = max_cycles; h.count = 0; h.done = 0; h.ticket = 0; h.mean = 0.0;
+	AFM_HIP(hipMemcpyAsync(dctl, &h, sizeof(SolveCtl), hipMemcpyHostToDevice, c->stream));
+	const size_t N = c->N;
+	const unsigned rb = red_blocks(N);
+	const unsigned T = c->levels;
+	const double off = (double) (c->n * c->n), dg = -4.0 * off;
+	// mean (and, for a relative tolerance, RMS of rhs - mean) before anything
+	// else, as the oracle computes it (mg/solver.h:224 does the mean after the
+	// nested start; the value is the same because rhs is not touched there).
+	hipLaunchKernelGGL(k_mean, dim3(rb), dim3(256), 0, c->stream, rhs, N, 0, 0.0, dctl, c->partials);
+	AFM_LAUNCH_CHECK();
+	if (tol_mode == 1) {
+		hipLaunchKernelGGL(k_mean, dim3(rb), dim3(256), 0, c->stream, rhs, N, 1, tol, dctl, c->partials);
+		AFM_LAUNCH_CHECK();
+	}
+	AFM_TRY(nested(c, rhs, nu1, nu2));
+	// residual of the nested start, mean removed from rhs in place
+	if (precision == 0)
+		hipLaunchKernelGGL((k_update_resid<double, double>), dim3(rb), dim3(256), 0, c->stream, c->XSa,
+		                   (const double*) nullptr, rhs, c->XSa, c->RES, (int) c->nx, (int) c->ny, off, dg, 1, 1, dctl, c->partials);
+	else
+		hipLaunchKernelGGL((k_update_resid<float, float>), dim3(rb), dim3(256), 0, c->stream, c->XSa,
+		                   (const float*) nullptr, rhs, c->XSa, c->RES32, (int) c->nx, (int) c->ny, off, dg, 1, 1, dctl, c->partials);
+	AFM_LAUNCH_CHECK();
+	const int* done = &dctl->done;
+	int enq = 0;   // cycles enqueued
+	int batch = c->last_mg_cycles > 0 ? c->last_mg_cycles : 2;
+	double* xs[2] = {c->XSa, c->XSb};
+	for (;;) {
+		if (batch > max_cycles - enq) batch = max_cycles - enq;
+		for (int s = 0; s < batch; ++s) {
+			double* cur = xs[enq & 1];
+			double* nxt = xs[(enq + 1) & 1];
+			if (precision == 0) {
+				AFM_TRY(vcycle<double>(c, T, c->RES, nu1, nu2, done));
+				hipLaunchKernelGGL((k_update_resid<double, double>), dim3(rb), dim3(256), 0, c->stream, cur,
+				                   (const double*) c->lv[T].Xb, rhs, nxt, c->RES, (int) c->nx, (int) c->ny, off, dg, 0, 0, dctl, c->partials);
+			} else {
+				AFM_TRY(vcycle<float>(c, T, c->RES32, nu1, nu2, done));
+				hipLaunchKernelGGL((k_update_resid<float, float>), dim3(rb), dim3(256), 0, c->stream, cur,
+				                   (const float*) c->lv[T].Xb32, rhs, nxt, c->RES32, (int) c->nx, (int) c->ny, off, dg, 0, 0, dctl, c->partials);
+			}
+			AFM_LAUNCH_CHECK();
+			++enq;
+		}
+		AFM_TRY(read_ctl(c, 0));
+		if (c->hctl[0].done || enq >= max_cycles) break;
+		batch = 2;
+	}
+	const int cnt = c->hctl[0].count;
+	AFM_HIP(hipMemcpyAsync(rhs, xs[cnt & 1], N * 8, hipMemcpyDeviceToDevice, c->stream));
+	c->last_mg_cycles = cnt;
+	if (cycles) *cycles = cnt;
+	if (final_rms) *final_rms = c->hctl[0].res;
+	if (!c->hctl[0].done) { set_error("multigrid did not converge within max_cycles"); return -4; }
+	return 0;
+}
+
+// One V-cycle recurse(T, rhs) on the residual equation, result in place.
+int mg_vcycle(Ctx* c, double* rhs_inout, int nu1, int nu2)
+{
+	AFM_TRY(vcycle<double>(c, c->levels, rhs_inout, nu1, nu2, nullptr));
+	AFM_HIP(hipMemcpyAsync(rhs_inout, c->lv[c->levels].Xb, c->N * 8, hipMemcpyDeviceToDevice, c->stream));
+	return 0;
+}
+
+// ---- single operators at a global level (n >= 64), for parity tests ----
+__global__ void k_lap_apply(const double* __restrict__ x, double* __restrict__ y, int nx, int ny, double off, double dg)
+{
+	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+	size_t N = (size_t) nx * ny;
+	if (i >= N) return;
+	int gx = (int) (i % nx), gy = (int) (i / nx);
+	size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;
+	size_t in_ = gy == ny - 1 ? gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;
+	y[i] = stencil5<double, true>(off, dg, x[i], x[ie], x[iw], x[in_], x[is], gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
+}
+
+static int check_level(Ctx* c, unsigned level)
+{
+	if (level > c->levels || level <= c->lc) { set_error("level must be a global level (n >= 64) of the context"); return -1; }
+	return 0;
+}
+
+int mg_apply_laplacian(Ctx* c, unsigned level, const double* x, double* y)
+{
+	if (level > c->levels) { set_error("level out of range"); return -1; }
+	const LevelBuf& L = c->lv[level];
+	size_t N = (size_t) L.nx * L.ny;
+	if (L.nx < 3) { set_error("laplacian apply needs n >= 4"); return -1; }
+	hipLaunchKernelGGL(k_lap_apply, dim3((N + 255) / 256), dim3(256), 0, c->stream, x, y, (int) L.nx, (int) L.ny,
+	                   (double) (L.n * L.n), -4.0 * (double) (L.n * L.n));
+	AFM_LAUNCH_CHECK();
+	return 0;
+}
+
+int mg_smooth(Ctx* c, unsigned level, const double* x_in, const double* b, double* x_out, int sweeps, int zero_init)
+{
+	AFM_TRY(check_level(c, level));
+	if (sweeps < 1 || sweeps > 2) { set_error("sweeps must be 1 or 2"); return -1; }
+	return smooth_launch<double>(c, level, zero_init ? M_ZERO : M_NORMAL, sweeps, x_in, b, nullptr, x_out, nullptr);
+}
+
+int mg_restrict(Ctx* c, unsigned level, const double* fine, double* coarse)
+{
+	AFM_TRY(check_level(c, level));
+	return resid_restrict_launch<double>(c, level, false, fine, nullptr, coarse, nullptr, nullptr);
+}
+
+// fine = P coarse, overwritten, followed by no sweep: realised as the
+// M_PSET smoother path would smooth, so use a plain kernel instead.
+__global__ void k_prolong(const double* __restrict__ xc, double* __restrict__ xf, int nx, int ny)
+{
+	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+	if (i >= (size_t) nx * ny) return;
+	xf[i] = prolong_at(xc, (int) (i % nx), (int) (i / nx), nx >> 1, ny >> 1);
+}
+
+int mg_prolong(Ctx* c, unsigned level, const double* coarse, double* fine)
+{
+	if (level > c->levels || level < 1) { set_error("level out of range"); return -1; }
+	const LevelBuf& L = c->lv[level];
+	size_t N = (size_t) L.nx * L.ny;
+	hipLaunchKernelGGL(k_prolong, dim3((N + 255) / 256), dim3(256), 0, c->stream, coarse, fine, (int) L.nx, (int) L.ny);
+	AFM_LAUNCH_CHECK();
+	return 0;
+}
+
+}  // namespace afm

@@ -1,0 +1,547 @@
+// afm_ns.hip -- the Navier-Stokes step around the two solves, on gfx950.
+//
+//   k_advect_rhs   conservative centred advection (ns/solver.h:204-234, the
+//                  2-D branch) fused with the right-hand-side assembly of the
+//                  backward-Euler half step or Crank-Nicolson full step
+//                  (ns/solver.h:105-135);
+//   k_cg_*         CG on the periodic Helmholtz operator I + s L
+//                  (alg/cg.h:32-56, ns/helmholtz.h), both velocity components
+//                  in one launch (blockIdx.y = component), the stencil
+//                  product in the CSR entry order, dot products reduced
+//                  deterministically, the stop test on the GPU;
+//   k_div, k_grad_sub   projection (ns/projection.h:60-73, alg/div.h,
+//                  alg/grad.h) around the multigrid solve.
+#include "afm_device.h"
+
+namespace afm {
+
+// ---------------------------------------------------------------- advection
+__device__ inline void adv_at(const double* __restrict__ u, const double* __restrict__ w, int nx, int ny, unsigned n,
+                              size_t i, int x, int z, double& ua, double& wa)
+{
+	const size_t N = (size_t) nx * ny;
+	auto xm = [&](size_t j, int jx) { return jx == 0 ? j + (nx - 1) : j - 1; };
+	auto xp = [&](size_t j, int jx) { return jx == nx - 1 ? j - (nx - 1) : j + 1; };
+	auto zm = [&](size_t j, int jz) { return jz == 0 ? j + N - nx : j - nx; };
+	auto zp = [&](size_t j, int jz) { return jz == ny - 1 ? j - (N - nx) : j + nx; };
+	const int xP = x == nx - 1 ? 0 : x + 1, zP = z == ny - 1 ? 0 : z + 1;
+	const size_t ixp = xp(i, x), izp = zp(i, z);
+	auto uu = [&](size_t j, int jx) { double q = 0.5 * (u[j] + u[xm(j, jx)]); return q * q; };
+	auto ww = [&](size_t j, int jz) { double q = 0.5 * (w[j] + w[zm(j, jz)]); return q * q; };
+	auto uw = [&](size_t j, int jx, int jz) {
+		double q = 0.5 * (u[zm(j, jz)] + u[j]);
+		double r = 0.5 * (w[xm(j, jx)] + w[j]);
+		return q * r;
+	};
+	const double dv = 0.0 - 0.0;   // alg/div.h:54 with zero top/bottom
+	const double uw_i = uw(i, x, z);
+	ua = n * ((uu(ixp, xP) - uu(i, x)) + dv + (uw(izp, x, zP) - uw_i));
+	wa = n * ((uw(ixp, xP, z) - uw_i) + dv + (ww(izp, zP) - ww(i, z)));
+}
+
+// be != 0: rx = u + (dt/2)((1/rho) fx - adv_u), u_old = u.
+// be == 0: rx = (I + s L) u_old + dt ((1/rho) fx - adv_u).
+__global__ __launch_bounds__(256) void k_advect_rhs(const double* __restrict__ u, const double* __restrict__ v,
+                                                    const double* __restrict__ fx, const double* __restrict__ fy,
+                                                    double* __restrict__ uo, double* __restrict__ vo,
+                                                    double* __restrict__ rx, double* __restrict__ ry, int nx, int ny,
+                                                    unsigned n, double dt, double rho, int be, double hoff, double hdg)
+{
+	const size_t N = (size_t) nx * ny;
+	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+	if (i >= N) return;
+	const int x = (int) (i % nx), z = (int) (i / nx);
+	double ua, va;
+	adv_at(u, v, nx, ny, n, i, x, z, ua, va);
+	if (be) {
+		const double u0 = u[i], v0 = v[i];
+		uo[i] = u0;
+		vo[i] = v0;
+		rx[i] = u0 + (dt / 2) * (1 / rho * fx[i] - ua);
+		ry[i] = v0 + (dt / 2) * (1 / rho * fy[i] - va);
+	} else {
+		const size_t ie = x == nx - 1 ? i - (nx - 1) : i + 1, iw = x == 0 ? i + (nx - 1) : i - 1;
+		const size_t in_ = z == ny - 1 ? (size_t) x : i + nx, is = z == 0 ? i + N - nx : i - nx;
+		const bool we = x == nx - 1, ww = x == 0, wn = z == ny - 1, ws = z == 0;
+		double hu = stencil5<double, true>(hoff, hdg, uo[i], uo[ie], uo[iw], uo[in_], uo[is], we, ww, wn, ws);
+		double hv = stencil5<double, true>(hoff, hdg, vo[i], vo[ie], vo[iw], vo[in_], vo[is], we, ww, wn, ws);
+		rx[i] = hu + dt * (1 / rho * fx[i] - ua);
+		ry[i] = hv + dt * (1 / rho * fy[i] - va);
+	}
+}
+
+__global__ void k_advect(const double* __restrict__ u, const double* __restrict__ v, double* __restrict__ ua,
+                         double* __restrict__ va, int nx, int ny, unsigned n)
+{
+	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+	if (i >= (size_t) nx * ny) return;
+	adv_at(u, v, nx, ny, n, i, (int) (i % nx), (int) (i / nx), ua[i], va[i]);
+}
+
+// d = n ((u[x+1] - u) + dv + (v[y+1] - v))   (alg/div.h:38-59)
+__global__ void k_div(const double* __restrict__ u, const double* __restrict__ v, double* __restrict__ d, int nx,
+                      int ny, unsigned n)
+{
+	const size_t N = (size_t) nx * ny;
+	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+	if (i >= N) return;
+	const int x = (int) (i % nx), z = (int) (i / nx);
+	const size_t ixp = x == nx - 1 ? i - (nx - 1) : i + 1, izp = z == ny - 1 ? (size_t) x : i + nx;
+	const double dv = 0.0 - 0.0;
+	d[i] = n * ((u[ixp] - u[i]) + dv + (v[izp] - v[i]));
+}
+
+// u -= n (p - p[x-1]),  v -= n (p - p[y-1])   (alg/grad.h:36-43, ns/projection.h:70-72)
+__global__ void k_grad_sub(const double* __restrict__ p, double* __restrict__ u, double* __restrict__ v, int nx, int ny,
+                           unsigned n)
+{
+	const size_t N = (size_t) nx * ny;
+	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+	if (i >= N) return;
+	const int x = (int) (i % nx), z = (int) (i / nx);
+	const size_t ixm = x == 0 ? i + (nx - 1) : i - 1, izm = z == 0 ? i + N - nx : i - nx;
+	const double px = n * (p[i] - p[ixm]);
+	const double pz = n * (p[i] - p[izm]);
+	u[i] = u[i] - px;
+	v[i] = v[i] - pz;
+}
+
+__global__ void k_helm_apply(const double* __restrict__ x, double* __restrict__ y, int nx, int ny, double off, double dg)
+{
+	const size_t N = (size_t) nx * ny;
+	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+	if (i >= N) return;
+	const int gx = (int) (i % nx), gy = (int) (i / nx);
+	const size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;
+	const size_t in_ = gy == ny - 1 ? (size_t) gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;
+	y[i] = stencil5<double, true>(off, dg, x[i], x[ie], x[iw], x[in_], x[is], gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
+}
+
+// ---------------------------------------------------------------------- CG
+// Per-field pointers, indexed by blockIdx.y.
+struct CGArgs {
+	double* x[2];
+	const double* b[2];
+	double* r[2];
+	double* p[2][2];       // ping-pong
+	double* z[2];
+	SolveCtl* ctl[2];
+	double* partials[2];
+	int tol_mode;
+};
+
+// r = b - A x, p = r, n0 = r.r  (alg/cg.h:38-39); tol from |b| if relative.
+__global__ __launch_bounds__(256) void k_cg_init(CGArgs a, int nx, int ny, double off, double dg)
+{
+	const int f = blockIdx.y;
+	if (!a.x[f]) return;
+	const size_t N = (size_t) nx * ny;
+	const double* x = a.x[f];
+	const double* b = a.b[f];
+	double acc[2] = {0.0, 0.0};
+	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
+		const int gx = (int) (i % nx), gy = (int) (i / nx);
+		const size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;
+		const size_t in_ = gy == ny - 1 ? (size_t) gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;
+		double Ax = stencil5<double, true>(off, dg, x[i], x[ie], x[iw], x[in_], x[is], gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
+		double r = b[i] - Ax;
+		a.r[f][i] = r;
+		a.p[f][0][i] = r;
+		acc[0] += r * r;
+		acc[1] += b[i] * b[i];
+	}
+	double tot[2];
+	SolveCtl* c = a.ctl[f];
+	if (reduce_last<2>(acc, a.partials[f], &c->ticket, tot) && threadIdx.x == 0) {
+		if (a.tol_mode == 1) c->tol = c->tol * sqrt(tot[1]);
+		c->n0 = tot[0];
+		c->count = 0;
+		c->res = sqrt(tot[0]);
+		c->done = !(sqrt(tot[0]) > c->tol && 0 < c->max_iter);
+	}
+}
+
+// p' = (count == 0) ? p : p*mu + r ;  z = A p' ;  nu = n0 / (p'.z)
+__global__ __launch_bounds__(256) void k_cg_pz(CGArgs a, int nx, int ny, double off, double dg, int pin)
+{
+	const int f = blockIdx.y;
+	if (!a.x[f]) return;
+	SolveCtl* c = a.ctl[f];
+	if (c->done) return;
+	const size_t N = (size_t) nx * ny;
+	const double* p = a.p[f][pin];
+	double* po = a.p[f][pin ^ 1];
+	const double* r = a.r[f];
+	const bool first = c->count == 0;
+	const double mu = c->mu;
+	double acc[1] = {0.0};
+	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
+		const int gx = (int) (i % nx), gy = (int) (i / nx);
+		const size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;
+		const size_t in_ = gy == ny - 1 ? (size_t) gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;
+		double pc, pe, pw, pn, ps;
+		if (first) { pc = p[i]; pe = p[ie]; pw = p[iw]; pn = p[in_]; ps = p[is]; }
+		else {
+			pc = p[i] * mu + r[i]; pe = p[ie] * mu + r[ie]; pw = p[iw] * mu + r[iw];
+			pn = p[in_] * mu + r[in_]; ps = p[is] * mu + r[is];
+		}
+		double z = stencil5<double, true>(off, dg, pc, pe, pw, pn, ps, gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
+		po[i] = pc;
+		a.z[f][i] = z;
+		acc[0] += pc * z;
+	}
+	double tot[1];
+	if (reduce_last<1>(acc, a.partials[f], &c->ticket, tot) && threadIdx.x == 0) c->nu = c->n0 / tot[0];
+}
+
+// x += p nu ; r -= z nu ; n1 = r.r ; mu = n1/n0 ; n0 = n1 ; stop test
+__global__ __launch_bounds__(256) void k_cg_update(CGArgs a, int nx, int ny, int pcur)
+{
+	const int f = blockIdx.y;
+	if (!a.x[f]) return;
+	SolveCtl* c = a.ctl[f];
+	if (c->done) return;
+	const size_t N = (size_t) nx * ny;
+	const double nu = c->nu;
+	double* x = a.x[f];
+	double* r = a.r[f];
+	const double* p = a.p[f][pcur];
+	const double* z = a.z[f];
+	double acc[1] = {0.0};
+	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
+		x[i] += p[i] * nu;
+		double ri = r[i] - z[i] * nu;
+		r[i] = ri;
+		acc[0] += ri * ri;
+	}
+	double tot[1];
+	if (reduce_last<1>(acc, a.partials[f], &c->ticket, tot) && threadIdx.x == 0) {
+		double n1 = tot[0];
+		c->mu = n1 / c->n0;
+		c->n0 = n1;
+		c->count += 1;
+		c->res = sqrt(n1);
+		c->done = !(sqrt(n1) > c->tol && c->count < c->max_iter);
+	}
+}
+
+__global__ __launch_bounds__(256) void k_sum(const double* __restrict__ x, size_t N, SolveCtl* ctl, double* partials)
+{
+	double acc[1] = {0.0};
+	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) acc[0] += x[i];
+	double tot[1];
+	if (reduce_last<1>(acc, partials, &ctl->ticket, tot) && threadIdx.x == 0) ctl->sum2 = tot[0];
+}
+
+static inline unsigned rblocks(size_t N)
+{
+	size_t b = (N + 255) / 256;
+	return (unsigned) (b < (size_t) MAX_RED_BLOCKS ? b : (size_t) MAX_RED_BLOCKS);
+}
+
+int launch_sum(Ctx* c, const double* x, size_t N, double* out_host)
+{
+	SolveCtl h{};
+	AFM_HIP(hipMemcpyAsync(&c->ctl[3], &h, sizeof(SolveCtl), hipMemcpyHostToDevice, c->stream));
+	hipLaunchKernelGGL(k_sum, dim3(rblocks(N)), dim3(256), 0, c->stream, x, N, &c->ctl[3], c->partials);
+	AFM_LAUNCH_CHECK();
+	AFM_HIP(hipMemcpyAsync(&c->hctl[3], &c->ctl[3], sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
+	AFM_HIP(hipStreamSynchronize(c->stream));
+	*out_host = c->hctl[3].sum2;
+	return 0;
+}
+
+// -------------------------------------------------------------- host ops --
+static inline dim3 pgrid(size_t N) { return dim3((unsigned) ((N + 255) / 256)); }
+
+int op_div(Ctx* c, const double* u, const double* v, double* d)
+{
+	hipLaunchKernelGGL(k_div, pgrid(c->N), dim3(256), 0, c->stream, u, v, d, (int) c->nx, (int) c->ny, c->n);
+	AFM_LAUNCH_CHECK();
+	return 0;
+}
+
+int op_grad_sub(Ctx* c, const double* p, double* u, double* v)
+{
+	hipLaunchKernelGGL(k_grad_sub, pgrid(c->N), dim3(256), 0, c->stream, p, u, v, (int) c->nx, (int) c->ny, c->n);
+	AFM_LAUNCH_CHECK();
+	return 0;
+}
+
+int op_advect(Ctx* c, const double* u, const double* v, double* ua, double* va)
+{
+	hipLaunchKernelGGL(k_advect, pgrid(c->N), dim3(256), 0, c->stream, u, v, ua, va, (int) c->nx, (int) c->ny, c->n);
+	AFM_LAUNCH_CHECK();
+	return 0;
+}
+
+static void helm_coef(const Ctx* c, double scale, double& off, double& dg)
+{
+	off = scale * c->n * c->n;          // ns/helmholtz.h:68
+	dg = 1. - 4. * off;                 // I + s L5 on the periodic plane
+}
+
+int op_helm_apply(Ctx* c, double scale, const double* x, double* y)
+{
+	double off, dg;
+	helm_coef(c, scale, off, dg);
+	hipLaunchKernelGGL(k_helm_apply, pgrid(c->N), dim3(256), 0, c->stream, x, y, (int) c->nx, (int) c->ny, off, dg);
+	AFM_LAUNCH_CHECK();
+	return 0;
+}
+
+// CG on up to two fields at once.  Scratch: r, z, p0, p1 per field.
+struct CGWork {
+	double *r[2] = {nullptr, nullptr}, *z[2] = {nullptr, nullptr}, *p[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+};
+
+static int dalloc(std::vector<void*>& own, void** p, size_t bytes)
+{
+	AFM_HIP(hipMalloc(p, bytes));
+	own.push_back(*p);
+	return 0;
+}
+
+static int cg_alloc(Ctx* c, CGWork& w, std::vector<void*>& own)
+{
+	for (int f = 0; f < 2; ++f) {
+		AFM_TRY(dalloc(own, (void**) &w.r[f], c->N * 8));
+		AFM_TRY(dalloc(own, (void**) &w.z[f], c->N * 8));
+		AFM_TRY(dalloc(own, (void**) &w.p[f][0], c->N * 8));
+		AFM_TRY(dalloc(own, (void**) &w.p[f][1], c->N * 8));
+	}
+	return 0;
+}
+
+// Solves (I + scale L) x_f = b_f for the non-null fields; iters[f] receives
+// the iteration counts.
+int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* const b[2], double tol, int tol_mode,
+           int max_iters, int iters[2], double res[2])
+{
+	double off, dg;
+	helm_coef(c, scale, off, dg);
+	CGArgs a{};
+	for (int f = 0; f < 2; ++f) {
+		a.x[f] = x[f];
+		a.b[f] = b[f];
+		a.r[f] = w.r[f];
+		a.z[f] = w.z[f];
+		a.p[f][0] = w.p[f][0];
+		a.p[f][1] = w.p[f][1];
+		a.ctl[f] = &c->ctl[1 + f];
+		a.partials[f] = c->partials + (size_t) f * 2 * MAX_RED_BLOCKS;
+		SolveCtl h{};
+		h.tol = tol; h.max_iter = max_iters; h.done = x[f] ? 0 : 1;
+		AFM_HIP(hipMemcpyAsync(&c->ctl[1 + f], &h, sizeof(SolveCtl), hipMemcpyHostToDevice, c->stream));
+	}
+	a.tol_mode = tol_mode;
+	const unsigned rb = rblocks(c->N);
+	hipLaunchKernelGGL(k_cg_init, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg);
+	AFM_LAUNCH_CHECK();
+	int enq = 0;
+	int batch = c->last_cg_iters > 0 ? c->last_cg_iters : 4;
+	for (;;) {
+		if (batch > max_iters - enq) batch = max_iters - enq;
+		for (int s = 0; s < batch; ++s) {
+			int pin = enq & 1;
+			hipLaunchKernelGGL(k_cg_pz, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg, pin);
+			hipLaunchKernelGGL(k_cg_update, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, pin ^ 1);
+			AFM_LAUNCH_CHECK();
+			++enq;
+		}
+		AFM_HIP(hipMemcpyAsync(&c->hctl[1], &c->ctl[1], 2 * sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
+		AFM_HIP(hipStreamSynchronize(c->stream));
+		bool all = c->hctl[1].done && c->hctl[2].done;
+		if (all || enq >= max_iters) break;
+		batch = 2;
+	}
+	int mx = 0;
+	for (int f = 0; f < 2; ++f) {
+		if (iters) iters[f] = x[f] ? c->hctl[1 + f].count : 0;
+		if (res) res[f] = x[f] ? c->hctl[1 + f].res : 0.0;
+		if (x[f] && c->hctl[1 + f].count > mx) mx = c->hctl[1 + f].count;
+	}
+	c->last_cg_iters = mx;
+	for (int f = 0; f < 2; ++f)
+		if (x[f] && !c->hctl[1 + f].done) { set_error("CG did not converge within max_iters"); return -4; }
+	return 0;
+}
+
+// ------------------------------------------------------------ NS solver ---
+struct NS {
+	Ctx* c = nullptr;
+	double mu, rho, dt, tol;
+	int tol_mode, precision;
+	double *u, *v;                         // caller-owned, aliased (ns/solver.h:54-56)
+	double *uo = nullptr, *vo = nullptr, *rx = nullptr, *ry = nullptr, *phi = nullptr, *fz = nullptr;
+	CGWork cg;
+	bool half = false;
+	int last_cycles = 0, last_cg[2] = {0, 0};
+	int max_cycles = 200, max_cg = 10000;
+	std::vector<void*> own;
+	~NS() { for (void* p : own) (void) hipFree(p); }
+};
+
+int ns_project(NS* s, double* u, double* v)
+{
+	Ctx* c = s->c;
+	AFM_TRY(op_div(c, u, v, s->phi));
+	int cyc = 0;
+	AFM_TRY(mg_solve(c, s->phi, s->tol, s->tol_mode, 2, 2, s->precision, s->max_cycles, &cyc, nullptr));
+	s->last_cycles = cyc;
+	AFM_TRY(op_grad_sub(c, s->phi, u, v));
+	return 0;
+}
+
+int ns_create(Ctx* c, double mu, double rho, double dt, double tol, int tol_mode, int precision, double* u0,
+              double* v0, NS** out)
+{
+	NS* s = new NS();
+	s->c = c; s->mu = mu; s->rho = rho; s->dt = dt; s->tol = tol; s->tol_mode = tol_mode; s->precision = precision;
+	s->u = u0; s->v = v0;
+	int rc = 0;
+	std::vector<void*>& o = s->own;
+	if ((rc = dalloc(o, (void**) &s->uo, c->N * 8)) || (rc = dalloc(o, (void**) &s->vo, c->N * 8)) ||
+	    (rc = dalloc(o, (void**) &s->rx, c->N * 8)) || (rc = dalloc(o, (void**) &s->ry, c->N * 8)) ||
+	    (rc = dalloc(o, (void**) &s->phi, c->N * 8)) || (rc = cg_alloc(c, s->cg, o))) {
+		delete s;
+		return rc;
+	}
+	AFM_HIP(hipMemsetAsync(s->uo, 0, c->N * 8, c->stream));
+	AFM_HIP(hipMemsetAsync(s->vo, 0, c->N * 8, c->stream));
+	rc = ns_project(s, s->u, s->v);     // ns/solver.h:71
+	if (rc) { delete s; return rc; }
+	*out = s;
+	return 0;
+}
+
+// ns/solver.h:90-140
+int ns_step(NS* s, const double* fx, const double* fy)
+{
+	Ctx* c = s->c;
+	const double sm = s->mu * s->dt / (2 * s->rho);      // phhm scale
+	const double ss = -s->mu * s->dt / (2 * s->rho);     // phhs scale
+	double hoff, hdg;
+	helm_coef(c, sm, hoff, hdg);
+	hipLaunchKernelGGL(k_advect_rhs, pgrid(c->N), dim3(256), 0, c->stream, s->u, s->v, fx, fy, s->uo, s->vo, s->rx,
+	                   s->ry, (int) c->nx, (int) c->ny, c->n, s->dt, s->rho, s->half ? 0 : 1, hoff, hdg);
+	AFM_LAUNCH_CHECK();
+	double* xs[2] = {s->u, s->v};
+	const double* bs[2] = {s->rx, s->ry};
+	AFM_TRY(op_cg2(c, s->cg, ss, xs, bs, s->tol, s->tol_mode, s->max_cg, s->last_cg, nullptr));
+	AFM_TRY(ns_project(s, s->u, s->v));
+	s->half = !s->half;
+	return 0;
+}
+
+// ns/solver.h:86: p = phhs * phidt
+int ns_get_p(NS* s, double* p)
+{
+	return op_helm_apply(s->c, -s->mu * s->dt / (2 * s->rho), s->phi, p);
+}
+
+// ------------------------------------------------------------ IB driver ---
+// One dt (the reference has no driver; its intended coupling is the comment
+// at test/ib_ns_test.cc:80-90):
+//   F = -k (X - R);  f = S(X) F;              BE half step -> u^{n+1/2}
+//   X_h = X + (dt/2) J(X) u^{n+1/2}
+//   F = -k (X_h - R);  f = S(X_h) F;          CN full step -> u^{n+1}
+//   X = X + dt (0.5 (J(X_h) u^n + J(X_h) u^{n+1}))
+struct IB {
+	NS* ns = nullptr;
+	long m = 0;
+	double *X = nullptr, *R = nullptr, *k = nullptr, *ds = nullptr, *Xh = nullptr, *fx = nullptr, *fy = nullptr;
+	std::vector<void*> own;
+	~IB() { for (void* p : own) (void) hipFree(p); }
+};
+
+int ib_create(NS* ns, long m, const double* X, const double* R, const double* k, const double* ds, IB** out)
+{
+	Ctx* c = ns->c;
+	IB* b = new IB();
+	b->ns = ns;
+	b->m = m;
+	size_t mb = (size_t) (m > 0 ? m : 1) * 8;
+	int rc = 0;
+	std::vector<void*>& o = b->own;
+	if ((rc = dalloc(o, (void**) &b->X, 2 * mb)) || (rc = dalloc(o, (void**) &b->R, 2 * mb)) ||
+	    (rc = dalloc(o, (void**) &b->k, mb)) || (rc = dalloc(o, (void**) &b->ds, mb)) ||
+	    (rc = dalloc(o, (void**) &b->Xh, 2 * mb)) || (rc = dalloc(o, (void**) &b->fx, c->N * 8)) ||
+	    (rc = dalloc(o, (void**) &b->fy, c->N * 8)) || (rc = ensure_markers(c, m))) {
+		delete b;
+		return rc;
+	}
+	if (m > 0) {
+		AFM_HIP(hipMemcpyAsync(b->X, X, 2 * m * 8, hipMemcpyDeviceToDevice, c->stream));
+		AFM_HIP(hipMemcpyAsync(b->R, R, 2 * m * 8, hipMemcpyDeviceToDevice, c->stream));
+		AFM_HIP(hipMemcpyAsync(b->k, k, m * 8, hipMemcpyDeviceToDevice, c->stream));
+		AFM_HIP(hipMemcpyAsync(b->ds, ds, m * 8, hipMemcpyDeviceToDevice, c->stream));
+	}
+	*out = b;
+	return 0;
+}
+
+int ib_step(IB* b)
+{
+	NS* s = b->ns;
+	Ctx* c = s->c;
+	const double dt = s->dt;
+	AFM_TRY(launch_spread(c, 3, nullptr, nullptr, b->ds, b->X, b->R, b->k, b->m, b->fx, b->fy));
+	AFM_TRY(ns_step(s, b->fx, b->fy));
+	AFM_TRY(launch_ib_move_half(c, s->u, s->v, b->X, b->m, 0.5 * dt, b->Xh));
+	AFM_TRY(launch_spread(c, 3, nullptr, nullptr, b->ds, b->Xh, b->R, b->k, b->m, b->fx, b->fy));
+	AFM_TRY(ns_step(s, b->fx, b->fy));
+	AFM_TRY(launch_ib_move_full(c, s->uo, s->vo, s->u, s->v, b->Xh, b->m, dt, b->X));
+	return 0;
+}
+
+int ib_markers(IB* b, double* X_out)
+{
+	Ctx* c = b->ns->c;
+	if (b->m > 0) AFM_HIP(hipMemcpyAsync(X_out, b->X, 2 * b->m * 8, hipMemcpyDeviceToDevice, c->stream));
+	return 0;
+}
+
+Ctx* ib_ctx(IB* b) { return b->ns->c; }
+void ib_free(IB* b) { (void) hipStreamSynchronize(b->ns->c->stream); delete b; }
+Ctx* ns_ctx(NS* s) { return s->c; }
+void ns_free(NS* s) { (void) hipStreamSynchronize(s->c->stream); delete s; }
+
+int ns_stats(NS* s, int* cyc, int* cgu, int* cgv, int* half)
+{
+	if (cyc) *cyc = s->last_cycles;
+	if (cgu) *cgu = s->last_cg[0];
+	if (cgv) *cgv = s->last_cg[1];
+	if (half) *half = s->half ? 1 : 0;
+	return 0;
+}
+
+int ns_old(NS* s, const double** uo, const double** vo)
+{
+	*uo = s->uo;
+	*vo = s->vo;
+	return 0;
+}
+
+// Single-field CG with scratch owned for the duration of the call.
+int op_cg_single(Ctx* c, double scale, const double* b, double* x, double tol, int tol_mode, int max_iters,
+                 int* iters, double* final_norm)
+{
+	std::vector<void*> own;
+	CGWork w;
+	int rc = cg_alloc(c, w, own);
+	if (rc == 0) {
+		double* xs[2] = {x, nullptr};
+		const double* bs[2] = {b, nullptr};
+		int it[2];
+		double res[2];
+		rc = op_cg2(c, w, scale, xs, bs, tol, tol_mode, max_iters, it, res);
+		if (iters) *iters = it[0];
+		if (final_norm) *final_norm = res[0];
+	}
+	hipStreamSynchronize(c->stream);
+	for (void* p : own) (void) hipFree(p);
+	return rc;
+}
+
+}  // namespace afm

@@ -1,0 +1,166 @@
+#!/usr/bin/env python
+"""IB time steps per second on one MI355X.
+
+Workload (the reference's 2048^2 benchmark configuration): doubly periodic
+2048x2048 MAC grid; 64 elastic ellipses (centres U[0.1,0.9]^2, semi-axes
+U[0.02,0.05]) of 256 markers each (16,384 markers) tethered with k = 1 to
+equal-area circles; initial velocity from seeded Fourier modes |k| <= 8
+(|u_k| ~ |k|^-2, exactly divergence-free, RMS 1); rho = 1, mu = 1e-3,
+dt = 2e-5.  One step = spread, backward-Euler half step (CG Helmholtz x2 +
+multigrid projection), interpolation + marker update, spread, Crank-Nicolson
+full step, interpolation + marker update.  Solves stop at 1e-10 relative
+residual (multigrid: RMS; CG: 2-norm).
+
+Two forms are timed on identical inputs: plain FP64, and FP32 V(2,2) cycles
+inside FP64 iterative refinement (`value`, the headline).  Inputs are resident
+in HBM before the timed region; the timed region is K full steps bracketed by
+device synchronisation.
+
+The JSON line also carries the roofline of the dominant kernel (the fused
+2-sweep red-black smoother at 2048^2, timed with HIP events on the stream it
+runs on) and the CPU baseline (the oracle's restatement of the reference path:
+lexicographic Gauss-Seidel V(2,2), CSR CG, one core) on a one-step sample.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--n", type=int, default=2048)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--forms", default="fp64,mixed")
+    return p.parse_args()
+
+
+def run_form(A, W, ctx, precision, args, inputs, dev):
+    import torch
+    u0, v0, X, R, k, ds = inputs
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)
+    u, v = t(u0), t(v0)
+    ns = A.NSSolver(ctx, 1e-3, 1.0, 2e-5, 1e-10, u, v, tol_mode=1, precision=precision)
+    ib = A.IBSolver(ns, t(X), t(R), t(k), t(ds))
+    ib.step(args.warmup)
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ib.step(args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.sync()
+    st = ns.stats()
+    out = {"ms_per_step": 1e3 * (t1 - t0) / args.steps, "steps_per_s": args.steps / (t1 - t0),
+           "last_mg_cycles": st["cycles"], "last_cg_iters": [st["cg_u"], st["cg_v"]]}
+    ib.close(); ns.close()
+    return out
+
+
+def smoother_roofline(A, ctx, dev, reps=50):
+    """Average duration of the fused 2-sweep red-black smoother at the finest
+    level, timed with HIP events on the stream it is launched on (torch's
+    current stream, to which the context is bound)."""
+    import torch
+    from afmibins_amd import _lib
+    n = ctx.n
+    b = torch.randn((n, n), dtype=torch.float64, device=dev)
+    x = [torch.randn((n, n), dtype=torch.float64, device=dev) for _ in range(2)]
+    L = _lib.load()
+    for i in range(5):
+        ctx._call("afm_mg_smooth", ctx.levels, C.c_void_p(x[i & 1].data_ptr()), C.c_void_p(b.data_ptr()),
+                  C.c_void_p(x[(i + 1) & 1].data_ptr()), 2, 0)
+    s = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for i in range(reps):
+        ctx._call("afm_mg_smooth", ctx.levels, C.c_void_p(x[i & 1].data_ptr()), C.c_void_p(b.data_ptr()),
+                  C.c_void_p(x[(i + 1) & 1].data_ptr()), 2, 0)
+    e1.record(s)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    bytes_per_launch = 24.0 * n * n        # read x, read b, write x (2 sweeps fused)
+    gbs = bytes_per_launch / (ms * 1e-3) / 1e9
+    del L
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_smooth<double,2,M_NORMAL> (2 fused red-black sweeps)",
+            "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(ms * 1e3, 2)}
+
+
+def cpu_baseline(inputs, n):
+    from oracle import oracle as O
+    u0, v0, X, R, k, ds = inputs
+    O.ib_run(u0, v0, X, R, k, ds, n, 1e-3, 1.0, 2e-5, 1e-10, 1, tol_mode=1, smoother=0, delta_mode=0)
+    sec = O.lib().or_last_step_seconds()
+    return {"value": round(1.0 / sec, 4), "unit": "IB steps/s", "cores": 1, "kind": "port",
+            "sample": f"1 IB step of the same {n}x{n} / {X.shape[0]}-marker workload on one host core "
+                      f"({sec:.1f} s): the oracle's restatement of the reference path (lexicographic "
+                      f"Gauss-Seidel V(2,2) multigrid, CSR CG, glibc-cos delta)",
+            "host_nproc": os.cpu_count()}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(the path is single-GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import afmibins_amd as A
+    from afmibins_amd import workloads as W
+    dev = torch.device("cuda", 0)
+    n = args.n
+    inputs = W.config2_inputs(n=n)
+    ctx = A.Context(n, 0)
+    forms = {}
+    for name in args.forms.split(","):
+        prec = 0 if name == "fp64" else 1
+        forms[name] = run_form(A, W, ctx, prec, args, inputs, dev)
+        print(f"[bench] {name}: {forms[name]['ms_per_step']:.3f} ms/step, "
+              f"cycles {forms[name]['last_mg_cycles']}, cg {forms[name]['last_cg_iters']}", file=sys.stderr, flush=True)
+    roof = smoother_roofline(A, ctx, dev)
+    print(f"[bench] smoother: {roof['avg_launch_us']} us, {roof['achieved']} GB/s", file=sys.stderr, flush=True)
+    head = forms.get("mixed", forms.get("fp64"))
+    m = inputs[2].shape[0]
+    line = {
+        "metric": "IB steps/s @2048² grid, 16k markers: FP64 vs FP32-MG+FP64 refine at 1e-10 resid",
+        "value": round(head["steps_per_s"], 3),
+        "unit": "IB steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (f32 V-cycles in f64 refinement)" if "mixed" in forms else "f64",
+        "data": "synthetic (seeded membranes and Fourier velocity)",
+        "config": {"workload": f"2-D periodic IB, {n}x{n} MAC grid, {m} markers (64 membranes x 256), "
+                               "mu=1e-3, rho=1, dt=2e-5, V(2,2) MG + CG Helmholtz to 1e-10 relative",
+                   "grid": n, "markers": m},
+        "forms": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in f.items()}
+                  for k, f in forms.items()},
+        "marker_steps_per_s": round(head["steps_per_s"] * m, 1),
+        "roofline": roof,
+    }
+    if "fp64" in forms and "mixed" in forms:
+        line["mixed_over_fp64"] = round(forms["mixed"]["steps_per_s"] / forms["fp64"]["steps_per_s"], 3)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(inputs, n)
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,139 @@
+/*
+ * afm.h -- C ABI of libafm, the gfx950 (MI355X) implementation of the
+ * immersed-boundary hot path of the reference IB/Navier-Stokes library.
+ *
+ * Scope: the doubly periodic 2-D MAC grid (the reference's periodic (x, z)
+ * plane; u on x-faces at (i, j+1/2)/n, v on y-faces at (i+1/2, j)/n,
+ * pressure cell-centred), unit-square domains, n a power of two.
+ *
+ * Conventions
+ *   - Every pointer argument is DEVICE memory unless the name ends in _host.
+ *     Fields are nx*ny doubles, x fastest (idx = i + nx*j), as the reference
+ *     stores them (ib/eulerian_grid.h:146-148).  Markers are m (x, y) pairs,
+ *     interleaved.
+ *   - Calls are ordered on the context's stream.  Functions that return an
+ *     iteration count synchronise; the others are asynchronous.
+ *   - Return value 0 = success; -1 invalid argument; -3 HIP error;
+ *     -4 iteration limit reached (the reference loops forever instead).
+ *     afm_last_error() describes the last failure of the calling thread.
+ *   - One context per host thread; a context is not re-entrant.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * the reference's src/ tree).
+ */
+#ifndef AFM_H
+#define AFM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct afm_ctx afm_ctx;
+typedef struct afm_ns afm_ns;
+typedef struct afm_ib afm_ib;
+
+/* dt::geometry<Level, Width, Height, Depth> (dt/geometry.h:29-44), chosen at
+ * run time.  dim must be 2; n = 2^Level >= 8; width = depth = 1. */
+typedef struct afm_geom {
+	int dim;
+	unsigned n;
+	unsigned width;
+	unsigned depth;
+} afm_geom;
+
+/* Context: geometry, multigrid hierarchy and scratch on one device. */
+int afm_ctx_create(const afm_geom* g, int device, afm_ctx** out);
+int afm_ctx_destroy(afm_ctx* c);
+int afm_ctx_info(afm_ctx* c, unsigned* nx, unsigned* ny, unsigned* levels, unsigned* coarse_level);
+int afm_sync(afm_ctx* c);
+/* Order the context's work on an existing HIP stream (NULL: the context's
+ * own stream).  Pending work on the previous stream is waited for first. */
+int afm_set_stream(afm_ctx* c, void* hip_stream);
+const char* afm_last_error(void);
+
+/* ib::grid::delta_function (ib/grid.h:51-55) for n = the context's n:
+ * out[i] = delta(r[i]).  afm_delta_host evaluates the same code on the CPU. */
+int afm_delta(afm_ctx* c, const double* r, long count, double* out);
+int afm_delta_host(const double* r_host, long count, unsigned n, double* out_host);
+
+/* ib::grid::interpolate with marker targets (ib/grid.h:68-84 through
+ * ib/mac_grid.h:244-257): U[k] = sum over the 4x4 window of
+ * (delta_x delta_y h^2) field[idx].  comp 0 = u (x-faces), 1 = v (y-faces). */
+int afm_interp(afm_ctx* c, int comp, const double* field, const double* X, long m, double* U);
+
+/* ib::grid::interpolate with grid targets (ib/grid.h:68-84 with a
+ * lagrangian_grid source, ib/lagrangian_grid.h:58-67): f = sum_k
+ * (delta_x delta_y ds_k) F_k, summed per cell in ascending marker order
+ * (bit-reproducible; no float atomics).  f is overwritten. */
+int afm_spread(afm_ctx* c, int comp, const double* F, const double* ds, const double* X, long m, double* f);
+
+/* mg::laplacian<G> * x (mg/laplacian.h:46-153, dt/matrix.h:81-94) at a
+ * multigrid level (0 = 1x1 ... levels = finest). */
+int afm_laplacian_apply(afm_ctx* c, unsigned level, const double* x, double* y);
+
+/* Multigrid building blocks at a global level (n >= 64):
+ * red-black Gauss-Seidel sweeps of L x = b (alg/smooth.h:35-51 per row),
+ * x_out != x_in; full-weighting restriction (mg/solver.h:153-181);
+ * bilinear prolongation, overwriting fine (mg/solver.h:71-151);
+ * one V-cycle on the residual equation, rhs replaced by the correction
+ * (mg/solver.h:199-216). */
+int afm_mg_smooth(afm_ctx* c, unsigned level, const double* x_in, const double* b, double* x_out, int sweeps,
+                  int zero_init);
+int afm_mg_restrict(afm_ctx* c, unsigned level, const double* fine, double* coarse);
+int afm_mg_prolong(afm_ctx* c, unsigned level, const double* coarse, double* fine);
+int afm_mg_vcycle(afm_ctx* c, double* rhs_inout, int nu1, int nu2);
+
+/* mg::solver<G>(tol, nu1, nu2).solve(rhs) (mg/solver.h:218-237): rhs is
+ * replaced by the solution (its mean is arbitrary, as in the reference).
+ * tol_mode 0: stop when RMS(rhs - L x) <= tol (the reference's test);
+ * tol_mode 1: when RMS(r) <= tol * RMS(rhs - mean(rhs)).
+ * precision 0: FP64 throughout; 1: FP32 V-cycles inside FP64 iterative
+ * refinement (FP64 residual, update, reductions and coarsest solve).
+ * nu1, nu2 in {1, 2}.  *cycles = number of V-cycles (the reference's return). */
+int afm_mg_solve(afm_ctx* c, double* rhs_inout, double tol, int tol_mode, int nu1, int nu2, int precision,
+                 int max_cycles, int* cycles, double* final_rms);
+
+/* (I + scale L5) x (ns/helmholtz.h:61-175 for the periodic plane) and
+ * alg::conjugate_gradient on it (alg/cg.h:32-56): x is the initial guess and
+ * the result.  tol_mode 0: stop when |r|_2 <= tol (the reference's test);
+ * 1: when |r|_2 <= tol |b|_2. */
+int afm_helmholtz_apply(afm_ctx* c, double scale, const double* x, double* y);
+int afm_helmholtz_cg(afm_ctx* c, double scale, const double* b, double* x, double tol, int tol_mode, int max_iters,
+                     int* iters, double* final_norm);
+
+/* alg::div (alg/div.h:29-60), alg::grad + subtraction (alg/grad.h:29-44,
+ * ns/projection.h:69-72), ns::solver::advection (ns/solver.h:142-236). */
+int afm_div(afm_ctx* c, const double* u, const double* v, double* d);
+int afm_grad_sub(afm_ctx* c, const double* p, double* u, double* v);
+int afm_advect(afm_ctx* c, const double* u, const double* v, double* u_adv, double* v_adv);
+
+/* ns::solver<G>(mu, rho, dt, tol, u0, v0, w0) (ns/solver.h:61-72): u0, v0 are
+ * aliased and projected at creation; step() advances them in place
+ * (ns/solver.h:90-140: the first call is the backward-Euler half step, the
+ * second the Crank-Nicolson full step, alternating).  tol/tol_mode apply to
+ * both the multigrid and the CG solves; precision as for afm_mg_solve. */
+int afm_ns_create(afm_ctx* c, double mu, double rho, double dt, double tol, int tol_mode, int precision, double* u0,
+                  double* v0, afm_ns** out);
+int afm_ns_step(afm_ns* s, const double* fx, const double* fy);
+int afm_ns_project(afm_ns* s, double* u, double* v);             /* ns/projection.h:60-73 */
+int afm_ns_get_p(afm_ns* s, double* p);                         /* ns/solver.h:86 */
+int afm_ns_stats(afm_ns* s, int* last_cycles, int* last_cg_u, int* last_cg_v, int* have_half_stepped);
+int afm_ns_old(afm_ns* s, const double** u_old, const double** v_old);
+int afm_ns_destroy(afm_ns* s);
+
+/* Immersed-boundary time stepping (no reference counterpart: the reference
+ * stops at the coupling comment of test/ib_ns_test.cc:80-90).  Markers with
+ * tether springs F = -k (X - R) and arclength weights ds; one step is
+ *   f = S(X) F(X);  BE half step;  X_h = X + dt/2 J(X) u;
+ *   f = S(X_h) F(X_h);  CN full step;  X += dt/2 (J(X_h) u_old + J(X_h) u).
+ * X, R, k, ds are copied at creation. */
+int afm_ib_create(afm_ns* s, long m, const double* X, const double* R, const double* k, const double* ds,
+                  afm_ib** out);
+int afm_ib_step(afm_ib* b, int steps);
+int afm_ib_markers(afm_ib* b, double* X_out);
+int afm_ib_destroy(afm_ib* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

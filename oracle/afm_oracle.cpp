@@ -38,6 +38,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <chrono>
 #include <quadmath.h>
 
 namespace {
@@ -819,6 +820,11 @@ void or_advect2(unsigned nx, unsigned ny, unsigned n, const double* u, const dou
 	std::copy(b.begin(), b.end(), va);
 }
 
+// Wall time of the step loop of the last or_ib_run (excludes the initial
+// projection and setup), for the CPU baseline.
+static double g_last_step_seconds = 0.0;
+double or_last_step_seconds(void) { return g_last_step_seconds; }
+
 // ------------------------------------------------------- IB time stepping
 // Build-defined driver (the reference has none; its intended coupling is
 // the comment at test/ib_ns_test.cc:80-90).  Per dt:
@@ -901,7 +907,9 @@ unsigned or_ib_run(unsigned nx, unsigned ny, unsigned n, double mu, double rho, 
 	s.mode = delta_mode;
 	NS2 ns(s.g, mu, rho, dt, tol, tol_mode, smoother, &s.u, &s.v);
 	s.ns = &ns;
+	auto t0 = std::chrono::steady_clock::now();
 	for (int i = 0; i < steps; ++i) ib_step(s);
+	g_last_step_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 	std::copy(s.u.begin(), s.u.end(), u);
 	std::copy(s.v.begin(), s.v.end(), v);
 	std::copy(s.X.begin(), s.X.end(), X);

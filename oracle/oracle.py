@@ -76,6 +76,7 @@ def lib():
             "or_grad2": (None, [_U, _U, _U, _P, _P, _P]),
             "or_advect2": (None, [_U, _U, _U, _P, _P, _P, _P]),
             "or_ib_run": (_U, [_U, _U, _U, _D, _D, _D, _D, _I, _I, _I, _P, _P, _L, _P, _P, _P, _P, _I, _P]),
+            "or_last_step_seconds": (_D, []),
             "or_ns_run": (_U, [_U, _U, _U, _D, _D, _D, _D, _I, _I, _P, _P, _P, _P, _I, _P]),
         }
         for name, (res, args) in sig.items():

@@ -1,0 +1,301 @@
+"""Pins the CPU oracle: against the reference's own code (oracle/_ref, built
+from the reference headers by oracle/Makefile), against the reference's test
+fixtures, and against closed forms where the reference cannot run (the 2-D
+doubly periodic plane)."""
+import math
+
+import numpy as np
+import pytest
+
+P = None
+
+
+def _p(a):
+    from oracle.oracle import _ptr
+    return _ptr(a)
+
+
+# ----------------------------------------------------------- vs reference
+def test_delta_matches_reference(oracle, ref):
+    rng = np.random.default_rng(1)
+    for L in (4, 5, 8, 11, 12):
+        n = 2 ** L
+        r = np.concatenate([rng.uniform(-3.2 / n, 3.2 / n, 4000), np.arange(-12, 13) / (4.0 * n)])
+        ours = oracle.delta(r, n, 0)
+        theirs = np.array([ref.ref_delta(x, L) for x in r])
+        assert np.array_equal(ours, theirs)
+
+
+def test_delta_modes_differ_only_by_glibc_cos_rounding(oracle):
+    # mode 0 (glibc cos) and mode 1 (correctly rounded cos) agree except where
+    # glibc's cos is not correctly rounded: there they differ by one ulp of cos.
+    rng = np.random.default_rng(2)
+    n = 2048
+    r = rng.uniform(-2.0 / n, 2.0 / n, 100000)
+    a, b = oracle.delta(r, n, 0), oracle.delta(r, n, 1)
+    bad = np.nonzero(a != b)[0]
+    assert len(bad) < 0.005 * r.size
+    for i in bad[:50]:
+        arg = oracle.lib().or_delta_arg(r[i], n)
+        cg, cr = oracle.lib().or_cos_glibc(arg), oracle.lib().or_cos_cr(arg)
+        assert cg != cr and abs(cg - cr) <= np.spacing(abs(cr)) * 1.0001
+
+
+@pytest.mark.parametrize("geo,L", [(0, 4), (1, 5)])
+@pytest.mark.parametrize("comp", [0, 1, 2])
+def test_interp3_matches_reference(oracle, ref, geo, L, comp):
+    n = 2 ** L
+    M = ref.ref_markers()
+    rng = np.random.default_rng(10 * geo + comp)
+    X = np.ascontiguousarray(rng.uniform(3.5 / n, 1 - 3.5 / n, (M, 3)))
+    N = n * n * n if comp != 1 else n * (n - 1) * n
+    f = rng.standard_normal(N)
+    U = np.empty(M)
+    assert ref.ref_interp(geo, comp, _p(f), _p(X), _p(U)) == 0
+    assert np.array_equal(U, oracle.interp3(comp, f, X, n, mode=0))
+
+
+@pytest.mark.parametrize("comp", [0, 1, 2])
+def test_spread3_matches_reference_up_to_summation_order(oracle, ref, comp):
+    n, M = 16, ref.ref_markers()
+    rng = np.random.default_rng(30 + comp)
+    X = np.ascontiguousarray(rng.uniform(3.5 / n, 1 - 3.5 / n, (M, 3)))
+    F = rng.standard_normal(M)
+    ds = rng.uniform(0.5, 1.5, M) / n
+    N = n * n * n if comp != 1 else n * (n - 1) * n
+    fr = np.empty(N)
+    assert ref.ref_spread(0, comp, _p(F), _p(ds), _p(X), _p(fr)) == 0
+    fo = oracle.spread3(comp, F, ds, X, n, mode=0)
+    assert np.array_equal(fr != 0, fo != 0)
+    assert np.max(np.abs(fr - fo)) <= 1e-14 * np.max(np.abs(fr))
+
+
+GEOS = {0: (16, 16, 16, 16), 1: (32, 32, 32, 32), 2: (16, 16, 16, 2), 10: (16, 1, 16, 16), 11: (32, 1, 32, 32),
+        12: (64, 1, 64, 64)}
+
+
+@pytest.mark.parametrize("geo", [0, 1, 2, 10, 11, 12])
+def test_laplacian_spmv_matches_reference(oracle, ref, geo):
+    nx, ny, nz, n = GEOS[geo]
+    x = np.random.default_rng(geo).standard_normal(nx * ny * nz)
+    y = np.empty_like(x)
+    assert ref.ref_lap_apply(geo, _p(x), _p(y)) == 0
+    assert np.array_equal(y, oracle.stencil_apply(x, nx, ny, nz, n))
+
+
+@pytest.mark.parametrize("geo", [0, 1, 10, 11, 12])
+def test_lexicographic_smoother_matches_reference(oracle, ref, geo):
+    nx, ny, nz, n = GEOS[geo]
+    rng = np.random.default_rng(100 + geo)
+    x, b = rng.standard_normal(nx * ny * nz), rng.standard_normal(nx * ny * nz)
+    xr = x.copy()
+    assert ref.ref_smooth(geo, _p(xr), _p(b), 3) == 0
+    assert np.array_equal(xr, oracle.smooth(x, b, nx, ny, nz, n, smoother=0, sweeps=3))
+
+
+@pytest.mark.parametrize("geo,kind,scale", [(0, 0, 0.0), (10, 0, 0.0), (11, 0, 0.0), (0, 1, -1e-4), (1, 1, -3e-5)])
+def test_cg_matches_reference(oracle, ref, geo, kind, scale):
+    nx, ny, nz, n = GEOS[geo]
+    b = np.random.default_rng(200 + geo).standard_normal(nx * ny * nz)
+    if kind == 0:
+        b -= b.mean()
+    x0 = np.zeros_like(b)
+    xr = x0.copy()
+    it = ref.ref_cg(geo, kind, scale, _p(xr), _p(b), 1e-8)
+    xo, ito, _ = oracle.cg(x0, b, nx, ny, nz, n, kind, scale, 1e-8)
+    assert it == ito
+    assert np.array_equal(xr, xo)
+
+
+@pytest.mark.parametrize("geo", [0, 1])
+def test_helmholtz_csr_matches_reference(oracle, ref, geo):
+    nx, ny, nz, n = GEOS[geo]
+    nnz = ref.ref_csr(geo, 1, -2e-4, None, None, None)
+    rs = np.empty(nx * ny * nz + 1, dtype=np.int64)
+    col = np.empty(nnz, dtype=np.int64)
+    val = np.empty(nnz)
+    ref.ref_csr(geo, 1, -2e-4, _p(rs), _p(col), _p(val))
+    a, b, c = oracle.stencil_csr(nx, ny, nz, n, 1, -2e-4)
+    assert np.array_equal(rs, a) and np.array_equal(col, b) and np.array_equal(val, c)
+
+
+@pytest.mark.parametrize("geo", [0, 1])
+def test_restriction_prolongation_match_reference(oracle, ref, geo):
+    nx, ny, nz, n = GEOS[geo]
+    rng = np.random.default_rng(300 + geo)
+    f = rng.standard_normal(nx * ny * nz)
+    c8 = np.empty(nx * ny * nz // 8)
+    ref.ref_restrict(geo, _p(f), _p(c8))
+    assert np.array_equal(c8, oracle.restrict3(f, nx, ny, nz))
+    cc = rng.standard_normal(nx * ny * nz // 8)
+    fine = rng.standard_normal(nx * ny * nz)
+    fr = fine.copy()
+    ref.ref_prolong(geo, _p(cc), _p(fr))
+    assert np.array_equal(fr, oracle.prolong3(cc, fine, nx, ny, nz))
+    # 2-D prolongation = the reference's bilinear wall-row formula
+    # (mg/solver.h:141-148): its y = 0 fine row depends only on coarse row 0.
+    p2 = oracle.prolong2(cc.reshape(nz // 2, ny // 2, nx // 2)[:, 0, :])
+    assert np.array_equal(fr.reshape(nz, ny, nx)[:, 0, :], p2)
+
+
+def test_restriction_fixture_of_restr_test(oracle, ref):
+    # test/restr_test.cc:44-50 -- 2x2x2 -> 1: full weighting of 8 values.
+    fine = np.array([1.4457349937345032636e-19, -1.4222019353457418616e-19, 7.5757120710108818818e-20,
+                     1.8887852354789130725e-21, -1.0404187659601848787e-19, 9.443926177393482027e-22,
+                     1.888785235478816776e-21, 2.4074124304840448163e-35])
+    out = oracle.restrict3(fine, 2, 2, 2)
+    # every tap folds onto one of the 8 cells with total weight 1/8 each
+    assert out[0] == pytest.approx(fine.mean(), rel=1e-12, abs=1e-35)
+
+
+@pytest.mark.parametrize("geo,levels,W,H,D", [(0, 4, 1, 1, 1), (2, 1, 8, 8, 8)])
+def test_mg_solve_matches_reference(oracle, ref, geo, levels, W, H, D):
+    nx, ny, nz, n = GEOS[geo]
+    b = np.random.default_rng(400 + geo).uniform(0, 1, nx * ny * nz)
+    if geo == 2:   # test/mg_test2.cc:42-43: ones with a spike
+        b = np.ones(nx * ny * nz); b[0] = 2.0
+    br = b.copy()
+    c = ref.ref_mg_solve(geo, 1e-10, 2, 2, _p(br))
+    xo, co, _ = oracle.mg_solve(b, levels, dim=3, W=W, H=H, D=D, tol=1e-10, smoother=0)
+    assert c == co
+    assert np.array_equal(br, xo)
+
+
+@pytest.mark.parametrize("geo", [10, 11])
+def test_div_grad_2d_match_reference(oracle, ref, geo):
+    nx, ny, nz, n = GEOS[geo]
+    rng = np.random.default_rng(500 + geo)
+    u, w, p = rng.standard_normal(nx * nz), rng.standard_normal(nx * nz), rng.standard_normal(nx * nz)
+    v = np.zeros(1)
+    d, px, pz, py = np.empty_like(u), np.empty_like(u), np.empty_like(u), np.zeros(1)
+    ref.ref_div_grad(geo, _p(u), _p(v), _p(w), _p(d), _p(p), _p(px), _p(py), _p(pz))
+    d2 = oracle.div2(u.reshape(nz, nx), w.reshape(nz, nx), n)
+    a, b = oracle.grad2(p.reshape(nz, nx), n)
+    assert np.array_equal(d, d2.ravel()) and np.array_equal(px, a.ravel()) and np.array_equal(pz, b.ravel())
+
+
+# ------------------------------------------------------------ closed forms
+def test_delta_partition_of_unity_and_support(oracle):
+    n = 64
+    rng = np.random.default_rng(3)
+    for t in rng.uniform(0, 1, 200):
+        r = (np.arange(-1, 3) - t) / n      # the 4 taps around an offset t in [0,1)
+        s = oracle.delta(r, n, 1).sum() / n
+        assert abs(s - 1.0) < 4e-16
+    assert np.all(oracle.delta(np.array([2.0, -2.0]) / n, n, 1) == 0.0)
+    assert np.all(oracle.delta(np.array([3.0, -3.5, 10.0]) / n, n, 1) == 0.0)
+
+
+def test_interp2_reproduces_constants_and_linears(oracle):
+    n = 32
+    rng = np.random.default_rng(4)
+    X = rng.uniform(0.1, 0.9, (50, 2))
+    for comp in (0, 1):
+        U = oracle.interp2(comp, np.full((n, n), 3.0), X, n)
+        assert np.allclose(U, 3.0, rtol=1e-14)
+        # the cosine kernel has zero first moment only on average: a linear
+        # field is reproduced to O(h) (its even-odd sums are 1/2 each).
+        xs = np.arange(n) / n + (0.5 / n if comp == 1 else 0.0)
+        f = np.tile(xs, (n, 1))
+        U = oracle.interp2(comp, f, X, n)
+        assert np.max(np.abs(U - X[:, 0])) < 0.1 / n
+
+
+def test_spread_is_adjoint_of_interp(oracle):
+    # <S F, u>_h = <F, J u>_ds with the same weights (ib/grid.h:80 both ways)
+    n = 32
+    rng = np.random.default_rng(5)
+    X = rng.uniform(0, 1, (40, 2))
+    X[:20] = rng.uniform(3.0 / n, 1 - 3.0 / n, (20, 2))
+    F, ds = rng.standard_normal(40), rng.uniform(0.5, 1.5, 40) / n
+    u = rng.standard_normal((n, n))
+    for comp in (0, 1):
+        f = oracle.spread2(comp, F, ds, X, n, n, n)
+        J = oracle.interp2(comp, u, X, n)
+        lhs = np.sum(f * u) / n ** 2
+        rhs = np.sum(F * ds * J)
+        assert abs(lhs - rhs) <= 1e-13 * (abs(lhs) + abs(rhs))
+        # momentum: sum_grid f h^2 = sum_k F_k ds_k, for markers at least 3h
+        # from the seams (the reference evaluates delta at the unwrapped
+        # marker coordinate, ib/grid.h:73-79, so seam markers lose weight)
+        f = oracle.spread2(comp, F[:20], ds[:20], X[:20], n, n, n)
+        assert abs(f.sum() / n ** 2 - np.sum(F[:20] * ds[:20])) < 1e-13
+
+
+def test_restrict2_prolong2_closed_forms(oracle):
+    n = 16
+    one = np.ones((n, n))
+    assert np.array_equal(oracle.restrict2(one), np.ones((n // 2, n // 2)))
+    assert np.array_equal(oracle.prolong2(np.ones((n // 2, n // 2))), one)
+    # bilinear prolongation reproduces a coarse linear-in-x field away from the seam
+    xc = (np.arange(n // 2) + 0.5) / (n // 2)
+    fine = oracle.prolong2(np.tile(xc, (n // 2, 1)))
+    xf = (np.arange(n) + 0.5) / n
+    assert np.allclose(fine[:, 2:-2], np.tile(xf, (n, 1))[:, 2:-2], atol=1e-15)
+    # interior P = 4 R^T:  <R f, g>_coarse * 4 == <f, P g>_fine on periodic grids
+    rng = np.random.default_rng(6)
+    f, g = rng.standard_normal((n, n)), rng.standard_normal((n // 2, n // 2))
+    assert abs(4 * np.sum(oracle.restrict2(f) * g) - np.sum(f * oracle.prolong2(g))) < 1e-12
+
+
+def test_mg2_solves_fourier_mode_exactly(oracle):
+    # L u = f with f a Fourier mode: the discrete solution is f / lambda(k),
+    # lambda = -4 n^2 (sin^2(pi kx/n) + sin^2(pi ky/n)).
+    n, L = 64, 6
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    for kx, ky in ((1, 0), (2, 3), (7, 5)):
+        f = np.cos(2 * np.pi * (kx * X + ky * Y))
+        lam = -4 * n * n * (np.sin(np.pi * kx / n) ** 2 + np.sin(np.pi * ky / n) ** 2)
+        for sm in (0, 1):
+            u, cyc, _ = oracle.mg_solve(f, L, dim=2, tol=1e-13, tol_mode=1, smoother=sm)
+            u = u - u.mean()
+            assert np.max(np.abs(u - f / lam)) < 1e-11 * np.max(np.abs(f / lam)), (kx, ky, sm)
+            assert 3 <= cyc <= 30
+
+
+def test_rb_smoother_converges_like_lexicographic(oracle):
+    n, L = 64, 6
+    rhs = np.random.default_rng(7).standard_normal((n, n))
+    rhs -= rhs.mean()
+    _, c_rb, _ = oracle.mg_solve(rhs, L, dim=2, tol=1e-10, tol_mode=1, smoother=1)
+    _, c_lex, _ = oracle.mg_solve(rhs, L, dim=2, tol=1e-10, tol_mode=1, smoother=0)
+    assert abs(c_rb - c_lex) <= 3
+
+
+def test_helmholtz2_cg_solves_fourier_mode(oracle):
+    n, s = 32, -2e-4
+    x = np.arange(n) / n
+    X, Y = np.meshgrid(x, x)
+    f = np.sin(2 * np.pi * (3 * X + Y))
+    lam = 1 + s * (-4 * n * n * (np.sin(np.pi * 3 / n) ** 2 + np.sin(np.pi * 1 / n) ** 2))
+    sol, it, _ = oracle.cg(np.zeros(n * n), f.ravel(), n, 1, n, n, kind=2, scale=s, tol=1e-14)
+    assert np.max(np.abs(sol - (f / lam).ravel())) < 1e-13
+
+
+def test_div_of_grad_is_laplacian_and_projection_is_div_free(oracle):
+    n = 32
+    rng = np.random.default_rng(8)
+    p = rng.standard_normal((n, n))
+    gx, gy = oracle.grad2(p, n)
+    d = oracle.div2(gx, gy, n)
+    Lp = oracle.stencil_apply(p, n, 1, n, n).reshape(n, n)
+    assert np.allclose(d, Lp, rtol=0, atol=1e-10 * np.abs(Lp).max())
+
+
+def test_advection_of_uniform_flow_is_zero(oracle):
+    n = 16
+    u, v = np.full((n, n), 0.7), np.full((n, n), -0.3)
+    a, b = oracle.advect2(u, v, n)
+    assert np.all(a == 0) and np.all(b == 0)
+
+
+def test_ns_step_keeps_velocity_divergence_free(oracle):
+    from afmibins_amd import workloads as W
+    n = 32
+    u, v = W.fourier_velocity(n, 3, seed=1)
+    fx = np.random.default_rng(9).standard_normal((n, n))
+    fy = np.random.default_rng(10).standard_normal((n, n))
+    u2, v2, p, cyc = oracle.ns_run(u, v, fx, fy, n, 1e-2, 1.0, 1e-3, 1e-12, calls=2)
+    d = oracle.div2(u2, v2, n)
+    assert np.max(np.abs(d)) < 1e-9 * n * max(np.abs(u2).max(), 1)
