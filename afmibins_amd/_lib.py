@@ -28,6 +28,10 @@ SIGNATURES = {
     "afm_ctx_info": (_I, [_P, _UP, _UP, _UP, _UP]),
     "afm_sync": (_I, [_P]),
     "afm_set_stream": (_I, [_P, _P]),
+    "afm_alloc": (_I, [_P, C.c_ulong, C.POINTER(_P)]),
+    "afm_free": (_I, [_P, _P]),
+    "afm_copy_to_device": (_I, [_P, _P, _P, C.c_ulong]),
+    "afm_copy_to_host": (_I, [_P, _P, _P, C.c_ulong]),
     "afm_last_error": (C.c_char_p, []),
     "afm_delta": (_I, [_P, _P, _L, _P]),
     "afm_delta_host": (_I, [_P, _L, _U, _P]),
@@ -54,6 +58,7 @@ SIGNATURES = {
     "afm_ib_create": (_I, [_P, _L, _P, _P, _P, _P, C.POINTER(_P)]),
     "afm_ib_step": (_I, [_P, _I]),
     "afm_ib_markers": (_I, [_P, _P]),
+    "afm_ib_grad": (_I, [_P, _I, _I, _L, _DP, _P, _P]),
     "afm_ib_destroy": (_I, [_P]),
 }
 

@@ -270,6 +270,21 @@ class IBSolver:
             ctx._bind_stream()
         check(_lib.load().afm_ib_step(self._h, steps), "afm_ib_step")
 
+    def grad(self, steps, msize, checkpoint_every=20, need_grad=True):
+        """Run `steps` steps; return (J, dJ/dk, dJ/dR) by the discrete adjoint
+        (J = kinetic energy + centroid displacements of membranes of msize
+        markers).  Gradients are None with need_grad=False."""
+        ctx = self.ns.ctx
+        if torch.cuda.current_stream(ctx.device).cuda_stream != ctx._stream:
+            ctx._bind_stream()
+        J = C.c_double()
+        gk = torch.empty(self.m, dtype=torch.float64, device=ctx.device) if need_grad else None
+        gR = torch.empty((self.m, 2), dtype=torch.float64, device=ctx.device) if need_grad else None
+        check(_lib.load().afm_ib_grad(self._h, steps, checkpoint_every, msize, C.byref(J),
+                                      _ptr(gk) if need_grad else None, _ptr(gR) if need_grad else None),
+              "afm_ib_grad")
+        return J.value, gk, gR
+
     def markers(self):
         X = torch.empty((self.m, 2), dtype=torch.float64, device=self.ns.ctx.device)
         check(_lib.load().afm_ib_markers(self._h, _ptr(X)), "afm_ib_markers")

@@ -46,6 +46,7 @@ Ctx* ns_ctx(NS*);
 void ns_free(NS*);
 Ctx* ib_ctx(IB*);
 int ib_markers(IB*, double*);
+int ib_grad(IB*, int, int, long, double*, double*, double*);
 void ib_free(IB*);
 
 }  // namespace afm
@@ -141,6 +142,41 @@ int afm_sync(afm_ctx* c)
 		set_error("spreading: more than 4096 candidate markers around one 16x16 tile");
 		return -1;
 	}
+	return 0;
+}
+
+int afm_alloc(afm_ctx* c, unsigned long bytes, void** out)
+{
+	CHECK_ARG(c && out, "null argument");
+	AFM_TRY(use(c));
+	AFM_HIP(hipMalloc(out, bytes ? bytes : 8));
+	return 0;
+}
+
+int afm_free(afm_ctx* c, void* p)
+{
+	CHECK_ARG(c, "null argument");
+	AFM_TRY(use(c));
+	AFM_HIP(hipStreamSynchronize(c->stream));
+	if (p) AFM_HIP(hipFree(p));
+	return 0;
+}
+
+int afm_copy_to_device(afm_ctx* c, void* dst, const void* src, unsigned long bytes)
+{
+	CHECK_ARG(c && (bytes == 0 || (dst && src)), "null argument");
+	AFM_TRY(use(c));
+	AFM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+	AFM_HIP(hipStreamSynchronize(c->stream));
+	return 0;
+}
+
+int afm_copy_to_host(afm_ctx* c, void* dst, const void* src, unsigned long bytes)
+{
+	CHECK_ARG(c && (bytes == 0 || (dst && src)), "null argument");
+	AFM_TRY(use(c));
+	AFM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+	AFM_HIP(hipStreamSynchronize(c->stream));
 	return 0;
 }
 
@@ -336,6 +372,13 @@ int afm_ib_step(afm_ib* b, int steps)
 	AFM_TRY(use(ib_ctx(b->b)));
 	for (int i = 0; i < steps; ++i) AFM_TRY(ib_step(b->b));
 	return 0;
+}
+
+int afm_ib_grad(afm_ib* b, int steps, int every, long msize, double* J, double* gk, double* gR)
+{
+	CHECK_ARG(b, "null argument");
+	AFM_TRY(use(ib_ctx(b->b)));
+	return ib_grad(b->b, steps, every, msize, J, gk, gR);
 }
 
 int afm_ib_markers(afm_ib* b, double* X_out)

@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void k_spread_tile(int comp_mask, const Marker
                                                      const int* __restrict__ offs, const int* __restrict__ list,
                                                      unsigned nx, unsigned ny, int ntx, int nty,
                                                      double* __restrict__ fx, double* __restrict__ fy,
-                                                     int* __restrict__ err)
+                                                     int* __restrict__ err, int accumulate)
 {
 	__shared__ int cand[KMAX];
 	__shared__ int ncand;
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_spread_tile(int comp_mask, const Marker
 		K = KMAX;
 	}
 	if (K == 0) {
-		if (gx < (int) nx && gy < (int) ny) f[gx + (size_t) nx * gy] = 0.0;
+		if (!accumulate && gx < (int) nx && gy < (int) ny) f[gx + (size_t) nx * gy] = 0.0;
 		return;
 	}
 	// bitonic sort of the candidate marker indices (ascending)
@@ -222,7 +222,8 @@ __global__ __launch_bounds__(256) void k_spread_tile(int comp_mask, const Marker
 			acc += w * r.ds * r.F;
 		}
 	}
-	f[gx + (size_t) nx * gy] = acc;
+	const size_t o = gx + (size_t) nx * gy;
+	f[o] = accumulate ? f[o] + acc : acc;
 }
 
 int launch_delta(Ctx* c, const double* r, long count, unsigned n, double* out)
@@ -275,7 +276,7 @@ int ensure_markers(Ctx* c, long m)
 }
 
 int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, const double* ds, const double* X,
-                  const double* R, const double* k, long m, double* fx, double* fy)
+                  const double* R, const double* k, long m, double* fx, double* fy, int accumulate)
 {
 	const int nb1 = c->ntx * c->nty, nb = 2 * nb1;
 	int* counts = c->bins;
@@ -297,7 +298,7 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 		AFM_HIP(hipMemsetAsync(offs, 0, sizeof(int) * (nb + 1), c->stream));
 	}
 	hipLaunchKernelGGL(k_spread_tile, dim3(nb1, 2), dim3(256), 0, c->stream, comp_mask, c->rec, m, offs,
-	                   c->bin_list, c->nx, c->ny, c->ntx, c->nty, fx, fy, &c->ctl[3].err);
+	                   c->bin_list, c->nx, c->ny, c->ntx, c->nty, fx, fy, &c->ctl[3].err, accumulate);
 	AFM_LAUNCH_CHECK();
 	return 0;
 }

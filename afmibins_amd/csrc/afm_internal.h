@@ -2,6 +2,7 @@
 // HIP translation units of libafm.  Not part of the public C ABI.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -107,7 +108,8 @@ int ensure_markers(Ctx* c, long m);
 int launch_delta(Ctx* c, const double* r, long count, unsigned n, double* out);
 int launch_interp(Ctx* c, int comp, const double* field, const double* X, long m, double* U);
 int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, const double* ds,
-                  const double* X, const double* R, const double* k, long m, double* fx, double* fy);
+                  const double* X, const double* R, const double* k, long m, double* fx, double* fy,
+                  int accumulate = 0);
 int launch_ib_move_half(Ctx* c, const double* u, const double* v, const double* X, long m, double half, double* Xh);
 int launch_ib_move_full(Ctx* c, const double* u_old, const double* v_old, const double* u, const double* v,
                         const double* Xh, long m, double dt, double* X);
@@ -124,6 +126,18 @@ int mg_vcycle(Ctx* c, double* rhs_inout, int nu1, int nu2);
 
 // Reductions / helpers (afm_ns.hip)
 int launch_sum(Ctx* c, const double* x, size_t N, double* out_host);
+
+// Adjoint building blocks (afm_adj.hip)
+int adj_axpy(Ctx* c, double* y, const double* x, double a, size_t N);
+int adj_scale_copy(Ctx* c, double* y, const double* x, double a, size_t N);
+int adj_advect_T(Ctx* c, const double* ab, const double* bb, const double* u, const double* v, double* ub, double* vb);
+int adj_interp_dX(Ctx* c, int comp, const double* f, const double* X, const double* coef, long m, double* Xb);
+int adj_split_scale(Ctx* c, const double* Xb, double scale, long m, double* Ux, double* Uy);
+int adj_spring(Ctx* c, const double* P, const double* R, const double* k, const double* ds, const double* Jx,
+               const double* Jy, long m, double* kb, double* Rb, double* Pb, double* cx, double* cy);
+int adj_centroids(Ctx* c, const double* X, long nm, long msize, const double* c0, double* cout, double* Xb,
+                  double* part);
+int adj_kinetic(Ctx* c, const double* u, const double* v, double* ub, double* vb, double* out_host);
 
 }  // namespace afm
 

@@ -495,6 +495,199 @@ int ib_step(IB* b)
 	return 0;
 }
 
+// ------------------------------------------------------------- adjoint --
+// Reverse-mode gradient of J (kinetic energy + centroid displacement, see
+// afm_adj.hip) through `steps` IB steps, with checkpoints of (u, v, X) every
+// `every` steps and recomputation of each segment's tape in the reverse sweep.
+struct StepTape { double *u = nullptr, *v = nullptr, *uh = nullptr, *vh = nullptr, *X = nullptr, *Xh = nullptr; };
+
+static int d2d(Ctx* c, double* dst, const double* src, size_t count)
+{
+	AFM_HIP(hipMemcpyAsync(dst, src, count * 8, hipMemcpyDeviceToDevice, c->stream));
+	return 0;
+}
+
+static int ib_step_taped(IB* b, StepTape* tp)
+{
+	NS* s = b->ns;
+	Ctx* c = s->c;
+	const double dt = s->dt;
+	if (tp) {
+		AFM_TRY(d2d(c, tp->u, s->u, c->N)); AFM_TRY(d2d(c, tp->v, s->v, c->N)); AFM_TRY(d2d(c, tp->X, b->X, 2 * b->m));
+	}
+	AFM_TRY(launch_spread(c, 3, nullptr, nullptr, b->ds, b->X, b->R, b->k, b->m, b->fx, b->fy));
+	AFM_TRY(ns_step(s, b->fx, b->fy));
+	if (tp) { AFM_TRY(d2d(c, tp->uh, s->u, c->N)); AFM_TRY(d2d(c, tp->vh, s->v, c->N)); }
+	AFM_TRY(launch_ib_move_half(c, s->u, s->v, b->X, b->m, 0.5 * dt, b->Xh));
+	if (tp) AFM_TRY(d2d(c, tp->Xh, b->Xh, 2 * b->m));
+	AFM_TRY(launch_spread(c, 3, nullptr, nullptr, b->ds, b->Xh, b->R, b->k, b->m, b->fx, b->fy));
+	AFM_TRY(ns_step(s, b->fx, b->fy));
+	AFM_TRY(launch_ib_move_full(c, s->uo, s->vo, s->u, s->v, b->Xh, b->m, dt, b->X));
+	return 0;
+}
+
+int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, double* gR)
+{
+	NS* s = b->ns;
+	Ctx* c = s->c;
+	const size_t N = c->N;
+	const long m = b->m;
+	if (msize <= 0 || m % msize != 0) { set_error("msize must divide the marker count"); return -1; }
+	if (steps < 0 || every < 1) { set_error("steps >= 0 and checkpoint_every >= 1 required"); return -1; }
+	if (s->half) { set_error("the NS solver must be at a full step"); return -1; }
+	const long nm = m / msize;
+	const bool want_grad = gk || gR;
+	std::vector<void*> own;
+	bool oom = false;
+	auto A = [&](size_t count) -> double* {
+		void* p = nullptr;
+		if (hipMalloc(&p, (count ? count : 1) * 8) != hipSuccess) { oom = true; return nullptr; }
+		own.push_back(p);
+		return (double*) p;
+	};
+	int rc = 0;
+	auto fail = [&](int r) { (void) hipStreamSynchronize(c->stream); for (void* p : own) (void) hipFree(p); return r; };
+	const int nseg = steps > 0 ? (steps + every - 1) / every : 0;
+	double* c0 = A(2 * nm); double* c1 = A(2 * nm); double* part = A(nm > 0 ? nm : 1);
+	double* Xb = A(2 * m);
+	double* ub = A(N); double* vb = A(N);
+	if (!c0 || !c1 || !part || !Xb || !ub || !vb) { set_error("out of device memory (adjoint)"); return fail(-3); }
+	if ((rc = adj_centroids(c, b->X, nm, msize, nullptr, c0, nullptr, nullptr))) return fail(rc);
+	std::vector<double*> ck(want_grad ? 3 * nseg : 0);
+	for (size_t i = 0; i < ck.size(); ++i) {
+		ck[i] = A(i % 3 == 2 ? 2 * m : N);
+		if (!ck[i]) { set_error("out of device memory (checkpoints)"); return fail(-3); }
+	}
+	for (int st = 0; st < steps; ++st) {
+		if (want_grad && st % every == 0) {
+			int sg = st / every;
+			if ((rc = d2d(c, ck[3 * sg], s->u, N)) || (rc = d2d(c, ck[3 * sg + 1], s->v, N)) ||
+			    (rc = d2d(c, ck[3 * sg + 2], b->X, 2 * m)))
+				return fail(rc);
+		}
+		if ((rc = ib_step_taped(b, nullptr))) return fail(rc);
+	}
+	// objective and its adjoint at the final time
+	double ke = 0.0;
+	if ((rc = adj_kinetic(c, s->u, s->v, ub, vb, &ke))) return fail(rc);
+	if ((rc = adj_centroids(c, b->X, nm, msize, c0, c1, Xb, part))) return fail(rc);
+	std::vector<double> hp(nm > 0 ? nm : 1);
+	if (nm > 0 && hipMemcpy(hp.data(), part, nm * 8, hipMemcpyDeviceToHost) != hipSuccess) { set_error("copy"); return fail(-3); }
+	double J = ke;
+	for (long i = 0; i < nm; ++i) J += hp[i];
+	if (J_out) *J_out = J;
+	if (!want_grad || steps == 0) {
+		if (gk) AFM_HIP(hipMemsetAsync(gk, 0, m * 8, c->stream));
+		if (gR) AFM_HIP(hipMemsetAsync(gR, 0, 2 * m * 8, c->stream));
+		return fail(0);
+	}
+	// final state, kept to restore after the sweep
+	double* fu = A(N); double* fv = A(N); double* fX = A(2 * m);
+	std::vector<StepTape> tape(every);
+	for (auto& t : tape) { t.u = A(N); t.v = A(N); t.uh = A(N); t.vh = A(N); t.X = A(2 * m); t.Xh = A(2 * m); }
+	double* unb = A(N); double* vnb = A(N); double* uhb = A(N); double* vhb = A(N);
+	double* us = A(N); double* vs = A(N); double* rbu = A(N); double* rbv = A(N);
+	double* t1 = A(N); double* t2 = A(N);
+	double* Xnb = A(2 * m); double* Xhb = A(2 * m);
+	double* Ux = A(m); double* Uy = A(m); double* Jx = A(m); double* Jy = A(m);
+	double* cx = A(m); double* cy = A(m); double* volv = A(m);
+	double* kb = gk ? gk : A(m); double* Rb = gR ? gR : A(2 * m);
+	if (!fu || !fv || !fX || !unb || !vnb || !uhb || !vhb || !us || !vs || !rbu || !rbv || !t1 || !t2 || !Xnb ||
+	    !Xhb || !Ux || !Uy || !Jx || !Jy || !cx || !cy || !volv || !kb || !Rb || oom) {
+		set_error("out of device memory (adjoint tape)");
+		return fail(-3);
+	}
+	if ((rc = d2d(c, fu, s->u, N)) || (rc = d2d(c, fv, s->v, N)) || (rc = d2d(c, fX, b->X, 2 * m))) return fail(rc);
+	const double vol = 1. / (c->n * c->n);
+	{
+		std::vector<double> hv(m, vol);
+		AFM_HIP(hipMemcpy(volv, hv.data(), m * 8, hipMemcpyHostToDevice));
+	}
+	AFM_HIP(hipMemsetAsync(kb, 0, m * 8, c->stream));
+	AFM_HIP(hipMemsetAsync(Rb, 0, 2 * m * 8, c->stream));
+	const double dt = s->dt, rho = s->rho, half = 0.5 * dt;
+	const double sm = s->mu * s->dt / (2 * s->rho), ss = -s->mu * s->dt / (2 * s->rho);
+	auto solve_H = [&](double* xu, double* xv, const double* bu, const double* bv) -> int {
+		AFM_TRY(d2d(c, xu, bu, N));
+		AFM_TRY(d2d(c, xv, bv, N));
+		double* xs[2] = {xu, xv};
+		const double* bs[2] = {bu, bv};
+		return op_cg2(c, s->cg, ss, xs, bs, s->tol, s->tol_mode, s->max_cg, nullptr, nullptr);
+	};
+	for (int sg = nseg - 1; sg >= 0; --sg) {
+		const int len = std::min(every, steps - sg * every);
+		if ((rc = d2d(c, s->u, ck[3 * sg], N)) || (rc = d2d(c, s->v, ck[3 * sg + 1], N)) ||
+		    (rc = d2d(c, b->X, ck[3 * sg + 2], 2 * m)))
+			return fail(rc);
+		s->half = false;
+		for (int j = 0; j < len; ++j)
+			if ((rc = ib_step_taped(b, &tape[j]))) return fail(rc);
+		for (int j = len - 1; j >= 0; --j) {
+			const StepTape& tp = tape[j];
+			const double* unx = j == len - 1 ? (sg == nseg - 1 ? fu : ck[3 * (sg + 1)]) : tape[j + 1].u;
+			const double* vnx = j == len - 1 ? (sg == nseg - 1 ? fv : ck[3 * (sg + 1) + 1]) : tape[j + 1].v;
+			// step 7: X' = X + dt 0.5 (J(Xh) u^n + J(Xh) u^{n+1})
+			AFM_TRY(d2d(c, Xnb, Xb, 2 * m));
+			AFM_TRY(adj_split_scale(c, Xb, 0.5 * dt, m, Ux, Uy));
+			AFM_HIP(hipMemsetAsync(unb, 0, N * 8, c->stream));
+			AFM_HIP(hipMemsetAsync(vnb, 0, N * 8, c->stream));
+			AFM_TRY(launch_spread(c, 3, Ux, Uy, volv, tp.Xh, nullptr, nullptr, m, ub, vb, 1));
+			AFM_TRY(launch_spread(c, 3, Ux, Uy, volv, tp.Xh, nullptr, nullptr, m, unb, vnb, 1));
+			AFM_HIP(hipMemsetAsync(Xhb, 0, 2 * m * 8, c->stream));
+			AFM_TRY(adj_interp_dX(c, 0, tp.u, tp.Xh, Ux, m, Xhb));
+			AFM_TRY(adj_interp_dX(c, 0, unx, tp.Xh, Ux, m, Xhb));
+			AFM_TRY(adj_interp_dX(c, 1, tp.v, tp.Xh, Uy, m, Xhb));
+			AFM_TRY(adj_interp_dX(c, 1, vnx, tp.Xh, Uy, m, Xhb));
+			// step 6: Crank-Nicolson full step
+			AFM_TRY(d2d(c, us, ub, N)); AFM_TRY(d2d(c, vs, vb, N));
+			AFM_TRY(ns_project(s, us, vs));
+			AFM_TRY(solve_H(rbu, rbv, us, vs));
+			AFM_TRY(op_helm_apply(c, sm, rbu, t1)); AFM_TRY(adj_axpy(c, unb, t1, 1.0, N));
+			AFM_TRY(op_helm_apply(c, sm, rbv, t2)); AFM_TRY(adj_axpy(c, vnb, t2, 1.0, N));
+			AFM_TRY(adj_scale_copy(c, us, rbu, dt * (1 / rho), N));     // f1 adjoint
+			AFM_TRY(adj_scale_copy(c, vs, rbv, dt * (1 / rho), N));
+			AFM_TRY(adj_scale_copy(c, t1, rbu, -dt, N));                // advection adjoint
+			AFM_TRY(adj_scale_copy(c, t2, rbv, -dt, N));
+			AFM_HIP(hipMemsetAsync(uhb, 0, N * 8, c->stream));
+			AFM_HIP(hipMemsetAsync(vhb, 0, N * 8, c->stream));
+			AFM_TRY(adj_advect_T(c, t1, t2, tp.uh, tp.vh, uhb, vhb));
+			// step 5: f1 = S(Xh) F1, F1 = -k (Xh - R)
+			AFM_TRY(launch_interp(c, 0, us, tp.Xh, m, Jx));
+			AFM_TRY(launch_interp(c, 1, vs, tp.Xh, m, Jy));
+			AFM_TRY(adj_spring(c, tp.Xh, b->R, b->k, b->ds, Jx, Jy, m, kb, Rb, Xhb, cx, cy));
+			AFM_TRY(adj_interp_dX(c, 0, us, tp.Xh, cx, m, Xhb));
+			AFM_TRY(adj_interp_dX(c, 1, vs, tp.Xh, cy, m, Xhb));
+			// step 4: Xh = X + half J(X) u_h
+			AFM_TRY(adj_axpy(c, Xnb, Xhb, 1.0, 2 * m));
+			AFM_TRY(adj_split_scale(c, Xhb, half, m, Ux, Uy));
+			AFM_TRY(launch_spread(c, 3, Ux, Uy, volv, tp.X, nullptr, nullptr, m, uhb, vhb, 1));
+			AFM_TRY(adj_interp_dX(c, 0, tp.uh, tp.X, Ux, m, Xnb));
+			AFM_TRY(adj_interp_dX(c, 1, tp.vh, tp.X, Uy, m, Xnb));
+			// step 3: backward-Euler half step
+			AFM_TRY(ns_project(s, uhb, vhb));
+			AFM_TRY(solve_H(rbu, rbv, uhb, vhb));
+			AFM_TRY(adj_axpy(c, unb, rbu, 1.0, N));
+			AFM_TRY(adj_axpy(c, vnb, rbv, 1.0, N));
+			AFM_TRY(adj_scale_copy(c, us, rbu, (dt / 2) * (1 / rho), N));
+			AFM_TRY(adj_scale_copy(c, vs, rbv, (dt / 2) * (1 / rho), N));
+			AFM_TRY(adj_scale_copy(c, t1, rbu, -(dt / 2), N));
+			AFM_TRY(adj_scale_copy(c, t2, rbv, -(dt / 2), N));
+			AFM_TRY(adj_advect_T(c, t1, t2, tp.u, tp.v, unb, vnb));
+			// steps 1-2: f0 = S(X) F0, F0 = -k (X - R)
+			AFM_TRY(launch_interp(c, 0, us, tp.X, m, Jx));
+			AFM_TRY(launch_interp(c, 1, vs, tp.X, m, Jy));
+			AFM_TRY(adj_spring(c, tp.X, b->R, b->k, b->ds, Jx, Jy, m, kb, Rb, Xnb, cx, cy));
+			AFM_TRY(adj_interp_dX(c, 0, us, tp.X, cx, m, Xnb));
+			AFM_TRY(adj_interp_dX(c, 1, vs, tp.X, cy, m, Xnb));
+			std::swap(ub, unb); std::swap(vb, vnb); std::swap(Xb, Xnb);
+		}
+	}
+	// leave the solver at the final state, as a forward run would
+	if ((rc = d2d(c, s->u, fu, N)) || (rc = d2d(c, s->v, fv, N)) || (rc = d2d(c, b->X, fX, 2 * m))) return fail(rc);
+	s->half = false;
+	return fail(0);
+}
+
 int ib_markers(IB* b, double* X_out)
 {
 	Ctx* c = b->ns->c;

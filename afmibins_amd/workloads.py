@@ -116,6 +116,34 @@ CONFIGS = {
 }
 
 
+def clustered_structures(n_membranes=512, n_filaments=512, npts=256, blobs=8, sigma=0.03, seed=5):
+    """The combined stress case: closed membranes and open filaments whose
+    centres are drawn from `blobs` Gaussian clusters (sigma), clipped to
+    [0.15, 0.85]^2 so every marker stays > 3h from the periodic seams.
+    Filaments are straight segments of length U[0.05, 0.1] at a random angle,
+    tethered to their initial positions; ds = length / npts."""
+    rng = SplitMix64(seed)
+    cb = [(rng.uniform(0.3, 0.7), rng.uniform(0.3, 0.7)) for _ in range(blobs)]
+
+    def centre():
+        b = cb[int(rng.next() % blobs)]
+        return (min(0.85, max(0.15, b[0] + sigma * rng.normal())), min(0.85, max(0.15, b[1] + sigma * rng.normal())))
+
+    Xs, Rs, ks, dss = [], [], [], []
+    for _ in range(n_membranes):
+        cx, cy = centre()
+        a, b = rng.uniform(0.02, 0.05), rng.uniform(0.02, 0.05)
+        X, R, k, ds = ellipse(cx, cy, a, b, npts)
+        Xs.append(X); Rs.append(R); ks.append(k); dss.append(ds)
+    for _ in range(n_filaments):
+        cx, cy = centre()
+        L, th = rng.uniform(0.05, 0.1), rng.uniform(0.0, np.pi)
+        s = (np.arange(npts) / (npts - 1) - 0.5) * L
+        X = np.stack([cx + s * np.cos(th), cy + s * np.sin(th)], axis=1)
+        Xs.append(X); Rs.append(X.copy()); ks.append(np.ones(npts)); dss.append(np.full(npts, L / npts))
+    return np.concatenate(Xs), np.concatenate(Rs), np.concatenate(ks), np.concatenate(dss)
+
+
 def config2_inputs(n=2048, count=64, npts=256, seed=2, kmax=8):
     X, R, k, ds = membranes(count, npts, seed)
     u, v = fourier_velocity(n, kmax, seed + 1)

@@ -51,6 +51,13 @@ int afm_sync(afm_ctx* c);
 int afm_set_stream(afm_ctx* c, void* hip_stream);
 const char* afm_last_error(void);
 
+/* Device memory on the context's GPU for callers that hold no HIP headers
+ * (the C++ drop-in classes of include/afm/).  Copies are synchronous. */
+int afm_alloc(afm_ctx* c, unsigned long bytes, void** out);
+int afm_free(afm_ctx* c, void* p);
+int afm_copy_to_device(afm_ctx* c, void* dst, const void* src_host, unsigned long bytes);
+int afm_copy_to_host(afm_ctx* c, void* dst_host, const void* src, unsigned long bytes);
+
 /* ib::grid::delta_function (ib/grid.h:51-55) for n = the context's n:
  * out[i] = delta(r[i]).  afm_delta_host evaluates the same code on the CPU. */
 int afm_delta(afm_ctx* c, const double* r, long count, double* out);
@@ -131,6 +138,18 @@ int afm_ib_create(afm_ns* s, long m, const double* X, const double* R, const dou
                   afm_ib** out);
 int afm_ib_step(afm_ib* b, int steps);
 int afm_ib_markers(afm_ib* b, double* X_out);
+
+/* Differentiation (no reference counterpart).  Runs `steps` IB steps from the
+ * current state (which must be at a full step) and evaluates
+ *   J = 1/2 sum (u^2 + v^2) h^2 + sum_m |c_m(T) - c_m(0)|^2,
+ * c_m the centroid of membrane m = markers [m*msize, (m+1)*msize).  With
+ * grad_k / grad_R non-NULL (device, m and 2m doubles) it also returns dJ/dk
+ * and dJ/dR by the discrete adjoint of the step (adjoint transfer kernels,
+ * the same symmetric CG and multigrid solves), checkpointing (u, v, X) every
+ * `checkpoint_every` steps and recomputing each segment in the reverse
+ * sweep.  *J_host receives J.  The solver is left at the final state. */
+int afm_ib_grad(afm_ib* b, int steps, int checkpoint_every, long msize, double* J_host, double* grad_k,
+                double* grad_R);
 int afm_ib_destroy(afm_ib* b);
 
 #ifdef __cplusplus

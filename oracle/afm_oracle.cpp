@@ -934,3 +934,299 @@ unsigned or_ns_run(unsigned nx, unsigned ny, unsigned n, double mu, double rho, 
 }
 
 }  // extern "C"
+
+// ===================================================================
+// Discrete adjoint of the IB time step (build-only; no reference
+// counterpart).  Objective at the final time T:
+//   J = 1/2 sum (u^2 + v^2) h^2 + sum_m |c_m(T) - c_m(0)|^2,
+// c_m the centroid of membrane m (consecutive blocks of `msize` markers).
+// Solves are differentiated as exact linear solves (implicit
+// differentiation): the Helmholtz operators and the projection are
+// symmetric on the periodic grid, so their adjoints are the same solves.
+namespace {
+
+// d delta / dr for the window's arguments (|n r| <= 2), with the forward
+// argument theta: -(0.25 n)(pi n / 2) sin(theta).
+double ddelta(double r, unsigned n)
+{
+	if (std::abs((int) (n * r)) > 2.) return 0;
+	double a = delta_arg(r, n);
+	return -(0.25 * n) * (M_PI * n / 2) * ::sin(a);
+}
+
+// window weights and their derivatives with respect to the marker position
+void window2_d(int comp, double cx, double cy, const G2& g, int mode, size_t idx[16], double w[16],
+               double wx[16], double wy[16])
+{
+	double q0 = cx, q1 = cy;
+	const unsigned n = g.n;
+	if (comp == 0) q1 -= 0.5 / n; else q0 -= 0.5 / n;
+	q0 = std::fmod(std::fmod(q0, (double) g.W) + g.W, (double) g.W);
+	q1 = std::fmod(std::fmod(q1, (double) g.D) + g.D, (double) g.D);
+	const double fx = std::floor(q0 * n), fy = std::floor(q1 * n);
+	double dx[4], dy[4], ddx[4], ddy[4];
+	for (int d = 0; d < 4; ++d) {
+		double x = (fx + d - 1.0) / n, y = (fy + d - 1.0) / n;
+		if (comp == 0) y += 0.5 / n; else x += 0.5 / n;
+		dx[d] = delta(x - cx, n, mode); dy[d] = delta(y - cy, n, mode);
+		ddx[d] = ddelta(x - cx, n); ddy[d] = ddelta(y - cy, n);
+	}
+	for (int i = 0; i < 16; ++i) {
+		const int a = i % 4, b = i / 4;
+		idx[i] = ((size_t) fx + a + g.nx - 1) % g.nx + (size_t) g.nx * (((size_t) fy + b + g.ny - 1) % g.ny);
+		w[i] = dx[a] * dy[b];
+		wx[i] = -ddx[a] * dy[b];
+		wy[i] = -dx[a] * ddy[b];
+	}
+}
+
+// out += scale * J(X)^T Ub   (J = interpolation with volume h^2)
+void interp_T(int comp, const vec& Ub, const vec& X, double scale, vec& out, const G2& g, int mode)
+{
+	const double vol = 1. / (g.n * g.n);
+	size_t idx[16]; double w[16], wx[16], wy[16];
+	for (size_t k = 0; k < Ub.size(); ++k) {
+		window2_d(comp, X[2 * k], X[2 * k + 1], g, mode, idx, w, wx, wy);
+		for (int t = 0; t < 16; ++t) out[idx[t]] += scale * (w[t] * vol * Ub[k]);
+	}
+}
+
+// Xb[k] += coef_k * d(J f)_k / dX_k  (both position components)
+void interp_dX(int comp, const vec& f, const vec& X, const vec& coef, vec& Xb, const G2& g, int mode)
+{
+	const double vol = 1. / (g.n * g.n);
+	size_t idx[16]; double w[16], wx[16], wy[16];
+	for (size_t k = 0; k < coef.size(); ++k) {
+		window2_d(comp, X[2 * k], X[2 * k + 1], g, mode, idx, w, wx, wy);
+		double gx = 0.0, gy = 0.0;
+		for (int t = 0; t < 16; ++t) { gx += wx[t] * vol * f[idx[t]]; gy += wy[t] * vol * f[idx[t]]; }
+		Xb[2 * k] += coef[k] * gx;
+		Xb[2 * k + 1] += coef[k] * gy;
+	}
+}
+
+// (ub, vb) += A'(u, v)^T (ab, bb), A the advection of advect2
+void advect2_T(const vec& ab, const vec& bb, const vec& u, const vec& w, vec& ub, vec& wb, const G2& g)
+{
+	const size_t N = u.size();
+	vec uub(N, 0.0), uwb(N, 0.0), wwb(N, 0.0);
+	for (size_t i = 0; i < N; ++i) {
+		unsigned x = i % g.nx, z = i / g.nx;
+		size_t ixp = (x + 1) % g.nx + (size_t) g.nx * z, izp = x + (size_t) g.nx * ((z + 1) % g.ny);
+		// ua[i] = n ((uu[ixp] - uu[i]) + (uw[izp] - uw[i])); wa[i] = n ((uw[ixp] - uw[i]) + (ww[izp] - ww[i]))
+		uub[ixp] += g.n * ab[i]; uub[i] -= g.n * ab[i];
+		uwb[izp] += g.n * ab[i]; uwb[i] -= g.n * ab[i];
+		uwb[ixp] += g.n * bb[i]; uwb[i] -= g.n * bb[i];
+		wwb[izp] += g.n * bb[i]; wwb[i] -= g.n * bb[i];
+	}
+	for (size_t i = 0; i < N; ++i) {
+		unsigned x = i % g.nx, z = i / g.nx;
+		size_t ixm = (x + g.nx - 1) % g.nx + (size_t) g.nx * z, izm = x + (size_t) g.nx * ((z + g.ny - 1) % g.ny);
+		double q = 0.5 * (u[i] + u[ixm]);          // uu = q^2
+		double qb = 2 * q * uub[i];
+		ub[i] += 0.5 * qb; ub[ixm] += 0.5 * qb;
+		q = 0.5 * (w[i] + w[izm]);                 // ww = q^2
+		qb = 2 * q * wwb[i];
+		wb[i] += 0.5 * qb; wb[izm] += 0.5 * qb;
+		q = 0.5 * (u[izm] + u[i]);                 // uw = q r
+		double r = 0.5 * (w[ixm] + w[i]);
+		ub[izm] += 0.5 * r * uwb[i]; ub[i] += 0.5 * r * uwb[i];
+		wb[ixm] += 0.5 * q * uwb[i]; wb[i] += 0.5 * q * uwb[i];
+	}
+}
+
+struct Tape { vec u, v, X, uh, vh, Xh; };
+
+double objective(const IB2& s, const vec& X0c)
+{
+	const double h2 = 1.0 / ((double) s.g.n * s.g.n);
+	double J = 0.0;
+	for (size_t i = 0; i < s.u.size(); ++i) J += 0.5 * (s.u[i] * s.u[i] + s.v[i] * s.v[i]) * h2;
+	size_t nm = X0c.size() / 2;
+	long msize = s.m / (long) nm;
+	for (size_t mm = 0; mm < nm; ++mm) {
+		double cx = 0, cy = 0;
+		for (long j = 0; j < msize; ++j) { cx += s.X[2 * (mm * msize + j)]; cy += s.X[2 * (mm * msize + j) + 1]; }
+		cx /= msize; cy /= msize;
+		J += (cx - X0c[2 * mm]) * (cx - X0c[2 * mm]) + (cy - X0c[2 * mm + 1]) * (cy - X0c[2 * mm + 1]);
+	}
+	return J;
+}
+
+vec centroids(const vec& X, long m, long msize)
+{
+	long nm = m / msize;
+	vec c(2 * nm, 0.0);
+	for (long mm = 0; mm < nm; ++mm) {
+		for (long j = 0; j < msize; ++j) { c[2 * mm] += X[2 * (mm * msize + j)]; c[2 * mm + 1] += X[2 * (mm * msize + j) + 1]; }
+		c[2 * mm] /= msize; c[2 * mm + 1] /= msize;
+	}
+	return c;
+}
+
+// P applied to an adjoint velocity (the projection is symmetric)
+void project_adj(NS2& ns, vec& ub, vec& vb) { ns.project(ub, vb); }
+
+// H_s^{-1} b (H_s symmetric), initial guess b
+vec helm_inv(NS2& ns, const vec& b)
+{
+	vec x(b);
+	double t = ns.tol_mode == 1 ? ns.tol * sqrt(dot(b, b)) : ns.tol;
+	cg(x, ns.phhs, b, t, 100000, nullptr);
+	return x;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Runs `steps` IB steps and returns J; with grad_k / grad_R non-null also
+// the adjoint gradients dJ/dk (m) and dJ/dR (2m).
+double or_ib_grad(unsigned nx, unsigned ny, unsigned n, double mu, double rho, double dt, double tol, int tol_mode,
+                  int smoother, int delta_mode, const double* u0, const double* v0, long m, const double* X0,
+                  const double* R, const double* k, const double* ds, long msize, int steps,
+                  double* grad_k, double* grad_R)
+{
+	IB2 s;
+	s.g = G2{n, nx, ny, nx / n, ny / n};
+	const G2& g = s.g;
+	size_t N = (size_t) nx * ny;
+	s.u.assign(u0, u0 + N); s.v.assign(v0, v0 + N);
+	s.m = m;
+	s.X.assign(X0, X0 + 2 * m); s.R.assign(R, R + 2 * m);
+	s.k.assign(k, k + m); s.ds.assign(ds, ds + m);
+	s.mode = delta_mode;
+	NS2 ns(g, mu, rho, dt, tol, tol_mode, smoother, &s.u, &s.v);
+	s.ns = &ns;
+	vec c0 = centroids(s.X, m, msize);
+	std::vector<Tape> tape(steps);
+	const double half = 0.5 * dt;
+	for (int st = 0; st < steps; ++st) {
+		Tape& tp = tape[st];
+		tp.u = s.u; tp.v = s.v; tp.X = s.X;
+		vec Fx(m), Fy(m), fx(N), fy(N), Ux(m), Uy(m), Xh(2 * m);
+		spring(s, s.X, Fx, Fy);
+		spread_both(s, s.X, Fx, Fy, fx, fy);
+		ns.step(fx, fy);
+		tp.uh = s.u; tp.vh = s.v;
+		interp_both(s, s.u, s.v, s.X, Ux, Uy);
+		for (long i = 0; i < m; ++i) { Xh[2 * i] = s.X[2 * i] + half * Ux[i]; Xh[2 * i + 1] = s.X[2 * i + 1] + half * Uy[i]; }
+		tp.Xh = Xh;
+		spring(s, Xh, Fx, Fy);
+		spread_both(s, Xh, Fx, Fy, fx, fy);
+		ns.step(fx, fy);
+		vec U0x(m), U0y(m);
+		interp_both(s, ns.u_old, ns.w_old, Xh, U0x, U0y);
+		interp_both(s, s.u, s.v, Xh, Ux, Uy);
+		for (long i = 0; i < m; ++i) {
+			s.X[2 * i] = s.X[2 * i] + dt * (0.5 * (U0x[i] + Ux[i]));
+			s.X[2 * i + 1] = s.X[2 * i + 1] + dt * (0.5 * (U0y[i] + Uy[i]));
+		}
+	}
+	double J = objective(s, c0);
+	if (!grad_k && !grad_R) return J;
+
+	// ---- reverse sweep
+	const double h2 = 1.0 / ((double) n * n), vol = h2, sm = mu * dt / (2 * rho);
+	Csr Hm = stencil_csr(nx, 1, ny, n, 2, sm);
+	vec ub(N), vb(N), Xb(2 * m, 0.0), kb(m, 0.0), Rb(2 * m, 0.0);
+	for (size_t i = 0; i < N; ++i) { ub[i] = s.u[i] * h2; vb[i] = s.v[i] * h2; }
+	vec c1 = centroids(s.X, m, msize);
+	for (long i = 0; i < m; ++i) {
+		long mm = i / msize;
+		Xb[2 * i] = 2 * (c1[2 * mm] - c0[2 * mm]) / msize;
+		Xb[2 * i + 1] = 2 * (c1[2 * mm + 1] - c0[2 * mm + 1]) / msize;
+	}
+	const vec* unext_u = &s.u;   // u^{n+1}
+	const vec* unext_v = &s.v;
+	for (int st = steps - 1; st >= 0; --st) {
+		const Tape& tp = tape[st];
+		const vec& un = tp.u; const vec& vn = tp.v;
+		// step 7: X' = X + dt * 0.5 (J(Xh) u^n + J(Xh) u^{n+1})
+		vec Xn_b = Xb;
+		vec Ubx(m), Uby(m);
+		for (long i = 0; i < m; ++i) { Ubx[i] = 0.5 * dt * Xb[2 * i]; Uby[i] = 0.5 * dt * Xb[2 * i + 1]; }
+		vec un_b(N, 0.0), vn_b(N, 0.0);
+		interp_T(0, Ubx, tp.Xh, 1.0, ub, g, delta_mode);
+		interp_T(1, Uby, tp.Xh, 1.0, vb, g, delta_mode);
+		interp_T(0, Ubx, tp.Xh, 1.0, un_b, g, delta_mode);
+		interp_T(1, Uby, tp.Xh, 1.0, vn_b, g, delta_mode);
+		vec Xhb(2 * m, 0.0);
+		interp_dX(0, un, tp.Xh, Ubx, Xhb, g, delta_mode);
+		interp_dX(0, *unext_u, tp.Xh, Ubx, Xhb, g, delta_mode);
+		interp_dX(1, vn, tp.Xh, Uby, Xhb, g, delta_mode);
+		interp_dX(1, *unext_v, tp.Xh, Uby, Xhb, g, delta_mode);
+		// step 6: CN full step
+		vec us = ub, vs = vb;
+		project_adj(ns, us, vs);
+		vec rbu = helm_inv(ns, us), rbv = helm_inv(ns, vs);
+		vec t1 = spmv(Hm, rbu), t2 = spmv(Hm, rbv);
+		for (size_t i = 0; i < N; ++i) { un_b[i] += t1[i]; vn_b[i] += t2[i]; }
+		vec f1bu(N), f1bv(N), aub(N), avb(N);
+		for (size_t i = 0; i < N; ++i) {
+			f1bu[i] = dt * (1 / rho) * rbu[i]; f1bv[i] = dt * (1 / rho) * rbv[i];
+			aub[i] = -dt * rbu[i]; avb[i] = -dt * rbv[i];
+		}
+		vec uh_b(N, 0.0), vh_b(N, 0.0);
+		advect2_T(aub, avb, tp.uh, tp.vh, uh_b, vh_b, g);
+		// step 5: f1 = S(Xh) F1, F1 = -k (Xh - R)
+		{
+			vec Fx(m), Fy(m), Jx(m), Jy(m);
+			spring(s, tp.Xh, Fx, Fy);
+			interp_both(s, f1bu, f1bv, tp.Xh, Jx, Jy);
+			vec cx(m), cy(m);
+			for (long i = 0; i < m; ++i) {
+				double Fbx = s.ds[i] / vol * Jx[i], Fby = s.ds[i] / vol * Jy[i];
+				kb[i] += -(tp.Xh[2 * i] - s.R[2 * i]) * Fbx - (tp.Xh[2 * i + 1] - s.R[2 * i + 1]) * Fby;
+				Rb[2 * i] += s.k[i] * Fbx; Rb[2 * i + 1] += s.k[i] * Fby;
+				Xhb[2 * i] += -s.k[i] * Fbx; Xhb[2 * i + 1] += -s.k[i] * Fby;
+				cx[i] = s.ds[i] * Fx[i] / vol; cy[i] = s.ds[i] * Fy[i] / vol;
+			}
+			interp_dX(0, f1bu, tp.Xh, cx, Xhb, g, delta_mode);
+			interp_dX(1, f1bv, tp.Xh, cy, Xhb, g, delta_mode);
+		}
+		// step 4: Xh = X + half J(X) u_h
+		for (long i = 0; i < 2 * m; ++i) Xn_b[i] += Xhb[i];
+		{
+			vec Uhx(m), Uhy(m);
+			for (long i = 0; i < m; ++i) { Uhx[i] = half * Xhb[2 * i]; Uhy[i] = half * Xhb[2 * i + 1]; }
+			interp_T(0, Uhx, tp.X, 1.0, uh_b, g, delta_mode);
+			interp_T(1, Uhy, tp.X, 1.0, vh_b, g, delta_mode);
+			interp_dX(0, tp.uh, tp.X, Uhx, Xn_b, g, delta_mode);
+			interp_dX(1, tp.vh, tp.X, Uhy, Xn_b, g, delta_mode);
+		}
+		// step 3: BE half step
+		project_adj(ns, uh_b, vh_b);
+		rbu = helm_inv(ns, uh_b); rbv = helm_inv(ns, vh_b);
+		vec f0bu(N), f0bv(N);
+		for (size_t i = 0; i < N; ++i) {
+			un_b[i] += rbu[i]; vn_b[i] += rbv[i];
+			f0bu[i] = (dt / 2) * (1 / rho) * rbu[i]; f0bv[i] = (dt / 2) * (1 / rho) * rbv[i];
+			aub[i] = -(dt / 2) * rbu[i]; avb[i] = -(dt / 2) * rbv[i];
+		}
+		advect2_T(aub, avb, un, vn, un_b, vn_b, g);
+		// steps 1-2: f0 = S(X) F0, F0 = -k (X - R)
+		{
+			vec Fx(m), Fy(m), Jx(m), Jy(m);
+			spring(s, tp.X, Fx, Fy);
+			interp_both(s, f0bu, f0bv, tp.X, Jx, Jy);
+			vec cx(m), cy(m);
+			for (long i = 0; i < m; ++i) {
+				double Fbx = s.ds[i] / vol * Jx[i], Fby = s.ds[i] / vol * Jy[i];
+				kb[i] += -(tp.X[2 * i] - s.R[2 * i]) * Fbx - (tp.X[2 * i + 1] - s.R[2 * i + 1]) * Fby;
+				Rb[2 * i] += s.k[i] * Fbx; Rb[2 * i + 1] += s.k[i] * Fby;
+				Xn_b[2 * i] += -s.k[i] * Fbx; Xn_b[2 * i + 1] += -s.k[i] * Fby;
+				cx[i] = s.ds[i] * Fx[i] / vol; cy[i] = s.ds[i] * Fy[i] / vol;
+			}
+			interp_dX(0, f0bu, tp.X, cx, Xn_b, g, delta_mode);
+			interp_dX(1, f0bv, tp.X, cy, Xn_b, g, delta_mode);
+		}
+		ub = un_b; vb = vn_b; Xb = Xn_b;
+		unext_u = &tp.u; unext_v = &tp.v;
+	}
+	if (grad_k) std::copy(kb.begin(), kb.end(), grad_k);
+	if (grad_R) std::copy(Rb.begin(), Rb.end(), grad_R);
+	return J;
+}
+
+}  // extern "C"

@@ -77,6 +77,8 @@ def lib():
             "or_advect2": (None, [_U, _U, _U, _P, _P, _P, _P]),
             "or_ib_run": (_U, [_U, _U, _U, _D, _D, _D, _D, _I, _I, _I, _P, _P, _L, _P, _P, _P, _P, _I, _P]),
             "or_last_step_seconds": (_D, []),
+            "or_ib_grad": (_D, [_U, _U, _U, _D, _D, _D, _D, _I, _I, _I, _P, _P, _L, _P, _P, _P, _P, _L, _I, _P,
+                                _P]),
             "or_ns_run": (_U, [_U, _U, _U, _D, _D, _D, _D, _I, _I, _P, _P, _P, _P, _I, _P]),
         }
         for name, (res, args) in sig.items():
@@ -296,3 +298,18 @@ def ib_run(u, v, X, R, k, ds, n, mu, rho, dt, tol, steps, tol_mode=1, smoother=1
     cyc = lib().or_ib_run(nx, ny, n, mu, rho, dt, tol, tol_mode, smoother, delta_mode, _ptr(u), _ptr(v),
                           X.shape[0], _ptr(X), _ptr(R), _ptr(k), _ptr(ds), steps, C.byref(cgi))
     return u, v, X, cyc, cgi.value
+
+
+def ib_grad(u, v, X, R, k, ds, n, mu, rho, dt, tol, steps, msize, tol_mode=1, smoother=1, delta_mode=1,
+            grad=True):
+    """J (kinetic energy + centroid displacement) after `steps` IB steps and,
+    with grad=True, the adjoint gradients dJ/dk (m,) and dJ/dR (m, 2)."""
+    u, v, X, R, k, ds = (_f64(a) for a in (u, v, X, R, k, ds))
+    ny, nx = u.shape
+    m = X.shape[0]
+    gk = np.zeros(m) if grad else None
+    gR = np.zeros((m, 2)) if grad else None
+    J = lib().or_ib_grad(nx, ny, n, mu, rho, dt, tol, tol_mode, smoother, delta_mode, _ptr(u), _ptr(v), m, _ptr(X),
+                         _ptr(R), _ptr(k), _ptr(ds), msize, steps, _ptr(gk) if grad else None,
+                         _ptr(gR) if grad else None)
+    return (J, gk, gR) if grad else J

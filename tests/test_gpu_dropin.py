@@ -1,0 +1,29 @@
+"""The C++ drop-in headers: a program in the style of the reference's test/
+programs, compiled against include/afm/afm_dropin.hpp by build(), runs on
+the GPU through libafm."""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "dropin_main")
+
+
+def test_dropin_programs():
+    assert os.path.exists(BIN), "tests/cpp/dropin_main is built by __graft_entry__.build()"
+    out = subprocess.run([BIN], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    v = dict(line.split(None, 1) for line in out.stdout.strip().splitlines())
+    assert v["done"] == "1"
+    # the v-grid window of a marker at (0.5, 0.5) has one row at r = 2h,
+    # where the cosine delta is exactly zero: 12 of 16 taps are non-zero
+    assert int(v["ib_nonzero"]) == 12
+    assert abs(float(v["ib_value"]) - 1.0) < 1e-14
+    assert float(v["spread_momentum_error"]) < 1e-13
+    assert 1 <= int(v["mg_cycles"]) <= 20
+    assert float(v["mg_rel_error"]) < 1e-9
+    assert float(v["ns_zero_max"]) == 0.0
+    assert float(v["ns_uniform_error"]) < 1e-12

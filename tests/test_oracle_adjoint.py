@@ -1,0 +1,57 @@
+"""Pins the oracle's discrete adjoint of the IB step (the CPU FP64 reference
+gradient) by central finite differences, and the synthetic inputs by their
+defining properties."""
+import numpy as np
+
+from afmibins_amd import workloads as W
+
+
+def test_oracle_adjoint_matches_central_differences(oracle):
+    n, steps, msize = 32, 3, 32
+    u, v = W.fourier_velocity(n, 3, seed=4, rms=0.3)
+    X, R, k, ds = W.membranes(3, msize, seed=5, center=(0.3, 0.7), axes=(0.08, 0.15))
+    k = k * (1 + 0.3 * np.random.default_rng(1).standard_normal(k.shape))
+    a = dict(n=n, mu=1e-2, rho=1.0, dt=1e-3, tol=1e-13, steps=steps, msize=msize)
+    J, gk, gR = oracle.ib_grad(u, v, X, R, k, ds, **a)
+    assert J > 0 and np.all(np.isfinite(gk)) and np.all(np.isfinite(gR))
+    rng = np.random.default_rng(7)
+    h = 1e-5
+    for _ in range(3):
+        dk, dR = rng.standard_normal(k.shape), rng.standard_normal(R.shape)
+        Jp = oracle.ib_grad(u, v, X, R + h * dR, k + h * dk, ds, grad=False, **a)
+        Jm = oracle.ib_grad(u, v, X, R - h * dR, k - h * dk, ds, grad=False, **a)
+        fd = (Jp - Jm) / (2 * h)
+        ad = gk @ dk + np.sum(gR * dR)
+        assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+
+
+def test_oracle_adjoint_objective_matches_forward(oracle):
+    # J from the gradient run equals J evaluated from a plain forward run
+    n, steps, msize = 32, 2, 16
+    u, v = W.fourier_velocity(n, 2, seed=8, rms=0.2)
+    X, R, k, ds = W.membranes(2, msize, seed=9, center=(0.3, 0.7), axes=(0.08, 0.15))
+    J = oracle.ib_grad(u, v, X, R, k, ds, n, 1e-2, 1.0, 1e-3, 1e-12, steps, msize, grad=False)
+    u2, v2, X2, _, _ = oracle.ib_run(u, v, X, R, k, ds, n, 1e-2, 1.0, 1e-3, 1e-12, steps)
+    ke = 0.5 * np.sum(u2 ** 2 + v2 ** 2) / n ** 2
+    c0 = X.reshape(-1, msize, 2).mean(axis=1)
+    c1 = X2.reshape(-1, msize, 2).mean(axis=1)
+    assert abs(J - (ke + np.sum((c1 - c0) ** 2))) <= 1e-12 * J
+
+
+def test_fourier_velocity_is_divergence_free(oracle):
+    n = 64
+    u, v = W.fourier_velocity(n, 8, seed=2)
+    d = oracle.div2(u, v, n)
+    assert np.max(np.abs(d)) < 1e-10 * n
+    assert abs(np.sqrt((np.mean(u * u) + np.mean(v * v)) / 2) - 1.0) < 1e-12
+
+
+def test_membrane_generator(oracle):
+    X, R, k, ds = W.membranes(64, 256, seed=2)
+    assert X.shape == (16384, 2) and R.shape == (16384, 2)
+    assert np.all((X > 0.04) & (X < 0.96))
+    c = X.reshape(64, 256, 2).mean(axis=1)
+    assert np.all((c >= 0.1) & (c <= 0.9))
+    r0 = np.linalg.norm(R.reshape(64, 256, 2) - c[:, None, :], axis=2)
+    assert np.allclose(r0.min(axis=1), r0.max(axis=1))
+    assert np.allclose(ds.reshape(64, 256)[:, 0] * 256, 2 * np.pi * r0[:, 0])
