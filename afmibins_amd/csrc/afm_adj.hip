@@ -189,8 +189,14 @@ __global__ __launch_bounds__(256) void k_kinetic(const double* __restrict__ u, c
 		acc[0] += 0.5 * (u[i] * u[i] + v[i] * v[i]) * h2;
 		if (ub) { ub[i] = u[i] * h2; vb[i] = v[i] * h2; }
 	}
+	reduce_partial<1>(acc, partials);
+}
+
+__global__ __launch_bounds__(256) void k_kinetic_fin(unsigned nblk, SolveCtl* ctl, const double* partials)
+{
 	double tot[1];
-	if (reduce_last<1>(acc, partials, &ctl->ticket, tot) && threadIdx.x == 0) ctl->sum2 = tot[0];
+	red_total<1>(partials, nblk, tot);
+	if (threadIdx.x == 0) ctl->sum2 = tot[0];
 }
 
 // ------------------------------------------------------------- launchers --
@@ -261,6 +267,7 @@ int adj_kinetic(Ctx* c, const double* u, const double* v, double* ub, double* vb
 	unsigned nb = (unsigned) (b < (size_t) MAX_RED_BLOCKS ? b : (size_t) MAX_RED_BLOCKS);
 	const double h2 = 1.0 / ((double) c->n * c->n);
 	hipLaunchKernelGGL(k_kinetic, dim3(nb), dim3(256), 0, c->stream, u, v, c->N, h2, ub, vb, &c->ctl[3], c->partials);
+	hipLaunchKernelGGL(k_kinetic_fin, dim3(1), dim3(256), 0, c->stream, nb, &c->ctl[3], c->partials);
 	AFM_LAUNCH_CHECK();
 	AFM_HIP(hipMemcpyAsync(&c->hctl[3], &c->ctl[3], sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
 	AFM_HIP(hipStreamSynchronize(c->stream));

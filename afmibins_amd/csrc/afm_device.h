@@ -36,44 +36,33 @@ __device__ inline void block_sum(double (&v)[NV])
 	__syncthreads();
 }
 
-// Writes this block's NV partial sums, then returns true in the one block
-// that arrives last; that block receives the NV totals (summed over blocks in
-// index order, so the value does not depend on arrival order) in tot[].
-// The reduction group is the blocks sharing blockIdx.y (so one launch can
-// carry independent reductions, one per y slice, each with its own ticket and
-// partials slice).
+// Two-phase deterministic reduction.  Phase 1 (in the producing kernel):
+// each block writes its NV partial sums to partials[j * MAX_RED_BLOCKS +
+// blockIdx.x] -- no fence, no atomics (a device-scope fence per block writes
+// back the XCD's L2 and costs more than the kernel's own traffic).  Phase 2
+// (a one-block kernel launched next on the same stream): red_total sums the
+// partials in block order, so the result does not depend on scheduling.
 template<int NV>
-__device__ inline bool reduce_last(double (&v)[NV], double* partials, unsigned* ticket, double (&tot)[NV])
+__device__ inline void reduce_partial(double (&v)[NV], double* partials)
 {
-	__shared__ int am_last;
-	const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-	const int nthr = blockDim.x * blockDim.y;
-	const unsigned nblk = gridDim.x;
-	const unsigned bid = blockIdx.x;
 	block_sum<NV>(v);
-	if (tid == 0) {
+	if (threadIdx.x + threadIdx.y * blockDim.x == 0) {
 #pragma unroll
-		for (int j = 0; j < NV; ++j) partials[(size_t) j * MAX_RED_BLOCKS + bid] = v[j];
-		__threadfence();
-		unsigned t = atomicAdd(ticket, 1u);
-		am_last = (t == nblk - 1);
+		for (int j = 0; j < NV; ++j) partials[(size_t) j * MAX_RED_BLOCKS + blockIdx.x] = v[j];
 	}
-	__syncthreads();
-	if (!am_last) return false;
-	__threadfence();
-	double s[NV];
+}
+
+// Phase 2: totals in thread 0 (call from every thread of a one-block kernel).
+template<int NV>
+__device__ inline void red_total(const double* partials, unsigned nblk, double (&tot)[NV])
+{
+	const int tid = threadIdx.x;
 #pragma unroll
 	for (int j = 0; j < NV; ++j) {
-		s[j] = 0.0;
-		for (unsigned b = tid; b < nblk; b += nthr) s[j] += __builtin_nontemporal_load(&partials[(size_t) j * MAX_RED_BLOCKS + b]);
+		tot[j] = 0.0;
+		for (unsigned b = tid; b < nblk; b += blockDim.x) tot[j] += partials[(size_t) j * MAX_RED_BLOCKS + b];
 	}
-	block_sum<NV>(s);
-	if (tid == 0) {
-#pragma unroll
-		for (int j = 0; j < NV; ++j) tot[j] = s[j];
-		*ticket = 0u;
-	}
-	return true;
+	block_sum<NV>(tot);
 }
 
 // ------------------------------------------------------------ stencil ---

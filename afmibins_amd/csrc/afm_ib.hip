@@ -106,25 +106,29 @@ __global__ void k_spread_prep(int comp_mask, const double* __restrict__ Fx, cons
 	atomicAdd(&counts[comp * ntx * nty + r.tile], 1);
 }
 
-// Exclusive scan of the bin counts of both components (one block).
-__global__ void k_spread_scan(const int* __restrict__ counts, int* __restrict__ offs, int* __restrict__ cursor, int nbins)
+// Exclusive scan of the bin counts of both components (one block of 1024):
+// per-thread serial chunk sums, a Hillis-Steele scan of the chunk sums, then
+// the per-thread chunk scan.
+__global__ __launch_bounds__(1024) void k_spread_scan(const int* __restrict__ counts, int* __restrict__ offs,
+                                                      int* __restrict__ cursor, int nbins)
 {
 	__shared__ int part[1024];
 	const int tid = threadIdx.x, nt = blockDim.x;
 	const int per = (nbins + nt - 1) / nt;
-	int lo = tid * per, hi = min(nbins, lo + per);
+	const int lo = tid * per, hi = min(nbins, lo + per);
 	int s = 0;
 	for (int i = lo; i < hi; ++i) s += counts[i];
 	part[tid] = s;
 	__syncthreads();
-	if (tid == 0) {
-		int acc = 0;
-		for (int t = 0; t < nt; ++t) { int v = part[t]; part[t] = acc; acc += v; }
+	for (int d = 1; d < nt; d <<= 1) {
+		int v = tid >= d ? part[tid - d] : 0;
+		__syncthreads();
+		part[tid] += v;
+		__syncthreads();
 	}
-	__syncthreads();
-	int acc = part[tid];
+	int acc = part[tid] - s;                  // exclusive prefix of this chunk
 	for (int i = lo; i < hi; ++i) { offs[i] = acc; cursor[i] = 0; acc += counts[i]; }
-	if (tid == nt - 1) offs[nbins] = acc;
+	if (tid == nt - 1) offs[nbins] = part[nt - 1];
 }
 
 __global__ void k_spread_fill(int comp_mask, const MarkerRec* __restrict__ rec, long m, int nb1,
@@ -138,7 +142,9 @@ __global__ void k_spread_fill(int comp_mask, const MarkerRec* __restrict__ rec, 
 	list[offs[b] + slot] = (int) k;
 }
 
-// One workgroup (TS x TS lanes) per (tile, component).
+// One workgroup (TS x TS lanes) per (tile, component).  The distinct
+// neighbouring bins and their sizes are computed redundantly by every lane,
+// so empty tiles (the common case) write their zeros without a barrier.
 __global__ __launch_bounds__(256) void k_spread_tile(int comp_mask, const MarkerRec* __restrict__ rec, long m,
                                                      const int* __restrict__ offs, const int* __restrict__ list,
                                                      unsigned nx, unsigned ny, int ntx, int nty,
@@ -146,18 +152,14 @@ __global__ __launch_bounds__(256) void k_spread_tile(int comp_mask, const Marker
                                                      int* __restrict__ err, int accumulate)
 {
 	__shared__ int cand[KMAX];
-	__shared__ int ncand;
 	const int comp = blockIdx.y;
 	if (!((comp_mask >> comp) & 1)) return;
 	const int nb1 = ntx * nty;
 	const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
 	const int tx = tile % ntx, ty = tile / ntx;
 	const int tid = threadIdx.x;
-	const int cxl = tid % TS, cyl = tid / TS;
-	const int gx = tx * TS + cxl, gy = ty * TS + cyl;
+	const int gx = tx * TS + tid % TS, gy = ty * TS + tid / TS;
 	double* f = comp == 0 ? fx : fy;
-	if (tid == 0) ncand = 0;
-	__syncthreads();
 	// distinct neighbouring bins (periodic; fewer than 3 tiles per axis
 	// would otherwise visit a bin twice)
 	int bxs[3], bys[3], nbx = 0, nby = 0;
@@ -171,29 +173,27 @@ __global__ __launch_bounds__(256) void k_spread_tile(int comp_mask, const Marker
 		for (int j = 0; j < nby; ++j) dup |= bys[j] == b;
 		if (!dup) bys[nby++] = b;
 	}
+	int o0[9], cnt[9], nb = 0, K = 0;
 	for (int j = 0; j < nby; ++j)
 		for (int i = 0; i < nbx; ++i) {
-			int b = comp * nb1 + bxs[i] + ntx * bys[j];
-			int o0 = offs[b], o1 = offs[b + 1];
-			int cnt = o1 - o0;
-			if (cnt == 0) continue;
-			__syncthreads();
-			int base = ncand;
-			__syncthreads();
-			for (int t = tid; t < cnt; t += blockDim.x)
-				if (base + t < KMAX) cand[base + t] = list[o0 + t];
-			__syncthreads();
-			if (tid == 0) ncand = base + cnt;
-			__syncthreads();
+			const int b = comp * nb1 + bxs[i] + ntx * bys[j];
+			o0[nb] = offs[b];
+			cnt[nb] = offs[b + 1] - o0[nb];
+			K += cnt[nb];
+			++nb;
 		}
-	int K = ncand;
+	if (K == 0) {
+		if (!accumulate && gx < (int) nx && gy < (int) ny) f[gx + (size_t) nx * gy] = 0.0;
+		return;
+	}
 	if (K > KMAX) {
 		if (tid == 0) atomicOr(err, 1);
 		K = KMAX;
 	}
-	if (K == 0) {
-		if (!accumulate && gx < (int) nx && gy < (int) ny) f[gx + (size_t) nx * gy] = 0.0;
-		return;
+	for (int t = tid; t < K; t += blockDim.x) {
+		int q = t, j = 0;
+		while (q >= cnt[j]) { q -= cnt[j]; ++j; }
+		cand[t] = list[o0[j] + q];
 	}
 	// bitonic sort of the candidate marker indices (ascending)
 	int P = 1;
@@ -213,8 +213,9 @@ __global__ __launch_bounds__(256) void k_spread_tile(int comp_mask, const Marker
 		}
 	if (gx >= (int) nx || gy >= (int) ny) return;
 	double acc = 0.0;
+	const MarkerRec* rc = rec + comp * m;
 	for (int j = 0; j < K; ++j) {
-		const MarkerRec& r = rec[comp * m + cand[j]];
+		const MarkerRec& r = rc[cand[j]];
 		int a = gx - r.ix0; if (a < 0) a += nx;
 		int b = gy - r.iy0; if (b < 0) b += ny;
 		if (a < 4 && b < 4) {

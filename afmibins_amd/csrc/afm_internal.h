@@ -59,7 +59,7 @@ struct SolveCtl {
 
 constexpr int NCTL = 8;          // ctl[0] MG, ctl[1..2] CG fields, ctl[3] misc
 constexpr int RED_THREADS = 256;
-constexpr int MAX_RED_BLOCKS = 4096;
+constexpr int MAX_RED_BLOCKS = 1024;   // grid of the reducing kernels (grid-stride loops)
 
 // Per-marker, per-component transfer record used by spreading.
 struct MarkerRec {

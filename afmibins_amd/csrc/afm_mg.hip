@@ -24,7 +24,7 @@ namespace afm {
 constexpr int STX = 64, STY = 16;        // smoother tile
 constexpr int RTX = 32, RTY = 8;         // restriction tile (coarse cells)
 constexpr int CMAXL = 5;                 // finest level handled in LDS (32x32)
-constexpr int CTHREADS = 1024;
+constexpr int CTHREADS = 512;
 
 enum { M_NORMAL = 0, M_ZERO = 1, M_PADD = 2, M_PSET = 3 };
 
@@ -205,8 +205,17 @@ __global__ __launch_bounds__(256) void k_update_resid(const double* __restrict__
 		res[i] = (TR) r;
 		acc[0] += r * r;
 	}
+	reduce_partial<1>(acc, partials);
+}
+
+// RMS of the residual and the stop test (mg/solver.h:228), one block.
+__global__ __launch_bounds__(256) void k_update_fin(int first, unsigned nblk, size_t N, SolveCtl* __restrict__ ctl,
+                                                    const double* __restrict__ partials)
+{
+	if (!first && ctl->done) return;
 	double tot[1];
-	if (reduce_last<1>(acc, partials, &ctl->ticket, tot) && threadIdx.x == 0) {
+	red_total<1>(partials, nblk, tot);
+	if (threadIdx.x == 0) {
 		double rms = sqrt(tot[0]) / sqrt((double) N);
 		int count = first ? 0 : ctl->count + 1;
 		ctl->count = count;
@@ -215,8 +224,8 @@ __global__ __launch_bounds__(256) void k_update_resid(const double* __restrict__
 	}
 }
 
-// sum(rhs * 1) -> ctl->mean = sum / N ; second pass: tol = rel * RMS(rhs - mean)
-__global__ __launch_bounds__(256) void k_mean(const double* __restrict__ rhs, size_t N, int pass, double rel,
+// sum(rhs * 1) -> mean (pass 0); RMS(rhs - mean) -> relative tol (pass 1)
+__global__ __launch_bounds__(256) void k_mean(const double* __restrict__ rhs, size_t N, int pass,
                                               SolveCtl* __restrict__ ctl, double* __restrict__ partials)
 {
 	double acc[1] = {0.0};
@@ -225,8 +234,15 @@ __global__ __launch_bounds__(256) void k_mean(const double* __restrict__ rhs, si
 		if (pass == 0) acc[0] += rhs[i] * 1.0;
 		else { double t = rhs[i] - mean * 1.0; acc[0] += t * t; }
 	}
+	reduce_partial<1>(acc, partials);
+}
+
+__global__ __launch_bounds__(256) void k_mean_fin(int pass, double rel, unsigned nblk, size_t N,
+                                                  SolveCtl* __restrict__ ctl, const double* __restrict__ partials)
+{
 	double tot[1];
-	if (reduce_last<1>(acc, partials, &ctl->ticket, tot) && threadIdx.x == 0) {
+	red_total<1>(partials, nblk, tot);
+	if (threadIdx.x == 0) {
 		if (pass == 0) ctl->mean = tot[0] / (double) N;
 		else ctl->tol = rel * (sqrt(tot[0]) / sqrt((double) N));
 	}
@@ -241,27 +257,32 @@ __global__ void k_copy_cvt(const T* __restrict__ src, double* __restrict__ dst, 
 
 // ------------------------------------------------------------- coarse -----
 // Levels 0..Lt (n <= 32) of the unit square, one workgroup, all in LDS.
+// Per-point arithmetic is the global kernels': the 9-slot stencil order for
+// n >= 4 and the reference's folded rows (mg/laplacian.h:62-95) at n = 2.
+// Red points are visited directly (no idle lanes), and "x += P e" is fused
+// into the first red half-sweep: red updates read black neighbours as
+// x + P e, black updates never read their own old value.
 struct CoarseLDS {
 	double X[1365], B[1365], CP[1365], TMP[1024];
 };
 
 __device__ inline int lvl_off(int l) { return ((1 << (2 * l)) - 1) / 3; }   // sum_{j<l} 4^j
 
-__device__ void c_smooth(CoarseLDS& S, int l, double* x, const double* b, bool zero, int sweeps)
+// level-1 (2x2) smoothing with the general folded rows
+__device__ void c_smooth_l1(double* x, const double* b, bool zero, int sweeps)
 {
-	const int nx = 1 << l, N = nx * nx;
-	const double off = (double) ((1u << l) * (1u << l));
+	const int nx = 2, N = 4;
+	const double off = 4.0;
 	if (zero) {
-		for (int i = threadIdx.x; i < N; i += CTHREADS) x[i] = 0.0;
+		if (threadIdx.x < N) x[threadIdx.x] = 0.0;
 		__syncthreads();
 	}
 	for (int s = 0; s < sweeps; ++s)
 		for (int color = 0; color < 2; ++color) {
-			for (int i = threadIdx.x; i < N; i += CTHREADS) {
-				int gx = i % nx, gy = i / nx;
-				if (((gx + gy) & 1) != color) continue;
+			const int i = threadIdx.x;
+			if (i < N && (((i & 1) + (i >> 1)) & 1) == color) {
 				LapRow r;
-				lap_row(gx, gy, nx, nx, off, r);
+				lap_row(i & 1, i >> 1, nx, nx, off, r);
 				double value = 0.0, diagonal = 0.0;
 				for (int k = 0; k < r.cnt; ++k) {
 					if (r.col[k] == i) { diagonal = r.a[k]; continue; }
@@ -273,32 +294,71 @@ __device__ void c_smooth(CoarseLDS& S, int l, double* x, const double* b, bool z
 		}
 }
 
+// red-black sweeps at level l >= 2; zero: x starts at 0; xc: x += P xc first
+__device__ void c_smooth(int l, double* x, const double* b, bool zero, const double* xc, int sweeps)
+{
+	const int nx = 1 << l, msk = nx - 1, hx = nx >> 1, N2 = (nx * nx) >> 1;
+	const double off = (double) ((1u << l) * (1u << l)), dg = -4.0 * off;
+	for (int s = 0; s < sweeps; ++s)
+		for (int color = 0; color < 2; ++color) {
+			const bool first = s == 0 && color == 0;
+			for (int k = threadIdx.x; k < N2; k += CTHREADS) {
+				const int gy = k >> (l - 1), gx = ((k & (hx - 1)) << 1) + ((color + gy) & 1);
+				const int i = gx + (gy << l);
+				double value;
+				if (first && zero) value = 0.0;
+				else {
+					const int ex = (gx + 1) & msk, wx = (gx - 1) & msk, ny_ = (gy + 1) & msk, sy = (gy - 1) & msk;
+					double E = x[ex + (gy << l)], Wv = x[wx + (gy << l)], Nv = x[gx + (ny_ << l)], Sv = x[gx + (sy << l)];
+					if (first && xc) {
+						E = E + prolong_at(xc, ex, gy, hx, hx);
+						Wv = Wv + prolong_at(xc, wx, gy, hx, hx);
+						Nv = Nv + prolong_at(xc, gx, ny_, hx, hx);
+						Sv = Sv + prolong_at(xc, gx, sy, hx, hx);
+					}
+					value = stencil5<double, false>(off, dg, 0.0, E, Wv, Nv, Sv, gx == msk, gx == 0, gy == msk, gy == 0);
+				}
+				x[i] = (b[i] - value) / dg;
+			}
+			__syncthreads();
+		}
+}
+
 // dst(level l-1) = R src  (src = b - L x when x != nullptr)
 __device__ void c_restrict(CoarseLDS& S, int l, const double* x, const double* src, double* dst)
 {
-	const int nx = 1 << l, N = nx * nx, cn = nx >> 1;
+	const int nx = 1 << l, N = nx * nx, cn = nx >> 1, msk = nx - 1;
 	const double off = (double) ((1u << l) * (1u << l));
 	const double* in = src;
 	if (x) {
 		for (int i = threadIdx.x; i < N; i += CTHREADS) {
-			LapRow r;
-			lap_row(i % nx, i / nx, nx, nx, off, r);
-			double y = 0.0;
-			for (int k = 0; k < r.cnt; ++k) y += r.a[k] * x[r.col[k]];
+			const int gx = i & msk, gy = i >> l;
+			double y;
+			if (l >= 2) {
+				const int ex = (gx + 1) & msk, wx = (gx - 1) & msk, ny_ = (gy + 1) & msk, sy = (gy - 1) & msk;
+				y = stencil5<double, true>(off, -4.0 * off, x[i], x[ex + (gy << l)], x[wx + (gy << l)],
+				                           x[gx + (ny_ << l)], x[gx + (sy << l)], gx == msk, gx == 0, gy == msk, gy == 0);
+			} else {
+				LapRow r;
+				lap_row(gx, gy, nx, nx, off, r);
+				y = 0.0;
+				for (int k = 0; k < r.cnt; ++k) y += r.a[k] * x[r.col[k]];
+			}
 			S.TMP[i] = src[i] - y;
 		}
 		__syncthreads();
 		in = S.TMP;
 	}
 	for (int c = threadIdx.x; c < cn * cn; c += CTHREADS) {
-		int xcc = c % cn, zcc = c / cn;
-		int xcf = 2 * xcc + 1, zcf = 2 * zcc + 1;
+		const int xcc = c & (cn - 1), zcc = c >> (l - 1);
+		const int xcf = 2 * xcc + 1, zcf = 2 * zcc + 1;
 		double o = 0.0;
+#pragma unroll
 		for (int i = 0; i < 16; ++i) {
-			int dx = (i % 4) - 2, dz = i / 4 - 2;
-			int xx = (xcf + dx + nx) % nx, zz = (zcf + dz + nx) % nx;
-			int w = (4 - abs(2 * dx + 1)) * (4 - abs(2 * dz + 1));
-			o += w / 64. * in[xx + nx * zz];
+			const int dx = (i % 4) - 2, dz = i / 4 - 2;
+			const int xx = (xcf + dx + nx) & msk, zz = (zcf + dz + nx) & msk;
+			const int w = (4 - abs(2 * dx + 1)) * (4 - abs(2 * dz + 1));
+			o += w / 64. * in[xx + (zz << l)];
 		}
 		dst[c] = o;
 	}
@@ -310,7 +370,7 @@ __device__ void c_prolong(int l, const double* xc, double* dst, bool add)
 {
 	const int nx = 1 << l, N = nx * nx, h = nx >> 1;
 	for (int i = threadIdx.x; i < N; i += CTHREADS) {
-		double v = prolong_at(xc, i % nx, i / nx, h, h);
+		double v = prolong_at(xc, i & (nx - 1), i >> l, h, h);
 		dst[i] = add ? dst[i] + v : v;
 	}
 	__syncthreads();
@@ -331,13 +391,17 @@ __device__ void c_solve0(CoarseLDS& S)
 __device__ void c_recurse(CoarseLDS& S, int l, int nu1, int nu2)
 {
 	for (int k = l; k >= 1; --k) {
-		c_smooth(S, k, S.X + lvl_off(k), S.B + lvl_off(k), true, nu1);
+		if (k >= 2) c_smooth(k, S.X + lvl_off(k), S.B + lvl_off(k), true, nullptr, nu1);
+		else c_smooth_l1(S.X + lvl_off(k), S.B + lvl_off(k), true, nu1);
 		c_restrict(S, k, S.X + lvl_off(k), S.B + lvl_off(k), S.B + lvl_off(k - 1));
 	}
 	c_solve0(S);
 	for (int k = 1; k <= l; ++k) {
-		c_prolong(k, S.X + lvl_off(k - 1), S.X + lvl_off(k), true);
-		c_smooth(S, k, S.X + lvl_off(k), S.B + lvl_off(k), false, nu2);
+		if (k >= 2) c_smooth(k, S.X + lvl_off(k), S.B + lvl_off(k), false, S.X + lvl_off(k - 1), nu2);
+		else {
+			c_prolong(k, S.X + lvl_off(k - 1), S.X + lvl_off(k), true);
+			c_smooth_l1(S.X + lvl_off(k), S.B + lvl_off(k), false, nu2);
+		}
 	}
 }
 
@@ -351,7 +415,8 @@ __device__ void c_nested(CoarseLDS& S, int l, int nu1, int nu2)
 	for (int k = 1; k <= l; ++k) {
 		c_recurse(S, k - 1, nu1, nu2);
 		c_prolong(k, S.X + lvl_off(k - 1), S.B + lvl_off(k), false);
-		c_smooth(S, k, S.B + lvl_off(k), S.CP + lvl_off(k), false, nu2);
+		if (k >= 2) c_smooth(k, S.B + lvl_off(k), S.CP + lvl_off(k), false, nullptr, nu2);
+		else c_smooth_l1(S.B + lvl_off(k), S.CP + lvl_off(k), false, nu2);
 	}
 }
 
@@ -547,8 +612,9 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 	if (nu1 < 1 || nu1 > 2 || nu2 < 1 || nu2 > 2) { set_error("nu1, nu2 must be 1 or 2 (one fused launch per smoothing step)"); return -1; }
 	if (precision != 0 && precision != 1) { set_error("precision must be 0 (fp64) or 1 (fp32 V-cycle in fp64 refinement)"); return -1; }
 	SolveCtl* dctl = &c->ctl[0];
-	SolveCtl h{};
-	h.tol = tol; h.max_iter = max_cycles; h.count = 0; h.done = 0; h.ticket = 0; h.mean = 0.0;
+	SolveCtl& h = c->hctl[0];      // pinned staging
+	h = SolveCtl{};
+	h.tol = tol; h.max_iter = max_cycles;
 	AFM_HIP(hipMemcpyAsync(dctl, &h, sizeof(SolveCtl), hipMemcpyHostToDevice, c->stream));
 	const size_t N = c->N;
 	const unsigned rb = red_blocks(N);
@@ -557,10 +623,9 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 	// mean (and, for a relative tolerance, RMS of rhs - mean) before anything
 	// else, as the oracle computes it (mg/solver.h:224 does the mean after the
 	// nested start; the value is the same because rhs is not touched there).
-	hipLaunchKernelGGL(k_mean, dim3(rb), dim3(256), 0, c->stream, rhs, N, 0, 0.0, dctl, c->partials);
-	AFM_LAUNCH_CHECK();
-	if (tol_mode == 1) {
-		hipLaunchKernelGGL(k_mean, dim3(rb), dim3(256), 0, c->stream, rhs, N, 1, tol, dctl, c->partials);
+	for (int pass = 0; pass < (tol_mode == 1 ? 2 : 1); ++pass) {
+		hipLaunchKernelGGL(k_mean, dim3(rb), dim3(256), 0, c->stream, rhs, N, pass, dctl, c->partials);
+		hipLaunchKernelGGL(k_mean_fin, dim3(1), dim3(256), 0, c->stream, pass, tol, rb, N, dctl, c->partials);
 		AFM_LAUNCH_CHECK();
 	}
 	AFM_TRY(nested(c, rhs, nu1, nu2));
@@ -571,6 +636,7 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 	else
 		hipLaunchKernelGGL((k_update_resid<float, float>), dim3(rb), dim3(256), 0, c->stream, c->XSa,
 		                   (const float*) nullptr, rhs, c->XSa, c->RES32, (int) c->nx, (int) c->ny, off, dg, 1, 1, dctl, c->partials);
+	hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, 1, rb, N, dctl, c->partials);
 	AFM_LAUNCH_CHECK();
 	const int* done = &dctl->done;
 	int enq = 0;   // cycles enqueued
@@ -590,6 +656,7 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 				hipLaunchKernelGGL((k_update_resid<float, float>), dim3(rb), dim3(256), 0, c->stream, cur,
 				                   (const float*) c->lv[T].Xb32, rhs, nxt, c->RES32, (int) c->nx, (int) c->ny, off, dg, 0, 0, dctl, c->partials);
 			}
+			hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, 0, rb, N, dctl, c->partials);
 			AFM_LAUNCH_CHECK();
 			++enq;
 		}

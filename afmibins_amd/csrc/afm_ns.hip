@@ -118,19 +118,25 @@ __global__ void k_helm_apply(const double* __restrict__ x, double* __restrict__ 
 }
 
 // ---------------------------------------------------------------------- CG
-// Per-field pointers, indexed by blockIdx.y.
+// Per-field pointers, indexed by blockIdx.y (blockIdx.x in the one-block
+// finalize kernels).
 struct CGArgs {
 	double* x[2];
 	const double* b[2];
 	double* r[2];
 	double* p[2][2];       // ping-pong
-	double* z[2];
 	SolveCtl* ctl[2];
 	double* partials[2];
 	int tol_mode;
 };
 
-// r = b - A x, p = r, n0 = r.r  (alg/cg.h:38-39); tol from |b| if relative.
+#define CG_NEIGHBOURS                                                                              \
+	const int gx = (int) (i % nx), gy = (int) (i / nx);                                            \
+	const size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;    \
+	const size_t in_ = gy == ny - 1 ? (size_t) gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;    \
+	const bool we = gx == nx - 1, ww = gx == 0, wn = gy == ny - 1, ws = gy == 0;
+
+// r = b - A x, p = r, n0 = r.r  (alg/cg.h:38-39); |b|^2 for a relative tol.
 __global__ __launch_bounds__(256) void k_cg_init(CGArgs a, int nx, int ny, double off, double dg)
 {
 	const int f = blockIdx.y;
@@ -140,19 +146,25 @@ __global__ __launch_bounds__(256) void k_cg_init(CGArgs a, int nx, int ny, doubl
 	const double* b = a.b[f];
 	double acc[2] = {0.0, 0.0};
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
-		const int gx = (int) (i % nx), gy = (int) (i / nx);
-		const size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;
-		const size_t in_ = gy == ny - 1 ? (size_t) gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;
-		double Ax = stencil5<double, true>(off, dg, x[i], x[ie], x[iw], x[in_], x[is], gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
+		CG_NEIGHBOURS
+		double Ax = stencil5<double, true>(off, dg, x[i], x[ie], x[iw], x[in_], x[is], we, ww, wn, ws);
 		double r = b[i] - Ax;
 		a.r[f][i] = r;
 		a.p[f][0][i] = r;
 		acc[0] += r * r;
 		acc[1] += b[i] * b[i];
 	}
+	reduce_partial<2>(acc, a.partials[f]);
+}
+
+__global__ __launch_bounds__(256) void k_cg_init_fin(CGArgs a, unsigned nblk)
+{
+	const int f = blockIdx.x;
+	if (!a.x[f]) return;
 	double tot[2];
-	SolveCtl* c = a.ctl[f];
-	if (reduce_last<2>(acc, a.partials[f], &c->ticket, tot) && threadIdx.x == 0) {
+	red_total<2>(a.partials[f], nblk, tot);
+	if (threadIdx.x == 0) {
+		SolveCtl* c = a.ctl[f];
 		if (a.tol_mode == 1) c->tol = c->tol * sqrt(tot[1]);
 		c->n0 = tot[0];
 		c->count = 0;
@@ -161,8 +173,9 @@ __global__ __launch_bounds__(256) void k_cg_init(CGArgs a, int nx, int ny, doubl
 	}
 }
 
-// p' = (count == 0) ? p : p*mu + r ;  z = A p' ;  nu = n0 / (p'.z)
-__global__ __launch_bounds__(256) void k_cg_pz(CGArgs a, int nx, int ny, double off, double dg, int pin)
+// p' = (count == 0) ? p : p*mu + r  (alg/cg.h:51-52 of the previous
+// iteration), stored; partial p'.(A p')
+__global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double off, double dg, int pin)
 {
 	const int f = blockIdx.y;
 	if (!a.x[f]) return;
@@ -176,26 +189,32 @@ __global__ __launch_bounds__(256) void k_cg_pz(CGArgs a, int nx, int ny, double 
 	const double mu = c->mu;
 	double acc[1] = {0.0};
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
-		const int gx = (int) (i % nx), gy = (int) (i / nx);
-		const size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;
-		const size_t in_ = gy == ny - 1 ? (size_t) gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;
+		CG_NEIGHBOURS
 		double pc, pe, pw, pn, ps;
 		if (first) { pc = p[i]; pe = p[ie]; pw = p[iw]; pn = p[in_]; ps = p[is]; }
 		else {
 			pc = p[i] * mu + r[i]; pe = p[ie] * mu + r[ie]; pw = p[iw] * mu + r[iw];
 			pn = p[in_] * mu + r[in_]; ps = p[is] * mu + r[is];
 		}
-		double z = stencil5<double, true>(off, dg, pc, pe, pw, pn, ps, gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
+		double z = stencil5<double, true>(off, dg, pc, pe, pw, pn, ps, we, ww, wn, ws);
 		po[i] = pc;
-		a.z[f][i] = z;
 		acc[0] += pc * z;
 	}
-	double tot[1];
-	if (reduce_last<1>(acc, a.partials[f], &c->ticket, tot) && threadIdx.x == 0) c->nu = c->n0 / tot[0];
+	reduce_partial<1>(acc, a.partials[f]);
 }
 
-// x += p nu ; r -= z nu ; n1 = r.r ; mu = n1/n0 ; n0 = n1 ; stop test
-__global__ __launch_bounds__(256) void k_cg_update(CGArgs a, int nx, int ny, int pcur)
+// nu = n0 / (p.z)
+__global__ __launch_bounds__(256) void k_cg_p_fin(CGArgs a, unsigned nblk)
+{
+	const int f = blockIdx.x;
+	if (!a.x[f] || a.ctl[f]->done) return;
+	double tot[1];
+	red_total<1>(a.partials[f], nblk, tot);
+	if (threadIdx.x == 0) a.ctl[f]->nu = a.ctl[f]->n0 / tot[0];
+}
+
+// z = A p (recomputed, same bits), x += p nu, r -= z nu; partial r.r
+__global__ __launch_bounds__(256) void k_cg_update(CGArgs a, int nx, int ny, double off, double dg, int pcur)
 {
 	const int f = blockIdx.y;
 	if (!a.x[f]) return;
@@ -206,16 +225,29 @@ __global__ __launch_bounds__(256) void k_cg_update(CGArgs a, int nx, int ny, int
 	double* x = a.x[f];
 	double* r = a.r[f];
 	const double* p = a.p[f][pcur];
-	const double* z = a.z[f];
 	double acc[1] = {0.0};
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
-		x[i] += p[i] * nu;
-		double ri = r[i] - z[i] * nu;
+		CG_NEIGHBOURS
+		const double pc = p[i];
+		double z = stencil5<double, true>(off, dg, pc, p[ie], p[iw], p[in_], p[is], we, ww, wn, ws);
+		x[i] += pc * nu;
+		z *= nu;
+		double ri = r[i] - z;
 		r[i] = ri;
 		acc[0] += ri * ri;
 	}
+	reduce_partial<1>(acc, a.partials[f]);
+}
+
+// mu = n1/n0 ; n0 = n1 ; stop test (alg/cg.h:42, 48-50)
+__global__ __launch_bounds__(256) void k_cg_update_fin(CGArgs a, unsigned nblk)
+{
+	const int f = blockIdx.x;
+	if (!a.x[f] || a.ctl[f]->done) return;
 	double tot[1];
-	if (reduce_last<1>(acc, a.partials[f], &c->ticket, tot) && threadIdx.x == 0) {
+	red_total<1>(a.partials[f], nblk, tot);
+	if (threadIdx.x == 0) {
+		SolveCtl* c = a.ctl[f];
 		double n1 = tot[0];
 		c->mu = n1 / c->n0;
 		c->n0 = n1;
@@ -225,12 +257,18 @@ __global__ __launch_bounds__(256) void k_cg_update(CGArgs a, int nx, int ny, int
 	}
 }
 
-__global__ __launch_bounds__(256) void k_sum(const double* __restrict__ x, size_t N, SolveCtl* ctl, double* partials)
+__global__ __launch_bounds__(256) void k_sum(const double* __restrict__ x, size_t N, double* partials)
 {
 	double acc[1] = {0.0};
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) acc[0] += x[i];
+	reduce_partial<1>(acc, partials);
+}
+
+__global__ __launch_bounds__(256) void k_sum_fin(unsigned nblk, SolveCtl* ctl, const double* partials)
+{
 	double tot[1];
-	if (reduce_last<1>(acc, partials, &ctl->ticket, tot) && threadIdx.x == 0) ctl->sum2 = tot[0];
+	red_total<1>(partials, nblk, tot);
+	if (threadIdx.x == 0) ctl->sum2 = tot[0];
 }
 
 static inline unsigned rblocks(size_t N)
@@ -243,7 +281,8 @@ int launch_sum(Ctx* c, const double* x, size_t N, double* out_host)
 {
 	SolveCtl h{};
 	AFM_HIP(hipMemcpyAsync(&c->ctl[3], &h, sizeof(SolveCtl), hipMemcpyHostToDevice, c->stream));
-	hipLaunchKernelGGL(k_sum, dim3(rblocks(N)), dim3(256), 0, c->stream, x, N, &c->ctl[3], c->partials);
+	hipLaunchKernelGGL(k_sum, dim3(rblocks(N)), dim3(256), 0, c->stream, x, N, c->partials);
+	hipLaunchKernelGGL(k_sum_fin, dim3(1), dim3(256), 0, c->stream, rblocks(N), &c->ctl[3], c->partials);
 	AFM_LAUNCH_CHECK();
 	AFM_HIP(hipMemcpyAsync(&c->hctl[3], &c->ctl[3], sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
 	AFM_HIP(hipStreamSynchronize(c->stream));
@@ -292,7 +331,7 @@ int op_helm_apply(Ctx* c, double scale, const double* x, double* y)
 
 // CG on up to two fields at once.  Scratch: r, z, p0, p1 per field.
 struct CGWork {
-	double *r[2] = {nullptr, nullptr}, *z[2] = {nullptr, nullptr}, *p[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+	double *r[2] = {nullptr, nullptr}, *p[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 };
 
 static int dalloc(std::vector<void*>& own, void** p, size_t bytes)
@@ -306,7 +345,6 @@ static int cg_alloc(Ctx* c, CGWork& w, std::vector<void*>& own)
 {
 	for (int f = 0; f < 2; ++f) {
 		AFM_TRY(dalloc(own, (void**) &w.r[f], c->N * 8));
-		AFM_TRY(dalloc(own, (void**) &w.z[f], c->N * 8));
 		AFM_TRY(dalloc(own, (void**) &w.p[f][0], c->N * 8));
 		AFM_TRY(dalloc(own, (void**) &w.p[f][1], c->N * 8));
 	}
@@ -325,7 +363,6 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 		a.x[f] = x[f];
 		a.b[f] = b[f];
 		a.r[f] = w.r[f];
-		a.z[f] = w.z[f];
 		a.p[f][0] = w.p[f][0];
 		a.p[f][1] = w.p[f][1];
 		a.ctl[f] = &c->ctl[1 + f];
@@ -337,6 +374,7 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 	a.tol_mode = tol_mode;
 	const unsigned rb = rblocks(c->N);
 	hipLaunchKernelGGL(k_cg_init, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg);
+	hipLaunchKernelGGL(k_cg_init_fin, dim3(2), dim3(256), 0, c->stream, a, rb);
 	AFM_LAUNCH_CHECK();
 	int enq = 0;
 	int batch = c->last_cg_iters > 0 ? c->last_cg_iters : 4;
@@ -344,8 +382,10 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 		if (batch > max_iters - enq) batch = max_iters - enq;
 		for (int s = 0; s < batch; ++s) {
 			int pin = enq & 1;
-			hipLaunchKernelGGL(k_cg_pz, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg, pin);
-			hipLaunchKernelGGL(k_cg_update, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, pin ^ 1);
+			hipLaunchKernelGGL(k_cg_p, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg, pin);
+			hipLaunchKernelGGL(k_cg_p_fin, dim3(2), dim3(256), 0, c->stream, a, rb);
+			hipLaunchKernelGGL(k_cg_update, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg, pin ^ 1);
+			hipLaunchKernelGGL(k_cg_update_fin, dim3(2), dim3(256), 0, c->stream, a, rb);
 			AFM_LAUNCH_CHECK();
 			++enq;
 		}

@@ -42,6 +42,8 @@ def parse():
     p.add_argument("--n", type=int, default=2048)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--forms", default="fp64,mixed")
+    p.add_argument("--no-extras", action="store_true", help="skip the large-grid and transfer measurements")
+    p.add_argument("--kernel-only", action="store_true", help="only the smoother loop and a calibration copy")
     return p.parse_args()
 
 
@@ -67,35 +69,92 @@ def run_form(A, W, ctx, precision, args, inputs, dev):
     return out
 
 
-def smoother_roofline(A, ctx, dev, reps=50):
-    """Average duration of the fused 2-sweep red-black smoother at the finest
-    level, timed with HIP events on the stream it is launched on (torch's
-    current stream, to which the context is bound)."""
+def _events(dev):
     import torch
-    from afmibins_amd import _lib
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def smoother_roofline(A, ctx, dev, reps=50):
+    """Average duration of the fused 2-sweep red-black smoother on the
+    context's finest level, timed with HIP events on the stream it is launched
+    on (torch's current stream, to which the context is bound)."""
+    import torch
     n = ctx.n
     b = torch.randn((n, n), dtype=torch.float64, device=dev)
     x = [torch.randn((n, n), dtype=torch.float64, device=dev) for _ in range(2)]
-    L = _lib.load()
-    for i in range(5):
+
+    def launch(i):
         ctx._call("afm_mg_smooth", ctx.levels, C.c_void_p(x[i & 1].data_ptr()), C.c_void_p(b.data_ptr()),
                   C.c_void_p(x[(i + 1) & 1].data_ptr()), 2, 0)
+    for i in range(5):
+        launch(i)
     s = torch.cuda.current_stream(dev)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0, e1 = _events(dev)
     e0.record(s)
     for i in range(reps):
-        ctx._call("afm_mg_smooth", ctx.levels, C.c_void_p(x[i & 1].data_ptr()), C.c_void_p(b.data_ptr()),
-                  C.c_void_p(x[(i + 1) & 1].data_ptr()), 2, 0)
+        launch(i)
     e1.record(s)
     e1.synchronize()
     ms = e0.elapsed_time(e1) / reps
     bytes_per_launch = 24.0 * n * n        # read x, read b, write x (2 sweeps fused)
     gbs = bytes_per_launch / (ms * 1e-3) / 1e9
-    del L
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_smooth<double,2,M_NORMAL> (2 fused red-black sweeps)",
+            "kernel": f"k_smooth<double,2,M_NORMAL> (2 fused red-black sweeps, {n}x{n})",
             "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(ms * 1e3, 2)}
+
+
+def transfer_rates(A, W, dev, reps=10):
+    """Spreading and interpolation of the clustered 262,144-marker case on a
+    4096x4096 grid (both MAC components), timed per call with HIP events;
+    algorithmic bytes: spread 8 B per cell (dense write) + 32 B per marker
+    (X, F, ds); interpolation 16 + 16*8 + 8 = 152 B per marker."""
+    import torch
+    n = 4096
+    X, R, k, ds = W.clustered_structures()
+    m = X.shape[0]
+    ctx = A.Context(n, 0)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)
+    Xd, dsd = t(X), t(ds)
+    F = torch.randn(m, dtype=torch.float64, device=dev)
+    f = [torch.empty((n, n), dtype=torch.float64, device=dev) for _ in range(2)]
+    U = torch.randn((n, n), dtype=torch.float64, device=dev)
+    out = {}
+    for name, fn, nbytes in (
+            ("spread", lambda i: ctx.spread(i & 1, F, dsd, Xd, out=f[i & 1]), 8.0 * n * n + 32.0 * m),
+            ("interp", lambda i: ctx.interp(i & 1, U, Xd), 152.0 * m)):
+        for i in range(3):
+            fn(i)
+        s = torch.cuda.current_stream(dev)
+        e0, e1 = _events(dev)
+        e0.record(s)
+        for i in range(reps):
+            fn(i)
+        e1.record(s)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"us_per_call": round(ms * 1e3, 2), "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "frac_hbm": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "marker_per_s": round(m / (ms * 1e-3), 1)}
+    ctx.sync()
+    ctx.close()
+    out["case"] = f"{n}x{n} grid, {m} markers (512 membranes + 512 filaments x 256, 8 Gaussian clusters)"
+    return out
+
+
+def kernel_only(A, dev):
+    """For counter runs: the smoother at 2048^2 (50 launches) and a plain copy
+    of 256 MiB (the calibration kernel for FETCH_SIZE / WRITE_SIZE)."""
+    import torch
+    ctx = A.Context(2048, 0)
+    r = smoother_roofline(A, ctx, dev)
+    a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    ctx.close()
+    print(json.dumps({"smoother": r, "copy_bytes_each_way": a.numel() * 8}), flush=True)
 
 
 def cpu_baseline(inputs, n):
@@ -120,6 +179,9 @@ def main():
     import afmibins_amd as A
     from afmibins_amd import workloads as W
     dev = torch.device("cuda", 0)
+    if args.kernel_only:
+        kernel_only(A, dev)
+        return
     n = args.n
     inputs = W.config2_inputs(n=n)
     ctx = A.Context(n, 0)
@@ -156,6 +218,13 @@ def main():
     }
     if "fp64" in forms and "mixed" in forms:
         line["mixed_over_fp64"] = round(forms["mixed"]["steps_per_s"] / forms["fp64"]["steps_per_s"], 3)
+    if not args.no_extras:
+        big = A.Context(8192, 0)
+        line["roofline_hbm_8192"] = smoother_roofline(A, big, dev, reps=20)
+        big.close()
+        print(f"[bench] smoother 8192^2: {line['roofline_hbm_8192']['achieved']} GB/s", file=sys.stderr, flush=True)
+        line["transfer"] = transfer_rates(A, W, dev)
+        print(f"[bench] transfer: {line['transfer']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(inputs, n)
     ctx.close()
