@@ -28,6 +28,7 @@ SIGNATURES = {
     "afm_ctx_info": (_I, [_P, _UP, _UP, _UP, _UP]),
     "afm_sync": (_I, [_P]),
     "afm_set_stream": (_I, [_P, _P]),
+    "afm_own_stream": (_P, [_P]),
     "afm_alloc": (_I, [_P, C.c_ulong, C.POINTER(_P)]),
     "afm_free": (_I, [_P, _P]),
     "afm_copy_to_device": (_I, [_P, _P, _P, C.c_ulong]),

@@ -124,9 +124,11 @@ int afm_set_stream(afm_ctx* c, void* stream)
 	CHECK_ARG(c, "null context");
 	AFM_TRY(use(c));
 	AFM_HIP(hipStreamSynchronize(c->stream));
-	c->stream = stream ? (hipStream_t) stream : c->own_stream;
+	c->stream = (hipStream_t) stream;     // NULL: HIP's null stream
 	return 0;
 }
+
+void* afm_own_stream(afm_ctx* c) { return c ? (void*) c->own_stream : nullptr; }
 
 int afm_sync(afm_ctx* c)
 {

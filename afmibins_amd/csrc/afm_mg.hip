@@ -60,23 +60,39 @@ __global__ __launch_bounds__(256) void k_smooth(const T* __restrict__ xin, const
 	const int X0 = (tile % ntx) * STX, Y0 = (tile / ntx) * STY;
 	const int tid = threadIdx.x;
 	const int hx = nx >> 1, hy = ny >> 1;
-	for (int p = tid; p < LX * LY; p += 256) {
-		int lx = p % LX, ly = p / LX;
-		int gx = X0 - H + lx; gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
-		int gy = Y0 - H + ly; gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
-		T v;
-		if (MODE == M_ZERO) v = T(0);
-		else if (MODE == M_NORMAL) v = xin[gx + (size_t) nx * gy];
-		else if (MODE == M_PADD) v = xin[gx + (size_t) nx * gy] + prolong_at(xc, gx, gy, hx, hy);
-		else v = prolong_at(xc, gx, gy, hx, hy);
-		xs[p] = v;
+	// Stage the tile: every global load of this thread is issued before the
+	// first LDS store, so a wave keeps ~14 loads in flight instead of one.
+	constexpr int NXV = (LX * LY + 255) / 256, NBV = (BX * BY + 255) / 256;
+	T xv[NXV], bv[NBV];
+#pragma unroll
+	for (int k = 0; k < NXV; ++k) {
+		const int p = tid + 256 * k;
+		if (p < LX * LY) {
+			const int lx = p % LX, ly = p / LX;
+			int gx = X0 - H + lx; gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
+			int gy = Y0 - H + ly; gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
+			if (MODE == M_ZERO) xv[k] = T(0);
+			else if (MODE == M_NORMAL) xv[k] = xin[gx + (size_t) nx * gy];
+			else if (MODE == M_PADD) xv[k] = xin[gx + (size_t) nx * gy] + prolong_at(xc, gx, gy, hx, hy);
+			else xv[k] = prolong_at(xc, gx, gy, hx, hy);
+		}
 	}
-	for (int p = tid; p < BX * BY; p += 256) {
-		int lx = p % BX, ly = p / BX;
-		int gx = X0 - (H - 1) + lx; gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
-		int gy = Y0 - (H - 1) + ly; gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
-		bs[p] = b[gx + (size_t) nx * gy];
+#pragma unroll
+	for (int k = 0; k < NBV; ++k) {
+		const int p = tid + 256 * k;
+		if (p < BX * BY) {
+			const int lx = p % BX, ly = p / BX;
+			int gx = X0 - (H - 1) + lx; gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
+			int gy = Y0 - (H - 1) + ly; gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
+			bv[k] = b[gx + (size_t) nx * gy];
+		}
 	}
+#pragma unroll
+	for (int k = 0; k < NXV; ++k)
+		if (tid + 256 * k < LX * LY) xs[tid + 256 * k] = xv[k];
+#pragma unroll
+	for (int k = 0; k < NBV; ++k)
+		if (tid + 256 * k < BX * BY) bs[tid + 256 * k] = bv[k];
 	__syncthreads();
 #pragma unroll
 	for (int ph = 0; ph < 2 * S; ++ph) {
@@ -129,27 +145,52 @@ __global__ __launch_bounds__(256) void k_resid_restr(const T* __restrict__ x, co
 	const int C0x = (tile % ntx) * RTX, C0y = (tile / ntx) * RTY;
 	const int F0x = 2 * C0x - 1, F0y = 2 * C0y - 1;     // first fine cell of the taps
 	const int tid = threadIdx.x;
-	if (RESID) {
-		for (int p = tid; p < LX * LY; p += 256) {
-			int gx = F0x - 1 + p % LX, gy = F0y - 1 + p / LX;
+	// all global loads of a thread are issued before it touches LDS
+	constexpr int NXV = (LX * LY + 255) / 256, NRV = (FX * FY + 255) / 256;
+	T bv[NRV];
+#pragma unroll
+	for (int k = 0; k < NRV; ++k) {
+		const int p = tid + 256 * k;
+		if (p < FX * FY) {
+			int gx = F0x + p % FX, gy = F0y + p / FX;
 			gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
 			gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
-			xs[p] = x[gx + (size_t) nx * gy];
+			bv[k] = RESID ? b[gx + (size_t) nx * gy] : x[gx + (size_t) nx * gy];
 		}
+	}
+	if (RESID) {
+		T xv[NXV];
+#pragma unroll
+		for (int k = 0; k < NXV; ++k) {
+			const int p = tid + 256 * k;
+			if (p < LX * LY) {
+				int gx = F0x - 1 + p % LX, gy = F0y - 1 + p / LX;
+				gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
+				gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
+				xv[k] = x[gx + (size_t) nx * gy];
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < NXV; ++k)
+			if (tid + 256 * k < LX * LY) xs[tid + 256 * k] = xv[k];
 		__syncthreads();
 	}
-	for (int p = tid; p < FX * FY; p += 256) {
-		int lx = p % FX, ly = p / FX;
-		int gx = F0x + lx, gy = F0y + ly;
-		gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
-		gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
-		if (RESID) {
-			const int li = (lx + 1) + LX * (ly + 1);
-			T Lx = stencil5<T, true>(off, dg, xs[li], xs[li + 1], xs[li - 1], xs[li + LX], xs[li - LX],
-			                         gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
-			rs[p] = b[gx + (size_t) nx * gy] - Lx;
-		} else {
-			rs[p] = x[gx + (size_t) nx * gy];
+#pragma unroll
+	for (int k = 0; k < NRV; ++k) {
+		const int p = tid + 256 * k;
+		if (p < FX * FY) {
+			const int lx = p % FX, ly = p / FX;
+			if (RESID) {
+				int gx = F0x + lx, gy = F0y + ly;
+				gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
+				gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
+				const int li = (lx + 1) + LX * (ly + 1);
+				T Lx = stencil5<T, true>(off, dg, xs[li], xs[li + 1], xs[li - 1], xs[li + LX], xs[li - LX],
+				                         gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
+				rs[p] = bv[k] - Lx;
+			} else {
+				rs[p] = bv[k];
+			}
 		}
 	}
 	__syncthreads();

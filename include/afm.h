@@ -46,9 +46,12 @@ int afm_ctx_create(const afm_geom* g, int device, afm_ctx** out);
 int afm_ctx_destroy(afm_ctx* c);
 int afm_ctx_info(afm_ctx* c, unsigned* nx, unsigned* ny, unsigned* levels, unsigned* coarse_level);
 int afm_sync(afm_ctx* c);
-/* Order the context's work on an existing HIP stream (NULL: the context's
- * own stream).  Pending work on the previous stream is waited for first. */
+/* Order the context's work on an existing HIP stream (NULL: HIP's null
+ * stream, which is torch's default stream).  Pending work on the previous
+ * stream is waited for first.  A context starts on its own non-blocking
+ * stream, returned by afm_own_stream. */
 int afm_set_stream(afm_ctx* c, void* hip_stream);
+void* afm_own_stream(afm_ctx* c);
 const char* afm_last_error(void);
 
 /* Device memory on the context's GPU for callers that hold no HIP headers
