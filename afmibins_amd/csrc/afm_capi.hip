@@ -58,6 +58,11 @@ struct afm_ib { IB* b; };
 
 #define CHECK_ARG(cond, msg) do { if (!(cond)) { set_error(msg); return -1; } } while (0)
 
+// the grid kernels move two cells per vector access
+template<typename... P>
+static bool aligned16(P... p) { return ((((uintptr_t) (const void*) p & 15) == 0) && ...); }
+#define CHECK_ALIGNED(...) CHECK_ARG(aligned16(__VA_ARGS__), "field arrays must be 16-byte aligned")
+
 static int use(Ctx* c)
 {
 	AFM_HIP(hipSetDevice(c->device));
@@ -228,6 +233,7 @@ int afm_mg_smooth(afm_ctx* c, unsigned level, const double* x_in, const double* 
 {
 	CHECK_ARG(c && b && x_out && (zero_init || x_in), "null argument");
 	CHECK_ARG(x_in != x_out, "x_out must differ from x_in");
+	CHECK_ALIGNED(x_in, b, x_out);
 	AFM_TRY(use(c));
 	return mg_smooth(c, level, x_in, b, x_out, sweeps, zero_init);
 }
@@ -250,6 +256,7 @@ int afm_mg_vcycle(afm_ctx* c, double* rhs, int nu1, int nu2)
 {
 	CHECK_ARG(c && rhs, "null argument");
 	CHECK_ARG(nu1 >= 1 && nu1 <= 2 && nu2 >= 1 && nu2 <= 2, "nu1, nu2 must be 1 or 2");
+	CHECK_ALIGNED(rhs);
 	AFM_TRY(use(c));
 	return mg_vcycle(c, rhs, nu1, nu2);
 }
@@ -260,6 +267,7 @@ int afm_mg_solve(afm_ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 	CHECK_ARG(c && rhs, "null argument");
 	CHECK_ARG(tol_mode == 0 || tol_mode == 1, "tol_mode must be 0 or 1");
 	CHECK_ARG(tol >= 0 && max_cycles >= 0, "tol and max_cycles must be >= 0");
+	CHECK_ALIGNED(rhs);
 	AFM_TRY(use(c));
 	return mg_solve(c, rhs, tol, tol_mode, nu1, nu2, precision, max_cycles, cycles, final_rms);
 }

@@ -165,6 +165,41 @@ __device__ inline double ib_gather(const double* __restrict__ f, unsigned nx, un
 	return value;
 }
 
+// ------------------------------------------------------ lane exchange ----
+// Whole-wave DPP shifts by one lane (GFX9 wave_shl:1 / wave_shr:1): lane k
+// receives lane k+1's (SHL) or lane k-1's (SHR) value; the end lane keeps its
+// own.  No LDS traffic.
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
+
+template<int CTRL>
+__device__ inline double lane_shift(double v)
+{
+	int lo = __double2loint(v), hi = __double2hiint(v);
+	lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+	hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+	return __hiloint2double(hi, lo);
+}
+
+template<int CTRL>
+__device__ inline float lane_shift(float v)
+{
+	return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Two adjacent cells of one row, loaded and stored as one vector.
+template<typename T> struct Pair;
+template<> struct Pair<double> { using type = double2; };
+template<> struct Pair<float> { using type = float2; };
+
+// g in [-n, 2n) folded into [0, n)
+__device__ inline int wrap_once(int g, int n)
+{
+	g += (g < 0) ? n : 0;
+	g -= (g >= n) ? n : 0;
+	return g;
+}
+
 // XCD-aware block order: consecutive block ids go to different XCDs (round
 // robin over 8), so map them back to spatially consecutive tiles per XCD.
 __device__ inline unsigned xcd_tile(unsigned bid, unsigned nblk)

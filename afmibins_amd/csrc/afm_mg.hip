@@ -8,9 +8,9 @@
 // (the reference's lexicographic sweep is inherently serial); each row update
 // uses the reference's arithmetic (alg/smooth.h:39-50) and stencil order.
 //
-// Levels with n >= 64 run as LDS-tiled global kernels:
-//   k_smooth        nu sweeps per launch, temporally blocked (halo 2 nu),
-//                   optionally fused with "x += P e" or "x = P e";
+// Levels with n >= 64 run as tiled global kernels:
+//   k_smooth        nu sweeps per launch, temporally blocked (halo 2 nu) in
+//                   registers, optionally fused with "x += P e" or "x = P e";
 //   k_resid_restr   residual b - L x on a fine tile in LDS, restricted;
 //   k_update_resid  x += e, r = rhs - L x, deterministic RMS, stop test.
 // Levels with n <= 32 run inside one workgroup (k_coarse), all in LDS.
@@ -21,7 +21,6 @@
 
 namespace afm {
 
-constexpr int STX = 64, STY = 16;        // smoother tile
 constexpr int RTX = 32, RTY = 8;         // restriction tile (coarse cells)
 constexpr int CMAXL = 5;                 // finest level handled in LDS (32x32)
 constexpr int CTHREADS = 512;
@@ -44,86 +43,155 @@ __device__ inline T prolong_at(const T* __restrict__ xc, int gx, int gy, int hx,
 }
 
 // ---------------------------------------------------------------- smooth --
-template<typename T, int S, int MODE>
-__global__ __launch_bounds__(256) void k_smooth(const T* __restrict__ xin, const T* __restrict__ b,
-                                                const T* __restrict__ xc, T* __restrict__ xout, int nx, int ny,
-                                                T off, T dg, const int* __restrict__ done)
+// Register-resident red-black smoother: S sweeps (2 S half-sweeps) per launch.
+//
+// A block covers a region of 128 x LY cells (LY = NW * RPW) whose corner is
+// (X0 - H, Y0 - H), H = 2 S, and writes its inner (128 - 2H) x (LY - 2H)
+// cells.  Lane k of wave w holds region columns 2k and 2k+1 of rows
+// RPW*w .. RPW*w + RPW - 1, both x and b, in registers.  A red/black update of
+// the point in column 2k+o reads its other-colour E/W neighbours from its own
+// pair and from lane k-1 or k+1 (a DPP wave shift), its N/S neighbours from
+// the rows above and below; the first and last rows of each wave go through
+// LDS once per half-sweep.  Every update reads only neighbours one cell
+// further out, so after half-sweep ph the cells within H-1-ph of the output
+// are exact and the ring outside may hold anything (including the lane-shift
+// fill at the region's edge columns and the unset LDS rows above and below).
+//
+// The update is the reference's row arithmetic (alg/smooth.h:39-50) in the
+// stored column order (stencil5), with "/ diagonal" as "* (1/diagonal)": the
+// diagonal -4 n^2 is a power of two, so the two round identically.
+// MODE: M_NORMAL x = xin; M_ZERO x = 0; M_PADD x = xin + P xc; M_PSET x = P xc
+// (P: bilinear prolongation from the coarse level, mg/solver.h:71-151).
+template<typename T, int S, int NW, int RPW, int MODE, bool EDGE>
+__device__ inline void smooth_tile(const T* __restrict__ xin, const T* __restrict__ b, const T* __restrict__ xc,
+                                   T* __restrict__ xout, int nx, int ny, T off, T dg, T inv, int X0, int Y0, T* bnd)
 {
-	constexpr int H = 2 * S;
-	constexpr int LX = STX + 2 * H, LY = STY + 2 * H;
-	constexpr int BX = STX + 2 * (H - 1), BY = STY + 2 * (H - 1);
-	__shared__ T xs[LX * LY];
-	__shared__ T bs[BX * BY];
-	if (done && *done) return;
-	const unsigned ntx = nx / STX;
-	const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
-	const int X0 = (tile % ntx) * STX, Y0 = (tile / ntx) * STY;
-	const int tid = threadIdx.x;
-	const int hx = nx >> 1, hy = ny >> 1;
-	// Stage the tile: every global load of this thread is issued before the
-	// first LDS store, so a wave keeps ~14 loads in flight instead of one.
-	constexpr int NXV = (LX * LY + 255) / 256, NBV = (BX * BY + 255) / 256;
-	T xv[NXV], bv[NBV];
+	using T2 = typename Pair<T>::type;
+	constexpr int H = 2 * S, LY = NW * RPW;
+	static_assert(RPW % 2 == 0, "wave rows must start on an even row (fixed colour per register)");
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	const int gxu = X0 - H + 2 * lane;          // first column of the pair, unwrapped (even)
+	const int gx = EDGE ? wrap_once(gxu, nx) : gxu;
+	const int gyb = Y0 - H + RPW * w;           // first row of the wave, unwrapped (even)
+	T xr[RPW][2], br[RPW][2];
+	if (MODE == M_NORMAL || MODE == M_PADD) {
 #pragma unroll
-	for (int k = 0; k < NXV; ++k) {
-		const int p = tid + 256 * k;
-		if (p < LX * LY) {
-			const int lx = p % LX, ly = p / LX;
-			int gx = X0 - H + lx; gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
-			int gy = Y0 - H + ly; gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
-			if (MODE == M_ZERO) xv[k] = T(0);
-			else if (MODE == M_NORMAL) xv[k] = xin[gx + (size_t) nx * gy];
-			else if (MODE == M_PADD) xv[k] = xin[gx + (size_t) nx * gy] + prolong_at(xc, gx, gy, hx, hy);
-			else xv[k] = prolong_at(xc, gx, gy, hx, hy);
+		for (int i = 0; i < RPW; ++i) {
+			const int gy = EDGE ? wrap_once(gyb + i, ny) : gyb + i;
+			const T2 v = *(const T2*) (xin + (size_t) nx * gy + gx);
+			xr[i][0] = v.x; xr[i][1] = v.y;
 		}
 	}
 #pragma unroll
-	for (int k = 0; k < NBV; ++k) {
-		const int p = tid + 256 * k;
-		if (p < BX * BY) {
-			const int lx = p % BX, ly = p / BX;
-			int gx = X0 - (H - 1) + lx; gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
-			int gy = Y0 - (H - 1) + ly; gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
-			bv[k] = b[gx + (size_t) nx * gy];
+	for (int i = 0; i < RPW; ++i) {
+		const int gy = EDGE ? wrap_once(gyb + i, ny) : gyb + i;
+		const T2 v = *(const T2*) (b + (size_t) nx * gy + gx);
+		br[i][0] = v.x; br[i][1] = v.y;
+	}
+	if (MODE == M_ZERO) {
+#pragma unroll
+		for (int i = 0; i < RPW; ++i) xr[i][0] = xr[i][1] = T(0);
+	}
+	if (MODE == M_PADD || MODE == M_PSET) {
+		// coarse rows gyb/2 - 1 .. gyb/2 + RPW/2, columns m-1 .. m+1 (periodic)
+		const int hx = nx >> 1, hy = ny >> 1, m = gxu / 2;
+		int cc[3];
+#pragma unroll
+		for (int j = 0; j < 3; ++j) cc[j] = wrap_once(m - 1 + j, hx);
+		T cw[RPW / 2 + 2][3];
+#pragma unroll
+		for (int r = 0; r < RPW / 2 + 2; ++r) {
+			const size_t row = (size_t) hx * wrap_once(gyb / 2 - 1 + r, hy);
+#pragma unroll
+			for (int j = 0; j < 3; ++j) cw[r][j] = xc[row + cc[j]];
+		}
+#pragma unroll
+		for (int i = 0; i < RPW; ++i) {
+			// even fine row: primary coarse row i/2 + 1, other i/2; odd: (i+1)/2, (i+3)/2
+			const int pr = (i & 1) ? (i + 1) / 2 : i / 2 + 1, orr = (i & 1) ? (i + 3) / 2 : i / 2;
+#pragma unroll
+			for (int o = 0; o < 2; ++o) {
+				// even fine column: primary m, other m-1; odd: primary m, other m+1
+				const int oc = o ? 2 : 0;
+				const T P = cw[pr][1], Sx = cw[pr][oc], Sy = cw[orr][1], Dg = cw[orr][oc];
+				const T p = T(9. / 16.) * P + T(3. / 16.) * (Sx + Sy) + T(1. / 16.) * Dg;
+				xr[i][o] = (MODE == M_PADD) ? xr[i][o] + p : p;
+			}
 		}
 	}
-#pragma unroll
-	for (int k = 0; k < NXV; ++k)
-		if (tid + 256 * k < LX * LY) xs[tid + 256 * k] = xv[k];
-#pragma unroll
-	for (int k = 0; k < NBV; ++k)
-		if (tid + 256 * k < BX * BY) bs[tid + 256 * k] = bv[k];
+	// bnd[(slot * 2 + which) * 64 + lane][col]: slot w+1 holds wave w's first
+	// (which 0) and last (which 1) rows; slots 0 and NW+1 are never written.
+	auto B = [&](int slot, int which, int col) -> T& { return bnd[((slot * 2 + which) * 64 + lane) * 2 + col]; };
+	B(w + 1, 0, 0) = xr[0][0]; B(w + 1, 0, 1) = xr[0][1];
+	B(w + 1, 1, 0) = xr[RPW - 1][0]; B(w + 1, 1, 1) = xr[RPW - 1][1];
 	__syncthreads();
 #pragma unroll
 	for (int ph = 0; ph < 2 * S; ++ph) {
-		const int ring = H - 1 - ph;            // points within `ring` of the tile are valid after this phase
-		const int color = ph & 1;
-		const int RX = STX + 2 * ring, RY = STY + 2 * ring;
-		// only points of `color` are updated: walk the half of the region
-		for (int q = tid; q < (RX * RY + 1) / 2; q += 256) {
-			int ry = (2 * q) / RX;
-			int rx0 = (2 * q) % RX;
-			int lx = H - ring + rx0, ly = H - ring + ry;
-			int gx = X0 - H + lx, gy = Y0 - H + ly;
-			// parity of (gx + gy); shift rx by one when the row starts on the other colour
-			int gxn = gx + (gx < 0 ? nx : 0) - (gx >= nx ? nx : 0);
-			int gyn = gy + (gy < 0 ? ny : 0) - (gy >= ny ? ny : 0);
-			if (((gxn + gyn) & 1) != color) {
-				lx += 1; gx += 1; gxn += 1; if (gxn >= nx) gxn -= nx;
-				if (lx >= H - ring + RX) continue;
+		const int c = ph & 1;                   // colour (gx + gy) & 1 updated now
+		// the cell of colour c in row i of this wave is column o = (c + i) & 1
+		const T S0 = B(w, 1, c);                // row above the wave, column c
+		const T N1 = B(w + 2, 0, 1 - c);        // row below the wave, column 1 - c
+#pragma unroll
+		for (int i = 0; i < RPW; ++i) {
+			const int o = (c + i) & 1;
+			T We, Ea;
+			if (o == 0) { We = lane_shift<DPP_WAVE_SHR1>(xr[i][1]); Ea = xr[i][1]; }
+			else { We = xr[i][0]; Ea = lane_shift<DPP_WAVE_SHL1>(xr[i][0]); }
+			const T Sv = i > 0 ? xr[i - 1][o] : S0;
+			const T Nv = i < RPW - 1 ? xr[i + 1][o] : N1;
+			T v;
+			if (EDGE) {
+				const int px = wrap_once(gxu + o, nx), py = wrap_once(gyb + i, ny);
+				v = stencil5<T, false>(off, dg, T(0), Ea, We, Nv, Sv, px == nx - 1, px == 0, py == ny - 1, py == 0);
+			} else {
+				v = stencil5<T, false>(off, dg, T(0), Ea, We, Nv, Sv, false, false, false, false);
 			}
-			const int li = lx + LX * ly;
-			const int bi = (lx - 1) + BX * (ly - 1);
-			T e = xs[li + 1], w = xs[li - 1], nn = xs[li + LX], s = xs[li - LX];
-			T value = stencil5<T, false>(off, dg, T(0), e, w, nn, s, gxn == nx - 1, gxn == 0, gyn == ny - 1, gyn == 0);
-			xs[li] = (bs[bi] - value) / dg;
+			xr[i][o] = (br[i][o] - v) * inv;
 		}
-		__syncthreads();
+		if (ph < 2 * S - 1) {
+			B(w + 1, 0, c) = xr[0][c];
+			B(w + 1, 1, 1 - c) = xr[RPW - 1][1 - c];
+			__syncthreads();
+		}
 	}
-	for (int p = tid; p < STX * STY; p += 256) {
-		int lx = p % STX, ly = p / STX;
-		xout[(X0 + lx) + (size_t) nx * (Y0 + ly)] = xs[(lx + H) + LX * (ly + H)];
+	if (lane >= H / 2 && lane < 64 - H / 2 && gxu < nx) {
+#pragma unroll
+		for (int i = 0; i < RPW; ++i) {
+			const int ly = RPW * w + i;
+			if (ly >= H && ly < LY - H) {
+				T2 v;
+				v.x = xr[i][0]; v.y = xr[i][1];
+				*(T2*) (xout + (size_t) nx * (gyb + i) + gx) = v;
+			}
+		}
 	}
+}
+
+template<int S> struct SmoothShape;
+template<> struct SmoothShape<1> { static constexpr int NW = 6, RPW = 6; };
+template<> struct SmoothShape<2> { static constexpr int NW = 4, RPW = 10; };
+
+template<int S>
+constexpr int smooth_txo() { return 128 - 4 * S; }
+template<int S>
+constexpr int smooth_tyo() { return SmoothShape<S>::NW * SmoothShape<S>::RPW - 4 * S; }
+
+template<typename T, int S, int MODE>
+__global__ __launch_bounds__(64 * SmoothShape<S>::NW) void k_smooth(const T* __restrict__ xin, const T* __restrict__ b,
+                                                                    const T* __restrict__ xc, T* __restrict__ xout,
+                                                                    int nx, int ny, T off, T dg, T inv,
+                                                                    const int* __restrict__ done)
+{
+	constexpr int NW = SmoothShape<S>::NW, RPW = SmoothShape<S>::RPW, H = 2 * S, LY = NW * RPW;
+	constexpr int TXO = smooth_txo<S>(), TYO = smooth_tyo<S>();
+	__shared__ T bnd[(NW + 2) * 2 * 64 * 2];
+	if (done && *done) return;
+	const int ntx = (nx + TXO - 1) / TXO;
+	const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+	const int X0 = (int) (tile % ntx) * TXO, Y0 = (int) (tile / ntx) * TYO;
+	const bool edge = X0 - H < 0 || X0 - H + 128 > nx || Y0 - H < 0 || Y0 - H + LY > ny;
+	if (edge) smooth_tile<T, S, NW, RPW, MODE, true>(xin, b, xc, xout, nx, ny, off, dg, inv, X0, Y0, bnd);
+	else smooth_tile<T, S, NW, RPW, MODE, false>(xin, b, xc, xout, nx, ny, off, dg, inv, X0, Y0, bnd);
 }
 
 // ------------------------------------------------------ residual+restrict --
@@ -493,9 +561,13 @@ static int smooth_launch(Ctx* c, unsigned l, int mode, int sweeps, const T* xin,
                          const int* done)
 {
 	const LevelBuf& L = c->lv[l];
-	const T off = (T) (double) (L.n * L.n), dg = (T) (-6.0 * (double) (L.n * L.n) + (double) (L.n * L.n) + (double) (L.n * L.n));
-	dim3 g((L.nx / STX) * (L.ny / STY));
-#define AFM_SM(S_, M_) hipLaunchKernelGGL((k_smooth<T, S_, M_>), g, dim3(256), 0, c->stream, xin, b, xc, xout, (int) L.nx, (int) L.ny, off, dg, done)
+	const double n2 = (double) L.n * (double) L.n;
+	const T off = (T) n2, dg = (T) (-6.0 * n2 + n2 + n2), inv = T(1) / dg;   // dg = -4 n^2, a power of two
+	const int nx = (int) L.nx, ny = (int) L.ny;
+#define AFM_SM(S_, M_)                                                                                          \
+	hipLaunchKernelGGL((k_smooth<T, S_, M_>),                                                                   \
+	                   dim3(((nx + smooth_txo<S_>() - 1) / smooth_txo<S_>()) * (ny / smooth_tyo<S_>())),       \
+	                   dim3(64 * SmoothShape<S_>::NW), 0, c->stream, xin, b, xc, xout, nx, ny, off, dg, inv, done)
 	if (sweeps == 1) {
 		switch (mode) { case M_NORMAL: AFM_SM(1, M_NORMAL); break; case M_ZERO: AFM_SM(1, M_ZERO); break;
 		                case M_PADD: AFM_SM(1, M_PADD); break; default: AFM_SM(1, M_PSET); }
