@@ -23,7 +23,7 @@ namespace afm {
 
 constexpr int RTX = 32, RTY = 8;         // restriction tile (coarse cells)
 constexpr int CMAXL = 5;                 // finest level handled in LDS (32x32)
-constexpr int CTHREADS = 512;
+constexpr int CTHREADS = 256;
 
 enum { M_NORMAL = 0, M_ZERO = 1, M_PADD = 2, M_PSET = 3 };
 
@@ -365,168 +365,267 @@ __global__ void k_copy_cvt(const T* __restrict__ src, double* __restrict__ dst, 
 }
 
 // ------------------------------------------------------------- coarse -----
-// Levels 0..Lt (n <= 32) of the unit square, one workgroup, all in LDS.
+// Levels 0..Lt (n <= 32) of the unit square in one workgroup, all in LDS.
+// Level 5 (32x32) runs on all CTHREADS threads with block barriers; levels
+// <= 4 run in wave 0 alone, where LDS operations complete in issue order and
+// no block barrier is needed.  Levels are template parameters, so every loop
+// bound is a constant and nothing spills.
+//
 // Per-point arithmetic is the global kernels': the 9-slot stencil order for
-// n >= 4 and the reference's folded rows (mg/laplacian.h:62-95) at n = 2.
-// Red points are visited directly (no idle lanes), and "x += P e" is fused
-// into the first red half-sweep: red updates read black neighbours as
-// x + P e, black updates never read their own old value.
+// n >= 4, the reference's folded 2x2 rows at n = 2 (mg/laplacian.h:62-95
+// with nx = nz = 2), and "/ diagonal" as "* (1/diagonal)" (the diagonal is
+// -4 n^2, a power of two).  "x += P e" is fused into the first red half-sweep:
+// red updates read black neighbours as x + P e, black updates never read
+// their own old value.
 struct CoarseLDS {
 	double X[1365], B[1365], CP[1365], TMP[1024];
 };
 
-__device__ inline int lvl_off(int l) { return ((1 << (2 * l)) - 1) / 3; }   // sum_{j<l} 4^j
+constexpr int lvl_off(int l) { return ((1 << (2 * l)) - 1) / 3; }   // sum_{j<l} 4^j
 
-// level-1 (2x2) smoothing with the general folded rows
-__device__ void c_smooth_l1(double* x, const double* b, bool zero, int sweeps)
+template<int L> struct CLv { static constexpr int NT = L >= 5 ? CTHREADS : 64; };
+
+template<int NT>
+__device__ inline void csync()
 {
-	const int nx = 2, N = 4;
-	const double off = 4.0;
-	if (zero) {
-		if (threadIdx.x < N) x[threadIdx.x] = 0.0;
+	if constexpr (NT > 64) {
 		__syncthreads();
+	} else {
+		// one wave: order LDS traffic without a block barrier
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 	}
-	for (int s = 0; s < sweeps; ++s)
-		for (int color = 0; color < 2; ++color) {
-			const int i = threadIdx.x;
-			if (i < N && (((i & 1) + (i >> 1)) & 1) == color) {
-				LapRow r;
-				lap_row(i & 1, i >> 1, nx, nx, off, r);
-				double value = 0.0, diagonal = 0.0;
-				for (int k = 0; k < r.cnt; ++k) {
-					if (r.col[k] == i) { diagonal = r.a[k]; continue; }
-					value += r.a[k] * x[r.col[k]];
-				}
-				x[i] = (b[i] - value) / diagonal;
-			}
-			__syncthreads();
-		}
 }
 
-// red-black sweeps at level l >= 2; zero: x starts at 0; xc: x += P xc first
-__device__ void c_smooth(int l, double* x, const double* b, bool zero, const double* xc, int sweeps)
+// red-black sweeps at level L; zero: x starts at 0; xc: x += P xc first
+template<int L>
+__device__ void c_smooth(double* x, const double* b, bool zero, const double* xc, int sweeps)
 {
-	const int nx = 1 << l, msk = nx - 1, hx = nx >> 1, N2 = (nx * nx) >> 1;
-	const double off = (double) ((1u << l) * (1u << l)), dg = -4.0 * off;
-	for (int s = 0; s < sweeps; ++s)
-		for (int color = 0; color < 2; ++color) {
-			const bool first = s == 0 && color == 0;
-			for (int k = threadIdx.x; k < N2; k += CTHREADS) {
-				const int gy = k >> (l - 1), gx = ((k & (hx - 1)) << 1) + ((color + gy) & 1);
-				const int i = gx + (gy << l);
-				double value;
-				if (first && zero) value = 0.0;
-				else {
-					const int ex = (gx + 1) & msk, wx = (gx - 1) & msk, ny_ = (gy + 1) & msk, sy = (gy - 1) & msk;
-					double E = x[ex + (gy << l)], Wv = x[wx + (gy << l)], Nv = x[gx + (ny_ << l)], Sv = x[gx + (sy << l)];
-					if (first && xc) {
-						E = E + prolong_at(xc, ex, gy, hx, hx);
-						Wv = Wv + prolong_at(xc, wx, gy, hx, hx);
-						Nv = Nv + prolong_at(xc, gx, ny_, hx, hx);
-						Sv = Sv + prolong_at(xc, gx, sy, hx, hx);
+	constexpr int NT = CLv<L>::NT;
+	if constexpr (L == 1) {
+		// 2x2 with the folded rows: (x, y) couples to xo = (1-x, y) and
+		// yo = (x, 1-y), weight 2 off = 8, diagonal -16; the stored order puts
+		// xo first in row y = 0 and yo first in row y = 1.
+		const int i = threadIdx.x;
+		for (int s = 0; s < sweeps; ++s)
+			for (int color = 0; color < 2; ++color) {
+				const bool first = s == 0 && color == 0;
+				if (i < 4 && (((i & 1) + (i >> 1)) & 1) == color) {
+					double value;
+					if (first && zero) value = 0.0;
+					else {
+						double xo = x[i ^ 1], yo = x[i ^ 2];
+						if (first && xc) {
+							xo = xo + prolong_at(xc, (i ^ 1) & 1, (i ^ 1) >> 1, 1, 1);
+							yo = yo + prolong_at(xc, (i ^ 2) & 1, (i ^ 2) >> 1, 1, 1);
+						}
+						value = 0.0;
+						if ((i >> 1) == 0) { value += 8.0 * xo; value += 8.0 * yo; }
+						else { value += 8.0 * yo; value += 8.0 * xo; }
 					}
-					value = stencil5<double, false>(off, dg, 0.0, E, Wv, Nv, Sv, gx == msk, gx == 0, gy == msk, gy == 0);
+					x[i] = (b[i] - value) * (1.0 / -16.0);
 				}
-				x[i] = (b[i] - value) / dg;
+				csync<NT>();
 			}
-			__syncthreads();
-		}
+	} else {
+		constexpr int nx = 1 << L, msk = nx - 1, hx = nx >> 1, N2 = (nx * nx) >> 1;
+		constexpr double off = double(nx * nx), dg = -4.0 * off, inv = 1.0 / dg;
+		for (int s = 0; s < sweeps; ++s)
+			for (int color = 0; color < 2; ++color) {
+				const bool first = s == 0 && color == 0;
+#pragma unroll
+				for (int k0 = 0; k0 < N2; k0 += NT) {
+					const int k = k0 + (int) threadIdx.x;
+					if (N2 % NT != 0 && k >= N2) break;
+					const int gy = k >> (L - 1), gx = ((k & (hx - 1)) << 1) + ((color + gy) & 1);
+					const int i = gx + (gy << L);
+					double value;
+					if (first && zero) value = 0.0;
+					else {
+						const int ex = (gx + 1) & msk, wx = (gx - 1) & msk, ny_ = (gy + 1) & msk, sy = (gy - 1) & msk;
+						double E = x[ex + (gy << L)], Wv = x[wx + (gy << L)], Nv = x[gx + (ny_ << L)], Sv = x[gx + (sy << L)];
+						if (first && xc) {
+							E = E + prolong_at(xc, ex, gy, hx, hx);
+							Wv = Wv + prolong_at(xc, wx, gy, hx, hx);
+							Nv = Nv + prolong_at(xc, gx, ny_, hx, hx);
+							Sv = Sv + prolong_at(xc, gx, sy, hx, hx);
+						}
+						value = stencil5<double, false>(off, dg, 0.0, E, Wv, Nv, Sv, gx == msk, gx == 0, gy == msk, gy == 0);
+					}
+					x[i] = (b[i] - value) * inv;
+				}
+				csync<NT>();
+			}
+	}
 }
 
-// dst(level l-1) = R src  (src = b - L x when x != nullptr)
-__device__ void c_restrict(CoarseLDS& S, int l, const double* x, const double* src, double* dst)
+// dst(level L-1) = R src, src = b - L x when x != nullptr (mg/solver.h:153-181)
+template<int L>
+__device__ void c_restrict(CoarseLDS& S, const double* x, const double* src, double* dst)
 {
-	const int nx = 1 << l, N = nx * nx, cn = nx >> 1, msk = nx - 1;
-	const double off = (double) ((1u << l) * (1u << l));
+	constexpr int NT = CLv<L>::NT;
+	constexpr int nx = 1 << L, N = nx * nx, cn = nx >> 1, msk = nx - 1;
+	constexpr double off = double(nx * nx);
 	const double* in = src;
 	if (x) {
-		for (int i = threadIdx.x; i < N; i += CTHREADS) {
-			const int gx = i & msk, gy = i >> l;
+#pragma unroll
+		for (int i0 = 0; i0 < N; i0 += NT) {
+			const int i = i0 + (int) threadIdx.x;
+			if (N % NT != 0 && i >= N) break;
+			const int gx = i & msk, gy = i >> L;
 			double y;
-			if (l >= 2) {
+			if constexpr (L >= 2) {
 				const int ex = (gx + 1) & msk, wx = (gx - 1) & msk, ny_ = (gy + 1) & msk, sy = (gy - 1) & msk;
-				y = stencil5<double, true>(off, -4.0 * off, x[i], x[ex + (gy << l)], x[wx + (gy << l)],
-				                           x[gx + (ny_ << l)], x[gx + (sy << l)], gx == msk, gx == 0, gy == msk, gy == 0);
+				y = stencil5<double, true>(off, -4.0 * off, x[i], x[ex + (gy << L)], x[wx + (gy << L)],
+				                           x[gx + (ny_ << L)], x[gx + (sy << L)], gx == msk, gx == 0, gy == msk, gy == 0);
 			} else {
-				LapRow r;
-				lap_row(gx, gy, nx, nx, off, r);
+				// folded 2x2 rows in stored order: (0,0) d,xo,yo  (1,0) xo,d,yo
+				// (0,1) yo,d,xo  (1,1) yo,xo,d
+				const double c = x[i], xo = x[i ^ 1], yo = x[i ^ 2];
 				y = 0.0;
-				for (int k = 0; k < r.cnt; ++k) y += r.a[k] * x[r.col[k]];
+				if (i == 0) { y += -16.0 * c; y += 8.0 * xo; y += 8.0 * yo; }
+				else if (i == 1) { y += 8.0 * xo; y += -16.0 * c; y += 8.0 * yo; }
+				else if (i == 2) { y += 8.0 * yo; y += -16.0 * c; y += 8.0 * xo; }
+				else { y += 8.0 * yo; y += 8.0 * xo; y += -16.0 * c; }
 			}
 			S.TMP[i] = src[i] - y;
 		}
-		__syncthreads();
+		csync<NT>();
 		in = S.TMP;
 	}
-	for (int c = threadIdx.x; c < cn * cn; c += CTHREADS) {
-		const int xcc = c & (cn - 1), zcc = c >> (l - 1);
+#pragma unroll
+	for (int c0 = 0; c0 < cn * cn; c0 += NT) {
+		const int c = c0 + (int) threadIdx.x;
+		if ((cn * cn) % NT != 0 && c >= cn * cn) break;
+		const int xcc = c & (cn - 1), zcc = c >> (L - 1);
 		const int xcf = 2 * xcc + 1, zcf = 2 * zcc + 1;
 		double o = 0.0;
 #pragma unroll
-		for (int i = 0; i < 16; ++i) {
-			const int dx = (i % 4) - 2, dz = i / 4 - 2;
+		for (int t = 0; t < 16; ++t) {
+			const int dx = (t % 4) - 2, dz = t / 4 - 2;
 			const int xx = (xcf + dx + nx) & msk, zz = (zcf + dz + nx) & msk;
 			const int w = (4 - abs(2 * dx + 1)) * (4 - abs(2 * dz + 1));
-			o += w / 64. * in[xx + (zz << l)];
+			o += w / 64. * in[xx + (zz << L)];
 		}
 		dst[c] = o;
 	}
-	__syncthreads();
+	csync<NT>();
 }
 
-// dst(level l) (+)= P xc(level l-1)
-__device__ void c_prolong(int l, const double* xc, double* dst, bool add)
+// dst(level L) (+)= P xc(level L-1)
+template<int L>
+__device__ void c_prolong(const double* xc, double* dst, bool add)
 {
-	const int nx = 1 << l, N = nx * nx, h = nx >> 1;
-	for (int i = threadIdx.x; i < N; i += CTHREADS) {
-		double v = prolong_at(xc, i & (nx - 1), i >> l, h, h);
+	constexpr int NT = CLv<L>::NT;
+	constexpr int nx = 1 << L, N = nx * nx, h = nx >> 1;
+#pragma unroll
+	for (int i0 = 0; i0 < N; i0 += NT) {
+		const int i = i0 + (int) threadIdx.x;
+		if (N % NT != 0 && i >= N) break;
+		double v = prolong_at(xc, i & (nx - 1), i >> L, h, h);
 		dst[i] = add ? dst[i] + v : v;
 	}
-	__syncthreads();
+	csync<NT>();
 }
 
 // mg/solver.h:264-271 on the 1x1 coarsest grid: mean removal leaves 0 and
 // CG stops at once (the 1x1 laplacian has no stored entries).
-__device__ void c_solve0(CoarseLDS& S)
+__device__ inline void c_solve0(CoarseLDS& S)
 {
 	if (threadIdx.x == 0) {
 		double c = (1.0 * S.B[0]) / 1u;
 		S.B[0] -= 1.0 * c;
 		S.X[0] = S.B[0];
 	}
-	__syncthreads();
+	csync<64>();
 }
 
-__device__ void c_recurse(CoarseLDS& S, int l, int nu1, int nu2)
+// recurse(L, B_L) -> X_L (mg/solver.h:199-216).  L <= 4: wave 0 only;
+// L == 5: every thread.
+template<int L>
+__device__ void c_recurse(CoarseLDS& S, int nu1, int nu2)
 {
-	for (int k = l; k >= 1; --k) {
-		if (k >= 2) c_smooth(k, S.X + lvl_off(k), S.B + lvl_off(k), true, nullptr, nu1);
-		else c_smooth_l1(S.X + lvl_off(k), S.B + lvl_off(k), true, nu1);
-		c_restrict(S, k, S.X + lvl_off(k), S.B + lvl_off(k), S.B + lvl_off(k - 1));
-	}
-	c_solve0(S);
-	for (int k = 1; k <= l; ++k) {
-		if (k >= 2) c_smooth(k, S.X + lvl_off(k), S.B + lvl_off(k), false, S.X + lvl_off(k - 1), nu2);
-		else {
-			c_prolong(k, S.X + lvl_off(k - 1), S.X + lvl_off(k), true);
-			c_smooth_l1(S.X + lvl_off(k), S.B + lvl_off(k), false, nu2);
+	if constexpr (L == 0) {
+		c_solve0(S);
+	} else {
+		c_smooth<L>(S.X + lvl_off(L), S.B + lvl_off(L), true, nullptr, nu1);
+		c_restrict<L>(S, S.X + lvl_off(L), S.B + lvl_off(L), S.B + lvl_off(L - 1));
+		if constexpr (L == 5) {
+			if (threadIdx.x < 64) c_recurse<L - 1>(S, nu1, nu2);
+			__syncthreads();
+		} else {
+			c_recurse<L - 1>(S, nu1, nu2);
+		}
+		if constexpr (L >= 2) {
+			c_smooth<L>(S.X + lvl_off(L), S.B + lvl_off(L), false, S.X + lvl_off(L - 1), nu2);
+		} else {
+			c_prolong<1>(S.X, S.X + lvl_off(1), true);
+			c_smooth<1>(S.X + lvl_off(1), S.B + lvl_off(1), false, nullptr, nu2);
 		}
 	}
 }
 
-__device__ void c_nested(CoarseLDS& S, int l, int nu1, int nu2)
+// level dispatch for wave 0 (levels 0..4)
+__device__ inline void c_recurse_w0(CoarseLDS& S, int l, int nu1, int nu2)
 {
-	for (int k = l; k >= 1; --k) {
-		for (int i = threadIdx.x; i < (1 << (2 * k)); i += CTHREADS) S.CP[lvl_off(k) + i] = S.B[lvl_off(k) + i];
+	switch (l) {
+	case 0: c_recurse<0>(S, nu1, nu2); break;
+	case 1: c_recurse<1>(S, nu1, nu2); break;
+	case 2: c_recurse<2>(S, nu1, nu2); break;
+	case 3: c_recurse<3>(S, nu1, nu2); break;
+	default: c_recurse<4>(S, nu1, nu2); break;
+	}
+}
+
+// nested-iteration step at level k (mg/solver.h:183-197): down-leg copy and
+// restriction, or up-leg recurse(k-1), B_k = P X_{k-1}, smooth against CP_k.
+template<int L>
+__device__ void c_nested_down(CoarseLDS& S)
+{
+	constexpr int NT = CLv<L>::NT;
+	for (int i = threadIdx.x; i < (1 << (2 * L)); i += NT) S.CP[lvl_off(L) + i] = S.B[lvl_off(L) + i];
+	csync<NT>();
+	c_restrict<L>(S, nullptr, S.B + lvl_off(L), S.B + lvl_off(L - 1));
+}
+
+template<int L>
+__device__ void c_nested_up(CoarseLDS& S, int nu1, int nu2)
+{
+	if constexpr (L == 5) {
+		if (threadIdx.x < 64) c_recurse<4>(S, nu1, nu2);
 		__syncthreads();
-		c_restrict(S, k, nullptr, S.B + lvl_off(k), S.B + lvl_off(k - 1));
+	} else {
+		c_recurse<L - 1>(S, nu1, nu2);
 	}
-	for (int k = 1; k <= l; ++k) {
-		c_recurse(S, k - 1, nu1, nu2);
-		c_prolong(k, S.X + lvl_off(k - 1), S.B + lvl_off(k), false);
-		if (k >= 2) c_smooth(k, S.B + lvl_off(k), S.CP + lvl_off(k), false, nullptr, nu2);
-		else c_smooth_l1(S.B + lvl_off(k), S.CP + lvl_off(k), false, nu2);
+	c_prolong<L>(S.X + lvl_off(L - 1), S.B + lvl_off(L), false);
+	c_smooth<L>(S.B + lvl_off(L), S.CP + lvl_off(L), false, nullptr, nu2);
+}
+
+// the whole nested iteration over levels 1..Lt (called by every thread)
+__device__ void c_nested(CoarseLDS& S, int Lt, int nu1, int nu2)
+{
+	if (Lt >= 5) c_nested_down<5>(S);
+	if (threadIdx.x < 64) {
+		for (int k = Lt < 4 ? Lt : 4; k >= 1; --k) {
+			switch (k) {
+			case 1: c_nested_down<1>(S); break;
+			case 2: c_nested_down<2>(S); break;
+			case 3: c_nested_down<3>(S); break;
+			default: c_nested_down<4>(S); break;
+			}
+		}
+		for (int k = 1; k <= (Lt < 4 ? Lt : 4); ++k) {
+			switch (k) {
+			case 1: c_nested_up<1>(S, nu1, nu2); break;
+			case 2: c_nested_up<2>(S, nu1, nu2); break;
+			case 3: c_nested_up<3>(S, nu1, nu2); break;
+			default: c_nested_up<4>(S, nu1, nu2); break;
+			}
+		}
 	}
+	__syncthreads();
+	if (Lt >= 5) c_nested_up<5>(S, nu1, nu2);
 }
 
 // mode 0: recurse(Lt, in) -> out;  1: nested(Lt, in); recurse -> out;
@@ -542,7 +641,11 @@ __global__ __launch_bounds__(CTHREADS) void k_coarse(int mode, int Lt, const TI*
 	__syncthreads();
 	if (mode >= 1) c_nested(S, Lt, nu1, nu2);
 	if (mode <= 1) {
-		c_recurse(S, Lt, nu1, nu2);
+		if (Lt >= 5) c_recurse<5>(S, nu1, nu2);
+		else {
+			if (threadIdx.x < 64) c_recurse_w0(S, Lt, nu1, nu2);
+			__syncthreads();
+		}
 		for (int i = threadIdx.x; i < N; i += CTHREADS) out[i] = (TI) S.X[o + i];
 	} else {
 		for (int i = threadIdx.x; i < N; i += CTHREADS) out[i] = (TI) S.B[o + i];
