@@ -46,29 +46,65 @@ __device__ inline T prolong_at(const T* __restrict__ xc, int gx, int gy, int hx,
 // Register-resident red-black smoother: S sweeps (2 S half-sweeps) per launch.
 //
 // A block covers a region of 128 x LY cells (LY = NW * RPW) whose corner is
-// (X0 - H, Y0 - H), H = 2 S, and writes its inner (128 - 2H) x (LY - 2H)
-// cells.  Lane k of wave w holds region columns 2k and 2k+1 of rows
-// RPW*w .. RPW*w + RPW - 1, both x and b, in registers.  A red/black update of
-// the point in column 2k+o reads its other-colour E/W neighbours from its own
-// pair and from lane k-1 or k+1 (a DPP wave shift), its N/S neighbours from
-// the rows above and below; the first and last rows of each wave go through
-// LDS once per half-sweep.  Every update reads only neighbours one cell
-// further out, so after half-sweep ph the cells within H-1-ph of the output
-// are exact and the ring outside may hold anything (including the lane-shift
-// fill at the region's edge columns and the unset LDS rows above and below).
+// (X0 - H, Y0 - H) and writes its inner (128 - 2H) x (LY - 2H) cells (tiles
+// at the far edges are clipped).  Lane k of wave w holds region columns 2k
+// and 2k+1 of rows RPW*w .. RPW*w + RPW - 1, both x and b, in registers.  A
+// red/black update of the point in column 2k+o reads its other-colour E/W
+// neighbours from its own pair and from lane k-1 or k+1 (a DPP wave shift),
+// its N/S neighbours from the rows above and below; the first and last rows
+// of each wave go through LDS once per half-sweep.  Every update reads only
+// neighbours one cell further out, so after half-sweep ph the cells within
+// H-1-ph of the output are exact and the ring outside may hold anything
+// (including the lane-shift fill at the region's edge columns and the unset
+// LDS rows above and below).
 //
 // The update is the reference's row arithmetic (alg/smooth.h:39-50) in the
 // stored column order (stencil5), with "/ diagonal" as "* (1/diagonal)": the
 // diagonal -4 n^2 is a power of two, so the two round identically.
 // MODE: M_NORMAL x = xin; M_ZERO x = 0; M_PADD x = xin + P xc; M_PSET x = P xc
 // (P: bilinear prolongation from the coarse level, mg/solver.h:71-151).
-template<typename T, int S, int NW, int RPW, int MODE, bool EDGE>
+//
+// RESTRICT (halo H = 2S + 2): the smoothed x is also turned into the
+// residual b - L x on the output ring 1 and restricted with the 16-tap full
+// weighting (mg/solver.h:153-181) into rc, the coarse tile below the output.
+
+// Neighbours E, W, N, S of the cell (row i, column o) of a lane's strip: the
+// pair partner, the adjacent lane's column (DPP), the rows above and below
+// (the strip itself or the boundary rows `up` / `dn` of the adjacent waves).
+template<typename T, int RPW>
+__device__ inline void strip_nbrs(const T (&v)[RPW][2], int i, int o, T up, T dn, T& E, T& W, T& N, T& S)
+{
+	if (o == 0) { W = lane_shift<DPP_WAVE_SHR1>(v[i][1]); E = v[i][1]; }
+	else { W = v[i][0]; E = lane_shift<DPP_WAVE_SHL1>(v[i][0]); }
+	S = i > 0 ? v[i - 1][o] : up;
+	N = i < RPW - 1 ? v[i + 1][o] : dn;
+}
+
+// bnd[(slot * 2 + which) * 64 + lane][col]: slot w+1 holds wave w's first
+// (which 0) and last (which 1) rows; slots 0 and NW+1 are never written.
+template<typename T>
+__device__ inline T& bnd_at(T* bnd, int slot, int which, int lane, int col)
+{
+	return bnd[((slot * 2 + which) * 64 + lane) * 2 + col];
+}
+
+template<typename T, int RPW>
+__device__ inline void bnd_publish(T* bnd, const T (&v)[RPW][2], int w, int lane)
+{
+	bnd_at(bnd, w + 1, 0, lane, 0) = v[0][0]; bnd_at(bnd, w + 1, 0, lane, 1) = v[0][1];
+	bnd_at(bnd, w + 1, 1, lane, 0) = v[RPW - 1][0]; bnd_at(bnd, w + 1, 1, lane, 1) = v[RPW - 1][1];
+}
+
+template<int NW> constexpr int bnd_size() { return (NW + 2) * 2 * 64 * 2; }
+
+template<typename T, int S, int H, int NW, int RPW, int MODE, bool RESTRICT, bool EDGE>
 __device__ inline void smooth_tile(const T* __restrict__ xin, const T* __restrict__ b, const T* __restrict__ xc,
-                                   T* __restrict__ xout, int nx, int ny, T off, T dg, T inv, int X0, int Y0, T* bnd)
+                                   T* __restrict__ xout, T* __restrict__ rc, int nx, int ny, T off, T dg, T inv,
+                                   int X0, int Y0, T* bnd)
 {
 	using T2 = typename Pair<T>::type;
-	constexpr int H = 2 * S, LY = NW * RPW;
-	static_assert(RPW % 2 == 0, "wave rows must start on an even row (fixed colour per register)");
+	constexpr int LY = NW * RPW;
+	static_assert(RPW % 2 == 0 && H % 2 == 0, "strips start on even rows and columns (fixed colour per register)");
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 	const int gxu = X0 - H + 2 * lane;          // first column of the pair, unwrapped (even)
 	const int gx = EDGE ? wrap_once(gxu, nx) : gxu;
@@ -119,26 +155,19 @@ __device__ inline void smooth_tile(const T* __restrict__ xin, const T* __restric
 			}
 		}
 	}
-	// bnd[(slot * 2 + which) * 64 + lane][col]: slot w+1 holds wave w's first
-	// (which 0) and last (which 1) rows; slots 0 and NW+1 are never written.
-	auto B = [&](int slot, int which, int col) -> T& { return bnd[((slot * 2 + which) * 64 + lane) * 2 + col]; };
-	B(w + 1, 0, 0) = xr[0][0]; B(w + 1, 0, 1) = xr[0][1];
-	B(w + 1, 1, 0) = xr[RPW - 1][0]; B(w + 1, 1, 1) = xr[RPW - 1][1];
+	bnd_publish<T, RPW>(bnd, xr, w, lane);
 	__syncthreads();
 #pragma unroll
 	for (int ph = 0; ph < 2 * S; ++ph) {
 		const int c = ph & 1;                   // colour (gx + gy) & 1 updated now
 		// the cell of colour c in row i of this wave is column o = (c + i) & 1
-		const T S0 = B(w, 1, c);                // row above the wave, column c
-		const T N1 = B(w + 2, 0, 1 - c);        // row below the wave, column 1 - c
+		const T up = bnd_at(bnd, w, 1, lane, c);          // row above the wave, column c
+		const T dn = bnd_at(bnd, w + 2, 0, lane, 1 - c);  // row below the wave, column 1 - c
 #pragma unroll
 		for (int i = 0; i < RPW; ++i) {
 			const int o = (c + i) & 1;
-			T We, Ea;
-			if (o == 0) { We = lane_shift<DPP_WAVE_SHR1>(xr[i][1]); Ea = xr[i][1]; }
-			else { We = xr[i][0]; Ea = lane_shift<DPP_WAVE_SHL1>(xr[i][0]); }
-			const T Sv = i > 0 ? xr[i - 1][o] : S0;
-			const T Nv = i < RPW - 1 ? xr[i + 1][o] : N1;
+			T Ea, We, Nv, Sv;
+			strip_nbrs<T, RPW>(xr, i, o, up, dn, Ea, We, Nv, Sv);
 			T v;
 			if (EDGE) {
 				const int px = wrap_once(gxu + o, nx), py = wrap_once(gyb + i, ny);
@@ -149,8 +178,8 @@ __device__ inline void smooth_tile(const T* __restrict__ xin, const T* __restric
 			xr[i][o] = (br[i][o] - v) * inv;
 		}
 		if (ph < 2 * S - 1) {
-			B(w + 1, 0, c) = xr[0][c];
-			B(w + 1, 1, 1 - c) = xr[RPW - 1][1 - c];
+			bnd_at(bnd, w + 1, 0, lane, c) = xr[0][c];
+			bnd_at(bnd, w + 1, 1, lane, 1 - c) = xr[RPW - 1][1 - c];
 			__syncthreads();
 		}
 	}
@@ -158,11 +187,69 @@ __device__ inline void smooth_tile(const T* __restrict__ xin, const T* __restric
 #pragma unroll
 		for (int i = 0; i < RPW; ++i) {
 			const int ly = RPW * w + i;
-			if (ly >= H && ly < LY - H) {
+			if (ly >= H && ly < LY - H && gyb + i < ny) {
 				T2 v;
 				v.x = xr[i][0]; v.y = xr[i][1];
 				*(T2*) (xout + (size_t) nx * (gyb + i) + gx) = v;
 			}
+		}
+	}
+	const int cnx = nx >> 1, cny = ny >> 1;
+	if (RESTRICT) {
+		// residual b - L x (mg/laplacian.h rows in stored order, diagonal
+		// included) on every cell of the strip; exact on the output ring 1
+		T* bx = bnd + bnd_size<NW>();
+		T* brs = bx + bnd_size<NW>();
+		bnd_publish<T, RPW>(bx, xr, w, lane);
+		__syncthreads();
+		T up[2], dn[2];
+#pragma unroll
+		for (int o = 0; o < 2; ++o) { up[o] = bnd_at(bx, w, 1, lane, o); dn[o] = bnd_at(bx, w + 2, 0, lane, o); }
+		T (&rr)[RPW][2] = br;                  // the residual replaces b
+#pragma unroll
+		for (int i = 0; i < RPW; ++i)
+#pragma unroll
+			for (int o = 0; o < 2; ++o) {
+				T Ea, We, Nv, Sv;
+				strip_nbrs<T, RPW>(xr, i, o, up[o], dn[o], Ea, We, Nv, Sv);
+				T Lx;
+				if (EDGE) {
+					const int px = wrap_once(gxu + o, nx), py = wrap_once(gyb + i, ny);
+					Lx = stencil5<T, true>(off, dg, xr[i][o], Ea, We, Nv, Sv, px == nx - 1, px == 0, py == ny - 1, py == 0);
+				} else {
+					Lx = stencil5<T, true>(off, dg, xr[i][o], Ea, We, Nv, Sv, false, false, false, false);
+				}
+				rr[i][o] = br[i][o] - Lx;
+			}
+		bnd_publish<T, RPW>(brs, rr, w, lane);
+		__syncthreads();
+		// coarse cell (j, q) of the region takes fine rows 2q-1 .. 2q+2 and
+		// columns 2j-1 .. 2j+2: lane j-1's second column, this lane's pair,
+		// lane j+1's first column.  Taps in the reference order (x fastest).
+		T ru[2], rd[2];
+#pragma unroll
+		for (int o = 0; o < 2; ++o) { ru[o] = bnd_at(brs, w, 1, lane, o); rd[o] = bnd_at(brs, w + 2, 0, lane, o); }
+		const int j = lane;
+		const int cgx = (X0 - H) / 2 + j;
+		const bool colok = j >= H / 2 && j < 64 - H / 2 && cgx < cnx;
+#pragma unroll
+		for (int qq = 0; qq < RPW / 2; ++qq) {
+			const int q = RPW / 2 * w + qq;     // coarse row of the region
+			T o = T(0);
+#pragma unroll
+			for (int dz = 0; dz < 4; ++dz) {
+				const int i = 2 * qq - 1 + dz;    // fine row of the strip, -1 .. RPW
+				const T c0 = i < 0 ? ru[0] : (i >= RPW ? rd[0] : rr[i < 0 ? 0 : (i >= RPW ? RPW - 1 : i)][0]);
+				const T c1 = i < 0 ? ru[1] : (i >= RPW ? rd[1] : rr[i < 0 ? 0 : (i >= RPW ? RPW - 1 : i)][1]);
+				const T cl = lane_shift<DPP_WAVE_SHR1>(c1), cr = lane_shift<DPP_WAVE_SHL1>(c0);
+				const int wz = 4 - abs(2 * (dz - 2) + 1);
+				o += T((1 * wz) / 64.) * cl;
+				o += T((3 * wz) / 64.) * c0;
+				o += T((3 * wz) / 64.) * c1;
+				o += T((1 * wz) / 64.) * cr;
+			}
+			const int cgy = (Y0 - H) / 2 + q;
+			if (colok && q >= H / 2 && q < LY / 2 - H / 2 && cgy < cny) rc[(size_t) cnx * cgy + cgx] = o;
 		}
 	}
 }
@@ -171,27 +258,28 @@ template<int S> struct SmoothShape;
 template<> struct SmoothShape<1> { static constexpr int NW = 6, RPW = 6; };
 template<> struct SmoothShape<2> { static constexpr int NW = 4, RPW = 10; };
 
-template<int S>
-constexpr int smooth_txo() { return 128 - 4 * S; }
-template<int S>
-constexpr int smooth_tyo() { return SmoothShape<S>::NW * SmoothShape<S>::RPW - 4 * S; }
+// halo of a launch: 2 cells per sweep, 2 more for the residual + restriction
+template<int S, bool RESTRICT> constexpr int smooth_halo() { return 2 * S + (RESTRICT ? 2 : 0); }
+template<int S, bool RESTRICT> constexpr int smooth_txo() { return 128 - 2 * smooth_halo<S, RESTRICT>(); }
+template<int S, bool RESTRICT>
+constexpr int smooth_tyo() { return SmoothShape<S>::NW * SmoothShape<S>::RPW - 2 * smooth_halo<S, RESTRICT>(); }
 
-template<typename T, int S, int MODE>
+template<typename T, int S, int MODE, bool RESTRICT>
 __global__ __launch_bounds__(64 * SmoothShape<S>::NW) void k_smooth(const T* __restrict__ xin, const T* __restrict__ b,
                                                                     const T* __restrict__ xc, T* __restrict__ xout,
-                                                                    int nx, int ny, T off, T dg, T inv,
-                                                                    const int* __restrict__ done)
+                                                                    T* __restrict__ rc, int nx, int ny, T off, T dg,
+                                                                    T inv, const int* __restrict__ done)
 {
-	constexpr int NW = SmoothShape<S>::NW, RPW = SmoothShape<S>::RPW, H = 2 * S, LY = NW * RPW;
-	constexpr int TXO = smooth_txo<S>(), TYO = smooth_tyo<S>();
-	__shared__ T bnd[(NW + 2) * 2 * 64 * 2];
+	constexpr int NW = SmoothShape<S>::NW, RPW = SmoothShape<S>::RPW, LY = NW * RPW;
+	constexpr int H = smooth_halo<S, RESTRICT>(), TXO = smooth_txo<S, RESTRICT>(), TYO = smooth_tyo<S, RESTRICT>();
+	__shared__ T bnd[(RESTRICT ? 3 : 1) * bnd_size<NW>()];
 	if (done && *done) return;
 	const int ntx = (nx + TXO - 1) / TXO;
 	const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
 	const int X0 = (int) (tile % ntx) * TXO, Y0 = (int) (tile / ntx) * TYO;
 	const bool edge = X0 - H < 0 || X0 - H + 128 > nx || Y0 - H < 0 || Y0 - H + LY > ny;
-	if (edge) smooth_tile<T, S, NW, RPW, MODE, true>(xin, b, xc, xout, nx, ny, off, dg, inv, X0, Y0, bnd);
-	else smooth_tile<T, S, NW, RPW, MODE, false>(xin, b, xc, xout, nx, ny, off, dg, inv, X0, Y0, bnd);
+	if (edge) smooth_tile<T, S, H, NW, RPW, MODE, RESTRICT, true>(xin, b, xc, xout, rc, nx, ny, off, dg, inv, X0, Y0, bnd);
+	else smooth_tile<T, S, H, NW, RPW, MODE, RESTRICT, false>(xin, b, xc, xout, rc, nx, ny, off, dg, inv, X0, Y0, bnd);
 }
 
 // ------------------------------------------------------ residual+restrict --
@@ -659,24 +747,37 @@ static inline unsigned red_blocks(size_t N)
 	return (unsigned) (b < (size_t) MAX_RED_BLOCKS ? b : (size_t) MAX_RED_BLOCKS);
 }
 
+template<int S, bool RESTRICT>
+static inline unsigned smooth_blocks(int nx, int ny)
+{
+	constexpr int TXO = smooth_txo<S, RESTRICT>(), TYO = smooth_tyo<S, RESTRICT>();
+	return (unsigned) (((nx + TXO - 1) / TXO) * ((ny + TYO - 1) / TYO));
+}
+
+// sweeps red-black sweeps at level l, x0 given by `mode`; with rc != nullptr
+// also rc = R (b - L x) on level l-1 (mode must be M_ZERO then).
 template<typename T>
 static int smooth_launch(Ctx* c, unsigned l, int mode, int sweeps, const T* xin, const T* b, const T* xc, T* xout,
-                         const int* done)
+                         const int* done, T* rc = nullptr)
 {
 	const LevelBuf& L = c->lv[l];
 	const double n2 = (double) L.n * (double) L.n;
 	const T off = (T) n2, dg = (T) (-6.0 * n2 + n2 + n2), inv = T(1) / dg;   // dg = -4 n^2, a power of two
 	const int nx = (int) L.nx, ny = (int) L.ny;
-#define AFM_SM(S_, M_)                                                                                          \
-	hipLaunchKernelGGL((k_smooth<T, S_, M_>),                                                                   \
-	                   dim3(((nx + smooth_txo<S_>() - 1) / smooth_txo<S_>()) * (ny / smooth_tyo<S_>())),       \
-	                   dim3(64 * SmoothShape<S_>::NW), 0, c->stream, xin, b, xc, xout, nx, ny, off, dg, inv, done)
-	if (sweeps == 1) {
-		switch (mode) { case M_NORMAL: AFM_SM(1, M_NORMAL); break; case M_ZERO: AFM_SM(1, M_ZERO); break;
-		                case M_PADD: AFM_SM(1, M_PADD); break; default: AFM_SM(1, M_PSET); }
+#define AFM_SM(S_, M_, R_)                                                                                     \
+	hipLaunchKernelGGL((k_smooth<T, S_, M_, R_>), dim3(smooth_blocks<S_, R_>(nx, ny)),                        \
+	                   dim3(64 * SmoothShape<S_>::NW), 0, c->stream, xin, b, xc, xout, rc, nx, ny, off, dg, inv, done)
+	if (rc) {
+		if (mode != M_ZERO) { set_error("fused restriction needs a zero start"); return -1; }
+		if (sweeps == 1) AFM_SM(1, M_ZERO, true);
+		else if (sweeps == 2) AFM_SM(2, M_ZERO, true);
+		else { set_error("smoother supports 1 or 2 sweeps per launch"); return -1; }
+	} else if (sweeps == 1) {
+		switch (mode) { case M_NORMAL: AFM_SM(1, M_NORMAL, false); break; case M_ZERO: AFM_SM(1, M_ZERO, false); break;
+		                case M_PADD: AFM_SM(1, M_PADD, false); break; default: AFM_SM(1, M_PSET, false); }
 	} else if (sweeps == 2) {
-		switch (mode) { case M_NORMAL: AFM_SM(2, M_NORMAL); break; case M_ZERO: AFM_SM(2, M_ZERO); break;
-		                case M_PADD: AFM_SM(2, M_PADD); break; default: AFM_SM(2, M_PSET); }
+		switch (mode) { case M_NORMAL: AFM_SM(2, M_NORMAL, false); break; case M_ZERO: AFM_SM(2, M_ZERO, false); break;
+		                case M_PADD: AFM_SM(2, M_PADD, false); break; default: AFM_SM(2, M_PSET, false); }
 	} else {
 		set_error("smoother supports 1 or 2 sweeps per launch");
 		return -1;
@@ -753,8 +854,9 @@ static int vcycle(Ctx* c, unsigned l, const T* rhs_l, int nu1, int nu2, const in
 	if (l <= lc) return coarse_launch<T>(c, 0, (int) l, rhs_l, lvX<T>(c->lv[l], 1), nu1, nu2, done);
 	for (unsigned k = l; k > lc; --k) {
 		const T* bk = (k == l) ? rhs_l : lvB<T>(c->lv[k]);
-		AFM_TRY(smooth_n<T>(c, k, M_ZERO, nu1, nullptr, bk, nullptr, lvX<T>(c->lv[k], 0), lvX<T>(c->lv[k], 1), done));
-		AFM_TRY(resid_restrict_launch<T>(c, k, true, lvX<T>(c->lv[k], 0), bk, lvB<T>(c->lv[k - 1]), nullptr, done));
+		// pre-smoothing from zero, residual and restriction in one launch
+		AFM_TRY(smooth_launch<T>(c, k, M_ZERO, nu1, nullptr, bk, nullptr, lvX<T>(c->lv[k], 0), done,
+		                         lvB<T>(c->lv[k - 1])));
 	}
 	AFM_TRY(coarse_launch<T>(c, 0, (int) lc, lvB<T>(c->lv[lc]), lvX<T>(c->lv[lc], 1), nu1, nu2, done));
 	for (unsigned k = lc + 1; k <= l; ++k) {
