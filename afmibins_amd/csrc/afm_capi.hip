@@ -97,8 +97,12 @@ int afm_ctx_create(const afm_geom* g, int device, afm_ctx** out)
 	if (hipHostMalloc((void**) &c->hctl, sizeof(SolveCtl) * NCTL, hipHostMallocDefault) != hipSuccess) { set_error("hipHostMalloc failed"); return fail(-3); }
 	if ((rc = c->alloc((void**) &c->partials, sizeof(double) * MAX_RED_BLOCKS * 4))) return fail(rc);
 	if (hipMemsetAsync(c->ctl, 0, sizeof(SolveCtl) * NCTL, c->stream) != hipSuccess) { set_error("memset failed"); return fail(-3); }
-	c->ntx = (int) ((c->nx + 15) / 16); c->nty = (int) ((c->ny + 15) / 16);
-	if ((rc = c->alloc((void**) &c->bins, sizeof(int) * (6 * c->ntx * c->nty + 1)))) return fail(rc);
+	c->sbx = (int) (c->nx / 4); c->sby = (int) (c->ny / 4);
+	{
+		// counts, offsets (+1), cursors and up to 1024 scan-chunk sums for both components
+		const size_t nb = 2 * (size_t) c->sbx * c->sby;
+		if ((rc = c->alloc((void**) &c->bins, sizeof(int) * (3 * nb + 1 + 1024)))) return fail(rc);
+	}
 	if ((rc = mg_setup(c))) return fail(rc);
 	if (hipStreamSynchronize(c->stream) != hipSuccess) { set_error("sync failed"); return fail(-3); }
 	*out = c;
@@ -146,7 +150,7 @@ int afm_sync(afm_ctx* c)
 	if (err) {
 		int zero = 0;
 		hipMemcpy(&c->ctl[3].err, &zero, sizeof(int), hipMemcpyHostToDevice);
-		set_error("spreading: more than 4096 candidate markers around one 16x16 tile");
+		set_error("a kernel reported an error");
 		return -1;
 	}
 	return 0;

@@ -6,19 +6,20 @@
 // against the CPU restatement.
 //
 // Spreading (ib/grid.h:68-84, grid targets): the reference gathers, for
-// every grid point, the markers found by a kd-tree box query.  Here markers
-// are binned by the 16x16 tile of their window's base cell (integer
-// counting sort), and one workgroup per tile collects the candidates of the
-// 3x3 neighbouring bins, sorts them by marker index in LDS and lets each lane
-// (one grid cell) sum its terms (w * ds) * F in ascending marker order.
-// No float atomics: the result is identical run to run and equals a
-// sequential scatter over markers in index order.
+// every grid point, the markers found by a kd-tree box query.  Here every
+// (marker, component) is binned by the 4x4 block of its window's base cell:
+// counts, a chunked exclusive scan, an unordered fill, then each bin's short
+// list is sorted by marker index.  One lane per grid cell then merges the
+// (at most 4) bins whose base cells can reach it and sums its terms
+// (w * ds) * F in ascending marker order.  No float atomics and no capacity
+// limit: the result is identical run to run and equals a sequential scatter
+// over markers in index order.
 #include "afm_device.h"
 
 namespace afm {
 
-constexpr int TS = 16;            // spread tile edge (cells)
-constexpr int KMAX = 4096;        // candidate capacity per tile
+constexpr int SB = 4;             // spread bin edge (window base cells)
+constexpr int SCAN_BLOCKS = 1024; // at most this many scan chunks
 
 __global__ void k_delta(const double* __restrict__ r, long count, unsigned n, double* __restrict__ out)
 {
@@ -84,7 +85,7 @@ __global__ void k_ib_move_full(const double* __restrict__ uo, const double* __re
 __global__ void k_spread_prep(int comp_mask, const double* __restrict__ Fx, const double* __restrict__ Fy,
                               const double* __restrict__ ds, const double* __restrict__ X,
                               const double* __restrict__ R, const double* __restrict__ kk, long m,
-                              unsigned n, unsigned W, unsigned D, unsigned nx, unsigned ny, int ntx, int nty,
+                              unsigned n, unsigned W, unsigned D, unsigned nx, unsigned ny, int sbx, int nb1,
                               MarkerRec* __restrict__ rec, int* __restrict__ counts)
 {
 	long k = blockIdx.x * (long) blockDim.x + threadIdx.x;
@@ -98,37 +99,73 @@ __global__ void k_spread_prep(int comp_mask, const double* __restrict__ Fx, cons
 	else F = comp == 0 ? Fx[k] : Fy[k];
 	r.ds = ds[k];
 	r.F = F;
-	int hx = r.ix0 + 1; if (hx >= (int) nx) hx -= nx;
-	int hy = r.iy0 + 1; if (hy >= (int) ny) hy -= ny;
-	r.tile = hx / TS + ntx * (hy / TS);
+	r.tile = r.ix0 / SB + sbx * (r.iy0 / SB);
 	r.pad = 0;
 	rec[comp * m + k] = r;
-	atomicAdd(&counts[comp * ntx * nty + r.tile], 1);
+	atomicAdd(&counts[comp * nb1 + r.tile], 1);
 }
 
-// Exclusive scan of the bin counts of both components (one block of 1024):
-// per-thread serial chunk sums, a Hillis-Steele scan of the chunk sums, then
-// the per-thread chunk scan.
-__global__ __launch_bounds__(1024) void k_spread_scan(const int* __restrict__ counts, int* __restrict__ offs,
-                                                      int* __restrict__ cursor, int nbins)
+// Exclusive scan of the bin counts in two launches: chunk sums (block c
+// reduces bins [c C, (c+1) C), C = 256 * items), then block c adds the sums
+// of the chunks before it and scans its own chunk.  Integer sums, so the
+// result does not depend on the order of the additions.
+__device__ inline int block_int_sum(int v, int* red)
 {
-	__shared__ int part[1024];
-	const int tid = threadIdx.x, nt = blockDim.x;
-	const int per = (nbins + nt - 1) / nt;
-	const int lo = tid * per, hi = min(nbins, lo + per);
-	int s = 0;
-	for (int i = lo; i < hi; ++i) s += counts[i];
-	part[tid] = s;
+	const int tid = threadIdx.x;
+	red[tid] = v;
 	__syncthreads();
-	for (int d = 1; d < nt; d <<= 1) {
-		int v = tid >= d ? part[tid - d] : 0;
-		__syncthreads();
-		part[tid] += v;
+	for (int d = 128; d > 0; d >>= 1) {
+		if (tid < d) red[tid] += red[tid + d];
 		__syncthreads();
 	}
-	int acc = part[tid] - s;                  // exclusive prefix of this chunk
-	for (int i = lo; i < hi; ++i) { offs[i] = acc; cursor[i] = 0; acc += counts[i]; }
-	if (tid == nt - 1) offs[nbins] = part[nt - 1];
+	const int s = red[0];
+	__syncthreads();
+	return s;
+}
+
+__global__ __launch_bounds__(256) void k_spread_chunk(const int* __restrict__ counts, int* __restrict__ chunk,
+                                                      int nbins, int items)
+{
+	__shared__ int red[256];
+	const long lo = ((long) blockIdx.x * 256 + threadIdx.x) * items;
+	int s = 0;
+	for (int j = 0; j < items; ++j)
+		if (lo + j < nbins) s += counts[lo + j];
+	s = block_int_sum(s, red);
+	if (threadIdx.x == 0) chunk[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_spread_scan(const int* __restrict__ counts, const int* __restrict__ chunk,
+                                                     int* __restrict__ offs, int* __restrict__ cursor, int nbins,
+                                                     int items)
+{
+	__shared__ int red[256];
+	const int c = blockIdx.x, tid = threadIdx.x;
+	int base = 0;
+	for (int q = tid; q < c; q += 256) base += chunk[q];
+	base = block_int_sum(base, red);
+	const long lo = ((long) c * 256 + tid) * items;
+	int s = 0;
+	for (int j = 0; j < items; ++j)
+		if (lo + j < nbins) s += counts[lo + j];
+	red[tid] = s;
+	__syncthreads();
+	for (int d = 1; d < 256; d <<= 1) {
+		const int t = tid >= d ? red[tid - d] : 0;
+		__syncthreads();
+		red[tid] += t;
+		__syncthreads();
+	}
+	int acc = base + red[tid] - s;
+	for (int j = 0; j < items; ++j)
+		if (lo + j < nbins) {
+			const int v = counts[lo + j];
+			offs[lo + j] = acc;
+			cursor[lo + j] = 0;
+			acc += v;
+		}
+	// the lane holding the last bin closes the table
+	if (lo <= nbins - 1 && nbins - 1 < lo + items) offs[nbins] = acc;
 }
 
 __global__ void k_spread_fill(int comp_mask, const MarkerRec* __restrict__ rec, long m, int nb1,
@@ -142,80 +179,64 @@ __global__ void k_spread_fill(int comp_mask, const MarkerRec* __restrict__ rec, 
 	list[offs[b] + slot] = (int) k;
 }
 
-// One workgroup (TS x TS lanes) per (tile, component).  The distinct
-// neighbouring bins and their sizes are computed redundantly by every lane,
-// so empty tiles (the common case) write their zeros without a barrier.
-__global__ __launch_bounds__(256) void k_spread_tile(int comp_mask, const MarkerRec* __restrict__ rec, long m,
-                                                     const int* __restrict__ offs, const int* __restrict__ list,
-                                                     unsigned nx, unsigned ny, int ntx, int nty,
-                                                     double* __restrict__ fx, double* __restrict__ fy,
-                                                     int* __restrict__ err, int accumulate)
+// Each bin's list in ascending marker index (insertion sort; bins hold a few
+// markers).
+__global__ void k_spread_sort(const int* __restrict__ offs, int* __restrict__ list, int nbins)
 {
-	__shared__ int cand[KMAX];
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= nbins) return;
+	const int lo = offs[b], hi = offs[b + 1];
+	for (int i = lo + 1; i < hi; ++i) {
+		const int v = list[i];
+		int j = i - 1;
+		while (j >= lo && list[j] > v) { list[j + 1] = list[j]; --j; }
+		list[j + 1] = v;
+	}
+}
+
+// One lane per grid cell (blocks of 64 x 4 cells, blockIdx.y = component):
+// the markers whose window base lies in [gx-3, gx] x [gy-3, gy] (periodic)
+// are in at most 2 x 2 bins; their sorted lists are merged so that the terms
+// are summed in ascending marker index.
+__global__ __launch_bounds__(256) void k_spread_gather(int comp_mask, const MarkerRec* __restrict__ rec, long m,
+                                                       const int* __restrict__ offs, const int* __restrict__ list,
+                                                       unsigned nx, unsigned ny, int sbx, int sby, int nb1,
+                                                       double* __restrict__ fx, double* __restrict__ fy, int accumulate)
+{
 	const int comp = blockIdx.y;
 	if (!((comp_mask >> comp) & 1)) return;
-	const int nb1 = ntx * nty;
+	const int tpr = ((int) nx + 63) / 64;
 	const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
-	const int tx = tile % ntx, ty = tile / ntx;
-	const int tid = threadIdx.x;
-	const int gx = tx * TS + tid % TS, gy = ty * TS + tid / TS;
+	const int gx = (int) (tile % tpr) * 64 + (threadIdx.x & 63);
+	const int gy = (int) (tile / tpr) * 4 + (threadIdx.x >> 6);
+	if (gx >= (int) nx || gy >= (int) ny) return;
 	double* f = comp == 0 ? fx : fy;
-	// distinct neighbouring bins (periodic; fewer than 3 tiles per axis
-	// would otherwise visit a bin twice)
-	int bxs[3], bys[3], nbx = 0, nby = 0;
-	for (int d = -1; d <= 1; ++d) {
-		int b = (tx + d + ntx) % ntx;
-		bool dup = false;
-		for (int j = 0; j < nbx; ++j) dup |= bxs[j] == b;
-		if (!dup) bxs[nbx++] = b;
-		b = (ty + d + nty) % nty;
-		dup = false;
-		for (int j = 0; j < nby; ++j) dup |= bys[j] == b;
-		if (!dup) bys[nby++] = b;
-	}
-	int o0[9], cnt[9], nb = 0, K = 0;
+	int bx[2], by[2], nbx = 1, nby = 1;
+	bx[0] = (gx >= 3 ? gx - 3 : gx - 3 + (int) nx) / SB;
+	by[0] = (gy >= 3 ? gy - 3 : gy - 3 + (int) ny) / SB;
+	if (gx / SB != bx[0]) bx[nbx++] = gx / SB;
+	if (gy / SB != by[0]) by[nby++] = gy / SB;
+	int head[4], end[4], nl = 0;
 	for (int j = 0; j < nby; ++j)
 		for (int i = 0; i < nbx; ++i) {
-			const int b = comp * nb1 + bxs[i] + ntx * bys[j];
-			o0[nb] = offs[b];
-			cnt[nb] = offs[b + 1] - o0[nb];
-			K += cnt[nb];
-			++nb;
+			const int b = comp * nb1 + bx[i] + sbx * by[j];
+			head[nl] = offs[b];
+			end[nl] = offs[b + 1];
+			++nl;
 		}
-	if (K == 0) {
-		if (!accumulate && gx < (int) nx && gy < (int) ny) f[gx + (size_t) nx * gy] = 0.0;
-		return;
-	}
-	if (K > KMAX) {
-		if (tid == 0) atomicOr(err, 1);
-		K = KMAX;
-	}
-	for (int t = tid; t < K; t += blockDim.x) {
-		int q = t, j = 0;
-		while (q >= cnt[j]) { q -= cnt[j]; ++j; }
-		cand[t] = list[o0[j] + q];
-	}
-	// bitonic sort of the candidate marker indices (ascending)
-	int P = 1;
-	while (P < K) P <<= 1;
-	for (int t = K + tid; t < P; t += blockDim.x) cand[t] = 0x7fffffff;
-	__syncthreads();
-	for (int size = 2; size <= P; size <<= 1)
-		for (int stride = size >> 1; stride > 0; stride >>= 1) {
-			for (int t = tid; t < P / 2; t += blockDim.x) {
-				int lo = 2 * stride * (t / stride) + (t % stride);
-				int hi = lo + stride;
-				bool up = ((lo & size) == 0);
-				int a = cand[lo], b = cand[hi];
-				if ((a > b) == up) { cand[lo] = b; cand[hi] = a; }
-			}
-			__syncthreads();
-		}
-	if (gx >= (int) nx || gy >= (int) ny) return;
-	double acc = 0.0;
+	(void) sby;
 	const MarkerRec* rc = rec + comp * m;
-	for (int j = 0; j < K; ++j) {
-		const MarkerRec& r = rc[cand[j]];
+	double acc = 0.0;
+	for (;;) {
+		int best = -1, bk = 0x7fffffff;
+		for (int l = 0; l < nl; ++l)
+			if (head[l] < end[l]) {
+				const int k = list[head[l]];
+				if (k < bk) { bk = k; best = l; }
+			}
+		if (best < 0) break;
+		++head[best];
+		const MarkerRec& r = rc[bk];
 		int a = gx - r.ix0; if (a < 0) a += nx;
 		int b = gy - r.iy0; if (b < 0) b += ny;
 		if (a < 4 && b < 4) {
@@ -279,27 +300,35 @@ int ensure_markers(Ctx* c, long m)
 int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, const double* ds, const double* X,
                   const double* R, const double* k, long m, double* fx, double* fy, int accumulate)
 {
-	const int nb1 = c->ntx * c->nty, nb = 2 * nb1;
+	const int nb1 = c->sbx * c->sby, nb = 2 * nb1;
+	// chunk of 256 * items bins per scan block, at most SCAN_BLOCKS blocks
+	const int items = (int) (((long) nb + 256L * SCAN_BLOCKS - 1) / (256L * SCAN_BLOCKS));
+	const int nchunk = (int) (((long) nb + 256L * items - 1) / (256L * items));
 	int* counts = c->bins;
-	int* offs = c->bins + nb;          // nb + 1 entries
-	int* cursor = c->bins + 2 * nb + 1;
+	int* offs = counts + nb;           // nb + 1 entries
+	int* cursor = offs + nb + 1;
+	int* chunk = cursor + nb;          // nchunk entries
 	if (m > 0) {
 		AFM_TRY(ensure_markers(c, m));
 		AFM_HIP(hipMemsetAsync(counts, 0, sizeof(int) * nb, c->stream));
 		dim3 g((m + 255) / 256, 2);
 		hipLaunchKernelGGL(k_spread_prep, g, dim3(256), 0, c->stream, comp_mask, Fx, Fy, ds, X, R, k, m, c->n, c->W,
-		                   c->D, c->nx, c->ny, c->ntx, c->nty, c->rec, counts);
+		                   c->D, c->nx, c->ny, c->sbx, nb1, c->rec, counts);
 		AFM_LAUNCH_CHECK();
-		hipLaunchKernelGGL(k_spread_scan, dim3(1), dim3(1024), 0, c->stream, counts, offs, cursor, nb);
+		hipLaunchKernelGGL(k_spread_chunk, dim3(nchunk), dim3(256), 0, c->stream, counts, chunk, nb, items);
+		hipLaunchKernelGGL(k_spread_scan, dim3(nchunk), dim3(256), 0, c->stream, counts, chunk, offs, cursor, nb, items);
 		AFM_LAUNCH_CHECK();
 		hipLaunchKernelGGL(k_spread_fill, g, dim3(256), 0, c->stream, comp_mask, c->rec, m, nb1, offs, cursor,
 		                   c->bin_list);
 		AFM_LAUNCH_CHECK();
+		hipLaunchKernelGGL(k_spread_sort, dim3((nb + 255) / 256), dim3(256), 0, c->stream, offs, c->bin_list, nb);
+		AFM_LAUNCH_CHECK();
 	} else {
 		AFM_HIP(hipMemsetAsync(offs, 0, sizeof(int) * (nb + 1), c->stream));
 	}
-	hipLaunchKernelGGL(k_spread_tile, dim3(nb1, 2), dim3(256), 0, c->stream, comp_mask, c->rec, m, offs,
-	                   c->bin_list, c->nx, c->ny, c->ntx, c->nty, fx, fy, &c->ctl[3].err, accumulate);
+	const unsigned tiles = (unsigned) (((c->nx + 63) / 64) * ((c->ny + 3) / 4));
+	hipLaunchKernelGGL(k_spread_gather, dim3(tiles, 2), dim3(256), 0, c->stream, comp_mask, c->rec, m, offs,
+	                   c->bin_list, c->nx, c->ny, c->sbx, c->sby, nb1, fx, fy, accumulate);
 	AFM_LAUNCH_CHECK();
 	return 0;
 }

@@ -52,7 +52,7 @@ struct SolveCtl {
 	int count;        // completed iterations
 	int done;         // stop test has fired
 	int max_iter;
-	int err;          // kernel-detected error (e.g. spread bin overflow)
+	int err;          // kernel-detected error flag
 	unsigned ticket;  // last-block counter of the fused reductions
 	int pad[3];
 };
@@ -90,10 +90,10 @@ struct Ctx {
 	double* partials = nullptr;        // [MAX_RED_BLOCKS * 4]
 	// spreading
 	MarkerRec* rec = nullptr;          // [2 * mcap]
-	int* bins = nullptr;               // counts | offsets | cursors, 2 comps
+	int* bins = nullptr;               // counts | offsets | cursors | chunk sums, 2 comps
 	int* bin_list = nullptr;           // [2 * mcap]
 	long mcap = 0;
-	int ntx = 0, nty = 0;
+	int sbx = 0, sby = 0;              // spread bins per axis (4x4 base cells each)
 	int last_mg_cycles = 0;            // predicted batch size for the next solve
 	int last_cg_iters = 0;
 	std::vector<void*> owned;

@@ -82,6 +82,35 @@ def test_spread_clustered_markers(ctx_of, oracle, dev):
     ctx.sync()
 
 
+@pytest.mark.parametrize("n", [8, 16, 32])
+def test_spread_small_grids_and_seams(ctx_of, oracle, dev, n):
+    # two to eight bins per axis: the periodic bin pairs of a cell must not
+    # repeat or miss a bin; markers sit on and next to the seams
+    ctx = ctx_of(n)
+    rng = np.random.default_rng(100 + n)
+    X = rng.uniform(0.0, 1.0, (400, 2))
+    X[:6] = [[0.0, 0.0], [1.0 - 1e-12, 0.5], [0.5, 1.0 - 1e-12], [0.25 / n, 0.75 / n], [1.0 - 0.3 / n, 0.1 / n],
+             [0.5 / n, 1.0 - 0.5 / n]]
+    F, ds = rng.standard_normal(400), rng.uniform(0.5, 1.5, 400) / n
+    for comp in (0, 1):
+        got = H(ctx.spread(comp, T(F, dev), T(ds, dev), T(X, dev)))
+        assert np.array_equal(got, oracle.spread2(comp, F, ds, X, n, n, n, mode=1)), comp
+    ctx.sync()
+
+
+def test_spread_one_crowded_bin(ctx_of, oracle, dev):
+    # 2000 markers inside one 4x4 block of window bases: one long bin list
+    n = 128
+    ctx = ctx_of(n)
+    rng = np.random.default_rng(9)
+    X = (40.0 + rng.uniform(0.0, 2.0, (2000, 2))) / n
+    F, ds = rng.standard_normal(2000), np.full(2000, 1.0 / n)
+    for comp in (0, 1):
+        got = H(ctx.spread(comp, T(F, dev), T(ds, dev), T(X, dev)))
+        assert np.array_equal(got, oracle.spread2(comp, F, ds, X, n, n, n, mode=1)), comp
+    ctx.sync()
+
+
 def test_spread_empty(ctx_of, dev):
     ctx = ctx_of(64)
     e = torch.empty((0, 2), dtype=torch.float64, device=dev)
