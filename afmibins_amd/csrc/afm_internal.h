@@ -83,8 +83,9 @@ struct Ctx {
 	unsigned n = 0, nx = 0, ny = 0, W = 1, D = 1, levels = 0, lc = 0;
 	size_t N = 0;
 	std::vector<LevelBuf> lv;          // by level, 0 = coarsest
-	double *XSa = nullptr, *XSb = nullptr, *RES = nullptr;
-	float* RES32 = nullptr;
+	double *XSa = nullptr, *XSb = nullptr, *RES = nullptr, *RESb = nullptr;
+	float *RES32 = nullptr, *RES32b = nullptr;
+	double* upartials = nullptr;       // per-block sums of the multigrid update
 	SolveCtl* ctl = nullptr;           // device [NCTL]
 	SolveCtl* hctl = nullptr;          // pinned host mirror
 	double* partials = nullptr;        // [MAX_RED_BLOCKS * 4]

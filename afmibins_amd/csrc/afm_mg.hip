@@ -64,21 +64,13 @@ __device__ inline T prolong_at(const T* __restrict__ xc, int gx, int gy, int hx,
 // MODE: M_NORMAL x = xin; M_ZERO x = 0; M_PADD x = xin + P xc; M_PSET x = P xc
 // (P: bilinear prolongation from the coarse level, mg/solver.h:71-151).
 //
-// RESTRICT (halo H = 2S + 2): the smoothed x is also turned into the
-// residual b - L x on the output ring 1 and restricted with the 16-tap full
-// weighting (mg/solver.h:153-181) into rc, the coarse tile below the output.
-
-// Neighbours E, W, N, S of the cell (row i, column o) of a lane's strip: the
-// pair partner, the adjacent lane's column (DPP), the rows above and below
-// (the strip itself or the boundary rows `up` / `dn` of the adjacent waves).
-template<typename T, int RPW>
-__device__ inline void strip_nbrs(const T (&v)[RPW][2], int i, int o, T up, T dn, T& E, T& W, T& N, T& S)
-{
-	if (o == 0) { W = lane_shift<DPP_WAVE_SHR1>(v[i][1]); E = v[i][1]; }
-	else { W = v[i][0]; E = lane_shift<DPP_WAVE_SHL1>(v[i][0]); }
-	S = i > 0 ? v[i - 1][o] : up;
-	N = i < RPW - 1 ? v[i + 1][o] : dn;
-}
+// Epilogues (halo H = 2S + 2, so x is exact on the output ring 2):
+//   RESTRICT  the residual b - L x on the output ring 1, restricted with the
+//             16-tap full weighting (mg/solver.h:153-181) into the coarse tile;
+//   UPDATE    (finest level, after the V-cycle) xs' = xs + x and the outer
+//             residual r = rhs - L xs' in FP64 on the output tile, with the
+//             block's sum of r^2 (mg/solver.h:226-233).
+enum { EPI_STORE = 0, EPI_RESTRICT = 1, EPI_UPDATE = 2 };
 
 // bnd[(slot * 2 + which) * 64 + lane][col]: slot w+1 holds wave w's first
 // (which 0) and last (which 1) rows; slots 0 and NW+1 are never written.
@@ -88,6 +80,20 @@ __device__ inline T& bnd_at(T* bnd, int slot, int which, int lane, int col)
 	return bnd[((slot * 2 + which) * 64 + lane) * 2 + col];
 }
 
+template<int NW> constexpr int bnd_size() { return (NW + 2) * 2 * 64 * 2; }
+
+// Neighbours E, W, N, S of the cell (row i, column o) of a lane's strip v:
+// the pair partner, the adjacent lane's column (DPP), the rows above and
+// below (the strip itself or the boundary rows up / dn of the adjacent waves).
+template<typename T, int RPW>
+__device__ inline void strip_nbrs(const T (&v)[RPW][2], int i, int o, T up, T dn, T& E, T& W, T& N, T& S)
+{
+	if (o == 0) { W = lane_shift<DPP_WAVE_SHR1>(v[i][1]); E = v[i][1]; }
+	else { W = v[i][0]; E = lane_shift<DPP_WAVE_SHL1>(v[i][0]); }
+	S = i > 0 ? v[i - 1][o] : up;
+	N = i < RPW - 1 ? v[i + 1][o] : dn;
+}
+
 template<typename T, int RPW>
 __device__ inline void bnd_publish(T* bnd, const T (&v)[RPW][2], int w, int lane)
 {
@@ -95,131 +101,147 @@ __device__ inline void bnd_publish(T* bnd, const T (&v)[RPW][2], int w, int lane
 	bnd_at(bnd, w + 1, 1, lane, 0) = v[RPW - 1][0]; bnd_at(bnd, w + 1, 1, lane, 1) = v[RPW - 1][1];
 }
 
-template<int NW> constexpr int bnd_size() { return (NW + 2) * 2 * 64 * 2; }
-
-template<typename T, int S, int H, int NW, int RPW, int MODE, bool RESTRICT, bool EDGE>
-__device__ inline void smooth_tile(const T* __restrict__ xin, const T* __restrict__ b, const T* __restrict__ xc,
-                                   T* __restrict__ xout, T* __restrict__ rc, int nx, int ny, T off, T dg, T inv,
-                                   int X0, int Y0, T* bnd)
-{
+// One lane's strip of a tile: coordinates and the x / b registers.
+template<typename T, int S, int H, int NW, int RPW, bool EDGE>
+struct Strip {
 	using T2 = typename Pair<T>::type;
-	constexpr int LY = NW * RPW;
+	static constexpr int LY = NW * RPW;
 	static_assert(RPW % 2 == 0 && H % 2 == 0, "strips start on even rows and columns (fixed colour per register)");
-	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-	const int gxu = X0 - H + 2 * lane;          // first column of the pair, unwrapped (even)
-	const int gx = EDGE ? wrap_once(gxu, nx) : gxu;
-	const int gyb = Y0 - H + RPW * w;           // first row of the wave, unwrapped (even)
+	int lane, w, gxu, gx, gyb;
 	T xr[RPW][2], br[RPW][2];
-	if (MODE == M_NORMAL || MODE == M_PADD) {
+
+	__device__ Strip(int X0, int Y0, int nx)
+	{
+		lane = threadIdx.x & 63;
+		w = threadIdx.x >> 6;
+		gxu = X0 - H + 2 * lane;                // first column of the pair, unwrapped (even)
+		gx = EDGE ? wrap_once(gxu, nx) : gxu;
+		gyb = Y0 - H + RPW * w;                 // first row of the wave, unwrapped (even)
+	}
+	__device__ int row(int i, int ny) const { return EDGE ? wrap_once(gyb + i, ny) : gyb + i; }
+	// is (row i, column pair) part of the tile's output?
+	__device__ bool out_col(int nx) const { return lane >= H / 2 && lane < 64 - H / 2 && gxu < nx; }
+	__device__ bool out_row(int i, int ny) const
+	{
+		const int ly = RPW * w + i;
+		return ly >= H && ly < LY - H && gyb + i < ny;
+	}
+	// the 5-point stencil at (i, o) with the periodic wrap flags of EDGE tiles
+	template<bool DIAG, typename U>
+	__device__ U stencil(int i, int o, U off, U dg, U c, U E, U W, U N, U Sv, int nx, int ny) const
+	{
+		if (EDGE) {
+			const int px = wrap_once(gxu + o, nx), py = wrap_once(gyb + i, ny);
+			return stencil5<U, DIAG>(off, dg, c, E, W, N, Sv, px == nx - 1, px == 0, py == ny - 1, py == 0);
+		}
+		return stencil5<U, DIAG>(off, dg, c, E, W, N, Sv, false, false, false, false);
+	}
+
+	template<int MODE>
+	__device__ void load(const T* __restrict__ xin, const T* __restrict__ b, const T* __restrict__ xc, int nx, int ny)
+	{
+		if (MODE == M_NORMAL || MODE == M_PADD) {
+#pragma unroll
+			for (int i = 0; i < RPW; ++i) {
+				const T2 v = *(const T2*) (xin + (size_t) nx * row(i, ny) + gx);
+				xr[i][0] = v.x; xr[i][1] = v.y;
+			}
+		}
 #pragma unroll
 		for (int i = 0; i < RPW; ++i) {
-			const int gy = EDGE ? wrap_once(gyb + i, ny) : gyb + i;
-			const T2 v = *(const T2*) (xin + (size_t) nx * gy + gx);
-			xr[i][0] = v.x; xr[i][1] = v.y;
+			const T2 v = *(const T2*) (b + (size_t) nx * row(i, ny) + gx);
+			br[i][0] = v.x; br[i][1] = v.y;
 		}
-	}
+		if (MODE == M_ZERO) {
 #pragma unroll
-	for (int i = 0; i < RPW; ++i) {
-		const int gy = EDGE ? wrap_once(gyb + i, ny) : gyb + i;
-		const T2 v = *(const T2*) (b + (size_t) nx * gy + gx);
-		br[i][0] = v.x; br[i][1] = v.y;
-	}
-	if (MODE == M_ZERO) {
-#pragma unroll
-		for (int i = 0; i < RPW; ++i) xr[i][0] = xr[i][1] = T(0);
-	}
-	if (MODE == M_PADD || MODE == M_PSET) {
-		// coarse rows gyb/2 - 1 .. gyb/2 + RPW/2, columns m-1 .. m+1 (periodic)
-		const int hx = nx >> 1, hy = ny >> 1, m = gxu / 2;
-		int cc[3];
-#pragma unroll
-		for (int j = 0; j < 3; ++j) cc[j] = wrap_once(m - 1 + j, hx);
-		T cw[RPW / 2 + 2][3];
-#pragma unroll
-		for (int r = 0; r < RPW / 2 + 2; ++r) {
-			const size_t row = (size_t) hx * wrap_once(gyb / 2 - 1 + r, hy);
-#pragma unroll
-			for (int j = 0; j < 3; ++j) cw[r][j] = xc[row + cc[j]];
+			for (int i = 0; i < RPW; ++i) xr[i][0] = xr[i][1] = T(0);
 		}
+		if (MODE == M_PADD || MODE == M_PSET) {
+			// coarse rows gyb/2 - 1 .. gyb/2 + RPW/2, columns m-1 .. m+1 (periodic)
+			const int hx = nx >> 1, hy = ny >> 1, m = gxu / 2;
+			int cc[3];
 #pragma unroll
-		for (int i = 0; i < RPW; ++i) {
-			// even fine row: primary coarse row i/2 + 1, other i/2; odd: (i+1)/2, (i+3)/2
-			const int pr = (i & 1) ? (i + 1) / 2 : i / 2 + 1, orr = (i & 1) ? (i + 3) / 2 : i / 2;
+			for (int j = 0; j < 3; ++j) cc[j] = wrap_once(m - 1 + j, hx);
+			T cw[RPW / 2 + 2][3];
 #pragma unroll
-			for (int o = 0; o < 2; ++o) {
-				// even fine column: primary m, other m-1; odd: primary m, other m+1
-				const int oc = o ? 2 : 0;
-				const T P = cw[pr][1], Sx = cw[pr][oc], Sy = cw[orr][1], Dg = cw[orr][oc];
-				const T p = T(9. / 16.) * P + T(3. / 16.) * (Sx + Sy) + T(1. / 16.) * Dg;
-				xr[i][o] = (MODE == M_PADD) ? xr[i][o] + p : p;
+			for (int r = 0; r < RPW / 2 + 2; ++r) {
+				const size_t crow = (size_t) hx * wrap_once(gyb / 2 - 1 + r, hy);
+#pragma unroll
+				for (int j = 0; j < 3; ++j) cw[r][j] = xc[crow + cc[j]];
+			}
+#pragma unroll
+			for (int i = 0; i < RPW; ++i) {
+				// even fine row: primary coarse row i/2 + 1, other i/2; odd: (i+1)/2, (i+3)/2
+				const int pr = (i & 1) ? (i + 1) / 2 : i / 2 + 1, orr = (i & 1) ? (i + 3) / 2 : i / 2;
+#pragma unroll
+				for (int o = 0; o < 2; ++o) {
+					// even fine column: primary m, other m-1; odd: primary m, other m+1
+					const int oc = o ? 2 : 0;
+					const T P = cw[pr][1], Sx = cw[pr][oc], Sy = cw[orr][1], Dg = cw[orr][oc];
+					const T p = T(9. / 16.) * P + T(3. / 16.) * (Sx + Sy) + T(1. / 16.) * Dg;
+					xr[i][o] = (MODE == M_PADD) ? xr[i][o] + p : p;
+				}
 			}
 		}
 	}
-	bnd_publish<T, RPW>(bnd, xr, w, lane);
-	__syncthreads();
+
+	// 2S red-black half-sweeps; bnd: LDS boundary-row exchange
+	__device__ void sweeps(T* bnd, int nx, int ny, T off, T dg, T inv)
+	{
+		bnd_publish<T, RPW>(bnd, xr, w, lane);
+		__syncthreads();
 #pragma unroll
-	for (int ph = 0; ph < 2 * S; ++ph) {
-		const int c = ph & 1;                   // colour (gx + gy) & 1 updated now
-		// the cell of colour c in row i of this wave is column o = (c + i) & 1
-		const T up = bnd_at(bnd, w, 1, lane, c);          // row above the wave, column c
-		const T dn = bnd_at(bnd, w + 2, 0, lane, 1 - c);  // row below the wave, column 1 - c
+		for (int ph = 0; ph < 2 * S; ++ph) {
+			const int c = ph & 1;               // colour (gx + gy) & 1 updated now
+			// the cell of colour c in row i of this wave is column o = (c + i) & 1
+			const T up = bnd_at(bnd, w, 1, lane, c);          // row above the wave, column c
+			const T dn = bnd_at(bnd, w + 2, 0, lane, 1 - c);  // row below the wave, column 1 - c
 #pragma unroll
-		for (int i = 0; i < RPW; ++i) {
-			const int o = (c + i) & 1;
-			T Ea, We, Nv, Sv;
-			strip_nbrs<T, RPW>(xr, i, o, up, dn, Ea, We, Nv, Sv);
-			T v;
-			if (EDGE) {
-				const int px = wrap_once(gxu + o, nx), py = wrap_once(gyb + i, ny);
-				v = stencil5<T, false>(off, dg, T(0), Ea, We, Nv, Sv, px == nx - 1, px == 0, py == ny - 1, py == 0);
-			} else {
-				v = stencil5<T, false>(off, dg, T(0), Ea, We, Nv, Sv, false, false, false, false);
+			for (int i = 0; i < RPW; ++i) {
+				const int o = (c + i) & 1;
+				T Ea, We, Nv, Sv;
+				strip_nbrs<T, RPW>(xr, i, o, up, dn, Ea, We, Nv, Sv);
+				const T v = stencil<false, T>(i, o, off, dg, T(0), Ea, We, Nv, Sv, nx, ny);
+				xr[i][o] = (br[i][o] - v) * inv;
 			}
-			xr[i][o] = (br[i][o] - v) * inv;
-		}
-		if (ph < 2 * S - 1) {
-			bnd_at(bnd, w + 1, 0, lane, c) = xr[0][c];
-			bnd_at(bnd, w + 1, 1, lane, 1 - c) = xr[RPW - 1][1 - c];
-			__syncthreads();
+			if (ph < 2 * S - 1) {
+				bnd_at(bnd, w + 1, 0, lane, c) = xr[0][c];
+				bnd_at(bnd, w + 1, 1, lane, 1 - c) = xr[RPW - 1][1 - c];
+				__syncthreads();
+			}
 		}
 	}
-	if (lane >= H / 2 && lane < 64 - H / 2 && gxu < nx) {
+
+	__device__ void store(T* __restrict__ xout, int nx, int ny) const
+	{
+		if (!out_col(nx)) return;
 #pragma unroll
-		for (int i = 0; i < RPW; ++i) {
-			const int ly = RPW * w + i;
-			if (ly >= H && ly < LY - H && gyb + i < ny) {
+		for (int i = 0; i < RPW; ++i)
+			if (out_row(i, ny)) {
 				T2 v;
 				v.x = xr[i][0]; v.y = xr[i][1];
 				*(T2*) (xout + (size_t) nx * (gyb + i) + gx) = v;
 			}
-		}
 	}
-	const int cnx = nx >> 1, cny = ny >> 1;
-	if (RESTRICT) {
-		// residual b - L x (mg/laplacian.h rows in stored order, diagonal
-		// included) on every cell of the strip; exact on the output ring 1
-		T* bx = bnd + bnd_size<NW>();
-		T* brs = bx + bnd_size<NW>();
+
+	// residual b - L x (mg/laplacian.h rows in stored order, diagonal
+	// included) on every cell, exact on the output ring 1, restricted into
+	// rc; b is overwritten.  bx, brs: two more LDS exchange buffers.
+	__device__ void restrict_to(T* __restrict__ rc, T* bx, T* brs, int X0, int Y0, int nx, int ny, T off, T dg)
+	{
 		bnd_publish<T, RPW>(bx, xr, w, lane);
 		__syncthreads();
 		T up[2], dn[2];
 #pragma unroll
 		for (int o = 0; o < 2; ++o) { up[o] = bnd_at(bx, w, 1, lane, o); dn[o] = bnd_at(bx, w + 2, 0, lane, o); }
-		T (&rr)[RPW][2] = br;                  // the residual replaces b
+		T (&rr)[RPW][2] = br;
 #pragma unroll
 		for (int i = 0; i < RPW; ++i)
 #pragma unroll
 			for (int o = 0; o < 2; ++o) {
 				T Ea, We, Nv, Sv;
 				strip_nbrs<T, RPW>(xr, i, o, up[o], dn[o], Ea, We, Nv, Sv);
-				T Lx;
-				if (EDGE) {
-					const int px = wrap_once(gxu + o, nx), py = wrap_once(gyb + i, ny);
-					Lx = stencil5<T, true>(off, dg, xr[i][o], Ea, We, Nv, Sv, px == nx - 1, px == 0, py == ny - 1, py == 0);
-				} else {
-					Lx = stencil5<T, true>(off, dg, xr[i][o], Ea, We, Nv, Sv, false, false, false, false);
-				}
-				rr[i][o] = br[i][o] - Lx;
+				rr[i][o] = br[i][o] - stencil<true, T>(i, o, off, dg, xr[i][o], Ea, We, Nv, Sv, nx, ny);
 			}
 		bnd_publish<T, RPW>(brs, rr, w, lane);
 		__syncthreads();
@@ -229,8 +251,8 @@ __device__ inline void smooth_tile(const T* __restrict__ xin, const T* __restric
 		T ru[2], rd[2];
 #pragma unroll
 		for (int o = 0; o < 2; ++o) { ru[o] = bnd_at(brs, w, 1, lane, o); rd[o] = bnd_at(brs, w + 2, 0, lane, o); }
-		const int j = lane;
-		const int cgx = (X0 - H) / 2 + j;
+		const int cnx = nx >> 1, cny = ny >> 1;
+		const int j = lane, cgx = (X0 - H) / 2 + j;
 		const bool colok = j >= H / 2 && j < 64 - H / 2 && cgx < cnx;
 #pragma unroll
 		for (int qq = 0; qq < RPW / 2; ++qq) {
@@ -239,8 +261,9 @@ __device__ inline void smooth_tile(const T* __restrict__ xin, const T* __restric
 #pragma unroll
 			for (int dz = 0; dz < 4; ++dz) {
 				const int i = 2 * qq - 1 + dz;    // fine row of the strip, -1 .. RPW
-				const T c0 = i < 0 ? ru[0] : (i >= RPW ? rd[0] : rr[i < 0 ? 0 : (i >= RPW ? RPW - 1 : i)][0]);
-				const T c1 = i < 0 ? ru[1] : (i >= RPW ? rd[1] : rr[i < 0 ? 0 : (i >= RPW ? RPW - 1 : i)][1]);
+				const int ic = i < 0 ? 0 : (i >= RPW ? RPW - 1 : i);
+				const T c0 = i < 0 ? ru[0] : (i >= RPW ? rd[0] : rr[ic][0]);
+				const T c1 = i < 0 ? ru[1] : (i >= RPW ? rd[1] : rr[ic][1]);
 				const T cl = lane_shift<DPP_WAVE_SHR1>(c1), cr = lane_shift<DPP_WAVE_SHL1>(c0);
 				const int wz = 4 - abs(2 * (dz - 2) + 1);
 				o += T((1 * wz) / 64.) * cl;
@@ -252,34 +275,147 @@ __device__ inline void smooth_tile(const T* __restrict__ xin, const T* __restric
 			if (colok && q >= H / 2 && q < LY / 2 - H / 2 && cgy < cny) rc[(size_t) cnx * cgy + cgx] = o;
 		}
 	}
-}
+
+	// xs' = xs + x on the output ring 1, r = rhs - L xs' on the output tile
+	// (the reference's update and residual, mg/solver.h:226-233, in FP64).
+	// Writes xs', r and returns this lane's sum of r^2 (rows, then columns).
+	template<typename TR>
+	__device__ double update(const double* __restrict__ xs, const double* __restrict__ rhs, double* __restrict__ xs_out,
+	                         TR* __restrict__ res, double* bx, int nx, int ny, double off, double dg) const
+	{
+		double xn[RPW][2];
+		const bool ring1_col = lane >= H / 2 - 1 && lane <= 64 - H / 2;
+#pragma unroll
+		for (int i = 0; i < RPW; ++i) {
+			const int ly = RPW * w + i;
+			if (ring1_col && ly >= H - 1 && ly <= LY - H) {
+				const double2 v = *(const double2*) (xs + (size_t) nx * row(i, ny) + gx);
+				xn[i][0] = v.x + (double) xr[i][0];
+				xn[i][1] = v.y + (double) xr[i][1];
+			} else {
+				xn[i][0] = xn[i][1] = 0.0;
+			}
+		}
+		bnd_publish<double, RPW>(bx, xn, w, lane);
+		__syncthreads();
+		double up[2], dn[2];
+#pragma unroll
+		for (int o = 0; o < 2; ++o) { up[o] = bnd_at(bx, w, 1, lane, o); dn[o] = bnd_at(bx, w + 2, 0, lane, o); }
+		double acc = 0.0;
+		const bool oc = out_col(nx);
+#pragma unroll
+		for (int i = 0; i < RPW; ++i) {
+			double rv[2];
+#pragma unroll
+			for (int o = 0; o < 2; ++o) {
+				double Ea, We, Nv, Sv;
+				strip_nbrs<double, RPW>(xn, i, o, up[o], dn[o], Ea, We, Nv, Sv);
+				rv[o] = stencil<true, double>(i, o, off, dg, xn[i][o], Ea, We, Nv, Sv, nx, ny);
+			}
+			if (oc && out_row(i, ny)) {
+				const size_t at = (size_t) nx * (gyb + i) + gx;
+				const double2 r0 = *(const double2*) (rhs + at);
+				const double ra = r0.x - rv[0], rb = r0.y - rv[1];
+				double2 xo;
+				xo.x = xn[i][0]; xo.y = xn[i][1];
+				*(double2*) (xs_out + at) = xo;
+				typename Pair<TR>::type ro;
+				ro.x = (TR) ra; ro.y = (TR) rb;
+				*(typename Pair<TR>::type*) (res + at) = ro;
+				acc += ra * ra;
+				acc += rb * rb;
+			}
+		}
+		return acc;
+	}
+};
 
 template<int S> struct SmoothShape;
 template<> struct SmoothShape<1> { static constexpr int NW = 6, RPW = 6; };
 template<> struct SmoothShape<2> { static constexpr int NW = 4, RPW = 10; };
 
-// halo of a launch: 2 cells per sweep, 2 more for the residual + restriction
-template<int S, bool RESTRICT> constexpr int smooth_halo() { return 2 * S + (RESTRICT ? 2 : 0); }
-template<int S, bool RESTRICT> constexpr int smooth_txo() { return 128 - 2 * smooth_halo<S, RESTRICT>(); }
-template<int S, bool RESTRICT>
-constexpr int smooth_tyo() { return SmoothShape<S>::NW * SmoothShape<S>::RPW - 2 * smooth_halo<S, RESTRICT>(); }
+// halo of a launch: 2 cells per sweep, 2 more for a residual epilogue
+template<int S, int EPI> constexpr int smooth_halo() { return 2 * S + (EPI != EPI_STORE ? 2 : 0); }
+template<int S, int EPI> constexpr int smooth_txo() { return 128 - 2 * smooth_halo<S, EPI>(); }
+template<int S, int EPI>
+constexpr int smooth_tyo() { return SmoothShape<S>::NW * SmoothShape<S>::RPW - 2 * smooth_halo<S, EPI>(); }
 
-template<typename T, int S, int MODE, bool RESTRICT>
+template<int S, int EPI>
+static inline unsigned smooth_blocks(int nx, int ny)
+{
+	constexpr int TXO = smooth_txo<S, EPI>(), TYO = smooth_tyo<S, EPI>();
+	return (unsigned) (((nx + TXO - 1) / TXO) * ((ny + TYO - 1) / TYO));
+}
+
+// tile origin of this block and whether its region wraps
+template<int S, int EPI>
+__device__ inline bool smooth_tile_origin(int nx, int ny, int& X0, int& Y0)
+{
+	constexpr int H = smooth_halo<S, EPI>(), TXO = smooth_txo<S, EPI>(), TYO = smooth_tyo<S, EPI>();
+	constexpr int LY = SmoothShape<S>::NW * SmoothShape<S>::RPW;
+	const int ntx = (nx + TXO - 1) / TXO;
+	const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+	X0 = (int) (tile % ntx) * TXO;
+	Y0 = (int) (tile / ntx) * TYO;
+	return X0 - H < 0 || X0 - H + 128 > nx || Y0 - H < 0 || Y0 - H + LY > ny;
+}
+
+template<typename T, int S, int MODE, int EPI, bool EDGE>
+__device__ inline void smooth_body(const T* __restrict__ xin, const T* __restrict__ b, const T* __restrict__ xc,
+                                   T* __restrict__ xout, T* __restrict__ rc, int nx, int ny, T off, T dg, T inv,
+                                   int X0, int Y0, T* bnd)
+{
+	constexpr int NW = SmoothShape<S>::NW, RPW = SmoothShape<S>::RPW, H = smooth_halo<S, EPI>();
+	Strip<T, S, H, NW, RPW, EDGE> st(X0, Y0, nx);
+	st.template load<MODE>(xin, b, xc, nx, ny);
+	st.sweeps(bnd, nx, ny, off, dg, inv);
+	st.store(xout, nx, ny);
+	if (EPI == EPI_RESTRICT) st.restrict_to(rc, bnd + bnd_size<NW>(), bnd + 2 * bnd_size<NW>(), X0, Y0, nx, ny, off, dg);
+}
+
+template<typename T, int S, int MODE, int EPI>
 __global__ __launch_bounds__(64 * SmoothShape<S>::NW) void k_smooth(const T* __restrict__ xin, const T* __restrict__ b,
                                                                     const T* __restrict__ xc, T* __restrict__ xout,
                                                                     T* __restrict__ rc, int nx, int ny, T off, T dg,
                                                                     T inv, const int* __restrict__ done)
 {
-	constexpr int NW = SmoothShape<S>::NW, RPW = SmoothShape<S>::RPW, LY = NW * RPW;
-	constexpr int H = smooth_halo<S, RESTRICT>(), TXO = smooth_txo<S, RESTRICT>(), TYO = smooth_tyo<S, RESTRICT>();
-	__shared__ T bnd[(RESTRICT ? 3 : 1) * bnd_size<NW>()];
+	constexpr int NW = SmoothShape<S>::NW;
+	__shared__ T bnd[(EPI == EPI_RESTRICT ? 3 : 1) * bnd_size<NW>()];
 	if (done && *done) return;
-	const int ntx = (nx + TXO - 1) / TXO;
-	const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
-	const int X0 = (int) (tile % ntx) * TXO, Y0 = (int) (tile / ntx) * TYO;
-	const bool edge = X0 - H < 0 || X0 - H + 128 > nx || Y0 - H < 0 || Y0 - H + LY > ny;
-	if (edge) smooth_tile<T, S, H, NW, RPW, MODE, RESTRICT, true>(xin, b, xc, xout, rc, nx, ny, off, dg, inv, X0, Y0, bnd);
-	else smooth_tile<T, S, H, NW, RPW, MODE, RESTRICT, false>(xin, b, xc, xout, rc, nx, ny, off, dg, inv, X0, Y0, bnd);
+	int X0, Y0;
+	if (smooth_tile_origin<S, EPI>(nx, ny, X0, Y0))
+		smooth_body<T, S, MODE, EPI, true>(xin, b, xc, xout, rc, nx, ny, off, dg, inv, X0, Y0, bnd);
+	else
+		smooth_body<T, S, MODE, EPI, false>(xin, b, xc, xout, rc, nx, ny, off, dg, inv, X0, Y0, bnd);
+}
+
+// Finest-level post-smoothing fused with the outer update: e = smooth(e_pre +
+// P ec) against b (the V-cycle right-hand side, the current residual), then
+// xs' = xs + e, r = rhs - L xs' and the block's sum of r^2 in partials.
+template<typename TE, typename TR, int S>
+__global__ __launch_bounds__(64 * SmoothShape<S>::NW) void k_smooth_update(
+	const TE* __restrict__ epre, const TE* __restrict__ b, const TE* __restrict__ ec, const double* __restrict__ xs,
+	const double* __restrict__ rhs, double* __restrict__ xs_out, TR* __restrict__ res, int nx, int ny, TE off, TE dg,
+	TE inv, double off64, double dg64, const SolveCtl* __restrict__ ctl, double* __restrict__ partials)
+{
+	constexpr int NW = SmoothShape<S>::NW, RPW = SmoothShape<S>::RPW, H = smooth_halo<S, EPI_UPDATE>();
+	__shared__ TE bnd[bnd_size<NW>()];
+	__shared__ double bnd64[bnd_size<NW>()];
+	if (ctl->done) return;
+	int X0, Y0;
+	double acc[1];
+	if (smooth_tile_origin<S, EPI_UPDATE>(nx, ny, X0, Y0)) {
+		Strip<TE, S, H, NW, RPW, true> st(X0, Y0, nx);
+		st.template load<M_PADD>(epre, b, ec, nx, ny);
+		st.sweeps(bnd, nx, ny, off, dg, inv);
+		acc[0] = st.template update<TR>(xs, rhs, xs_out, res, bnd64, nx, ny, off64, dg64);
+	} else {
+		Strip<TE, S, H, NW, RPW, false> st(X0, Y0, nx);
+		st.template load<M_PADD>(epre, b, ec, nx, ny);
+		st.sweeps(bnd, nx, ny, off, dg, inv);
+		acc[0] = st.template update<TR>(xs, rhs, xs_out, res, bnd64, nx, ny, off64, dg64);
+	}
+	reduce_partial<1>(acc, partials);
 }
 
 // ------------------------------------------------------ residual+restrict --
@@ -747,13 +883,6 @@ static inline unsigned red_blocks(size_t N)
 	return (unsigned) (b < (size_t) MAX_RED_BLOCKS ? b : (size_t) MAX_RED_BLOCKS);
 }
 
-template<int S, bool RESTRICT>
-static inline unsigned smooth_blocks(int nx, int ny)
-{
-	constexpr int TXO = smooth_txo<S, RESTRICT>(), TYO = smooth_tyo<S, RESTRICT>();
-	return (unsigned) (((nx + TXO - 1) / TXO) * ((ny + TYO - 1) / TYO));
-}
-
 // sweeps red-black sweeps at level l, x0 given by `mode`; with rc != nullptr
 // also rc = R (b - L x) on level l-1 (mode must be M_ZERO then).
 template<typename T>
@@ -764,20 +893,20 @@ static int smooth_launch(Ctx* c, unsigned l, int mode, int sweeps, const T* xin,
 	const double n2 = (double) L.n * (double) L.n;
 	const T off = (T) n2, dg = (T) (-6.0 * n2 + n2 + n2), inv = T(1) / dg;   // dg = -4 n^2, a power of two
 	const int nx = (int) L.nx, ny = (int) L.ny;
-#define AFM_SM(S_, M_, R_)                                                                                     \
-	hipLaunchKernelGGL((k_smooth<T, S_, M_, R_>), dim3(smooth_blocks<S_, R_>(nx, ny)),                        \
+#define AFM_SM(S_, M_, E_)                                                                                     \
+	hipLaunchKernelGGL((k_smooth<T, S_, M_, E_>), dim3(smooth_blocks<S_, E_>(nx, ny)),                        \
 	                   dim3(64 * SmoothShape<S_>::NW), 0, c->stream, xin, b, xc, xout, rc, nx, ny, off, dg, inv, done)
 	if (rc) {
 		if (mode != M_ZERO) { set_error("fused restriction needs a zero start"); return -1; }
-		if (sweeps == 1) AFM_SM(1, M_ZERO, true);
-		else if (sweeps == 2) AFM_SM(2, M_ZERO, true);
+		if (sweeps == 1) AFM_SM(1, M_ZERO, EPI_RESTRICT);
+		else if (sweeps == 2) AFM_SM(2, M_ZERO, EPI_RESTRICT);
 		else { set_error("smoother supports 1 or 2 sweeps per launch"); return -1; }
 	} else if (sweeps == 1) {
-		switch (mode) { case M_NORMAL: AFM_SM(1, M_NORMAL, false); break; case M_ZERO: AFM_SM(1, M_ZERO, false); break;
-		                case M_PADD: AFM_SM(1, M_PADD, false); break; default: AFM_SM(1, M_PSET, false); }
+		switch (mode) { case M_NORMAL: AFM_SM(1, M_NORMAL, EPI_STORE); break; case M_ZERO: AFM_SM(1, M_ZERO, EPI_STORE); break;
+		                case M_PADD: AFM_SM(1, M_PADD, EPI_STORE); break; default: AFM_SM(1, M_PSET, EPI_STORE); }
 	} else if (sweeps == 2) {
-		switch (mode) { case M_NORMAL: AFM_SM(2, M_NORMAL, false); break; case M_ZERO: AFM_SM(2, M_ZERO, false); break;
-		                case M_PADD: AFM_SM(2, M_PADD, false); break; default: AFM_SM(2, M_PSET, false); }
+		switch (mode) { case M_NORMAL: AFM_SM(2, M_NORMAL, EPI_STORE); break; case M_ZERO: AFM_SM(2, M_ZERO, EPI_STORE); break;
+		                case M_PADD: AFM_SM(2, M_PADD, EPI_STORE); break; default: AFM_SM(2, M_PSET, EPI_STORE); }
 	} else {
 		set_error("smoother supports 1 or 2 sweeps per launch");
 		return -1;
@@ -785,6 +914,12 @@ static int smooth_launch(Ctx* c, unsigned l, int mode, int sweeps, const T* xin,
 #undef AFM_SM
 	AFM_LAUNCH_CHECK();
 	return 0;
+}
+
+// blocks of the fused post-smoothing + update launch (one partial sum each)
+static inline unsigned update_blocks(int nx, int ny, int sweeps)
+{
+	return sweeps == 1 ? smooth_blocks<1, EPI_UPDATE>(nx, ny) : smooth_blocks<2, EPI_UPDATE>(nx, ny);
 }
 
 // nu sweeps as launches of at most 2 sweeps; the first launch applies `mode`.
@@ -846,9 +981,23 @@ static int coarse_launch(Ctx* c, int mode, int Lt, const T* in, T* out, int nu1,
 	return 0;
 }
 
-// recurse(l, B_l) -> X_l (final in Xb); rhs of level l given explicitly.
+// The outer update fused into the finest post-smoothing (mg_solve).
 template<typename T>
-static int vcycle(Ctx* c, unsigned l, const T* rhs_l, int nu1, int nu2, const int* done)
+struct UpdateArgs {
+	const double* xs;       // current iterate
+	const double* rhs;      // right-hand side (mean removed)
+	double* xs_out;         // next iterate
+	T* res_out;             // next residual (the next V-cycle's right-hand side)
+	SolveCtl* ctl;
+	double* partials;       // one r^2 sum per block
+};
+
+// recurse(l, B_l) -> X_l (final in Xb); rhs of level l given explicitly.
+// With up (and l above the LDS levels), the last post-smoothing instead
+// produces the outer update and residual of mg_solve.
+template<typename T>
+static int vcycle(Ctx* c, unsigned l, const T* rhs_l, int nu1, int nu2, const int* done,
+                  const UpdateArgs<T>* up = nullptr)
 {
 	const unsigned lc = c->lc;
 	if (l <= lc) return coarse_launch<T>(c, 0, (int) l, rhs_l, lvX<T>(c->lv[l], 1), nu1, nu2, done);
@@ -861,8 +1010,24 @@ static int vcycle(Ctx* c, unsigned l, const T* rhs_l, int nu1, int nu2, const in
 	AFM_TRY(coarse_launch<T>(c, 0, (int) lc, lvB<T>(c->lv[lc]), lvX<T>(c->lv[lc], 1), nu1, nu2, done));
 	for (unsigned k = lc + 1; k <= l; ++k) {
 		const T* bk = (k == l) ? rhs_l : lvB<T>(c->lv[k]);
-		AFM_TRY(smooth_n<T>(c, k, M_PADD, nu2, lvX<T>(c->lv[k], 0), bk, lvX<T>(c->lv[k - 1], 1), lvX<T>(c->lv[k], 1),
-		                    lvX<T>(c->lv[k], 0), done));
+		if (k == l && up) {
+			const LevelBuf& L = c->lv[k];
+			const double n2 = (double) L.n * (double) L.n;
+			const T off = (T) n2, dg = (T) (-6.0 * n2 + n2 + n2), inv = T(1) / dg;
+			const int nx = (int) L.nx, ny = (int) L.ny;
+#define AFM_SU(S_)                                                                                              \
+			hipLaunchKernelGGL((k_smooth_update<T, T, S_>), dim3(smooth_blocks<S_, EPI_UPDATE>(nx, ny)),       \
+			                   dim3(64 * SmoothShape<S_>::NW), 0, c->stream, lvX<T>(c->lv[k], 0), bk,          \
+			                   lvX<T>(c->lv[k - 1], 1), up->xs, up->rhs, up->xs_out, up->res_out, nx, ny, off, dg, \
+			                   inv, n2, -4.0 * n2, up->ctl, up->partials)
+			if (nu2 == 1) AFM_SU(1);
+			else AFM_SU(2);
+#undef AFM_SU
+			AFM_LAUNCH_CHECK();
+		} else {
+			AFM_TRY(smooth_n<T>(c, k, M_PADD, nu2, lvX<T>(c->lv[k], 0), bk, lvX<T>(c->lv[k - 1], 1),
+			                    lvX<T>(c->lv[k], 1), lvX<T>(c->lv[k], 0), done));
+		}
 	}
 	return 0;
 }
@@ -913,7 +1078,16 @@ int mg_setup(Ctx* c)
 	AFM_TRY(c->alloc((void**) &c->XSa, c->N * 8));
 	AFM_TRY(c->alloc((void**) &c->XSb, c->N * 8));
 	AFM_TRY(c->alloc((void**) &c->RES, c->N * 8));
+	AFM_TRY(c->alloc((void**) &c->RESb, c->N * 8));
 	AFM_TRY(c->alloc((void**) &c->RES32, c->N * 4));
+	AFM_TRY(c->alloc((void**) &c->RES32b, c->N * 4));
+	{
+		// one partial sum per block of the largest reduction of a solve
+		size_t nb = MAX_RED_BLOCKS;
+		nb = std::max<size_t>(nb, update_blocks((int) c->nx, (int) c->ny, 1));
+		nb = std::max<size_t>(nb, update_blocks((int) c->nx, (int) c->ny, 2));
+		AFM_TRY(c->alloc((void**) &c->upartials, nb * 8));
+	}
 	return 0;
 }
 
@@ -957,24 +1131,43 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 	hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, 1, rb, N, dctl, c->partials);
 	AFM_LAUNCH_CHECK();
 	const int* done = &dctl->done;
+	// above the LDS levels the update rides on the last post-smoothing
+	// launch, with the residual double-buffered (read as b, written as r)
+	const bool fused = T > c->lc;
+	const unsigned ub = fused ? update_blocks((int) c->nx, (int) c->ny, nu2) : rb;
 	int enq = 0;   // cycles enqueued
 	int batch = c->last_mg_cycles > 0 ? c->last_mg_cycles : 2;
 	double* xs[2] = {c->XSa, c->XSb};
+	double* res64[2] = {c->RES, c->RESb};
+	float* res32[2] = {c->RES32, c->RES32b};
 	for (;;) {
 		if (batch > max_cycles - enq) batch = max_cycles - enq;
 		for (int s = 0; s < batch; ++s) {
 			double* cur = xs[enq & 1];
 			double* nxt = xs[(enq + 1) & 1];
+			const int rc = fused ? (enq & 1) : 0, rn = fused ? ((enq + 1) & 1) : 0;
 			if (precision == 0) {
-				AFM_TRY(vcycle<double>(c, T, c->RES, nu1, nu2, done));
-				hipLaunchKernelGGL((k_update_resid<double, double>), dim3(rb), dim3(256), 0, c->stream, cur,
-				                   (const double*) c->lv[T].Xb, rhs, nxt, c->RES, (int) c->nx, (int) c->ny, off, dg, 0, 0, dctl, c->partials);
+				if (fused) {
+					const UpdateArgs<double> up{cur, rhs, nxt, res64[rn], dctl, c->upartials};
+					AFM_TRY(vcycle<double>(c, T, res64[rc], nu1, nu2, done, &up));
+				} else {
+					AFM_TRY(vcycle<double>(c, T, res64[rc], nu1, nu2, done));
+					hipLaunchKernelGGL((k_update_resid<double, double>), dim3(rb), dim3(256), 0, c->stream, cur,
+					                   (const double*) c->lv[T].Xb, rhs, nxt, res64[rn], (int) c->nx, (int) c->ny, off, dg, 0,
+					                   0, dctl, c->upartials);
+				}
 			} else {
-				AFM_TRY(vcycle<float>(c, T, c->RES32, nu1, nu2, done));
-				hipLaunchKernelGGL((k_update_resid<float, float>), dim3(rb), dim3(256), 0, c->stream, cur,
-				                   (const float*) c->lv[T].Xb32, rhs, nxt, c->RES32, (int) c->nx, (int) c->ny, off, dg, 0, 0, dctl, c->partials);
+				if (fused) {
+					const UpdateArgs<float> up{cur, rhs, nxt, res32[rn], dctl, c->upartials};
+					AFM_TRY(vcycle<float>(c, T, res32[rc], nu1, nu2, done, &up));
+				} else {
+					AFM_TRY(vcycle<float>(c, T, res32[rc], nu1, nu2, done));
+					hipLaunchKernelGGL((k_update_resid<float, float>), dim3(rb), dim3(256), 0, c->stream, cur,
+					                   (const float*) c->lv[T].Xb32, rhs, nxt, res32[rn], (int) c->nx, (int) c->ny, off, dg, 0,
+					                   0, dctl, c->upartials);
+				}
 			}
-			hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, 0, rb, N, dctl, c->partials);
+			hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, 0, ub, N, dctl, c->upartials);
 			AFM_LAUNCH_CHECK();
 			++enq;
 		}
