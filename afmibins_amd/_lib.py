@@ -29,6 +29,8 @@ SIGNATURES = {
     "afm_sync": (_I, [_P]),
     "afm_set_stream": (_I, [_P, _P]),
     "afm_own_stream": (_P, [_P]),
+    "afm_update_timing": (_I, [_P, _I]),
+    "afm_update_timing_read": (_I, [_P, _DP, C.POINTER(_L)]),
     "afm_alloc": (_I, [_P, C.c_ulong, C.POINTER(_P)]),
     "afm_free": (_I, [_P, _P]),
     "afm_copy_to_device": (_I, [_P, _P, _P, C.c_ulong]),

@@ -68,6 +68,17 @@ class Context:
         torch.cuda.synchronize(self.device)
         check(_lib.load().afm_sync(self._h), "afm_sync")
 
+    def update_timing(self, enable=True):
+        """HIP-event timing of the finest-level post-smoothing + update launch
+        of every multigrid solve on this context (enable resets)."""
+        self._call("afm_update_timing", 1 if enable else 0)
+
+    def update_timing_read(self):
+        """(average duration in microseconds, number of launches timed)."""
+        avg, n = C.c_double(), C.c_long()
+        check(_lib.load().afm_update_timing_read(self._h, C.byref(avg), C.byref(n)), "afm_update_timing_read")
+        return avg.value, n.value
+
     def close(self):
         if self._h:
             torch.cuda.synchronize(self.device)

@@ -15,6 +15,8 @@ Ctx::~Ctx()
 	if (rec) (void) hipFree(rec);
 	if (bin_list) (void) hipFree(bin_list);
 	if (hctl) (void) hipHostFree(hctl);
+	for (hipEvent_t e : kt_ev)
+		if (e) (void) hipEventDestroy(e);
 	if (own_stream) (void) hipStreamDestroy(own_stream);
 }
 
@@ -138,6 +140,26 @@ int afm_set_stream(afm_ctx* c, void* stream)
 }
 
 void* afm_own_stream(afm_ctx* c) { return c ? (void*) c->own_stream : nullptr; }
+
+int afm_update_timing(afm_ctx* c, int enable)
+{
+	CHECK_ARG(c, "null context");
+	AFM_TRY(use(c));
+	if (enable && !c->kt_ev[0])
+		for (hipEvent_t& e : c->kt_ev) AFM_HIP(hipEventCreate(&e));
+	c->kt_on = enable != 0;
+	c->kt_ms = 0.0;
+	c->kt_launches = 0;
+	return 0;
+}
+
+int afm_update_timing_read(afm_ctx* c, double* avg_us, long* launches)
+{
+	CHECK_ARG(c && avg_us && launches, "null argument");
+	*launches = c->kt_launches;
+	*avg_us = c->kt_launches ? 1e3 * c->kt_ms / (double) c->kt_launches : 0.0;
+	return 0;
+}
 
 int afm_sync(afm_ctx* c)
 {

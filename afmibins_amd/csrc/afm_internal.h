@@ -96,6 +96,13 @@ struct Ctx {
 	long mcap = 0;
 	int sbx = 0, sby = 0;              // spread bins per axis (4x4 base cells each)
 	int last_mg_cycles = 0;            // predicted batch size for the next solve
+	// opt-in HIP-event timing of the finest-level post-smoothing + update
+	// launch of mg_solve (the largest single launch of a solve)
+	static constexpr int KT_SLOTS = 64;
+	bool kt_on = false;
+	hipEvent_t kt_ev[2 * KT_SLOTS] = {};
+	double kt_ms = 0.0;
+	long kt_launches = 0;
 	int last_cg_iters = 0;
 	std::vector<void*> owned;
 	~Ctx();

@@ -990,6 +990,7 @@ struct UpdateArgs {
 	T* res_out;             // next residual (the next V-cycle's right-hand side)
 	SolveCtl* ctl;
 	double* partials;       // one r^2 sum per block
+	int slot;               // timing event pair, or -1
 };
 
 // recurse(l, B_l) -> X_l (final in Xb); rhs of level l given explicitly.
@@ -1020,8 +1021,11 @@ static int vcycle(Ctx* c, unsigned l, const T* rhs_l, int nu1, int nu2, const in
 			                   dim3(64 * SmoothShape<S_>::NW), 0, c->stream, lvX<T>(c->lv[k], 0), bk,          \
 			                   lvX<T>(c->lv[k - 1], 1), up->xs, up->rhs, up->xs_out, up->res_out, nx, ny, off, dg, \
 			                   inv, n2, -4.0 * n2, up->ctl, up->partials)
+			const int slot = up->slot;
+			if (slot >= 0) AFM_HIP(hipEventRecord(c->kt_ev[2 * slot], c->stream));
 			if (nu2 == 1) AFM_SU(1);
 			else AFM_SU(2);
+			if (slot >= 0) AFM_HIP(hipEventRecord(c->kt_ev[2 * slot + 1], c->stream));
 #undef AFM_SU
 			AFM_LAUNCH_CHECK();
 		} else {
@@ -1142,13 +1146,15 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 	float* res32[2] = {c->RES32, c->RES32b};
 	for (;;) {
 		if (batch > max_cycles - enq) batch = max_cycles - enq;
+		const int enq0 = enq;
 		for (int s = 0; s < batch; ++s) {
+			const int slot = (c->kt_on && fused && s < Ctx::KT_SLOTS) ? s : -1;
 			double* cur = xs[enq & 1];
 			double* nxt = xs[(enq + 1) & 1];
 			const int rc = fused ? (enq & 1) : 0, rn = fused ? ((enq + 1) & 1) : 0;
 			if (precision == 0) {
 				if (fused) {
-					const UpdateArgs<double> up{cur, rhs, nxt, res64[rn], dctl, c->upartials};
+					const UpdateArgs<double> up{cur, rhs, nxt, res64[rn], dctl, c->upartials, slot};
 					AFM_TRY(vcycle<double>(c, T, res64[rc], nu1, nu2, done, &up));
 				} else {
 					AFM_TRY(vcycle<double>(c, T, res64[rc], nu1, nu2, done));
@@ -1158,7 +1164,7 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 				}
 			} else {
 				if (fused) {
-					const UpdateArgs<float> up{cur, rhs, nxt, res32[rn], dctl, c->upartials};
+					const UpdateArgs<float> up{cur, rhs, nxt, res32[rn], dctl, c->upartials, slot};
 					AFM_TRY(vcycle<float>(c, T, res32[rc], nu1, nu2, done, &up));
 				} else {
 					AFM_TRY(vcycle<float>(c, T, res32[rc], nu1, nu2, done));
@@ -1172,6 +1178,15 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 			++enq;
 		}
 		AFM_TRY(read_ctl(c, 0));
+		if (c->kt_on && fused) {
+			// launches of cycles that ran (later ones exit at once)
+			for (int s = 0; s < batch && s < Ctx::KT_SLOTS && enq0 + s < c->hctl[0].count; ++s) {
+				float ms = 0.f;
+				AFM_HIP(hipEventElapsedTime(&ms, c->kt_ev[2 * s], c->kt_ev[2 * s + 1]));
+				c->kt_ms += ms;
+				++c->kt_launches;
+			}
+		}
 		if (c->hctl[0].done || enq >= max_cycles) break;
 		batch = 2;
 	}

@@ -109,6 +109,33 @@ def fourier_rhs(n, modes, kmax, seed):
     return f - f.mean()
 
 
+def fourier_rhs_device(n, modes, kmax, seed, device):
+    """fourier_rhs built on the GPU (torch float64): the same modes from the
+    same SplitMix64 stream, each cos(a + b) expanded into two outer products
+    so the whole field is one matrix product."""
+    import torch
+    rng = SplitMix64(seed)
+    kxs, kys, amps, phs = [], [], [], []
+    for _ in range(modes):
+        while True:
+            kx = int(rng.next() % (2 * kmax + 1)) - kmax
+            ky = int(rng.next() % (2 * kmax + 1)) - kmax
+            if kx or ky:
+                break
+        amps.append(rng.normal() * math.hypot(kx, ky) ** -2)
+        phs.append(rng.uniform(0.0, 2.0 * np.pi))
+        kxs.append(kx)
+        kys.append(ky)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=device)
+    x = (torch.arange(n, dtype=torch.float64, device=device) + 0.5) / n
+    ax = 2.0 * np.pi * x[:, None] * t(kxs)[None, :] + t(phs)[None, :]      # n x modes
+    by = 2.0 * np.pi * x[:, None] * t(kys)[None, :]
+    a = t(amps)
+    # f[y, x] = sum_m a_m (cos ax cos by - sin ax sin by)
+    f = (torch.cos(by) * a) @ torch.cos(ax).T - (torch.sin(by) * a) @ torch.sin(ax).T
+    return f - f.mean()
+
+
 # BASELINE configurations (sizes and physics) used by bench.py and the tests.
 CONFIGS = {
     "c1": dict(n=128, membranes=None, mu=0.01, rho=1.0, dt=1e-4, steps=200, tol=1e-10),

@@ -16,10 +16,13 @@ inside FP64 iterative refinement (`value`, the headline).  Inputs are resident
 in HBM before the timed region; the timed region is K full steps bracketed by
 device synchronisation.
 
-The JSON line also carries the roofline of the dominant kernel (the fused
-2-sweep red-black smoother at 2048^2, timed with HIP events on the stream it
-runs on) and the CPU baseline (the oracle's restatement of the reference path:
-lexicographic Gauss-Seidel V(2,2), CSR CG, one core) on a one-step sample.
+`roofline` is the dominant launch of the headline form: the finest-level
+post-smoothing fused with the FP64 outer update (k_smooth_update), timed with
+HIP events on the stream it runs on during extra steps of the same workload.
+Algorithmic bytes per finest cell: read e_pre and b (FP32, 4 + 4), the coarse
+correction (4 per 4 cells = 1), xs and rhs (FP64, 8 + 8); write xs and r
+(8 + 4): 37 B (50 B in the FP64 form).  `cpu_baseline` is the oracle's
+restatement of the reference path on one host core over a one-step sample.
 """
 import argparse
 import ctypes as C
@@ -32,6 +35,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+UPDATE_BYTES = {"mixed": 37.0, "fp64": 50.0}      # per finest cell, see above
+PMC_FILE = os.path.join(ROOT, "profiles", "k_smooth_update_f32_2048_pmc.txt")
 
 
 def parse():
@@ -42,12 +47,34 @@ def parse():
     p.add_argument("--n", type=int, default=2048)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--forms", default="fp64,mixed")
-    p.add_argument("--no-extras", action="store_true", help="skip the large-grid and transfer measurements")
-    p.add_argument("--kernel-only", action="store_true", help="only the smoother loop and a calibration copy")
+    p.add_argument("--no-extras", action="store_true", help="skip the smoother, transfer and solver-only measurements")
+    p.add_argument("--kernel-only", action="store_true", help="a few mixed steps and a calibration copy (counter runs)")
     return p.parse_args()
 
 
-def run_form(A, W, ctx, precision, args, inputs, dev):
+def hbm_roofline(nbytes, us, kernel, extra=None):
+    gbs = nbytes / (us * 1e-6) / 1e9
+    r = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+         "bytes_per_launch": nbytes, "avg_launch_us": round(us, 2)}
+    if extra:
+        r.update(extra)
+    return r
+
+
+def pmc_traffic():
+    """HBM bytes per launch of the roofline kernel from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    try:
+        for line in open(PMC_FILE):
+            if line.startswith("hbm_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def run_form(A, ctx, precision, args, inputs, dev, timing_steps=0):
     import torch
     u0, v0, X, R, k, ds = inputs
     t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)
@@ -65,19 +92,40 @@ def run_form(A, W, ctx, precision, args, inputs, dev):
     st = ns.stats()
     out = {"ms_per_step": 1e3 * (t1 - t0) / args.steps, "steps_per_s": args.steps / (t1 - t0),
            "last_mg_cycles": st["cycles"], "last_cg_iters": [st["cg_u"], st["cg_v"]]}
+    if timing_steps:
+        ctx.update_timing(True)
+        ib.step(timing_steps)
+        torch.cuda.synchronize()
+        out["update_us"], out["update_launches"] = ctx.update_timing_read()
+        ctx.update_timing(False)
+    out["u_final"] = u.clone()
     ib.close(); ns.close()
     return out
 
 
-def _events(dev):
+def _events():
     import torch
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
 
+def time_launches(fn, reps, warm=3):
+    """Average duration of fn() launches, HIP events on torch's current
+    stream (the stream the library is bound to)."""
+    import torch
+    for i in range(warm):
+        fn(i)
+    s = torch.cuda.current_stream()
+    e0, e1 = _events()
+    e0.record(s)
+    for i in range(reps):
+        fn(i)
+    e1.record(s)
+    e1.synchronize()
+    return 1e3 * e0.elapsed_time(e1) / reps
+
+
 def smoother_roofline(A, ctx, dev, reps=50):
-    """Average duration of the fused 2-sweep red-black smoother on the
-    context's finest level, timed with HIP events on the stream it is launched
-    on (torch's current stream, to which the context is bound)."""
+    """The 2-sweep red-black smoother (FP64, plain mode) on the finest level."""
     import torch
     n = ctx.n
     b = torch.randn((n, n), dtype=torch.float64, device=dev)
@@ -86,22 +134,9 @@ def smoother_roofline(A, ctx, dev, reps=50):
     def launch(i):
         ctx._call("afm_mg_smooth", ctx.levels, C.c_void_p(x[i & 1].data_ptr()), C.c_void_p(b.data_ptr()),
                   C.c_void_p(x[(i + 1) & 1].data_ptr()), 2, 0)
-    for i in range(5):
-        launch(i)
-    s = torch.cuda.current_stream(dev)
-    e0, e1 = _events(dev)
-    e0.record(s)
-    for i in range(reps):
-        launch(i)
-    e1.record(s)
-    e1.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    bytes_per_launch = 24.0 * n * n        # read x, read b, write x (2 sweeps fused)
-    gbs = bytes_per_launch / (ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"k_smooth<double,2,M_NORMAL> (2 fused red-black sweeps, {n}x{n})",
-            "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(ms * 1e3, 2)}
+    us = time_launches(launch, reps)
+    return hbm_roofline(24.0 * n * n, us, f"k_smooth<double,2,M_NORMAL> (2 fused red-black sweeps, {n}x{n}; "
+                                          "read x, b, write x: 24 B/cell)")
 
 
 def transfer_rates(A, W, dev, reps=10):
@@ -123,38 +158,73 @@ def transfer_rates(A, W, dev, reps=10):
     for name, fn, nbytes in (
             ("spread", lambda i: ctx.spread(i & 1, F, dsd, Xd, out=f[i & 1]), 8.0 * n * n + 32.0 * m),
             ("interp", lambda i: ctx.interp(i & 1, U, Xd), 152.0 * m)):
-        for i in range(3):
-            fn(i)
-        s = torch.cuda.current_stream(dev)
-        e0, e1 = _events(dev)
-        e0.record(s)
-        for i in range(reps):
-            fn(i)
-        e1.record(s)
-        e1.synchronize()
-        ms = e0.elapsed_time(e1) / reps
-        out[name] = {"us_per_call": round(ms * 1e3, 2), "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1),
-                     "frac_hbm": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "marker_per_s": round(m / (ms * 1e-3), 1)}
+        us = time_launches(fn, reps)
+        out[name] = {"us_per_call": round(us, 2), "GB/s": round(nbytes / (us * 1e-6) / 1e9, 1),
+                     "frac_hbm": round(nbytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                     "marker_per_s": round(m / (us * 1e-6), 1)}
     ctx.sync()
     ctx.close()
     out["case"] = f"{n}x{n} grid, {m} markers (512 membranes + 512 filaments x 256, 8 Gaussian clusters)"
     return out
 
 
-def kernel_only(A, dev):
-    """For counter runs: the smoother at 2048^2 (50 launches) and a plain copy
-    of 256 MiB (the calibration kernel for FETCH_SIZE / WRITE_SIZE)."""
+def solver_only(A, W, dev, n=4096, reps=3):
+    """Poisson by multigrid (FP64 vs FP32 V(2,2) in FP64 refinement) and
+    Helmholtz (I - dt mu L, dt = 2e-5, mu = 1e-3) by CG, both to 1e-10
+    relative residual, on a seeded 256-mode |k|^-2 right-hand side."""
     import torch
+    ctx = A.Context(n, 0)
+    f = W.fourier_rhs_device(n, 256, 32, 11, dev)
+    res = {"case": f"{n}x{n} periodic, 256 Fourier modes |k|<=32, |k|^-2 amplitudes, mean removed"}
+    sol = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        x = f.clone()
+        cyc, rms = ctx.mg_solve(x, 1e-10, 1, 2, 2, prec)        # warm (and the solution)
+        sol[name] = x
+        xs = [f.clone() for _ in range(reps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(reps):
+            ctx.mg_solve(xs[i], 1e-10, 1, 2, 2, prec)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / reps
+        res[f"poisson_{name}"] = {"ms_per_solve": round(ms, 3), "cycles": cyc, "final_rel_rms": rms}
+    d = sol["mixed"] - sol["fp64"]
+    d = d - d.mean()
+    res["poisson_mixed_over_fp64"] = round(res["poisson_fp64"]["ms_per_solve"] / res["poisson_mixed"]["ms_per_solve"], 3)
+    res["poisson_mixed_vs_fp64_rel_err"] = float(d.norm() / (sol["fp64"] - sol["fp64"].mean()).norm())
+    scale = -2e-5 * 1e-3
+    x = torch.zeros_like(f)
+    it, nrm = ctx.helmholtz_cg(scale, f, x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        x.zero_()
+        ctx.helmholtz_cg(scale, f, x)
+    torch.cuda.synchronize()
+    res["helmholtz_cg_fp64"] = {"ms_per_solve": round(1e3 * (time.perf_counter() - t0) / reps, 3), "iters": it}
+    ctx.sync()
+    ctx.close()
+    return res
+
+
+def kernel_only(A, W, dev):
+    """For counter runs: three mixed-form steps of the benchmark workload and
+    a plain 256 MiB copy (the calibration kernel for FETCH_SIZE / WRITE_SIZE)."""
+    import torch
+    inputs = W.config2_inputs(n=2048)
     ctx = A.Context(2048, 0)
-    r = smoother_roofline(A, ctx, dev)
+
+    class Args:
+        warmup, steps = 1, 2
+    r = run_form(A, ctx, 1, Args, inputs, dev)
     a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
     b = torch.empty_like(a)
     for _ in range(5):
         b.copy_(a)
     torch.cuda.synchronize()
     ctx.close()
-    print(json.dumps({"smoother": r, "copy_bytes_each_way": a.numel() * 8}), flush=True)
+    print(json.dumps({"ms_per_step": r["ms_per_step"], "copy_bytes_each_way": a.numel() * 8}), flush=True)
 
 
 def cpu_baseline(inputs, n):
@@ -162,7 +232,7 @@ def cpu_baseline(inputs, n):
     u0, v0, X, R, k, ds = inputs
     O.ib_run(u0, v0, X, R, k, ds, n, 1e-3, 1.0, 2e-5, 1e-10, 1, tol_mode=1, smoother=0, delta_mode=0)
     sec = O.lib().or_last_step_seconds()
-    return {"value": round(1.0 / sec, 4), "unit": "IB steps/s", "cores": 1, "kind": "port",
+    return {"value": round(1.0 / sec, 5), "unit": "IB steps/s", "cores": 1, "kind": "port",
             "sample": f"1 IB step of the same {n}x{n} / {X.shape[0]}-marker workload on one host core "
                       f"({sec:.1f} s): the oracle's restatement of the reference path (lexicographic "
                       f"Gauss-Seidel V(2,2) multigrid, CSR CG, glibc-cos delta)",
@@ -180,7 +250,7 @@ def main():
     from afmibins_amd import workloads as W
     dev = torch.device("cuda", 0)
     if args.kernel_only:
-        kernel_only(A, dev)
+        kernel_only(A, W, dev)
         return
     n = args.n
     inputs = W.config2_inputs(n=n)
@@ -188,13 +258,21 @@ def main():
     forms = {}
     for name in args.forms.split(","):
         prec = 0 if name == "fp64" else 1
-        forms[name] = run_form(A, W, ctx, prec, args, inputs, dev)
-        print(f"[bench] {name}: {forms[name]['ms_per_step']:.3f} ms/step, "
-              f"cycles {forms[name]['last_mg_cycles']}, cg {forms[name]['last_cg_iters']}", file=sys.stderr, flush=True)
-    roof = smoother_roofline(A, ctx, dev)
-    print(f"[bench] smoother: {roof['avg_launch_us']} us, {roof['achieved']} GB/s", file=sys.stderr, flush=True)
-    head = forms.get("mixed", forms.get("fp64"))
+        forms[name] = run_form(A, ctx, prec, args, inputs, dev, timing_steps=5)
+        print(f"[bench] {name}: {forms[name]['ms_per_step']:.3f} ms/step, cycles {forms[name]['last_mg_cycles']}, "
+              f"cg {forms[name]['last_cg_iters']}, update launch {forms[name]['update_us']:.2f} us",
+              file=sys.stderr, flush=True)
+    head_name = "mixed" if "mixed" in forms else "fp64"
+    head = forms[head_name]
     m = inputs[2].shape[0]
+    cells = float(n) * n
+    prec_txt = "float, float" if head_name == "mixed" else "double, double"
+    roof = hbm_roofline(UPDATE_BYTES[head_name] * cells, head["update_us"],
+                        f"k_smooth_update<{prec_txt}, 2> ({n}x{n}: finest post-smoothing + FP64 outer update, "
+                        f"{UPDATE_BYTES[head_name]:.0f} B/cell)",
+                        {"launches_timed": head["update_launches"]})
+    if head_name == "mixed":
+        roof["traffic"] = pmc_traffic()
     line = {
         "metric": "IB steps/s @2048² grid, 16k markers: FP64 vs FP32-MG+FP64 refine at 1e-10 resid",
         "value": round(head["steps_per_s"], 3),
@@ -206,25 +284,33 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (f32 V-cycles in f64 refinement)" if "mixed" in forms else "f64",
+        "dtype": "f64 (f32 V-cycles in f64 refinement)" if head_name == "mixed" else "f64",
         "data": "synthetic (seeded membranes and Fourier velocity)",
         "config": {"workload": f"2-D periodic IB, {n}x{n} MAC grid, {m} markers (64 membranes x 256), "
                                "mu=1e-3, rho=1, dt=2e-5, V(2,2) MG + CG Helmholtz to 1e-10 relative",
                    "grid": n, "markers": m},
-        "forms": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in f.items()}
+        "forms": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in f.items() if kk != "u_final"}
                   for k, f in forms.items()},
         "marker_steps_per_s": round(head["steps_per_s"] * m, 1),
         "roofline": roof,
     }
     if "fp64" in forms and "mixed" in forms:
         line["mixed_over_fp64"] = round(forms["mixed"]["steps_per_s"] / forms["fp64"]["steps_per_s"], 3)
+        a, b = forms["mixed"]["u_final"], forms["fp64"]["u_final"]
+        line["mixed_vs_fp64_u_rel_err"] = float((a - b).norm() / b.norm())
+        line["roofline_fp64_form"] = hbm_roofline(UPDATE_BYTES["fp64"] * cells, forms["fp64"]["update_us"],
+                                                  f"k_smooth_update<double, double, 2> ({n}x{n}, 50 B/cell)")
     if not args.no_extras:
+        line["smoother_2048"] = smoother_roofline(A, ctx, dev)
         big = A.Context(8192, 0)
-        line["roofline_hbm_8192"] = smoother_roofline(A, big, dev, reps=20)
+        line["smoother_8192"] = smoother_roofline(A, big, dev, reps=20)
         big.close()
-        print(f"[bench] smoother 8192^2: {line['roofline_hbm_8192']['achieved']} GB/s", file=sys.stderr, flush=True)
+        print(f"[bench] smoother: {line['smoother_2048']['achieved']} GB/s at 2048^2, "
+              f"{line['smoother_8192']['achieved']} GB/s at 8192^2", file=sys.stderr, flush=True)
         line["transfer"] = transfer_rates(A, W, dev)
         print(f"[bench] transfer: {line['transfer']}", file=sys.stderr, flush=True)
+        line["solver_4096"] = solver_only(A, W, dev)
+        print(f"[bench] solver-only: {line['solver_4096']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(inputs, n)
     ctx.close()

@@ -54,6 +54,13 @@ int afm_set_stream(afm_ctx* c, void* hip_stream);
 void* afm_own_stream(afm_ctx* c);
 const char* afm_last_error(void);
 
+/* Measurement (no reference counterpart): HIP-event timing, on the context's
+ * stream, of the finest-level post-smoothing + outer-update launch of
+ * afm_mg_solve (the largest single launch of a solve), counting only cycles
+ * that ran.  afm_update_timing(c, 1) enables and resets; 0 disables. */
+int afm_update_timing(afm_ctx* c, int enable);
+int afm_update_timing_read(afm_ctx* c, double* avg_us, long* launches);
+
 /* Device memory on the context's GPU for callers that hold no HIP headers
  * (the C++ drop-in classes of include/afm/).  Copies are synchronous. */
 int afm_alloc(afm_ctx* c, unsigned long bytes, void** out);

@@ -55,3 +55,13 @@ def test_membrane_generator(oracle):
     r0 = np.linalg.norm(R.reshape(64, 256, 2) - c[:, None, :], axis=2)
     assert np.allclose(r0.min(axis=1), r0.max(axis=1))
     assert np.allclose(ds.reshape(64, 256)[:, 0] * 256, 2 * np.pi * r0[:, 0])
+
+
+def test_fourier_rhs_device_matches_host_construction():
+    # the GPU-side generator used by the 4096^2 solver-only measurement builds
+    # the same field as the numpy one (checked on torch's CPU device)
+    import torch
+    n = 48
+    a = W.fourier_rhs(n, 40, 12, seed=11)
+    b = W.fourier_rhs_device(n, 40, 12, 11, torch.device("cpu")).numpy()
+    assert np.allclose(a, b, rtol=0, atol=1e-12 * np.abs(a).max())
