@@ -164,6 +164,25 @@ def test_vcycle_bit_exact(ctx_of, oracle, dev, n):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("nu1,nu2", [(1, 1), (1, 2), (2, 1)])
+def test_vcycle_and_solve_other_sweep_counts(ctx_of, oracle, dev, nu1, nu2):
+    # one-sweep launches of the fused pre-smoothing/restriction and of the
+    # fused post-smoothing/update kernels
+    n = 128
+    ctx = ctx_of(n)
+    L = int(np.log2(n))
+    r = np.random.default_rng(nu1 + 3 * nu2).standard_normal((n, n))
+    got = H(ctx.mg_vcycle(T(r, dev), nu1=nu1, nu2=nu2))
+    assert np.array_equal(got, oracle.mg_vcycle(r, L, dim=2, nu1=nu1, nu2=nu2, smoother=1))
+    rhs = W.fourier_rhs(n, 64, 16, seed=7) + 0.1
+    x = T(rhs, dev)
+    cyc, rms = ctx.mg_solve(x, tol=1e-10, tol_mode=1, nu1=nu1, nu2=nu2, precision=0)
+    want, cyc_o, _ = oracle.mg_solve(rhs, L, dim=2, tol=1e-10, tol_mode=1, nu1=nu1, nu2=nu2, smoother=1)
+    assert abs(cyc - cyc_o) <= 1
+    d = (H(x) - H(x).mean()) - (want - want.mean())
+    assert np.max(np.abs(d)) <= 1e-12 * np.max(np.abs(want - want.mean()))
+
+
 @pytest.mark.parametrize("n", [32, 128, 512])
 def test_mg_solve_matches_oracle(ctx_of, oracle, dev, n):
     ctx = ctx_of(n)
