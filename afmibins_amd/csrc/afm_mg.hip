@@ -22,8 +22,8 @@
 namespace afm {
 
 constexpr int RTX = 32, RTY = 8;         // restriction tile (coarse cells)
-constexpr int CMAXL = 5;                 // finest level handled in LDS (32x32)
-constexpr int CTHREADS = 256;
+constexpr int CMAXL = 6;                 // finest level handled in LDS (64x64)
+constexpr int CTHREADS = 1024;
 
 enum { M_NORMAL = 0, M_ZERO = 1, M_PADD = 2, M_PSET = 3 };
 
@@ -589,8 +589,8 @@ __global__ void k_copy_cvt(const T* __restrict__ src, double* __restrict__ dst, 
 }
 
 // ------------------------------------------------------------- coarse -----
-// Levels 0..Lt (n <= 32) of the unit square in one workgroup, all in LDS.
-// Level 5 (32x32) runs on all CTHREADS threads with block barriers; levels
+// Levels 0..Lt (n <= 64) of the unit square in one workgroup, all in LDS.
+// Levels 6 and 5 run on all CTHREADS threads with block barriers; levels
 // <= 4 run in wave 0 alone, where LDS operations complete in issue order and
 // no block barrier is needed.  Levels are template parameters, so every loop
 // bound is a constant and nothing spills.
@@ -602,12 +602,19 @@ __global__ void k_copy_cvt(const T* __restrict__ src, double* __restrict__ dst, 
 // red updates read black neighbours as x + P e, black updates never read
 // their own old value.
 struct CoarseLDS {
-	double X[1365], B[1365], CP[1365], TMP[1024];
+	double X[5461], B[5461], TMP[4096];   // levels 0..6 stacked; TMP: one residual
 };
 
 constexpr int lvl_off(int l) { return ((1 << (2 * l)) - 1) / 3; }   // sum_{j<l} 4^j
 
 template<int L> struct CLv { static constexpr int NT = L >= 5 ? CTHREADS : 64; };
+
+// run a wave-0 level from code that every thread executes
+#define C_WAVE0(call)                  \
+	do {                               \
+		if (threadIdx.x < 64) { call; } \
+		__syncthreads();               \
+	} while (0)
 
 template<int NT>
 __device__ inline void csync()
@@ -766,7 +773,7 @@ __device__ inline void c_solve0(CoarseLDS& S)
 }
 
 // recurse(L, B_L) -> X_L (mg/solver.h:199-216).  L <= 4: wave 0 only;
-// L == 5: every thread.
+// L >= 5: every thread.
 template<int L>
 __device__ void c_recurse(CoarseLDS& S, int nu1, int nu2)
 {
@@ -775,12 +782,9 @@ __device__ void c_recurse(CoarseLDS& S, int nu1, int nu2)
 	} else {
 		c_smooth<L>(S.X + lvl_off(L), S.B + lvl_off(L), true, nullptr, nu1);
 		c_restrict<L>(S, S.X + lvl_off(L), S.B + lvl_off(L), S.B + lvl_off(L - 1));
-		if constexpr (L == 5) {
-			if (threadIdx.x < 64) c_recurse<L - 1>(S, nu1, nu2);
-			__syncthreads();
-		} else {
-			c_recurse<L - 1>(S, nu1, nu2);
-		}
+		if constexpr (L - 1 >= 5) c_recurse<L - 1>(S, nu1, nu2);
+		else if constexpr (L == 5) C_WAVE0(c_recurse<L - 1>(S, nu1, nu2));
+		else c_recurse<L - 1>(S, nu1, nu2);
 		if constexpr (L >= 2) {
 			c_smooth<L>(S.X + lvl_off(L), S.B + lvl_off(L), false, S.X + lvl_off(L - 1), nu2);
 		} else {
@@ -802,33 +806,41 @@ __device__ inline void c_recurse_w0(CoarseLDS& S, int l, int nu1, int nu2)
 	}
 }
 
-// nested-iteration step at level k (mg/solver.h:183-197): down-leg copy and
-// restriction, or up-leg recurse(k-1), B_k = P X_{k-1}, smooth against CP_k.
+// recurse(l) called by every thread
+__device__ inline void c_recurse_all(CoarseLDS& S, int l, int nu1, int nu2)
+{
+	if (l >= 6) c_recurse<6>(S, nu1, nu2);
+	else if (l == 5) c_recurse<5>(S, nu1, nu2);
+	else C_WAVE0(c_recurse_w0(S, l, nu1, nu2));
+}
+
+// Nested-iteration steps at level L (mg/solver.h:183-197).  Down: B_{L-1} =
+// R B_L.  Up: recurse(L-1); X_L = P X_{L-1}; smooth X_L against B_L (still
+// the restricted right-hand side); B_L = X_L, the nested result that the
+// next level's recursion takes as its right-hand side.
 template<int L>
 __device__ void c_nested_down(CoarseLDS& S)
 {
-	constexpr int NT = CLv<L>::NT;
-	for (int i = threadIdx.x; i < (1 << (2 * L)); i += NT) S.CP[lvl_off(L) + i] = S.B[lvl_off(L) + i];
-	csync<NT>();
 	c_restrict<L>(S, nullptr, S.B + lvl_off(L), S.B + lvl_off(L - 1));
 }
 
 template<int L>
 __device__ void c_nested_up(CoarseLDS& S, int nu1, int nu2)
 {
-	if constexpr (L == 5) {
-		if (threadIdx.x < 64) c_recurse<4>(S, nu1, nu2);
-		__syncthreads();
-	} else {
-		c_recurse<L - 1>(S, nu1, nu2);
-	}
-	c_prolong<L>(S.X + lvl_off(L - 1), S.B + lvl_off(L), false);
-	c_smooth<L>(S.B + lvl_off(L), S.CP + lvl_off(L), false, nullptr, nu2);
+	constexpr int NT = CLv<L>::NT;
+	if constexpr (L - 1 >= 5) c_recurse<L - 1>(S, nu1, nu2);
+	else if constexpr (L == 5) C_WAVE0(c_recurse<L - 1>(S, nu1, nu2));
+	else c_recurse<L - 1>(S, nu1, nu2);
+	c_prolong<L>(S.X + lvl_off(L - 1), S.X + lvl_off(L), false);
+	c_smooth<L>(S.X + lvl_off(L), S.B + lvl_off(L), false, nullptr, nu2);
+	for (int i = threadIdx.x; i < (1 << (2 * L)); i += NT) S.B[lvl_off(L) + i] = S.X[lvl_off(L) + i];
+	csync<NT>();
 }
 
 // the whole nested iteration over levels 1..Lt (called by every thread)
 __device__ void c_nested(CoarseLDS& S, int Lt, int nu1, int nu2)
 {
+	if (Lt >= 6) c_nested_down<6>(S);
 	if (Lt >= 5) c_nested_down<5>(S);
 	if (threadIdx.x < 64) {
 		for (int k = Lt < 4 ? Lt : 4; k >= 1; --k) {
@@ -850,6 +862,7 @@ __device__ void c_nested(CoarseLDS& S, int Lt, int nu1, int nu2)
 	}
 	__syncthreads();
 	if (Lt >= 5) c_nested_up<5>(S, nu1, nu2);
+	if (Lt >= 6) c_nested_up<6>(S, nu1, nu2);
 }
 
 // mode 0: recurse(Lt, in) -> out;  1: nested(Lt, in); recurse -> out;
@@ -865,11 +878,7 @@ __global__ __launch_bounds__(CTHREADS) void k_coarse(int mode, int Lt, const TI*
 	__syncthreads();
 	if (mode >= 1) c_nested(S, Lt, nu1, nu2);
 	if (mode <= 1) {
-		if (Lt >= 5) c_recurse<5>(S, nu1, nu2);
-		else {
-			if (threadIdx.x < 64) c_recurse_w0(S, Lt, nu1, nu2);
-			__syncthreads();
-		}
+		c_recurse_all(S, Lt, nu1, nu2);
 		for (int i = threadIdx.x; i < N; i += CTHREADS) out[i] = (TI) S.X[o + i];
 	} else {
 		for (int i = threadIdx.x; i < N; i += CTHREADS) out[i] = (TI) S.B[o + i];
@@ -1221,7 +1230,8 @@ __global__ void k_lap_apply(const double* __restrict__ x, double* __restrict__ y
 
 static int check_level(Ctx* c, unsigned level)
 {
-	if (level > c->levels || level <= c->lc) { set_error("level must be a global level (n >= 64) of the context"); return -1; }
+	// the tiled kernels need n >= 64 (their regions wrap at most once)
+	if (level > c->levels || level < 6) { set_error("level must be a level of the context with n >= 64"); return -1; }
 	return 0;
 }
 
