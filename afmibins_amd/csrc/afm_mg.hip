@@ -281,7 +281,7 @@ struct Strip {
 	// Writes xs', r and returns this lane's sum of r^2 (rows, then columns).
 	template<typename TR>
 	__device__ double update(const double* __restrict__ xs, const double* __restrict__ rhs, double* __restrict__ xs_out,
-	                         TR* __restrict__ res, double* bx, int nx, int ny, double off, double dg) const
+	                         TR* __restrict__ res, double* bx, int nx, int ny, double off, double dg, bool write) const
 	{
 		double xn[RPW][2];
 		const bool ring1_col = lane >= H / 2 - 1 && lane <= 64 - H / 2;
@@ -316,12 +316,14 @@ struct Strip {
 				const size_t at = (size_t) nx * (gyb + i) + gx;
 				const double2 r0 = *(const double2*) (rhs + at);
 				const double ra = r0.x - rv[0], rb = r0.y - rv[1];
-				double2 xo;
-				xo.x = xn[i][0]; xo.y = xn[i][1];
-				*(double2*) (xs_out + at) = xo;
-				typename Pair<TR>::type ro;
-				ro.x = (TR) ra; ro.y = (TR) rb;
-				*(typename Pair<TR>::type*) (res + at) = ro;
+				if (write) {
+					double2 xo;
+					xo.x = xn[i][0]; xo.y = xn[i][1];
+					*(double2*) (xs_out + at) = xo;
+					typename Pair<TR>::type ro;
+					ro.x = (TR) ra; ro.y = (TR) rb;
+					*(typename Pair<TR>::type*) (res + at) = ro;
+				}
 				acc += ra * ra;
 				acc += rb * rb;
 			}
@@ -401,19 +403,21 @@ __global__ __launch_bounds__(64 * SmoothShape<S>::NW) void k_smooth_update(
 	constexpr int NW = SmoothShape<S>::NW, RPW = SmoothShape<S>::RPW, H = smooth_halo<S, EPI_UPDATE>();
 	__shared__ TE bnd[bnd_size<NW>()];
 	__shared__ double bnd64[bnd_size<NW>()];
-	if (ctl->done) return;
+	// the stop flag only gates the stores of the new iterate and residual, so
+	// its load is never waited for before the work
+	const int stop = ctl->done;
 	int X0, Y0;
 	double acc[1];
 	if (smooth_tile_origin<S, EPI_UPDATE>(nx, ny, X0, Y0)) {
 		Strip<TE, S, H, NW, RPW, true> st(X0, Y0, nx);
 		st.template load<M_PADD>(epre, b, ec, nx, ny);
 		st.sweeps(bnd, nx, ny, off, dg, inv);
-		acc[0] = st.template update<TR>(xs, rhs, xs_out, res, bnd64, nx, ny, off64, dg64);
+		acc[0] = st.template update<TR>(xs, rhs, xs_out, res, bnd64, nx, ny, off64, dg64, stop == 0);
 	} else {
 		Strip<TE, S, H, NW, RPW, false> st(X0, Y0, nx);
 		st.template load<M_PADD>(epre, b, ec, nx, ny);
 		st.sweeps(bnd, nx, ny, off, dg, inv);
-		acc[0] = st.template update<TR>(xs, rhs, xs_out, res, bnd64, nx, ny, off64, dg64);
+		acc[0] = st.template update<TR>(xs, rhs, xs_out, res, bnd64, nx, ny, off64, dg64, stop == 0);
 	}
 	reduce_partial<1>(acc, partials);
 }
@@ -1004,13 +1008,17 @@ struct UpdateArgs {
 
 // recurse(l, B_l) -> X_l (final in Xb); rhs of level l given explicitly.
 // With up (and l above the LDS levels), the last post-smoothing instead
-// produces the outer update and residual of mg_solve.
+// produces the outer update and residual of mg_solve.  Within a solve only
+// that launch (and its finaliser) consult the stop flag: the other launches
+// write level scratch only, so a cycle enqueued after convergence runs them
+// harmlessly instead of waiting on the flag at the start of every launch.
 template<typename T>
 static int vcycle(Ctx* c, unsigned l, const T* rhs_l, int nu1, int nu2, const int* done,
                   const UpdateArgs<T>* up = nullptr)
 {
 	const unsigned lc = c->lc;
 	if (l <= lc) return coarse_launch<T>(c, 0, (int) l, rhs_l, lvX<T>(c->lv[l], 1), nu1, nu2, done);
+	if (up) done = nullptr;
 	for (unsigned k = l; k > lc; --k) {
 		const T* bk = (k == l) ? rhs_l : lvB<T>(c->lv[k]);
 		// pre-smoothing from zero, residual and restriction in one launch
@@ -1197,7 +1205,7 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 			}
 		}
 		if (c->hctl[0].done || enq >= max_cycles) break;
-		batch = 2;
+		batch = 1;                  // beyond the prediction, one cycle at a time
 	}
 	const int cnt = c->hctl[0].count;
 	AFM_HIP(hipMemcpyAsync(rhs, xs[cnt & 1], N * 8, hipMemcpyDeviceToDevice, c->stream));
