@@ -14,6 +14,7 @@ Ctx::~Ctx()
 	for (void* p : owned) (void) hipFree(p);
 	if (rec) (void) hipFree(rec);
 	if (bin_list) (void) hipFree(bin_list);
+	if (srec) (void) hipFree(srec);
 	if (hctl) (void) hipHostFree(hctl);
 	for (hipEvent_t e : kt_ev)
 		if (e) (void) hipEventDestroy(e);

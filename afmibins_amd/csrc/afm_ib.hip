@@ -180,8 +180,9 @@ __global__ void k_spread_fill(int comp_mask, const MarkerRec* __restrict__ rec, 
 }
 
 // Each bin's list in ascending marker index (insertion sort; bins hold a few
-// markers).
-__global__ void k_spread_sort(const int* __restrict__ offs, int* __restrict__ list, int nbins)
+// markers), then the bin's records copied out in that order.
+__global__ void k_spread_sort(const int* __restrict__ offs, int* __restrict__ list, const MarkerRec* __restrict__ rec,
+                              long m, int nb1, BinRec* __restrict__ srec, int nbins)
 {
 	const int b = blockIdx.x * blockDim.x + threadIdx.x;
 	if (b >= nbins) return;
@@ -192,16 +193,29 @@ __global__ void k_spread_sort(const int* __restrict__ offs, int* __restrict__ li
 		while (j >= lo && list[j] > v) { list[j + 1] = list[j]; --j; }
 		list[j + 1] = v;
 	}
+	const MarkerRec* rc = rec + (b / nb1) * m;
+	for (int i = lo; i < hi; ++i) {
+		const int k = list[i];
+		const MarkerRec& r = rc[k];
+		BinRec o;
+		o.key = k;
+		o.base = r.ix0 | (r.iy0 << 16);
+#pragma unroll
+		for (int t = 0; t < 4; ++t) { o.dx[t] = r.dx[t]; o.dy[t] = r.dy[t]; }
+		o.ds = r.ds;
+		o.F = r.F;
+		srec[i] = o;
+	}
 }
 
 // One lane per grid cell (blocks of 64 x 4 cells, blockIdx.y = component):
 // the markers whose window base lies in [gx-3, gx] x [gy-3, gy] (periodic)
-// are in at most 2 x 2 bins; their sorted lists are merged so that the terms
-// are summed in ascending marker index.
-__global__ __launch_bounds__(256) void k_spread_gather(int comp_mask, const MarkerRec* __restrict__ rec, long m,
-                                                       const int* __restrict__ offs, const int* __restrict__ list,
-                                                       unsigned nx, unsigned ny, int sbx, int sby, int nb1,
-                                                       double* __restrict__ fx, double* __restrict__ fy, int accumulate)
+// are in at most 2 x 2 bins; their sorted records are merged so that the
+// terms are summed in ascending marker index.
+__global__ __launch_bounds__(256) void k_spread_gather(int comp_mask, const BinRec* __restrict__ srec,
+                                                       const int* __restrict__ offs, unsigned nx, unsigned ny, int sbx,
+                                                       int nb1, double* __restrict__ fx, double* __restrict__ fy,
+                                                       int accumulate)
 {
 	const int comp = blockIdx.y;
 	if (!((comp_mask >> comp) & 1)) return;
@@ -216,7 +230,7 @@ __global__ __launch_bounds__(256) void k_spread_gather(int comp_mask, const Mark
 	by[0] = (gy >= 3 ? gy - 3 : gy - 3 + (int) ny) / SB;
 	if (gx / SB != bx[0]) bx[nbx++] = gx / SB;
 	if (gy / SB != by[0]) by[nby++] = gy / SB;
-	int head[4], end[4], nl = 0;
+	int head[4], end[4], key[4], nl = 0;
 	for (int j = 0; j < nby; ++j)
 		for (int i = 0; i < nbx; ++i) {
 			const int b = comp * nb1 + bx[i] + sbx * by[j];
@@ -224,21 +238,18 @@ __global__ __launch_bounds__(256) void k_spread_gather(int comp_mask, const Mark
 			end[nl] = offs[b + 1];
 			++nl;
 		}
-	(void) sby;
-	const MarkerRec* rc = rec + comp * m;
+	for (int l = 0; l < nl; ++l) key[l] = head[l] < end[l] ? srec[head[l]].key : 0x7fffffff;
 	double acc = 0.0;
 	for (;;) {
 		int best = -1, bk = 0x7fffffff;
 		for (int l = 0; l < nl; ++l)
-			if (head[l] < end[l]) {
-				const int k = list[head[l]];
-				if (k < bk) { bk = k; best = l; }
-			}
+			if (key[l] < bk) { bk = key[l]; best = l; }
 		if (best < 0) break;
+		const BinRec& r = srec[head[best]];
 		++head[best];
-		const MarkerRec& r = rc[bk];
-		int a = gx - r.ix0; if (a < 0) a += nx;
-		int b = gy - r.iy0; if (b < 0) b += ny;
+		key[best] = head[best] < end[best] ? srec[head[best]].key : 0x7fffffff;
+		int a = gx - (r.base & 0xffff); if (a < 0) a += nx;
+		int b = gy - (r.base >> 16); if (b < 0) b += ny;
 		if (a < 4 && b < 4) {
 			double w = r.dx[a] * r.dy[b];
 			acc += w * r.ds * r.F;
@@ -291,8 +302,10 @@ int ensure_markers(Ctx* c, long m)
 	while (cap < m) cap *= 2;
 	if (c->rec) { (void) hipFree(c->rec); c->rec = nullptr; }
 	if (c->bin_list) { (void) hipFree(c->bin_list); c->bin_list = nullptr; }
+	if (c->srec) { (void) hipFree(c->srec); c->srec = nullptr; }
 	AFM_HIP(hipMalloc(&c->rec, sizeof(MarkerRec) * 2 * cap));
 	AFM_HIP(hipMalloc(&c->bin_list, sizeof(int) * 2 * cap));
+	AFM_HIP(hipMalloc(&c->srec, sizeof(BinRec) * 2 * cap));
 	c->mcap = cap;
 	return 0;
 }
@@ -321,14 +334,15 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 		hipLaunchKernelGGL(k_spread_fill, g, dim3(256), 0, c->stream, comp_mask, c->rec, m, nb1, offs, cursor,
 		                   c->bin_list);
 		AFM_LAUNCH_CHECK();
-		hipLaunchKernelGGL(k_spread_sort, dim3((nb + 255) / 256), dim3(256), 0, c->stream, offs, c->bin_list, nb);
+		hipLaunchKernelGGL(k_spread_sort, dim3((nb + 255) / 256), dim3(256), 0, c->stream, offs, c->bin_list, c->rec, m,
+		                   nb1, c->srec, nb);
 		AFM_LAUNCH_CHECK();
 	} else {
 		AFM_HIP(hipMemsetAsync(offs, 0, sizeof(int) * (nb + 1), c->stream));
 	}
 	const unsigned tiles = (unsigned) (((c->nx + 63) / 64) * ((c->ny + 3) / 4));
-	hipLaunchKernelGGL(k_spread_gather, dim3(tiles, 2), dim3(256), 0, c->stream, comp_mask, c->rec, m, offs,
-	                   c->bin_list, c->nx, c->ny, c->sbx, c->sby, nb1, fx, fy, accumulate);
+	hipLaunchKernelGGL(k_spread_gather, dim3(tiles, 2), dim3(256), 0, c->stream, comp_mask, c->srec, offs, c->nx, c->ny,
+	                   c->sbx, nb1, fx, fy, accumulate);
 	AFM_LAUNCH_CHECK();
 	return 0;
 }

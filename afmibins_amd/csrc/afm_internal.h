@@ -69,6 +69,14 @@ struct MarkerRec {
 	double ds, F;           // source volume and value
 };
 
+// The same record in bin order, as the spreading gather reads it: marker
+// index, window base (ix0 | iy0 << 16) and the terms' factors.
+struct BinRec {
+	int key, base;
+	double dx[4], dy[4];
+	double ds, F;
+};
+
 // ------------------------------------------------------------- context --
 struct LevelBuf {
 	unsigned n = 0, nx = 0, ny = 0;
@@ -93,6 +101,7 @@ struct Ctx {
 	MarkerRec* rec = nullptr;          // [2 * mcap]
 	int* bins = nullptr;               // counts | offsets | cursors | chunk sums, 2 comps
 	int* bin_list = nullptr;           // [2 * mcap]
+	BinRec* srec = nullptr;            // [2 * mcap], records in bin order
 	long mcap = 0;
 	int sbx = 0, sby = 0;              // spread bins per axis (4x4 base cells each)
 	int last_mg_cycles = 0;            // predicted batch size for the next solve
