@@ -1,9 +1,9 @@
 // afm_ns.hip -- the Navier-Stokes step around the two solves, on gfx950.
 //
-//   k_advect_rhs   conservative centred advection (ns/solver.h:204-234, the
-//                  2-D branch) fused with the right-hand-side assembly of the
-//                  backward-Euler half step or Crank-Nicolson full step
-//                  (ns/solver.h:105-135);
+//   k_advect_cg_init  conservative centred advection (ns/solver.h:204-234,
+//                  the 2-D branch) fused with the right-hand-side assembly of
+//                  the backward-Euler half step or Crank-Nicolson full step
+//                  (ns/solver.h:105-135) and the start of both CG solves;
 //   k_cg_*         CG on the periodic Helmholtz operator I + s L
 //                  (alg/cg.h:32-56, ns/helmholtz.h), both velocity components
 //                  in one launch (blockIdx.y = component), the stencil
@@ -37,37 +37,6 @@ __device__ inline void adv_at(const double* __restrict__ u, const double* __rest
 	const double uw_i = uw(i, x, z);
 	ua = n * ((uu(ixp, xP) - uu(i, x)) + dv + (uw(izp, x, zP) - uw_i));
 	wa = n * ((uw(ixp, xP, z) - uw_i) + dv + (ww(izp, zP) - ww(i, z)));
-}
-
-// be != 0: rx = u + (dt/2)((1/rho) fx - adv_u), u_old = u.
-// be == 0: rx = (I + s L) u_old + dt ((1/rho) fx - adv_u).
-__global__ __launch_bounds__(256) void k_advect_rhs(const double* __restrict__ u, const double* __restrict__ v,
-                                                    const double* __restrict__ fx, const double* __restrict__ fy,
-                                                    double* __restrict__ uo, double* __restrict__ vo,
-                                                    double* __restrict__ rx, double* __restrict__ ry, int nx, int ny,
-                                                    unsigned n, double dt, double rho, int be, double hoff, double hdg)
-{
-	const size_t N = (size_t) nx * ny;
-	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
-	if (i >= N) return;
-	const int x = (int) (i % nx), z = (int) (i / nx);
-	double ua, va;
-	adv_at(u, v, nx, ny, n, i, x, z, ua, va);
-	if (be) {
-		const double u0 = u[i], v0 = v[i];
-		uo[i] = u0;
-		vo[i] = v0;
-		rx[i] = u0 + (dt / 2) * (1 / rho * fx[i] - ua);
-		ry[i] = v0 + (dt / 2) * (1 / rho * fy[i] - va);
-	} else {
-		const size_t ie = x == nx - 1 ? i - (nx - 1) : i + 1, iw = x == 0 ? i + (nx - 1) : i - 1;
-		const size_t in_ = z == ny - 1 ? (size_t) x : i + nx, is = z == 0 ? i + N - nx : i - nx;
-		const bool we = x == nx - 1, ww = x == 0, wn = z == ny - 1, ws = z == 0;
-		double hu = stencil5<double, true>(hoff, hdg, uo[i], uo[ie], uo[iw], uo[in_], uo[is], we, ww, wn, ws);
-		double hv = stencil5<double, true>(hoff, hdg, vo[i], vo[ie], vo[iw], vo[in_], vo[is], we, ww, wn, ws);
-		rx[i] = hu + dt * (1 / rho * fx[i] - ua);
-		ry[i] = hv + dt * (1 / rho * fy[i] - va);
-	}
 }
 
 __global__ void k_advect(const double* __restrict__ u, const double* __restrict__ v, double* __restrict__ ua,
@@ -136,7 +105,8 @@ struct CGArgs {
 	const size_t in_ = gy == ny - 1 ? (size_t) gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;    \
 	const bool we = gx == nx - 1, ww = gx == 0, wn = gy == ny - 1, ws = gy == 0;
 
-// r = b - A x, p = r, n0 = r.r  (alg/cg.h:38-39); |b|^2 for a relative tol.
+// r = b - A x, n0 = r.r  (alg/cg.h:38-39); |b|^2 for a relative tol.  p = r
+// is not stored: the first p update reads r.
 __global__ __launch_bounds__(256) void k_cg_init(CGArgs a, int nx, int ny, double off, double dg)
 {
 	const int f = blockIdx.y;
@@ -150,7 +120,6 @@ __global__ __launch_bounds__(256) void k_cg_init(CGArgs a, int nx, int ny, doubl
 		double Ax = stencil5<double, true>(off, dg, x[i], x[ie], x[iw], x[in_], x[is], we, ww, wn, ws);
 		double r = b[i] - Ax;
 		a.r[f][i] = r;
-		a.p[f][0][i] = r;
 		acc[0] += r * r;
 		acc[1] += b[i] * b[i];
 	}
@@ -173,7 +142,7 @@ __global__ __launch_bounds__(256) void k_cg_init_fin(CGArgs a, unsigned nblk)
 	}
 }
 
-// p' = (count == 0) ? p : p*mu + r  (alg/cg.h:51-52 of the previous
+// p' = (count == 0) ? r : p*mu + r  (alg/cg.h:39 and 51-52 of the previous
 // iteration), stored; partial p'.(A p')
 __global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double off, double dg, int pin)
 {
@@ -191,7 +160,7 @@ __global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double o
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
 		CG_NEIGHBOURS
 		double pc, pe, pw, pn, ps;
-		if (first) { pc = p[i]; pe = p[ie]; pw = p[iw]; pn = p[in_]; ps = p[is]; }
+		if (first) { pc = r[i]; pe = r[ie]; pw = r[iw]; pn = r[in_]; ps = r[is]; }   // p = r
 		else {
 			pc = p[i] * mu + r[i]; pe = p[ie] * mu + r[ie]; pw = p[iw] * mu + r[iw];
 			pn = p[in_] * mu + r[in_]; ps = p[is] * mu + r[is];
@@ -269,6 +238,54 @@ __global__ __launch_bounds__(256) void k_sum_fin(unsigned nblk, SolveCtl* ctl, c
 	double tot[1];
 	red_total<1>(partials, nblk, tot);
 	if (threadIdx.x == 0) ctl->sum2 = tot[0];
+}
+
+// k_advect_rhs fused with k_cg_init for the two velocity solves of a step:
+// the right-hand side b = rx (or ry) is formed per cell and never stored;
+// r = b - A u with u the initial guess (ns/solver.h:105-135 then
+// alg/cg.h:38-39), partial sums of r.r and b.b per field.
+__global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* __restrict__ fx,
+                                                        const double* __restrict__ fy, double* __restrict__ uo,
+                                                        double* __restrict__ vo, int nx, int ny, unsigned n, double dt,
+                                                        double rho, int be, double hoff, double hdg, double off,
+                                                        double dg)
+{
+	const size_t N = (size_t) nx * ny;
+	const double* u = a.x[0];
+	const double* v = a.x[1];
+	double acc_u[2] = {0.0, 0.0}, acc_v[2] = {0.0, 0.0};
+	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
+		const int x = (int) (i % nx), z = (int) (i / nx);
+		double ua, va;
+		adv_at(u, v, nx, ny, n, i, x, z, ua, va);
+		const size_t ie = x == nx - 1 ? i - (nx - 1) : i + 1, iw = x == 0 ? i + (nx - 1) : i - 1;
+		const size_t in_ = z == ny - 1 ? (size_t) x : i + nx, is = z == 0 ? i + N - nx : i - nx;
+		const bool we = x == nx - 1, ww = x == 0, wn = z == ny - 1, ws = z == 0;
+		const double u0 = u[i], v0 = v[i];
+		double rx, ry;
+		if (be) {
+			uo[i] = u0;
+			vo[i] = v0;
+			rx = u0 + (dt / 2) * (1 / rho * fx[i] - ua);
+			ry = v0 + (dt / 2) * (1 / rho * fy[i] - va);
+		} else {
+			double hu = stencil5<double, true>(hoff, hdg, uo[i], uo[ie], uo[iw], uo[in_], uo[is], we, ww, wn, ws);
+			double hv = stencil5<double, true>(hoff, hdg, vo[i], vo[ie], vo[iw], vo[in_], vo[is], we, ww, wn, ws);
+			rx = hu + dt * (1 / rho * fx[i] - ua);
+			ry = hv + dt * (1 / rho * fy[i] - va);
+		}
+		const double Au = stencil5<double, true>(off, dg, u0, u[ie], u[iw], u[in_], u[is], we, ww, wn, ws);
+		const double Av = stencil5<double, true>(off, dg, v0, v[ie], v[iw], v[in_], v[is], we, ww, wn, ws);
+		const double r_u = rx - Au, r_v = ry - Av;
+		a.r[0][i] = r_u;
+		a.r[1][i] = r_v;
+		acc_u[0] += r_u * r_u;
+		acc_u[1] += rx * rx;
+		acc_v[0] += r_v * r_v;
+		acc_v[1] += ry * ry;
+	}
+	reduce_partial<2>(acc_u, a.partials[0]);
+	reduce_partial<2>(acc_v, a.partials[1]);
 }
 
 static inline unsigned rblocks(size_t N)
@@ -353,8 +370,17 @@ static int cg_alloc(Ctx* c, CGWork& w, std::vector<void*>& own)
 
 // Solves (I + scale L) x_f = b_f for the non-null fields; iters[f] receives
 // the iteration counts.
+// The right-hand side of a velocity step, formed inside the CG start instead
+// of being stored (ns_step).
+struct AdvectInit {
+	const double *fx, *fy;
+	double *uo, *vo;
+	double dt, rho, hoff, hdg;
+	int be;
+};
+
 int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* const b[2], double tol, int tol_mode,
-           int max_iters, int iters[2], double res[2])
+           int max_iters, int iters[2], double res[2], const AdvectInit* adv = nullptr)
 {
 	double off, dg;
 	helm_coef(c, scale, off, dg);
@@ -373,7 +399,11 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 	}
 	a.tol_mode = tol_mode;
 	const unsigned rb = rblocks(c->N);
-	hipLaunchKernelGGL(k_cg_init, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg);
+	if (adv)
+		hipLaunchKernelGGL(k_advect_cg_init, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy, adv->uo, adv->vo,
+		                   (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff, adv->hdg, off, dg);
+	else
+		hipLaunchKernelGGL(k_cg_init, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg);
 	hipLaunchKernelGGL(k_cg_init_fin, dim3(2), dim3(256), 0, c->stream, a, rb);
 	AFM_LAUNCH_CHECK();
 	int enq = 0;
@@ -413,7 +443,7 @@ struct NS {
 	double mu, rho, dt, tol;
 	int tol_mode, precision;
 	double *u, *v;                         // caller-owned, aliased (ns/solver.h:54-56)
-	double *uo = nullptr, *vo = nullptr, *rx = nullptr, *ry = nullptr, *phi = nullptr, *fz = nullptr;
+	double *uo = nullptr, *vo = nullptr, *phi = nullptr, *fz = nullptr;
 	CGWork cg;
 	bool half = false;
 	int last_cycles = 0, last_cg[2] = {0, 0};
@@ -442,7 +472,6 @@ int ns_create(Ctx* c, double mu, double rho, double dt, double tol, int tol_mode
 	int rc = 0;
 	std::vector<void*>& o = s->own;
 	if ((rc = dalloc(o, (void**) &s->uo, c->N * 8)) || (rc = dalloc(o, (void**) &s->vo, c->N * 8)) ||
-	    (rc = dalloc(o, (void**) &s->rx, c->N * 8)) || (rc = dalloc(o, (void**) &s->ry, c->N * 8)) ||
 	    (rc = dalloc(o, (void**) &s->phi, c->N * 8)) || (rc = cg_alloc(c, s->cg, o))) {
 		delete s;
 		return rc;
@@ -463,12 +492,11 @@ int ns_step(NS* s, const double* fx, const double* fy)
 	const double ss = -s->mu * s->dt / (2 * s->rho);     // phhs scale
 	double hoff, hdg;
 	helm_coef(c, sm, hoff, hdg);
-	hipLaunchKernelGGL(k_advect_rhs, pgrid(c->N), dim3(256), 0, c->stream, s->u, s->v, fx, fy, s->uo, s->vo, s->rx,
-	                   s->ry, (int) c->nx, (int) c->ny, c->n, s->dt, s->rho, s->half ? 0 : 1, hoff, hdg);
-	AFM_LAUNCH_CHECK();
+	// advection and the step's right-hand side are formed inside the CG start
+	const AdvectInit adv{fx, fy, s->uo, s->vo, s->dt, s->rho, hoff, hdg, s->half ? 0 : 1};
 	double* xs[2] = {s->u, s->v};
-	const double* bs[2] = {s->rx, s->ry};
-	AFM_TRY(op_cg2(c, s->cg, ss, xs, bs, s->tol, s->tol_mode, s->max_cg, s->last_cg, nullptr));
+	const double* bs[2] = {nullptr, nullptr};
+	AFM_TRY(op_cg2(c, s->cg, ss, xs, bs, s->tol, s->tol_mode, s->max_cg, s->last_cg, nullptr, &adv));
 	AFM_TRY(ns_project(s, s->u, s->v));
 	s->half = !s->half;
 	return 0;
