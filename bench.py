@@ -49,6 +49,12 @@ def parse():
     p.add_argument("--forms", default="fp64,mixed")
     p.add_argument("--no-extras", action="store_true", help="skip the smoother, transfer and solver-only measurements")
     p.add_argument("--kernel-only", action="store_true", help="a few mixed steps and a calibration copy (counter runs)")
+    p.add_argument("--grad", action="store_true",
+                   help="only the differentiation check: 1024^2, 32 membranes (8,192 markers), 200 steps, "
+                        "checkpoints every 20, FD along 32 directions")
+    p.add_argument("--stress", action="store_true",
+                   help="only the combined stress case: 4096^2, 262,144 clustered markers, mixed precision, "
+                        "50 steps with checkpoints every 10, dt 1e-5, FD along 4 directions")
     return p.parse_args()
 
 
@@ -208,6 +214,52 @@ def solver_only(A, W, dev, n=4096, reps=3):
     return res
 
 
+def gradient_check(A, W, dev, n, structures, msize, steps, every, precision, dt, mu, ndir, tol, label):
+    """dJ/dk and dJ/dR by the adjoint (checkpoint every `every` steps) against
+    central finite differences, h = 1e-6, along `ndir` seeded directions;
+    J = 1/2 sum |u(T)|^2 h^2 + sum over structures |c(T) - c(0)|^2."""
+    import numpy as np
+    import torch
+    from afmibins_amd.autograd import IBSetup, run
+    X, R, k, ds = structures
+    u0, v0 = W.fourier_velocity(n, 8, seed=3)
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+    ctx = A.Context(n, 0)
+    st = IBSetup(ctx, t(u0), t(v0), t(X), t(ds), mu, 1.0, dt, tol, steps, msize, every, precision)
+    kt, Rt = t(k), t(R)
+    run(st, kt, Rt, False)                      # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, gk, gR = run(st, kt, Rt, True)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    run(st, kt, Rt, False)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    gk, gR = gk.cpu().numpy(), gR.cpu().numpy()
+    print(f"[bench] {label}: forward {t_fwd:.2f} s, forward + adjoint {t_grad:.2f} s", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(17)
+    h, errs = 1e-6, []
+    for _ in range(ndir):
+        dk, dR = rng.standard_normal(k.shape), rng.standard_normal(R.shape)
+        Jp = run(st, t(k + h * dk), t(R + h * dR), False)[0]
+        Jm = run(st, t(k - h * dk), t(R - h * dR), False)[0]
+        fd = (Jp - Jm) / (2 * h)
+        ad = float(gk @ dk + np.sum(gR * dR))
+        errs.append(abs(fd - ad) / abs(ad))
+        print(f"[bench] FD direction {len(errs)}: fd {fd:.12e} adjoint {ad:.12e} rel {errs[-1]:.2e}",
+              file=sys.stderr, flush=True)
+    ctx.close()
+    m = X.shape[0]
+    return {"case": label, "grid": n, "markers": m, "steps": steps, "checkpoint_every": every,
+            "precision": "mixed" if precision else "fp64", "solver_tol_rel": tol, "J": J,
+            "forward_s": round(t_fwd, 3), "forward_steps_per_s": round(steps / t_fwd, 2),
+            "forward_plus_adjoint_s": round(t_grad, 3), "adjoint_over_forward": round(t_grad / t_fwd, 2),
+            "fd_directions": ndir, "fd_h": h, "fd_rel_err_max": max(errs), "fd_rel_err_median": float(np.median(errs)),
+            "grad_k_norm": float(np.linalg.norm(gk)), "grad_R_norm": float(np.linalg.norm(gR))}
+
+
 def kernel_only(A, W, dev):
     """For counter runs: three mixed-form steps of the benchmark workload and
     a plain 256 MiB copy (the calibration kernel for FETCH_SIZE / WRITE_SIZE)."""
@@ -251,6 +303,16 @@ def main():
     dev = torch.device("cuda", 0)
     if args.kernel_only:
         kernel_only(A, W, dev)
+        return
+    if args.grad:
+        X, R, k, ds = W.membranes(32, 256, seed=2)
+        print(json.dumps(gradient_check(A, W, dev, 1024, (X, R, k, ds), 256, 200, 20, 0, 2e-5, 1e-3, 32, 1e-12,
+                                        "differentiation: 1024^2, 32 membranes x 256")), flush=True)
+        return
+    if args.stress:
+        print(json.dumps(gradient_check(A, W, dev, 4096, W.clustered_structures(), 256, 50, 10, 1, 1e-5, 1e-3, 4,
+                                        1e-10, "stress: 4096^2, 512 membranes + 512 filaments x 256, 8 clusters")),
+              flush=True)
         return
     n = args.n
     inputs = W.config2_inputs(n=n)
