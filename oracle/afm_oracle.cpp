@@ -893,9 +893,10 @@ void ib_step(IB2& s)
 // Runs `steps` IB steps in place.  u, v: nx*ny each (projected on entry, as
 // the ns::solver constructor does, ns/solver.h:71); X, R: 2m; k, ds: m.
 // Returns total V-cycles; *cg_iters receives total CG iterations.
+// p_out (optional): the potential phi of the last projection (ns/projection.h:69).
 unsigned or_ib_run(unsigned nx, unsigned ny, unsigned n, double mu, double rho, double dt, double tol, int tol_mode,
                    int smoother, int delta_mode, double* u, double* v, long m, double* X, const double* R,
-                   const double* k, const double* ds, int steps, unsigned* cg_iters)
+                   const double* k, const double* ds, int steps, unsigned* cg_iters, double* p_out)
 {
 	IB2 s;
 	s.g = G2{n, nx, ny, nx / n, ny / n};
@@ -913,6 +914,7 @@ unsigned or_ib_run(unsigned nx, unsigned ny, unsigned n, double mu, double rho, 
 	std::copy(s.u.begin(), s.u.end(), u);
 	std::copy(s.v.begin(), s.v.end(), v);
 	std::copy(s.X.begin(), s.X.end(), X);
+	if (p_out) std::copy(ns.p_last.begin(), ns.p_last.end(), p_out);
 	if (cg_iters) *cg_iters = s.cg_iters;
 	return s.cycles;
 }

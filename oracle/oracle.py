@@ -75,7 +75,7 @@ def lib():
             "or_div2": (None, [_U, _U, _U, _P, _P, _P]),
             "or_grad2": (None, [_U, _U, _U, _P, _P, _P]),
             "or_advect2": (None, [_U, _U, _U, _P, _P, _P, _P]),
-            "or_ib_run": (_U, [_U, _U, _U, _D, _D, _D, _D, _I, _I, _I, _P, _P, _L, _P, _P, _P, _P, _I, _P]),
+            "or_ib_run": (_U, [_U, _U, _U, _D, _D, _D, _D, _I, _I, _I, _P, _P, _L, _P, _P, _P, _P, _I, _P, _P]),
             "or_last_step_seconds": (_D, []),
             "or_ib_grad": (_D, [_U, _U, _U, _D, _D, _D, _D, _I, _I, _I, _P, _P, _L, _P, _P, _P, _P, _L, _I, _P,
                                 _P]),
@@ -289,15 +289,19 @@ def ns_run(u, v, fx, fy, n, mu, rho, dt, tol, tol_mode=1, smoother=1, calls=2):
     return u, v, p, cyc
 
 
-def ib_run(u, v, X, R, k, ds, n, mu, rho, dt, tol, steps, tol_mode=1, smoother=1, delta_mode=1):
+def ib_run(u, v, X, R, k, ds, n, mu, rho, dt, tol, steps, tol_mode=1, smoother=1, delta_mode=1, want_phi=False):
+    """IB steps; returns (u, v, X, cycles, cg_iters) and, with want_phi, the
+    potential phi of the last projection as a sixth value."""
     u, v = _f64(u).copy(), _f64(v).copy()
     X = _f64(X).copy()
     R, k, ds = _f64(R), _f64(k), _f64(ds)
     ny, nx = u.shape
     cgi = C.c_uint(0)
+    phi = np.zeros_like(u) if want_phi else None
     cyc = lib().or_ib_run(nx, ny, n, mu, rho, dt, tol, tol_mode, smoother, delta_mode, _ptr(u), _ptr(v),
-                          X.shape[0], _ptr(X), _ptr(R), _ptr(k), _ptr(ds), steps, C.byref(cgi))
-    return u, v, X, cyc, cgi.value
+                          X.shape[0], _ptr(X), _ptr(R), _ptr(k), _ptr(ds), steps, C.byref(cgi),
+                          _ptr(phi) if want_phi else None)
+    return (u, v, X, cyc, cgi.value, phi) if want_phi else (u, v, X, cyc, cgi.value)
 
 
 def ib_grad(u, v, X, R, k, ds, n, mu, rho, dt, tol, steps, msize, tol_mode=1, smoother=1, delta_mode=1,

@@ -302,3 +302,46 @@ def test_ib_step_is_run_to_run_reproducible(afm, dev):
         ib.close(); ns.close(); ctx.close()
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
+
+
+_REF_CASE = {}
+
+
+def _reference_case_oracle(oracle, n, steps, mu, rho, dt, tol):
+    # the oracle run (about 20 s on one core) is shared by both precisions
+    key = (n, steps, mu, rho, dt, tol)
+    if key not in _REF_CASE:
+        X, R, k, ds = W.single_ellipse(512)
+        u0 = np.zeros((n, n))
+        _REF_CASE[key] = oracle.ib_run(u0, u0, X, R, k, ds, n, mu, rho, dt, tol, steps, tol_mode=1, smoother=1,
+                                       want_phi=True)
+    return _REF_CASE[key]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_reference_cpu_case_matches_oracle(afm, oracle, dev, precision):
+    # the CPU reference configuration: 128^2, one ellipse of 512 markers
+    # (semi-axes 0.20 / 0.15) tethered with k = 1 to the circle r = 0.1732,
+    # fluid at rest, rho = 1, mu = 0.01, dt = 1e-4, 200 steps, V(2,2) to 1e-10
+    n, steps, mu, rho, dt, tol = 128, 200, 0.01, 1.0, 1e-4, 1e-10
+    X, R, k, ds = W.single_ellipse(512)
+    u0 = np.zeros((n, n))
+    ctx = afm.Context(n, 0)
+    u, v = T(u0, dev), T(u0, dev)
+    ns = afm.NSSolver(ctx, mu, rho, dt, tol, u, v, tol_mode=1, precision=precision)
+    ib = afm.IBSolver(ns, T(X, dev), T(R, dev), T(k, dev), T(ds, dev))
+    ib.step(steps)
+    p = H(ns.get_p())
+    ctx.sync()
+    uo, vo, Xo, _, _, phi = _reference_case_oracle(oracle, n, steps, mu, rho, dt, tol)
+    # p = (I - mu dt / (2 rho) L) phi  (ns/solver.h:86), means removed
+    lap = n * n * (np.roll(phi, 1, 0) + np.roll(phi, -1, 0) + np.roll(phi, 1, 1) + np.roll(phi, -1, 1) - 4 * phi)
+    po = phi - mu * dt / (2 * rho) * lap
+    sc = max(np.abs(uo).max(), np.abs(vo).max())
+    du = max(np.abs(H(u) - uo).max(), np.abs(H(v) - vo).max()) / sc
+    dX = np.abs(H(ib.markers()) - Xo).max()
+    dp = np.abs((p - p.mean()) - (po - po.mean())).max() / np.abs(po - po.mean()).max()
+    assert du <= (1e-9 if precision == 0 else 1e-7), du
+    assert dX <= (1e-11 if precision == 0 else 1e-9), dX
+    assert dp <= (1e-8 if precision == 0 else 1e-6), dp
+    ib.close(); ns.close(); ctx.close()
