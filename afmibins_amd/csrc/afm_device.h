@@ -147,6 +147,30 @@ __device__ inline void ib_window(int comp, double cx, double cy, unsigned n, uns
 	}
 }
 
+// One factor of ib_window: lane d8 < 4 returns dx[d8], lane 4 + d returns
+// dy[d]; every lane returns the base cell.  Same operations as ib_window.
+__device__ inline double ib_window_factor(int comp, double cx, double cy, unsigned n, unsigned W, unsigned D,
+                                          unsigned nx, unsigned ny, int d8, int& ix0, int& iy0)
+{
+	double q0 = cx, q1 = cy;
+	if (comp == 0) q1 -= 0.5 / n; else q0 -= 0.5 / n;
+	q0 = fmod(fmod(q0, (double) W) + W, (double) W);
+	q1 = fmod(fmod(q1, (double) D) + D, (double) D);
+	double fx = floor(q0 * n), fy = floor(q1 * n);
+	if (!(fx >= 0.0 && fx <= (double) nx)) fx = 0.0;
+	if (!(fy >= 0.0 && fy <= (double) ny)) fy = 0.0;
+	ix0 = (int) (((size_t) fx + nx - 1) % nx);
+	iy0 = (int) (((size_t) fy + ny - 1) % ny);
+	if (d8 < 4) {
+		double x = (fx + d8 - 1.0) / n;
+		if (comp != 0) x += 0.5 / n;
+		return delta(x - cx, n);
+	}
+	double y = (fy + (d8 - 4) - 1.0) / n;
+	if (comp == 0) y += 0.5 / n;
+	return delta(y - cy, n);
+}
+
 // sum_taps (w * vol) * field[idx], taps in the reference order (x fastest).
 __device__ inline double ib_gather(const double* __restrict__ f, unsigned nx, unsigned ny, int ix0, int iy0,
                                    const double dx[4], const double dy[4], double vol)

@@ -39,70 +39,78 @@ __global__ void k_interp(int comp, const double* __restrict__ f, const double* _
 	U[k] = ib_gather(f, nx, ny, ix0, iy0, dx, dy, vol);
 }
 
-// X_h = X + half * (J_u(X) u, J_v(X) v)
+// X_h = X + half * (J_u(X) u, J_v(X) v).  The windows are the ones the
+// spreading of the same step stored for X (rec[comp * m + k]): the same
+// delta factors, not computed twice.
 __global__ void k_ib_move_half(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ X,
-                               long m, unsigned n, unsigned W, unsigned D, unsigned nx, unsigned ny, double half,
-                               double* __restrict__ Xh)
+                               const MarkerRec* __restrict__ rec, long m, unsigned n, unsigned nx, unsigned ny,
+                               double half, double* __restrict__ Xh)
 {
 	long k = blockIdx.x * (long) blockDim.x + threadIdx.x;
 	if (k >= m) return;
-	const double cx = X[2 * k], cy = X[2 * k + 1];
 	const double vol = 1. / (n * n);
-	int ix0, iy0;
-	double dx[4], dy[4];
-	ib_window(0, cx, cy, n, W, D, nx, ny, ix0, iy0, dx, dy);
-	double Ux = ib_gather(u, nx, ny, ix0, iy0, dx, dy, vol);
-	ib_window(1, cx, cy, n, W, D, nx, ny, ix0, iy0, dx, dy);
-	double Uy = ib_gather(v, nx, ny, ix0, iy0, dx, dy, vol);
-	Xh[2 * k] = cx + half * Ux;
-	Xh[2 * k + 1] = cy + half * Uy;
+	const MarkerRec& a = rec[k];
+	const MarkerRec& b = rec[m + k];
+	const double Ux = ib_gather(u, nx, ny, a.ix0, a.iy0, a.dx, a.dy, vol);
+	const double Uy = ib_gather(v, nx, ny, b.ix0, b.iy0, b.dx, b.dy, vol);
+	Xh[2 * k] = X[2 * k] + half * Ux;
+	Xh[2 * k + 1] = X[2 * k + 1] + half * Uy;
 }
 
-// X = X + dt * (0.5 * (J(X_h) u_old + J(X_h) u))
+// X = X + dt * (0.5 * (J(X_h) u_old + J(X_h) u)), windows of X_h from the
+// step's second spreading.
 __global__ void k_ib_move_full(const double* __restrict__ uo, const double* __restrict__ vo,
                                const double* __restrict__ u, const double* __restrict__ v,
-                               const double* __restrict__ Xh, long m, unsigned n, unsigned W, unsigned D,
-                               unsigned nx, unsigned ny, double dt, double* __restrict__ X)
+                               const MarkerRec* __restrict__ rec, long m, unsigned n, unsigned nx, unsigned ny,
+                               double dt, double* __restrict__ X)
 {
 	long k = blockIdx.x * (long) blockDim.x + threadIdx.x;
 	if (k >= m) return;
-	const double cx = Xh[2 * k], cy = Xh[2 * k + 1];
 	const double vol = 1. / (n * n);
-	int ix0, iy0;
-	double dx[4], dy[4];
-	ib_window(0, cx, cy, n, W, D, nx, ny, ix0, iy0, dx, dy);
-	double U0x = ib_gather(uo, nx, ny, ix0, iy0, dx, dy, vol);
-	double U1x = ib_gather(u, nx, ny, ix0, iy0, dx, dy, vol);
-	ib_window(1, cx, cy, n, W, D, nx, ny, ix0, iy0, dx, dy);
-	double U0y = ib_gather(vo, nx, ny, ix0, iy0, dx, dy, vol);
-	double U1y = ib_gather(v, nx, ny, ix0, iy0, dx, dy, vol);
+	const MarkerRec& a = rec[k];
+	const MarkerRec& b = rec[m + k];
+	const double U0x = ib_gather(uo, nx, ny, a.ix0, a.iy0, a.dx, a.dy, vol);
+	const double U1x = ib_gather(u, nx, ny, a.ix0, a.iy0, a.dx, a.dy, vol);
+	const double U0y = ib_gather(vo, nx, ny, b.ix0, b.iy0, b.dx, b.dy, vol);
+	const double U1y = ib_gather(v, nx, ny, b.ix0, b.iy0, b.dx, b.dy, vol);
 	X[2 * k] = X[2 * k] + dt * (0.5 * (U0x + U1x));
 	X[2 * k + 1] = X[2 * k + 1] + dt * (0.5 * (U0y + U1y));
 }
 
-// Per (marker, component): window, source value and bin.  With R/k given,
-// the source is the tether force F = -k (X - R); otherwise F[comp][k].
+// Per (marker, component): window, source value and bin, eight lanes per
+// pair: lane d < 4 evaluates the x factor dx[d], lane 4 + d the y factor
+// dy[d] (ib_window's operations, one factor each), lane 0 the rest.  With
+// R/k given, the source is the tether force F = -k (X - R); otherwise
+// F[comp][k].
 __global__ void k_spread_prep(int comp_mask, const double* __restrict__ Fx, const double* __restrict__ Fy,
                               const double* __restrict__ ds, const double* __restrict__ X,
                               const double* __restrict__ R, const double* __restrict__ kk, long m,
                               unsigned n, unsigned W, unsigned D, unsigned nx, unsigned ny, int sbx, int nb1,
                               MarkerRec* __restrict__ rec, int* __restrict__ counts)
 {
-	long k = blockIdx.x * (long) blockDim.x + threadIdx.x;
-	int comp = blockIdx.y;
+	const long t = blockIdx.x * (long) blockDim.x + threadIdx.x;
+	const long k = t >> 3;
+	const int d8 = (int) (t & 7);
+	const int comp = blockIdx.y;
 	if (k >= m || !((comp_mask >> comp) & 1)) return;
 	const double cx = X[2 * k], cy = X[2 * k + 1];
-	MarkerRec r;
-	ib_window(comp, cx, cy, n, W, D, nx, ny, r.ix0, r.iy0, r.dx, r.dy);
-	double F;
-	if (R) F = comp == 0 ? -kk[k] * (cx - R[2 * k]) : -kk[k] * (cy - R[2 * k + 1]);
-	else F = comp == 0 ? Fx[k] : Fy[k];
-	r.ds = ds[k];
-	r.F = F;
-	r.tile = r.ix0 / SB + sbx * (r.iy0 / SB);
-	r.pad = 0;
-	rec[comp * m + k] = r;
-	atomicAdd(&counts[comp * nb1 + r.tile], 1);
+	int ix0, iy0;
+	const double f = ib_window_factor(comp, cx, cy, n, W, D, nx, ny, d8, ix0, iy0);
+	MarkerRec& r = rec[comp * m + k];
+	if (d8 < 4) r.dx[d8] = f;
+	else r.dy[d8 - 4] = f;
+	if (d8 == 0) {
+		double F;
+		if (R) F = comp == 0 ? -kk[k] * (cx - R[2 * k]) : -kk[k] * (cy - R[2 * k + 1]);
+		else F = comp == 0 ? Fx[k] : Fy[k];
+		r.ix0 = ix0;
+		r.iy0 = iy0;
+		r.ds = ds[k];
+		r.F = F;
+		r.tile = ix0 / SB + sbx * (iy0 / SB);
+		r.pad = 0;
+		atomicAdd(&counts[comp * nb1 + r.tile], 1);
+	}
 }
 
 // Exclusive scan of the bin counts in two launches: chunk sums (block c
@@ -279,8 +287,8 @@ int launch_interp(Ctx* c, int comp, const double* field, const double* X, long m
 int launch_ib_move_half(Ctx* c, const double* u, const double* v, const double* X, long m, double half, double* Xh)
 {
 	if (m <= 0) return 0;
-	hipLaunchKernelGGL(k_ib_move_half, dim3((m + 255) / 256), dim3(256), 0, c->stream, u, v, X, m, c->n, c->W, c->D,
-	                   c->nx, c->ny, half, Xh);
+	hipLaunchKernelGGL(k_ib_move_half, dim3((m + 255) / 256), dim3(256), 0, c->stream, u, v, X, c->rec, m, c->n, c->nx,
+	                   c->ny, half, Xh);
 	AFM_LAUNCH_CHECK();
 	return 0;
 }
@@ -289,8 +297,9 @@ int launch_ib_move_full(Ctx* c, const double* uo, const double* vo, const double
                         const double* Xh, long m, double dt, double* X)
 {
 	if (m <= 0) return 0;
-	hipLaunchKernelGGL(k_ib_move_full, dim3((m + 255) / 256), dim3(256), 0, c->stream, uo, vo, u, v, Xh, m, c->n,
-	                   c->W, c->D, c->nx, c->ny, dt, X);
+	(void) Xh;   // its windows are in c->rec
+	hipLaunchKernelGGL(k_ib_move_full, dim3((m + 255) / 256), dim3(256), 0, c->stream, uo, vo, u, v, c->rec, m, c->n,
+	                   c->nx, c->ny, dt, X);
 	AFM_LAUNCH_CHECK();
 	return 0;
 }
@@ -324,8 +333,8 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 	if (m > 0) {
 		AFM_TRY(ensure_markers(c, m));
 		AFM_HIP(hipMemsetAsync(counts, 0, sizeof(int) * nb, c->stream));
-		dim3 g((m + 255) / 256, 2);
-		hipLaunchKernelGGL(k_spread_prep, g, dim3(256), 0, c->stream, comp_mask, Fx, Fy, ds, X, R, k, m, c->n, c->W,
+		dim3 g((m + 255) / 256, 2), g8((8 * m + 255) / 256, 2);
+		hipLaunchKernelGGL(k_spread_prep, g8, dim3(256), 0, c->stream, comp_mask, Fx, Fy, ds, X, R, k, m, c->n, c->W,
 		                   c->D, c->nx, c->ny, c->sbx, nb1, c->rec, counts);
 		AFM_LAUNCH_CHECK();
 		hipLaunchKernelGGL(k_spread_chunk, dim3(nchunk), dim3(256), 0, c->stream, counts, chunk, nb, items);
