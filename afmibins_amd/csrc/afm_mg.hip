@@ -425,8 +425,10 @@ __global__ __launch_bounds__(64 * SmoothShape<S>::NW) void k_smooth_update(
 // ------------------------------------------------------ residual+restrict --
 // RESID: coarse[c] = sum_taps (w/64) * (b - L x)[fine]; else sum of x itself.
 // With cp != nullptr the fine input is also copied to cp (nested iteration).
-template<typename T, bool RESID>
-__global__ __launch_bounds__(256) void k_resid_restr(const T* __restrict__ x, const T* __restrict__ b,
+// Without RESID the fine input may be FP64 for an FP32 restriction (TS):
+// each value is rounded once on load, and cp receives the rounded copy.
+template<typename T, bool RESID, typename TS = T>
+__global__ __launch_bounds__(256) void k_resid_restr(const TS* __restrict__ x, const T* __restrict__ b,
                                                      T* __restrict__ rc, T* __restrict__ cp, int nx, int ny,
                                                      T off, T dg, const int* __restrict__ done)
 {
@@ -451,7 +453,7 @@ __global__ __launch_bounds__(256) void k_resid_restr(const T* __restrict__ x, co
 			int gx = F0x + p % FX, gy = F0y + p / FX;
 			gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
 			gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
-			bv[k] = RESID ? b[gx + (size_t) nx * gy] : x[gx + (size_t) nx * gy];
+			bv[k] = RESID ? b[gx + (size_t) nx * gy] : (T) x[gx + (size_t) nx * gy];
 		}
 	}
 	if (RESID) {
@@ -503,7 +505,7 @@ __global__ __launch_bounds__(256) void k_resid_restr(const T* __restrict__ x, co
 		// each fine cell of the tile's own 2x2 blocks, copied once
 		for (int p = tid; p < 4 * RTX * RTY; p += 256) {
 			int fx = 2 * C0x + p % (2 * RTX), fy = 2 * C0y + p / (2 * RTX);
-			cp[fx + (size_t) nx * fy] = x[fx + (size_t) nx * fy];
+			cp[fx + (size_t) nx * fy] = (T) x[fx + (size_t) nx * fy];
 		}
 	}
 }
@@ -529,7 +531,9 @@ __global__ __launch_bounds__(256) void k_update_resid(const double* __restrict__
 		const size_t iw = gx == 0 ? i + (nx - 1) : i - 1;
 		const size_t in_ = gy == ny - 1 ? gx : i + nx;
 		const size_t is = gy == 0 ? i + N - nx : i - nx;
-		double c = xs[i], E = xs[ie], Wv = xs[iw], Nv = xs[in_], Sv = xs[is];
+		// xs == nullptr: the iterate is zero and e (below) is the whole start
+		double c = 0.0, E = 0.0, Wv = 0.0, Nv = 0.0, Sv = 0.0;
+		if (xs) { c = xs[i]; E = xs[ie]; Wv = xs[iw]; Nv = xs[in_]; Sv = xs[is]; }
 		if (e) {
 			c = c + (double) e[i]; E = E + (double) e[ie]; Wv = Wv + (double) e[iw];
 			Nv = Nv + (double) e[in_]; Sv = Sv + (double) e[is];
@@ -1053,6 +1057,17 @@ static int vcycle(Ctx* c, unsigned l, const T* rhs_l, int nu1, int nu2, const in
 	return 0;
 }
 
+// rc = R x at level l-1 in FP32 from an FP64 fine field; cp = x rounded.
+static int restrict_cvt_launch(Ctx* c, unsigned l, const double* x, float* rc, float* cp)
+{
+	const LevelBuf& L = c->lv[l];
+	dim3 g(((L.nx / 2) / RTX) * ((L.ny / 2) / RTY));
+	hipLaunchKernelGGL((k_resid_restr<float, false, double>), g, dim3(256), 0, c->stream, x, (const float*) nullptr, rc,
+	                   cp, (int) L.nx, (int) L.ny, 0.f, 0.f, (const int*) nullptr);
+	AFM_LAUNCH_CHECK();
+	return 0;
+}
+
 // nested(T, rhs) -> XSa (mg/solver.h:183-197 unrolled over the levels).
 static int nested(Ctx* c, double* rhs, int nu1, int nu2)
 {
@@ -1070,6 +1085,30 @@ static int nested(Ctx* c, double* rhs, int nu1, int nu2)
 		const double* cpk = (k == T) ? rhs : c->lv[k].CP;
 		AFM_TRY(smooth_n<double>(c, k, M_PSET, nu2, nullptr, cpk, c->lv[k - 1].Xb, dst, c->lv[k].Xa, nullptr));
 		if (k < T) AFM_TRY(vcycle<double>(c, k, c->lv[k].B, nu1, nu2, nullptr));
+	}
+	return 0;
+}
+
+// The mixed form's start (levels above the LDS ones): the same nested
+// iteration in FP32.  A start only has to be close; the FP64 refinement
+// that follows sets the accuracy.  rhs is rounded once, in the first
+// restriction, which keeps the rounded copy as the finest right-hand side;
+// the level copies (CP) reuse the FP64 buffers.  Result: FP32, finest
+// level, in RES32b.
+static int nested32(Ctx* c, const double* rhs, int nu1, int nu2)
+{
+	const unsigned T = c->levels, lc = c->lc;
+	float* bT = c->lv[T].B32;
+	auto cp32 = [c](unsigned k) { return reinterpret_cast<float*>(c->lv[k].CP); };
+	AFM_TRY(restrict_cvt_launch(c, T, rhs, c->lv[T - 1].B32, bT));
+	for (unsigned k = T - 1; k > lc; --k)
+		AFM_TRY(resid_restrict_launch<float>(c, k, false, c->lv[k].B32, nullptr, c->lv[k - 1].B32, cp32(k), nullptr));
+	AFM_TRY(coarse_launch<float>(c, 1, (int) lc, c->lv[lc].B32, c->lv[lc].Xb32, nu1, nu2, nullptr));
+	for (unsigned k = lc + 1; k <= T; ++k) {
+		float* dst = (k == T) ? c->RES32b : c->lv[k].B32;
+		const float* cpk = (k == T) ? bT : cp32(k);
+		AFM_TRY(smooth_n<float>(c, k, M_PSET, nu2, nullptr, cpk, c->lv[k - 1].Xb32, dst, c->lv[k].Xa32, nullptr));
+		if (k < T) AFM_TRY(vcycle<float>(c, k, c->lv[k].B32, nu1, nu2, nullptr));
 	}
 	return 0;
 }
@@ -1141,14 +1180,23 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 		hipLaunchKernelGGL(k_mean_fin, dim3(1), dim3(256), 0, c->stream, pass, tol, rb, N, dctl, c->partials);
 		AFM_LAUNCH_CHECK();
 	}
-	AFM_TRY(nested(c, rhs, nu1, nu2));
-	// residual of the nested start, mean removed from rhs in place
-	if (precision == 0)
-		hipLaunchKernelGGL((k_update_resid<double, double>), dim3(rb), dim3(256), 0, c->stream, c->XSa,
-		                   (const double*) nullptr, rhs, c->XSa, c->RES, (int) c->nx, (int) c->ny, off, dg, 1, 1, dctl, c->partials);
-	else
-		hipLaunchKernelGGL((k_update_resid<float, float>), dim3(rb), dim3(256), 0, c->stream, c->XSa,
-		                   (const float*) nullptr, rhs, c->XSa, c->RES32, (int) c->nx, (int) c->ny, off, dg, 1, 1, dctl, c->partials);
+	// nested start and its residual, mean removed from rhs in place
+	if (precision == 1 && T > c->lc) {
+		AFM_TRY(nested32(c, rhs, nu1, nu2));
+		hipLaunchKernelGGL((k_update_resid<float, float>), dim3(rb), dim3(256), 0, c->stream, (const double*) nullptr,
+		                   (const float*) c->RES32b, rhs, c->XSa, c->RES32, (int) c->nx, (int) c->ny, off, dg, 1, 1,
+		                   dctl, c->partials);
+	} else {
+		AFM_TRY(nested(c, rhs, nu1, nu2));
+		if (precision == 0)
+			hipLaunchKernelGGL((k_update_resid<double, double>), dim3(rb), dim3(256), 0, c->stream, c->XSa,
+			                   (const double*) nullptr, rhs, c->XSa, c->RES, (int) c->nx, (int) c->ny, off, dg, 1, 1, dctl,
+			                   c->partials);
+		else
+			hipLaunchKernelGGL((k_update_resid<float, float>), dim3(rb), dim3(256), 0, c->stream, c->XSa,
+			                   (const float*) nullptr, rhs, c->XSa, c->RES32, (int) c->nx, (int) c->ny, off, dg, 1, 1,
+			                   dctl, c->partials);
+	}
 	hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, 1, rb, N, dctl, c->partials);
 	AFM_LAUNCH_CHECK();
 	const int* done = &dctl->done;
