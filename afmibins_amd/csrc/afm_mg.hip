@@ -332,29 +332,36 @@ struct Strip {
 	}
 };
 
-template<int S> struct SmoothShape;
-template<> struct SmoothShape<1> { static constexpr int NW = 6, RPW = 6; };
-template<> struct SmoothShape<2> { static constexpr int NW = 4, RPW = 10; };
+// Waves per block and rows per wave.  SH = 0 for the large levels (fewer
+// halo rows per output row); SH = 1 for levels of at most SMALL_CELLS cells,
+// whose launches are few blocks on an idle chip: there a block's serial
+// chain of row updates sets the time, so each wave holds fewer rows.
+template<int S, int SH = 0> struct SmoothShape;
+template<> struct SmoothShape<1, 0> { static constexpr int NW = 6, RPW = 6; };
+template<> struct SmoothShape<2, 0> { static constexpr int NW = 4, RPW = 10; };
+template<> struct SmoothShape<1, 1> { static constexpr int NW = 4, RPW = 4; };
+template<> struct SmoothShape<2, 1> { static constexpr int NW = 6, RPW = 4; };
+constexpr size_t SMALL_CELLS = (size_t) 512 * 512;
 
 // halo of a launch: 2 cells per sweep, 2 more for a residual epilogue
 template<int S, int EPI> constexpr int smooth_halo() { return 2 * S + (EPI != EPI_STORE ? 2 : 0); }
 template<int S, int EPI> constexpr int smooth_txo() { return 128 - 2 * smooth_halo<S, EPI>(); }
-template<int S, int EPI>
-constexpr int smooth_tyo() { return SmoothShape<S>::NW * SmoothShape<S>::RPW - 2 * smooth_halo<S, EPI>(); }
+template<int S, int EPI, int SH = 0>
+constexpr int smooth_tyo() { return SmoothShape<S, SH>::NW * SmoothShape<S, SH>::RPW - 2 * smooth_halo<S, EPI>(); }
 
-template<int S, int EPI>
+template<int S, int EPI, int SH = 0>
 static inline unsigned smooth_blocks(int nx, int ny)
 {
-	constexpr int TXO = smooth_txo<S, EPI>(), TYO = smooth_tyo<S, EPI>();
+	constexpr int TXO = smooth_txo<S, EPI>(), TYO = smooth_tyo<S, EPI, SH>();
 	return (unsigned) (((nx + TXO - 1) / TXO) * ((ny + TYO - 1) / TYO));
 }
 
 // tile origin of this block and whether its region wraps
-template<int S, int EPI>
+template<int S, int EPI, int SH = 0>
 __device__ inline bool smooth_tile_origin(int nx, int ny, int& X0, int& Y0)
 {
-	constexpr int H = smooth_halo<S, EPI>(), TXO = smooth_txo<S, EPI>(), TYO = smooth_tyo<S, EPI>();
-	constexpr int LY = SmoothShape<S>::NW * SmoothShape<S>::RPW;
+	constexpr int H = smooth_halo<S, EPI>(), TXO = smooth_txo<S, EPI>(), TYO = smooth_tyo<S, EPI, SH>();
+	constexpr int LY = SmoothShape<S, SH>::NW * SmoothShape<S, SH>::RPW;
 	const int ntx = (nx + TXO - 1) / TXO;
 	const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
 	X0 = (int) (tile % ntx) * TXO;
@@ -362,12 +369,12 @@ __device__ inline bool smooth_tile_origin(int nx, int ny, int& X0, int& Y0)
 	return X0 - H < 0 || X0 - H + 128 > nx || Y0 - H < 0 || Y0 - H + LY > ny;
 }
 
-template<typename T, int S, int MODE, int EPI, bool EDGE>
+template<typename T, int S, int MODE, int EPI, bool EDGE, int SH>
 __device__ inline void smooth_body(const T* __restrict__ xin, const T* __restrict__ b, const T* __restrict__ xc,
                                    T* __restrict__ xout, T* __restrict__ rc, int nx, int ny, T off, T dg, T inv,
                                    int X0, int Y0, T* bnd)
 {
-	constexpr int NW = SmoothShape<S>::NW, RPW = SmoothShape<S>::RPW, H = smooth_halo<S, EPI>();
+	constexpr int NW = SmoothShape<S, SH>::NW, RPW = SmoothShape<S, SH>::RPW, H = smooth_halo<S, EPI>();
 	Strip<T, S, H, NW, RPW, EDGE> st(X0, Y0, nx);
 	st.template load<MODE>(xin, b, xc, nx, ny);
 	st.sweeps(bnd, nx, ny, off, dg, inv);
@@ -375,20 +382,20 @@ __device__ inline void smooth_body(const T* __restrict__ xin, const T* __restric
 	if (EPI == EPI_RESTRICT) st.restrict_to(rc, bnd + bnd_size<NW>(), bnd + 2 * bnd_size<NW>(), X0, Y0, nx, ny, off, dg);
 }
 
-template<typename T, int S, int MODE, int EPI>
-__global__ __launch_bounds__(64 * SmoothShape<S>::NW) void k_smooth(const T* __restrict__ xin, const T* __restrict__ b,
+template<typename T, int S, int MODE, int EPI, int SH>
+__global__ __launch_bounds__((64 * SmoothShape<S, SH>::NW)) void k_smooth(const T* __restrict__ xin, const T* __restrict__ b,
                                                                     const T* __restrict__ xc, T* __restrict__ xout,
                                                                     T* __restrict__ rc, int nx, int ny, T off, T dg,
                                                                     T inv, const int* __restrict__ done)
 {
-	constexpr int NW = SmoothShape<S>::NW;
+	constexpr int NW = SmoothShape<S, SH>::NW;
 	__shared__ T bnd[(EPI == EPI_RESTRICT ? 3 : 1) * bnd_size<NW>()];
 	if (done && *done) return;
 	int X0, Y0;
-	if (smooth_tile_origin<S, EPI>(nx, ny, X0, Y0))
-		smooth_body<T, S, MODE, EPI, true>(xin, b, xc, xout, rc, nx, ny, off, dg, inv, X0, Y0, bnd);
+	if (smooth_tile_origin<S, EPI, SH>(nx, ny, X0, Y0))
+		smooth_body<T, S, MODE, EPI, true, SH>(xin, b, xc, xout, rc, nx, ny, off, dg, inv, X0, Y0, bnd);
 	else
-		smooth_body<T, S, MODE, EPI, false>(xin, b, xc, xout, rc, nx, ny, off, dg, inv, X0, Y0, bnd);
+		smooth_body<T, S, MODE, EPI, false, SH>(xin, b, xc, xout, rc, nx, ny, off, dg, inv, X0, Y0, bnd);
 }
 
 // Finest-level post-smoothing fused with the outer update: e = smooth(e_pre +
@@ -910,9 +917,15 @@ static int smooth_launch(Ctx* c, unsigned l, int mode, int sweeps, const T* xin,
 	const double n2 = (double) L.n * (double) L.n;
 	const T off = (T) n2, dg = (T) (-6.0 * n2 + n2 + n2), inv = T(1) / dg;   // dg = -4 n^2, a power of two
 	const int nx = (int) L.nx, ny = (int) L.ny;
-#define AFM_SM(S_, M_, E_)                                                                                     \
-	hipLaunchKernelGGL((k_smooth<T, S_, M_, E_>), dim3(smooth_blocks<S_, E_>(nx, ny)),                        \
-	                   dim3(64 * SmoothShape<S_>::NW), 0, c->stream, xin, b, xc, xout, rc, nx, ny, off, dg, inv, done)
+#define AFM_SM2(S_, M_, E_, SH_)                                                                               \
+	hipLaunchKernelGGL((k_smooth<T, S_, M_, E_, SH_>), dim3(smooth_blocks<S_, E_, SH_>(nx, ny)),              \
+	                   dim3(64 * SmoothShape<S_, SH_>::NW), 0, c->stream, xin, b, xc, xout, rc, nx, ny, off, dg, inv, done)
+#define AFM_SM(S_, M_, E_)                          \
+	do {                                            \
+		if (small) AFM_SM2(S_, M_, E_, 1);          \
+		else AFM_SM2(S_, M_, E_, 0);                \
+	} while (0)
+	const bool small = (size_t) nx * ny <= SMALL_CELLS;
 	if (rc) {
 		if (mode != M_ZERO) { set_error("fused restriction needs a zero start"); return -1; }
 		if (sweeps == 1) AFM_SM(1, M_ZERO, EPI_RESTRICT);
@@ -929,6 +942,7 @@ static int smooth_launch(Ctx* c, unsigned l, int mode, int sweeps, const T* xin,
 		return -1;
 	}
 #undef AFM_SM
+#undef AFM_SM2
 	AFM_LAUNCH_CHECK();
 	return 0;
 }
