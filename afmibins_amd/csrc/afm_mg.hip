@@ -339,8 +339,8 @@ struct Strip {
 template<int S, int SH = 0> struct SmoothShape;
 template<> struct SmoothShape<1, 0> { static constexpr int NW = 6, RPW = 6; };
 template<> struct SmoothShape<2, 0> { static constexpr int NW = 4, RPW = 10; };
-template<> struct SmoothShape<1, 1> { static constexpr int NW = 4, RPW = 4; };
-template<> struct SmoothShape<2, 1> { static constexpr int NW = 6, RPW = 4; };
+template<> struct SmoothShape<1, 1> { static constexpr int NW = 8, RPW = 2; };
+template<> struct SmoothShape<2, 1> { static constexpr int NW = 8, RPW = 2; };
 constexpr size_t SMALL_CELLS = (size_t) 512 * 512;
 
 // halo of a launch: 2 cells per sweep, 2 more for a residual epilogue
