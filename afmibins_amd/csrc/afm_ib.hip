@@ -208,6 +208,7 @@ __global__ void k_spread_sort(const int* __restrict__ offs, int* __restrict__ li
 		BinRec o;
 		o.key = k;
 		o.base = r.ix0 | (r.iy0 << 16);
+		o.pad0 = o.pad1 = 0;
 #pragma unroll
 		for (int t = 0; t < 4; ++t) { o.dx[t] = r.dx[t]; o.dy[t] = r.dy[t]; }
 		o.ds = r.ds;
@@ -216,55 +217,95 @@ __global__ void k_spread_sort(const int* __restrict__ offs, int* __restrict__ li
 	}
 }
 
-// One lane per grid cell (blocks of 64 x 4 cells, blockIdx.y = component):
-// the markers whose window base lies in [gx-3, gx] x [gy-3, gy] (periodic)
-// are in at most 2 x 2 bins; their sorted records are merged so that the
-// terms are summed in ascending marker index.
+// One lane per bin (blockIdx.y = component).  The 4x4 cells of bin (bx, by)
+// take their terms from markers whose window base lies at most 3 cells
+// before them, i.e. in bins (bx-1 .. bx) x (by-1 .. by) (periodic).  The
+// lane merges those four lists, each sorted by marker, into ascending
+// marker order and adds each marker's terms ((dx[a] dy[b]) ds) F to the
+// cells of the bin that its window covers, so every cell sums its terms in
+// ascending marker index -- one merge for 16 cells.
 __global__ __launch_bounds__(256) void k_spread_gather(int comp_mask, const BinRec* __restrict__ srec,
-                                                       const int* __restrict__ offs, unsigned nx, unsigned ny, int sbx,
+                                                       const int* __restrict__ offs, unsigned nx, int sbx, int sby,
                                                        int nb1, double* __restrict__ fx, double* __restrict__ fy,
                                                        int accumulate)
 {
 	const int comp = blockIdx.y;
 	if (!((comp_mask >> comp) & 1)) return;
-	const int tpr = ((int) nx + 63) / 64;
-	const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
-	const int gx = (int) (tile % tpr) * 64 + (threadIdx.x & 63);
-	const int gy = (int) (tile / tpr) * 4 + (threadIdx.x >> 6);
-	if (gx >= (int) nx || gy >= (int) ny) return;
-	double* f = comp == 0 ? fx : fy;
-	int bx[2], by[2], nbx = 1, nby = 1;
-	bx[0] = (gx >= 3 ? gx - 3 : gx - 3 + (int) nx) / SB;
-	by[0] = (gy >= 3 ? gy - 3 : gy - 3 + (int) ny) / SB;
-	if (gx / SB != bx[0]) bx[nbx++] = gx / SB;
-	if (gy / SB != by[0]) by[nby++] = gy / SB;
-	int head[4], end[4], key[4], nl = 0;
-	for (int j = 0; j < nby; ++j)
-		for (int i = 0; i < nbx; ++i) {
-			const int b = comp * nb1 + bx[i] + sbx * by[j];
-			head[nl] = offs[b];
-			end[nl] = offs[b + 1];
-			++nl;
-		}
-	for (int l = 0; l < nl; ++l) key[l] = head[l] < end[l] ? srec[head[l]].key : 0x7fffffff;
-	double acc = 0.0;
-	for (;;) {
-		int best = -1, bk = 0x7fffffff;
-		for (int l = 0; l < nl; ++l)
-			if (key[l] < bk) { bk = key[l]; best = l; }
-		if (best < 0) break;
-		const BinRec& r = srec[head[best]];
-		++head[best];
-		key[best] = head[best] < end[best] ? srec[head[best]].key : 0x7fffffff;
-		int a = gx - (r.base & 0xffff); if (a < 0) a += nx;
-		int b = gy - (r.base >> 16); if (b < 0) b += ny;
-		if (a < 4 && b < 4) {
-			double w = r.dx[a] * r.dy[b];
-			acc += w * r.ds * r.F;
-		}
+	// consecutive blocks go round-robin to the 8 XCDs; markers cluster, so
+	// this spreads the long merges of a dense region over all of them
+	const int t = (int) blockIdx.x * 256 + (int) threadIdx.x;
+	if (t >= nb1) return;
+	const int bx = t % sbx, by = t / sbx;
+	const int bxm = bx == 0 ? sbx - 1 : bx - 1, bym = by == 0 ? sby - 1 : by - 1;
+	const int* o = offs + comp * nb1;
+	const int bins[4] = {bxm + sbx * bym, bx + sbx * bym, bxm + sbx * by, bx + sbx * by};
+	int head[4], end[4], key[4];
+#pragma unroll
+	for (int l = 0; l < 4; ++l) {
+		head[l] = o[bins[l]];
+		end[l] = o[bins[l] + 1];
 	}
-	const size_t o = gx + (size_t) nx * gy;
-	f[o] = accumulate ? f[o] + acc : acc;
+#pragma unroll
+	for (int l = 0; l < 4; ++l) key[l] = head[l] < end[l] ? srec[head[l]].key : 0x7fffffff;
+	const int X = SB * bx, Y = SB * by;
+	double acc[SB][SB];
+#pragma unroll
+	for (int j = 0; j < SB; ++j)
+#pragma unroll
+		for (int i = 0; i < SB; ++i) acc[j][i] = 0.0;
+	for (;;) {
+		// the smallest head key (markers are distinct, so no ties)
+		int best = 0, bk = key[0];
+#pragma unroll
+		for (int l = 1; l < 4; ++l)
+			if (key[l] < bk) { bk = key[l]; best = l; }
+		if (bk == 0x7fffffff) break;
+		int h = 0, e = 0;
+#pragma unroll
+		for (int l = 0; l < 4; ++l)
+			if (l == best) { h = head[l]; e = end[l]; }
+		// the whole record in registers first (six 16-byte loads), so the
+		// factor selection below picks values, not addresses
+		const double2* rv = reinterpret_cast<const double2*>(srec + h);
+		const int4 kb = *reinterpret_cast<const int4*>(srec + h);
+		const double2 sF = rv[1], x01 = rv[2], x23 = rv[3], y01 = rv[4], y23 = rv[5];
+		const int nk = h + 1 < e ? srec[h + 1].key : 0x7fffffff;
+#pragma unroll
+		for (int l = 0; l < 4; ++l)
+			if (l == best) { head[l] = h + 1; key[l] = nk; }
+		const double rdx[4] = {x01.x, x01.y, x23.x, x23.y}, rdy[4] = {y01.x, y01.y, y23.x, y23.y};
+		// window column a = X + i - base_x, row b = Y + j - base_y
+		int ox = X - (kb.y & 0xffff), oy = Y - (kb.y >> 16);
+		if (ox < -3) ox += (int) nx;
+		if (oy < -3) oy += (int) (SB * sby);
+		double wx[SB], wy[SB];
+		bool vx[SB], vy[SB];
+#pragma unroll
+		for (int i = 0; i < SB; ++i) {
+			const int a = ox + i, b = oy + i;
+			vx[i] = (unsigned) a < 4u;
+			vy[i] = (unsigned) b < 4u;
+			wx[i] = a <= 0 ? rdx[0] : (a == 1 ? rdx[1] : (a == 2 ? rdx[2] : rdx[3]));
+			wy[i] = b <= 0 ? rdy[0] : (b == 1 ? rdy[1] : (b == 2 ? rdy[2] : rdy[3]));
+		}
+		const double ds = sF.x, F = sF.y;
+		// every cell's term, kept only where the window covers the cell
+		// (a select, not a branch: the lanes' windows differ)
+#pragma unroll
+		for (int j = 0; j < SB; ++j)
+#pragma unroll
+			for (int i = 0; i < SB; ++i) {
+				const double w = wx[i] * wy[j];
+				const double sum = acc[j][i] + w * ds * F;
+				acc[j][i] = (vx[i] && vy[j]) ? sum : acc[j][i];
+			}
+	}
+#pragma unroll
+	for (int j = 0; j < SB; ++j) {
+		double* row = (comp == 0 ? fx : fy) + (size_t) nx * (Y + j) + X;
+#pragma unroll
+		for (int i = 0; i < SB; ++i) row[i] = accumulate ? row[i] + acc[j][i] : acc[j][i];
+	}
 }
 
 int launch_delta(Ctx* c, const double* r, long count, unsigned n, double* out)
@@ -349,9 +390,8 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 	} else {
 		AFM_HIP(hipMemsetAsync(offs, 0, sizeof(int) * (nb + 1), c->stream));
 	}
-	const unsigned tiles = (unsigned) (((c->nx + 63) / 64) * ((c->ny + 3) / 4));
-	hipLaunchKernelGGL(k_spread_gather, dim3(tiles, 2), dim3(256), 0, c->stream, comp_mask, c->srec, offs, c->nx, c->ny,
-	                   c->sbx, nb1, fx, fy, accumulate);
+	hipLaunchKernelGGL(k_spread_gather, dim3((nb1 + 255) / 256, 2), dim3(256), 0, c->stream, comp_mask, c->srec, offs,
+	                   c->nx, c->sbx, c->sby, nb1, fx, fy, accumulate);
 	AFM_LAUNCH_CHECK();
 	return 0;
 }

@@ -70,12 +70,14 @@ struct MarkerRec {
 };
 
 // The same record in bin order, as the spreading gather reads it: marker
-// index, window base (ix0 | iy0 << 16) and the terms' factors.
-struct BinRec {
-	int key, base;
-	double dx[4], dy[4];
+// index, window base (ix0 | iy0 << 16) and the terms' factors.  96 bytes,
+// 16-aligned: the gather loads a record as six 16-byte vectors.
+struct alignas(16) BinRec {
+	int key, base, pad0, pad1;
 	double ds, F;
+	double dx[4], dy[4];
 };
+static_assert(sizeof(BinRec) == 96, "BinRec is read as six 16-byte vectors");
 
 // ------------------------------------------------------------- context --
 struct LevelBuf {
