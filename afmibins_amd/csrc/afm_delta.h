@@ -113,7 +113,26 @@ AFM_HD inline dd sin_coef(int j)
 	}
 }
 
+// a + b without the careful low-order sum of dd_add; fine when the sum does
+// not cancel (every step of the series below adds a smaller term to a larger)
+AFM_HD inline dd dd_add_nc(dd a, dd b)
+{
+	dd s = two_sum(a.hi, b.hi);
+	double e = s.lo + (a.lo + b.lo);
+	return fast_two_sum(s.hi, e);
+}
+
 // Correctly rounded cos(x) for |x| < 8 (the delta needs |x| <= 3*pi/2).
+//
+// Ziv's strategy.  The fast evaluation takes the series to degree 10 in z =
+// y^2 with the first five coefficients in double-double and the other six in
+// double: its relative error is below 2^-75 (tail rounding < 2^-75.5,
+// truncation < 2^-77 for cos and < 2^-82 for sin on |y| <= pi/4).  If the
+// rounding of every value within 2^-70 of it is the same, that is the
+// correctly rounded result; otherwise (about one argument in 2^16) the full
+// degree-14 double-double series (2^-105) decides.  The cos and sin series
+// share one loop with the coefficients chosen per lane: the four taps of a
+// window are pi/2 apart in x, so neighbouring lanes alternate between them.
 AFM_HD inline double cos_cr(double x)
 {
 	const double P1 = 0x1.921fb54442d18p+0, P2 = 0x1.1a62633145c08p-54,
@@ -127,17 +146,60 @@ AFM_HD inline double cos_cr(double x)
 	dd y = fast_two_sum(s.hi, ylo);
 	dd z = dd_mul(y, y);
 	int q = ((int) k) & 3;
-	dd r;
-	if ((q & 1) == 0) {
-		dd p = cos_coef(14);
-		for (int j = 13; j >= 0; --j) p = dd_add(dd_mul(p, z), cos_coef(j));
-		r = p;
-	} else {
-		dd p = sin_coef(13);
-		for (int j = 12; j >= 0; --j) p = dd_add(dd_mul(p, z), sin_coef(j));
-		r = dd_mul(p, y);
+	const bool odd = (q & 1) != 0;
+	// fast: tail c5..c10 in double (Horner on z.hi), then c4..c0 in dd
+	double pt = odd ? sin_coef(10).hi : cos_coef(10).hi;
+#pragma unroll
+	for (int j = 9; j >= 5; --j) pt = fma(pt, z.hi, odd ? sin_coef(j).hi : cos_coef(j).hi);
+	dd p = {pt, 0.0};
+#pragma unroll
+	for (int j = 4; j >= 0; --j) {
+		const dd cj = odd ? sin_coef(j) : cos_coef(j);
+		p = dd_add_nc(dd_mul(p, z), cj);
+	}
+	const dd one = {1.0, 0.0};
+	dd r = dd_mul(p, odd ? y : one);   // times 1 is exact
+	const double e = 0x1p-70 * fabs(r.hi);
+	if (!(r.hi + (r.lo + e) == r.hi && r.hi + (r.lo - e) == r.hi)) {
+		if (!odd) {
+			dd pc = cos_coef(14);
+			for (int j = 13; j >= 0; --j) pc = dd_add(dd_mul(pc, z), cos_coef(j));
+			r = pc;
+		} else {
+			dd ps = sin_coef(13);
+			for (int j = 12; j >= 0; --j) ps = dd_add(dd_mul(ps, z), sin_coef(j));
+			r = dd_mul(ps, y);
+		}
 	}
 	// cos(y + k pi/2): q=0 cos y, q=1 -sin y, q=2 -cos y, q=3 sin y
+	double v = r.hi;
+	return (q == 1 || q == 2) ? -v : v;
+}
+
+// The degree-14 series alone (the definition the fast path reproduces);
+// kept for the tests that compare the two.
+AFM_HD inline double cos_cr_full(double x)
+{
+	const double P1 = 0x1.921fb54442d18p+0, P2 = 0x1.1a62633145c08p-54,
+	             P3 = -0x1.1f1976b7ed8f8p-106, P4 = -0x1.dd660ce2ff7d1p-157;
+	double ax = fabs(x);
+	double k = rint(ax * 0x1.45f306dc9c883p-1);
+	double t = ax - k * P1;
+	dd s = two_sum(t, -(k * P2));
+	double ylo = (s.lo - k * P3) - k * P4;
+	dd y = fast_two_sum(s.hi, ylo);
+	dd z = dd_mul(y, y);
+	int q = ((int) k) & 3;
+	dd r;
+	if ((q & 1) == 0) {
+		dd pc = cos_coef(14);
+		for (int j = 13; j >= 0; --j) pc = dd_add(dd_mul(pc, z), cos_coef(j));
+		r = pc;
+	} else {
+		dd ps = sin_coef(13);
+		for (int j = 12; j >= 0; --j) ps = dd_add(dd_mul(ps, z), sin_coef(j));
+		r = dd_mul(ps, y);
+	}
 	double v = r.hi;
 	return (q == 1 || q == 2) ? -v : v;
 }

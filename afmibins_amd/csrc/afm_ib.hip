@@ -1,19 +1,19 @@
 // afm_ib.hip -- Lagrangian <-> Eulerian transfer on gfx950.
 //
-// Interpolation (ib/grid.h:68-84, marker targets): one lane per marker, the
-// 4 + 4 delta factors evaluated once, 16 taps gathered in the reference's
-// order.  Same arithmetic as the reference, so the result is bit-exact
-// against the CPU restatement.
+// Interpolation (ib/grid.h:68-84, marker targets): eight lanes per marker,
+// one delta factor each, 16 taps summed in the reference's order.  Same
+// arithmetic as the reference, so the result is bit-exact against the CPU
+// restatement.
 //
 // Spreading (ib/grid.h:68-84, grid targets): the reference gathers, for
 // every grid point, the markers found by a kd-tree box query.  Here every
 // (marker, component) is binned by the 4x4 block of its window's base cell:
-// counts, a chunked exclusive scan, an unordered fill, then each bin's short
-// list is sorted by marker index.  One lane per grid cell then merges the
-// (at most 4) bins whose base cells can reach it and sums its terms
-// (w * ds) * F in ascending marker order.  No float atomics and no capacity
-// limit: the result is identical run to run and equals a sequential scatter
-// over markers in index order.
+// counts, a chunked exclusive scan, an unordered fill, then each record is
+// written to its bin in ascending marker index (its rank in the bin).  One
+// lane per bin then merges the four bins whose base cells can reach its 16
+// cells and sums each cell's terms (w * ds) * F in ascending marker order.
+// No float atomics and no capacity limit: the result is identical run to run
+// and equals a sequential scatter over markers in index order.
 #include "afm_device.h"
 
 namespace afm {
@@ -27,16 +27,34 @@ __global__ void k_delta(const double* __restrict__ r, long count, unsigned n, do
 	if (i < count) out[i] = delta(r[i], n);
 }
 
+// Eight lanes per marker: lane d evaluates one delta factor (d < 4: dx[d],
+// else dy[d - 4]) and the products of taps d and d + 8 (tap a + 4 b, x
+// fastest); lane 0 of the group adds the 16 products in tap order.
 __global__ void k_interp(int comp, const double* __restrict__ f, const double* __restrict__ X, long m,
                          unsigned n, unsigned W, unsigned D, unsigned nx, unsigned ny, double* __restrict__ U)
 {
-	long k = blockIdx.x * (long) blockDim.x + threadIdx.x;
-	if (k >= m) return;
+	const long t = blockIdx.x * (long) blockDim.x + threadIdx.x;
+	const long k = t >> 3;
+	const int d8 = (int) (t & 7);
+	// every lane of the wave takes part in the shuffles: no early exit
+	const long kk = k < m ? k : m - 1;
 	int ix0, iy0;
-	double dx[4], dy[4];
-	ib_window(comp, X[2 * k], X[2 * k + 1], n, W, D, nx, ny, ix0, iy0, dx, dy);
+	const double fct = ib_window_factor(comp, X[2 * kk], X[2 * kk + 1], n, W, D, nx, ny, d8, ix0, iy0);
+	const int a = d8 & 3, b = d8 >> 2;
+	const double wx = __shfl(fct, a, 8), wy0 = __shfl(fct, 4 + b, 8), wy1 = __shfl(fct, 6 + b, 8);
 	const double vol = 1. / (n * n);
-	U[k] = ib_gather(f, nx, ny, ix0, iy0, dx, dy, vol);
+	int ix = ix0 + a; if (ix >= (int) nx) ix -= nx;
+	int iy = iy0 + b; if (iy >= (int) ny) iy -= ny;
+	int iy2 = iy0 + b + 2; if (iy2 >= (int) ny) iy2 -= ny;
+	const double w0 = wx * wy0, w1 = wx * wy1;
+	const double p0 = w0 * vol * f[ix + (size_t) nx * iy];
+	const double p1 = w1 * vol * f[ix + (size_t) nx * iy2];
+	double value = 0.0;
+#pragma unroll
+	for (int q = 0; q < 8; ++q) value += __shfl(p0, q, 8);
+#pragma unroll
+	for (int q = 0; q < 8; ++q) value += __shfl(p1, q, 8);
+	if (k < m && d8 == 0) U[k] = value;
 }
 
 // X_h = X + half * (J_u(X) u, J_v(X) v).  The windows are the ones the
@@ -187,34 +205,30 @@ __global__ void k_spread_fill(int comp_mask, const MarkerRec* __restrict__ rec, 
 	list[offs[b] + slot] = (int) k;
 }
 
-// Each bin's list in ascending marker index (insertion sort; bins hold a few
-// markers), then the bin's records copied out in that order.
-__global__ void k_spread_sort(const int* __restrict__ offs, int* __restrict__ list, const MarkerRec* __restrict__ rec,
-                              long m, int nb1, BinRec* __restrict__ srec, int nbins)
+// Records in bin order, each bin in ascending marker index: one lane per
+// (marker, component) counts the smaller marker indices in its bin's
+// (unordered) list -- its rank -- and writes its record to that slot.
+// Bins hold a few markers, so the count is a short scan of cached indices.
+__global__ void k_spread_rank(int comp_mask, const int* __restrict__ offs, const int* __restrict__ list,
+                              const MarkerRec* __restrict__ rec, long m, int nb1, BinRec* __restrict__ srec)
 {
-	const int b = blockIdx.x * blockDim.x + threadIdx.x;
-	if (b >= nbins) return;
+	const long k = blockIdx.x * (long) blockDim.x + threadIdx.x;
+	const int comp = blockIdx.y;
+	if (k >= m || !((comp_mask >> comp) & 1)) return;
+	const MarkerRec& r = rec[comp * m + k];
+	const int b = comp * nb1 + r.tile;
 	const int lo = offs[b], hi = offs[b + 1];
-	for (int i = lo + 1; i < hi; ++i) {
-		const int v = list[i];
-		int j = i - 1;
-		while (j >= lo && list[j] > v) { list[j + 1] = list[j]; --j; }
-		list[j + 1] = v;
-	}
-	const MarkerRec* rc = rec + (b / nb1) * m;
-	for (int i = lo; i < hi; ++i) {
-		const int k = list[i];
-		const MarkerRec& r = rc[k];
-		BinRec o;
-		o.key = k;
-		o.base = r.ix0 | (r.iy0 << 16);
-		o.pad0 = o.pad1 = 0;
+	int rank = 0;
+	for (int i = lo; i < hi; ++i) rank += list[i] < (int) k;
+	BinRec o;
+	o.key = (int) k;
+	o.base = r.ix0 | (r.iy0 << 16);
+	o.pad0 = o.pad1 = 0;
+	o.ds = r.ds;
+	o.F = r.F;
 #pragma unroll
-		for (int t = 0; t < 4; ++t) { o.dx[t] = r.dx[t]; o.dy[t] = r.dy[t]; }
-		o.ds = r.ds;
-		o.F = r.F;
-		srec[i] = o;
-	}
+	for (int t = 0; t < 4; ++t) { o.dx[t] = r.dx[t]; o.dy[t] = r.dy[t]; }
+	srec[lo + rank] = o;
 }
 
 // One lane per bin (blockIdx.y = component).  The 4x4 cells of bin (bx, by)
@@ -231,11 +245,14 @@ __global__ __launch_bounds__(256) void k_spread_gather(int comp_mask, const BinR
 {
 	const int comp = blockIdx.y;
 	if (!((comp_mask >> comp) & 1)) return;
-	// consecutive blocks go round-robin to the 8 XCDs; markers cluster, so
-	// this spreads the long merges of a dense region over all of them
-	const int t = (int) blockIdx.x * 256 + (int) threadIdx.x;
-	if (t >= nb1) return;
-	const int bx = t % sbx, by = t / sbx;
+	// a block is 16 x 16 bins, a wave 16 x 4 (neighbouring bins hold
+	// similar numbers of markers, so a wave's lanes merge similar lengths);
+	// consecutive blocks go round-robin to the 8 XCDs, which spreads the long
+	// merges of a dense cluster over all of them
+	const int bpr = (sbx + 15) / 16;
+	const int bx = 16 * ((int) blockIdx.x % bpr) + ((int) threadIdx.x & 15);
+	const int by = 16 * ((int) blockIdx.x / bpr) + ((int) threadIdx.x >> 4);
+	if (bx >= sbx || by >= sby) return;
 	const int bxm = bx == 0 ? sbx - 1 : bx - 1, bym = by == 0 ? sby - 1 : by - 1;
 	const int* o = offs + comp * nb1;
 	const int bins[4] = {bxm + sbx * bym, bx + sbx * bym, bxm + sbx * by, bx + sbx * by};
@@ -319,7 +336,7 @@ int launch_delta(Ctx* c, const double* r, long count, unsigned n, double* out)
 int launch_interp(Ctx* c, int comp, const double* field, const double* X, long m, double* U)
 {
 	if (m <= 0) return 0;
-	hipLaunchKernelGGL(k_interp, dim3((m + 255) / 256), dim3(256), 0, c->stream, comp, field, X, m, c->n, c->W,
+	hipLaunchKernelGGL(k_interp, dim3((8 * m + 255) / 256), dim3(256), 0, c->stream, comp, field, X, m, c->n, c->W,
 	                   c->D, c->nx, c->ny, U);
 	AFM_LAUNCH_CHECK();
 	return 0;
@@ -384,13 +401,14 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 		hipLaunchKernelGGL(k_spread_fill, g, dim3(256), 0, c->stream, comp_mask, c->rec, m, nb1, offs, cursor,
 		                   c->bin_list);
 		AFM_LAUNCH_CHECK();
-		hipLaunchKernelGGL(k_spread_sort, dim3((nb + 255) / 256), dim3(256), 0, c->stream, offs, c->bin_list, c->rec, m,
-		                   nb1, c->srec, nb);
+		hipLaunchKernelGGL(k_spread_rank, g, dim3(256), 0, c->stream, comp_mask, offs, c->bin_list, c->rec, m, nb1,
+		                   c->srec);
 		AFM_LAUNCH_CHECK();
 	} else {
 		AFM_HIP(hipMemsetAsync(offs, 0, sizeof(int) * (nb + 1), c->stream));
 	}
-	hipLaunchKernelGGL(k_spread_gather, dim3((nb1 + 255) / 256, 2), dim3(256), 0, c->stream, comp_mask, c->srec, offs,
+	const unsigned gblocks = (unsigned) (((c->sbx + 15) / 16) * ((c->sby + 15) / 16));
+	hipLaunchKernelGGL(k_spread_gather, dim3(gblocks, 2), dim3(256), 0, c->stream, comp_mask, c->srec, offs,
 	                   c->nx, c->sbx, c->sby, nb1, fx, fy, accumulate);
 	AFM_LAUNCH_CHECK();
 	return 0;

@@ -25,10 +25,13 @@ def H(t):
 
 # ----------------------------------------------------------------- delta
 def test_delta_bit_exact(ctx_of, oracle, dev):
-    for n in (64, 2048):
+    # cos is evaluated by a fast series plus a rounding test, with the full
+    # double-double series for about 1 argument in 2^16: the 1.6M uniform
+    # arguments at n = 2048 take that path a few dozen times
+    for n, cnt in ((64, 400000), (2048, 1600000)):
         ctx = ctx_of(n)
         rng = np.random.default_rng(n)
-        r = np.concatenate([rng.uniform(-3.2 / n, 3.2 / n, 400000),
+        r = np.concatenate([rng.uniform(-3.2 / n, 3.2 / n, cnt),
                             (rng.integers(-3, 4, 20000) + rng.integers(-1000, 1001, 20000) * 2.0 ** -40) / n,
                             np.array([0.0, -0.0, 2.0 / n, -2.0 / n, 1.0 / n, 0.5 / n, 1e-300])])
         got = H(ctx.delta(T(r, dev)))
