@@ -9,9 +9,10 @@
 // every grid point, the markers found by a kd-tree box query.  Here every
 // (marker, component) is binned by the 4x4 block of its window's base cell:
 // counts, a chunked exclusive scan, an unordered fill, then each record is
-// written to its bin in ascending marker index (its rank in the bin).  One
-// lane per bin then merges the four bins whose base cells can reach its 16
-// cells and sums each cell's terms (w * ds) * F in ascending marker order.
+// written to its bin in ascending marker index (its rank in the bin).  Four
+// lanes per bin (one per cell row) then merge the four bins whose base cells
+// can reach its 16 cells and sum each cell's terms (w * ds) * F in ascending
+// marker order.
 // No float atomics and no capacity limit: the result is identical run to run
 // and equals a sequential scatter over markers in index order.
 #include "afm_device.h"
@@ -231,98 +232,205 @@ __global__ void k_spread_rank(int comp_mask, const int* __restrict__ offs, const
 	srec[lo + rank] = o;
 }
 
-// One lane per bin (blockIdx.y = component).  The 4x4 cells of bin (bx, by)
-// take their terms from markers whose window base lies at most 3 cells
-// before them, i.e. in bins (bx-1 .. bx) x (by-1 .. by) (periodic).  The
-// lane merges those four lists, each sorted by marker, into ascending
-// marker order and adds each marker's terms ((dx[a] dy[b]) ds) F to the
-// cells of the bin that its window covers, so every cell sums its terms in
-// ascending marker index -- one merge for 16 cells.
+// Spreading gather.  The 4x4 cells of bin (bx, by) take their terms from
+// markers whose window base lies at most 3 cells before them, i.e. in bins
+// (bx-1 .. bx) x (by-1 .. by) (periodic).  Four lanes per bin, one per cell
+// row: each merges those four lists, each sorted by marker, into ascending
+// marker order (the four lanes run the same merge in the same instructions)
+// and adds each marker's terms ((dx[a] dy[b]) ds) F to the cells of its row
+// that the window covers, so every cell sums its terms in ascending marker
+// index.
+//
+// A block is 8 x 8 bins.  It first copies the keys of its 9 x 9 bins into
+// LDS, so the merge, a chain of dependent key reads, runs on LDS; a record's
+// factors are loaded from global memory one merge step ahead of their use.
+// Blocks with fewer than GMIN or more than GCAP records, and grids of fewer
+// than 8 x 8 bins, merge from global memory instead (same order, same sums).
+constexpr int GB = 8;        // gather block edge, bins
+constexpr int GCAP = 2048;   // records staged per block
+constexpr int GMIN = 64;     // fewer: short lists, not worth the copy
+
+// a record's values in registers (named fields, no arrays: the factor
+// selection below must stay a choice among registers)
+struct RecVals {
+	int base;
+	double ds, F, x0, x1, x2, x3, y0, y1, y2, y3;
+};
+
+__device__ inline RecVals load_rec(const BinRec* __restrict__ srec, int g)
+{
+	const double2* rv = reinterpret_cast<const double2*>(srec + g);
+	const int4 kb = *reinterpret_cast<const int4*>(srec + g);
+	const double2 sF = rv[1], x01 = rv[2], x23 = rv[3], y01 = rv[4], y23 = rv[5];
+	RecVals v;
+	v.base = kb.y;
+	v.ds = sF.x; v.F = sF.y;
+	v.x0 = x01.x; v.x1 = x01.y; v.x2 = x23.x; v.x3 = x23.y;
+	v.y0 = y01.x; v.y1 = y01.y; v.y2 = y23.x; v.y3 = y23.y;
+	return v;
+}
+
+// the record's terms added to the cells (X .. X+3, Yj) that its window covers
+__device__ inline void add_terms(const RecVals r, int X, int Yj, int nx, int ny, double (&acc)[SB])
+{
+	// window column a = X + i - base_x, row b = Yj - base_y
+	int ox = X - (r.base & 0xffff), b = Yj - (r.base >> 16);
+	if (ox < -3) ox += nx;
+	if (b < -3) b += ny;
+	const bool vy = (unsigned) b < 4u;
+	const double wy = b <= 0 ? r.y0 : (b == 1 ? r.y1 : (b == 2 ? r.y2 : r.y3));
+	// every cell's term, kept only where the window covers the cell (a
+	// select, not a branch: the lanes' windows differ)
+#pragma unroll
+	for (int i = 0; i < SB; ++i) {
+		const int a = ox + i;
+		const double wx = a <= 0 ? r.x0 : (a == 1 ? r.x1 : (a == 2 ? r.x2 : r.x3));
+		const double w = wx * wy;
+		const double sum = acc[i] + w * r.ds * r.F;
+		acc[i] = (vy && (unsigned) a < 4u) ? sum : acc[i];
+	}
+}
+
+// Merge of the four lists [head, end) in ascending key.  STAGED: positions
+// index lkey / lidx in LDS (key, record index); else they index srec.
+template<bool STAGED>
+__device__ inline void merge_terms(const BinRec* __restrict__ srec, const int* lkey, const int* lidx, int (&head)[4],
+                                   const int (&end)[4], int X, int Yj, int nx, int ny, double (&acc)[SB])
+{
+	int key[4];
+#pragma unroll
+	for (int l = 0; l < 4; ++l) key[l] = head[l] < end[l] ? (STAGED ? lkey[head[l]] : srec[head[l]].key) : 0x7fffffff;
+	// take the smallest head (markers are distinct: no ties); its record
+	// index, or -1 when every list is spent
+	auto pick = [&]() -> int {
+		int best = 0, bk = key[0];
+#pragma unroll
+		for (int l = 1; l < 4; ++l)
+			if (key[l] < bk) { bk = key[l]; best = l; }
+		if (bk == 0x7fffffff) return -1;
+		int h = 0, e = 0;
+#pragma unroll
+		for (int l = 0; l < 4; ++l)
+			if (l == best) { h = head[l]; e = end[l]; }
+		const int nk = h + 1 < e ? (STAGED ? lkey[h + 1] : srec[h + 1].key) : 0x7fffffff;
+#pragma unroll
+		for (int l = 0; l < 4; ++l)
+			if (l == best) { head[l] = h + 1; key[l] = nk; }
+		return STAGED ? lidx[h] : h;
+	};
+	// the next record's loads go out before this one's terms are added; two
+	// register sets in turn, so no copy waits on loads in flight
+	int ga = pick();
+	if (ga < 0) return;
+	RecVals ra = load_rec(srec, ga), rb;
+	for (;;) {
+		const int gb = pick();
+		rb = load_rec(srec, gb < 0 ? ga : gb);
+		add_terms(ra, X, Yj, nx, ny, acc);
+		if (gb < 0) break;
+		ga = pick();
+		ra = load_rec(srec, ga < 0 ? gb : ga);
+		add_terms(rb, X, Yj, nx, ny, acc);
+		if (ga < 0) break;
+	}
+}
+
 __global__ __launch_bounds__(256) void k_spread_gather(int comp_mask, const BinRec* __restrict__ srec,
                                                        const int* __restrict__ offs, unsigned nx, int sbx, int sby,
                                                        int nb1, double* __restrict__ fx, double* __restrict__ fy,
                                                        int accumulate)
 {
+	constexpr int NR = 2 * (GB + 1);   // srec ranges staged per block
+	__shared__ int lkey[GCAP], lidx[GCAP];
+	__shared__ int rs[NR], rp[NR + 1];
 	const int comp = blockIdx.y;
 	if (!((comp_mask >> comp) & 1)) return;
-	// a block is 16 x 16 bins, a wave 16 x 4 (neighbouring bins hold
-	// similar numbers of markers, so a wave's lanes merge similar lengths);
-	// consecutive blocks go round-robin to the 8 XCDs, which spreads the long
-	// merges of a dense cluster over all of them
-	const int bpr = (sbx + 15) / 16;
-	const int bx = 16 * ((int) blockIdx.x % bpr) + ((int) threadIdx.x & 15);
-	const int by = 16 * ((int) blockIdx.x / bpr) + ((int) threadIdx.x >> 4);
-	if (bx >= sbx || by >= sby) return;
-	const int bxm = bx == 0 ? sbx - 1 : bx - 1, bym = by == 0 ? sby - 1 : by - 1;
+	const int tid = (int) threadIdx.x;
+	// consecutive blocks go round-robin to the 8 XCDs, which spreads the
+	// long merges of a dense cluster over all of them
+	const int bpr = (sbx + GB - 1) / GB;
+	const int bx0 = GB * ((int) blockIdx.x % bpr), by0 = GB * ((int) blockIdx.x / bpr);
 	const int* o = offs + comp * nb1;
-	const int bins[4] = {bxm + sbx * bym, bx + sbx * bym, bxm + sbx * by, bx + sbx * by};
-	int head[4], end[4], key[4];
+	const int ny = SB * sby;
+	// lane: cell row j of bin (tx, ty) of the block.  Its four bins' ranges
+	// are read first, so their latency overlaps the block's staging
+	const int j = tid & 3, q = tid >> 2;
+	const int tx = q & (GB - 1), ty = q / GB;
+	const int bx = bx0 + tx, by = by0 + ty;
+	const bool live = bx < sbx && by < sby;
+	int bs[4], bn[4];
 #pragma unroll
 	for (int l = 0; l < 4; ++l) {
-		head[l] = o[bins[l]];
-		end[l] = o[bins[l] + 1];
+		// bins (bx-1, by-1), (bx, by-1), (bx-1, by), (bx, by)
+		const int gx = bx - 1 + (l & 1) < 0 ? sbx - 1 : bx - 1 + (l & 1);
+		const int gy = by - 1 + (l >> 1) < 0 ? sby - 1 : by - 1 + (l >> 1);
+		const int b = gx + sbx * gy;
+		bs[l] = live ? o[b] : 0;
+		bn[l] = live ? o[b + 1] - bs[l] : 0;
 	}
-#pragma unroll
-	for (int l = 0; l < 4; ++l) key[l] = head[l] < end[l] ? srec[head[l]].key : 0x7fffffff;
-	const int X = SB * bx, Y = SB * by;
-	double acc[SB][SB];
-#pragma unroll
-	for (int j = 0; j < SB; ++j)
-#pragma unroll
-		for (int i = 0; i < SB; ++i) acc[j][i] = 0.0;
-	for (;;) {
-		// the smallest head key (markers are distinct, so no ties)
-		int best = 0, bk = key[0];
-#pragma unroll
-		for (int l = 1; l < 4; ++l)
-			if (key[l] < bk) { bk = key[l]; best = l; }
-		if (bk == 0x7fffffff) break;
-		int h = 0, e = 0;
-#pragma unroll
-		for (int l = 0; l < 4; ++l)
-			if (l == best) { h = head[l]; e = end[l]; }
-		// the whole record in registers first (six 16-byte loads), so the
-		// factor selection below picks values, not addresses
-		const double2* rv = reinterpret_cast<const double2*>(srec + h);
-		const int4 kb = *reinterpret_cast<const int4*>(srec + h);
-		const double2 sF = rv[1], x01 = rv[2], x23 = rv[3], y01 = rv[4], y23 = rv[5];
-		const int nk = h + 1 < e ? srec[h + 1].key : 0x7fffffff;
-#pragma unroll
-		for (int l = 0; l < 4; ++l)
-			if (l == best) { head[l] = h + 1; key[l] = nk; }
-		const double rdx[4] = {x01.x, x01.y, x23.x, x23.y}, rdy[4] = {y01.x, y01.y, y23.x, y23.y};
-		// window column a = X + i - base_x, row b = Y + j - base_y
-		int ox = X - (kb.y & 0xffff), oy = Y - (kb.y >> 16);
-		if (ox < -3) ox += (int) nx;
-		if (oy < -3) oy += (int) (SB * sby);
-		double wx[SB], wy[SB];
-		bool vx[SB], vy[SB];
-#pragma unroll
-		for (int i = 0; i < SB; ++i) {
-			const int a = ox + i, b = oy + i;
-			vx[i] = (unsigned) a < 4u;
-			vy[i] = (unsigned) b < 4u;
-			wx[i] = a <= 0 ? rdx[0] : (a == 1 ? rdx[1] : (a == 2 ? rdx[2] : rdx[3]));
-			wy[i] = b <= 0 ? rdy[0] : (b == 1 ? rdy[1] : (b == 2 ? rdy[2] : rdy[3]));
-		}
-		const double ds = sF.x, F = sF.y;
-		// every cell's term, kept only where the window covers the cell
-		// (a select, not a branch: the lanes' windows differ)
-#pragma unroll
-		for (int j = 0; j < SB; ++j)
-#pragma unroll
-			for (int i = 0; i < SB; ++i) {
-				const double w = wx[i] * wy[j];
-				const double sum = acc[j][i] + w * ds * F;
-				acc[j][i] = (vx[i] && vy[j]) ? sum : acc[j][i];
+	bool staged = sbx >= GB && sby >= GB;
+	if (staged) {
+		// local bin (lx, ly) in 0..GB is global (bx0 - 1 + lx, by0 - 1 + ly).
+		// Per local row two srec ranges: bin lx = 0, then bins lx = 1..GB
+		// (consecutive in a row, so contiguous in srec)
+		if (tid < 64) {
+			int cnt = 0;
+			if (tid < NR) {
+				const int ly = tid >> 1;
+				const int gy = by0 - 1 + ly < 0 ? sby - 1 : by0 - 1 + ly;
+				const int b = (tid & 1) ? bx0 + sbx * gy : (bx0 == 0 ? sbx - 1 : bx0 - 1) + sbx * gy;
+				const int e = (tid & 1) ? o[b + GB] : o[b + 1];
+				rs[tid] = o[b];
+				cnt = e - o[b];
 			}
-	}
+			// inclusive scan over the wave
 #pragma unroll
-	for (int j = 0; j < SB; ++j) {
-		double* row = (comp == 0 ? fx : fy) + (size_t) nx * (Y + j) + X;
-#pragma unroll
-		for (int i = 0; i < SB; ++i) row[i] = accumulate ? row[i] + acc[j][i] : acc[j][i];
+			for (int d = 1; d < 64; d <<= 1) {
+				const int t = __shfl_up(cnt, d, 64);
+				if (tid >= d) cnt += t;
+			}
+			if (tid < NR) rp[tid + 1] = cnt;
+			if (tid == 0) rp[0] = 0;
+		}
+		__syncthreads();
+		const int total = rp[NR];
+		staged = total >= GMIN && total <= GCAP;
+		if (staged) {
+			for (int p = tid; p < total; p += 256) {
+				int lo = 0, hi = NR - 1;   // last range with rp <= p
+				while (lo < hi) {
+					const int mid = (lo + hi + 1) >> 1;
+					if (rp[mid] <= p) lo = mid; else hi = mid - 1;
+				}
+				const int g = rs[lo] + (p - rp[lo]);
+				lkey[p] = srec[g].key;
+				lidx[p] = g;
+			}
+		}
+		__syncthreads();
 	}
+	if (!live) return;
+	int head[4], end[4];
+#pragma unroll
+	for (int l = 0; l < 4; ++l) {
+		if (staged) {
+			const int lx = tx + (l & 1), ly = ty + (l >> 1);
+			const int ri = 2 * ly + (lx > 0 ? 1 : 0);
+			head[l] = rp[ri] + (bs[l] - rs[ri]);
+		} else {
+			head[l] = bs[l];
+		}
+		end[l] = head[l] + bn[l];
+	}
+	const int X = SB * bx, Yj = SB * by + j;
+	double acc[SB];
+#pragma unroll
+	for (int i = 0; i < SB; ++i) acc[i] = 0.0;
+	if (staged) merge_terms<true>(srec, lkey, lidx, head, end, X, Yj, (int) nx, ny, acc);
+	else merge_terms<false>(srec, lkey, lidx, head, end, X, Yj, (int) nx, ny, acc);
+	double* row = (comp == 0 ? fx : fy) + (size_t) nx * Yj + X;
+#pragma unroll
+	for (int i = 0; i < SB; ++i) row[i] = accumulate ? row[i] + acc[i] : acc[i];
 }
 
 int launch_delta(Ctx* c, const double* r, long count, unsigned n, double* out)
@@ -407,7 +515,7 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 	} else {
 		AFM_HIP(hipMemsetAsync(offs, 0, sizeof(int) * (nb + 1), c->stream));
 	}
-	const unsigned gblocks = (unsigned) (((c->sbx + 15) / 16) * ((c->sby + 15) / 16));
+	const unsigned gblocks = (unsigned) (((c->sbx + GB - 1) / GB) * ((c->sby + GB - 1) / GB));
 	hipLaunchKernelGGL(k_spread_gather, dim3(gblocks, 2), dim3(256), 0, c->stream, comp_mask, c->srec, offs,
 	                   c->nx, c->sbx, c->sby, nb1, fx, fy, accumulate);
 	AFM_LAUNCH_CHECK();

@@ -101,13 +101,16 @@ def test_spread_small_grids_and_seams(ctx_of, oracle, dev, n):
     ctx.sync()
 
 
-def test_spread_one_crowded_bin(ctx_of, oracle, dev):
-    # 2000 markers inside one 4x4 block of window bases: one long bin list
+@pytest.mark.parametrize("m", [1500, 3000])
+def test_spread_one_crowded_bin(ctx_of, oracle, dev, m):
+    # m markers inside one 4x4 block of window bases: one long bin list.
+    # 1500 fit the gather's per-block staging (2048 records), 3000 do not
+    # and take the merge from global memory
     n = 128
     ctx = ctx_of(n)
     rng = np.random.default_rng(9)
-    X = (40.0 + rng.uniform(0.0, 2.0, (2000, 2))) / n
-    F, ds = rng.standard_normal(2000), np.full(2000, 1.0 / n)
+    X = (40.0 + rng.uniform(0.0, 2.0, (m, 2))) / n
+    F, ds = rng.standard_normal(m), np.full(m, 1.0 / n)
     for comp in (0, 1):
         got = H(ctx.spread(comp, T(F, dev), T(ds, dev), T(X, dev)))
         assert np.array_equal(got, oracle.spread2(comp, F, ds, X, n, n, n, mode=1)), comp
