@@ -644,6 +644,33 @@ __device__ inline void csync()
 	}
 }
 
+// A cell's four stencil neighbours in stencil5's summation order (no
+// diagonal term): the wrap flags of a cell only fix the order in which its
+// neighbours are added -- the vertical terms of slots 0-1, the horizontal
+// pair, the vertical terms of slots 6-7 -- so the order is found with a few
+// integer selects instead of stencil5's eight conditional double adds.
+struct CellOps {
+	int i, a0, a1, a2, a3;
+};
+
+template<int L>
+__device__ inline CellOps cell_ops(int gx, int gy)
+{
+	constexpr int nx = 1 << L, msk = nx - 1;
+	const bool we = gx == msk, ww = gx == 0, wn = gy == msk, ws = gy == 0;
+	const int iE = ((gx + 1) & msk) + (gy << L), iW = ((gx - 1) & msk) + (gy << L);
+	const int iN = gx + (((gy + 1) & msk) << L), iS = gx + (((gy - 1) & msk) << L);
+	// horizontal pair: E before W at either x edge, W before E inside
+	const int h0 = (we || ww) ? iE : iW, h1 = (we || ww) ? iW : iE;
+	CellOps o;
+	o.i = gx + (gy << L);
+	// y = ny-1: N, S first; y = 0: N, S last; else S first, N last
+	if (wn) { o.a0 = iN; o.a1 = iS; o.a2 = h0; o.a3 = h1; }
+	else if (ws) { o.a0 = h0; o.a1 = h1; o.a2 = iN; o.a3 = iS; }
+	else { o.a0 = iS; o.a1 = h0; o.a2 = h1; o.a3 = iN; }
+	return o;
+}
+
 // red-black sweeps at level L; zero: x starts at 0; xc: x += P xc first
 template<int L>
 __device__ void c_smooth(double* x, const double* b, bool zero, const double* xc, int sweeps)
@@ -680,6 +707,24 @@ __device__ void c_smooth(double* x, const double* b, bool zero, const double* xc
 		for (int s = 0; s < sweeps; ++s)
 			for (int color = 0; color < 2; ++color) {
 				const bool first = s == 0 && color == 0;
+				if (L >= 5 && !(first && (zero || xc))) {
+					// the issue-bound levels (all threads, several cells each)
+#pragma unroll
+					for (int k0 = 0; k0 < N2; k0 += NT) {
+						const int k = k0 + (int) threadIdx.x;
+						if (N2 % NT != 0 && k >= N2) break;
+						const int gy = k >> (L - 1), gx = ((k & (hx - 1)) << 1) + ((color + gy) & 1);
+						const CellOps o = cell_ops<L>(gx, gy);
+						double value = 0.0;
+						value += off * x[o.a0];
+						value += off * x[o.a1];
+						value += off * x[o.a2];
+						value += off * x[o.a3];
+						x[o.i] = (b[o.i] - value) * inv;
+					}
+					csync<NT>();
+					continue;
+				}
 #pragma unroll
 				for (int k0 = 0; k0 < N2; k0 += NT) {
 					const int k = k0 + (int) threadIdx.x;
