@@ -338,7 +338,7 @@ struct Strip {
 // chain of row updates sets the time, so each wave holds fewer rows.
 template<int S, int SH = 0> struct SmoothShape;
 template<> struct SmoothShape<1, 0> { static constexpr int NW = 6, RPW = 6; };
-template<> struct SmoothShape<2, 0> { static constexpr int NW = 4, RPW = 10; };
+template<> struct SmoothShape<2, 0> { static constexpr int NW = 16, RPW = 4; };
 template<> struct SmoothShape<1, 1> { static constexpr int NW = 8, RPW = 2; };
 template<> struct SmoothShape<2, 1> { static constexpr int NW = 8, RPW = 2; };
 constexpr size_t SMALL_CELLS = (size_t) 512 * 512;

@@ -135,7 +135,8 @@ def test_laplacian_bit_exact(ctx_of, oracle, dev, n):
     assert np.array_equal(got.ravel(), oracle.stencil_apply(x, n, 1, n, n))
 
 
-@pytest.mark.parametrize("n", [64, 128, 512])
+# 64..512 run the short-strip tile shape, 1024 the tall one
+@pytest.mark.parametrize("n", [64, 128, 512, 1024])
 @pytest.mark.parametrize("sweeps,zero", [(1, False), (2, False), (2, True), (1, True)])
 def test_rb_smoother_bit_exact(ctx_of, oracle, dev, n, sweeps, zero):
     ctx = ctx_of(n)
@@ -157,7 +158,7 @@ def test_restrict_prolong_bit_exact(ctx_of, oracle, dev, n):
     assert np.array_equal(H(ctx.mg_prolong(T(c, dev))), oracle.prolong2(c))
 
 
-@pytest.mark.parametrize("n", [16, 32, 64, 256])
+@pytest.mark.parametrize("n", [16, 32, 64, 256, 1024])
 def test_vcycle_bit_exact(ctx_of, oracle, dev, n):
     # One V(2,2) cycle has no reductions on the unit square (the 1x1 coarsest
     # level is zero after mean removal), so it reproduces the oracle's
