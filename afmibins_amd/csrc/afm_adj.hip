@@ -50,7 +50,8 @@ __global__ __launch_bounds__(256) void k_advect_T(const double* __restrict__ ab,
 	const size_t N = (size_t) nx * ny;
 	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
 	if (i >= N) return;
-	const int x = (int) (i % nx), z = (int) (i / nx);
+	int x, z;
+	cell_xy(i, nx, x, z);
 	auto id = [&](int a, int b) -> size_t {
 		a = (a + nx) % nx; b = (b + ny) % ny;
 		return (size_t) a + (size_t) nx * b;

@@ -224,6 +224,15 @@ __device__ inline int wrap_once(int g, int n)
 	return g;
 }
 
+// Column and row of cell i of a grid nx wide.  Every grid here is a power of
+// two wide (the context rejects other sizes), so this is a mask and a shift
+// -- the 64-bit i % nx and i / nx it replaces are long division sequences.
+__device__ inline void cell_xy(size_t i, int nx, int& gx, int& gy)
+{
+	gx = (int) ((unsigned) i & (unsigned) (nx - 1));
+	gy = (int) (i >> __builtin_ctz((unsigned) nx));
+}
+
 // XCD-aware block order: consecutive block ids go to different XCDs (round
 // robin over 8), so map them back to spatially consecutive tiles per XCD.
 __device__ inline unsigned xcd_tile(unsigned bid, unsigned nblk)

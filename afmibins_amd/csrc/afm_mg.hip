@@ -533,7 +533,8 @@ __global__ __launch_bounds__(256) void k_update_resid(const double* __restrict__
 	const double mean = ctl->mean;
 	double acc[1] = {0.0};
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
-		const int gx = (int) (i % nx), gy = (int) (i / nx);
+		int gx, gy;
+	cell_xy(i, nx, gx, gy);
 		const size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1;
 		const size_t iw = gx == 0 ? i + (nx - 1) : i - 1;
 		const size_t in_ = gy == ny - 1 ? gx : i + nx;
@@ -1337,7 +1338,8 @@ __global__ void k_lap_apply(const double* __restrict__ x, double* __restrict__ y
 	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
 	size_t N = (size_t) nx * ny;
 	if (i >= N) return;
-	int gx = (int) (i % nx), gy = (int) (i / nx);
+	int gx, gy;
+	cell_xy(i, nx, gx, gy);
 	size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;
 	size_t in_ = gy == ny - 1 ? gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;
 	y[i] = stencil5<double, true>(off, dg, x[i], x[ie], x[iw], x[in_], x[is], gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);

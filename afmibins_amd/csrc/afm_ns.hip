@@ -44,7 +44,9 @@ __global__ void k_advect(const double* __restrict__ u, const double* __restrict_
 {
 	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
 	if (i >= (size_t) nx * ny) return;
-	adv_at(u, v, nx, ny, n, i, (int) (i % nx), (int) (i / nx), ua[i], va[i]);
+	int gx, gy;
+	cell_xy(i, nx, gx, gy);
+	adv_at(u, v, nx, ny, n, i, gx, gy, ua[i], va[i]);
 }
 
 // d = n ((u[x+1] - u) + dv + (v[y+1] - v))   (alg/div.h:38-59)
@@ -54,7 +56,8 @@ __global__ void k_div(const double* __restrict__ u, const double* __restrict__ v
 	const size_t N = (size_t) nx * ny;
 	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
 	if (i >= N) return;
-	const int x = (int) (i % nx), z = (int) (i / nx);
+	int x, z;
+	cell_xy(i, nx, x, z);
 	const size_t ixp = x == nx - 1 ? i - (nx - 1) : i + 1, izp = z == ny - 1 ? (size_t) x : i + nx;
 	const double dv = 0.0 - 0.0;
 	d[i] = n * ((u[ixp] - u[i]) + dv + (v[izp] - v[i]));
@@ -67,7 +70,8 @@ __global__ void k_grad_sub(const double* __restrict__ p, double* __restrict__ u,
 	const size_t N = (size_t) nx * ny;
 	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
 	if (i >= N) return;
-	const int x = (int) (i % nx), z = (int) (i / nx);
+	int x, z;
+	cell_xy(i, nx, x, z);
 	const size_t ixm = x == 0 ? i + (nx - 1) : i - 1, izm = z == 0 ? i + N - nx : i - nx;
 	const double px = n * (p[i] - p[ixm]);
 	const double pz = n * (p[i] - p[izm]);
@@ -80,7 +84,8 @@ __global__ void k_helm_apply(const double* __restrict__ x, double* __restrict__ 
 	const size_t N = (size_t) nx * ny;
 	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
 	if (i >= N) return;
-	const int gx = (int) (i % nx), gy = (int) (i / nx);
+	int gx, gy;
+	cell_xy(i, nx, gx, gy);
 	const size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;
 	const size_t in_ = gy == ny - 1 ? (size_t) gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;
 	y[i] = stencil5<double, true>(off, dg, x[i], x[ie], x[iw], x[in_], x[is], gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
@@ -100,7 +105,8 @@ struct CGArgs {
 };
 
 #define CG_NEIGHBOURS                                                                              \
-	const int gx = (int) (i % nx), gy = (int) (i / nx);                                            \
+	int gx, gy;                                                                                    \
+	cell_xy(i, nx, gx, gy);                                                                        \
 	const size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;    \
 	const size_t in_ = gy == ny - 1 ? (size_t) gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;    \
 	const bool we = gx == nx - 1, ww = gx == 0, wn = gy == ny - 1, ws = gy == 0;
@@ -255,7 +261,8 @@ __global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* 
 	const double* v = a.x[1];
 	double acc_u[2] = {0.0, 0.0}, acc_v[2] = {0.0, 0.0};
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
-		const int x = (int) (i % nx), z = (int) (i / nx);
+		int x, z;
+	cell_xy(i, nx, x, z);
 		double ua, va;
 		adv_at(u, v, nx, ny, n, i, x, z, ua, va);
 		const size_t ie = x == nx - 1 ? i - (nx - 1) : i + 1, iw = x == 0 ? i + (nx - 1) : i - 1;
