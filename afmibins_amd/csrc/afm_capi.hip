@@ -98,7 +98,7 @@ int afm_ctx_create(const afm_geom* g, int device, afm_ctx** out)
 	c->stream = c->own_stream;
 	if ((rc = c->alloc((void**) &c->ctl, sizeof(SolveCtl) * NCTL))) return fail(rc);
 	if (hipHostMalloc((void**) &c->hctl, sizeof(SolveCtl) * NCTL, hipHostMallocDefault) != hipSuccess) { set_error("hipHostMalloc failed"); return fail(-3); }
-	if ((rc = c->alloc((void**) &c->partials, sizeof(double) * MAX_RED_BLOCKS * 4))) return fail(rc);
+	if ((rc = c->alloc((void**) &c->partials, sizeof(double) * MAX_RED_BLOCKS * 8))) return fail(rc);
 	if (hipMemsetAsync(c->ctl, 0, sizeof(SolveCtl) * NCTL, c->stream) != hipSuccess) { set_error("memset failed"); return fail(-3); }
 	c->sbx = (int) (c->nx / 4); c->sby = (int) (c->ny / 4);
 	{

@@ -47,6 +47,7 @@ struct SolveCtl {
 	double tol;       // absolute threshold (RMS for MG, 2-norm for CG)
 	double res;       // current residual norm
 	double n0, nu, mu;
+	double n0p[2];    // CG: r.r after k updates in n0p[k & 1] (read and written by consecutive kernels)
 	double mean;      // mean removed from the MG right-hand side
 	double sum2;      // scratch second sum
 	int count;        // completed iterations
@@ -98,7 +99,7 @@ struct Ctx {
 	double* upartials = nullptr;       // per-block sums of the multigrid update
 	SolveCtl* ctl = nullptr;           // device [NCTL]
 	SolveCtl* hctl = nullptr;          // pinned host mirror
-	double* partials = nullptr;        // [MAX_RED_BLOCKS * 4]
+	double* partials = nullptr;        // [MAX_RED_BLOCKS * 8]
 	// spreading
 	MarkerRec* rec = nullptr;          // [2 * mcap]
 	int* bins = nullptr;               // counts | offsets | cursors | chunk sums, 2 comps

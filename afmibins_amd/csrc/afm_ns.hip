@@ -93,14 +93,18 @@ __global__ void k_helm_apply(const double* __restrict__ x, double* __restrict__ 
 
 // ---------------------------------------------------------------------- CG
 // Per-field pointers, indexed by blockIdx.y (blockIdx.x in the one-block
-// finalize kernels).
+// finalize kernels).  Partial sums per field: the start writes two values
+// per block at pu (r.r) and pu + MAX_RED_BLOCKS (b.b), the p update one at
+// pp, the x / r update one at pu -- so a kernel never overwrites the
+// partials that it (or its finaliser step) is still reading.
 struct CGArgs {
 	double* x[2];
 	const double* b[2];
 	double* r[2];
 	double* p[2][2];       // ping-pong
 	SolveCtl* ctl[2];
-	double* partials[2];
+	double* pu[2];
+	double* pp[2];
 	int tol_mode;
 };
 
@@ -129,7 +133,7 @@ __global__ __launch_bounds__(256) void k_cg_init(CGArgs a, int nx, int ny, doubl
 		acc[0] += r * r;
 		acc[1] += b[i] * b[i];
 	}
-	reduce_partial<2>(acc, a.partials[f]);
+	reduce_partial<2>(acc, a.pu[f]);
 }
 
 __global__ __launch_bounds__(256) void k_cg_init_fin(CGArgs a, unsigned nblk)
@@ -137,31 +141,82 @@ __global__ __launch_bounds__(256) void k_cg_init_fin(CGArgs a, unsigned nblk)
 	const int f = blockIdx.x;
 	if (!a.x[f]) return;
 	double tot[2];
-	red_total<2>(a.partials[f], nblk, tot);
+	red_total<2>(a.pu[f], nblk, tot);
 	if (threadIdx.x == 0) {
 		SolveCtl* c = a.ctl[f];
 		if (a.tol_mode == 1) c->tol = c->tol * sqrt(tot[1]);
 		c->n0 = tot[0];
+		c->n0p[0] = tot[0];
 		c->count = 0;
 		c->res = sqrt(tot[0]);
 		c->done = !(sqrt(tot[0]) > c->tol && 0 < c->max_iter);
 	}
 }
 
-// p' = (count == 0) ? r : p*mu + r  (alg/cg.h:39 and 51-52 of the previous
-// iteration), stored; partial p'.(A p')
-__global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double off, double dg, int pin)
+// the stop flag as the block found it at its start, the same in all its waves
+// (with fold, block 0 of the same launch may set it meanwhile)
+__device__ inline bool block_done(const SolveCtl* c)
+{
+	__shared__ int d;
+	if (threadIdx.x == 0) d = c->done;
+	__syncthreads();
+	return d != 0;
+}
+
+// the sum of a kernel's per-block partials, in every thread of the block
+// (red_total's fixed order, so every block, and the one-block finalisers,
+// get the same bits)
+__device__ inline double block_total(const double* partials, unsigned nblk)
+{
+	__shared__ double t;
+	double tot[1];
+	red_total<1>(partials, nblk, tot);
+	if (threadIdx.x == 0) t = tot[0];
+	__syncthreads();
+	return t;
+}
+
+// The stop test after update `it` (alg/cg.h:42, 48-50): n1 = r.r from the
+// update's partials, mu = n1 / n0.  Written to ctl by one thread; n1 goes to
+// n0p[it & 1] while the old n0 is read from n0p[(it - 1) & 1], so the other
+// blocks of the same launch still read the old value.
+__device__ inline bool cg_stop_test(CGArgs& a, int f, int it, double n1, double& mu, bool writer)
+{
+	SolveCtl* c = a.ctl[f];
+	mu = n1 / c->n0p[(it - 1) & 1];
+	const bool done = !(sqrt(n1) > c->tol && it < c->max_iter);
+	if (writer) {
+		c->mu = mu;
+		c->n0 = n1;
+		c->n0p[it & 1] = n1;
+		c->count = it;
+		c->res = sqrt(n1);
+		c->done = done;
+	}
+	return done;
+}
+
+// Iteration `it` (0-based), first half: p' = (it == 0) ? r : p*mu + r
+// (alg/cg.h:39 and 51-52 of the previous iteration), stored; partial p'.(A p').
+// With fold, every block first does the previous iteration's stop test from
+// the update's partials (block 0 records it), instead of a separate launch.
+__global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double off, double dg, int pin, int it,
+                                              int fold, unsigned nblk)
 {
 	const int f = blockIdx.y;
 	if (!a.x[f]) return;
 	SolveCtl* c = a.ctl[f];
-	if (c->done) return;
+	if (block_done(c)) return;
+	double mu = c->mu;
+	if (fold) {
+		const double n1 = block_total(a.pu[f], nblk);
+		if (cg_stop_test(a, f, it, n1, mu, blockIdx.x == 0 && threadIdx.x == 0)) return;
+	}
 	const size_t N = (size_t) nx * ny;
 	const double* p = a.p[f][pin];
 	double* po = a.p[f][pin ^ 1];
 	const double* r = a.r[f];
-	const bool first = c->count == 0;
-	const double mu = c->mu;
+	const bool first = it == 0;
 	double acc[1] = {0.0};
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
 		CG_NEIGHBOURS
@@ -175,28 +230,21 @@ __global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double o
 		po[i] = pc;
 		acc[0] += pc * z;
 	}
-	reduce_partial<1>(acc, a.partials[f]);
+	reduce_partial<1>(acc, a.pp[f]);
 }
 
-// nu = n0 / (p.z)
-__global__ __launch_bounds__(256) void k_cg_p_fin(CGArgs a, unsigned nblk)
-{
-	const int f = blockIdx.x;
-	if (!a.x[f] || a.ctl[f]->done) return;
-	double tot[1];
-	red_total<1>(a.partials[f], nblk, tot);
-	if (threadIdx.x == 0) a.ctl[f]->nu = a.ctl[f]->n0 / tot[0];
-}
-
-// z = A p (recomputed, same bits), x += p nu, r -= z nu; partial r.r
-__global__ __launch_bounds__(256) void k_cg_update(CGArgs a, int nx, int ny, double off, double dg, int pcur)
+// Second half: nu = n0 / (p.z) from the p kernel's partials (every block,
+// same bits), z = A p (recomputed, same bits), x += p nu, r -= z nu;
+// partial r.r
+__global__ __launch_bounds__(256) void k_cg_update(CGArgs a, int nx, int ny, double off, double dg, int pcur, int it,
+                                                   unsigned nblk)
 {
 	const int f = blockIdx.y;
 	if (!a.x[f]) return;
 	SolveCtl* c = a.ctl[f];
 	if (c->done) return;
+	const double nu = c->n0p[it & 1] / block_total(a.pp[f], nblk);
 	const size_t N = (size_t) nx * ny;
-	const double nu = c->nu;
 	double* x = a.x[f];
 	double* r = a.r[f];
 	const double* p = a.p[f][pcur];
@@ -211,25 +259,18 @@ __global__ __launch_bounds__(256) void k_cg_update(CGArgs a, int nx, int ny, dou
 		r[i] = ri;
 		acc[0] += ri * ri;
 	}
-	reduce_partial<1>(acc, a.partials[f]);
+	reduce_partial<1>(acc, a.pu[f]);
 }
 
-// mu = n1/n0 ; n0 = n1 ; stop test (alg/cg.h:42, 48-50)
-__global__ __launch_bounds__(256) void k_cg_update_fin(CGArgs a, unsigned nblk)
+// the stop test after update `it`, as its own launch (the last iteration
+// of a batch, before the host reads the control block)
+__global__ __launch_bounds__(256) void k_cg_update_fin(CGArgs a, int it, unsigned nblk)
 {
 	const int f = blockIdx.x;
 	if (!a.x[f] || a.ctl[f]->done) return;
-	double tot[1];
-	red_total<1>(a.partials[f], nblk, tot);
-	if (threadIdx.x == 0) {
-		SolveCtl* c = a.ctl[f];
-		double n1 = tot[0];
-		c->mu = n1 / c->n0;
-		c->n0 = n1;
-		c->count += 1;
-		c->res = sqrt(n1);
-		c->done = !(sqrt(n1) > c->tol && c->count < c->max_iter);
-	}
+	const double n1 = block_total(a.pu[f], nblk);
+	double mu;
+	cg_stop_test(a, f, it, n1, mu, threadIdx.x == 0);
 }
 
 __global__ __launch_bounds__(256) void k_sum(const double* __restrict__ x, size_t N, double* partials)
@@ -291,8 +332,8 @@ __global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* 
 		acc_v[0] += r_v * r_v;
 		acc_v[1] += ry * ry;
 	}
-	reduce_partial<2>(acc_u, a.partials[0]);
-	reduce_partial<2>(acc_v, a.partials[1]);
+	reduce_partial<2>(acc_u, a.pu[0]);
+	reduce_partial<2>(acc_v, a.pu[1]);
 }
 
 static inline unsigned rblocks(size_t N)
@@ -399,7 +440,8 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 		a.p[f][0] = w.p[f][0];
 		a.p[f][1] = w.p[f][1];
 		a.ctl[f] = &c->ctl[1 + f];
-		a.partials[f] = c->partials + (size_t) f * 2 * MAX_RED_BLOCKS;
+		a.pu[f] = c->partials + (size_t) f * 4 * MAX_RED_BLOCKS;
+		a.pp[f] = a.pu[f] + 2 * MAX_RED_BLOCKS;
 		SolveCtl h{};
 		h.tol = tol; h.max_iter = max_iters; h.done = x[f] ? 0 : 1;
 		AFM_HIP(hipMemcpyAsync(&c->ctl[1 + f], &h, sizeof(SolveCtl), hipMemcpyHostToDevice, c->stream));
@@ -417,15 +459,19 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 	int batch = c->last_cg_iters > 0 ? c->last_cg_iters : 4;
 	for (;;) {
 		if (batch > max_iters - enq) batch = max_iters - enq;
+		// two launches per iteration: each folds the finaliser of the one
+		// before it; the batch's last update is finalised on its own
 		for (int s = 0; s < batch; ++s) {
-			int pin = enq & 1;
-			hipLaunchKernelGGL(k_cg_p, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg, pin);
-			hipLaunchKernelGGL(k_cg_p_fin, dim3(2), dim3(256), 0, c->stream, a, rb);
-			hipLaunchKernelGGL(k_cg_update, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg, pin ^ 1);
-			hipLaunchKernelGGL(k_cg_update_fin, dim3(2), dim3(256), 0, c->stream, a, rb);
+			const int pin = enq & 1;
+			hipLaunchKernelGGL(k_cg_p, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg, pin,
+			                   enq, s > 0 ? 1 : 0, rb);
+			hipLaunchKernelGGL(k_cg_update, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg,
+			                   pin ^ 1, enq, rb);
 			AFM_LAUNCH_CHECK();
 			++enq;
 		}
+		hipLaunchKernelGGL(k_cg_update_fin, dim3(2), dim3(256), 0, c->stream, a, enq, rb);
+		AFM_LAUNCH_CHECK();
 		AFM_HIP(hipMemcpyAsync(&c->hctl[1], &c->ctl[1], 2 * sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
 		AFM_HIP(hipStreamSynchronize(c->stream));
 		bool all = c->hctl[1].done && c->hctl[2].done;
