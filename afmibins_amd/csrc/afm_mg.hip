@@ -382,11 +382,57 @@ __device__ inline void smooth_body(const T* __restrict__ xin, const T* __restric
 	if (EPI == EPI_RESTRICT) st.restrict_to(rc, bnd + bnd_size<NW>(), bnd + 2 * bnd_size<NW>(), X0, Y0, nx, ny, off, dg);
 }
 
+// The multigrid stop test of the previous V-cycle (k_update_fin's work),
+// folded into the first launch of the next cycle: block 0 alone runs it and
+// nothing else in that launch reads the control block.  The cycle index
+// comes from the host, so no block reads a count being written.
+struct MGFold {
+	SolveCtl* ctl;            // nullptr: no test in this launch
+	const double* partials;   // the update kernel's r^2 per block
+	unsigned nblk;
+	int it;                   // cycles completed once the tested one is counted
+	double N;                 // cells of the finest level
+};
+
+// red_total over a 256-thread block, computed by the first four waves of a
+// larger one (same order, so the same bits as k_update_fin); thread 0
+__device__ inline double red_total_256(const double* __restrict__ partials, unsigned nblk)
+{
+	__shared__ double wsum[4];
+	const int tid = (int) threadIdx.x;
+	double s = 0.0;
+	if (tid < 256)
+		for (unsigned b = tid; b < nblk; b += 256) s += partials[b];
+	for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+	if (tid < 256 && (tid & 63) == 0) wsum[tid >> 6] = s;
+	__syncthreads();
+	double t = 0.0;
+	if (tid == 0)
+		for (int w = 0; w < 4; ++w) t += wsum[w];
+	return t;
+}
+
+__device__ inline void mg_fold_test(const MGFold& f)
+{
+	__shared__ int already;
+	if (threadIdx.x == 0) already = f.ctl->done;
+	__syncthreads();
+	if (already) return;
+	const double tot = red_total_256(f.partials, f.nblk);
+	if (threadIdx.x == 0) {
+		SolveCtl* c = f.ctl;
+		const double rms = sqrt(tot) / sqrt(f.N);
+		c->count = f.it;
+		c->res = rms;
+		c->done = !(rms > c->tol && f.it < c->max_iter);
+	}
+}
+
 template<typename T, int S, int MODE, int EPI, int SH>
 __global__ __launch_bounds__((64 * SmoothShape<S, SH>::NW)) void k_smooth(const T* __restrict__ xin, const T* __restrict__ b,
                                                                     const T* __restrict__ xc, T* __restrict__ xout,
                                                                     T* __restrict__ rc, int nx, int ny, T off, T dg,
-                                                                    T inv, const int* __restrict__ done)
+                                                                    T inv, const int* __restrict__ done, MGFold fold)
 {
 	constexpr int NW = SmoothShape<S, SH>::NW;
 	__shared__ T bnd[(EPI == EPI_RESTRICT ? 3 : 1) * bnd_size<NW>()];
@@ -396,6 +442,9 @@ __global__ __launch_bounds__((64 * SmoothShape<S, SH>::NW)) void k_smooth(const 
 		smooth_body<T, S, MODE, EPI, true, SH>(xin, b, xc, xout, rc, nx, ny, off, dg, inv, X0, Y0, bnd);
 	else
 		smooth_body<T, S, MODE, EPI, false, SH>(xin, b, xc, xout, rc, nx, ny, off, dg, inv, X0, Y0, bnd);
+	// after the block's own tile: block 0 is among the first to start, so the
+	// test overlaps the rest of the grid
+	if (fold.ctl && blockIdx.x == 0) mg_fold_test(fold);
 }
 
 // Finest-level post-smoothing fused with the outer update: e = smooth(e_pre +
@@ -957,15 +1006,17 @@ static inline unsigned red_blocks(size_t N)
 // also rc = R (b - L x) on level l-1 (mode must be M_ZERO then).
 template<typename T>
 static int smooth_launch(Ctx* c, unsigned l, int mode, int sweeps, const T* xin, const T* b, const T* xc, T* xout,
-                         const int* done, T* rc = nullptr)
+                         const int* done, T* rc = nullptr, const MGFold* fold = nullptr)
 {
+	const MGFold fd = fold ? *fold : MGFold{nullptr, nullptr, 0, 0, 0.0};
 	const LevelBuf& L = c->lv[l];
 	const double n2 = (double) L.n * (double) L.n;
 	const T off = (T) n2, dg = (T) (-6.0 * n2 + n2 + n2), inv = T(1) / dg;   // dg = -4 n^2, a power of two
 	const int nx = (int) L.nx, ny = (int) L.ny;
 #define AFM_SM2(S_, M_, E_, SH_)                                                                               \
 	hipLaunchKernelGGL((k_smooth<T, S_, M_, E_, SH_>), dim3(smooth_blocks<S_, E_, SH_>(nx, ny)),              \
-	                   dim3(64 * SmoothShape<S_, SH_>::NW), 0, c->stream, xin, b, xc, xout, rc, nx, ny, off, dg, inv, done)
+	                   dim3(64 * SmoothShape<S_, SH_>::NW), 0, c->stream, xin, b, xc, xout, rc, nx, ny, off, dg, inv, done,  \
+	                   fd)
 #define AFM_SM(S_, M_, E_)                          \
 	do {                                            \
 		if (small) AFM_SM2(S_, M_, E_, 1);          \
@@ -1068,6 +1119,7 @@ struct UpdateArgs {
 	SolveCtl* ctl;
 	double* partials;       // one r^2 sum per block
 	int slot;               // timing event pair, or -1
+	const MGFold* fold;     // the previous cycle's stop test, or nullptr
 };
 
 // recurse(l, B_l) -> X_l (final in Xb); rhs of level l given explicitly.
@@ -1087,7 +1139,7 @@ static int vcycle(Ctx* c, unsigned l, const T* rhs_l, int nu1, int nu2, const in
 		const T* bk = (k == l) ? rhs_l : lvB<T>(c->lv[k]);
 		// pre-smoothing from zero, residual and restriction in one launch
 		AFM_TRY(smooth_launch<T>(c, k, M_ZERO, nu1, nullptr, bk, nullptr, lvX<T>(c->lv[k], 0), done,
-		                         lvB<T>(c->lv[k - 1])));
+		                         lvB<T>(c->lv[k - 1]), (k == l && up) ? up->fold : nullptr));
 	}
 	AFM_TRY(coarse_launch<T>(c, 0, (int) lc, lvB<T>(c->lv[lc]), lvX<T>(c->lv[lc], 1), nu1, nu2, done));
 	for (unsigned k = lc + 1; k <= l; ++k) {
@@ -1277,9 +1329,14 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 			double* cur = xs[enq & 1];
 			double* nxt = xs[(enq + 1) & 1];
 			const int rc = fused ? (enq & 1) : 0, rn = fused ? ((enq + 1) & 1) : 0;
+			// the stop test of the cycle before rides on this cycle's first
+			// launch (all but the batch's first cycle; the batch's last cycle
+			// is tested by its own launch below, before the host reads it)
+			const MGFold fold{dctl, c->upartials, ub, enq, (double) N};
+			const MGFold* fp = s > 0 ? &fold : nullptr;
 			if (precision == 0) {
 				if (fused) {
-					const UpdateArgs<double> up{cur, rhs, nxt, res64[rn], dctl, c->upartials, slot};
+					const UpdateArgs<double> up{cur, rhs, nxt, res64[rn], dctl, c->upartials, slot, fp};
 					AFM_TRY(vcycle<double>(c, T, res64[rc], nu1, nu2, done, &up));
 				} else {
 					AFM_TRY(vcycle<double>(c, T, res64[rc], nu1, nu2, done));
@@ -1289,7 +1346,7 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 				}
 			} else {
 				if (fused) {
-					const UpdateArgs<float> up{cur, rhs, nxt, res32[rn], dctl, c->upartials, slot};
+					const UpdateArgs<float> up{cur, rhs, nxt, res32[rn], dctl, c->upartials, slot, fp};
 					AFM_TRY(vcycle<float>(c, T, res32[rc], nu1, nu2, done, &up));
 				} else {
 					AFM_TRY(vcycle<float>(c, T, res32[rc], nu1, nu2, done));
@@ -1298,9 +1355,13 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 					                   0, dctl, c->upartials);
 				}
 			}
-			hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, 0, ub, N, dctl, c->upartials);
+			if (!fused) hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, 0, ub, N, dctl, c->upartials);
 			AFM_LAUNCH_CHECK();
 			++enq;
+		}
+		if (fused) {
+			hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, 0, ub, N, dctl, c->upartials);
+			AFM_LAUNCH_CHECK();
 		}
 		AFM_TRY(read_ctl(c, 0));
 		if (c->kt_on && fused) {
