@@ -332,15 +332,20 @@ struct Strip {
 	}
 };
 
-// Waves per block and rows per wave.  SH = 0 for the large levels (fewer
-// halo rows per output row); SH = 1 for levels of at most SMALL_CELLS cells,
-// whose launches are few blocks on an idle chip: there a block's serial
-// chain of row updates sets the time, so each wave holds fewer rows.
+// Waves per block and rows per wave.  SH = 0 for the large levels (tall
+// 1024-thread tiles: few halo rows per output row); SH = 1 for levels of at
+// most SMALL_CELLS cells, whose launches are few blocks on an idle chip:
+// there a block's serial chain of row updates sets the time, so each wave
+// holds fewer rows; SH = 2 for plain smoothing launches (M_NORMAL, the
+// smoother entry point) at the large levels, which stream fastest in
+// 256-thread tiles.
 template<int S, int SH = 0> struct SmoothShape;
 template<> struct SmoothShape<1, 0> { static constexpr int NW = 6, RPW = 6; };
 template<> struct SmoothShape<2, 0> { static constexpr int NW = 16, RPW = 4; };
 template<> struct SmoothShape<1, 1> { static constexpr int NW = 8, RPW = 2; };
 template<> struct SmoothShape<2, 1> { static constexpr int NW = 8, RPW = 2; };
+template<> struct SmoothShape<1, 2> { static constexpr int NW = 6, RPW = 6; };
+template<> struct SmoothShape<2, 2> { static constexpr int NW = 4, RPW = 10; };
 constexpr size_t SMALL_CELLS = (size_t) 512 * 512;
 
 // halo of a launch: 2 cells per sweep, 2 more for a residual epilogue
@@ -1022,6 +1027,11 @@ static int smooth_launch(Ctx* c, unsigned l, int mode, int sweeps, const T* xin,
 		if (small) AFM_SM2(S_, M_, E_, 1);          \
 		else AFM_SM2(S_, M_, E_, 0);                \
 	} while (0)
+#define AFM_SMN(S_)                                         \
+	do {                                                    \
+		if (small) AFM_SM2(S_, M_NORMAL, EPI_STORE, 1);     \
+		else AFM_SM2(S_, M_NORMAL, EPI_STORE, 2);           \
+	} while (0)
 	const bool small = (size_t) nx * ny <= SMALL_CELLS;
 	if (rc) {
 		if (mode != M_ZERO) { set_error("fused restriction needs a zero start"); return -1; }
@@ -1029,16 +1039,17 @@ static int smooth_launch(Ctx* c, unsigned l, int mode, int sweeps, const T* xin,
 		else if (sweeps == 2) AFM_SM(2, M_ZERO, EPI_RESTRICT);
 		else { set_error("smoother supports 1 or 2 sweeps per launch"); return -1; }
 	} else if (sweeps == 1) {
-		switch (mode) { case M_NORMAL: AFM_SM(1, M_NORMAL, EPI_STORE); break; case M_ZERO: AFM_SM(1, M_ZERO, EPI_STORE); break;
+		switch (mode) { case M_NORMAL: AFM_SMN(1); break; case M_ZERO: AFM_SM(1, M_ZERO, EPI_STORE); break;
 		                case M_PADD: AFM_SM(1, M_PADD, EPI_STORE); break; default: AFM_SM(1, M_PSET, EPI_STORE); }
 	} else if (sweeps == 2) {
-		switch (mode) { case M_NORMAL: AFM_SM(2, M_NORMAL, EPI_STORE); break; case M_ZERO: AFM_SM(2, M_ZERO, EPI_STORE); break;
+		switch (mode) { case M_NORMAL: AFM_SMN(2); break; case M_ZERO: AFM_SM(2, M_ZERO, EPI_STORE); break;
 		                case M_PADD: AFM_SM(2, M_PADD, EPI_STORE); break; default: AFM_SM(2, M_PSET, EPI_STORE); }
 	} else {
 		set_error("smoother supports 1 or 2 sweeps per launch");
 		return -1;
 	}
 #undef AFM_SM
+#undef AFM_SMN
 #undef AFM_SM2
 	AFM_LAUNCH_CHECK();
 	return 0;

@@ -135,7 +135,8 @@ def test_laplacian_bit_exact(ctx_of, oracle, dev, n):
     assert np.array_equal(got.ravel(), oracle.stencil_apply(x, n, 1, n, n))
 
 
-# 64..512 run the short-strip tile shape, 1024 the tall one
+# 64..512 run the short-strip tile shape; at 1024 plain smoothing runs the
+# 4 x 10-row tiles and the zero-start launches the tall 16 x 4-row ones
 @pytest.mark.parametrize("n", [64, 128, 512, 1024])
 @pytest.mark.parametrize("sweeps,zero", [(1, False), (2, False), (2, True), (1, True)])
 def test_rb_smoother_bit_exact(ctx_of, oracle, dev, n, sweeps, zero):
