@@ -585,7 +585,9 @@ __global__ __launch_bounds__(256) void k_update_resid(const double* __restrict__
 	if (!first && ctl->done) return;
 	const size_t N = (size_t) nx * ny;
 	const double mean = ctl->mean;
-	double acc[1] = {0.0};
+	// acc[1]: sum of (rhs - mean)^2 for a relative tolerance (with sub_mean:
+	// the same terms in the same order as a separate pass over rhs would add)
+	double acc[2] = {0.0, 0.0};
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
 		int gx, gy;
 	cell_xy(i, nx, gx, gy);
@@ -602,23 +604,35 @@ __global__ __launch_bounds__(256) void k_update_resid(const double* __restrict__
 			xs_out[i] = c;
 		}
 		double r0 = rhs[i];
-		if (sub_mean) { r0 = r0 - mean * 1.0; rhs[i] = r0; }
+		if (sub_mean) { r0 = r0 - mean * 1.0; rhs[i] = r0; acc[1] += r0 * r0; }
 		double Lx = stencil5<double, true>(off, dg, c, E, Wv, Nv, Sv, gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
 		double r = r0 - Lx;
 		res[i] = (TR) r;
 		acc[0] += r * r;
 	}
-	reduce_partial<1>(acc, partials);
+	if (sub_mean) reduce_partial<2>(acc, partials);
+	else {
+		double a1[1] = {acc[0]};
+		reduce_partial<1>(a1, partials);
+	}
 }
 
 // RMS of the residual and the stop test (mg/solver.h:228), one block.
+// first == 2: the first call with a relative tolerance, which is rel times
+// the RMS of rhs - mean summed by k_update_resid (partials[MAX_RED_BLOCKS..])
 __global__ __launch_bounds__(256) void k_update_fin(int first, unsigned nblk, size_t N, SolveCtl* __restrict__ ctl,
-                                                    const double* __restrict__ partials)
+                                                    const double* __restrict__ partials, double rel = 0.0)
 {
 	if (!first && ctl->done) return;
-	double tot[1];
-	red_total<1>(partials, nblk, tot);
+	double tot[2];
+	if (first == 2) red_total<2>(partials, nblk, tot);
+	else {
+		double t1[1];
+		red_total<1>(partials, nblk, t1);
+		tot[0] = t1[0];
+	}
 	if (threadIdx.x == 0) {
+		if (first == 2) ctl->tol = rel * (sqrt(tot[1]) / sqrt((double) N));
 		double rms = sqrt(tot[0]) / sqrt((double) N);
 		int count = first ? 0 : ctl->count + 1;
 		ctl->count = count;
@@ -629,8 +643,19 @@ __global__ __launch_bounds__(256) void k_update_fin(int first, unsigned nblk, si
 
 // sum(rhs * 1) -> mean (pass 0); RMS(rhs - mean) -> relative tol (pass 1)
 __global__ __launch_bounds__(256) void k_mean(const double* __restrict__ rhs, size_t N, int pass,
-                                              SolveCtl* __restrict__ ctl, double* __restrict__ partials)
+                                              SolveCtl* __restrict__ ctl, double* __restrict__ partials,
+                                              double tol = 0.0, int max_iter = 0)
 {
+	// pass 0 also starts the solve's control block (nothing in this launch
+	// reads it)
+	if (pass == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+		ctl->tol = tol;
+		ctl->max_iter = max_iter;
+		ctl->count = 0;
+		ctl->done = 0;
+		ctl->err = 0;
+		ctl->res = 0.0;
+	}
 	double acc[1] = {0.0};
 	const double mean = pass ? ctl->mean : 0.0;
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
@@ -1287,22 +1312,18 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 	if (nu1 < 1 || nu1 > 2 || nu2 < 1 || nu2 > 2) { set_error("nu1, nu2 must be 1 or 2 (one fused launch per smoothing step)"); return -1; }
 	if (precision != 0 && precision != 1) { set_error("precision must be 0 (fp64) or 1 (fp32 V-cycle in fp64 refinement)"); return -1; }
 	SolveCtl* dctl = &c->ctl[0];
-	SolveCtl& h = c->hctl[0];      // pinned staging
-	h = SolveCtl{};
-	h.tol = tol; h.max_iter = max_cycles;
-	AFM_HIP(hipMemcpyAsync(dctl, &h, sizeof(SolveCtl), hipMemcpyHostToDevice, c->stream));
 	const size_t N = c->N;
 	const unsigned rb = red_blocks(N);
 	const unsigned T = c->levels;
 	const double off = (double) (c->n * c->n), dg = -4.0 * off;
-	// mean (and, for a relative tolerance, RMS of rhs - mean) before anything
-	// else, as the oracle computes it (mg/solver.h:224 does the mean after the
-	// nested start; the value is the same because rhs is not touched there).
-	for (int pass = 0; pass < (tol_mode == 1 ? 2 : 1); ++pass) {
-		hipLaunchKernelGGL(k_mean, dim3(rb), dim3(256), 0, c->stream, rhs, N, pass, dctl, c->partials);
-		hipLaunchKernelGGL(k_mean_fin, dim3(1), dim3(256), 0, c->stream, pass, tol, rb, N, dctl, c->partials);
-		AFM_LAUNCH_CHECK();
-	}
+	// the mean before anything else, as the oracle computes it
+	// (mg/solver.h:224 does the mean after the nested start; the value is the
+	// same because rhs is not touched there); for a relative tolerance the RMS
+	// of rhs - mean is summed by the start's residual launch below
+	hipLaunchKernelGGL(k_mean, dim3(rb), dim3(256), 0, c->stream, rhs, N, 0, dctl, c->partials,
+	                   tol_mode == 1 ? 0.0 : tol, max_cycles);
+	hipLaunchKernelGGL(k_mean_fin, dim3(1), dim3(256), 0, c->stream, 0, tol, rb, N, dctl, c->partials);
+	AFM_LAUNCH_CHECK();
 	// nested start and its residual, mean removed from rhs in place
 	if (precision == 1 && T > c->lc) {
 		AFM_TRY(nested32(c, rhs, nu1, nu2));
@@ -1320,7 +1341,8 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 			                   (const float*) nullptr, rhs, c->XSa, c->RES32, (int) c->nx, (int) c->ny, off, dg, 1, 1,
 			                   dctl, c->partials);
 	}
-	hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, 1, rb, N, dctl, c->partials);
+	hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, tol_mode == 1 ? 2 : 1, rb, N, dctl, c->partials,
+	                   tol);
 	AFM_LAUNCH_CHECK();
 	const int* done = &dctl->done;
 	// above the LDS levels the update rides on the last post-smoothing

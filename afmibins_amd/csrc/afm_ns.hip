@@ -115,11 +115,25 @@ struct CGArgs {
 	const size_t in_ = gy == ny - 1 ? (size_t) gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;    \
 	const bool we = gx == nx - 1, ww = gx == 0, wn = gy == ny - 1, ws = gy == 0;
 
+// Start field f's control block (block 0 of the start launch; the stop test
+// of a field not being solved reads as fired).
+__device__ inline void cg_ctl_start(CGArgs& a, int f, double tol, int max_iters)
+{
+	SolveCtl* c = a.ctl[f];
+	c->tol = tol;
+	c->max_iter = max_iters;
+	c->count = 0;
+	c->done = a.x[f] ? 0 : 1;
+	c->err = 0;
+}
+
 // r = b - A x, n0 = r.r  (alg/cg.h:38-39); |b|^2 for a relative tol.  p = r
 // is not stored: the first p update reads r.
-__global__ __launch_bounds__(256) void k_cg_init(CGArgs a, int nx, int ny, double off, double dg)
+__global__ __launch_bounds__(256) void k_cg_init(CGArgs a, int nx, int ny, double off, double dg, double tol,
+                                                 int max_iters)
 {
 	const int f = blockIdx.y;
+	if (blockIdx.x == 0 && threadIdx.x == 0) cg_ctl_start(a, f, tol, max_iters);
 	if (!a.x[f]) return;
 	const size_t N = (size_t) nx * ny;
 	const double* x = a.x[f];
@@ -295,8 +309,12 @@ __global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* 
                                                         const double* __restrict__ fy, double* __restrict__ uo,
                                                         double* __restrict__ vo, int nx, int ny, unsigned n, double dt,
                                                         double rho, int be, double hoff, double hdg, double off,
-                                                        double dg)
+                                                        double dg, double tol, int max_iters)
 {
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		cg_ctl_start(a, 0, tol, max_iters);
+		cg_ctl_start(a, 1, tol, max_iters);
+	}
 	const size_t N = (size_t) nx * ny;
 	const double* u = a.x[0];
 	const double* v = a.x[1];
@@ -442,17 +460,16 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 		a.ctl[f] = &c->ctl[1 + f];
 		a.pu[f] = c->partials + (size_t) f * 4 * MAX_RED_BLOCKS;
 		a.pp[f] = a.pu[f] + 2 * MAX_RED_BLOCKS;
-		SolveCtl h{};
-		h.tol = tol; h.max_iter = max_iters; h.done = x[f] ? 0 : 1;
-		AFM_HIP(hipMemcpyAsync(&c->ctl[1 + f], &h, sizeof(SolveCtl), hipMemcpyHostToDevice, c->stream));
 	}
 	a.tol_mode = tol_mode;
 	const unsigned rb = rblocks(c->N);
 	if (adv)
 		hipLaunchKernelGGL(k_advect_cg_init, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy, adv->uo, adv->vo,
-		                   (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff, adv->hdg, off, dg);
+		                   (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff, adv->hdg, off, dg,
+		                   tol, max_iters);
 	else
-		hipLaunchKernelGGL(k_cg_init, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg);
+		hipLaunchKernelGGL(k_cg_init, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg, tol,
+		                   max_iters);
 	hipLaunchKernelGGL(k_cg_init_fin, dim3(2), dim3(256), 0, c->stream, a, rb);
 	AFM_LAUNCH_CHECK();
 	int enq = 0;
