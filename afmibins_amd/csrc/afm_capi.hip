@@ -105,6 +105,9 @@ int afm_ctx_create(const afm_geom* g, int device, afm_ctx** out)
 		// counts, offsets (+1), cursors and up to 1024 scan-chunk sums for both components
 		const size_t nb = 2 * (size_t) c->sbx * c->sby;
 		if ((rc = c->alloc((void**) &c->bins, sizeof(int) * (3 * nb + 1 + 1024)))) return fail(rc);
+		// the bin counts start at zero once; each spreading's scan returns
+		// them to zero after reading them
+		if (hipMemsetAsync(c->bins, 0, sizeof(int) * nb, c->stream) != hipSuccess) { set_error("memset failed"); return fail(-3); }
 	}
 	if ((rc = mg_setup(c))) return fail(rc);
 	if (hipStreamSynchronize(c->stream) != hipSuccess) { set_error("sync failed"); return fail(-3); }

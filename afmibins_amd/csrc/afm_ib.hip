@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_spread_chunk(const int* __restrict__ co
 	if (threadIdx.x == 0) chunk[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(256) void k_spread_scan(const int* __restrict__ counts, const int* __restrict__ chunk,
+__global__ __launch_bounds__(256) void k_spread_scan(int* __restrict__ counts, const int* __restrict__ chunk,
                                                      int* __restrict__ offs, int* __restrict__ cursor, int nbins,
                                                      int items)
 {
@@ -189,6 +189,7 @@ __global__ __launch_bounds__(256) void k_spread_scan(const int* __restrict__ cou
 			const int v = counts[lo + j];
 			offs[lo + j] = acc;
 			cursor[lo + j] = 0;
+			counts[lo + j] = 0;   // the next spreading's counts start at zero
 			acc += v;
 		}
 	// the lane holding the last bin closes the table
@@ -498,7 +499,6 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 	int* chunk = cursor + nb;          // nchunk entries
 	if (m > 0) {
 		AFM_TRY(ensure_markers(c, m));
-		AFM_HIP(hipMemsetAsync(counts, 0, sizeof(int) * nb, c->stream));
 		dim3 g((m + 255) / 256, 2), g8((8 * m + 255) / 256, 2);
 		hipLaunchKernelGGL(k_spread_prep, g8, dim3(256), 0, c->stream, comp_mask, Fx, Fy, ds, X, R, k, m, c->n, c->W,
 		                   c->D, c->nx, c->ny, c->sbx, nb1, c->rec, counts);

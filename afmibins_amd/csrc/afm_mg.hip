@@ -1307,7 +1307,7 @@ static int read_ctl(Ctx* c, int i)
 }
 
 int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, int precision, int max_cycles,
-             int* cycles, double* final_rms)
+             int* cycles, double* final_rms, const double** sol)
 {
 	if (nu1 < 1 || nu1 > 2 || nu2 < 1 || nu2 > 2) { set_error("nu1, nu2 must be 1 or 2 (one fused launch per smoothing step)"); return -1; }
 	if (precision != 0 && precision != 1) { set_error("precision must be 0 (fp64) or 1 (fp32 V-cycle in fp64 refinement)"); return -1; }
@@ -1410,7 +1410,10 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 		batch = 1;                  // beyond the prediction, one cycle at a time
 	}
 	const int cnt = c->hctl[0].count;
-	AFM_HIP(hipMemcpyAsync(rhs, xs[cnt & 1], N * 8, hipMemcpyDeviceToDevice, c->stream));
+	// the solution replaces rhs (mg/solver.h:218-237), unless the caller
+	// takes it where it is (valid until the context's next solve)
+	if (sol) *sol = xs[cnt & 1];
+	else AFM_HIP(hipMemcpyAsync(rhs, xs[cnt & 1], N * 8, hipMemcpyDeviceToDevice, c->stream));
 	c->last_mg_cycles = cnt;
 	if (cycles) *cycles = cnt;
 	if (final_rms) *final_rms = c->hctl[0].res;

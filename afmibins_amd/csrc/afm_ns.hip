@@ -64,8 +64,10 @@ __global__ void k_div(const double* __restrict__ u, const double* __restrict__ v
 }
 
 // u -= n (p - p[x-1]),  v -= n (p - p[y-1])   (alg/grad.h:36-43, ns/projection.h:70-72)
+// u -= n (p - p_west), v -= n (p - p_south); with pcopy, p is also copied
+// there (the projection's potential kept for get_p)
 __global__ void k_grad_sub(const double* __restrict__ p, double* __restrict__ u, double* __restrict__ v, int nx, int ny,
-                           unsigned n)
+                           unsigned n, double* __restrict__ pcopy)
 {
 	const size_t N = (size_t) nx * ny;
 	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
@@ -77,6 +79,7 @@ __global__ void k_grad_sub(const double* __restrict__ p, double* __restrict__ u,
 	const double pz = n * (p[i] - p[izm]);
 	u[i] = u[i] - px;
 	v[i] = v[i] - pz;
+	if (pcopy) pcopy[i] = p[i];
 }
 
 __global__ void k_helm_apply(const double* __restrict__ x, double* __restrict__ y, int nx, int ny, double off, double dg)
@@ -385,7 +388,8 @@ int op_div(Ctx* c, const double* u, const double* v, double* d)
 
 int op_grad_sub(Ctx* c, const double* p, double* u, double* v)
 {
-	hipLaunchKernelGGL(k_grad_sub, pgrid(c->N), dim3(256), 0, c->stream, p, u, v, (int) c->nx, (int) c->ny, c->n);
+	hipLaunchKernelGGL(k_grad_sub, pgrid(c->N), dim3(256), 0, c->stream, p, u, v, (int) c->nx, (int) c->ny, c->n,
+	                   (double*) nullptr);
 	AFM_LAUNCH_CHECK();
 	return 0;
 }
@@ -527,9 +531,14 @@ int ns_project(NS* s, double* u, double* v)
 	Ctx* c = s->c;
 	AFM_TRY(op_div(c, u, v, s->phi));
 	int cyc = 0;
-	AFM_TRY(mg_solve(c, s->phi, s->tol, s->tol_mode, 2, 2, s->precision, s->max_cycles, &cyc, nullptr));
+	const double* sol = nullptr;
+	AFM_TRY(mg_solve(c, s->phi, s->tol, s->tol_mode, 2, 2, s->precision, s->max_cycles, &cyc, nullptr, &sol));
 	s->last_cycles = cyc;
-	AFM_TRY(op_grad_sub(c, s->phi, u, v));
+	// the gradient is taken where the solve left the potential, and the
+	// potential is stored to phi in the same pass
+	hipLaunchKernelGGL(k_grad_sub, pgrid(c->N), dim3(256), 0, c->stream, sol, u, v, (int) c->nx, (int) c->ny, c->n,
+	                   s->phi);
+	AFM_LAUNCH_CHECK();
 	return 0;
 }
 
