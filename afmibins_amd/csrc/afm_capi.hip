@@ -345,7 +345,7 @@ int afm_ns_create(afm_ctx* c, double mu, double rho, double dt, double tol, int 
 	CHECK_ARG(c && u0 && v0 && out, "null argument");
 	CHECK_ARG(rho > 0 && dt > 0 && mu >= 0 && tol >= 0, "need rho > 0, dt > 0, mu >= 0, tol >= 0");
 	CHECK_ARG(tol_mode == 0 || tol_mode == 1, "tol_mode must be 0 or 1");
-	CHECK_ARG(precision == 0 || precision == 1, "precision must be 0 or 1");
+	CHECK_ARG(precision >= 0 && precision <= 3, "precision must be 0, 1, 2 or 3 (as for afm_mg_solve)");
 	AFM_TRY(use(c));
 	NS* s = nullptr;
 	AFM_TRY(ns_create(c, mu, rho, dt, tol, tol_mode, precision, u0, v0, &s));

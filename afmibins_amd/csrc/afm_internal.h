@@ -50,6 +50,11 @@ struct SolveCtl {
 	double n0p[2];    // CG: r.r after k updates in n0p[k & 1] (read and written by consecutive kernels)
 	double mean;      // mean removed from the MG right-hand side
 	double sum2;      // scratch second sum
+	// flexible CG around the multigrid preconditioner: (z.r, alpha) of
+	// iteration k in fg[k & 1]; p_{k-1}.r_k and p_{k-1}.L p_{k-1} after the
+	// stop test of iteration k-1
+	double fg[2][2];
+	double frho, fdel;
 	int count;        // completed iterations
 	int done;         // stop test has fired
 	int max_iter;
@@ -97,6 +102,10 @@ struct Ctx {
 	double *XSa = nullptr, *XSb = nullptr, *RES = nullptr, *RESb = nullptr;
 	float *RES32 = nullptr, *RES32b = nullptr;
 	double* upartials = nullptr;       // per-block sums of the multigrid update
+	double* P64[2] = {};               // flexible CG search directions (ping-pong)
+	double* fpartials = nullptr;       // per-block z.r, z.r_prev, z.Lz [3 * fstride]
+	unsigned fstride = 0;
+	double* gpartials[2] = {};         // per-block r.r, p.r, p.Lp (3 columns), by iteration parity
 	SolveCtl* ctl = nullptr;           // device [NCTL]
 	SolveCtl* hctl = nullptr;          // pinned host mirror
 	double* partials = nullptr;        // [MAX_RED_BLOCKS * 8]
@@ -108,6 +117,7 @@ struct Ctx {
 	long mcap = 0;
 	int sbx = 0, sby = 0;              // spread bins per axis (4x4 base cells each)
 	int last_mg_cycles = 0;            // predicted batch size for the next solve
+	int n_cu = 256;                    // compute units of the device (persistent grids)
 	// opt-in HIP-event timing of the finest-level post-smoothing + update
 	// launch of mg_solve (the largest single launch of a solve)
 	static constexpr int KT_SLOTS = 64;

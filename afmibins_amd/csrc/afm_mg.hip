@@ -17,12 +17,14 @@
 // The stop test runs on the GPU (SolveCtl); every kernel of a cycle checks
 // ctl->done first, so the host enqueues cycles in batches and synchronises
 // once per batch.
+#include <type_traits>
 #include "afm_device.h"
 
 namespace afm {
 
 constexpr int RTX = 32, RTY = 8;         // restriction tile (coarse cells)
 constexpr int CMAXL = 6;                 // finest level handled in LDS (64x64)
+constexpr int FCG_BLOCKS = 4096;         // most blocks (and the partial-sum stride) of the flexible-CG update
 constexpr int CTHREADS = 1024;
 
 enum { M_NORMAL = 0, M_ZERO = 1, M_PADD = 2, M_PSET = 3 };
@@ -330,6 +332,47 @@ struct Strip {
 		}
 		return acc;
 	}
+
+	// Flexible CG: z = x (exact on the output ring 2) as the preconditioned
+	// residual.  Stores z on the output tile and adds this lane's z.r_k (r_k
+	// as the cycle took it: b, in registers), z.r_{k-1} (rkm1, rounded the
+	// same way) and z.(L z), products and sums in FP64 (rows, then columns).
+	__device__ void fcg_dots(T* __restrict__ z, const T* __restrict__ rkm1, double* bx, int nx, int ny, double off,
+	                         double dg, double (&acc)[3]) const
+	{
+		double zz[RPW][2];
+#pragma unroll
+		for (int i = 0; i < RPW; ++i) { zz[i][0] = (double) xr[i][0]; zz[i][1] = (double) xr[i][1]; }
+		bnd_publish<double, RPW>(bx, zz, w, lane);
+		__syncthreads();
+		double up[2], dn[2];
+#pragma unroll
+		for (int o = 0; o < 2; ++o) { up[o] = bnd_at(bx, w, 1, lane, o); dn[o] = bnd_at(bx, w + 2, 0, lane, o); }
+		const bool oc = out_col(nx);
+#pragma unroll
+		for (int i = 0; i < RPW; ++i) {
+			double lz[2];
+#pragma unroll
+			for (int o = 0; o < 2; ++o) {
+				double Ea, We, Nv, Sv;
+				strip_nbrs<double, RPW>(zz, i, o, up[o], dn[o], Ea, We, Nv, Sv);
+				lz[o] = stencil<true, double>(i, o, off, dg, zz[i][o], Ea, We, Nv, Sv, nx, ny);
+			}
+			if (oc && out_row(i, ny)) {
+				const size_t at = (size_t) nx * (gyb + i) + gx;
+				T2 zo;
+				zo.x = xr[i][0]; zo.y = xr[i][1];
+				*(T2*) (z + at) = zo;
+				const T2 q = *(const T2*) (rkm1 + at);
+				acc[0] += zz[i][0] * (double) br[i][0];
+				acc[0] += zz[i][1] * (double) br[i][1];
+				acc[1] += zz[i][0] * (double) q.x;
+				acc[1] += zz[i][1] * (double) q.y;
+				acc[2] += zz[i][0] * lz[0];
+				acc[2] += zz[i][1] * lz[1];
+			}
+		}
+	}
 };
 
 // Waves per block and rows per wave.  SH = 0 for the large levels (tall
@@ -397,24 +440,53 @@ struct MGFold {
 	unsigned nblk;
 	int it;                   // cycles completed once the tested one is counted
 	double N;                 // cells of the finest level
+	int fcg;                  // flexible CG: also p.r and p.Lp (columns 1, 2, stride FCG_BLOCKS)
 };
 
 // red_total over a 256-thread block, computed by the first four waves of a
-// larger one (same order, so the same bits as k_update_fin); thread 0
+// larger one (same order, so the same bits as the one-block finalisers);
+// NV columns of partials, `stride` apart; totals in thread 0
+template<int NV>
+__device__ inline void red_totals_256(const double* __restrict__ partials, size_t stride, unsigned nblk,
+                                      double (&t)[NV])
+{
+	__shared__ double wsum[NV][4];
+	const int tid = (int) threadIdx.x;
+#pragma unroll
+	for (int j = 0; j < NV; ++j) {
+		double s = 0.0;
+		if (tid < 256)
+			for (unsigned b = tid; b < nblk; b += 256) s += partials[j * stride + b];
+		for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+		if (tid < 256 && (tid & 63) == 0) wsum[j][tid >> 6] = s;
+	}
+	__syncthreads();
+#pragma unroll
+	for (int j = 0; j < NV; ++j) {
+		t[j] = 0.0;
+		if (tid == 0)
+			for (int w = 0; w < 4; ++w) t[j] += wsum[j][w];
+	}
+}
+
 __device__ inline double red_total_256(const double* __restrict__ partials, unsigned nblk)
 {
-	__shared__ double wsum[4];
-	const int tid = (int) threadIdx.x;
-	double s = 0.0;
-	if (tid < 256)
-		for (unsigned b = tid; b < nblk; b += 256) s += partials[b];
-	for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-	if (tid < 256 && (tid & 63) == 0) wsum[tid >> 6] = s;
-	__syncthreads();
-	double t = 0.0;
-	if (tid == 0)
-		for (int w = 0; w < 4; ++w) t += wsum[w];
-	return t;
+	double t[1];
+	red_totals_256<1>(partials, 0, nblk, t);
+	return t[0];
+}
+
+// The stop test after a flexible-CG update (r.r in column 0) and the two
+// carried scalars p_{k}.r_{k+1}, p_k.L p_k for the next iteration; thread 0
+// writes.  Same sums, same order, in the fold and in k_fcg_fin.
+__device__ inline void fcg_stop_write(SolveCtl* c, const double (&t)[3], int it, double N)
+{
+	const double rms = sqrt(t[0]) / sqrt(N);
+	c->count = it;
+	c->res = rms;
+	c->frho = t[1];
+	c->fdel = t[2];
+	c->done = !(rms > c->tol && it < c->max_iter);
 }
 
 __device__ inline void mg_fold_test(const MGFold& f)
@@ -423,6 +495,12 @@ __device__ inline void mg_fold_test(const MGFold& f)
 	if (threadIdx.x == 0) already = f.ctl->done;
 	__syncthreads();
 	if (already) return;
+	if (f.fcg) {
+		double t[3];
+		red_totals_256<3>(f.partials, FCG_BLOCKS, f.nblk, t);
+		if (threadIdx.x == 0) fcg_stop_write(f.ctl, t, f.it, f.N);
+		return;
+	}
 	const double tot = red_total_256(f.partials, f.nblk);
 	if (threadIdx.x == 0) {
 		SolveCtl* c = f.ctl;
@@ -481,6 +559,328 @@ __global__ __launch_bounds__(64 * SmoothShape<S>::NW) void k_smooth_update(
 		acc[0] = st.template update<TR>(xs, rhs, xs_out, res, bnd64, nx, ny, off64, dg64, stop == 0);
 	}
 	reduce_partial<1>(acc, partials);
+}
+
+// ---------------------------------------------------------- flexible CG --
+// The mixed form's outer iteration: CG in FP64 on L x = b (mean removed),
+// preconditioned by one V(nu1, nu2) cycle in FP32 (z = V(r)).  The V-cycle
+// is not a symmetric operator (red-then-black in both smoothing legs) and
+// its FP32 rounding varies with r, so the search direction uses the
+// Polak-Ribiere / flexible coefficient (Golub and Ye, "Inexact
+// preconditioned conjugate gradient method with inner-outer iteration",
+// SIAM J. Sci. Comput. 21, 1999):
+//   z_k = V(r_k);  beta_k = z_k.(r_k - r_{k-1}) / z_{k-1}.r_{k-1};
+//   p_k = z_k + beta_k p_{k-1};  alpha_k = p_k.r_k / p_k.L p_k;
+//   x_{k+1} = x_k + alpha_k p_k;  r_{k+1} = b - L x_{k+1}  (the true
+//   residual, so the stop test is the reference's RMS(b - L x) <= tol).
+// Two launches per iteration besides the V-cycle: its last post-smoothing
+// (k_smooth_fcg, or k_fcg_dots under a coarse-kernel cycle) stores z and
+// sums z.r_k, z.r_{k-1}, z.L z; k_fcg_update forms beta and alpha from
+// those sums and the carried scalars (p.L p expanded through L p_k = L z_k
+// + beta q_{k-1}, q_{k-1} = (r_{k-1} - r_k) / alpha_{k-1}, L symmetric), then
+// p, x and r in one pass.  The residual is formed in FP64 (its sum of
+// squares, the stop test, p.r) and stored rounded to the cycle's type: the
+// cycle takes it so, and beta's and alpha's z.r terms use that same rounded
+// vector (products and sums in FP64).  A rounding there changes only the
+// step coefficients, by about 1e-7 relative; the search directions, the
+// solution and the tested residual stay FP64.  All sums are fixed-order.
+
+// block sums with an explicit column stride (the dots launch can have more
+// blocks than MAX_RED_BLOCKS)
+template<int NV>
+__device__ inline void reduce_partial_s(double (&v)[NV], double* partials, size_t stride)
+{
+	block_sum<NV>(v);
+	if (threadIdx.x == 0) {
+#pragma unroll
+		for (int j = 0; j < NV; ++j) partials[j * stride + blockIdx.x] = v[j];
+	}
+}
+
+// Finest-level post-smoothing of the preconditioner cycle: z = smooth(e_pre
+// + P ec) against b (the cycle's right-hand side, r_k rounded), stored, with
+// the block's z.r_k, z.r_{k-1} and z.L z (at k = 0 rkm1 is b, and the
+// column unused).
+template<typename TE, int S>
+__global__ __launch_bounds__(64 * SmoothShape<S>::NW) void k_smooth_fcg(
+	const TE* __restrict__ epre, const TE* __restrict__ b, const TE* __restrict__ ec, TE* __restrict__ z,
+	const TE* __restrict__ rkm1, int nx, int ny, TE off, TE dg, TE inv, double off64, double dg64,
+	double* __restrict__ partials, unsigned stride)
+{
+	constexpr int NW = SmoothShape<S>::NW, RPW = SmoothShape<S>::RPW, H = smooth_halo<S, EPI_UPDATE>();
+	__shared__ TE bnd[bnd_size<NW>()];
+	__shared__ double bnd64[bnd_size<NW>()];
+	int X0, Y0;
+	double acc[3] = {0.0, 0.0, 0.0};
+	if (smooth_tile_origin<S, EPI_UPDATE>(nx, ny, X0, Y0)) {
+		Strip<TE, S, H, NW, RPW, true> st(X0, Y0, nx);
+		st.template load<M_PADD>(epre, b, ec, nx, ny);
+		st.sweeps(bnd, nx, ny, off, dg, inv);
+		st.fcg_dots(z, rkm1, bnd64, nx, ny, off64, dg64, acc);
+	} else {
+		Strip<TE, S, H, NW, RPW, false> st(X0, Y0, nx);
+		st.template load<M_PADD>(epre, b, ec, nx, ny);
+		st.sweeps(bnd, nx, ny, off, dg, inv);
+		st.fcg_dots(z, rkm1, bnd64, nx, ny, off64, dg64, acc);
+	}
+	reduce_partial_s<3>(acc, partials, stride);
+}
+
+// the same sums for a z that is already stored (grids of the coarse kernel)
+template<typename TE>
+__global__ __launch_bounds__(256) void k_fcg_dots(const TE* __restrict__ z, const TE* __restrict__ rk,
+                                                  const TE* __restrict__ rkm1, int nx, int ny, double off,
+                                                  double dg, double* __restrict__ partials)
+{
+	const size_t N = (size_t) nx * ny;
+	double acc[3] = {0.0, 0.0, 0.0};
+	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
+		int gx, gy;
+		cell_xy(i, nx, gx, gy);
+		const size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;
+		const size_t in_ = gy == ny - 1 ? (size_t) gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;
+		const double zc = (double) z[i];
+		const double lz = stencil5<double, true>(off, dg, zc, (double) z[ie], (double) z[iw], (double) z[in_],
+		                                         (double) z[is], gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
+		acc[0] += zc * (double) rk[i];
+		acc[1] += zc * (double) rkm1[i];
+		acc[2] += zc * lz;
+	}
+	reduce_partial<3>(acc, partials);
+}
+
+template<typename TE>
+struct FcgArgs {
+	const TE* z;              // z_k
+	const double* p_old;      // p_{k-1}; nullptr at k == 0
+	double* p_new;            // p_k
+	const double* x_old;      // x_k; nullptr at k == 0 (x_0 = 0)
+	double* x_new;            // x_{k+1}
+	const double* b;          // right-hand side, mean removed
+	TE* r;                    // r_{k+1} rounded to TE (the next cycle's right-hand side)
+	SolveCtl* ctl;
+	const double* fpart;      // the dots launch's sums, 3 columns `fstride` apart
+	unsigned fstride, fnblk;
+	double* gpart;            // this launch's r.r, p.r, p.L p (stride FCG_BLOCKS)
+	int k;
+};
+
+// beta_k and alpha_k from the dots launch's partials (every block sums them
+// in the same order, so all get the same bits) and the carried scalars;
+// block 0 records (z.r_k, alpha_k) in the parity slot k & 1, which no block
+// of this launch reads.  Returns the stop flag as the launch found it.
+template<typename TE, bool FIRST>
+__device__ inline bool fcg_coefficients(const FcgArgs<TE>& a, double& beta, double& alpha)
+{
+	__shared__ double coef[2];
+	__shared__ int stop;
+	SolveCtl* c = a.ctl;
+	const int tid = (int) threadIdx.x, nt = (int) blockDim.x;
+	double gam = 0.0, alp = 1.0, del = 0.0, rho = 0.0;
+	if (tid == 0) {
+		stop = c->done;
+		if (!FIRST) {
+			gam = c->fg[(a.k - 1) & 1][0];
+			alp = c->fg[(a.k - 1) & 1][1];
+			del = c->fdel;
+			rho = c->frho;
+		}
+	}
+	double s[3] = {0.0, 0.0, 0.0};
+	for (unsigned q0 = 0; q0 < a.fnblk; q0 += 2 * nt) {
+		// two partials per thread and column in flight
+		double v[3][2];
+#pragma unroll
+		for (int u = 0; u < 2; ++u) {
+			const unsigned q = q0 + u * nt + tid;
+#pragma unroll
+			for (int j = 0; j < 3; ++j) v[j][u] = q < a.fnblk ? a.fpart[(size_t) j * a.fstride + q] : 0.0;
+		}
+#pragma unroll
+		for (int j = 0; j < 3; ++j) { s[j] += v[j][0]; s[j] += v[j][1]; }
+	}
+	block_sum<3>(s);
+	if (tid == 0) {
+		double b = 0.0, pLp = s[2], pr = s[0];
+		if (!FIRST) {
+			b = (s[0] - s[1]) / gam;
+			const double zq = (s[1] - s[0]) / alp;
+			pLp = s[2] + 2.0 * b * zq + b * b * del;
+			pr = s[0] + b * rho;
+		}
+		const double al = pr / pLp;
+		coef[0] = b;
+		coef[1] = al;
+		if (blockIdx.x == 0 && !stop) { c->fg[a.k & 1][0] = s[0]; c->fg[a.k & 1][1] = al; }
+	}
+	__syncthreads();
+	beta = coef[0];
+	alpha = coef[1];
+	return stop != 0;
+}
+
+// The update over tiles of 128 x 64 cells in registers, like the smoother:
+// lane k of wave w holds columns 2k, 2k+1 of rows 4w .. 4w+3 of a region
+// whose outer ring of 2 cells is halo; p_k and x_{k+1} are formed on the
+// whole region, E/W neighbours by DPP, N/S through LDS boundary rows, and
+// r_{k+1} = b - L x_{k+1} on the inner 124 x 60 cells.  Every cell of the
+// grid is written by one tile; a halo cell's p and x are formed from the
+// same inputs by the same operations, so its value equals the stored one.
+// One block per CU walks tiles blockIdx.x, + gridDim.x, ... (one prologue
+// per block; the partial sums per block follow that fixed order).
+constexpr int FU_NW = 16, FU_RPW = 4, FU_H = 2, FU_LY = FU_NW * FU_RPW;
+constexpr int FU_TXO = 128 - 2 * FU_H, FU_TYO = FU_LY - 2 * FU_H;
+
+static inline unsigned fcg_update_blocks(int nx, int ny)
+{
+	return (unsigned) (((nx + FU_TXO - 1) / FU_TXO) * ((ny + FU_TYO - 1) / FU_TYO));
+}
+
+template<typename TE, bool FIRST>
+__global__ __launch_bounds__(64 * FU_NW) void k_fcg_update(FcgArgs<TE> a, int nx, int ny, double off, double dg)
+{
+	using TE2 = typename Pair<TE>::type;
+	__shared__ double bp[bnd_size<FU_NW>()], bx[bnd_size<FU_NW>()];
+	double beta, alpha;
+	if (fcg_coefficients<TE, FIRST>(a, beta, alpha)) return;
+	const int lane = (int) threadIdx.x & 63, w = (int) threadIdx.x >> 6;
+	const int ntx = (nx + FU_TXO - 1) / FU_TXO;
+	const unsigned ntiles = (unsigned) (ntx * ((ny + FU_TYO - 1) / FU_TYO));
+	double acc[3] = {0.0, 0.0, 0.0};
+	for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
+		const unsigned tile = xcd_tile(t, ntiles);
+		const int X0 = (int) (tile % ntx) * FU_TXO, Y0 = (int) (tile / ntx) * FU_TYO;
+		const int gxu = X0 - FU_H + 2 * lane, gx = wrap_once(gxu, nx);
+		const int gyb = Y0 - FU_H + FU_RPW * w;
+		double pr[FU_RPW][2], xr[FU_RPW][2];
+		{
+			TE2 zv[FU_RPW];
+			double2 pv[FU_RPW], xv[FU_RPW];
+#pragma unroll
+			for (int i = 0; i < FU_RPW; ++i) {
+				const size_t at = (size_t) nx * wrap_once(gyb + i, ny) + gx;
+				zv[i] = *(const TE2*) (a.z + at);
+				if (!FIRST) {
+					pv[i] = *(const double2*) (a.p_old + at);
+					xv[i] = *(const double2*) (a.x_old + at);
+				}
+			}
+#pragma unroll
+			for (int i = 0; i < FU_RPW; ++i) {
+				const double z0 = (double) zv[i].x, z1 = (double) zv[i].y;
+				pr[i][0] = FIRST ? z0 : z0 + beta * pv[i].x;
+				pr[i][1] = FIRST ? z1 : z1 + beta * pv[i].y;
+				xr[i][0] = FIRST ? alpha * pr[i][0] : xv[i].x + alpha * pr[i][0];
+				xr[i][1] = FIRST ? alpha * pr[i][1] : xv[i].y + alpha * pr[i][1];
+			}
+		}
+		bnd_publish<double, FU_RPW>(bp, pr, w, lane);
+		bnd_publish<double, FU_RPW>(bx, xr, w, lane);
+		__syncthreads();
+		double pu[2], pd[2], xu[2], xd[2];
+#pragma unroll
+		for (int o = 0; o < 2; ++o) {
+			pu[o] = bnd_at(bp, w, 1, lane, o); pd[o] = bnd_at(bp, w + 2, 0, lane, o);
+			xu[o] = bnd_at(bx, w, 1, lane, o); xd[o] = bnd_at(bx, w + 2, 0, lane, o);
+		}
+		const bool oc = lane >= FU_H / 2 && lane < 64 - FU_H / 2 && gxu < nx;
+#pragma unroll
+		for (int i = 0; i < FU_RPW; ++i) {
+			// the lane shifts run with every lane active (a DPP read of a lane
+			// masked off returns the reader's own value), the output test after
+			double Lx[2], Lp[2];
+#pragma unroll
+			for (int o = 0; o < 2; ++o) {
+				double E, W, Nn, S;
+				const int px = wrap_once(gx + o, nx), py = wrap_once(gyb + i, ny);
+				const bool we = px == nx - 1, ww = px == 0, wn = py == ny - 1, ws = py == 0;
+				strip_nbrs<double, FU_RPW>(xr, i, o, xu[o], xd[o], E, W, Nn, S);
+				Lx[o] = stencil5<double, true>(off, dg, xr[i][o], E, W, Nn, S, we, ww, wn, ws);
+				strip_nbrs<double, FU_RPW>(pr, i, o, pu[o], pd[o], E, W, Nn, S);
+				Lp[o] = stencil5<double, true>(off, dg, pr[i][o], E, W, Nn, S, we, ww, wn, ws);
+			}
+			const int ly = FU_RPW * w + i;
+			if (!(oc && ly >= FU_H && ly < FU_LY - FU_H && gyb + i < ny)) continue;
+			const size_t at = (size_t) nx * (gyb + i) + gx;
+			const double2 bv = *(const double2*) (a.b + at);
+			double rv[2];
+#pragma unroll
+			for (int o = 0; o < 2; ++o) {
+				rv[o] = (o ? bv.y : bv.x) - Lx[o];
+				acc[0] += rv[o] * rv[o];
+				acc[1] += pr[i][o] * rv[o];
+				acc[2] += pr[i][o] * Lp[o];
+			}
+			double2 po, xo;
+			po.x = pr[i][0]; po.y = pr[i][1];
+			xo.x = xr[i][0]; xo.y = xr[i][1];
+			*(double2*) (a.p_new + at) = po;
+			*(double2*) (a.x_new + at) = xo;
+			TE2 ro;
+			ro.x = (TE) rv[0]; ro.y = (TE) rv[1];
+			*(TE2*) (a.r + at) = ro;
+		}
+		__syncthreads();     // the next tile rewrites the boundary rows
+	}
+	reduce_partial_s<3>(acc, a.gpart, FCG_BLOCKS);
+}
+
+// The same update, one cell per thread (grids below the tile size).
+template<typename TE, bool FIRST>
+__global__ __launch_bounds__(256) void k_fcg_update_1d(FcgArgs<TE> a, int nx, int ny, double off, double dg)
+{
+	double beta, alpha;
+	if (fcg_coefficients<TE, FIRST>(a, beta, alpha)) return;
+	const size_t N = (size_t) nx * ny;
+	double acc[3] = {0.0, 0.0, 0.0};
+	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
+		int gx, gy;
+		cell_xy(i, nx, gx, gy);
+		const size_t nb[5] = {i, gx == nx - 1 ? i - (nx - 1) : i + 1, gx == 0 ? i + (nx - 1) : i - 1,
+		                      gy == ny - 1 ? (size_t) gx : i + nx, gy == 0 ? i + N - nx : i - nx};
+		double zv[5], pv[5], xv[5];
+#pragma unroll
+		for (int t = 0; t < 5; ++t) zv[t] = (double) a.z[nb[t]];
+		if (!FIRST) {
+#pragma unroll
+			for (int t = 0; t < 5; ++t) { pv[t] = a.p_old[nb[t]]; xv[t] = a.x_old[nb[t]]; }
+		}
+		const double bi = a.b[i];
+#pragma unroll
+		for (int t = 0; t < 5; ++t) {
+			const double p = FIRST ? zv[t] : zv[t] + beta * pv[t];
+			pv[t] = p;
+			xv[t] = FIRST ? alpha * p : xv[t] + alpha * p;
+		}
+		const bool we = gx == nx - 1, ww = gx == 0, wn = gy == ny - 1, ws = gy == 0;
+		const double Lx = stencil5<double, true>(off, dg, xv[0], xv[1], xv[2], xv[3], xv[4], we, ww, wn, ws);
+		const double Lp = stencil5<double, true>(off, dg, pv[0], pv[1], pv[2], pv[3], pv[4], we, ww, wn, ws);
+		const double r = bi - Lx;
+		a.p_new[i] = pv[0];
+		a.x_new[i] = xv[0];
+		a.r[i] = (TE) r;
+		acc[0] += r * r;
+		acc[1] += pv[0] * r;
+		acc[2] += pv[0] * Lp;
+	}
+	reduce_partial_s<3>(acc, a.gpart, FCG_BLOCKS);
+}
+
+// the stop test after update `it - 1` and the carried scalars, one block
+// (the last iteration of a batch, before the host reads the control block)
+__global__ __launch_bounds__(256) void k_fcg_fin(int it, unsigned nblk, double N, SolveCtl* __restrict__ ctl,
+                                                 const double* __restrict__ gpart)
+{
+	if (ctl->done) return;
+	double t[3];
+	red_totals_256<3>(gpart, FCG_BLOCKS, nblk, t);
+	if (threadIdx.x == 0) fcg_stop_write(ctl, t, it, N);
+}
+
+__global__ void k_zero(double* __restrict__ x, size_t N)
+{
+	const size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+	if (i < N) x[i] = 0.0;
 }
 
 // ------------------------------------------------------ residual+restrict --
@@ -1156,6 +1556,13 @@ struct UpdateArgs {
 	double* partials;       // one r^2 sum per block
 	int slot;               // timing event pair, or -1
 	const MGFold* fold;     // the previous cycle's stop test, or nullptr
+	// flexible CG: the last post-smoothing stores z and sums z.r_k, z.r_{k-1},
+	// z.L z instead of the update (xs, rhs, xs_out, res_out unused)
+	int fcg;
+	const T* rkm1;
+	T* z;
+	double* fpart;
+	unsigned fstride;
 };
 
 // recurse(l, B_l) -> X_l (final in Xb); rhs of level l given explicitly.
@@ -1190,12 +1597,24 @@ static int vcycle(Ctx* c, unsigned l, const T* rhs_l, int nu1, int nu2, const in
 			                   dim3(64 * SmoothShape<S_>::NW), 0, c->stream, lvX<T>(c->lv[k], 0), bk,          \
 			                   lvX<T>(c->lv[k - 1], 1), up->xs, up->rhs, up->xs_out, up->res_out, nx, ny, off, dg, \
 			                   inv, n2, -4.0 * n2, up->ctl, up->partials)
+#define AFM_SF(S_)                                                                                              \
+			hipLaunchKernelGGL((k_smooth_fcg<T, S_>), dim3(smooth_blocks<S_, EPI_UPDATE>(nx, ny)),                \
+			                   dim3(64 * SmoothShape<S_>::NW), 0, c->stream, lvX<T>(c->lv[k], 0), bk,              \
+			                   lvX<T>(c->lv[k - 1], 1), up->z, up->rkm1, nx, ny, off, dg, inv, n2, -4.0 * n2,       \
+			                   up->fpart, up->fstride)
 			const int slot = up->slot;
 			if (slot >= 0) AFM_HIP(hipEventRecord(c->kt_ev[2 * slot], c->stream));
-			if (nu2 == 1) AFM_SU(1);
-			else AFM_SU(2);
+			if (up->fcg) {
+				if (nu2 == 1) AFM_SF(1);
+				else AFM_SF(2);
+			} else if (nu2 == 1) {
+				AFM_SU(1);
+			} else {
+				AFM_SU(2);
+			}
 			if (slot >= 0) AFM_HIP(hipEventRecord(c->kt_ev[2 * slot + 1], c->stream));
 #undef AFM_SU
+#undef AFM_SF
 			AFM_LAUNCH_CHECK();
 		} else {
 			AFM_TRY(smooth_n<T>(c, k, M_PADD, nu2, lvX<T>(c->lv[k], 0), bk, lvX<T>(c->lv[k - 1], 1),
@@ -1263,6 +1682,9 @@ static int nested32(Ctx* c, const double* rhs, int nu1, int nu2)
 
 int mg_setup(Ctx* c)
 {
+	int cus = 0;
+	AFM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+	c->n_cu = std::max(1, std::min(cus, FCG_BLOCKS));
 	unsigned T = 0;
 	while ((1u << T) < c->n) ++T;
 	c->levels = T;
@@ -1295,6 +1717,14 @@ int mg_setup(Ctx* c)
 		nb = std::max<size_t>(nb, update_blocks((int) c->nx, (int) c->ny, 1));
 		nb = std::max<size_t>(nb, update_blocks((int) c->nx, (int) c->ny, 2));
 		AFM_TRY(c->alloc((void**) &c->upartials, nb * 8));
+		// flexible CG: the dots launch's three columns, its search directions
+		// and the update launch's sums (alternating by iteration)
+		c->fstride = (unsigned) nb;
+		AFM_TRY(c->alloc((void**) &c->fpartials, 3 * nb * 8));
+		for (int j = 0; j < 2; ++j) {
+			AFM_TRY(c->alloc((void**) &c->gpartials[j], 3 * (size_t) FCG_BLOCKS * 8));
+			AFM_TRY(c->alloc((void**) &c->P64[j], c->N * 8));
+		}
 	}
 	return 0;
 }
@@ -1306,11 +1736,146 @@ static int read_ctl(Ctx* c, int i)
 	return 0;
 }
 
+// The solution and the solve's end, shared by both outer iterations: x is
+// where the last update left it (the caller takes it there with sol).
+static int mg_finish(Ctx* c, double* rhs, const double* x, int* cycles, double* final_rms, const double** sol)
+{
+	const int cnt = c->hctl[0].count;
+	if (sol) *sol = x;
+	else AFM_HIP(hipMemcpyAsync(rhs, x, c->N * 8, hipMemcpyDeviceToDevice, c->stream));
+	c->last_mg_cycles = cnt;
+	if (cycles) *cycles = cnt;
+	if (final_rms) *final_rms = c->hctl[0].res;
+	// the stop flag also fires at the cycle limit: convergence is the test
+	if (!(c->hctl[0].res <= c->hctl[0].tol)) { set_error("multigrid did not converge within max_cycles"); return -4; }
+	return 0;
+}
+
+// Flexible CG preconditioned by one V-cycle in T per iteration (see
+// k_fcg_update), from x_0 = 0; the stop test and tolerance as mg_solve's.
+// k_mean has run (ctl started, mean in ctl).
+template<typename T>
+static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, int max_cycles, int* cycles,
+                     double* final_rms, const double** sol)
+{
+	constexpr bool F32 = std::is_same<T, float>::value;
+	SolveCtl* dctl = &c->ctl[0];
+	const size_t N = c->N;
+	const unsigned rb = red_blocks(N), Lt = c->levels;
+	const int nx = (int) c->nx, ny = (int) c->ny;
+	// the update: register tiles where the grid has at least one (and the
+	// tiles' partial sums fit), else one cell per thread
+	const bool tiled = nx >= 128 && ny >= FU_LY;
+	const size_t gb0 = (N + 255) / 256;
+	const unsigned gb = tiled ? std::min(fcg_update_blocks(nx, ny), (unsigned) c->n_cu)
+	                          : (unsigned) (gb0 < (size_t) FCG_BLOCKS ? gb0 : (size_t) FCG_BLOCKS);
+	const double off = (double) (c->n * c->n), dg = -4.0 * off;
+	double* xs[2] = {c->XSa, c->XSb};
+	// r_k, rounded to the cycle's type, alternating: the cycle of iteration
+	// k reads r[k & 1], the update writes r[(k + 1) & 1]
+	T* r[2];
+	if (F32) { r[0] = (T*) c->RES32; r[1] = (T*) c->RES32b; }
+	else { r[0] = (T*) c->RES; r[1] = (T*) c->RESb; }
+	// r_0 = rhs - mean, written back to rhs (mg/solver.h:224-225) and, rounded,
+	// as the first cycle's right-hand side; the relative tolerance
+	hipLaunchKernelGGL((k_update_resid<T, T>), dim3(rb), dim3(256), 0, c->stream, (const double*) nullptr,
+	                   (const T*) nullptr, rhs, (double*) nullptr, r[0], nx, ny, off, dg, 1, 1, dctl, c->partials);
+	hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, tol_mode == 1 ? 2 : 1, rb, N, dctl, c->partials,
+	                   tol);
+	AFM_LAUNCH_CHECK();
+	const int* done = &dctl->done;
+	const bool fused = Lt > c->lc;     // else every cycle is one coarse-kernel launch
+	const unsigned fnblk = fused ? update_blocks(nx, ny, nu2) : rb;
+	const unsigned fstride = fused ? c->fstride : (unsigned) MAX_RED_BLOCKS;
+	T* z = lvX<T>(c->lv[Lt], 1);
+	int enq = 0;
+	int batch = c->last_mg_cycles > 0 ? c->last_mg_cycles : 2;
+	for (;;) {
+		if (batch > max_cycles - enq) batch = max_cycles - enq;
+		const int enq0 = enq;
+		for (int s = 0; s < batch; ++s) {
+			const int k = enq;
+			const T* rk = r[k & 1];
+			const T* rkm1 = k ? r[(k + 1) & 1] : rk;      // r_{k-1} (at k = 0 unused)
+			if (fused) {
+				// the stop test of iteration k-1 rides on this cycle's first
+				// launch (all but a batch's first iteration)
+				const MGFold fold{dctl, c->gpartials[(k + 1) & 1], gb, k, (double) N, 1};
+				UpdateArgs<T> up{};
+				up.ctl = dctl;
+				up.slot = -1;
+				up.fold = s > 0 ? &fold : nullptr;
+				up.fcg = 1;
+				up.rkm1 = rkm1;
+				up.z = z;
+				up.fpart = c->fpartials;
+				up.fstride = fstride;
+				AFM_TRY(vcycle<T>(c, Lt, rk, nu1, nu2, done, &up));
+			} else {
+				AFM_TRY(vcycle<T>(c, Lt, rk, nu1, nu2, done));
+				hipLaunchKernelGGL((k_fcg_dots<T>), dim3(rb), dim3(256), 0, c->stream, z, rk, rkm1, nx, ny, off, dg,
+				                   c->fpartials);
+			}
+			const FcgArgs<T> a{z, k ? c->P64[(k + 1) & 1] : nullptr, c->P64[k & 1], k ? xs[k & 1] : nullptr,
+			                   xs[(k + 1) & 1], rhs, r[(k + 1) & 1], dctl, c->fpartials, fstride, fnblk,
+			                   c->gpartials[k & 1], k};
+			const int slot = (c->kt_on && k >= 1 && s < Ctx::KT_SLOTS) ? s : -1;
+			if (slot >= 0) AFM_HIP(hipEventRecord(c->kt_ev[2 * slot], c->stream));
+			if (tiled) {
+				if (k == 0)
+					hipLaunchKernelGGL((k_fcg_update<T, true>), dim3(gb), dim3(64 * FU_NW), 0, c->stream, a, nx, ny, off, dg);
+				else
+					hipLaunchKernelGGL((k_fcg_update<T, false>), dim3(gb), dim3(64 * FU_NW), 0, c->stream, a, nx, ny, off, dg);
+			} else if (k == 0) {
+				hipLaunchKernelGGL((k_fcg_update_1d<T, true>), dim3(gb), dim3(256), 0, c->stream, a, nx, ny, off, dg);
+			} else {
+				hipLaunchKernelGGL((k_fcg_update_1d<T, false>), dim3(gb), dim3(256), 0, c->stream, a, nx, ny, off, dg);
+			}
+			if (slot >= 0) AFM_HIP(hipEventRecord(c->kt_ev[2 * slot + 1], c->stream));
+			if (!fused)
+				hipLaunchKernelGGL(k_fcg_fin, dim3(1), dim3(256), 0, c->stream, k + 1, gb, (double) N, dctl,
+				                   c->gpartials[k & 1]);
+			AFM_LAUNCH_CHECK();
+			++enq;
+		}
+		if (fused) {
+			hipLaunchKernelGGL(k_fcg_fin, dim3(1), dim3(256), 0, c->stream, enq, gb, (double) N, dctl,
+			                   c->gpartials[(enq + 1) & 1]);
+			AFM_LAUNCH_CHECK();
+		}
+		AFM_TRY(read_ctl(c, 0));
+		if (c->kt_on) {
+			// update launches that ran (later ones exit at once)
+			for (int s = 0; s < batch && s < Ctx::KT_SLOTS; ++s) {
+				const int k = enq0 + s;
+				if (k < 1 || k >= c->hctl[0].count) continue;
+				float ms = 0.f;
+				AFM_HIP(hipEventElapsedTime(&ms, c->kt_ev[2 * s], c->kt_ev[2 * s + 1]));
+				c->kt_ms += ms;
+				++c->kt_launches;
+			}
+		}
+		if (c->hctl[0].done || enq >= max_cycles) break;
+		batch = 1;
+	}
+	const int cnt = c->hctl[0].count;
+	if (cnt == 0) {
+		// r_0 already met the tolerance: x = 0
+		hipLaunchKernelGGL(k_zero, dim3((unsigned) ((N + 255) / 256)), dim3(256), 0, c->stream, xs[0], N);
+		AFM_LAUNCH_CHECK();
+	}
+	return mg_finish(c, rhs, xs[cnt & 1], cycles, final_rms, sol);
+}
+
 int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, int precision, int max_cycles,
              int* cycles, double* final_rms, const double** sol)
 {
 	if (nu1 < 1 || nu1 > 2 || nu2 < 1 || nu2 > 2) { set_error("nu1, nu2 must be 1 or 2 (one fused launch per smoothing step)"); return -1; }
-	if (precision != 0 && precision != 1) { set_error("precision must be 0 (fp64) or 1 (fp32 V-cycle in fp64 refinement)"); return -1; }
+	if (precision < 0 || precision > 3) {
+		set_error("precision must be 0 (fp64), 1 (fp32 V-cycle preconditioning fp64 CG), 2 (fp32 V-cycles in fp64 "
+		          "refinement) or 3 (fp64 V-cycle preconditioning fp64 CG)");
+		return -1;
+	}
 	SolveCtl* dctl = &c->ctl[0];
 	const size_t N = c->N;
 	const unsigned rb = red_blocks(N);
@@ -1324,8 +1889,11 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 	                   tol_mode == 1 ? 0.0 : tol, max_cycles);
 	hipLaunchKernelGGL(k_mean_fin, dim3(1), dim3(256), 0, c->stream, 0, tol, rb, N, dctl, c->partials);
 	AFM_LAUNCH_CHECK();
-	// nested start and its residual, mean removed from rhs in place
-	if (precision == 1 && T > c->lc) {
+	if (precision == 1) return fcg_solve<float>(c, rhs, tol, tol_mode, nu1, nu2, max_cycles, cycles, final_rms, sol);
+	if (precision == 3) return fcg_solve<double>(c, rhs, tol, tol_mode, nu1, nu2, max_cycles, cycles, final_rms, sol);
+	// the reference's iteration (precision 0), or the same in FP32 V-cycles
+	// (2): nested start and its residual, mean removed from rhs in place
+	if (precision == 2 && T > c->lc) {
 		AFM_TRY(nested32(c, rhs, nu1, nu2));
 		hipLaunchKernelGGL((k_update_resid<float, float>), dim3(rb), dim3(256), 0, c->stream, (const double*) nullptr,
 		                   (const float*) c->RES32b, rhs, c->XSa, c->RES32, (int) c->nx, (int) c->ny, off, dg, 1, 1,
@@ -1409,16 +1977,9 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 		if (c->hctl[0].done || enq >= max_cycles) break;
 		batch = 1;                  // beyond the prediction, one cycle at a time
 	}
-	const int cnt = c->hctl[0].count;
 	// the solution replaces rhs (mg/solver.h:218-237), unless the caller
 	// takes it where it is (valid until the context's next solve)
-	if (sol) *sol = xs[cnt & 1];
-	else AFM_HIP(hipMemcpyAsync(rhs, xs[cnt & 1], N * 8, hipMemcpyDeviceToDevice, c->stream));
-	c->last_mg_cycles = cnt;
-	if (cycles) *cycles = cnt;
-	if (final_rms) *final_rms = c->hctl[0].res;
-	if (!c->hctl[0].done) { set_error("multigrid did not converge within max_cycles"); return -4; }
-	return 0;
+	return mg_finish(c, rhs, xs[c->hctl[0].count & 1], cycles, final_rms, sol);
 }
 
 // One V-cycle recurse(T, rhs) on the residual equation, result in place.

@@ -506,8 +506,12 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 		if (x[f] && c->hctl[1 + f].count > mx) mx = c->hctl[1 + f].count;
 	}
 	c->last_cg_iters = mx;
+	// the stop flag also fires at the iteration limit: convergence is the test
 	for (int f = 0; f < 2; ++f)
-		if (x[f] && !c->hctl[1 + f].done) { set_error("CG did not converge within max_iters"); return -4; }
+		if (x[f] && !(c->hctl[1 + f].res <= c->hctl[1 + f].tol)) {
+			set_error("CG did not converge within max_iters");
+			return -4;
+		}
 	return 0;
 }
 

@@ -11,18 +11,27 @@ multigrid projection), interpolation + marker update, spread, Crank-Nicolson
 full step, interpolation + marker update.  Solves stop at 1e-10 relative
 residual (multigrid: RMS; CG: 2-norm).
 
-Two forms are timed on identical inputs: plain FP64, and FP32 V(2,2) cycles
-inside FP64 iterative refinement (`value`, the headline).  Inputs are resident
-in HBM before the timed region; the timed region is K full steps bracketed by
-device synchronisation.
+Two forms are timed on identical inputs: plain FP64 (the reference's
+multigrid iteration: nested start, then V(2,2) cycles until the residual test
+holds), and the mixed form (`value`, the headline): flexible CG in FP64 --
+true residual, search directions, inner products, solution -- preconditioned
+by one FP32 V(2,2) cycle per iteration.  `--forms` also takes mixed_ir (the
+reference's iteration with FP32 cycles, FP64 iterative refinement) and
+fp64_cg (the CG form with FP64 cycles), which split the mixed form's gain
+into its precision and Krylov parts.  Inputs are resident in HBM before the
+timed region; the timed region is K full steps bracketed by device
+synchronisation.
 
-`roofline` is the dominant launch of the headline form: the finest-level
-post-smoothing fused with the FP64 outer update (k_smooth_update), timed with
-HIP events on the stream it runs on during extra steps of the same workload.
-Algorithmic bytes per finest cell: read e_pre and b (FP32, 4 + 4), the coarse
-correction (4 per 4 cells = 1), xs and rhs (FP64, 8 + 8); write xs and r
-(8 + 4): 37 B (50 B in the FP64 form).  `cpu_baseline` is the oracle's
-restatement of the reference path on one host core over a one-step sample.
+`roofline` is the headline form's largest FP64 launch, timed with HIP events
+on the stream it runs on during extra steps of the same workload: the CG
+update k_fcg_update (iterations after the first), which reads z (FP32, 4 B),
+p, x and b (FP64, 8 + 8 + 8) and writes p, x, r (8 + 8 + 8) and r rounded
+(4): 56 B per finest cell.  The FP64 form's is the finest-level
+post-smoothing fused with the outer update (k_smooth_update): read e_pre and
+b (8 + 8), the coarse correction (8 per 4 cells = 2), xs and rhs (8 + 8);
+write xs and r (8 + 8): 50 B (37 B with FP32 cycles).  `cpu_baseline` is the
+oracle's restatement of the reference path on one host core over a one-step
+sample.
 """
 import argparse
 import ctypes as C
@@ -35,8 +44,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-UPDATE_BYTES = {"mixed": 37.0, "fp64": 50.0}      # per finest cell, see above
-PMC_FILE = os.path.join(ROOT, "profiles", "k_smooth_update_f32_2048_pmc.txt")
+# form: (precision code of afm_mg_solve, timed launch, its bytes per finest cell, counter summary)
+FORMS = {
+    "fp64": (0, "k_smooth_update<double, double, 2>", 50.0, None),
+    "mixed": (1, "k_fcg_update<float>", 56.0, "k_fcg_update_f32_2048_pmc.txt"),
+    "mixed_ir": (2, "k_smooth_update<float, float, 2>", 37.0, "k_smooth_update_f32_2048_pmc.txt"),
+    "fp64_cg": (3, "k_fcg_update<double>", 56.0, None),
+}
+DTYPE = {"fp64": "f64", "mixed": "f64 (f32 V-cycles preconditioning f64 CG)",
+         "mixed_ir": "f64 (f32 V-cycles in f64 refinement)", "fp64_cg": "f64 (f64 V-cycles preconditioning f64 CG)"}
 
 
 def parse():
@@ -68,11 +84,13 @@ def hbm_roofline(nbytes, us, kernel, extra=None):
     return r
 
 
-def pmc_traffic():
-    """HBM bytes per launch of the roofline kernel from the committed counter
-    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+def pmc_traffic(name):
+    """HBM bytes per launch of a form's roofline kernel from its committed
+    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    if not name:
+        return None
     try:
-        for line in open(PMC_FILE):
+        for line in open(os.path.join(ROOT, "profiles", name)):
             if line.startswith("hbm_bytes_per_launch:"):
                 return float(line.split(":")[1])
     except OSError:
@@ -175,7 +193,8 @@ def transfer_rates(A, W, dev, reps=10):
 
 
 def solver_only(A, W, dev, n=4096, reps=3):
-    """Poisson by multigrid (FP64 vs FP32 V(2,2) in FP64 refinement) and
+    """Poisson by multigrid (the FP64 reference iteration, the mixed CG form
+    and FP32 cycles in the reference iteration) and
     Helmholtz (I - dt mu L, dt = 2e-5, mu = 1e-3) by CG, both to 1e-10
     relative residual, on a seeded 256-mode |k|^-2 right-hand side."""
     import torch
@@ -183,7 +202,7 @@ def solver_only(A, W, dev, n=4096, reps=3):
     f = W.fourier_rhs_device(n, 256, 32, 11, dev)
     res = {"case": f"{n}x{n} periodic, 256 Fourier modes |k|<=32, |k|^-2 amplitudes, mean removed"}
     sol = {}
-    for name, prec in (("fp64", 0), ("mixed", 1)):
+    for name, prec in (("fp64", 0), ("mixed", 1), ("mixed_ir", 2)):
         x = f.clone()
         cyc, rms = ctx.mg_solve(x, 1e-10, 1, 2, 2, prec)        # warm (and the solution)
         sol[name] = x
@@ -198,6 +217,8 @@ def solver_only(A, W, dev, n=4096, reps=3):
     d = sol["mixed"] - sol["fp64"]
     d = d - d.mean()
     res["poisson_mixed_over_fp64"] = round(res["poisson_fp64"]["ms_per_solve"] / res["poisson_mixed"]["ms_per_solve"], 3)
+    res["poisson_mixed_ir_over_fp64"] = round(res["poisson_fp64"]["ms_per_solve"]
+                                              / res["poisson_mixed_ir"]["ms_per_solve"], 3)
     res["poisson_mixed_vs_fp64_rel_err"] = float(d.norm() / (sol["fp64"] - sol["fp64"].mean()).norm())
     scale = -2e-5 * 1e-3
     x = torch.zeros_like(f)
@@ -319,8 +340,7 @@ def main():
     ctx = A.Context(n, 0)
     forms = {}
     for name in args.forms.split(","):
-        prec = 0 if name == "fp64" else 1
-        forms[name] = run_form(A, ctx, prec, args, inputs, dev, timing_steps=5)
+        forms[name] = run_form(A, ctx, FORMS[name][0], args, inputs, dev, timing_steps=5)
         print(f"[bench] {name}: {forms[name]['ms_per_step']:.3f} ms/step, cycles {forms[name]['last_mg_cycles']}, "
               f"cg {forms[name]['last_cg_iters']}, update launch {forms[name]['update_us']:.2f} us",
               file=sys.stderr, flush=True)
@@ -328,13 +348,13 @@ def main():
     head = forms[head_name]
     m = inputs[2].shape[0]
     cells = float(n) * n
-    prec_txt = "float, float" if head_name == "mixed" else "double, double"
-    roof = hbm_roofline(UPDATE_BYTES[head_name] * cells, head["update_us"],
-                        f"k_smooth_update<{prec_txt}, 2> ({n}x{n}: finest post-smoothing + FP64 outer update, "
-                        f"{UPDATE_BYTES[head_name]:.0f} B/cell)",
-                        {"launches_timed": head["update_launches"]})
-    if head_name == "mixed":
-        roof["traffic"] = pmc_traffic()
+    def form_roof(name):
+        _, kern, bpc, pmc = FORMS[name]
+        r = hbm_roofline(bpc * cells, forms[name]["update_us"], f"{kern} ({n}x{n}, {bpc:.0f} B/cell)",
+                         {"launches_timed": forms[name]["update_launches"]})
+        r["traffic"] = pmc_traffic(pmc)
+        return r
+    roof = form_roof(head_name)
     line = {
         "metric": "IB steps/s @2048² grid, 16k markers: FP64 vs FP32-MG+FP64 refine at 1e-10 resid",
         "value": round(head["steps_per_s"], 3),
@@ -346,7 +366,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (f32 V-cycles in f64 refinement)" if head_name == "mixed" else "f64",
+        "dtype": DTYPE[head_name],
         "data": "synthetic (seeded membranes and Fourier velocity)",
         "config": {"workload": f"2-D periodic IB, {n}x{n} MAC grid, {m} markers (64 membranes x 256), "
                                "mu=1e-3, rho=1, dt=2e-5, V(2,2) MG + CG Helmholtz to 1e-10 relative",
@@ -360,8 +380,11 @@ def main():
         line["mixed_over_fp64"] = round(forms["mixed"]["steps_per_s"] / forms["fp64"]["steps_per_s"], 3)
         a, b = forms["mixed"]["u_final"], forms["fp64"]["u_final"]
         line["mixed_vs_fp64_u_rel_err"] = float((a - b).norm() / b.norm())
-        line["roofline_fp64_form"] = hbm_roofline(UPDATE_BYTES["fp64"] * cells, forms["fp64"]["update_us"],
-                                                  f"k_smooth_update<double, double, 2> ({n}x{n}, 50 B/cell)")
+    for name in forms:
+        if name != head_name:
+            line[f"roofline_{name}_form"] = form_roof(name)
+        if name not in ("fp64", "mixed") and "fp64" in forms:
+            line[f"{name}_over_fp64"] = round(forms[name]["steps_per_s"] / forms["fp64"]["steps_per_s"], 3)
     if not args.no_extras:
         line["smoother_2048"] = smoother_roofline(A, ctx, dev)
         big = A.Context(8192, 0)

@@ -55,9 +55,9 @@ void* afm_own_stream(afm_ctx* c);
 const char* afm_last_error(void);
 
 /* Measurement (no reference counterpart): HIP-event timing, on the context's
- * stream, of the finest-level post-smoothing + outer-update launch of
- * afm_mg_solve (the largest single launch of a solve), counting only cycles
- * that ran.  afm_update_timing(c, 1) enables and resets; 0 disables. */
+ * stream, of the outer-update launch of afm_mg_solve (precision 0 and 2: the
+ * finest-level post-smoothing fused with the update; 1 and 3: the CG update
+ * of iterations after the first), counting only launches that ran.  afm_update_timing(c, 1) enables and resets; 0 disables. */
 int afm_update_timing(afm_ctx* c, int enable);
 int afm_update_timing_read(afm_ctx* c, double* avg_us, long* launches);
 
@@ -104,9 +104,19 @@ int afm_mg_vcycle(afm_ctx* c, double* rhs_inout, int nu1, int nu2);
  * replaced by the solution (its mean is arbitrary, as in the reference).
  * tol_mode 0: stop when RMS(rhs - L x) <= tol (the reference's test);
  * tol_mode 1: when RMS(r) <= tol * RMS(rhs - mean(rhs)).
- * precision 0: FP64 throughout; 1: FP32 V-cycles inside FP64 iterative
- * refinement (FP64 residual, update, reductions and coarsest solve).
- * nu1, nu2 in {1, 2}.  *cycles = number of V-cycles (the reference's return). */
+ * precision 0: FP64 throughout, the reference's iteration (nested start,
+ * then x += V(r) until the test holds);
+ * 1: the mixed form -- flexible CG in FP64 (true residual, search
+ * directions, inner products, solution) preconditioned by one FP32 V-cycle
+ * per iteration, from x = 0; the coarsest levels of every cycle in FP64;
+ * 2: the reference's iteration with FP32 V-cycles (FP64 iterative
+ * refinement: FP64 residual, update, reductions and coarsest solve);
+ * 3: as 1 with FP64 V-cycles.
+ * Forms 1 and 3 reach the same residual test with fewer V-cycles; their
+ * solution differs from the reference's by the arbitrary constant of the
+ * periodic problem (the mean) and by at most the tolerance otherwise.
+ * nu1, nu2 in {1, 2}.  *cycles = V-cycles (the reference's return; for 1
+ * and 3 the CG iterations, one V-cycle each). */
 int afm_mg_solve(afm_ctx* c, double* rhs_inout, double tol, int tol_mode, int nu1, int nu2, int precision,
                  int max_cycles, int* cycles, double* final_rms);
 

@@ -149,7 +149,7 @@ class solver {
 	double tol;
 	unsigned nu1, nu2;
 public:
-	int precision = 0;          // 1: FP32 V-cycles inside FP64 refinement
+	int precision = 0;          // as afm_mg_solve's (1: FP32 V-cycle preconditioning FP64 CG)
 	solver(double tol_, unsigned nu1_, unsigned nu2_) : tol(tol_), nu1(nu1_), nu2(nu2_) {}
 	unsigned solve(dt::vector<G::size>& rhs)
 	{
