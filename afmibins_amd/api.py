@@ -242,6 +242,19 @@ class NSSolver:
         self._lib("afm_ns_get_p", _ptr(p))
         return p
 
+    def get_start(self, kind):
+        """The CG forms' start potential of step kind 0 (backward-Euler half
+        step) or 1 (Crank-Nicolson full step), or None while there is none."""
+        phi = torch.empty_like(self.u)
+        ok = C.c_int()
+        self._lib("afm_ns_get_start", kind, _ptr(phi), C.byref(ok))
+        return phi if ok.value else None
+
+    def set_start(self, kind, phi):
+        if phi is not None:
+            self.ctx._f(phi, (self.ctx.ny, self.ctx.nx), "phi")
+        self._lib("afm_ns_set_start", kind, _ptr(phi) if phi is not None else None)
+
     def stats(self):
         a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         self._lib("afm_ns_stats", C.byref(a), C.byref(b), C.byref(c), C.byref(d))

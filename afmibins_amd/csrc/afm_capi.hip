@@ -33,7 +33,9 @@ struct NS;
 struct IB;
 int ns_create(Ctx*, double, double, double, double, int, int, double*, double*, NS**);
 int ns_step(NS*, const double*, const double*);
-int ns_project(NS*, double*, double*);
+int ns_project(NS*, double*, double*, int);
+int ns_get_start(NS*, int, double*, int*);
+int ns_set_start(NS*, int, const double*);
 int ns_get_p(NS*, double*);
 int ib_create(NS*, long, const double*, const double*, const double*, const double*, IB**);
 int ib_step(IB*);
@@ -364,7 +366,7 @@ int afm_ns_project(afm_ns* s, double* u, double* v)
 {
 	CHECK_ARG(s && u && v, "null argument");
 	AFM_TRY(use(ns_ctx(s->s)));
-	return ns_project(s->s, u, v);
+	return ns_project(s->s, u, v, -1);
 }
 
 int afm_ns_get_p(afm_ns* s, double* p)
@@ -378,6 +380,22 @@ int afm_ns_stats(afm_ns* s, int* cyc, int* cgu, int* cgv, int* half)
 {
 	CHECK_ARG(s, "null argument");
 	return ns_stats(s->s, cyc, cgu, cgv, half);
+}
+
+int afm_ns_get_start(afm_ns* s, int kind, double* phi, int* valid)
+{
+	CHECK_ARG(s && phi && valid, "null argument");
+	CHECK_ALIGNED(phi);
+	AFM_TRY(use(ns_ctx(s->s)));
+	return ns_get_start(s->s, kind, phi, valid);
+}
+
+int afm_ns_set_start(afm_ns* s, int kind, const double* phi)
+{
+	CHECK_ARG(s, "null argument");
+	if (phi) CHECK_ALIGNED(phi);
+	AFM_TRY(use(ns_ctx(s->s)));
+	return ns_set_start(s->s, kind, phi);
 }
 
 int afm_ns_old(afm_ns* s, const double** uo, const double** vo)

@@ -147,9 +147,11 @@ int launch_ib_move_full(Ctx* c, const double* u_old, const double* v_old, const 
 // Multigrid (afm_mg.hip)
 int mg_setup(Ctx* c);
 // sol != nullptr: rhs is left as the mean-free right-hand side and *sol
-// points at the solution inside the context (valid until its next solve)
+// points at the solution inside the context (valid until its next solve).
+// x0: the CG forms' (precision 1, 3) initial guess (nullptr: zero); the
+// reference's iteration always starts from its nested iteration.
 int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, int precision, int max_cycles,
-             int* cycles, double* final_rms, const double** sol = nullptr);
+             int* cycles, double* final_rms, const double** sol = nullptr, const double* x0 = nullptr);
 int mg_apply_laplacian(Ctx* c, unsigned level, const double* x, double* y);
 int mg_smooth(Ctx* c, unsigned level, const double* x_in, const double* b, double* x_out, int sweeps, int zero_init);
 int mg_restrict(Ctx* c, unsigned level, const double* fine, double* coarse);

@@ -654,7 +654,7 @@ struct FcgArgs {
 	const TE* z;              // z_k
 	const double* p_old;      // p_{k-1}; nullptr at k == 0
 	double* p_new;            // p_k
-	const double* x_old;      // x_k; nullptr at k == 0 (x_0 = 0)
+	const double* x_old;      // x_k; nullptr when x_0 = 0 (k == 0 without a start)
 	double* x_new;            // x_{k+1}
 	const double* b;          // right-hand side, mean removed
 	TE* r;                    // r_{k+1} rounded to TE (the next cycle's right-hand side)
@@ -728,7 +728,7 @@ __device__ inline bool fcg_coefficients(const FcgArgs<TE>& a, double& beta, doub
 // same inputs by the same operations, so its value equals the stored one.
 // One block per CU walks tiles blockIdx.x, + gridDim.x, ... (one prologue
 // per block; the partial sums per block follow that fixed order).
-constexpr int FU_NW = 16, FU_RPW = 4, FU_H = 2, FU_LY = FU_NW * FU_RPW;
+constexpr int FU_NW = 16, FU_RPW = 2, FU_H = 2, FU_LY = FU_NW * FU_RPW;
 constexpr int FU_TXO = 128 - 2 * FU_H, FU_TYO = FU_LY - 2 * FU_H;
 
 static inline unsigned fcg_update_blocks(int nx, int ny)
@@ -736,7 +736,7 @@ static inline unsigned fcg_update_blocks(int nx, int ny)
 	return (unsigned) (((nx + FU_TXO - 1) / FU_TXO) * ((ny + FU_TYO - 1) / FU_TYO));
 }
 
-template<typename TE, bool FIRST>
+template<typename TE, bool FIRST, bool XZ>
 __global__ __launch_bounds__(64 * FU_NW) void k_fcg_update(FcgArgs<TE> a, int nx, int ny, double off, double dg)
 {
 	using TE2 = typename Pair<TE>::type;
@@ -760,18 +760,16 @@ __global__ __launch_bounds__(64 * FU_NW) void k_fcg_update(FcgArgs<TE> a, int nx
 			for (int i = 0; i < FU_RPW; ++i) {
 				const size_t at = (size_t) nx * wrap_once(gyb + i, ny) + gx;
 				zv[i] = *(const TE2*) (a.z + at);
-				if (!FIRST) {
-					pv[i] = *(const double2*) (a.p_old + at);
-					xv[i] = *(const double2*) (a.x_old + at);
-				}
+				if (!FIRST) pv[i] = *(const double2*) (a.p_old + at);
+				if (!XZ) xv[i] = *(const double2*) (a.x_old + at);
 			}
 #pragma unroll
 			for (int i = 0; i < FU_RPW; ++i) {
 				const double z0 = (double) zv[i].x, z1 = (double) zv[i].y;
 				pr[i][0] = FIRST ? z0 : z0 + beta * pv[i].x;
 				pr[i][1] = FIRST ? z1 : z1 + beta * pv[i].y;
-				xr[i][0] = FIRST ? alpha * pr[i][0] : xv[i].x + alpha * pr[i][0];
-				xr[i][1] = FIRST ? alpha * pr[i][1] : xv[i].y + alpha * pr[i][1];
+				xr[i][0] = XZ ? alpha * pr[i][0] : xv[i].x + alpha * pr[i][0];
+				xr[i][1] = XZ ? alpha * pr[i][1] : xv[i].y + alpha * pr[i][1];
 			}
 		}
 		bnd_publish<double, FU_RPW>(bp, pr, w, lane);
@@ -826,7 +824,7 @@ __global__ __launch_bounds__(64 * FU_NW) void k_fcg_update(FcgArgs<TE> a, int nx
 }
 
 // The same update, one cell per thread (grids below the tile size).
-template<typename TE, bool FIRST>
+template<typename TE, bool FIRST, bool XZ>
 __global__ __launch_bounds__(256) void k_fcg_update_1d(FcgArgs<TE> a, int nx, int ny, double off, double dg)
 {
 	double beta, alpha;
@@ -843,14 +841,18 @@ __global__ __launch_bounds__(256) void k_fcg_update_1d(FcgArgs<TE> a, int nx, in
 		for (int t = 0; t < 5; ++t) zv[t] = (double) a.z[nb[t]];
 		if (!FIRST) {
 #pragma unroll
-			for (int t = 0; t < 5; ++t) { pv[t] = a.p_old[nb[t]]; xv[t] = a.x_old[nb[t]]; }
+			for (int t = 0; t < 5; ++t) pv[t] = a.p_old[nb[t]];
+		}
+		if (!XZ) {
+#pragma unroll
+			for (int t = 0; t < 5; ++t) xv[t] = a.x_old[nb[t]];
 		}
 		const double bi = a.b[i];
 #pragma unroll
 		for (int t = 0; t < 5; ++t) {
 			const double p = FIRST ? zv[t] : zv[t] + beta * pv[t];
 			pv[t] = p;
-			xv[t] = FIRST ? alpha * p : xv[t] + alpha * p;
+			xv[t] = XZ ? alpha * p : xv[t] + alpha * p;
 		}
 		const bool we = gx == nx - 1, ww = gx == 0, wn = gy == ny - 1, ws = gy == 0;
 		const double Lx = stencil5<double, true>(off, dg, xv[0], xv[1], xv[2], xv[3], xv[4], we, ww, wn, ws);
@@ -1756,7 +1758,7 @@ static int mg_finish(Ctx* c, double* rhs, const double* x, int* cycles, double* 
 // k_mean has run (ctl started, mean in ctl).
 template<typename T>
 static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, int max_cycles, int* cycles,
-                     double* final_rms, const double** sol)
+                     double* final_rms, const double** sol, const double* x0)
 {
 	constexpr bool F32 = std::is_same<T, float>::value;
 	SolveCtl* dctl = &c->ctl[0];
@@ -1776,10 +1778,11 @@ static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 	T* r[2];
 	if (F32) { r[0] = (T*) c->RES32; r[1] = (T*) c->RES32b; }
 	else { r[0] = (T*) c->RES; r[1] = (T*) c->RESb; }
-	// r_0 = rhs - mean, written back to rhs (mg/solver.h:224-225) and, rounded,
-	// as the first cycle's right-hand side; the relative tolerance
-	hipLaunchKernelGGL((k_update_resid<T, T>), dim3(rb), dim3(256), 0, c->stream, (const double*) nullptr,
-	                   (const T*) nullptr, rhs, (double*) nullptr, r[0], nx, ny, off, dg, 1, 1, dctl, c->partials);
+	// r_0 = rhs - mean - L x_0, with rhs - mean written back to rhs
+	// (mg/solver.h:224-225) and r_0, rounded, as the first cycle's right-hand
+	// side; the relative tolerance
+	hipLaunchKernelGGL((k_update_resid<T, T>), dim3(rb), dim3(256), 0, c->stream, x0, (const T*) nullptr, rhs,
+	                   (double*) nullptr, r[0], nx, ny, off, dg, 1, 1, dctl, c->partials);
 	hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, tol_mode == 1 ? 2 : 1, rb, N, dctl, c->partials,
 	                   tol);
 	AFM_LAUNCH_CHECK();
@@ -1816,21 +1819,21 @@ static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 				hipLaunchKernelGGL((k_fcg_dots<T>), dim3(rb), dim3(256), 0, c->stream, z, rk, rkm1, nx, ny, off, dg,
 				                   c->fpartials);
 			}
-			const FcgArgs<T> a{z, k ? c->P64[(k + 1) & 1] : nullptr, c->P64[k & 1], k ? xs[k & 1] : nullptr,
+			const FcgArgs<T> a{z, k ? c->P64[(k + 1) & 1] : nullptr, c->P64[k & 1], k ? xs[k & 1] : x0,
 			                   xs[(k + 1) & 1], rhs, r[(k + 1) & 1], dctl, c->fpartials, fstride, fnblk,
 			                   c->gpartials[k & 1], k};
 			const int slot = (c->kt_on && k >= 1 && s < Ctx::KT_SLOTS) ? s : -1;
 			if (slot >= 0) AFM_HIP(hipEventRecord(c->kt_ev[2 * slot], c->stream));
-			if (tiled) {
-				if (k == 0)
-					hipLaunchKernelGGL((k_fcg_update<T, true>), dim3(gb), dim3(64 * FU_NW), 0, c->stream, a, nx, ny, off, dg);
-				else
-					hipLaunchKernelGGL((k_fcg_update<T, false>), dim3(gb), dim3(64 * FU_NW), 0, c->stream, a, nx, ny, off, dg);
-			} else if (k == 0) {
-				hipLaunchKernelGGL((k_fcg_update_1d<T, true>), dim3(gb), dim3(256), 0, c->stream, a, nx, ny, off, dg);
-			} else {
-				hipLaunchKernelGGL((k_fcg_update_1d<T, false>), dim3(gb), dim3(256), 0, c->stream, a, nx, ny, off, dg);
-			}
+			const dim3 blk(tiled ? 64 * FU_NW : 256);
+#define AFM_FU(F_, Z_)                                                                                    \
+			do {                                                                                          \
+				if (tiled) hipLaunchKernelGGL((k_fcg_update<T, F_, Z_>), dim3(gb), blk, 0, c->stream, a, nx, ny, off, dg);    \
+				else hipLaunchKernelGGL((k_fcg_update_1d<T, F_, Z_>), dim3(gb), blk, 0, c->stream, a, nx, ny, off, dg);   \
+			} while (0)
+			if (k > 0) AFM_FU(false, false);
+			else if (x0) AFM_FU(true, false);
+			else AFM_FU(true, true);
+#undef AFM_FU
 			if (slot >= 0) AFM_HIP(hipEventRecord(c->kt_ev[2 * slot + 1], c->stream));
 			if (!fused)
 				hipLaunchKernelGGL(k_fcg_fin, dim3(1), dim3(256), 0, c->stream, k + 1, gb, (double) N, dctl,
@@ -1860,15 +1863,16 @@ static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 	}
 	const int cnt = c->hctl[0].count;
 	if (cnt == 0) {
-		// r_0 already met the tolerance: x = 0
-		hipLaunchKernelGGL(k_zero, dim3((unsigned) ((N + 255) / 256)), dim3(256), 0, c->stream, xs[0], N);
+		// r_0 already met the tolerance: x = x_0
+		if (x0) AFM_HIP(hipMemcpyAsync(xs[0], x0, N * 8, hipMemcpyDeviceToDevice, c->stream));
+		else hipLaunchKernelGGL(k_zero, dim3((unsigned) ((N + 255) / 256)), dim3(256), 0, c->stream, xs[0], N);
 		AFM_LAUNCH_CHECK();
 	}
 	return mg_finish(c, rhs, xs[cnt & 1], cycles, final_rms, sol);
 }
 
 int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, int precision, int max_cycles,
-             int* cycles, double* final_rms, const double** sol)
+             int* cycles, double* final_rms, const double** sol, const double* x0)
 {
 	if (nu1 < 1 || nu1 > 2 || nu2 < 1 || nu2 > 2) { set_error("nu1, nu2 must be 1 or 2 (one fused launch per smoothing step)"); return -1; }
 	if (precision < 0 || precision > 3) {
@@ -1889,8 +1893,8 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 	                   tol_mode == 1 ? 0.0 : tol, max_cycles);
 	hipLaunchKernelGGL(k_mean_fin, dim3(1), dim3(256), 0, c->stream, 0, tol, rb, N, dctl, c->partials);
 	AFM_LAUNCH_CHECK();
-	if (precision == 1) return fcg_solve<float>(c, rhs, tol, tol_mode, nu1, nu2, max_cycles, cycles, final_rms, sol);
-	if (precision == 3) return fcg_solve<double>(c, rhs, tol, tol_mode, nu1, nu2, max_cycles, cycles, final_rms, sol);
+	if (precision == 1) return fcg_solve<float>(c, rhs, tol, tol_mode, nu1, nu2, max_cycles, cycles, final_rms, sol, x0);
+	if (precision == 3) return fcg_solve<double>(c, rhs, tol, tol_mode, nu1, nu2, max_cycles, cycles, final_rms, sol, x0);
 	// the reference's iteration (precision 0), or the same in FP32 V-cycles
 	// (2): nested start and its residual, mean removed from rhs in place
 	if (precision == 2 && T > c->lc) {

@@ -522,6 +522,13 @@ struct NS {
 	int tol_mode, precision;
 	double *u, *v;                         // caller-owned, aliased (ns/solver.h:54-56)
 	double *uo = nullptr, *vo = nullptr, *phi = nullptr, *fz = nullptr;
+	// The CG forms (precision 1, 3) start a step's projection from the
+	// potential of the previous step of the same kind (0: backward-Euler half
+	// step, 1: Crank-Nicolson full step), once there is one; the potential
+	// of every projection is kept where get_p reads it (phi_last).
+	double* start[2] = {nullptr, nullptr};
+	bool start_ok[2] = {false, false};
+	const double* phi_last = nullptr;
 	CGWork cg;
 	bool half = false;
 	int last_cycles = 0, last_cg[2] = {0, 0};
@@ -530,19 +537,25 @@ struct NS {
 	~NS() { for (void* p : own) (void) hipFree(p); }
 };
 
-int ns_project(NS* s, double* u, double* v)
+// kind: the step kind whose start potential the solve uses and replaces
+// (0, 1), or -1 for a projection outside the time steps
+int ns_project(NS* s, double* u, double* v, int kind)
 {
 	Ctx* c = s->c;
 	AFM_TRY(op_div(c, u, v, s->phi));
 	int cyc = 0;
 	const double* sol = nullptr;
-	AFM_TRY(mg_solve(c, s->phi, s->tol, s->tol_mode, 2, 2, s->precision, s->max_cycles, &cyc, nullptr, &sol));
+	double* keep = (kind >= 0 && s->start[kind]) ? s->start[kind] : s->phi;
+	const double* x0 = (keep != s->phi && s->start_ok[kind]) ? keep : nullptr;
+	AFM_TRY(mg_solve(c, s->phi, s->tol, s->tol_mode, 2, 2, s->precision, s->max_cycles, &cyc, nullptr, &sol, x0));
 	s->last_cycles = cyc;
 	// the gradient is taken where the solve left the potential, and the
-	// potential is stored to phi in the same pass
+	// potential is stored (to phi, or as the kind's next start) in the same pass
 	hipLaunchKernelGGL(k_grad_sub, pgrid(c->N), dim3(256), 0, c->stream, sol, u, v, (int) c->nx, (int) c->ny, c->n,
-	                   s->phi);
+	                   keep);
 	AFM_LAUNCH_CHECK();
+	if (keep != s->phi) s->start_ok[kind] = true;
+	s->phi_last = keep;
 	return 0;
 }
 
@@ -559,9 +572,13 @@ int ns_create(Ctx* c, double mu, double rho, double dt, double tol, int tol_mode
 		delete s;
 		return rc;
 	}
+	if (precision == 1 || precision == 3)
+		for (int k = 0; k < 2; ++k)
+			if ((rc = dalloc(o, (void**) &s->start[k], c->N * 8))) { delete s; return rc; }
+	s->phi_last = s->phi;
 	AFM_HIP(hipMemsetAsync(s->uo, 0, c->N * 8, c->stream));
 	AFM_HIP(hipMemsetAsync(s->vo, 0, c->N * 8, c->stream));
-	rc = ns_project(s, s->u, s->v);     // ns/solver.h:71
+	rc = ns_project(s, s->u, s->v, -1);     // ns/solver.h:71
 	if (rc) { delete s; return rc; }
 	*out = s;
 	return 0;
@@ -580,7 +597,7 @@ int ns_step(NS* s, const double* fx, const double* fy)
 	double* xs[2] = {s->u, s->v};
 	const double* bs[2] = {nullptr, nullptr};
 	AFM_TRY(op_cg2(c, s->cg, ss, xs, bs, s->tol, s->tol_mode, s->max_cg, s->last_cg, nullptr, &adv));
-	AFM_TRY(ns_project(s, s->u, s->v));
+	AFM_TRY(ns_project(s, s->u, s->v, s->half ? 1 : 0));
 	s->half = !s->half;
 	return 0;
 }
@@ -588,7 +605,7 @@ int ns_step(NS* s, const double* fx, const double* fy)
 // ns/solver.h:86: p = phhs * phidt
 int ns_get_p(NS* s, double* p)
 {
-	return op_helm_apply(s->c, -s->mu * s->dt / (2 * s->rho), s->phi, p);
+	return op_helm_apply(s->c, -s->mu * s->dt / (2 * s->rho), s->phi_last, p);
 }
 
 // ------------------------------------------------------------ IB driver ---
@@ -704,18 +721,39 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 	double* ub = A(N); double* vb = A(N);
 	if (!c0 || !c1 || !part || !Xb || !ub || !vb) { set_error("out of device memory (adjoint)"); return fail(-3); }
 	if ((rc = adj_centroids(c, b->X, nm, msize, nullptr, c0, nullptr, nullptr))) return fail(rc);
-	std::vector<double*> ck(want_grad ? 3 * nseg : 0);
+	// a checkpoint: u, v, X and, for the CG forms, the projections' start
+	// potentials (so that a segment's recomputation repeats the forward run
+	// bit for bit)
+	const bool warm = s->start[0] != nullptr;
+	const int K = warm ? 5 : 3;
+	std::vector<double*> ck(want_grad ? K * nseg : 0);
+	std::vector<char> ck_ok(want_grad ? 2 * nseg : 0);
 	for (size_t i = 0; i < ck.size(); ++i) {
-		ck[i] = A(i % 3 == 2 ? 2 * m : N);
+		ck[i] = A(i % K == 2 ? 2 * m : N);
 		if (!ck[i]) { set_error("out of device memory (checkpoints)"); return fail(-3); }
 	}
-	for (int st = 0; st < steps; ++st) {
-		if (want_grad && st % every == 0) {
-			int sg = st / every;
-			if ((rc = d2d(c, ck[3 * sg], s->u, N)) || (rc = d2d(c, ck[3 * sg + 1], s->v, N)) ||
-			    (rc = d2d(c, ck[3 * sg + 2], b->X, 2 * m)))
-				return fail(rc);
+	auto save_ck = [&](int sg) -> int {
+		AFM_TRY(d2d(c, ck[K * sg], s->u, N));
+		AFM_TRY(d2d(c, ck[K * sg + 1], s->v, N));
+		AFM_TRY(d2d(c, ck[K * sg + 2], b->X, 2 * m));
+		for (int k = 0; warm && k < 2; ++k) {
+			AFM_TRY(d2d(c, ck[K * sg + 3 + k], s->start[k], N));
+			ck_ok[2 * sg + k] = s->start_ok[k];
 		}
+		return 0;
+	};
+	auto load_ck = [&](int sg) -> int {
+		AFM_TRY(d2d(c, s->u, ck[K * sg], N));
+		AFM_TRY(d2d(c, s->v, ck[K * sg + 1], N));
+		AFM_TRY(d2d(c, b->X, ck[K * sg + 2], 2 * m));
+		for (int k = 0; warm && k < 2; ++k) {
+			AFM_TRY(d2d(c, s->start[k], ck[K * sg + 3 + k], N));
+			s->start_ok[k] = ck_ok[2 * sg + k] != 0;
+		}
+		return 0;
+	};
+	for (int st = 0; st < steps; ++st) {
+		if (want_grad && st % every == 0 && (rc = save_ck(st / every))) return fail(rc);
 		if ((rc = ib_step_taped(b, nullptr))) return fail(rc);
 	}
 	// objective and its adjoint at the final time
@@ -734,6 +772,8 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 	}
 	// final state, kept to restore after the sweep
 	double* fu = A(N); double* fv = A(N); double* fX = A(2 * m);
+	double* fs[2] = {warm ? A(N) : nullptr, warm ? A(N) : nullptr};
+	const bool fs_ok[2] = {s->start_ok[0], s->start_ok[1]};
 	std::vector<StepTape> tape(every);
 	for (auto& t : tape) { t.u = A(N); t.v = A(N); t.uh = A(N); t.vh = A(N); t.X = A(2 * m); t.Xh = A(2 * m); }
 	double* unb = A(N); double* vnb = A(N); double* uhb = A(N); double* vhb = A(N);
@@ -744,11 +784,13 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 	double* cx = A(m); double* cy = A(m); double* volv = A(m);
 	double* kb = gk ? gk : A(m); double* Rb = gR ? gR : A(2 * m);
 	if (!fu || !fv || !fX || !unb || !vnb || !uhb || !vhb || !us || !vs || !rbu || !rbv || !t1 || !t2 || !Xnb ||
-	    !Xhb || !Ux || !Uy || !Jx || !Jy || !cx || !cy || !volv || !kb || !Rb || oom) {
+	    !Xhb || !Ux || !Uy || !Jx || !Jy || !cx || !cy || !volv || !kb || !Rb || (warm && (!fs[0] || !fs[1])) || oom) {
 		set_error("out of device memory (adjoint tape)");
 		return fail(-3);
 	}
 	if ((rc = d2d(c, fu, s->u, N)) || (rc = d2d(c, fv, s->v, N)) || (rc = d2d(c, fX, b->X, 2 * m))) return fail(rc);
+	for (int k = 0; warm && k < 2; ++k)
+		if ((rc = d2d(c, fs[k], s->start[k], N))) return fail(rc);
 	const double vol = 1. / (c->n * c->n);
 	{
 		std::vector<double> hv(m, vol);
@@ -767,16 +809,14 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 	};
 	for (int sg = nseg - 1; sg >= 0; --sg) {
 		const int len = std::min(every, steps - sg * every);
-		if ((rc = d2d(c, s->u, ck[3 * sg], N)) || (rc = d2d(c, s->v, ck[3 * sg + 1], N)) ||
-		    (rc = d2d(c, b->X, ck[3 * sg + 2], 2 * m)))
-			return fail(rc);
+		if ((rc = load_ck(sg))) return fail(rc);
 		s->half = false;
 		for (int j = 0; j < len; ++j)
 			if ((rc = ib_step_taped(b, &tape[j]))) return fail(rc);
 		for (int j = len - 1; j >= 0; --j) {
 			const StepTape& tp = tape[j];
-			const double* unx = j == len - 1 ? (sg == nseg - 1 ? fu : ck[3 * (sg + 1)]) : tape[j + 1].u;
-			const double* vnx = j == len - 1 ? (sg == nseg - 1 ? fv : ck[3 * (sg + 1) + 1]) : tape[j + 1].v;
+			const double* unx = j == len - 1 ? (sg == nseg - 1 ? fu : ck[K * (sg + 1)]) : tape[j + 1].u;
+			const double* vnx = j == len - 1 ? (sg == nseg - 1 ? fv : ck[K * (sg + 1) + 1]) : tape[j + 1].v;
 			// step 7: X' = X + dt 0.5 (J(Xh) u^n + J(Xh) u^{n+1})
 			AFM_TRY(d2d(c, Xnb, Xb, 2 * m));
 			AFM_TRY(adj_split_scale(c, Xb, 0.5 * dt, m, Ux, Uy));
@@ -791,7 +831,7 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 			AFM_TRY(adj_interp_dX(c, 1, vnx, tp.Xh, Uy, m, Xhb));
 			// step 6: Crank-Nicolson full step
 			AFM_TRY(d2d(c, us, ub, N)); AFM_TRY(d2d(c, vs, vb, N));
-			AFM_TRY(ns_project(s, us, vs));
+			AFM_TRY(ns_project(s, us, vs, -1));
 			AFM_TRY(solve_H(rbu, rbv, us, vs));
 			AFM_TRY(op_helm_apply(c, sm, rbu, t1)); AFM_TRY(adj_axpy(c, unb, t1, 1.0, N));
 			AFM_TRY(op_helm_apply(c, sm, rbv, t2)); AFM_TRY(adj_axpy(c, vnb, t2, 1.0, N));
@@ -815,7 +855,7 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 			AFM_TRY(adj_interp_dX(c, 0, tp.uh, tp.X, Ux, m, Xnb));
 			AFM_TRY(adj_interp_dX(c, 1, tp.vh, tp.X, Uy, m, Xnb));
 			// step 3: backward-Euler half step
-			AFM_TRY(ns_project(s, uhb, vhb));
+			AFM_TRY(ns_project(s, uhb, vhb, -1));
 			AFM_TRY(solve_H(rbu, rbv, uhb, vhb));
 			AFM_TRY(adj_axpy(c, unb, rbu, 1.0, N));
 			AFM_TRY(adj_axpy(c, vnb, rbv, 1.0, N));
@@ -835,6 +875,11 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 	}
 	// leave the solver at the final state, as a forward run would
 	if ((rc = d2d(c, s->u, fu, N)) || (rc = d2d(c, s->v, fv, N)) || (rc = d2d(c, b->X, fX, 2 * m))) return fail(rc);
+	for (int k = 0; warm && k < 2; ++k) {
+		if ((rc = d2d(c, s->start[k], fs[k], N))) return fail(rc);
+		s->start_ok[k] = fs_ok[k];
+	}
+	if (warm) s->phi_last = s->start[1];     // the last forward projection: a full step's
 	s->half = false;
 	return fail(0);
 }
@@ -850,6 +895,28 @@ Ctx* ib_ctx(IB* b) { return b->ns->c; }
 void ib_free(IB* b) { (void) hipStreamSynchronize(b->ns->c->stream); delete b; }
 Ctx* ns_ctx(NS* s) { return s->c; }
 void ns_free(NS* s) { (void) hipStreamSynchronize(s->c->stream); delete s; }
+
+// the CG forms' start potential of a step kind (see NS): copy out, or in
+int ns_get_start(NS* s, int kind, double* phi, int* valid)
+{
+	if (kind < 0 || kind > 1) { set_error("kind must be 0 (half step) or 1 (full step)"); return -1; }
+	*valid = s->start[kind] && s->start_ok[kind];
+	if (*valid) AFM_HIP(hipMemcpyAsync(phi, s->start[kind], s->c->N * 8, hipMemcpyDeviceToDevice, s->c->stream));
+	return 0;
+}
+
+int ns_set_start(NS* s, int kind, const double* phi)
+{
+	if (kind < 0 || kind > 1) { set_error("kind must be 0 (half step) or 1 (full step)"); return -1; }
+	if (!s->start[kind]) {
+		if (!phi) return 0;
+		set_error("only the CG forms (precision 1, 3) start from a potential");
+		return -1;
+	}
+	s->start_ok[kind] = phi != nullptr;
+	if (phi) AFM_HIP(hipMemcpyAsync(s->start[kind], phi, s->c->N * 8, hipMemcpyDeviceToDevice, s->c->stream));
+	return 0;
+}
 
 int ns_stats(NS* s, int* cyc, int* cgu, int* cgv, int* half)
 {

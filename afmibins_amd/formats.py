@@ -10,10 +10,12 @@ precision reproduces the reference's text exactly).
 
 A checkpoint holds what an IB run needs to continue: the velocity fields, the
 marker state (positions, tether points, stiffness, source volumes), the grid
-and fluid parameters and the step count.  Between IB steps nothing else
-carries over (each step's backward-Euler half step takes its old velocity
-from u, v).  It is one ``.npz`` file, written and read with numpy alone and
-never unpickled; the parameters sit in a JSON string array inside it.
+and fluid parameters and the step count, and -- for the CG forms of the
+projection (precision 1, 3) -- the two start potentials those solves begin
+from.  Between IB steps nothing else carries over (each step's
+backward-Euler half step takes its old velocity from u, v).  It is one
+``.npz`` file, written and read with numpy alone and never unpickled; the
+parameters sit in a JSON string array inside it.
 """
 import json
 import math
@@ -24,7 +26,7 @@ import numpy as np
 _NUM = r"[-+]?(?:nan|inf|(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?)"
 _VEC = re.compile(r"\[\s*((?:" + _NUM + r"\s*(?:,\s*" + _NUM + r"\s*)*)?)\]'")
 _PT = re.compile(r"(\d+)\s*\(\s*(" + _NUM + r")\s*,\s*(" + _NUM + r")\s*,\s*(" + _NUM + r")\s*\)")
-CHECKPOINT_VERSION = 1
+CHECKPOINT_VERSION = 2        # 2: optional start potentials (version 1 files are read too)
 
 
 def _num(tok):
@@ -82,14 +84,21 @@ def format_point(index, x, y, z, precision=6):
 _ARRAYS = ("u", "v", "X", "R", "k", "ds")
 
 
-def save_checkpoint(path, u, v, X, R, k, ds, *, n, mu, rho, dt, tol, tol_mode=1, precision=0, step=0):
+def save_checkpoint(path, u, v, X, R, k, ds, *, n, mu, rho, dt, tol, tol_mode=1, precision=0, step=0,
+                    starts=(None, None)):
     """Write an IB state.  Arrays may be torch tensors (any device) or numpy
-    arrays; they are stored as float64."""
+    arrays; they are stored as float64.  starts: the solver's start
+    potentials (NSSolver.get_start(0), get_start(1)), None where unset."""
     def host(a):
         if hasattr(a, "detach"):
             a = a.detach().cpu().numpy()
         return np.ascontiguousarray(a, dtype=np.float64)
     arrays = {name: host(a) for name, a in zip(_ARRAYS, (u, v, X, R, k, ds))}
+    for j, phi in enumerate(starts):
+        if phi is not None:
+            arrays[f"start{j}"] = host(phi)
+            if arrays[f"start{j}"].shape != (n, n):
+                raise ValueError(f"start potential {j} must be {n} x {n}")
     m = arrays["X"].shape[0]
     if arrays["u"].shape != (n, n) or arrays["v"].shape != (n, n):
         raise ValueError(f"u and v must be {n} x {n}")
@@ -98,7 +107,8 @@ def save_checkpoint(path, u, v, X, R, k, ds, *, n, mu, rho, dt, tol, tol_mode=1,
         raise ValueError("X, R must be (m, 2) and k, ds (m,)")
     meta = {"format": "afmibins-ib-checkpoint", "version": CHECKPOINT_VERSION, "n": int(n), "mu": float(mu),
             "rho": float(rho), "dt": float(dt), "tol": float(tol), "tol_mode": int(tol_mode),
-            "precision": int(precision), "step": int(step), "markers": int(m)}
+            "precision": int(precision), "step": int(step), "markers": int(m),
+            "starts": [s is not None for s in starts]}
     np.savez(path, meta=np.array([json.dumps(meta)]), **arrays)
 
 
@@ -108,9 +118,13 @@ def load_checkpoint(path):
         meta = json.loads(str(z["meta"][0]))
         if meta.get("format") != "afmibins-ib-checkpoint":
             raise ValueError(f"{path} is not an IB checkpoint")
-        if meta.get("version") != CHECKPOINT_VERSION:
-            raise ValueError(f"{path}: checkpoint version {meta.get('version')} (this build reads {CHECKPOINT_VERSION})")
+        if meta.get("version") not in (1, CHECKPOINT_VERSION):
+            raise ValueError(f"{path}: checkpoint version {meta.get('version')} (this build reads 1 and "
+                             f"{CHECKPOINT_VERSION})")
         arrays = {name: np.array(z[name], dtype=np.float64) for name in _ARRAYS}
+        for j, present in enumerate(meta.get("starts", [False, False])):
+            if present:
+                arrays[f"start{j}"] = np.array(z[f"start{j}"], dtype=np.float64)
     return meta, arrays
 
 
@@ -129,5 +143,8 @@ def restore(path, device="cuda:0"):
     # they are written back into the solver's (aliased) u, v
     u.copy_(t(a["u"]))
     v.copy_(t(a["v"]))
+    for j in range(2):
+        if f"start{j}" in a:
+            ns.set_start(j, t(a[f"start{j}"]))
     ib = A.IBSolver(ns, t(a["X"]), t(a["R"]), t(a["k"]), t(a["ds"]))
     return ctx, ns, ib, u, v, meta

@@ -146,6 +146,17 @@ int afm_ns_project(afm_ns* s, double* u, double* v);             /* ns/projectio
 int afm_ns_get_p(afm_ns* s, double* p);                         /* ns/solver.h:86 */
 int afm_ns_stats(afm_ns* s, int* last_cycles, int* last_cg_u, int* last_cg_v, int* have_half_stepped);
 int afm_ns_old(afm_ns* s, const double** u_old, const double** v_old);
+/* The CG forms' start potentials (precision 1 and 3; no reference
+ * counterpart: the reference's solve starts from its nested iteration).
+ * Such a solver starts a step's projection from the potential of the
+ * previous step of the same kind -- kind 0 the backward-Euler half step's,
+ * 1 the Crank-Nicolson full step's -- once there is one.  get copies it to
+ * phi (n x n on the device) and sets *valid (0: the next solve of that kind
+ * starts from zero and phi is untouched); set copies phi in (NULL: start
+ * from zero again).  A checkpoint of such a solver saves and restores both
+ * to continue bit for bit. */
+int afm_ns_get_start(afm_ns* s, int kind, double* phi, int* valid);
+int afm_ns_set_start(afm_ns* s, int kind, const double* phi);
 int afm_ns_destroy(afm_ns* s);
 
 /* Immersed-boundary time stepping (no reference counterpart: the reference

@@ -65,6 +65,14 @@ def test_checkpoint_round_trip_and_checks(tmp_path):
     assert meta["mu"] == 1e-3 and meta["dt"] == 2e-5
     for name, a in arrays.items():
         assert np.array_equal(got[name], a), name
+    assert "start0" not in got and "start1" not in got
+    # the CG forms' start potentials: either, both or none
+    s1 = rng.standard_normal((n, n))
+    FM.save_checkpoint(p, **arrays, n=n, mu=1e-3, rho=1.0, dt=2e-5, tol=1e-10, precision=1, starts=(None, s1))
+    meta, got = FM.load_checkpoint(p)
+    assert meta["starts"] == [False, True] and "start0" not in got and np.array_equal(got["start1"], s1)
+    with pytest.raises(ValueError):
+        FM.save_checkpoint(p, **arrays, n=n, mu=1e-3, rho=1.0, dt=2e-5, tol=1e-10, starts=(s1[:3], None))
     with pytest.raises(ValueError):
         FM.save_checkpoint(p, **dict(arrays, u=arrays["u"][:3]), n=n, mu=1e-3, rho=1.0, dt=2e-5, tol=1e-10)
     bad = str(tmp_path / "other.npz")
