@@ -15,8 +15,9 @@ Two forms are timed on identical inputs: plain FP64 (the reference's
 multigrid iteration: nested start, then V(2,2) cycles until the residual test
 holds), and the mixed form (`value`, the headline): flexible CG in FP64 --
 true residual, search directions, inner products, solution -- preconditioned
-by one FP32 V(2,2) cycle per iteration.  `--forms` also takes mixed_ir (the
-reference's iteration with FP32 cycles, FP64 iterative refinement) and
+by one FP32 V(2,2) cycle per iteration, each step's projection starting from
+the previous potential of the same step kind.  `--forms` also takes mixed_ir
+(the reference's iteration with FP32 cycles, FP64 iterative refinement) and
 fp64_cg (the CG form with FP64 cycles), which split the mixed form's gain
 into its precision and Krylov parts.  Inputs are resident in HBM before the
 timed region; the timed region is K full steps bracketed by device
@@ -25,13 +26,13 @@ synchronisation.
 `roofline` is the headline form's largest FP64 launch, timed with HIP events
 on the stream it runs on during extra steps of the same workload: the CG
 update k_fcg_update (iterations after the first), which reads z (FP32, 4 B),
-p, x and b (FP64, 8 + 8 + 8) and writes p, x, r (8 + 8 + 8) and r rounded
-(4): 56 B per finest cell.  The FP64 form's is the finest-level
-post-smoothing fused with the outer update (k_smooth_update): read e_pre and
-b (8 + 8), the coarse correction (8 per 4 cells = 2), xs and rhs (8 + 8);
-write xs and r (8 + 8): 50 B (37 B with FP32 cycles).  `cpu_baseline` is the
-oracle's restatement of the reference path on one host core over a one-step
-sample.
+p, x and b (FP64, 8 + 8 + 8) and writes p and x (8 + 8) and r rounded to the
+cycle's FP32 (4): 48 B per finest cell (56 B with FP64 cycles).  The FP64
+form's is the finest-level post-smoothing fused with the outer update
+(k_smooth_update): read e_pre and b (8 + 8), the coarse correction (8 per 4
+cells = 2), xs and rhs (8 + 8); write xs and r (8 + 8): 50 B (37 B with FP32
+cycles).  `cpu_baseline` is the oracle's restatement of the reference path on
+one host core over a one-step sample.
 """
 import argparse
 import ctypes as C
@@ -47,7 +48,7 @@ HBM_PEAK_GBS = 8000.0
 # form: (precision code of afm_mg_solve, timed launch, its bytes per finest cell, counter summary)
 FORMS = {
     "fp64": (0, "k_smooth_update<double, double, 2>", 50.0, None),
-    "mixed": (1, "k_fcg_update<float>", 56.0, "k_fcg_update_f32_2048_pmc.txt"),
+    "mixed": (1, "k_fcg_update<float>", 48.0, "k_fcg_update_f32_2048_pmc.txt"),
     "mixed_ir": (2, "k_smooth_update<float, float, 2>", 37.0, "k_smooth_update_f32_2048_pmc.txt"),
     "fp64_cg": (3, "k_fcg_update<double>", 56.0, None),
 }
