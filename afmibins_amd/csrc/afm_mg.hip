@@ -978,8 +978,10 @@ __global__ __launch_bounds__(256) void k_resid_restr(const TS* __restrict__ x, c
 // ------------------------------------------------------- update + residual --
 // xs_new = xs + e (e may be null), r = rhs' - L xs_new, where rhs' = rhs -
 // mean when `sub_mean` (written back to rhs).  Deterministic RMS of r and the
-// stop test (mg/solver.h:226-233) in the last block.
-template<typename TE, typename TR>
+// stop test (mg/solver.h:226-233) in the last block.  XS / E: xs / e given
+// (template flags, so a cell's loads issue together; update_resid_launch
+// picks the variant from the pointers).
+template<typename TE, typename TR, bool XS, bool E>
 __global__ __launch_bounds__(256) void k_update_resid(const double* __restrict__ xs, const TE* __restrict__ e,
                                                       double* __restrict__ rhs, double* __restrict__ xs_out,
                                                       TR* __restrict__ res, int nx, int ny, double off, double dg,
@@ -1000,16 +1002,16 @@ __global__ __launch_bounds__(256) void k_update_resid(const double* __restrict__
 		const size_t in_ = gy == ny - 1 ? gx : i + nx;
 		const size_t is = gy == 0 ? i + N - nx : i - nx;
 		// xs == nullptr: the iterate is zero and e (below) is the whole start
-		double c = 0.0, E = 0.0, Wv = 0.0, Nv = 0.0, Sv = 0.0;
-		if (xs) { c = xs[i]; E = xs[ie]; Wv = xs[iw]; Nv = xs[in_]; Sv = xs[is]; }
-		if (e) {
-			c = c + (double) e[i]; E = E + (double) e[ie]; Wv = Wv + (double) e[iw];
+		double c = 0.0, Ev = 0.0, Wv = 0.0, Nv = 0.0, Sv = 0.0;
+		if (XS) { c = xs[i]; Ev = xs[ie]; Wv = xs[iw]; Nv = xs[in_]; Sv = xs[is]; }
+		if (E) {
+			c = c + (double) e[i]; Ev = Ev + (double) e[ie]; Wv = Wv + (double) e[iw];
 			Nv = Nv + (double) e[in_]; Sv = Sv + (double) e[is];
 			xs_out[i] = c;
 		}
 		double r0 = rhs[i];
 		if (sub_mean) { r0 = r0 - mean * 1.0; rhs[i] = r0; acc[1] += r0 * r0; }
-		double Lx = stencil5<double, true>(off, dg, c, E, Wv, Nv, Sv, gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
+		double Lx = stencil5<double, true>(off, dg, c, Ev, Wv, Nv, Sv, gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
 		double r = r0 - Lx;
 		res[i] = (TR) r;
 		acc[0] += r * r;
@@ -1019,6 +1021,22 @@ __global__ __launch_bounds__(256) void k_update_resid(const double* __restrict__
 		double a1[1] = {acc[0]};
 		reduce_partial<1>(a1, partials);
 	}
+}
+
+template<typename TE, typename TR>
+static void update_resid_launch(Ctx* c, unsigned grid, const double* xs, const TE* e, double* rhs, double* xs_out,
+                                TR* res, double off, double dg, int sub_mean, int first, SolveCtl* ctl,
+                                double* partials)
+{
+	const int nx = (int) c->nx, ny = (int) c->ny;
+#define AFM_UR(X_, E_)                                                                                           \
+	hipLaunchKernelGGL((k_update_resid<TE, TR, X_, E_>), dim3(grid), dim3(256), 0, c->stream, xs, e, rhs, xs_out, res, \
+	                   nx, ny, off, dg, sub_mean, first, ctl, partials)
+	if (xs && e) AFM_UR(true, true);
+	else if (xs) AFM_UR(true, false);
+	else if (e) AFM_UR(false, true);
+	else AFM_UR(false, false);
+#undef AFM_UR
 }
 
 // RMS of the residual and the stop test (mg/solver.h:228), one block.
@@ -1783,8 +1801,8 @@ static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 	// r_0 = rhs - mean - L x_0, with rhs - mean written back to rhs
 	// (mg/solver.h:224-225) and r_0, rounded, as the first cycle's right-hand
 	// side; the relative tolerance
-	hipLaunchKernelGGL((k_update_resid<T, T>), dim3(rb), dim3(256), 0, c->stream, x0, (const T*) nullptr, rhs,
-	                   (double*) nullptr, r[0], nx, ny, off, dg, 1, 1, dctl, c->partials);
+	update_resid_launch<T, T>(c, rb, x0, (const T*) nullptr, rhs, (double*) nullptr,
+	                          r[0], off, dg, 1, 1, dctl, c->partials);
 	hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, tol_mode == 1 ? 2 : 1, rb, N, dctl, c->partials,
 	                   tol);
 	AFM_LAUNCH_CHECK();
@@ -1901,19 +1919,16 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 	// (2): nested start and its residual, mean removed from rhs in place
 	if (precision == 2 && T > c->lc) {
 		AFM_TRY(nested32(c, rhs, nu1, nu2));
-		hipLaunchKernelGGL((k_update_resid<float, float>), dim3(rb), dim3(256), 0, c->stream, (const double*) nullptr,
-		                   (const float*) c->RES32b, rhs, c->XSa, c->RES32, (int) c->nx, (int) c->ny, off, dg, 1, 1,
-		                   dctl, c->partials);
+		update_resid_launch<float, float>(c, rb, (const double*) nullptr, (const float*) c->RES32b, rhs, c->XSa,
+		                                  c->RES32, off, dg, 1, 1, dctl, c->partials);
 	} else {
 		AFM_TRY(nested(c, rhs, nu1, nu2));
 		if (precision == 0)
-			hipLaunchKernelGGL((k_update_resid<double, double>), dim3(rb), dim3(256), 0, c->stream, c->XSa,
-			                   (const double*) nullptr, rhs, c->XSa, c->RES, (int) c->nx, (int) c->ny, off, dg, 1, 1, dctl,
-			                   c->partials);
+			update_resid_launch<double, double>(c, rb, c->XSa, (const double*) nullptr, rhs, c->XSa,
+			                                    c->RES, off, dg, 1, 1, dctl, c->partials);
 		else
-			hipLaunchKernelGGL((k_update_resid<float, float>), dim3(rb), dim3(256), 0, c->stream, c->XSa,
-			                   (const float*) nullptr, rhs, c->XSa, c->RES32, (int) c->nx, (int) c->ny, off, dg, 1, 1,
-			                   dctl, c->partials);
+			update_resid_launch<float, float>(c, rb, c->XSa, (const float*) nullptr, rhs, c->XSa,
+			                                  c->RES32, off, dg, 1, 1, dctl, c->partials);
 	}
 	hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, tol_mode == 1 ? 2 : 1, rb, N, dctl, c->partials,
 	                   tol);
@@ -1947,9 +1962,8 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 					AFM_TRY(vcycle<double>(c, T, res64[rc], nu1, nu2, done, &up));
 				} else {
 					AFM_TRY(vcycle<double>(c, T, res64[rc], nu1, nu2, done));
-					hipLaunchKernelGGL((k_update_resid<double, double>), dim3(rb), dim3(256), 0, c->stream, cur,
-					                   (const double*) c->lv[T].Xb, rhs, nxt, res64[rn], (int) c->nx, (int) c->ny, off, dg, 0,
-					                   0, dctl, c->upartials);
+					update_resid_launch<double, double>(c, rb, cur, (const double*) c->lv[T].Xb, rhs, nxt,
+					                                    res64[rn], off, dg, 0, 0, dctl, c->upartials);
 				}
 			} else {
 				if (fused) {
@@ -1957,9 +1971,8 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 					AFM_TRY(vcycle<float>(c, T, res32[rc], nu1, nu2, done, &up));
 				} else {
 					AFM_TRY(vcycle<float>(c, T, res32[rc], nu1, nu2, done));
-					hipLaunchKernelGGL((k_update_resid<float, float>), dim3(rb), dim3(256), 0, c->stream, cur,
-					                   (const float*) c->lv[T].Xb32, rhs, nxt, res32[rn], (int) c->nx, (int) c->ny, off, dg, 0,
-					                   0, dctl, c->upartials);
+					update_resid_launch<float, float>(c, rb, cur, (const float*) c->lv[T].Xb32, rhs, nxt,
+					                                  res32[rn], off, dg, 0, 0, dctl, c->upartials);
 				}
 			}
 			if (!fused) hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, 0, ub, N, dctl, c->upartials);
