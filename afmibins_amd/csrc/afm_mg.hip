@@ -720,17 +720,18 @@ __device__ inline bool fcg_coefficients(const FcgArgs<TE>& a, double& beta, doub
 }
 
 // The update over tiles of 128 x 32 cells in registers, like the smoother:
-// lane k of wave w holds columns 2k, 2k+1 of rows 2w, 2w+1 of a region whose
-// outer ring of 2 cells is halo; p_k and x_{k+1} are formed on the whole
-// region, E/W neighbours by DPP, N/S through LDS boundary rows, and r_{k+1} =
-// b - L x_{k+1} on the inner 124 x 28 cells (short tiles: at 2048^2 the 1258
-// tiles spread over one block per CU more evenly than 595 taller ones, which
-// outweighs their 7% more halo reads).  Every cell of the
+// lane k of wave w holds columns 2k, 2k+1 of rows 4w .. 4w+3 of a region
+// whose outer ring of 2 cells is halo; p_k and x_{k+1} are formed on the
+// whole region, E/W neighbours by DPP, N/S through LDS boundary rows, and
+// r_{k+1} = b - L x_{k+1} on the inner 124 x 28 cells.  Every cell of the
 // grid is written by one tile; a halo cell's p and x are formed from the
 // same inputs by the same operations, so its value equals the stored one.
-// One block per CU walks tiles blockIdx.x, + gridDim.x, ... (one prologue
-// per block; the partial sums per block follow that fixed order).
-constexpr int FU_NW = 16, FU_RPW = 2, FU_H = 2, FU_LY = FU_NW * FU_RPW;
+// Two 512-thread blocks per CU walk tiles blockIdx.x, + gridDim.x, ... (one
+// coefficient prologue per block; the partial sums per block follow that
+// fixed order): while one block computes, the other's loads are in flight.
+// At 2048^2 the 1258 short tiles spread over the blocks more evenly than
+// 595 taller ones, which outweighs their 7% more halo reads.
+constexpr int FU_NW = 8, FU_RPW = 4, FU_H = 2, FU_LY = FU_NW * FU_RPW;
 constexpr int FU_TXO = 128 - 2 * FU_H, FU_TYO = FU_LY - 2 * FU_H;
 
 static inline unsigned fcg_update_blocks(int nx, int ny)
@@ -1990,7 +1991,7 @@ static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 	// tiles' partial sums fit), else one cell per thread
 	const bool tiled = nx >= 128 && ny >= FU_LY;
 	const size_t gb0 = (N + 255) / 256;
-	const unsigned gb = tiled ? std::min(fcg_update_blocks(nx, ny), (unsigned) c->n_cu)
+	const unsigned gb = tiled ? std::min(fcg_update_blocks(nx, ny), 2u * (unsigned) c->n_cu)
 	                          : (unsigned) (gb0 < (size_t) FCG_BLOCKS ? gb0 : (size_t) FCG_BLOCKS);
 	const double off = (double) (c->n * c->n), dg = -4.0 * off;
 	double* xs[2] = {c->XSa, c->XSb};
