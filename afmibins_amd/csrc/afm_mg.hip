@@ -1428,193 +1428,6 @@ __device__ void c_nested(CoarseLDS& S, int Lt, int nu1, int nu2)
 	if (Lt >= 6) c_nested_up<6>(S, nu1, nu2);
 }
 
-// The CG forms' coarse-grid correction: L x = b solved exactly on a level of
-// at most 64 x 64 cells, in place of the V-cycle's levels below it.  The
-// periodic 5-point L is circulant, so its eigenvectors are the discrete
-// Fourier modes: x = IDFT2(DFT2(b) / lambda) with lambda(kx, ky) =
-// -4 n^2 (sin^2(pi kx / n) + sin^2(pi ky / n)) and the constant mode set to
-// zero (the mean removal of the reference's coarsest solve).  Each 1-D
-// transform of n = 2^L <= 64 points runs in one wave, one element per lane
-// (radix 2, bit-reversed input, lane exchanges by shuffle); rows, then
-// columns through LDS (row stride padded to 65), FP64 for both forms.  A
-// preconditioner only: the reference's own iteration keeps its V-cycle.
-__device__ inline int bit_reverse(int k, int L) { return (int) (__builtin_bitreverse32((unsigned) k) >> (32 - L)); }
-
-// The value of lane k ^ M (M a power of two, partners in the same line
-// group): M = 1, 2 by DPP quad permutations, M = 4, 8 by DPP row shifts
-// (lane k + M for the upper element of a pair, k - M for the lower), all in
-// registers; M >= 16 through the LDS crossbar (ds_bpermute).  Every lane of
-// the wave is active here (DPP reads of masked-off lanes return the
-// reader's own value).
-template<int CTRL>
-__device__ inline double dpp_f64(double v)
-{
-	int lo = __double2loint(v), hi = __double2hiint(v);
-	lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
-	hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
-	return __hiloint2double(hi, lo);
-}
-
-template<int M>
-__device__ inline double lane_xor(double v, bool lower)
-{
-	if constexpr (M == 1) return dpp_f64<0xB1>(v);            // quad_perm [1, 0, 3, 2]
-	else if constexpr (M == 2) return dpp_f64<0x4E>(v);       // quad_perm [2, 3, 0, 1]
-	else if constexpr (M == 4 || M == 8) {
-		const double up = dpp_f64<0x100 + M>(v);              // row_shl:M, lane k <- k + M
-		const double dn = dpp_f64<0x110 + M>(v);              // row_shr:M, lane k <- k - M
-		return lower ? dn : up;
-	} else {
-		return __shfl_xor(v, M, 64);
-	}
-}
-
-// stage ST of NL lines of 2^L points (see fft_lanes)
-template<int L, int NL, int ST>
-__device__ inline void fft_stage(double (&re)[NL], double (&im)[NL], double sign, const double (&wr)[L],
-                                 const double (&wi)[L])
-{
-	constexpr int M = 1 << ST;
-	const int k = (int) threadIdx.x & ((1 << L) - 1);
-	const bool lower = (k & M) != 0;
-	const double cr = wr[ST], ci = sign * wi[ST];
-	double pr[NL], pi[NL];
-#pragma unroll
-	for (int j = 0; j < NL; ++j) { pr[j] = lane_xor<M>(re[j], lower); pi[j] = lane_xor<M>(im[j], lower); }
-#pragma unroll
-	for (int j = 0; j < NL; ++j) {
-		// the pair (a, b): a + w b to the upper lane, a - w b to the lower
-		const double br = lower ? re[j] : pr[j], bi = lower ? im[j] : pi[j];
-		const double ar = lower ? pr[j] : re[j], ai = lower ? pi[j] : im[j];
-		const double tr = br * cr - bi * ci, ti = br * ci + bi * cr;
-		re[j] = lower ? ar - tr : ar + tr;
-		im[j] = lower ? ai - ti : ai + ti;
-	}
-}
-
-// NL lines of 2^L points at once (their exchanges overlap), lane k of its
-// group of 2^L lanes holding element bit_reverse(k) of each on entry and
-// element k on exit; sign -1 forward, +1 inverse; (wr, wi)[st]: this lane's
-// twiddle exp(i 2 pi t / 2^L) of stage st, t = (k mod 2^st) 2^(L-1-st)
-template<int L, int NL>
-__device__ inline void fft_lanes(double (&re)[NL], double (&im)[NL], double sign, const double (&wr)[L],
-                                 const double (&wi)[L])
-{
-	fft_stage<L, NL, 0>(re, im, sign, wr, wi);
-	if constexpr (L > 1) fft_stage<L, NL, 1>(re, im, sign, wr, wi);
-	if constexpr (L > 2) fft_stage<L, NL, 2>(re, im, sign, wr, wi);
-	if constexpr (L > 3) fft_stage<L, NL, 3>(re, im, sign, wr, wi);
-	if constexpr (L > 4) fft_stage<L, NL, 4>(re, im, sign, wr, wi);
-	if constexpr (L > 5) fft_stage<L, NL, 5>(re, im, sign, wr, wi);
-}
-
-constexpr int FFT_P = 65;   // padded LDS row stride of the 2-D transform
-
-struct FftLDS {
-	double re[64 * FFT_P], im[64 * FFT_P];
-	double twr[32], twi[32], s2[64];
-};
-
-template<typename TI, int L>
-__device__ inline void coarse_exact(const TI* __restrict__ in, TI* __restrict__ out, FftLDS& S)
-{
-	constexpr int N = 1 << L, GROUPS = CTHREADS >> L;
-	constexpr int NL = N > GROUPS ? N / GROUPS : 1;        // lines per group
-	double* re = S.re;
-	double* im = S.im;
-	double* twr = S.twr;
-	double* twi = S.twi;
-	double* s2 = S.s2;
-	const int tid = (int) threadIdx.x, lane = tid & 63;
-	const int k = tid & (N - 1), rk = bit_reverse(k, L), group = tid >> L;
-	const bool active = group * NL < N;                     // groups beyond the lines idle
-	if (tid < N / 2) {
-		double sn, cs;
-		sincospi(2.0 * tid / N, &sn, &cs);
-		twr[tid] = cs;
-		twi[tid] = sn;
-	}
-	if (tid < N) {
-		double sn, cs;
-		sincospi((double) tid / N, &sn, &cs);
-		s2[tid] = sn * sn;
-	}
-	double xr[NL], xi[NL];
-	if (active) {
-#pragma unroll
-		for (int j = 0; j < NL; ++j) { xr[j] = (double) in[(group * NL + j) * N + rk]; xi[j] = 0.0; }
-	}
-	__syncthreads();
-	double wr[L], wi[L];
-#pragma unroll
-	for (int st = 0; st < L; ++st) {
-		const int t = (k & ((1 << st) - 1)) << (L - 1 - st);
-		wr[st] = twr[t];
-		wi[st] = twi[t];
-	}
-	// rows: forward transforms along x
-	if (active) {
-		fft_lanes<L, NL>(xr, xi, -1.0, wr, wi);
-#pragma unroll
-		for (int j = 0; j < NL; ++j) {
-			re[(group * NL + j) * FFT_P + k] = xr[j];
-			im[(group * NL + j) * FFT_P + k] = xi[j];
-		}
-	}
-	__syncthreads();
-	// columns: forward along y, the division by lambda, inverse along y
-	if (active) {
-		const double n2 = (double) N * (double) N;
-#pragma unroll
-		for (int j = 0; j < NL; ++j) {
-			xr[j] = re[rk * FFT_P + group * NL + j];
-			xi[j] = im[rk * FFT_P + group * NL + j];
-		}
-		fft_lanes<L, NL>(xr, xi, -1.0, wr, wi);
-		const int src = (lane & ~(N - 1)) | rk;      // the inverse takes bit-reversed input
-#pragma unroll
-		for (int j = 0; j < NL; ++j) {
-			const int col = group * NL + j;
-			const double f = (col == 0 && k == 0) ? 0.0 : 1.0 / (-4.0 * n2 * (s2[col] + s2[k]));
-			xr[j] = __shfl(xr[j] * f, src, 64);
-			xi[j] = __shfl(xi[j] * f, src, 64);
-		}
-		fft_lanes<L, NL>(xr, xi, 1.0, wr, wi);
-#pragma unroll
-		for (int j = 0; j < NL; ++j) {
-			re[k * FFT_P + group * NL + j] = xr[j];
-			im[k * FFT_P + group * NL + j] = xi[j];
-		}
-	}
-	__syncthreads();
-	// rows: inverse along x, real part, 1 / N^2
-	if (active) {
-#pragma unroll
-		for (int j = 0; j < NL; ++j) {
-			xr[j] = re[(group * NL + j) * FFT_P + rk];
-			xi[j] = im[(group * NL + j) * FFT_P + rk];
-		}
-		fft_lanes<L, NL>(xr, xi, 1.0, wr, wi);
-		const double inv = 1.0 / ((double) N * (double) N);
-#pragma unroll
-		for (int j = 0; j < NL; ++j) out[(group * NL + j) * N + k] = (TI) (xr[j] * inv);
-	}
-}
-
-template<typename TI>
-__global__ __launch_bounds__(CTHREADS) void k_coarse_exact(int L, const TI* __restrict__ in, TI* __restrict__ out,
-                                                           const int* __restrict__ done)
-{
-	__shared__ FftLDS S;
-	if (done && *done) return;
-	switch (L) {
-	case 3: coarse_exact<TI, 3>(in, out, S); break;
-	case 4: coarse_exact<TI, 4>(in, out, S); break;
-	case 5: coarse_exact<TI, 5>(in, out, S); break;
-	default: coarse_exact<TI, 6>(in, out, S); break;
-	}
-}
-
 // mode 0: recurse(Lt, in) -> out;  1: nested(Lt, in); recurse -> out;
 // mode 2: nested(Lt, in) -> out (the approximate solution itself).
 template<typename TI>
@@ -1755,14 +1568,6 @@ static int coarse_launch(Ctx* c, int mode, int Lt, const T* in, T* out, int nu1,
 	return 0;
 }
 
-template<typename T>
-static int coarse_exact_launch(Ctx* c, int Lt, const T* in, T* out, const int* done)
-{
-	hipLaunchKernelGGL((k_coarse_exact<T>), dim3(1), dim3(CTHREADS), 0, c->stream, Lt, in, out, done);
-	AFM_LAUNCH_CHECK();
-	return 0;
-}
-
 // The outer update fused into the finest post-smoothing (mg_solve).
 template<typename T>
 struct UpdateArgs {
@@ -1784,8 +1589,6 @@ struct UpdateArgs {
 };
 
 // recurse(l, B_l) -> X_l (final in Xb); rhs of level l given explicitly.
-// exact: the levels of the LDS kernel are one exact solve (k_coarse_exact;
-// the CG forms' preconditioner) instead of the V-cycle's recursion.
 // With up (and l above the LDS levels), the last post-smoothing instead
 // produces the outer update and residual of mg_solve.  Within a solve only
 // that launch (and its finaliser) consult the stop flag: the other launches
@@ -1793,13 +1596,10 @@ struct UpdateArgs {
 // harmlessly instead of waiting on the flag at the start of every launch.
 template<typename T>
 static int vcycle(Ctx* c, unsigned l, const T* rhs_l, int nu1, int nu2, const int* done,
-                  const UpdateArgs<T>* up = nullptr, bool exact = false)
+                  const UpdateArgs<T>* up = nullptr)
 {
 	const unsigned lc = c->lc;
-	if (l <= lc) {
-		if (exact) return coarse_exact_launch<T>(c, (int) l, rhs_l, lvX<T>(c->lv[l], 1), done);
-		return coarse_launch<T>(c, 0, (int) l, rhs_l, lvX<T>(c->lv[l], 1), nu1, nu2, done);
-	}
+	if (l <= lc) return coarse_launch<T>(c, 0, (int) l, rhs_l, lvX<T>(c->lv[l], 1), nu1, nu2, done);
 	if (up) done = nullptr;
 	for (unsigned k = l; k > lc; --k) {
 		const T* bk = (k == l) ? rhs_l : lvB<T>(c->lv[k]);
@@ -1807,8 +1607,7 @@ static int vcycle(Ctx* c, unsigned l, const T* rhs_l, int nu1, int nu2, const in
 		AFM_TRY(smooth_launch<T>(c, k, M_ZERO, nu1, nullptr, bk, nullptr, lvX<T>(c->lv[k], 0), done,
 		                         lvB<T>(c->lv[k - 1]), (k == l && up) ? up->fold : nullptr));
 	}
-	if (exact) AFM_TRY(coarse_exact_launch<T>(c, (int) lc, lvB<T>(c->lv[lc]), lvX<T>(c->lv[lc], 1), done));
-	else AFM_TRY(coarse_launch<T>(c, 0, (int) lc, lvB<T>(c->lv[lc]), lvX<T>(c->lv[lc], 1), nu1, nu2, done));
+	AFM_TRY(coarse_launch<T>(c, 0, (int) lc, lvB<T>(c->lv[lc]), lvX<T>(c->lv[lc], 1), nu1, nu2, done));
 	for (unsigned k = lc + 1; k <= l; ++k) {
 		const T* bk = (k == l) ? rhs_l : lvB<T>(c->lv[k]);
 		if (k == l && up) {
@@ -2035,9 +1834,9 @@ static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 				up.z = z;
 				up.fpart = c->fpartials;
 				up.fstride = fstride;
-				AFM_TRY(vcycle<T>(c, Lt, rk, nu1, nu2, done, &up, true));
+				AFM_TRY(vcycle<T>(c, Lt, rk, nu1, nu2, done, &up));
 			} else {
-				AFM_TRY(vcycle<T>(c, Lt, rk, nu1, nu2, done, nullptr, true));
+				AFM_TRY(vcycle<T>(c, Lt, rk, nu1, nu2, done));
 				hipLaunchKernelGGL((k_fcg_dots<T>), dim3(rb), dim3(256), 0, c->stream, z, rk, rkm1, nx, ny, off, dg,
 				                   c->fpartials);
 			}

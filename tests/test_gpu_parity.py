@@ -225,8 +225,8 @@ def test_mixed_precision_solve(ctx_of, oracle, dev, n, precision):
     x64, xm = H(a), H(b)
     r = (rhs - rhs.mean()).ravel() - oracle.stencil_apply(xm, n, 1, n, n)
     true_rms = np.sqrt(np.mean(r * r))
-    # (the two sums differ in order; a residual at rounding level, as the
-    # exact coarse solve leaves at n = 32, differs by rounding only)
+    # (the two sums differ in order; a residual at rounding level differs by
+    # rounding only)
     assert abs(true_rms - rm) <= 1e-6 * rm + 1e-12 * rel, (true_rms, rm)
     x64 -= x64.mean(); xm -= xm.mean()
     assert np.max(np.abs(xm - x64)) <= 1e-8 * np.max(np.abs(x64))
@@ -236,12 +236,11 @@ def test_mixed_precision_solve(ctx_of, oracle, dev, n, precision):
         assert cm <= c64 + 3
 
 
-@pytest.mark.parametrize("n", [32, 64, 256])
+@pytest.mark.parametrize("n", [32, 256])
 def test_cg_forms_edge_cases(ctx_of, dev, n):
-    """The CG forms: bitwise run-to-run identical; at n <= 64 the exact
-    coarse solve makes the preconditioner the inverse; a right-hand side
-    that is constant (zero after mean removal) gives x = 0 after no cycle; a
-    cycle limit below the need is reported as non-convergence."""
+    """The CG forms: bitwise run-to-run identical; a right-hand side that is
+    constant (zero after mean removal) gives x = 0 after no cycle; a cycle
+    limit below the need is reported as non-convergence."""
     ctx = ctx_of(n)
     rhs = W.fourier_rhs(n, 64, min(16, n // 4), seed=5 * n)
     outs = []
@@ -250,20 +249,11 @@ def test_cg_forms_edge_cases(ctx_of, dev, n):
         cyc, _ = ctx.mg_solve(x, tol=1e-10, tol_mode=1, precision=1)
         outs.append((cyc, H(x)))
     assert outs[0][0] == outs[1][0] and np.array_equal(outs[0][1], outs[1][1])
-    if n <= 64:
-        # the whole grid is the coarse kernel's: with FP64 cycles the
-        # preconditioner is the exact inverse (FFT solve) and one iteration
-        # meets the tolerance; FP32 rounding of its input and output leaves
-        # about 1e-5 of the residual per iteration
-        x = T(rhs, dev)
-        cyc3, _ = ctx.mg_solve(x, tol=1e-10, tol_mode=1, precision=3)
-        assert cyc3 == 1, cyc3
-        assert outs[0][0] <= 3, outs[0][0]
     x = T(np.full((n, n), 2.5), dev)
     cyc, rms = ctx.mg_solve(x, tol=1e-10, tol_mode=0, precision=1)
     assert cyc == 0 and rms == 0.0 and not H(x).any()
     with pytest.raises(afm_error()):
-        ctx.mg_solve(T(rhs, dev), tol=1e-14, tol_mode=1, precision=1, max_cycles=1)
+        ctx.mg_solve(T(rhs, dev), tol=1e-10, tol_mode=1, precision=1, max_cycles=2)
     # the context still solves after the failed call
     x = T(rhs, dev)
     cyc, _ = ctx.mg_solve(x, tol=1e-10, tol_mode=1, precision=1)
