@@ -46,6 +46,9 @@ SIGNATURES = {
     "afm_mg_prolong": (_I, [_P, _U, _P, _P]),
     "afm_mg_vcycle": (_I, [_P, _P, _I, _I]),
     "afm_mg_solve": (_I, [_P, _P, _D, _I, _I, _I, _I, _I, _IP, _DP]),
+    "afm_laplacian3_apply": (_I, [_P, _U, _P, _P]),
+    "afm_mg3_vcycle": (_I, [_P, _U, _P, _I, _I]),
+    "afm_mg3_solve": (_I, [_P, _U, _P, _D, _I, _I, _I, _I, _IP, _DP]),
     "afm_helmholtz_apply": (_I, [_P, _D, _P, _P]),
     "afm_helmholtz_cg": (_I, [_P, _D, _P, _P, _D, _I, _I, _IP, _DP]),
     "afm_div": (_I, [_P, _P, _P, _P]),

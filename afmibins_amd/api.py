@@ -162,6 +162,36 @@ class Context:
                    C.byref(rms))
         return cyc.value, rms.value
 
+    # -- the 3-D mode: n^3 cells, walls in y (shape (n, n, n) as (z, y, x)) --
+    def _f3(self, t, name):
+        if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or t.device != self.device \
+                or not t.is_contiguous() or t.dim() != 3 or len(set(t.shape)) != 1:
+            raise TypeError(f"{name} must be a contiguous float64 (n, n, n) tensor on {self.device}")
+        n = t.shape[0]
+        if n < 2 or n & (n - 1):
+            raise ValueError(f"{name}: n = {n} is not a power of two >= 2")
+        return n.bit_length() - 1
+
+    def laplacian3(self, x):
+        L = self._f3(x, "x")
+        y = torch.empty_like(x)
+        self._call("afm_laplacian3_apply", L, _ptr(x), _ptr(y))
+        return y
+
+    def mg3_vcycle(self, rhs, nu1=2, nu2=2):
+        """One V-cycle recurse(levels, rhs) of the 3-D solver, in place."""
+        L = self._f3(rhs, "rhs")
+        self._call("afm_mg3_vcycle", L, _ptr(rhs), nu1, nu2)
+        return rhs
+
+    def mg3_solve(self, rhs, tol=1e-10, tol_mode=1, nu1=2, nu2=2, max_cycles=200):
+        """In place: rhs <- solution of the 3-D walled problem.  Returns
+        (cycles, final RMS residual)."""
+        L = self._f3(rhs, "rhs")
+        cyc, rms = C.c_int(), C.c_double()
+        self._call("afm_mg3_solve", L, _ptr(rhs), tol, tol_mode, nu1, nu2, max_cycles, C.byref(cyc), C.byref(rms))
+        return cyc.value, rms.value
+
     # -- ns operators -----------------------------------------------------
     def helmholtz(self, scale, x):
         self._f(x, (self.ny, self.nx), "x")

@@ -293,6 +293,39 @@ int afm_mg_vcycle(afm_ctx* c, double* rhs, int nu1, int nu2)
 	return mg_vcycle(c, rhs, nu1, nu2);
 }
 
+// the 3-D mode: n = 2^levels per axis, 1 <= levels <= 9 (512^3)
+#define CHECK_MG3(c, levels, nu1, nu2)                                                                  \
+	CHECK_ARG(c, "null context");                                                                       \
+	CHECK_ARG((levels) >= 1 && (levels) <= 9, "levels must be 1..9 (n = 2^levels cells per axis)");   \
+	CHECK_ARG((nu1) >= 0 && (nu1) <= 16 && (nu2) >= 0 && (nu2) <= 16, "nu1, nu2 must be 0..16")
+
+int afm_laplacian3_apply(afm_ctx* c, unsigned levels, const double* x, double* y)
+{
+	CHECK_MG3(c, levels, 0, 0);
+	CHECK_ARG(x && y && x != y, "null or aliased argument");
+	AFM_TRY(use(c));
+	return mg3_laplacian(c, levels, x, y);
+}
+
+int afm_mg3_vcycle(afm_ctx* c, unsigned levels, double* rhs, int nu1, int nu2)
+{
+	CHECK_MG3(c, levels, nu1, nu2);
+	CHECK_ARG(rhs, "null argument");
+	AFM_TRY(use(c));
+	return mg3_vcycle(c, levels, rhs, nu1, nu2);
+}
+
+int afm_mg3_solve(afm_ctx* c, unsigned levels, double* rhs, double tol, int tol_mode, int nu1, int nu2, int max_cycles,
+                  int* cycles, double* final_rms)
+{
+	CHECK_MG3(c, levels, nu1, nu2);
+	CHECK_ARG(rhs, "null argument");
+	CHECK_ARG(tol_mode == 0 || tol_mode == 1, "tol_mode must be 0 or 1");
+	CHECK_ARG(tol >= 0 && max_cycles >= 0, "tol and max_cycles must be >= 0");
+	AFM_TRY(use(c));
+	return mg3_solve(c, levels, rhs, tol, tol_mode, nu1, nu2, max_cycles, cycles, final_rms);
+}
+
 int afm_mg_solve(afm_ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, int precision, int max_cycles,
                  int* cycles, double* final_rms)
 {

@@ -157,6 +157,11 @@ int mg_smooth(Ctx* c, unsigned level, const double* x_in, const double* b, doubl
 int mg_restrict(Ctx* c, unsigned level, const double* fine, double* coarse);
 int mg_prolong(Ctx* c, unsigned level, const double* coarse, double* fine);
 int mg_vcycle(Ctx* c, double* rhs_inout, int nu1, int nu2);
+// The 3-D mode (afm_mg3.hip): n x n x n cells, n = 2^L, walls in y
+int mg3_laplacian(Ctx* c, unsigned L, const double* x, double* y);
+int mg3_vcycle(Ctx* c, unsigned L, double* rhs, int nu1, int nu2);
+int mg3_solve(Ctx* c, unsigned L, double* rhs, double tol, int tol_mode, int nu1, int nu2, int max_cycles,
+              int* cycles, double* final_rms);
 
 // Reductions / helpers (afm_ns.hip)
 int launch_sum(Ctx* c, const double* x, size_t N, double* out_host);

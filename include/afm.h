@@ -120,6 +120,23 @@ int afm_mg_vcycle(afm_ctx* c, double* rhs_inout, int nu1, int nu2);
 int afm_mg_solve(afm_ctx* c, double* rhs_inout, double tol, int tol_mode, int nu1, int nu2, int precision,
                  int max_cycles, int* cycles, double* final_rms);
 
+/* The reference's native 3-D geometry for the multigrid solve: n x n x n
+ * cells, n = 2^levels (1 <= levels <= 9), periodic in x and z, walls
+ * (homogeneous Neumann) in y -- mg::solver<geometry<levels,1,1,1>>
+ * (mg/solver.h:35-271, dt/geometry.h), the geometry of the reference's
+ * test/mg_test3.cc.  Fields are x fastest, then y, then z.  The smoother is
+ * red-black Gauss-Seidel with the reference's row arithmetic
+ * (alg/smooth.h:35-51); restriction, prolongation, the nested start and the
+ * solve loop are the reference's.  Device pointers on the context's device
+ * and stream; level scratch is allocated per call (a parity path, not
+ * tuned).  afm_laplacian3_apply: y = L x (mg/laplacian.h:46-153); mg3_vcycle:
+ * one recurse(levels, rhs), rhs replaced by the result; mg3_solve: as
+ * afm_mg_solve's precision 0 on this geometry. */
+int afm_laplacian3_apply(afm_ctx* c, unsigned levels, const double* x, double* y);
+int afm_mg3_vcycle(afm_ctx* c, unsigned levels, double* rhs_inout, int nu1, int nu2);
+int afm_mg3_solve(afm_ctx* c, unsigned levels, double* rhs_inout, double tol, int tol_mode, int nu1, int nu2,
+                  int max_cycles, int* cycles, double* final_rms);
+
 /* (I + scale L5) x (ns/helmholtz.h:61-175 for the periodic plane) and
  * alg::conjugate_gradient on it (alg/cg.h:32-56): x is the initial guess and
  * the result.  tol_mode 0: stop when |r|_2 <= tol (the reference's test);
