@@ -162,7 +162,9 @@ unsigned cg(vec& x, const Csr& A, const vec& bin, double tol, unsigned max_iter,
 // periodic Helmholtz ns/helmholtz.h:61-175 when wall == false).  Entries are
 // emitted in the reference's order, which is ascending column index.
 // kind 0: laplacian; kind 1: helmholtz_periodic (3-D walls, ny > 1 closure);
-// kind 2: the 2-D doubly periodic Helmholtz I + s*L5 (ny == 1, no wall).
+// kind 2: the 2-D doubly periodic Helmholtz I + s*L5 (ny == 1, no wall);
+// kind 3: helmholtz_dirichlet (ns/helmholtz.h:218-319) of the v grid, called
+// with ny = the cell grid's ny - 1 (its rows are the interior y faces).
 Csr stencil_csr(unsigned nx, unsigned ny, unsigned nz, unsigned n, int kind, double scale)
 {
 	Csr A;
@@ -175,6 +177,7 @@ Csr stencil_csr(unsigned nx, unsigned ny, unsigned nz, unsigned n, int kind, dou
 		unsigned x = i % nx, y = (i % ((size_t) nx * ny)) / nx, z = i / ((size_t) nx * ny);
 		double off, diagonal;
 		if (kind == 0) { off = n * n; diagonal = -6.0 * off; }
+		else if (kind == 3) { off = scale * n * n; diagonal = 1. - scale * 6. * (double) (n * n); }
 		else { off = scale * n * n; diagonal = kind == 2 ? 1. - 4. * off : 1. - 6. * off; }
 		double left = off, right = off, top = off, bottom = off, front = off, back = off;
 		if (nx == 1) { diagonal += left + right; left = right = 0; }
@@ -188,6 +191,9 @@ Csr stencil_csr(unsigned nx, unsigned ny, unsigned nz, unsigned n, int kind, dou
 			if (ny == 1) { diagonal -= 3 * (top + bottom); top = bottom = 0; }
 			else if (y == 0) { diagonal -= 2 * bottom; top += off / 3; bottom = 0; }
 			else if (y == ny - 1) { diagonal -= 2 * top; bottom += off / 3; top = 0; }
+		} else if (kind == 3) {
+			if (y == 0) bottom = 0;          // the wall values enter through boundary()
+			if (y == ny - 1) top = 0;
 		} else {
 			top = bottom = 0;   // 2-D: the y axis does not exist
 		}
@@ -614,6 +620,215 @@ struct NS2 {
 	}
 };
 
+// ------------------------------------------- 3-D wall-bounded mode ---------
+// The reference's native geometry: nx x ny x nz cells, periodic in x and z,
+// walls in y; u, w on the nx*ny*nz x- and z-faces, v on the nx*(ny-1)*nz
+// interior y-faces, wall values in nx*nz vectors (index x + nx*z).
+
+// alg/div.h:29-58 (v has nx*(ny-1)*nz entries, top/bottom nx*nz).
+void div3(vec& d, const vec& u, const vec& v, const vec& w, const vec& top, const vec& bottom,
+          unsigned nx, unsigned ny, unsigned nz, unsigned n)
+{
+	const size_t N = (size_t) nx * ny * nz;
+	d.assign(N, 0.0);
+	for (size_t i = 0; i < N; ++i) {
+		unsigned x = i % nx, y = (i % ((size_t) nx * ny)) / nx, z = i / ((size_t) nx * ny);
+		size_t ixz = x + (size_t) nx * z;
+		double dv;
+		if (ny > 1) {
+			if (y == 0) dv = v[i - (size_t) nx * z] - bottom[ixz];
+			else if (y == ny - 1) dv = top[ixz] - v[i - (size_t) nx * (z + 1)];
+			else dv = v[i - (size_t) nx * z] - v[i - (size_t) nx * (z + 1)];
+		} else {
+			dv = top[ixz] - bottom[ixz];
+		}
+		size_t ixp = (x + 1) % nx + (size_t) nx * (y + (size_t) ny * z);
+		size_t izp = x + (size_t) nx * (y + (size_t) ny * ((z + 1) % nz));
+		d[i] = n * ((u[ixp] - u[i]) + dv + (w[izp] - w[i]));
+	}
+}
+
+// alg/grad.h:32-43.
+void grad3(vec& px, vec& py, vec& pz, const vec& p, unsigned nx, unsigned ny, unsigned nz, unsigned n)
+{
+	const size_t N = (size_t) nx * ny * nz;
+	px.assign(N, 0.0);
+	pz.assign(N, 0.0);
+	py.assign((size_t) nx * (ny - 1) * nz, 0.0);
+	for (size_t i = 0; i < N; ++i) {
+		unsigned x = i % nx, y = (i % ((size_t) nx * ny)) / nx, z = i / ((size_t) nx * ny);
+		size_t ixm = (x + nx - 1) % nx + (size_t) nx * (y + (size_t) ny * z);
+		size_t izm = x + (size_t) nx * (y + (size_t) ny * ((z + nz - 1) % nz));
+		px[i] = n * (p[i] - p[ixm]);
+		pz[i] = n * (p[i] - p[izm]);
+		if (y > 0) py[i - (size_t) nx * (z + 1)] = n * (p[i] - p[i - nx]);
+	}
+}
+
+// ns/solver.h:142-202 (the ny > 1 branch): centred products on the cell
+// grid and the edges, then three divergences; the v equation's divergence
+// runs on the (ny-1)-row grid with the wall v^2 values as top and bottom.
+void advect3(vec& uadv, vec& vadv, vec& wadv, const vec& u, const vec& v, const vec& w,
+             unsigned nx, unsigned ny, unsigned nz, unsigned n)
+{
+	const size_t N = (size_t) nx * ny * nz, Nv = (size_t) nx * (ny - 1) * nz, Nxz = (size_t) nx * nz;
+	vec uu(N), ww(N), uw(N), uv(Nv), vw(Nv), vv((size_t) nx * (ny - 2) * nz), vvt(Nxz), vvb(Nxz), zero(Nxz, 0.0);
+	for (size_t i = 0; i < N; ++i) {
+		unsigned x = i % nx, y = (i % ((size_t) nx * ny)) / nx, z = i / ((size_t) nx * ny);
+		size_t ixz = x + (size_t) nx * z, iv = i - (size_t) nx * z;
+		size_t ixm = (x + nx - 1) % nx + (size_t) nx * (y + (size_t) ny * z);
+		size_t izm = x + (size_t) nx * (y + (size_t) ny * ((z + nz - 1) % nz));
+		double q = 0.5 * (u[i] + u[ixm]);
+		uu[i] = q * q;
+		if (y == 0) {
+			q = 0.5 * (v[iv]);
+			vvb[ixz] = q * q;
+		} else if (y == ny - 1) {
+			q = 0.5 * (v[i - (size_t) nx * (z + 1)]);
+			vvt[ixz] = q * q;
+		} else {
+			q = 0.5 * (v[iv] + v[i - (size_t) nx * (z + 1)]);
+			vv[i - (size_t) nx * (2 * z + 1)] = q * q;
+		}
+		q = 0.5 * (w[i] + w[izm]);
+		ww[i] = q * q;
+		if (y < ny - 1) {
+			size_t ivxm = (x + nx - 1) % nx + (size_t) nx * (y + (size_t) (ny - 1) * z);
+			size_t ivzm = x + (size_t) nx * (y + (size_t) (ny - 1) * ((z + nz - 1) % nz));
+			q = 0.5 * (u[i + nx] + u[i]);
+			double r = 0.5 * (v[ivxm] + v[iv]);
+			uv[iv] = q * r;
+			q = 0.5 * (v[ivzm] + v[iv]);
+			r = 0.5 * (w[i + nx] + w[i]);
+			vw[iv] = q * r;
+		}
+		q = 0.5 * (u[izm] + u[i]);
+		double r = 0.5 * (w[ixm] + w[i]);
+		uw[i] = q * r;
+	}
+	div3(uadv, uu, uv, uw, zero, zero, nx, ny, nz, n);
+	div3(vadv, uv, vv, vw, vvt, vvb, nx, ny - 1, nz, n);
+	div3(wadv, uw, vw, ww, zero, zero, nx, ny, nz, n);
+}
+
+// helmholtz_periodic::boundary (ns/helmholtz.h:177-196, ny > 1) and
+// helmholtz_dirichlet::boundary (ns/helmholtz.h:321-334): the wall values
+// into the first and last rows of a right-hand side.
+vec hbound3(int kind, const vec& top, const vec& bottom, double scale, unsigned nx, unsigned ny, unsigned nz, unsigned n)
+{
+	if (kind == 1) {
+		vec b((size_t) nx * ny * nz, 0.0);
+		for (size_t i = 0; i < (size_t) nx * nz; ++i) {
+			unsigned x = i % nx, z = i / nx;
+			b[x + (size_t) nx * ny * z] = 8. / 3. * scale * n * n * bottom[i];
+			b[x + (size_t) nx * (ny - 1 + (size_t) ny * z)] = 8. / 3. * scale * n * n * top[i];
+		}
+		return b;
+	}
+	vec b((size_t) nx * (ny - 1) * nz, 0.0);
+	for (size_t i = 0; i < (size_t) nx * nz; ++i) {
+		unsigned x = i % nx, z = i / nx;
+		b[x + (size_t) nx * (ny - 1) * z] = scale * n * n * bottom[i];
+		b[x + (size_t) nx * (ny - 2 + (size_t) (ny - 1) * z)] = scale * n * n * top[i];
+	}
+	return b;
+}
+
+// projection::boundary (ns/projection.h:75-106, ny > 1), as written: the
+// loop runs over nx*ny wall points and the z-component terms read phidtx at
+// the second row (pzb2, pzt2); only cubes (ny == nz) keep it in bounds.
+void proj_boundary3(vec& ut, vec& ub, vec& wt, vec& wb, const vec& phx, const vec& phz, unsigned nx, unsigned ny)
+{
+	for (size_t i = 0; i < (size_t) nx * ny; ++i) {
+		unsigned x = i % nx, z = i / nx;
+		size_t ixzb = x + (size_t) nx * ny * z, ixzt = x + (size_t) nx * ((size_t) ny * (z + 1) - 1);
+		double pxb1 = phx[ixzb], pxb2 = phx[ixzb + nx], pxt1 = phx[ixzt], pxt2 = phx[ixzt - nx],
+		       pzb1 = phz[ixzb], pzb2 = phx[ixzb + nx], pzt1 = phz[ixzt], pzt2 = phx[ixzt - nx];
+		ut[i] += (9 * pxt1 - pxt2) / 8;
+		ub[i] += (9 * pxb1 - pxb2) / 8;
+		wt[i] += (9 * pzt1 - pzt2) / 8;
+		wb[i] += (9 * pzb1 - pzb2) / 8;
+	}
+}
+
+// ns::solver<geometry<L,1,1,1>> (ns/solver.h:37-140, ns/projection.h:60-73):
+// n^3 cells; the multigrid and the CG stop on absolute tolerances as in the
+// reference.  smoother 0 is the reference's lexicographic sweep, 1 red-black.
+struct NS3 {
+	unsigned n, levels;
+	size_t N, Nv;
+	double mu, rho, dt, tol;
+	Csr phhm, phhs, dhhm, dhhs;
+	vec *u, *v, *w, u_old, v_old, w_old, phx, phy, phz, phi;
+	bool half = false;
+	unsigned last_cycles = 0, last_cg = 0;
+	MG* mg = nullptr;
+
+	NS3(unsigned L, double mu_, double rho_, double k, double tol_, int smoother, vec* u0, vec* v0, vec* w0)
+		: n(1u << L), levels(L), mu(mu_), rho(rho_), dt(k), tol(tol_), u(u0), v(v0), w(w0)
+	{
+		N = (size_t) n * n * n;
+		Nv = (size_t) n * (n - 1) * n;
+		phhm = stencil_csr(n, n, n, n, 1, mu * k / (2 * rho));
+		phhs = stencil_csr(n, n, n, n, 1, -mu * k / (2 * rho));
+		dhhm = stencil_csr(n, n - 1, n, n, 3, mu * k / (2 * rho));
+		dhhs = stencil_csr(n, n - 1, n, n, 3, -mu * k / (2 * rho));
+		phx.assign(N, 0.0); phz.assign(N, 0.0); phi.assign(N, 0.0); phy.assign(Nv, 0.0);
+		mg = new MG(3, L, 1, 1, 1, tol, 2, 2, smoother);
+		project(*u, *v, *w);
+	}
+	~NS3() { delete mg; }
+	NS3(const NS3&) = delete;
+	NS3& operator=(const NS3&) = delete;
+	void project(vec& uu, vec& vv, vec& ww)
+	{
+		vec zero((size_t) n * n, 0.0);
+		div3(phi, uu, vv, ww, zero, zero, n, n, n, n);
+		last_cycles = mg->solve(phi, 100000, nullptr);
+		grad3(phx, phy, phz, phi, n, n, n, n);
+		for (size_t i = 0; i < N; ++i) uu[i] -= phx[i];
+		for (size_t i = 0; i < Nv; ++i) vv[i] -= phy[i];
+		for (size_t i = 0; i < N; ++i) ww[i] -= phz[i];
+	}
+	// ns/solver.h:90-140
+	void step(vec& ut, vec& ub, vec& wt, vec& wb, const vec& fx, const vec& fy, const vec& fz)
+	{
+		vec uadv, vadv, wadv, rx(N), ry(Nv), rz(N);
+		advect3(uadv, vadv, wadv, *u, *v, *w, n, n, n, n);
+		const double sm = mu * dt / (2 * rho), ss = -mu * dt / (2 * rho);
+		if (!half) {
+			u_old = *u; v_old = *v; w_old = *w;
+			proj_boundary3(ut, ub, wt, wb, phx, phz, n, n);
+			vec bu = hbound3(1, ut, ub, ss, n, n, n, n), bw = hbound3(1, wt, wb, ss, n, n, n, n);
+			for (double& e : bu) e *= -1;
+			for (double& e : bw) e *= -1;
+			for (size_t i = 0; i < N; ++i) {
+				rx[i] = u_old[i] + bu[i] + (dt / 2) * (1 / rho * fx[i] - uadv[i]);
+				rz[i] = w_old[i] + bw[i] + (dt / 2) * (1 / rho * fz[i] - wadv[i]);
+			}
+			for (size_t i = 0; i < Nv; ++i) ry[i] = v_old[i] + (dt / 2) * (1 / rho * fy[i] - vadv[i]);
+		} else {
+			proj_boundary3(ut, ub, wt, wb, phx, phz, n, n);
+			vec bu = hbound3(1, ut, ub, sm, n, n, n, n), bw = hbound3(1, wt, wb, sm, n, n, n, n);
+			vec su = hbound3(1, ut, ub, ss, n, n, n, n), sw = hbound3(1, wt, wb, ss, n, n, n, n);
+			for (size_t i = 0; i < N; ++i) { bu[i] -= su[i]; bw[i] -= sw[i]; }
+			vec hu = spmv(phhm, u_old), hv = spmv(dhhm, v_old), hw = spmv(phhm, w_old);
+			for (size_t i = 0; i < N; ++i) {
+				rx[i] = hu[i] + bu[i] + dt * (1 / rho * fx[i] - uadv[i]);
+				rz[i] = hw[i] + bw[i] + dt * (1 / rho * fz[i] - wadv[i]);
+			}
+			for (size_t i = 0; i < Nv; ++i) ry[i] = hv[i] + dt * (1 / rho * fy[i] - vadv[i]);
+		}
+		last_cg = cg(*u, phhs, rx, tol, 1000000, nullptr);
+		last_cg += cg(*v, dhhs, ry, tol, 1000000, nullptr);
+		last_cg += cg(*w, phhs, rz, tol, 1000000, nullptr);
+		project(*u, *v, *w);
+		half = !half;
+	}
+	// ns/solver.h:83: p = phhs * phidt
+	vec pressure() const { return spmv(phhs, phi); }
+};
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -933,6 +1148,82 @@ unsigned or_ns_run(unsigned nx, unsigned ny, unsigned n, double mu, double rho, 
 	std::copy(vv.begin(), vv.end(), v);
 	if (p_out) std::copy(ns.p_last.begin(), ns.p_last.end(), p_out);
 	return ns.last_cycles;
+}
+
+// --- 3-D wall-bounded mode (n^3 cells; v, py have n*(n-1)*n entries) ---
+void or_div3(unsigned nx, unsigned ny, unsigned nz, unsigned n, const double* u, const double* v, const double* w,
+             const double* top, const double* bottom, double* d)
+{
+	size_t N = (size_t) nx * ny * nz, Nv = (size_t) nx * (ny - 1) * nz, Nxz = (size_t) nx * nz;
+	vec dv, uv(u, u + N), vv(v, v + Nv), wv(w, w + N), tv(top, top + Nxz), bv(bottom, bottom + Nxz);
+	div3(dv, uv, vv, wv, tv, bv, nx, ny, nz, n);
+	std::copy(dv.begin(), dv.end(), d);
+}
+
+void or_grad3(unsigned n, const double* p, double* px, double* py, double* pz)
+{
+	size_t N = (size_t) n * n * n;
+	vec pv(p, p + N), a, b, c;
+	grad3(a, b, c, pv, n, n, n, n);
+	std::copy(a.begin(), a.end(), px);
+	std::copy(b.begin(), b.end(), py);
+	std::copy(c.begin(), c.end(), pz);
+}
+
+void or_advect3(unsigned n, const double* u, const double* v, const double* w, double* ua, double* va, double* wa)
+{
+	size_t N = (size_t) n * n * n, Nv = (size_t) n * (n - 1) * n;
+	vec uv(u, u + N), vv(v, v + Nv), wv(w, w + N), a, b, c;
+	advect3(a, b, c, uv, vv, wv, n, n, n, n);
+	std::copy(a.begin(), a.end(), ua);
+	std::copy(b.begin(), b.end(), va);
+	std::copy(c.begin(), c.end(), wa);
+}
+
+// kind 1 periodic-cell (n^3 rows), 3 Dirichlet (n*(n-1)*n rows)
+void or_hbound3(int kind, unsigned n, double scale, const double* top, const double* bottom, double* b)
+{
+	size_t Nxz = (size_t) n * n;
+	vec tv(top, top + Nxz), bv(bottom, bottom + Nxz);
+	vec r = hbound3(kind, tv, bv, scale, n, n, n, n);
+	std::copy(r.begin(), r.end(), b);
+}
+
+void or_proj_boundary3(unsigned n, const double* phx, const double* phz, double* ut, double* ub, double* wt,
+                       double* wb)
+{
+	size_t N = (size_t) n * n * n, Nxz = (size_t) n * n;
+	vec a(phx, phx + N), c(phz, phz + N), t1(ut, ut + Nxz), b1(ub, ub + Nxz), t2(wt, wt + Nxz), b2(wb, wb + Nxz);
+	proj_boundary3(t1, b1, t2, b2, a, c, n, n);
+	std::copy(t1.begin(), t1.end(), ut);
+	std::copy(b1.begin(), b1.end(), ub);
+	std::copy(t2.begin(), t2.end(), wt);
+	std::copy(b2.begin(), b2.end(), wb);
+}
+
+// ns::solver<geometry<L,1,1,1>>: construct (projects u, v, w), then `steps`
+// step() calls with fixed forces; the wall vectors are updated in place as
+// the reference's step does.  p_out (n^3) gets get_p(); stats[0] the last
+// projection's V-cycles, stats[1] the last step's CG iterations.
+void or_ns3_run(unsigned L, double mu, double rho, double dt, double tol, int smoother, double* u, double* v,
+                double* w, double* ut, double* ub, double* wt, double* wb, const double* fx, const double* fy,
+                const double* fz, int steps, double* p_out, unsigned* stats)
+{
+	unsigned n = 1u << L;
+	size_t N = (size_t) n * n * n, Nv = (size_t) n * (n - 1) * n, Nxz = (size_t) n * n;
+	vec uv(u, u + N), vv(v, v + Nv), wv(w, w + N), fxv(fx, fx + N), fyv(fy, fy + Nv), fzv(fz, fz + N);
+	vec t1(ut, ut + Nxz), b1(ub, ub + Nxz), t2(wt, wt + Nxz), b2(wb, wb + Nxz);
+	NS3 ns(L, mu, rho, dt, tol, smoother, &uv, &vv, &wv);
+	for (int s = 0; s < steps; ++s) ns.step(t1, b1, t2, b2, fxv, fyv, fzv);
+	std::copy(uv.begin(), uv.end(), u);
+	std::copy(vv.begin(), vv.end(), v);
+	std::copy(wv.begin(), wv.end(), w);
+	std::copy(t1.begin(), t1.end(), ut);
+	std::copy(b1.begin(), b1.end(), ub);
+	std::copy(t2.begin(), t2.end(), wt);
+	std::copy(b2.begin(), b2.end(), wb);
+	if (p_out) { vec p = ns.pressure(); std::copy(p.begin(), p.end(), p_out); }
+	if (stats) { stats[0] = ns.last_cycles; stats[1] = ns.last_cg; }
 }
 
 }  // extern "C"

@@ -80,6 +80,12 @@ def lib():
             "or_ib_grad": (_D, [_U, _U, _U, _D, _D, _D, _D, _I, _I, _I, _P, _P, _L, _P, _P, _P, _P, _L, _I, _P,
                                 _P]),
             "or_ns_run": (_U, [_U, _U, _U, _D, _D, _D, _D, _I, _I, _P, _P, _P, _P, _I, _P]),
+            "or_div3": (None, [_U, _U, _U, _U, _P, _P, _P, _P, _P, _P]),
+            "or_grad3": (None, [_U, _P, _P, _P, _P]),
+            "or_advect3": (None, [_U, _P, _P, _P, _P, _P, _P]),
+            "or_hbound3": (None, [_I, _U, _D, _P, _P, _P]),
+            "or_proj_boundary3": (None, [_U, _P, _P, _P, _P, _P, _P]),
+            "or_ns3_run": (None, [_U, _D, _D, _D, _D, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -111,6 +117,7 @@ def ref():
             "ref_prolong": (_I, [_I, _P, _P]),
             "ref_mg_solve": (_L, [_I, _D, _U, _U, _P]),
             "ref_div_grad": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P]),
+            "ref_ns3_run": (_I, [_I, _D, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(R, name)
@@ -287,6 +294,66 @@ def ns_run(u, v, fx, fy, n, mu, rho, dt, tol, tol_mode=1, smoother=1, calls=2):
     cyc = lib().or_ns_run(nx, ny, n, mu, rho, dt, tol, tol_mode, smoother, _ptr(u), _ptr(v), _ptr(fx), _ptr(fy),
                           calls, _ptr(p))
     return u, v, p, cyc
+
+
+# ---- 3-D wall-bounded mode: n^3 cells, arrays indexed [z, y, x]; u, w, p
+# (n, n, n), v (n, n-1, n), wall vectors (n, n) as [z, x] ----
+def div3(u, v, w, top=None, bottom=None):
+    u, v, w = _f64(u), _f64(v), _f64(w)
+    nz, ny, nx = u.shape
+    top = np.zeros((nz, nx)) if top is None else _f64(top)
+    bottom = np.zeros((nz, nx)) if bottom is None else _f64(bottom)
+    d = np.empty_like(u)
+    lib().or_div3(nx, ny, nz, nx, _ptr(u), _ptr(v), _ptr(w), _ptr(top), _ptr(bottom), _ptr(d))
+    return d
+
+
+def grad3(p):
+    p = _f64(p)
+    n = p.shape[0]
+    px, pz = np.empty_like(p), np.empty_like(p)
+    py = np.empty((n, n - 1, n))
+    lib().or_grad3(n, _ptr(p), _ptr(px), _ptr(py), _ptr(pz))
+    return px, py, pz
+
+
+def advect3(u, v, w):
+    u, v, w = _f64(u), _f64(v), _f64(w)
+    n = u.shape[0]
+    a, c = np.empty_like(u), np.empty_like(u)
+    b = np.empty((n, n - 1, n))
+    lib().or_advect3(n, _ptr(u), _ptr(v), _ptr(w), _ptr(a), _ptr(b), _ptr(c))
+    return a, b, c
+
+
+def hbound3(kind, n, scale, top, bottom):
+    """helmholtz_periodic (kind 1) / helmholtz_dirichlet (kind 3) boundary()."""
+    top, bottom = _f64(top), _f64(bottom)
+    b = np.empty((n, n, n) if kind == 1 else (n, n - 1, n))
+    lib().or_hbound3(kind, n, scale, _ptr(top), _ptr(bottom), _ptr(b))
+    return b
+
+
+def proj_boundary3(phx, phz, ut, ub, wt, wb):
+    phx, phz = _f64(phx), _f64(phz)
+    ut, ub, wt, wb = (_f64(a).copy() for a in (ut, ub, wt, wb))
+    lib().or_proj_boundary3(phx.shape[0], _ptr(phx), _ptr(phz), _ptr(ut), _ptr(ub), _ptr(wt), _ptr(wb))
+    return ut, ub, wt, wb
+
+
+def ns3_run(L, u, v, w, walls, f, mu, rho, dt, tol, steps, smoother=1):
+    """ns::solver<geometry<L,1,1,1>>: construct over (u, v, w), `steps`
+    step() calls with forces f = (fx, fy, fz) and wall vectors
+    walls = (ut, ub, wt, wb) (updated as the reference updates them).
+    Returns (u, v, w, walls, p, (last V-cycles, last step's CG iterations))."""
+    u, v, w = (_f64(a).copy() for a in (u, v, w))
+    ut, ub, wt, wb = (_f64(a).copy() for a in walls)
+    fx, fy, fz = (_f64(a) for a in f)
+    p = np.empty_like(u)
+    st = np.zeros(2, dtype=np.uint32)
+    lib().or_ns3_run(L, mu, rho, dt, tol, smoother, _ptr(u), _ptr(v), _ptr(w), _ptr(ut), _ptr(ub), _ptr(wt), _ptr(wb),
+                     _ptr(fx), _ptr(fy), _ptr(fz), steps, _ptr(p), st.ctypes.data_as(C.c_void_p))
+    return u, v, w, (ut, ub, wt, wb), p, (int(st[0]), int(st[1]))
 
 
 def ib_run(u, v, X, R, k, ds, n, mu, rho, dt, tol, steps, tol_mode=1, smoother=1, delta_mode=1, want_phi=False):

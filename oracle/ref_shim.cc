@@ -23,6 +23,7 @@
 #include "mg/laplacian.h"
 #include "mg/solver.h"
 #include "ns/helmholtz.h"
+#include "ns/solver.h"
 #include "ib/grid.h"
 #include "ib/eulerian_grid.h"
 #include "ib/mac_grid.h"
@@ -142,6 +143,16 @@ template<class G>
 long csr_export(int kind, double scale, long* rs, long* col, double* val)
 {
 	if (kind == 0) return -1;   // mg::laplacian keeps its arrays private: pinned through SpMV
+	if (kind == 3) {
+		if (G::ny < 2) return -1;
+		CsrView<ns::helmholtz_dirichlet<G>> A(scale);
+		constexpr unsigned rows = G::nx * (G::ny - 1) * G::nz;
+		if (rs) {
+			for (unsigned i = 0; i <= rows; ++i) rs[i] = A.row_start[i];
+			for (size_t k = 0; k < A.values.size(); ++k) { col[k] = A.column[k]; val[k] = A.values[k]; }
+		}
+		return (long) A.values.size();
+	}
 	CsrView<ns::helmholtz_periodic<G>> A(scale);
 	if (rs) {
 		for (unsigned i = 0; i <= SZ<G>; ++i) rs[i] = A.row_start[i];
@@ -164,6 +175,27 @@ void div_grad(const double* u, const double* v, const double* w, double* d,
 	alg::grad<nx, ny, nz, n>(gx, gy, gz, pp);
 	for (unsigned i = 0; i < nx * ny * nz; ++i) { d[i] = dd[i]; px[i] = gx[i]; pz[i] = gz[i]; }
 	for (unsigned i = 0; i < nx * (ny - 1) * nz; ++i) py[i] = gy[i];
+}
+
+// ns::solver<G> (ns/solver.h:37-140): construct over u, v, w (projects
+// them), run `steps` step() calls with fixed forces; wall vectors in/out.
+template<class G>
+void ns_run(double mu, double rho, double dt, double tol, double* u, double* v, double* w, double* ut, double* ub,
+            double* wt, double* wb, const double* fx, const double* fy, const double* fz, int steps, double* p)
+{
+	constexpr unsigned nx = G::nx, ny = G::ny, nz = G::nz, N = nx * ny * nz, Nv = nx * (ny - 1) * nz, Nxz = nx * nz;
+	dt::vector<N> uu, ww, ffx, ffz;
+	dt::vector<Nv> vv, ffy;
+	dt::vector<Nxz> t1, b1, t2, b2;
+	for (unsigned i = 0; i < N; ++i) { uu[i] = u[i]; ww[i] = w[i]; ffx[i] = fx[i]; ffz[i] = fz[i]; }
+	for (unsigned i = 0; i < Nv; ++i) { vv[i] = v[i]; ffy[i] = fy[i]; }
+	for (unsigned i = 0; i < Nxz; ++i) { t1[i] = ut[i]; b1[i] = ub[i]; t2[i] = wt[i]; b2[i] = wb[i]; }
+	ns::solver<G> s(mu, rho, dt, tol, uu, vv, ww);
+	for (int k = 0; k < steps; ++k) s.step(t1, b1, t2, b2, ffx, ffy, ffz);
+	for (unsigned i = 0; i < N; ++i) { u[i] = uu[i]; w[i] = ww[i]; }
+	for (unsigned i = 0; i < Nv; ++i) v[i] = vv[i];
+	for (unsigned i = 0; i < Nxz; ++i) { ut[i] = t1[i]; ub[i] = b1[i]; wt[i] = t2[i]; wb[i] = b2[i]; }
+	if (p) { auto pp = s.get_p(); for (unsigned i = 0; i < N; ++i) p[i] = pp[i]; }
 }
 
 }  // namespace
@@ -248,7 +280,8 @@ long ref_cg(int geo, int kind, double scale, double* x, const double* b, double 
 	return -1;
 }
 
-// CSR arrays of ns::helmholtz_periodic (kind 1; kind 0 is not exported).
+// CSR arrays of ns::helmholtz_periodic (kind 1) and ns::helmholtz_dirichlet
+// (kind 3, nx*(ny-1)*nz rows); kind 0 is not exported.
 // With rs == NULL only the entry count is returned.
 long ref_csr(int geo, int kind, double scale, long* rs, long* col, double* val)
 {
@@ -334,6 +367,18 @@ int ref_div_grad(int geo, const double* u, const double* v, const double* w, dou
 	case 0: div_grad<G16>(u, v, w, d, p, px, py, pz); return 0;
 	case 10: div_grad<P16>(u, v, w, d, p, px, py, pz); return 0;
 	case 11: div_grad<P32>(u, v, w, d, p, px, py, pz); return 0;
+	}
+	return -1;
+}
+
+// ns::solver steps on the 3-D geometries (0: 16^3, 1: 32^3).
+int ref_ns3_run(int geo, double mu, double rho, double dt, double tol, double* u, double* v, double* w, double* ut,
+                double* ub, double* wt, double* wb, const double* fx, const double* fy, const double* fz, int steps,
+                double* p)
+{
+	switch (geo) {
+	case 0: ns_run<G16>(mu, rho, dt, tol, u, v, w, ut, ub, wt, wb, fx, fy, fz, steps, p); return 0;
+	case 1: ns_run<G32>(mu, rho, dt, tol, u, v, w, ut, ub, wt, wb, fx, fy, fz, steps, p); return 0;
 	}
 	return -1;
 }
