@@ -7,4 +7,4 @@ The compute path is libafm.so (afmibins_amd/lib); this package is its Python
 mirror (ctypes + torch device memory).
 """
 from ._lib import AfmError, load  # noqa: F401
-from .api import Context, MGSolver, NSSolver, IBSolver, delta_host  # noqa: F401
+from .api import Context, MGSolver, NSSolver, NS3Solver, IBSolver, delta_host  # noqa: F401

@@ -49,6 +49,16 @@ SIGNATURES = {
     "afm_laplacian3_apply": (_I, [_P, _U, _P, _P]),
     "afm_mg3_vcycle": (_I, [_P, _U, _P, _I, _I]),
     "afm_mg3_solve": (_I, [_P, _U, _P, _D, _I, _I, _I, _I, _IP, _DP]),
+    "afm_ns3_create": (_I, [_P, _U, _D, _D, _D, _D, _P, _P, _P, C.POINTER(_P)]),
+    "afm_ns3_step": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "afm_ns3_get_p": (_I, [_P, _P]),
+    "afm_ns3_stats": (_I, [_P, _IP, _IP, _IP]),
+    "afm_ns3_destroy": (_I, [_P]),
+    "afm_div3": (_I, [_P, _U, _P, _P, _P, _P, _P, _P]),
+    "afm_grad3": (_I, [_P, _U, _P, _P, _P, _P]),
+    "afm_advect3": (_I, [_P, _U, _P, _P, _P, _P, _P, _P]),
+    "afm_helmholtz3_apply": (_I, [_P, _U, _I, _D, _P, _P]),
+    "afm_helmholtz3_cg": (_I, [_P, _U, _I, _D, _P, _P, _D, _I, _IP, _DP]),
     "afm_helmholtz_apply": (_I, [_P, _D, _P, _P]),
     "afm_helmholtz_cg": (_I, [_P, _D, _P, _P, _D, _I, _I, _IP, _DP]),
     "afm_div": (_I, [_P, _P, _P, _P]),

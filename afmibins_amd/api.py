@@ -192,6 +192,70 @@ class Context:
         self._call("afm_mg3_solve", L, _ptr(rhs), tol, tol_mode, nu1, nu2, max_cycles, C.byref(cyc), C.byref(rms))
         return cyc.value, rms.value
 
+    # -- 3-D wall-bounded Navier-Stokes operators: u, w, p (n, n, n), v
+    # (n, n-1, n) on the interior y-faces, wall vectors (n, n), all [z, y, x]
+    def _v3(self, t, n, name):
+        return self._f(t, (n, n - 1, n), name)
+
+    def div3(self, u, v, w, top=None, bottom=None):
+        L = self._f3(u, "u")
+        n = 1 << L
+        self._v3(v, n, "v")
+        self._f(w, (n, n, n), "w")
+        z = None
+        if top is None or bottom is None:
+            z = torch.zeros((n, n), dtype=torch.float64, device=self.device)
+        top = z if top is None else self._f(top, (n, n), "top")
+        bottom = z if bottom is None else self._f(bottom, (n, n), "bottom")
+        d = torch.empty_like(u)
+        self._call("afm_div3", L, _ptr(u), _ptr(v), _ptr(w), _ptr(top), _ptr(bottom), _ptr(d))
+        return d
+
+    def grad3(self, p):
+        L = self._f3(p, "p")
+        n = 1 << L
+        px, pz = torch.empty_like(p), torch.empty_like(p)
+        py = torch.empty((n, n - 1, n), dtype=torch.float64, device=self.device)
+        self._call("afm_grad3", L, _ptr(p), _ptr(px), _ptr(py), _ptr(pz))
+        return px, py, pz
+
+    def advect3(self, u, v, w):
+        L = self._f3(u, "u")
+        n = 1 << L
+        self._v3(v, n, "v")
+        self._f(w, (n, n, n), "w")
+        a, c = torch.empty_like(u), torch.empty_like(u)
+        b = torch.empty_like(v)
+        self._call("afm_advect3", L, _ptr(u), _ptr(v), _ptr(w), _ptr(a), _ptr(b), _ptr(c))
+        return a, b, c
+
+    def _h3(self, kind, x, name):
+        if kind == 1:
+            return self._f3(x, name)
+        if kind != 3:
+            raise ValueError("kind must be 1 (helmholtz_periodic) or 3 (helmholtz_dirichlet)")
+        self._f(x, None, name)
+        if x.dim() != 3 or x.shape[0] != x.shape[2] or x.shape[1] != x.shape[0] - 1:
+            raise TypeError(f"{name} must have shape (n, n-1, n)")
+        n = x.shape[0]
+        if n < 2 or n & (n - 1):
+            raise ValueError(f"{name}: n = {n} is not a power of two >= 2")
+        return n.bit_length() - 1
+
+    def helmholtz3(self, kind, scale, x):
+        L = self._h3(kind, x, "x")
+        y = torch.empty_like(x)
+        self._call("afm_helmholtz3_apply", L, kind, scale, _ptr(x), _ptr(y))
+        return y
+
+    def helmholtz3_cg(self, kind, scale, b, x, tol=1e-10, max_iters=100000):
+        """In place on x (the initial guess); returns (iterations, |r|_2)."""
+        L = self._h3(kind, b, "b")
+        self._f(x, tuple(b.shape), "x")
+        it, res = C.c_int(), C.c_double()
+        self._call("afm_helmholtz3_cg", L, kind, scale, _ptr(b), _ptr(x), tol, max_iters, C.byref(it), C.byref(res))
+        return it.value, res.value
+
     # -- ns operators -----------------------------------------------------
     def helmholtz(self, scale, x):
         self._f(x, (self.ny, self.nx), "x")
@@ -294,6 +358,75 @@ class NSSolver:
         if self._h:
             torch.cuda.synchronize(self.ctx.device)
             _lib.load().afm_ns_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class NS3Solver:
+    """ns::solver<geometry<L,1,1,1>>(mu, rho, dt, tol, u0, v0, w0)
+    (ns/solver.h:37-236): the reference's 3-D wall-bounded solver on n^3
+    cells, n = 2^L.  u0, w0 are (n, n, n), v0 (n, n-1, n), indexed [z, y, x];
+    they are aliased, projected at creation and advanced in place by step().
+    tol is the reference's absolute bound for the multigrid and CG solves.
+    Runs on any Context's device and stream (the 2-D grid is not used)."""
+
+    def __init__(self, ctx, mu, rho, dt, tol, u0, v0, w0):
+        self.ctx = ctx
+        L = ctx._f3(u0, "u0")
+        self.n = n = 1 << L
+        ctx._v3(v0, n, "v0")
+        ctx._f(w0, (n, n, n), "w0")
+        self.u, self.v, self.w = u0, v0, w0
+        h = C.c_void_p()
+        ctx._call("afm_ns3_create", L, mu, rho, dt, tol, _ptr(u0), _ptr(v0), _ptr(w0), C.byref(h))
+        self._h = h
+        self.mu, self.rho, self.dt, self.tol = mu, rho, dt, tol
+
+    def _lib(self, name, *args):
+        s = torch.cuda.current_stream(self.ctx.device).cuda_stream
+        if s != self.ctx._stream:
+            self.ctx._bind_stream()
+        return check(getattr(_lib.load(), name)(self._h, *args), name)
+
+    def step(self, ut, ub, wt, wb, fx, fy, fz):
+        """ns/solver.h:90-140.  ut, ub, wt, wb (n, n) receive projection::
+        boundary's correction in place, as the reference's step does."""
+        n, f = self.n, self.ctx._f
+        for t, name in ((ut, "ut"), (ub, "ub"), (wt, "wt"), (wb, "wb")):
+            f(t, (n, n), name)
+        f(fx, (n, n, n), "fx")
+        f(fy, (n, n - 1, n), "fy")
+        f(fz, (n, n, n), "fz")
+        self._lib("afm_ns3_step", _ptr(ut), _ptr(ub), _ptr(wt), _ptr(wb), _ptr(fx), _ptr(fy), _ptr(fz))
+
+    def get_u(self):
+        return self.u
+
+    def get_v(self):
+        return self.v
+
+    def get_w(self):
+        return self.w
+
+    def get_p(self):
+        p = torch.empty_like(self.u)
+        self._lib("afm_ns3_get_p", _ptr(p))
+        return p
+
+    def stats(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self._lib("afm_ns3_stats", C.byref(a), C.byref(b), C.byref(c))
+        return {"cycles": a.value, "cg": b.value, "have_half_stepped": bool(c.value)}
+
+    def close(self):
+        if self._h:
+            torch.cuda.synchronize(self.ctx.device)
+            _lib.load().afm_ns3_destroy(self._h)
             self._h = None
 
     def __del__(self):

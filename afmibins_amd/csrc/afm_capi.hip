@@ -60,6 +60,7 @@ using namespace afm;
 
 struct afm_ns { NS* s; };
 struct afm_ib { IB* b; };
+struct afm_ns3 { NS3* s; afm_ctx* c; };
 
 #define CHECK_ARG(cond, msg) do { if (!(cond)) { set_error(msg); return -1; } } while (0)
 
@@ -324,6 +325,98 @@ int afm_mg3_solve(afm_ctx* c, unsigned levels, double* rhs, double tol, int tol_
 	CHECK_ARG(tol >= 0 && max_cycles >= 0, "tol and max_cycles must be >= 0");
 	AFM_TRY(use(c));
 	return mg3_solve(c, levels, rhs, tol, tol_mode, nu1, nu2, max_cycles, cycles, final_rms);
+}
+
+#define CHECK_NS3_LEVELS(levels) CHECK_ARG((levels) >= 1 && (levels) <= 8, "levels must be 1..8 (n = 2^levels)")
+
+int afm_ns3_create(afm_ctx* c, unsigned levels, double mu, double rho, double dt, double tol, double* u0, double* v0,
+                   double* w0, afm_ns3** out)
+{
+	CHECK_ARG(c && u0 && v0 && w0 && out, "null argument");
+	CHECK_NS3_LEVELS(levels);
+	CHECK_ARG(rho > 0 && dt > 0 && mu >= 0 && tol > 0, "need rho > 0, dt > 0, mu >= 0, tol > 0");
+	AFM_TRY(use(c));
+	NS3* s = nullptr;
+	AFM_TRY(ns3_create(c, levels, mu, rho, dt, tol, u0, v0, w0, &s));
+	*out = new afm_ns3{s, c};
+	return 0;
+}
+
+int afm_ns3_step(afm_ns3* s, double* ut, double* ub, double* wt, double* wb, const double* fx, const double* fy,
+                 const double* fz)
+{
+	CHECK_ARG(s && ut && ub && wt && wb && fx && fy && fz, "null argument");
+	AFM_TRY(use(s->c));
+	return ns3_step(s->s, ut, ub, wt, wb, fx, fy, fz);
+}
+
+int afm_ns3_get_p(afm_ns3* s, double* p)
+{
+	CHECK_ARG(s && p, "null argument");
+	AFM_TRY(use(s->c));
+	return ns3_get_p(s->s, p);
+}
+
+int afm_ns3_stats(afm_ns3* s, int* last_cycles, int* last_cg, int* have_half_stepped)
+{
+	CHECK_ARG(s, "null argument");
+	ns3_stats(s->s, last_cycles, last_cg, have_half_stepped);
+	return 0;
+}
+
+int afm_ns3_destroy(afm_ns3* s)
+{
+	if (!s) return 0;
+	(void) use(s->c);
+	ns3_destroy(s->s);
+	delete s;
+	return 0;
+}
+
+int afm_div3(afm_ctx* c, unsigned levels, const double* u, const double* v, const double* w, const double* top,
+             const double* bottom, double* d)
+{
+	CHECK_ARG(c && u && v && w && top && bottom && d, "null argument");
+	CHECK_NS3_LEVELS(levels);
+	AFM_TRY(use(c));
+	return ns3_div(c, levels, u, v, w, top, bottom, d);
+}
+
+int afm_grad3(afm_ctx* c, unsigned levels, const double* p, double* px, double* py, double* pz)
+{
+	CHECK_ARG(c && p && px && py && pz, "null argument");
+	CHECK_NS3_LEVELS(levels);
+	AFM_TRY(use(c));
+	return ns3_grad(c, levels, p, px, py, pz);
+}
+
+int afm_advect3(afm_ctx* c, unsigned levels, const double* u, const double* v, const double* w, double* u_adv,
+                double* v_adv, double* w_adv)
+{
+	CHECK_ARG(c && u && v && w && u_adv && v_adv && w_adv, "null argument");
+	CHECK_NS3_LEVELS(levels);
+	AFM_TRY(use(c));
+	return ns3_advect(c, levels, u, v, w, u_adv, v_adv, w_adv);
+}
+
+int afm_helmholtz3_apply(afm_ctx* c, unsigned levels, int kind, double scale, const double* x, double* y)
+{
+	CHECK_ARG(c && x && y && x != y, "null or aliased argument");
+	CHECK_NS3_LEVELS(levels);
+	CHECK_ARG(kind == 1 || kind == 3, "kind must be 1 (helmholtz_periodic) or 3 (helmholtz_dirichlet)");
+	AFM_TRY(use(c));
+	return ns3_helmholtz_apply(c, levels, kind, scale, x, y);
+}
+
+int afm_helmholtz3_cg(afm_ctx* c, unsigned levels, int kind, double scale, const double* b, double* x, double tol,
+                      int max_iters, int* iters, double* final_norm)
+{
+	CHECK_ARG(c && b && x, "null argument");
+	CHECK_NS3_LEVELS(levels);
+	CHECK_ARG(kind == 1 || kind == 3, "kind must be 1 (helmholtz_periodic) or 3 (helmholtz_dirichlet)");
+	CHECK_ARG(tol >= 0 && max_iters >= 0, "tol and max_iters must be >= 0");
+	AFM_TRY(use(c));
+	return ns3_helmholtz_cg(c, levels, kind, scale, b, x, tol, max_iters, iters, final_norm);
 }
 
 int afm_mg_solve(afm_ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, int precision, int max_cycles,

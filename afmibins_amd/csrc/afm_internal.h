@@ -162,6 +162,28 @@ int mg3_laplacian(Ctx* c, unsigned L, const double* x, double* y);
 int mg3_vcycle(Ctx* c, unsigned L, double* rhs, int nu1, int nu2);
 int mg3_solve(Ctx* c, unsigned L, double* rhs, double tol, int tol_mode, int nu1, int nu2, int max_cycles,
               int* cycles, double* final_rms);
+// a persistent 3-D multigrid workspace (level buffers, iterate, residual)
+struct Mg3;
+int mg3_create(Ctx* c, unsigned L, int nu1, int nu2, Mg3** out);
+int mg3_solve_with(Mg3* m, double* rhs, double tol, int tol_mode, int max_cycles, int* cycles, double* final_rms);
+void mg3_destroy(Mg3* m);
+// the 3-D Navier-Stokes solver (afm_ns3.hip)
+struct NS3;
+int ns3_create(Ctx* c, unsigned L, double mu, double rho, double dt, double tol, double* u, double* v, double* w,
+               NS3** out);
+int ns3_step(NS3* s, double* ut, double* ub, double* wt, double* wb, const double* fx, const double* fy,
+             const double* fz);
+int ns3_get_p(NS3* s, double* p);
+void ns3_stats(const NS3* s, int* cycles, int* cg, int* half);
+void ns3_destroy(NS3* s);
+int ns3_div(Ctx* c, unsigned L, const double* u, const double* v, const double* w, const double* top,
+            const double* bottom, double* d);
+int ns3_grad(Ctx* c, unsigned L, const double* p, double* px, double* py, double* pz);
+int ns3_advect(Ctx* c, unsigned L, const double* u, const double* v, const double* w, double* ua, double* va,
+               double* wa);
+int ns3_helmholtz_apply(Ctx* c, unsigned L, int kind, double scale, const double* x, double* y);
+int ns3_helmholtz_cg(Ctx* c, unsigned L, int kind, double scale, const double* b, double* x, double tol,
+                     int max_iters, int* iters, double* final_norm);
 
 // Reductions / helpers (afm_ns.hip)
 int launch_sum(Ctx* c, const double* x, size_t N, double* out_host);

@@ -263,6 +263,15 @@ struct Mg3Ctx {
 	double* scal;
 };
 
+// The workspace of mg3_solve_with: the levels, plus the iterate, the
+// residual and a scratch copy of the right-hand side at level L, apart from
+// the level buffers that recurse3 / nested3 use.
+struct Mg3 {
+	Mg3Ctx m;
+	double *xs = nullptr, *res = nullptr, *copyb = nullptr;
+	std::vector<void*> own;
+};
+
 // sum (sq = 0) or sum of squares (sq = 1) of a device vector into *out (device)
 static int sum3(Ctx* c, const double* x, size_t N, int sq, double* out)
 {
@@ -379,26 +388,52 @@ static int read_scalar(Ctx* c, const double* d, double* h)
 	return 0;
 }
 
+int mg3_create(Ctx* c, unsigned L, int nu1, int nu2, Mg3** out)
+{
+	Mg3* g = new Mg3;
+	int rc = mg3_alloc(c, L, nu1, nu2, g->m, g->own);
+	const size_t N = cells3(L);
+	for (double** p : {&g->xs, &g->res, &g->copyb}) {
+		if (rc) break;
+		if (hipMalloc((void**) p, N * 8) != hipSuccess) { set_error("out of device memory (3-D multigrid)"); rc = -3; break; }
+		g->own.push_back(*p);
+	}
+	if (rc) { mg3_destroy(g); return rc; }
+	*out = g;
+	return 0;
+}
+
+void mg3_destroy(Mg3* g)
+{
+	if (!g) return;
+	mg3_free(g->m.c, g->own);
+	delete g;
+}
+
 // mg::solver<G>::solve (mg/solver.h:218-237): rhs replaced by the solution
 int mg3_solve(Ctx* c, unsigned L, double* rhs, double tol, int tol_mode, int nu1, int nu2, int max_cycles,
               int* cycles, double* final_rms)
 {
-	Mg3Ctx m;
-	std::vector<void*> own;
-	int rc = mg3_alloc(c, L, nu1, nu2, m, own);
-	auto fail = [&](int r) { mg3_free(c, own); return r; };
-	if (rc) return fail(rc);
+	Mg3* g = nullptr;
+	int rc = mg3_create(c, L, nu1, nu2, &g);
+	if (rc) return rc;
+	rc = mg3_solve_with(g, rhs, tol, tol_mode, max_cycles, cycles, final_rms);
+	mg3_destroy(g);
+	return rc;
+}
+
+int mg3_solve_with(Mg3* g, double* rhs, double tol, int tol_mode, int max_cycles, int* cycles, double* final_rms)
+{
+	Mg3Ctx& m = g->m;
+	Ctx* c = m.c;
+	const unsigned L = m.L;
+	auto fail = [](int r) { return r; };
+	int rc = 0;
 	const size_t N = cells3(L);
 	double* sc = m.scal;
-	// the iterate, the residual and a scratch copy of rhs, apart from the
-	// level buffers that recurse3 / nested3 use at level L
-	double* xs = nullptr;
-	double* res = nullptr;
-	double* copyb = nullptr;
-	for (double** p : {&xs, &res, &copyb}) {
-		if (hipMalloc((void**) p, N * 8) != hipSuccess) { set_error("out of device memory (3-D multigrid)"); return fail(-3); }
-		own.push_back(*p);
-	}
+	double* xs = g->xs;
+	double* res = g->res;
+	double* copyb = g->copyb;
 	double h_sq = 0.0;
 	// the relative tolerance: RMS of rhs - mean
 	if ((rc = sum3(c, rhs, N, 0, sc))) return fail(rc);

@@ -31,6 +31,7 @@ extern "C" {
 typedef struct afm_ctx afm_ctx;
 typedef struct afm_ns afm_ns;
 typedef struct afm_ib afm_ib;
+typedef struct afm_ns3 afm_ns3;
 
 /* dt::geometry<Level, Width, Height, Depth> (dt/geometry.h:29-44), chosen at
  * run time.  dim must be 2; n = 2^Level >= 8; width = depth = 1. */
@@ -136,6 +137,42 @@ int afm_laplacian3_apply(afm_ctx* c, unsigned levels, const double* x, double* y
 int afm_mg3_vcycle(afm_ctx* c, unsigned levels, double* rhs_inout, int nu1, int nu2);
 int afm_mg3_solve(afm_ctx* c, unsigned levels, double* rhs_inout, double tol, int tol_mode, int nu1, int nu2,
                   int max_cycles, int* cycles, double* final_rms);
+
+/* The reference's 3-D wall-bounded Navier-Stokes solver on the same geometry
+ * (1 <= levels <= 8): ns::solver<geometry<levels,1,1,1>> (ns/solver.h:37-236)
+ * with its projection (ns/projection.h:60-122).  u0, w0 hold n^3 x- and
+ * z-face velocities, v0 the n*(n-1)*n interior y-face velocities; they are
+ * aliased, projected at creation, and advanced in place by step(), which
+ * alternates the backward-Euler half step and the Crank-Nicolson full step.
+ * ut, ub, wt, wb are the n*n top and bottom wall values of u and w (index
+ * x + n*z); step() adds projection::boundary's correction to them in place,
+ * as the reference does to the caller's vectors.  fx, fz: n^3, fy:
+ * n*(n-1)*n.  tol is the reference's absolute bound for the multigrid (RMS)
+ * and the CG (|r|_2) solves; the multigrid smoother is red-black.  get_p:
+ * p = H_s phi (ns/solver.h:83).  Level and solver scratch is allocated at
+ * creation. */
+int afm_ns3_create(afm_ctx* c, unsigned levels, double mu, double rho, double dt, double tol, double* u0, double* v0,
+                   double* w0, afm_ns3** out);
+int afm_ns3_step(afm_ns3* s, double* ut, double* ub, double* wt, double* wb, const double* fx, const double* fy,
+                 const double* fz);
+int afm_ns3_get_p(afm_ns3* s, double* p);
+int afm_ns3_stats(afm_ns3* s, int* last_cycles, int* last_cg, int* have_half_stepped);
+int afm_ns3_destroy(afm_ns3* s);
+/* Its operators on the n^3 cube.  div3: alg::div (alg/div.h:29-58) with wall
+ * values top/bottom (n*n); grad3: alg::grad (alg/grad.h:32-43), py has
+ * n*(n-1)*n entries; advect3: ns::solver::advection (ns/solver.h:142-202);
+ * helmholtz3: kind 1 ns::helmholtz_periodic (ns/helmholtz.h:61-175, n^3
+ * rows), kind 3 ns::helmholtz_dirichlet (ns/helmholtz.h:218-319, n*(n-1)*n
+ * rows), and alg::conjugate_gradient on it (alg/cg.h:32-56; x is the
+ * initial guess and the result, stop when |r|_2 <= tol). */
+int afm_div3(afm_ctx* c, unsigned levels, const double* u, const double* v, const double* w, const double* top,
+             const double* bottom, double* d);
+int afm_grad3(afm_ctx* c, unsigned levels, const double* p, double* px, double* py, double* pz);
+int afm_advect3(afm_ctx* c, unsigned levels, const double* u, const double* v, const double* w, double* u_adv,
+                double* v_adv, double* w_adv);
+int afm_helmholtz3_apply(afm_ctx* c, unsigned levels, int kind, double scale, const double* x, double* y);
+int afm_helmholtz3_cg(afm_ctx* c, unsigned levels, int kind, double scale, const double* b, double* x, double tol,
+                      int max_iters, int* iters, double* final_norm);
 
 /* (I + scale L5) x (ns/helmholtz.h:61-175 for the periodic plane) and
  * alg::conjugate_gradient on it (alg/cg.h:32-56): x is the initial guess and
