@@ -59,6 +59,8 @@ SIGNATURES = {
     "afm_advect3": (_I, [_P, _U, _P, _P, _P, _P, _P, _P]),
     "afm_helmholtz3_apply": (_I, [_P, _U, _I, _D, _P, _P]),
     "afm_helmholtz3_cg": (_I, [_P, _U, _I, _D, _P, _P, _D, _I, _IP, _DP]),
+    "afm_interp3": (_I, [_P, _U, _I, _P, _P, _L, _P]),
+    "afm_spread3": (_I, [_P, _U, _I, _P, _P, _P, _L, _P]),
     "afm_helmholtz_apply": (_I, [_P, _D, _P, _P]),
     "afm_helmholtz_cg": (_I, [_P, _D, _P, _P, _D, _I, _I, _IP, _DP]),
     "afm_div": (_I, [_P, _P, _P, _P]),

@@ -256,6 +256,36 @@ class Context:
         self._call("afm_helmholtz3_cg", L, kind, scale, _ptr(b), _ptr(x), tol, max_iters, C.byref(it), C.byref(res))
         return it.value, res.value
 
+    def _markers3(self, X):
+        self._f(X, None, "X")
+        if X.dim() != 2 or X.shape[1] != 3:
+            raise ValueError("X must have shape (m, 3)")
+        return X.shape[0]
+
+    def interp3(self, comp, field, X):
+        """Velocity component comp (0 u, 1 v, 2 w) of the 3-D MAC grid at the
+        markers X (m, 3); field is (n, n, n), or (n, n-1, n) for v."""
+        n = field.shape[0] if isinstance(field, torch.Tensor) and field.dim() == 3 else 0
+        shape = (n, n - 1, n) if comp == 1 else (n, n, n)
+        L = self._h3(3 if comp == 1 else 1, self._f(field, shape, "field"), "field")
+        m = self._markers3(X)
+        U = torch.empty(m, dtype=torch.float64, device=self.device)
+        self._call("afm_interp3", L, comp, _ptr(field), _ptr(X), m, _ptr(U))
+        return U
+
+    def spread3(self, comp, F, ds, X, n):
+        """Forces F (m,) with source volumes ds (m,) at markers X (m, 3) onto
+        component comp of the n^3 MAC grid: (n, n, n), or (n, n-1, n) for v."""
+        m = self._markers3(X)
+        self._f(F, (m,), "F")
+        self._f(ds, (m,), "ds")
+        if n < 2 or n & (n - 1):
+            raise ValueError(f"n = {n} is not a power of two >= 2")
+        shape = (n, n - 1, n) if comp == 1 else (n, n, n)
+        f = torch.empty(shape, dtype=torch.float64, device=self.device)
+        self._call("afm_spread3", n.bit_length() - 1, comp, _ptr(F), _ptr(ds), _ptr(X), m, _ptr(f))
+        return f
+
     # -- ns operators -----------------------------------------------------
     def helmholtz(self, scale, x):
         self._f(x, (self.ny, self.nx), "x")

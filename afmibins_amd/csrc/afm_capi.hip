@@ -419,6 +419,29 @@ int afm_helmholtz3_cg(afm_ctx* c, unsigned levels, int kind, double scale, const
 	return ns3_helmholtz_cg(c, levels, kind, scale, b, x, tol, max_iters, iters, final_norm);
 }
 
+#define CHECK_IB3(c, levels, comp, m)                                                                 \
+	CHECK_ARG(c, "null context");                                                                     \
+	CHECK_NS3_LEVELS(levels);                                                                         \
+	CHECK_ARG((comp) >= 0 && (comp) <= 2, "comp must be 0 (u), 1 (v) or 2 (w)");                      \
+	CHECK_ARG((m) >= 0 && (m) <= (1L << 24), "m must be 0..2^24 markers")
+
+int afm_interp3(afm_ctx* c, unsigned levels, int comp, const double* field, const double* X, long m, double* U)
+{
+	CHECK_IB3(c, levels, comp, m);
+	CHECK_ARG(m == 0 || (field && X && U), "null argument");
+	AFM_TRY(use(c));
+	return ib3_interp(c, levels, comp, field, X, m, U);
+}
+
+int afm_spread3(afm_ctx* c, unsigned levels, int comp, const double* F, const double* ds, const double* X, long m,
+                double* f)
+{
+	CHECK_IB3(c, levels, comp, m);
+	CHECK_ARG(f && (m == 0 || (F && ds && X)), "null argument");
+	AFM_TRY(use(c));
+	return ib3_spread(c, levels, comp, F, ds, X, m, f);
+}
+
 int afm_mg_solve(afm_ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, int precision, int max_cycles,
                  int* cycles, double* final_rms)
 {

@@ -184,6 +184,9 @@ int ns3_advect(Ctx* c, unsigned L, const double* u, const double* v, const doubl
 int ns3_helmholtz_apply(Ctx* c, unsigned L, int kind, double scale, const double* x, double* y);
 int ns3_helmholtz_cg(Ctx* c, unsigned L, int kind, double scale, const double* b, double* x, double tol,
                      int max_iters, int* iters, double* final_norm);
+// the IB transfer on the 3-D MAC grid (afm_ib3.hip)
+int ib3_interp(Ctx* c, unsigned L, int comp, const double* field, const double* X, long m, double* U);
+int ib3_spread(Ctx* c, unsigned L, int comp, const double* F, const double* ds, const double* X, long m, double* f);
 
 // Reductions / helpers (afm_ns.hip)
 int launch_sum(Ctx* c, const double* x, size_t N, double* out_host);

@@ -173,6 +173,18 @@ int afm_advect3(afm_ctx* c, unsigned levels, const double* u, const double* v, c
 int afm_helmholtz3_apply(afm_ctx* c, unsigned levels, int kind, double scale, const double* x, double* y);
 int afm_helmholtz3_cg(afm_ctx* c, unsigned levels, int kind, double scale, const double* b, double* x, double tol,
                       int max_iters, int* iters, double* final_norm);
+/* ib::grid::interpolate (ib/grid.h:68-84) on the same 3-D MAC grid, for
+ * comp 0 (u), 1 (v: the mac_grid<y> window of ib/mac_grid.h:114-132 on the
+ * n*(n-1)*n interior y-faces) or 2 (w); X holds m markers as (x, y, z).
+ * interp3: U[k] = sum over the 64 taps of (w vol) field[idx], in the
+ * reference's tap order.  spread3: f = sum over markers of (w ds_k) F_k per
+ * face, summed per face in marker order (run-to-run identical, no float
+ * atomics); f is overwritten.  Taps the grid does not store -- the v
+ * window's wall faces and the u / w windows' ghost row above the top wall,
+ * which the reference indexes out of range -- read and receive nothing. */
+int afm_interp3(afm_ctx* c, unsigned levels, int comp, const double* field, const double* X, long m, double* U);
+int afm_spread3(afm_ctx* c, unsigned levels, int comp, const double* F, const double* ds, const double* X, long m,
+                double* f);
 
 /* (I + scale L5) x (ns/helmholtz.h:61-175 for the periodic plane) and
  * alg::conjugate_gradient on it (alg/cg.h:32-56): x is the initial guess and

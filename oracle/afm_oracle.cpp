@@ -485,6 +485,15 @@ void window3(int comp, const double c[3], unsigned n, unsigned W, unsigned H, un
 		p[1] = comp == 1 ? (fy + diy) / n : (fy + diy - 1.0) / n;
 		p[2] = (fz + diz - 1.0) / n;
 		if (p[1] < 0 || p[1] > H) { idx[i] = SIZE_MAX; w[i] = 0; continue; }
+		// Taps the grid does not store, which the reference indexes out of
+		// range (undefined; its own tests keep markers 3.5 cells from the
+		// walls): the v window's wall faces y = 0 and y = H (rows -1 and
+		// ny - 1 of the (ny-1)-row array), and the u / w windows' ghost row
+		// above the top wall (y = H + h/2, row ny; the test above keeps
+		// p[1] == H, while the bottom ghost row is dropped by p[1] < 0).
+		// Here such taps read nothing and receive nothing (zero wall velocity
+		// and no ghost cells).
+		if (comp == 1 ? (iy == 0 || iy == ny) : iy > ny) { idx[i] = SIZE_MAX; w[i] = 0; continue; }
 		if (comp == 1) {
 			p[0] = p[0] + 0.5 / n; p[2] = p[2] + 0.5 / n;
 			idx[i] = ix + (size_t) nx * (iy - 1 + (size_t) (ny - 1) * iz);

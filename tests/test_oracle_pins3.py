@@ -72,6 +72,36 @@ def test_ns3_steps_match_reference(oracle, ref, geo, steps):
 
 
 # ------------------------------------------------------------ properties
+def test_spread3_is_adjoint_of_interp3_up_to_the_walls(oracle):
+    """<S F, u>_h = <F, J u>_ds for all three MAC components with markers
+    anywhere, including within 2h of the walls (where the v window's wall
+    faces are skipped both ways) and of the periodic seams; momentum
+    sum f h^3 = sum F ds for markers at least 3h from walls and seams."""
+    n = 16
+    rng = np.random.default_rng(8)
+    X = rng.uniform(0, 1, (60, 3))
+    X[:10, 1] = rng.uniform(0, 2.0 / n, 10)
+    X[10:20, 1] = rng.uniform(1 - 2.0 / n, 1, 10)
+    X[30:, :] = rng.uniform(3.0 / n, 1 - 3.0 / n, (30, 3))
+    F, ds = rng.standard_normal(60), rng.uniform(0.5, 1.5, 60) / n
+    for comp in (0, 1, 2):
+        shape = (n, n - 1, n) if comp == 1 else (n, n, n)
+        u = rng.standard_normal(shape)
+        f = oracle.spread3(comp, F, ds, X, n, mode=1)
+        J = oracle.interp3(comp, u, X, n, mode=1)
+        lhs = np.sum(f * u.ravel()) / n ** 3
+        rhs = np.sum(F * ds * J)
+        assert abs(lhs - rhs) <= 1e-13 * (abs(lhs) + abs(rhs))
+        fi = oracle.spread3(comp, F[30:], ds[30:], X[30:], n, mode=1)
+        assert abs(fi.sum() / n ** 3 - np.sum(F[30:] * ds[30:])) < 1e-13
+    # a v window next to a wall loses exactly its wall-face taps: constant v
+    # interpolates to the stored-face share of the unit partition
+    Xw = np.array([[0.43, 0.2 / n, 0.61], [0.43, 0.5, 0.61], [0.43, 1 - 0.7 / n, 0.61]])
+    J = oracle.interp3(1, np.ones((n, n - 1, n)), Xw, n, mode=1)
+    assert abs(J[1] - 1.0) < 1e-12 and J[0] < 0.9 and J[2] < 0.9
+
+
+# ------------------------------------------------------- properties (NS)
 def test_dirichlet_helmholtz_is_symmetric_and_boundary_is_its_wall_column(oracle):
     """The Dirichlet rows are I + s*L7 on the interior faces with the wall
     faces eliminated: symmetric, and boundary() is the eliminated column
