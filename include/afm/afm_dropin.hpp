@@ -47,6 +47,21 @@ struct Buffer {
 	void get(double* h, size_t count) const { check(afm_copy_to_host(c, h, p, count * 8), "copy_to_host"); }
 };
 
+// the context the 3-D geometry's calls run on (their arrays are their own;
+// only the device and stream of the context are used)
+inline afm_ctx* context3()
+{
+	static std::unique_ptr<afm_ctx, int (*)(afm_ctx*)> holder(nullptr, afm_ctx_destroy);
+	if (!holder) {
+		afm_geom g{2, 8, 1, 1};
+		const char* dev = std::getenv("AFM_DEVICE");
+		afm_ctx* c = nullptr;
+		check(afm_ctx_create(&g, dev ? std::atoi(dev) : 0, &c), "afm_ctx_create");
+		holder.reset(c);
+	}
+	return holder.get();
+}
+
 template<class G>
 afm_ctx* context()
 {
@@ -70,6 +85,17 @@ struct plane {
 	static constexpr unsigned level = Level, width = Width, depth = Depth;
 	static constexpr unsigned n = 1u << Level;
 	static constexpr unsigned nx = Width * n, ny = Depth * n, size = nx * ny;
+};
+
+// dt/geometry.h:29-60: the reference's 3-D geometry, n = 2^Level cells per
+// unit length, walls in y.  mg::solver and ns::solver accept the cube
+// geometry<Level, 1, 1, 1> (the reference's own tests' geometry).
+template<unsigned Level, unsigned Width, unsigned Height, unsigned Depth>
+struct geometry {
+	static constexpr unsigned factor = 2;
+	static constexpr unsigned n = 1u << Level;
+	static constexpr unsigned nx = Width * n, ny = Height * n, nz = Depth * n, size = nx * ny * nz;
+	static constexpr unsigned width = Width, height = Height, depth = Depth, level = Level;
 };
 
 // dt/point.h:29-48, in the plane.
@@ -164,6 +190,29 @@ public:
 	}
 };
 
+// mg::solver<geometry<L,1,1,1>>: the 3-D solve with walls in y (red-black
+// smoothing, the reference's absolute RMS tolerance).
+template<unsigned L, unsigned W, unsigned H, unsigned D>
+class solver<dt::geometry<L, W, H, D>> {
+	static_assert(W == 1 && H == 1 && D == 1 && L >= 1, "the 3-D solve takes geometry<L, 1, 1, 1>, L >= 1");
+	typedef dt::geometry<L, W, H, D> G;
+	double tol;
+	unsigned nu1, nu2;
+public:
+	solver(double tol_, unsigned nu1_, unsigned nu2_) : tol(tol_), nu1(nu1_), nu2(nu2_) {}
+	unsigned solve(dt::vector<G::size>& rhs)
+	{
+		afm_ctx* c = afm_detail::context3();
+		afm_detail::Buffer b(c, G::size * 8);
+		b.put(rhs.data(), G::size);
+		int cyc = 0;
+		afm_detail::check(afm_mg3_solve(c, L, b.d(), tol, 0, (int) nu1, (int) nu2, 100000, &cyc, nullptr),
+		                  "afm_mg3_solve");
+		b.get(rhs.data(), G::size);
+		return (unsigned) cyc;
+	}
+};
+
 }  // namespace mg
 
 namespace ns {
@@ -233,6 +282,76 @@ public:
 		afm_detail::check(afm_ns_get_p(s, p.d()), "afm_ns_get_p");
 		dt::vector<G::size> out;
 		p.get(out.data(), G::size);
+		return out;
+	}
+};
+
+// ns::solver<geometry<L,1,1,1>>(mu, rho, k, tol, u0, v0, w0) (ns/solver.h:
+// 37-140): the reference's 3-D solver with walls in y.  u0, v0, w0 are
+// aliased (v0 holds the nx*(ny-1)*nz interior y-faces) and projected at
+// construction; step(ut, ub, wt, wb, fx, fy, fz) alternates the
+// backward-Euler half step and the Crank-Nicolson full step, leaves the new
+// velocity in u0, v0, w0, and adds projection::boundary's correction to the
+// wall vectors in place, as the reference does.  tol is the reference's
+// absolute bound for the multigrid and CG solves.
+template<unsigned L, unsigned W, unsigned H, unsigned D>
+class solver<dt::geometry<L, W, H, D>> {
+	static_assert(W == 1 && H == 1 && D == 1 && L >= 1, "the 3-D solver takes geometry<L, 1, 1, 1>, L >= 1");
+	typedef dt::geometry<L, W, H, D> G;
+	static constexpr unsigned nx = G::nx, ny = G::ny, nz = G::nz;
+	static constexpr unsigned N = nx * ny * nz, Nv = nx * (ny - 1) * nz, Nxz = nx * nz;
+	afm_ctx* c;
+	dt::vector<N>& u_curr;
+	dt::vector<Nv>& v_curr;
+	dt::vector<N>& w_curr;
+	afm_detail::Buffer du, dv, dw, fx_d, fy_d, fz_d, ut_d, ub_d, wt_d, wb_d;
+	afm_ns3* s = nullptr;
+public:
+	solver(double mu, double rho, double k, double tol, dt::vector<N>& u0, dt::vector<Nv>& v0, dt::vector<N>& w0)
+		: c(afm_detail::context3()), u_curr(u0), v_curr(v0), w_curr(w0), du(c, N * 8), dv(c, Nv * 8),
+		  dw(c, N * 8), fx_d(c, N * 8), fy_d(c, Nv * 8), fz_d(c, N * 8), ut_d(c, Nxz * 8), ub_d(c, Nxz * 8),
+		  wt_d(c, Nxz * 8), wb_d(c, Nxz * 8)
+	{
+		du.put(u0.data(), N);
+		dv.put(v0.data(), Nv);
+		dw.put(w0.data(), N);
+		afm_detail::check(afm_ns3_create(c, L, mu, rho, k, tol, du.d(), dv.d(), dw.d(), &s), "afm_ns3_create");
+		du.get(u0.data(), N);
+		dv.get(v0.data(), Nv);
+		dw.get(w0.data(), N);
+	}
+	~solver() { afm_ns3_destroy(s); }
+	solver(const solver&) = delete;
+	solver& operator=(const solver&) = delete;
+	void step(dt::vector<Nxz>& ut, dt::vector<Nxz>& ub, dt::vector<Nxz>& wt, dt::vector<Nxz>& wb,
+	          const dt::vector<N>& fx, const dt::vector<Nv>& fy, const dt::vector<N>& fz)
+	{
+		ut_d.put(ut.data(), Nxz);
+		ub_d.put(ub.data(), Nxz);
+		wt_d.put(wt.data(), Nxz);
+		wb_d.put(wb.data(), Nxz);
+		fx_d.put(fx.data(), N);
+		fy_d.put(fy.data(), Nv);
+		fz_d.put(fz.data(), N);
+		afm_detail::check(afm_ns3_step(s, ut_d.d(), ub_d.d(), wt_d.d(), wb_d.d(), fx_d.d(), fy_d.d(), fz_d.d()),
+		                  "afm_ns3_step");
+		du.get(u_curr.data(), N);
+		dv.get(v_curr.data(), Nv);
+		dw.get(w_curr.data(), N);
+		ut_d.get(ut.data(), Nxz);
+		ub_d.get(ub.data(), Nxz);
+		wt_d.get(wt.data(), Nxz);
+		wb_d.get(wb.data(), Nxz);
+	}
+	const dt::vector<N>& get_u() { return u_curr; }
+	const dt::vector<Nv>& get_v() { return v_curr; }
+	const dt::vector<N>& get_w() { return w_curr; }
+	dt::vector<N> get_p()
+	{
+		afm_detail::Buffer p(c, N * 8);
+		afm_detail::check(afm_ns3_get_p(s, p.d()), "afm_ns3_get_p");
+		dt::vector<N> out;
+		p.get(out.data(), N);
 		return out;
 	}
 };

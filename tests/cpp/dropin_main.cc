@@ -106,6 +106,65 @@ int main()
 		for (unsigned i = 0; i < N; ++i) e = std::fmax(e, std::fabs(u0[i] - (0.5 + dt_)) + std::fabs(v0[i]));
 		std::printf("ns_uniform_error %.3e\n", e);
 	}
+	// mg_test3: the reference's 3-D geometry (16^3, walls in y), a U(0, 1)
+	// right-hand side; the residual is checked here with a 7-point stencil
+	// written out independently (wall neighbours fold into the diagonal)
+	{
+		typedef dt::geometry<4u, 1u, 1u, 1u> g3;
+		constexpr unsigned n = g3::n, M = g3::size;
+		dt::vector<M> b, rhs;
+		unsigned long long s = 12345;
+		for (unsigned i = 0; i < M; ++i) {
+			s = s * 6364136223846793005ull + 1442695040888963407ull;
+			b[i] = rhs[i] = (double) (s >> 11) / 9007199254740992.0;
+		}
+		mg::solver<g3> solver(1e-12, 2, 2);
+		unsigned cycles = solver.solve(b);
+		double mean = 0.0, r2 = 0.0;
+		for (unsigned i = 0; i < M; ++i) mean += rhs[i];
+		mean /= M;
+		for (unsigned i = 0; i < M; ++i) {
+			unsigned x = i % n, y = (i / n) % n, z = i / (n * n);
+			double lx = 0.0;
+			int cnt = 0;
+			auto at = [&](unsigned a, unsigned bb, unsigned cc) { return b[a + n * (bb + n * cc)]; };
+			lx += at((x + 1) % n, y, z) + at((x + n - 1) % n, y, z) + at(x, y, (z + 1) % n) + at(x, y, (z + n - 1) % n);
+			cnt += 4;
+			if (y > 0) { lx += at(x, y - 1, z); ++cnt; }
+			if (y < n - 1) { lx += at(x, y + 1, z); ++cnt; }
+			lx = (double) n * n * (lx - cnt * b[i]);
+			const double r = rhs[i] - mean - lx;
+			r2 += r * r;
+		}
+		std::printf("mg3_cycles %u\nmg3_rms %.3e\n", cycles, std::sqrt(r2 / M));
+	}
+	// ns_test: the reference's 3-D solver with the initial profile its
+	// comment suggests, u0 = (H - y) y, no-slip walls, no force: the flow
+	// stays a function of y alone (v = w = 0) and decays by diffusion
+	{
+		typedef dt::geometry<4u, 1u, 1u, 1u> g3;
+		constexpr unsigned n = g3::n, nx = g3::nx, ny = g3::ny, nz = g3::nz;
+		dt::vector<nx * ny * nz> u0, w0, fx, fz;
+		dt::vector<nx * (ny - 1) * nz> v0, fy;
+		dt::vector<nx * nz> ut, ub, wt, wb;
+		for (unsigned i = 0; i < nx * ny * nz; ++i) {
+			double y = (double) ((i % (nx * ny)) / nx) / n;
+			u0[i] = (1.0 - y) * y;
+		}
+		const unsigned mid = nx * (ny / 2);
+		const double before = u0[mid];
+		ns::solver<g3> s(1.0, 1.0, 5e-5, 1e-12, u0, v0, w0);
+		s.step(ut, ub, wt, wb, fx, fy, fz);
+		s.step(ut, ub, wt, wb, fx, fy, fz);
+		double vw = 0.0, spread = 0.0;
+		for (unsigned i = 0; i < nx * (ny - 1) * nz; ++i) vw = std::fmax(vw, std::fabs(v0[i]));
+		for (unsigned i = 0; i < nx * ny * nz; ++i) {
+			vw = std::fmax(vw, std::fabs(w0[i]));
+			const unsigned y = (i % (nx * ny)) / nx;
+			spread = std::fmax(spread, std::fabs(u0[i] - u0[nx * y]));
+		}
+		std::printf("ns3_vw_max %.3e\nns3_xz_spread %.3e\nns3_u_decay %.6e\n", vw, spread, before - u0[mid]);
+	}
 	std::printf("done 1\n");
 	return 0;
 }

@@ -27,3 +27,12 @@ def test_dropin_programs():
     assert float(v["mg_rel_error"]) < 1e-9
     assert float(v["ns_zero_max"]) == 0.0
     assert float(v["ns_uniform_error"]) < 1e-12
+    # the reference's 3-D geometry through the same headers
+    assert 1 <= int(v["mg3_cycles"]) <= 30
+    assert float(v["mg3_rms"]) <= 1.05e-12    # the solve's own test is <= 1e-12 in its summation order
+    assert float(v["ns3_vw_max"]) < 1e-10
+    assert float(v["ns3_xz_spread"]) < 1e-12
+    # u = (1 - y) y has u'' = -2: over one full step (dt = 5e-5, mu = rho = 1)
+    # the mid-channel value falls by 2 mu dt (the discrete laplacian of a
+    # quadratic is exact away from the walls)
+    assert abs(float(v["ns3_u_decay"]) - 2 * 5e-5) < 1e-9

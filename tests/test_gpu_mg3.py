@@ -63,6 +63,21 @@ def test_mg3_solve_matches_oracle(ctx_of, oracle, dev, L):
     assert np.max(np.abs(dl)) <= 1e-8 * np.max(np.abs(lex - lex.mean()))
 
 
+def test_mg3_solves_the_mg_test3_fixture(ctx_of, dev):
+    """test/mg_test3.cc's right-hand side (tests/golden/mg_test3.npz, with
+    the reference's own solution at the absolute tolerance 1e-15): the GPU
+    solve (red-black order) meets the tolerance and agrees to rounding."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "mg_test3.npz"))
+    ctx = ctx_of(64)
+    x = T(g["b"].reshape(16, 16, 16), dev)
+    cyc, res = ctx.mg3_solve(x, tol=float(g["tol"]), tol_mode=0)
+    got, want = H(x).ravel(), g["x"]
+    assert res <= float(g["tol"]) and abs(cyc - int(g["cycles"])) <= 3
+    d = (got - got.mean()) - (want - want.mean())
+    assert np.max(np.abs(d)) <= 1e-12 * np.max(np.abs(want - want.mean()))
+
+
 def test_mg3_rejects_bad_input(ctx_of, dev):
     from afmibins_amd._lib import AfmError
     ctx = ctx_of(64)

@@ -71,6 +71,17 @@ def test_ns3_steps_match_reference(oracle, ref, geo, steps):
     assert np.array_equal(po, pr)
 
 
+def test_mg_test3_fixture(oracle):
+    """test/mg_test3.cc's right-hand side, solved by the reference's
+    mg::solver<geometry<4,1,1,1>> (tests/golden/make_mg_test3.py): the
+    oracle's lexicographic solve gives the same cycles and bits."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "mg_test3.npz"))
+    x, c, _ = oracle.mg_solve(g["b"].reshape(16, 16, 16), 4, dim=3, tol=float(g["tol"]), tol_mode=0, smoother=0)
+    assert c == int(g["cycles"])
+    assert np.array_equal(x.ravel(), g["x"])
+
+
 # ------------------------------------------------------------ properties
 def test_spread3_is_adjoint_of_interp3_up_to_the_walls(oracle):
     """<S F, u>_h = <F, J u>_ds for all three MAC components with markers
