@@ -307,12 +307,16 @@ __global__ __launch_bounds__(256) void k_sum_fin(unsigned nblk, SolveCtl* ctl, c
 // k_advect_rhs fused with k_cg_init for the two velocity solves of a step:
 // the right-hand side b = rx (or ry) is formed per cell and never stored;
 // r = b - A u with u the initial guess (ns/solver.h:105-135 then
-// alg/cg.h:38-39), partial sums of r.r and b.b per field.
+// alg/cg.h:38-39), partial sums of r.r and b.b per field.  With STORE_B the
+// right-hand sides are stored to bu, bv instead and only b.b is summed (the
+// Neumann start, k_cg_start_neu, forms x0 and r).
+template<bool STORE_B>
 __global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* __restrict__ fx,
                                                         const double* __restrict__ fy, double* __restrict__ uo,
                                                         double* __restrict__ vo, int nx, int ny, unsigned n, double dt,
                                                         double rho, int be, double hoff, double hdg, double off,
-                                                        double dg, double tol, int max_iters)
+                                                        double dg, double tol, int max_iters, double* __restrict__ bu,
+                                                        double* __restrict__ bv)
 {
 	if (blockIdx.x == 0 && threadIdx.x == 0) {
 		cg_ctl_start(a, 0, tol, max_iters);
@@ -343,6 +347,13 @@ __global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* 
 			rx = hu + dt * (1 / rho * fx[i] - ua);
 			ry = hv + dt * (1 / rho * fy[i] - va);
 		}
+		if (STORE_B) {
+			bu[i] = rx;
+			bv[i] = ry;
+			acc_u[1] += rx * rx;
+			acc_v[1] += ry * ry;
+			continue;
+		}
 		const double Au = stencil5<double, true>(off, dg, u0, u[ie], u[iw], u[in_], u[is], we, ww, wn, ws);
 		const double Av = stencil5<double, true>(off, dg, v0, v[ie], v[iw], v[in_], v[is], we, ww, wn, ws);
 		const double r_u = rx - Au, r_v = ry - Av;
@@ -355,6 +366,52 @@ __global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* 
 	}
 	reduce_partial<2>(acc_u, a.pu[0]);
 	reduce_partial<2>(acc_v, a.pu[1]);
+}
+
+// (A b)_j for the periodic Helmholtz row of cell j (its own wrap flags, so
+// the value is the same bits in every thread that forms it)
+__device__ inline double helm_row_at(const double* __restrict__ b, size_t j, int nx, int ny, size_t N, double off,
+                                     double dg)
+{
+	int gx, gy;
+	cell_xy(j, nx, gx, gy);
+	const size_t ie = gx == nx - 1 ? j - (nx - 1) : j + 1, iw = gx == 0 ? j + (nx - 1) : j - 1;
+	const size_t in_ = gy == ny - 1 ? (size_t) gx : j + nx, is = gy == 0 ? j + N - nx : j - nx;
+	return stencil5<double, true>(off, dg, b[j], b[ie], b[iw], b[in_], b[is], gx == nx - 1, gx == 0, gy == ny - 1,
+	                              gy == 0);
+}
+
+// The CG forms' start for the velocity solves (no reference counterpart: the
+// reference starts from the current velocity).  With A = I - sL,
+// x0 = b + (I - A) b = 2b - A b, the first two terms of the Neumann series of
+// A^-1, so r0 = b - A x0 = (I - A)^2 b: on the step's smooth right-hand
+// sides a residual 2-3 orders below the one of the current velocity (one CG
+// iteration fewer at 2048^2).  x0 at the four neighbours is formed here too
+// (b on a two-cell ring, read through the caches); writes x and r, partial
+// sums of r.r (b.b was summed by the right-hand-side launch).
+__global__ __launch_bounds__(256) void k_cg_start_neu(CGArgs a, const double* __restrict__ b0,
+                                                      const double* __restrict__ b1, int nx, int ny, double off,
+                                                      double dg)
+{
+	const int f = blockIdx.y;
+	const double* b = f ? b1 : b0;
+	const size_t N = (size_t) nx * ny;
+	double* x = a.x[f];
+	double* r = a.r[f];
+	double acc[1] = {0.0};
+	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
+		CG_NEIGHBOURS
+		const double xc = 2.0 * b[i] - helm_row_at(b, i, nx, ny, N, off, dg);
+		const double xe = 2.0 * b[ie] - helm_row_at(b, ie, nx, ny, N, off, dg);
+		const double xw = 2.0 * b[iw] - helm_row_at(b, iw, nx, ny, N, off, dg);
+		const double xn = 2.0 * b[in_] - helm_row_at(b, in_, nx, ny, N, off, dg);
+		const double xs = 2.0 * b[is] - helm_row_at(b, is, nx, ny, N, off, dg);
+		const double ri = b[i] - stencil5<double, true>(off, dg, xc, xe, xw, xn, xs, we, ww, wn, ws);
+		x[i] = xc;
+		r[i] = ri;
+		acc[0] += ri * ri;
+	}
+	reduce_partial<1>(acc, a.pu[f]);
 }
 
 static inline unsigned rblocks(size_t N)
@@ -447,6 +504,7 @@ struct AdvectInit {
 	double *uo, *vo;
 	double dt, rho, hoff, hdg;
 	int be;
+	int neumann;      // the CG forms' start x0 = 2b - A b (k_cg_start_neu)
 };
 
 int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* const b[2], double tol, int tol_mode,
@@ -467,10 +525,18 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 	}
 	a.tol_mode = tol_mode;
 	const unsigned rb = rblocks(c->N);
-	if (adv)
-		hipLaunchKernelGGL(k_advect_cg_init, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy, adv->uo, adv->vo,
-		                   (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff, adv->hdg, off, dg,
-		                   tol, max_iters);
+	if (adv && adv->neumann) {
+		// b goes to the first p buffers: the first p update writes p[.][1]
+		// and reads r, so p[.][0] is free until the second one
+		hipLaunchKernelGGL(k_advect_cg_init<true>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy, adv->uo,
+		                   adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff, adv->hdg,
+		                   off, dg, tol, max_iters, w.p[0][0], w.p[1][0]);
+		hipLaunchKernelGGL(k_cg_start_neu, dim3(rb, 2), dim3(256), 0, c->stream, a, w.p[0][0], w.p[1][0],
+		                   (int) c->nx, (int) c->ny, off, dg);
+	} else if (adv)
+		hipLaunchKernelGGL(k_advect_cg_init<false>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy, adv->uo,
+		                   adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff, adv->hdg,
+		                   off, dg, tol, max_iters, nullptr, nullptr);
 	else
 		hipLaunchKernelGGL(k_cg_init, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg, tol,
 		                   max_iters);
@@ -593,7 +659,8 @@ int ns_step(NS* s, const double* fx, const double* fy)
 	double hoff, hdg;
 	helm_coef(c, sm, hoff, hdg);
 	// advection and the step's right-hand side are formed inside the CG start
-	const AdvectInit adv{fx, fy, s->uo, s->vo, s->dt, s->rho, hoff, hdg, s->half ? 0 : 1};
+	const AdvectInit adv{fx, fy, s->uo, s->vo, s->dt, s->rho, hoff, hdg, s->half ? 0 : 1,
+	                     (s->precision == 1 || s->precision == 3) ? 1 : 0};
 	double* xs[2] = {s->u, s->v};
 	const double* bs[2] = {nullptr, nullptr};
 	AFM_TRY(op_cg2(c, s->cg, ss, xs, bs, s->tol, s->tol_mode, s->max_cg, s->last_cg, nullptr, &adv));
