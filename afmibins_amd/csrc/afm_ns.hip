@@ -309,7 +309,7 @@ __global__ __launch_bounds__(256) void k_sum_fin(unsigned nblk, SolveCtl* ctl, c
 // r = b - A u with u the initial guess (ns/solver.h:105-135 then
 // alg/cg.h:38-39), partial sums of r.r and b.b per field.  With STORE_B the
 // right-hand sides are stored to bu, bv instead and only b.b is summed (the
-// Neumann start, k_cg_start_neu, forms x0 and r).
+// Neumann start, k_helm_pair / k_cg_start_x0 / k_cg_start_r, forms x0 and r).
 template<bool STORE_B>
 __global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* __restrict__ fx,
                                                         const double* __restrict__ fy, double* __restrict__ uo,
@@ -399,28 +399,36 @@ __global__ __launch_bounds__(256) void k_helm_pair(const double* __restrict__ b0
 // x0 = b + (I - A) b + (I - A)^2 b = 3b - 3c + A c, the first three terms of
 // the Neumann series of A^-1, so r0 = b - A x0 = (I - A)^3 b: on the step's
 // smooth right-hand sides a residual 4 orders below the one of the current
-// velocity (2 CG iterations instead of 4-5 at 2048^2).  x0 at the four
-// neighbours is formed here too (c on a two-cell ring, read through the
-// caches); writes x and r, partial sums of r.r (b.b was summed by the
+// velocity (2-3 CG iterations instead of 4-5 at 2048^2).  Three 5-point
+// passes: c = A b (k_helm_pair), x0 (here, written to x), r = b - A x0
+// (k_cg_start_r, with the partial sums of r.r; b.b was summed by the
 // right-hand-side launch).
-__global__ __launch_bounds__(256) void k_cg_start_neu(CGArgs a, const double* __restrict__ b0,
-                                                      const double* __restrict__ b1, const double* __restrict__ c0,
-                                                      const double* __restrict__ c1, int nx, int ny, double off,
-                                                      double dg)
+__global__ __launch_bounds__(256) void k_cg_start_x0(CGArgs a, const double* __restrict__ b0,
+                                                     const double* __restrict__ b1, const double* __restrict__ c0,
+                                                     const double* __restrict__ c1, int nx, int ny, double off,
+                                                     double dg)
 {
 	const int f = blockIdx.y;
 	const double* b = f ? b1 : b0;
 	const double* cb = f ? c1 : c0;
 	const size_t N = (size_t) nx * ny;
 	double* x = a.x[f];
+	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256)
+		x[i] = 3.0 * b[i] - 3.0 * cb[i] + helm_row_at(cb, i, nx, ny, N, off, dg);
+}
+
+__global__ __launch_bounds__(256) void k_cg_start_r(CGArgs a, const double* __restrict__ b0,
+                                                    const double* __restrict__ b1, int nx, int ny, double off,
+                                                    double dg)
+{
+	const int f = blockIdx.y;
+	const double* b = f ? b1 : b0;
+	const size_t N = (size_t) nx * ny;
+	const double* x = a.x[f];
 	double* r = a.r[f];
 	double acc[1] = {0.0};
-	auto x0 = [&](size_t j) { return 3.0 * b[j] - 3.0 * cb[j] + helm_row_at(cb, j, nx, ny, N, off, dg); };
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
-		CG_NEIGHBOURS
-		const double xc = x0(i), xe = x0(ie), xw = x0(iw), xn = x0(in_), xs = x0(is);
-		const double ri = b[i] - stencil5<double, true>(off, dg, xc, xe, xw, xn, xs, we, ww, wn, ws);
-		x[i] = xc;
+		const double ri = b[i] - helm_row_at(x, i, nx, ny, N, off, dg);
 		r[i] = ri;
 		acc[0] += ri * ri;
 	}
@@ -517,7 +525,7 @@ struct AdvectInit {
 	double *uo, *vo;
 	double dt, rho, hoff, hdg;
 	int be;
-	int neumann;      // the CG forms' start x0 = 3b - 3Ab + A^2 b (k_cg_start_neu)
+	int neumann;      // the CG forms' start x0 = 3b - 3Ab + A^2 b (k_cg_start_x0)
 };
 
 int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* const b[2], double tol, int tol_mode,
@@ -546,8 +554,10 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 		                   off, dg, tol, max_iters, w.p[0][0], w.p[1][0]);
 		hipLaunchKernelGGL(k_helm_pair, dim3(rb, 2), dim3(256), 0, c->stream, w.p[0][0], w.p[1][0], w.p[0][1],
 		                   w.p[1][1], (int) c->nx, (int) c->ny, off, dg);
-		hipLaunchKernelGGL(k_cg_start_neu, dim3(rb, 2), dim3(256), 0, c->stream, a, w.p[0][0], w.p[1][0], w.p[0][1],
+		hipLaunchKernelGGL(k_cg_start_x0, dim3(rb, 2), dim3(256), 0, c->stream, a, w.p[0][0], w.p[1][0], w.p[0][1],
 		                   w.p[1][1], (int) c->nx, (int) c->ny, off, dg);
+		hipLaunchKernelGGL(k_cg_start_r, dim3(rb, 2), dim3(256), 0, c->stream, a, w.p[0][0], w.p[1][0], (int) c->nx,
+		                   (int) c->ny, off, dg);
 	} else if (adv)
 		hipLaunchKernelGGL(k_advect_cg_init<false>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy, adv->uo,
 		                   adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff, adv->hdg,
