@@ -315,6 +315,40 @@ def test_ns_steps_match_oracle(afm, oracle, dev, n):
     ns.close(); ctx.close()
 
 
+@pytest.mark.parametrize("n", [256, 1024])
+def test_cg_forms_neumann_start(afm, oracle, dev, n):
+    """The CG forms start the velocity solves from x0 = 3b - 3Ab + A^2 b: the
+    same steps as the FP64 form (which starts from the current velocity, as
+    the reference does) within the solves' tolerance, with no more CG
+    iterations, and the oracle's steps within the same bound."""
+    u0, v0 = W.fourier_velocity(n, 8, seed=n + 1)
+    rng = np.random.default_rng(n)
+    fx, fy = rng.standard_normal((n, n)), rng.standard_normal((n, n))
+    mu, rho, dt, tol = 1e-3, 1.0, 2e-5, 1e-11
+    out = {}
+    for precision in (0, 1):
+        ctx = afm.Context(n, 0)
+        u, v = T(u0, dev), T(v0, dev)
+        ns = afm.NSSolver(ctx, mu, rho, dt, tol, u, v, tol_mode=1, precision=precision)
+        cg = []
+        for _ in range(4):
+            ns.step(T(fx, dev), T(fy, dev))
+            st = ns.stats()
+            cg.append(max(st["cg_u"], st["cg_v"]))
+        out[precision] = (H(u), H(v), cg)
+        ns.close(); ctx.close()
+    (u_a, v_a, cg_a), (u_b, v_b, cg_b) = out[0], out[1]
+    print(f"n={n} CG iterations per step: FP64 form {cg_a}, CG form {cg_b}", flush=True)
+    assert all(b <= a for a, b in zip(cg_a, cg_b)) and sum(cg_b) < sum(cg_a)
+    # the mixed form's projection uses FP32 cycles to the same relative
+    # residual: differences of the order of the tolerance (as in the IB test)
+    sc = max(np.abs(u_a).max(), np.abs(v_a).max())
+    assert max(np.abs(u_b - u_a).max(), np.abs(v_b - v_a).max()) <= 1e-8 * sc
+    if n <= 256:
+        uo, vo, _, _ = oracle.ns_run(u0, v0, fx, fy, n, mu, rho, dt, tol, tol_mode=1, smoother=1, calls=4)
+        assert max(np.abs(u_b - uo).max(), np.abs(v_b - vo).max()) <= 1e-8 * sc
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_ib_steps_match_oracle(afm, oracle, dev, precision):
     n, steps = 128, 3
