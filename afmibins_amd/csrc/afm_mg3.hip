@@ -427,8 +427,6 @@ int mg3_solve_with(Mg3* g, double* rhs, double tol, int tol_mode, int max_cycles
 	Mg3Ctx& m = g->m;
 	Ctx* c = m.c;
 	const unsigned L = m.L;
-	auto fail = [](int r) { return r; };
-	int rc = 0;
 	const size_t N = cells3(L);
 	double* sc = m.scal;
 	double* xs = g->xs;
@@ -436,35 +434,39 @@ int mg3_solve_with(Mg3* g, double* rhs, double tol, int tol_mode, int max_cycles
 	double* copyb = g->copyb;
 	double h_sq = 0.0;
 	// the relative tolerance: RMS of rhs - mean
-	if ((rc = sum3(c, rhs, N, 0, sc))) return fail(rc);
+	AFM_TRY(sum3(c, rhs, N, 0, sc));
 	hipLaunchKernelGGL(k_copy3, g3(N), dim3(256), 0, c->stream, copyb, rhs, N);
 	hipLaunchKernelGGL(k_sub3, g3(N), dim3(256), 0, c->stream, copyb, sc, 1.0 / (double) N, N);
-	if ((rc = sum3(c, copyb, N, 1, sc + 1)) || (rc = read_scalar(c, sc + 1, &h_sq))) return fail(rc);
+	AFM_TRY(sum3(c, copyb, N, 1, sc + 1));
+	AFM_TRY(read_scalar(c, sc + 1, &h_sq));
 	const double t = tol_mode == 1 ? tol * (sqrt(h_sq) / sqrt((double) N)) : tol;
 	// x = rhs; nested(L, x)
 	hipLaunchKernelGGL(k_copy3, g3(N), dim3(256), 0, c->stream, xs, rhs, N);
-	if ((rc = nested3(m, L, xs))) return fail(rc);
+	AFM_TRY(nested3(m, L, xs));
 	// rhs -= mean (c = dot(rhs, ones) / N, as the reference computes it)
-	if ((rc = sum3(c, rhs, N, 0, sc))) return fail(rc);
+	AFM_TRY(sum3(c, rhs, N, 0, sc));
 	hipLaunchKernelGGL(k_sub3, g3(N), dim3(256), 0, c->stream, rhs, sc, 1.0 / (double) N, N);
 	hipLaunchKernelGGL(k_resid3, g3(N), dim3(256), 0, c->stream, xs, rhs, res, 1u << L);
-	if ((rc = sum3(c, res, N, 1, sc + 1)) || (rc = read_scalar(c, sc + 1, &h_sq))) return fail(rc);
+	AFM_TRY(sum3(c, res, N, 1, sc + 1));
+	AFM_TRY(read_scalar(c, sc + 1, &h_sq));
 	double r = sqrt(h_sq) / sqrt((double) N);
 	int count = 0;
 	while (r > t && count < max_cycles) {
-		if ((rc = recurse3(m, L, res))) return fail(rc);
+		AFM_TRY(recurse3(m, L, res));
 		hipLaunchKernelGGL(k_axpy3, g3(N), dim3(256), 0, c->stream, xs, res, N);
 		hipLaunchKernelGGL(k_resid3, g3(N), dim3(256), 0, c->stream, xs, rhs, res, 1u << L);
-		if ((rc = sum3(c, res, N, 1, sc + 1)) || (rc = read_scalar(c, sc + 1, &h_sq))) return fail(rc);
+		AFM_TRY(sum3(c, res, N, 1, sc + 1));
+		AFM_TRY(read_scalar(c, sc + 1, &h_sq));
 		r = sqrt(h_sq) / sqrt((double) N);
 		++count;
 	}
 	hipLaunchKernelGGL(k_copy3, g3(N), dim3(256), 0, c->stream, rhs, xs, N);
+	AFM_LAUNCH_CHECK();
 	if (cycles) *cycles = count;
 	if (final_rms) *final_rms = r;
-	rc = r <= t ? 0 : -4;
-	if (rc) set_error("3-D multigrid did not converge within max_cycles");
-	return fail(rc);
+	if (r <= t) return 0;
+	set_error("3-D multigrid did not converge within max_cycles");
+	return -4;
 }
 
 }  // namespace afm
