@@ -72,6 +72,7 @@ SIGNATURES = {
     "afm_ns_get_p": (_I, [_P, _P]),
     "afm_ns_stats": (_I, [_P, _IP, _IP, _IP, _IP]),
     "afm_ns_old": (_I, [_P, C.POINTER(_P), C.POINTER(_P)]),
+    "afm_ns_last_residual": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "afm_ns_get_start": (_I, [_P, _I, _P, _IP]),
     "afm_ns_set_start": (_I, [_P, _I, _P]),
     "afm_ns_destroy": (_I, [_P]),

@@ -384,6 +384,12 @@ class NSSolver:
         self._lib("afm_ns_stats", C.byref(a), C.byref(b), C.byref(c), C.byref(d))
         return {"cycles": a.value, "cg_u": b.value, "cg_v": c.value, "have_half_stepped": bool(d.value)}
 
+    def last_residual(self):
+        """(final RMS residual, its bound) of the last projection's solve."""
+        a, b = C.c_double(), C.c_double()
+        self._lib("afm_ns_last_residual", C.byref(a), C.byref(b))
+        return a.value, b.value
+
     def close(self):
         if self._h:
             torch.cuda.synchronize(self.ctx.device)

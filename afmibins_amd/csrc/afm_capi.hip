@@ -46,6 +46,7 @@ int op_helm_apply(Ctx*, double, const double*, double*);
 struct CGWork;
 int op_cg_single(Ctx*, double, const double*, double*, double, int, int, int*, double*);
 int ns_stats(NS*, int*, int*, int*, int*);
+int ns_last_residual(NS*, double*, double*);
 int ns_old(NS*, const double**, const double**);
 Ctx* ns_ctx(NS*);
 void ns_free(NS*);
@@ -529,6 +530,12 @@ int afm_ns_stats(afm_ns* s, int* cyc, int* cgu, int* cgv, int* half)
 {
 	CHECK_ARG(s, "null argument");
 	return ns_stats(s->s, cyc, cgu, cgv, half);
+}
+
+int afm_ns_last_residual(afm_ns* s, double* final_rms, double* threshold)
+{
+	CHECK_ARG(s && final_rms && threshold, "null argument");
+	return ns_last_residual(s->s, final_rms, threshold);
 }
 
 int afm_ns_get_start(afm_ns* s, int kind, double* phi, int* valid)

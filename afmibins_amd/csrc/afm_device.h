@@ -121,7 +121,7 @@ __device__ inline void lap_row(int x, int y, int nx, int ny, double off, LapRow&
 
 // ------------------------------------------------------------- IB window --
 // The 4x4 window of one MAC component around marker (cx, cy):
-// ib/eulerian_grid.h:151-180 through ib/mac_grid.h:244-257, restricted to the
+// ib/eulerian_grid.h:64-93 through ib/mac_grid.h:49-73, restricted to the
 // periodic (x, z) plane.  comp 0 (u, x-faces) is shifted by -h/2 in y before
 // the search, comp 1 (v, y-faces) in x.  Tap (a, b) is the cell
 // ((ix0 + a) % nx, (iy0 + b) % ny) with weight dx[a] * dy[b].

@@ -623,6 +623,7 @@ struct NS {
 	CGWork cg;
 	bool half = false;
 	int last_cycles = 0, last_cg[2] = {0, 0};
+	double last_rms = 0.0, last_thr = 0.0;   // the last projection's final RMS residual and its bound
 	int max_cycles = 200, max_cg = 10000;
 	std::vector<void*> own;
 	~NS() { for (void* p : own) (void) hipFree(p); }
@@ -640,6 +641,8 @@ int ns_project(NS* s, double* u, double* v, int kind)
 	const double* x0 = (keep != s->phi && s->start_ok[kind]) ? keep : nullptr;
 	AFM_TRY(mg_solve(c, s->phi, s->tol, s->tol_mode, 2, 2, s->precision, s->max_cycles, &cyc, nullptr, &sol, x0));
 	s->last_cycles = cyc;
+	s->last_rms = c->hctl[0].res;
+	s->last_thr = c->hctl[0].tol;
 	// the gradient is taken where the solve left the potential, and the
 	// potential is stored (to phi, or as the kind's next start) in the same pass
 	hipLaunchKernelGGL(k_grad_sub, pgrid(c->N), dim3(256), 0, c->stream, sol, u, v, (int) c->nx, (int) c->ny, c->n,
@@ -684,8 +687,12 @@ int ns_step(NS* s, const double* fx, const double* fy)
 	double hoff, hdg;
 	helm_coef(c, sm, hoff, hdg);
 	// advection and the step's right-hand side are formed inside the CG start
-	const AdvectInit adv{fx, fy, s->uo, s->vo, s->dt, s->rho, hoff, hdg, s->half ? 0 : 1,
-	                     (s->precision == 1 || s->precision == 3) ? 1 : 0};
+	// The Neumann start x0 = 3b - 3Ab + A^2 b leaves r0 = (I - A)^3 b; A = I - ss L
+	// has its spectrum in [1, 1 + 8|ss|n^2], so the start only helps while
+	// 8|ss|n^2 is well below 1.  Beyond that the CG forms start from the
+	// current velocity, as the FP64 form always does.
+	const bool neumann = (s->precision == 1 || s->precision == 3) && 8.0 * std::fabs(ss) * c->n * c->n < 0.5;
+	const AdvectInit adv{fx, fy, s->uo, s->vo, s->dt, s->rho, hoff, hdg, s->half ? 0 : 1, neumann ? 1 : 0};
 	double* xs[2] = {s->u, s->v};
 	const double* bs[2] = {nullptr, nullptr};
 	AFM_TRY(op_cg2(c, s->cg, ss, xs, bs, s->tol, s->tol_mode, s->max_cg, s->last_cg, nullptr, &adv));
@@ -746,6 +753,7 @@ int ib_step(IB* b)
 	NS* s = b->ns;
 	Ctx* c = s->c;
 	const double dt = s->dt;
+	if (s->half) { set_error("the NS solver must be at a full step"); return -1; }
 	AFM_TRY(launch_spread(c, 3, nullptr, nullptr, b->ds, b->X, b->R, b->k, b->m, b->fx, b->fy));
 	AFM_TRY(ns_step(s, b->fx, b->fy));
 	AFM_TRY(launch_ib_move_half(c, s->u, s->v, b->X, b->m, 0.5 * dt, b->Xh));
@@ -880,7 +888,13 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 		set_error("out of device memory (adjoint tape)");
 		return fail(-3);
 	}
-	if ((rc = d2d(c, fu, s->u, N)) || (rc = d2d(c, fv, s->v, N)) || (rc = d2d(c, fX, b->X, 2 * m))) return fail(rc);
+	// the reverse sweep's projections overwrite phi, which get_p reads in the
+	// FP64 and FP32-cycle forms (the CG forms read the full step's start)
+	double* fphi = A(N);
+	if (!fphi) { set_error("out of device memory (adjoint tape)"); return fail(-3); }
+	if ((rc = d2d(c, fu, s->u, N)) || (rc = d2d(c, fv, s->v, N)) || (rc = d2d(c, fX, b->X, 2 * m)) ||
+	    (rc = d2d(c, fphi, s->phi, N)))
+		return fail(rc);
 	for (int k = 0; warm && k < 2; ++k)
 		if ((rc = d2d(c, fs[k], s->start[k], N))) return fail(rc);
 	const double vol = 1. / (c->n * c->n);
@@ -966,12 +980,14 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 		}
 	}
 	// leave the solver at the final state, as a forward run would
-	if ((rc = d2d(c, s->u, fu, N)) || (rc = d2d(c, s->v, fv, N)) || (rc = d2d(c, b->X, fX, 2 * m))) return fail(rc);
+	if ((rc = d2d(c, s->u, fu, N)) || (rc = d2d(c, s->v, fv, N)) || (rc = d2d(c, b->X, fX, 2 * m)) ||
+	    (rc = d2d(c, s->phi, fphi, N)))
+		return fail(rc);
 	for (int k = 0; warm && k < 2; ++k) {
 		if ((rc = d2d(c, s->start[k], fs[k], N))) return fail(rc);
 		s->start_ok[k] = fs_ok[k];
 	}
-	if (warm) s->phi_last = s->start[1];     // the last forward projection: a full step's
+	s->phi_last = warm ? s->start[1] : s->phi;     // the last forward projection: a full step's
 	s->half = false;
 	return fail(0);
 }
@@ -1016,6 +1032,13 @@ int ns_stats(NS* s, int* cyc, int* cgu, int* cgv, int* half)
 	if (cgu) *cgu = s->last_cg[0];
 	if (cgv) *cgv = s->last_cg[1];
 	if (half) *half = s->half ? 1 : 0;
+	return 0;
+}
+
+int ns_last_residual(NS* s, double* rms, double* thr)
+{
+	*rms = s->last_rms;
+	*thr = s->last_thr;
 	return 0;
 }
 

@@ -9,7 +9,7 @@
  * Conventions
  *   - Every pointer argument is DEVICE memory unless the name ends in _host.
  *     Fields are nx*ny doubles, x fastest (idx = i + nx*j), as the reference
- *     stores them (ib/eulerian_grid.h:146-148).  Markers are m (x, y) pairs,
+ *     stores them (ib/eulerian_grid.h:54-62).  Markers are m (x, y) pairs,
  *     interleaved.
  *   - Calls are ordered on the context's stream.  Functions that return an
  *     iteration count synchronise; the others are asynchronous.
@@ -75,7 +75,7 @@ int afm_delta(afm_ctx* c, const double* r, long count, double* out);
 int afm_delta_host(const double* r_host, long count, unsigned n, double* out_host);
 
 /* ib::grid::interpolate with marker targets (ib/grid.h:68-84 through
- * ib/mac_grid.h:244-257): U[k] = sum over the 4x4 window of
+ * ib/mac_grid.h:49-73): U[k] = sum over the 4x4 window of
  * (delta_x delta_y h^2) field[idx].  comp 0 = u (x-faces), 1 = v (y-faces). */
 int afm_interp(afm_ctx* c, int comp, const double* field, const double* X, long m, double* U);
 
@@ -212,6 +212,12 @@ int afm_ns_project(afm_ns* s, double* u, double* v);             /* ns/projectio
 int afm_ns_get_p(afm_ns* s, double* p);                         /* ns/solver.h:86 */
 int afm_ns_stats(afm_ns* s, int* last_cycles, int* last_cg_u, int* last_cg_v, int* have_half_stepped);
 int afm_ns_old(afm_ns* s, const double** u_old, const double** v_old);
+/* The last projection's multigrid solve (no reference counterpart: the
+ * reference's solve returns only its cycle count, mg/solver.h:218-237):
+ * *final_rms = RMS(div - L phi) of the returned potential, which is the RMS
+ * divergence left in the projected velocity; *threshold = the bound it was
+ * held to (tol, or tol * RMS(div - mean(div)) for tol_mode 1). */
+int afm_ns_last_residual(afm_ns* s, double* final_rms, double* threshold);
 /* The CG forms' start potentials (precision 1 and 3; no reference
  * counterpart: the reference's solve starts from its nested iteration).
  * Such a solver starts a step's projection from the potential of the

@@ -10,7 +10,7 @@
 // FMA contraction changes a rounding.
 //
 // Grid convention.  The reference is 3-D, x and z periodic, y walled, with
-// fields stored x fastest: idx = x + nx*(y + ny*z) (ib/eulerian_grid.h:146-148).
+// fields stored x fastest: idx = x + nx*(y + ny*z) (ib/eulerian_grid.h:54-62).
 // The doubly periodic 2-D mode used by the product is the reference's
 // periodic (x, z) plane with y dropped; here it is stored as the 3-D layout
 // with ny == 1, so "z" below is the 2-D "y".  Functions named *2 are the 2-D
@@ -429,8 +429,8 @@ struct MG {
 };
 
 // ---------------------------------------------------------- IB windows -----
-// 2-D window of ib/eulerian_grid.h:151-180 as used through the generic
-// ib/mac_grid.h:244-257 (the reference's x and z components).  comp 0 (u,
+// 2-D window of ib/eulerian_grid.h:64-93 as used through the generic
+// ib/mac_grid.h:49-73 (the reference's x and z components).  comp 0 (u,
 // x-faces) is shifted in the 2-D y axis, comp 1 (v, y-faces) in x.
 // Returns the 16 (index, weight) pairs in tap order.
 void window2(int comp, double cx, double cy, unsigned n, unsigned W, unsigned D, unsigned nx, unsigned ny,
@@ -459,8 +459,8 @@ void window2(int comp, double cx, double cy, unsigned n, unsigned W, unsigned D,
 	}
 }
 
-// 3-D windows: generic components ib/mac_grid.h:244-257 + eulerian search
-// ib/eulerian_grid.h:151-180; v component ib/mac_grid.h:291-320.  Entries
+// 3-D windows: generic components ib/mac_grid.h:49-73 + eulerian search
+// ib/eulerian_grid.h:64-93; v component ib/mac_grid.h:97-136.  Entries
 // skipped by the reference's y filter get idx = SIZE_MAX.
 size_t sz_of(double f) { return (size_t) (int64_t) f; }
 

@@ -198,9 +198,45 @@ void ns_run(double mu, double rho, double dt, double tol, double* u, double* v, 
 	if (p) { auto pp = s.get_p(); for (unsigned i = 0; i < N; ++i) p[i] = pp[i]; }
 }
 
+// ---- ns::solver<G>::advection (ns/solver.h:142-236) and its outputs u_adv,
+// v_adv, w_adv are private.  An explicit instantiation may name private
+// members (access is not checked there), and a friend it defines hands out
+// the member pointers; the reference's code is then called as it stands.
+template<class Tag, typename Tag::type Ptr>
+struct Expose { friend typename Tag::type member_of(Tag) { return Ptr; } };
+
+constexpr unsigned N16 = SZ<G16>, Nv16 = G16::nx * (G16::ny - 1) * G16::nz;
+struct AdvFn { typedef void (ns::solver<G16>::*type)(dt::vector<N16>&, dt::vector<Nv16>&, dt::vector<N16>&);
+               friend type member_of(AdvFn); };
+struct AdvU { typedef dt::vector<N16> ns::solver<G16>::*type; friend type member_of(AdvU); };
+struct AdvV { typedef dt::vector<Nv16> ns::solver<G16>::*type; friend type member_of(AdvV); };
+struct AdvW { typedef dt::vector<N16> ns::solver<G16>::*type; friend type member_of(AdvW); };
+template struct Expose<AdvFn, &ns::solver<G16>::advection>;
+template struct Expose<AdvU, &ns::solver<G16>::u_adv>;
+template struct Expose<AdvV, &ns::solver<G16>::v_adv>;
+template struct Expose<AdvW, &ns::solver<G16>::w_adv>;
+
 }  // namespace
 
 extern "C" {
+
+// ns::solver<geometry<4,1,1,1>>::advection on caller data (u, w: 16^3;
+// v: 16*15*16 interior y-faces); the solver itself is built over zero fields.
+int ref_advect(const double* u, const double* v, const double* w, double* ua, double* va, double* wa)
+{
+	dt::vector<N16> u0, w0, uu, ww;
+	dt::vector<Nv16> v0, vv;
+	ns::solver<G16> s(1e-2, 1.0, 1e-3, 1e-12, u0, v0, w0);
+	for (unsigned i = 0; i < N16; ++i) { uu[i] = u[i]; ww[i] = w[i]; }
+	for (unsigned i = 0; i < Nv16; ++i) vv[i] = v[i];
+	(s.*member_of(AdvFn()))(uu, vv, ww);
+	const dt::vector<N16>& a = s.*member_of(AdvU());
+	const dt::vector<Nv16>& b = s.*member_of(AdvV());
+	const dt::vector<N16>& c = s.*member_of(AdvW());
+	for (unsigned i = 0; i < N16; ++i) { ua[i] = a[i]; wa[i] = c[i]; }
+	for (unsigned i = 0; i < Nv16; ++i) va[i] = b[i];
+	return 0;
+}
 
 // ib/grid.h:51-55 for n = 2^level, level 1..12.
 double ref_delta(double r, unsigned level)
