@@ -109,11 +109,8 @@ def fourier_rhs(n, modes, kmax, seed):
     return f - f.mean()
 
 
-def fourier_rhs_device(n, modes, kmax, seed, device):
-    """fourier_rhs built on the GPU (torch float64): the same modes from the
-    same SplitMix64 stream, each cos(a + b) expanded into two outer products
-    so the whole field is one matrix product."""
-    import torch
+def fourier_modes(modes, kmax, seed):
+    """The (kx, ky, amplitude, phase) lists of fourier_rhs, in its order."""
     rng = SplitMix64(seed)
     kxs, kys, amps, phs = [], [], [], []
     for _ in range(modes):
@@ -126,14 +123,50 @@ def fourier_rhs_device(n, modes, kmax, seed, device):
         phs.append(rng.uniform(0.0, 2.0 * np.pi))
         kxs.append(kx)
         kys.append(ky)
+    return kxs, kys, amps, phs
+
+
+def mode_sum_device(n, kxs, kys, amps, phs, device, offset=0.5):
+    """sum_m a_m cos(2 pi (kx_m x + ky_m y) + ph_m) at x = (i + offset)/n,
+    y = (j + offset)/n on the GPU (torch float64), as [y, x]: each cos(a + b)
+    is expanded into two outer products, so the field is one matrix product."""
+    import torch
     t = lambda a: torch.tensor(a, dtype=torch.float64, device=device)
-    x = (torch.arange(n, dtype=torch.float64, device=device) + 0.5) / n
+    x = (torch.arange(n, dtype=torch.float64, device=device) + offset) / n
     ax = 2.0 * np.pi * x[:, None] * t(kxs)[None, :] + t(phs)[None, :]      # n x modes
     by = 2.0 * np.pi * x[:, None] * t(kys)[None, :]
     a = t(amps)
-    # f[y, x] = sum_m a_m (cos ax cos by - sin ax sin by)
-    f = (torch.cos(by) * a) @ torch.cos(ax).T - (torch.sin(by) * a) @ torch.sin(ax).T
+    return (torch.cos(by) * a) @ torch.cos(ax).T - (torch.sin(by) * a) @ torch.sin(ax).T
+
+
+def fourier_rhs_device(n, modes, kmax, seed, device):
+    """fourier_rhs built on the GPU: the same modes from the same SplitMix64
+    stream (to rounding of the sums)."""
+    f = mode_sum_device(n, *fourier_modes(modes, kmax, seed), device)
     return f - f.mean()
+
+
+def fourier_velocity_device(n, kmax, seed, device, rms=1.0):
+    """fourier_velocity built on the GPU: the same psi (same modes, same
+    stream order) sampled at the cell corners by mode_sum_device, then the
+    same discrete curl and normalisation."""
+    import torch
+    rng = SplitMix64(seed)
+    kxs, kys, amps, phs = [], [], [], []
+    for kx in range(-kmax, kmax + 1):
+        for ky in range(-kmax, kmax + 1):
+            kk = math.hypot(kx, ky)
+            if kk == 0 or kk > kmax:
+                continue
+            amps.append(rng.normal() * kk ** -3)
+            phs.append(rng.uniform(0.0, 2.0 * np.pi))
+            kxs.append(kx)
+            kys.append(ky)
+    psi = mode_sum_device(n, kxs, kys, amps, phs, device, offset=0.0)
+    u = (torch.roll(psi, -1, 0) - psi) * n
+    v = -(torch.roll(psi, -1, 1) - psi) * n
+    s = math.sqrt((float(torch.mean(u * u)) + float(torch.mean(v * v))) / 2.0)
+    return u * (rms / s), v * (rms / s)
 
 
 # BASELINE configurations (sizes and physics) used by bench.py and the tests.
