@@ -66,6 +66,9 @@ def parse():
     p.add_argument("--forms", default="fp64,mixed")
     p.add_argument("--no-extras", action="store_true", help="skip the smoother, transfer and solver-only measurements")
     p.add_argument("--kernel-only", action="store_true", help="a few mixed steps and a calibration copy (counter runs)")
+    p.add_argument("--transfer-only", action="store_true",
+                   help="spreading and interpolation of the 4096^2 / 262,144-marker stress case and a calibration "
+                        "copy (counter runs)")
     p.add_argument("--grad", action="store_true",
                    help="only the differentiation check: 1024^2, 32 membranes (8,192 markers), 200 steps, "
                         "checkpoints every 20, FD along 32 directions")
@@ -292,13 +295,20 @@ def kernel_only(A, W, dev):
     class Args:
         warmup, steps = 1, 2
     r = run_form(A, ctx, 1, Args, inputs, dev)
+    ctx.close()
+    print(json.dumps({"ms_per_step": r["ms_per_step"], "copy_bytes_each_way": calibration_copy(dev)}), flush=True)
+
+
+def calibration_copy(dev):
+    """Five copies of a 256 MiB buffer: the known traffic that scales the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
     a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
     b = torch.empty_like(a)
     for _ in range(5):
         b.copy_(a)
     torch.cuda.synchronize()
-    ctx.close()
-    print(json.dumps({"ms_per_step": r["ms_per_step"], "copy_bytes_each_way": a.numel() * 8}), flush=True)
+    return a.numel() * 8
 
 
 def cpu_baseline(inputs, n):
@@ -325,6 +335,10 @@ def main():
     dev = torch.device("cuda", 0)
     if args.kernel_only:
         kernel_only(A, W, dev)
+        return
+    if args.transfer_only:
+        out = transfer_rates(A, W, dev)
+        print(json.dumps({"transfer": out, "copy_bytes_each_way": calibration_copy(dev)}), flush=True)
         return
     if args.grad:
         X, R, k, ds = W.membranes(32, 256, seed=2)
