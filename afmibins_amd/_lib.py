@@ -73,6 +73,7 @@ SIGNATURES = {
     "afm_ns_stats": (_I, [_P, _IP, _IP, _IP, _IP]),
     "afm_ns_old": (_I, [_P, C.POINTER(_P), C.POINTER(_P)]),
     "afm_ns_last_residual": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "afm_helmholtz_cg_noalloc": (_I, [_P, _D, _P, _P, _D, _I, _I, _IP, _DP]),
     "afm_ns_get_start": (_I, [_P, _I, _P, _IP]),
     "afm_ns_set_start": (_I, [_P, _I, _P]),
     "afm_ns_destroy": (_I, [_P]),

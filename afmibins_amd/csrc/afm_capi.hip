@@ -45,6 +45,8 @@ int op_advect(Ctx*, const double*, const double*, double*, double*);
 int op_helm_apply(Ctx*, double, const double*, double*);
 struct CGWork;
 int op_cg_single(Ctx*, double, const double*, double*, double, int, int, int*, double*);
+int op_cg_single_noalloc(Ctx*, double, const double*, double*, double, int, int, int*, double*);
+int cg_scratch_setup(int);
 int ns_stats(NS*, int*, int*, int*, int*);
 int ns_last_residual(NS*, double*, double*);
 int ns_old(NS*, const double**, const double**);
@@ -91,6 +93,7 @@ int afm_ctx_create(const afm_geom* g, int device, afm_ctx** out)
 	AFM_HIP(hipGetDeviceCount(&ndev));
 	CHECK_ARG(device >= 0 && device < ndev, "device index out of range");
 	AFM_HIP(hipSetDevice(device));
+	AFM_TRY(cg_scratch_setup(device));
 	afm_ctx* c = new afm_ctx();
 	c->device = device;
 	c->n = g->n; c->W = g->width; c->D = g->depth;
@@ -468,6 +471,15 @@ int afm_helmholtz_cg(afm_ctx* c, double scale, const double* b, double* x, doubl
 	CHECK_ARG(tol_mode == 0 || tol_mode == 1, "tol_mode must be 0 or 1");
 	AFM_TRY(use(c));
 	return op_cg_single(c, scale, b, x, tol, tol_mode, max_iters, iters, final_norm);
+}
+
+int afm_helmholtz_cg_noalloc(afm_ctx* c, double scale, const double* b, double* x, double tol, int tol_mode,
+                             int max_iters, int* iters, double* final_norm)
+{
+	CHECK_ARG(c && b && x, "null argument");
+	CHECK_ARG(tol_mode == 0 || tol_mode == 1, "tol_mode must be 0 or 1");
+	AFM_TRY(use(c));
+	return op_cg_single_noalloc(c, scale, b, x, tol, tol_mode, max_iters, iters, final_norm);
 }
 
 int afm_div(afm_ctx* c, const double* u, const double* v, double* d)

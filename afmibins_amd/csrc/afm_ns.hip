@@ -11,6 +11,9 @@
 //                  deterministically, the stop test on the GPU;
 //   k_div, k_grad_sub   projection (ns/projection.h:60-73, alg/div.h,
 //                  alg/grad.h) around the multigrid solve.
+#include <mutex>
+#include <string>
+
 #include "afm_device.h"
 
 namespace afm {
@@ -1047,6 +1050,67 @@ int ns_old(NS* s, const double** uo, const double** vo)
 	*uo = s->uo;
 	*vo = s->vo;
 	return 0;
+}
+
+// ------------------------------------------ allocation-free Helmholtz CG --
+// A variant of op_cg_single for callers that cannot allocate device memory.
+// The residual and the two search directions of its one field live in this
+// file-scope array, of which every GPU holds its own copy until the process
+// ends.  Host code never passes `scratch` itself (its name is a host-side
+// stand-in, not a device address): cg_scratch_setup looks up the device
+// address once per device, with that device current, and the solve uses
+// that.  Every vector in it is written before it is read (r and the first p
+// by the CG start, the second p by the first iteration), so nothing needs
+// clearing between calls; calls that overlap on one GPU race on it.
+constexpr size_t CG_SCRATCH_CELLS = (size_t) 2048 * 2048;   // 3 vectors: 96 MiB
+constexpr int CG_SCRATCH_DEVICES = 64;
+static __device__ __align__(256) double scratch[3 * CG_SCRATCH_CELLS];
+static double* scratch_dev[CG_SCRATCH_DEVICES];
+static std::mutex scratch_mu;
+
+// once per device (from the context's creation), with `device` current
+int cg_scratch_setup(int device)
+{
+	std::lock_guard<std::mutex> lock(scratch_mu);
+	if (device < 0 || device >= CG_SCRATCH_DEVICES) { set_error("device index beyond the scratch table"); return -1; }
+	if (scratch_dev[device]) return 0;
+	void* p = nullptr;
+	const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+	if (e != hipSuccess || !p) {
+		set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+		return -3;
+	}
+	scratch_dev[device] = static_cast<double*>(p);
+	return 0;
+}
+
+int op_cg_single_noalloc(Ctx* c, double scale, const double* b, double* x, double tol, int tol_mode, int max_iters,
+                         int* iters, double* final_norm)
+{
+	if (c->N > CG_SCRATCH_CELLS) {
+		set_error("the allocation-free CG takes grids of at most 2048 x 2048 cells");
+		return -1;
+	}
+	double* s = nullptr;
+	{
+		std::lock_guard<std::mutex> lock(scratch_mu);
+		s = (c->device >= 0 && c->device < CG_SCRATCH_DEVICES) ? scratch_dev[c->device] : nullptr;
+	}
+	if (!s) { set_error("the allocation-free CG's buffer is not set up on this device"); return -3; }
+	CGWork w;     // the second field stays null: the CG kernels skip a field whose x is null
+	w.r[0] = s;
+	w.p[0][0] = s + c->N;
+	w.p[0][1] = s + 2 * c->N;
+	double* xs[2] = {x, nullptr};
+	const double* bs[2] = {b, nullptr};
+	int it[2];
+	double res[2];
+	const int rc = op_cg2(c, w, scale, xs, bs, tol, tol_mode, max_iters, it, res);
+	if (rc == 0 || rc == -4) {
+		if (iters) *iters = it[0];
+		if (final_norm) *final_norm = res[0];
+	}
+	return rc;
 }
 
 // Single-field CG with scratch owned for the duration of the call.

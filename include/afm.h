@@ -193,6 +193,18 @@ int afm_spread3(afm_ctx* c, unsigned levels, int comp, const double* F, const do
 int afm_helmholtz_apply(afm_ctx* c, double scale, const double* x, double* y);
 int afm_helmholtz_cg(afm_ctx* c, double scale, const double* b, double* x, double tol, int tol_mode, int max_iters,
                      int* iters, double* final_norm);
+/* afm_helmholtz_cg without any allocation (no counterpart in the reference:
+ * for callers that cannot allocate device memory -- code that prepares HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * CG's residual and search directions live in a buffer the library holds on
+ * each GPU for the life of the process (3 * 2048^2 doubles, 96 MiB, set up
+ * when the first context on that GPU is created), so the grid must have at
+ * most 2048 x 2048 cells (-1 otherwise, with nothing run).  Nothing in the
+ * buffer carries over between calls, but calls that overlap on one GPU
+ * (from several contexts or threads) race on it.  Same arguments, kernels
+ * and results as afm_helmholtz_cg. */
+int afm_helmholtz_cg_noalloc(afm_ctx* c, double scale, const double* b, double* x, double tol, int tol_mode,
+                             int max_iters, int* iters, double* final_norm);
 
 /* alg::div (alg/div.h:29-60), alg::grad + subtraction (alg/grad.h:29-44,
  * ns/projection.h:69-72), ns::solver::advection (ns/solver.h:142-236). */
