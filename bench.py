@@ -31,8 +31,9 @@ cycle's FP32 (4): 48 B per finest cell (56 B with FP64 cycles).  The FP64
 form's is the finest-level post-smoothing fused with the outer update
 (k_smooth_update): read e_pre and b (8 + 8), the coarse correction (8 per 4
 cells = 2), xs and rhs (8 + 8); write xs and r (8 + 8): 50 B (37 B with FP32
-cycles).  `cpu_baseline` is the oracle's restatement of the reference path on
-one host core over a one-step sample.
+cycles).  `cpu_baseline` is the oracle's restatement of the reference path
+(kd-tree gather spreading, lexicographic multigrid, CSR CG, glibc cos) on one
+host core over a three-step sample of the same workload.
 """
 import argparse
 import ctypes as C
@@ -63,7 +64,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--n", type=int, default=2048)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--forms", default="fp64,mixed")
+    p.add_argument("--forms", default="fp64,mixed,mixed_ir,fp64_cg")
     p.add_argument("--no-extras", action="store_true", help="skip the smoother, transfer and solver-only measurements")
     p.add_argument("--kernel-only", action="store_true", help="a few mixed steps and a calibration copy (counter runs)")
     p.add_argument("--transfer-only", action="store_true",
@@ -311,15 +312,24 @@ def calibration_copy(dev):
     return a.numel() * 8
 
 
-def cpu_baseline(inputs, n):
+def cpu_baseline(inputs, n, steps=3):
+    """The oracle's restatement of the reference path on one host core:
+    lexicographic Gauss-Seidel V(2,2) multigrid with the nested start, CSR CG
+    for the Helmholtz solves, the delta with glibc's cos of the x87 argument,
+    and spreading as the reference does it (every face gathers the markers
+    within 2/n from a kd-tree); interpolation by the marker windows."""
     from oracle import oracle as O
     u0, v0, X, R, k, ds = inputs
-    O.ib_run(u0, v0, X, R, k, ds, n, 1e-3, 1.0, 2e-5, 1e-10, 1, tol_mode=1, smoother=0, delta_mode=0)
-    sec = O.lib().or_last_step_seconds()
+    O.lib().or_set_spread_gather(1)
+    try:
+        O.ib_run(u0, v0, X, R, k, ds, n, 1e-3, 1.0, 2e-5, 1e-10, steps, tol_mode=1, smoother=0, delta_mode=0)
+    finally:
+        O.lib().or_set_spread_gather(0)
+    sec = O.lib().or_last_step_seconds() / steps
     return {"value": round(1.0 / sec, 5), "unit": "IB steps/s", "cores": 1, "kind": "port",
-            "sample": f"1 IB step of the same {n}x{n} / {X.shape[0]}-marker workload on one host core "
-                      f"({sec:.1f} s): the oracle's restatement of the reference path (lexicographic "
-                      f"Gauss-Seidel V(2,2) multigrid, CSR CG, glibc-cos delta)",
+            "sample": f"{steps} IB steps of the same {n}x{n} / {X.shape[0]}-marker workload on one host core "
+                      f"({sec:.1f} s per step): the oracle's restatement of the reference path (lexicographic "
+                      f"Gauss-Seidel V(2,2) multigrid, CSR CG, glibc-cos delta, kd-tree gather spreading)",
             "host_nproc": os.cpu_count()}
 
 

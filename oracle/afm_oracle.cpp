@@ -841,6 +841,51 @@ struct NS3 {
 }  // namespace
 
 // =================================================================== C ABI
+// Spreading as the reference computes it: every face gathers the markers in
+// the box of half-width 2/n around it from a kd-tree (ib/grid.h:68-84 over
+// ib/lagrangian_grid.h:58-67 and dt/kdtree.h:39-146: inclusive box, one
+// axis per tree level, periodic images of markers within 2/n of a seam at
+// +-1), and sums delta_x delta_y ds F over them in traversal order.  The
+// same terms as or_spread2 in another order; or_spread2's scatter is what
+// the parity tests use, this gather what the CPU baseline times.
+struct KdTree2 {
+	struct Node { double p[2]; long k; int left = -1, right = -1, axis = 0; };
+	std::vector<Node> nodes;
+	int build(std::vector<long>& ids, long lo, long hi, int axis, const double* X)
+	{
+		if (lo >= hi) return -1;
+		const long mid = lo + (hi - lo) / 2;
+		std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi,
+		                 [&](long a, long b) { return X[2 * a + axis] < X[2 * b + axis]; });
+		const int me = (int) nodes.size();
+		nodes.push_back(Node());
+		nodes[me].k = ids[mid];
+		nodes[me].p[0] = X[2 * ids[mid]];
+		nodes[me].p[1] = X[2 * ids[mid] + 1];
+		nodes[me].axis = axis;
+		const int l = build(ids, lo, mid, axis ^ 1, X);
+		const int r = build(ids, mid + 1, hi, axis ^ 1, X);
+		nodes[me].left = l;
+		nodes[me].right = r;
+		return me;
+	}
+	// visit(k, sx, sy) for every marker k whose image (X + (sx, sy)) lies in
+	// the box of half-width h around q
+	template<class Visit>
+	void query(int node, const double q[2], double h, double sx, double sy, Visit&& visit) const
+	{
+		while (node >= 0) {
+			const Node& nd = nodes[node];
+			const double c = nd.p[nd.axis] + (nd.axis ? sy : sx);
+			if (q[nd.axis] + h < c) { node = nd.left; continue; }
+			if (q[nd.axis] - h > c) { node = nd.right; continue; }
+			query(nd.right, q, h, sx, sy, visit);
+			if (std::fabs(nd.p[0] + sx - q[0]) <= h && std::fabs(nd.p[1] + sy - q[1]) <= h) visit(nd.k, sx, sy);
+			node = nd.left;
+		}
+	}
+};
+
 extern "C" {
 
 double or_delta(double r, unsigned n, int mode) { return delta(r, n, mode); }
@@ -882,6 +927,37 @@ void or_spread2(int comp, const double* F, const double* ds, const double* X, lo
 		window2(comp, X[2 * k], X[2 * k + 1], n, W, D, nx, ny, mode, idx, w);
 		for (int t = 0; t < 16; ++t) f[idx[t]] += w[t] * ds[k] * F[k];
 	}
+}
+
+void or_spread2_gather(int comp, const double* F, const double* ds, const double* X, long m,
+                       unsigned nx, unsigned ny, unsigned n, unsigned W, unsigned D, double* f, int mode)
+{
+	KdTree2 t;
+	std::vector<long> ids(m);
+	for (long k = 0; k < m; ++k) ids[k] = k;
+	t.nodes.reserve(m);
+	const int root = t.build(ids, 0, m, 0, X);
+	const double h = 2.0 / n;
+	for (unsigned j = 0; j < ny; ++j)
+		for (unsigned i = 0; i < nx; ++i) {
+			double q[2] = {(double) i / n, (double) j / n};
+			if (comp == 0) q[1] += 0.5 / n; else q[0] += 0.5 / n;
+			double value = 0.0;
+			auto add = [&](long k, double sx, double sy) {
+				const double w = delta(X[2 * k] + sx - q[0], n, mode) * delta(X[2 * k + 1] + sy - q[1], n, mode);
+				value += w * ds[k] * F[k];
+			};
+			for (int a = -1; a <= 1; ++a)
+				for (int b = -1; b <= 1; ++b) {
+					const double sx = a * (double) W, sy = b * (double) D;
+					// images one period left (right) only matter for faces within h of
+					// the left (right) seam
+					if ((a == -1 && q[0] > h) || (a == 1 && q[0] < W - h)) continue;
+					if ((b == -1 && q[1] > h) || (b == 1 && q[1] < D - h)) continue;
+					t.query(root, q, h, sx, sy, add);
+				}
+			f[i + (size_t) nx * j] = value;
+		}
 }
 
 // Per-marker window of one component: base cell, 16 indices, 16 weights.
@@ -1074,10 +1150,14 @@ void spring(const IB2& s, const vec& X, vec& Fx, vec& Fy)
 	}
 }
 
+static int g_spread_gather = 0;   // 1: the IB step spreads by the kd-tree gather
+void or_set_spread_gather(int on) { g_spread_gather = on; }
+
 void spread_both(const IB2& s, const vec& X, const vec& Fx, const vec& Fy, vec& fx, vec& fy)
 {
-	or_spread2(0, Fx.data(), s.ds.data(), X.data(), s.m, s.g.nx, s.g.ny, s.g.n, s.g.W, s.g.D, fx.data(), s.mode);
-	or_spread2(1, Fy.data(), s.ds.data(), X.data(), s.m, s.g.nx, s.g.ny, s.g.n, s.g.W, s.g.D, fy.data(), s.mode);
+	auto sp = g_spread_gather ? or_spread2_gather : or_spread2;
+	sp(0, Fx.data(), s.ds.data(), X.data(), s.m, s.g.nx, s.g.ny, s.g.n, s.g.W, s.g.D, fx.data(), s.mode);
+	sp(1, Fy.data(), s.ds.data(), X.data(), s.m, s.g.nx, s.g.ny, s.g.n, s.g.W, s.g.D, fy.data(), s.mode);
 }
 
 void interp_both(const IB2& s, const vec& u, const vec& v, const vec& X, vec& Ux, vec& Uy)

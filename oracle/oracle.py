@@ -58,6 +58,8 @@ def lib():
             "or_delta_arg_many": (None, [_P, _L, _U, _P]),
             "or_interp2": (None, [_I, _P, _U, _U, _U, _U, _U, _P, _L, _P, _I]),
             "or_spread2": (None, [_I, _P, _P, _P, _L, _U, _U, _U, _U, _U, _P, _I]),
+            "or_spread2_gather": (None, [_I, _P, _P, _P, _L, _U, _U, _U, _U, _U, _P, _I]),
+            "or_set_spread_gather": (None, [_I]),
             "or_window2": (None, [_I, _D, _D, _U, _U, _U, _U, _U, _I, _P, _P]),
             "or_interp3": (None, [_I, _P, _U, _U, _U, _U, _U, _U, _U, _P, _L, _P, _I]),
             "or_spread3": (None, [_I, _P, _P, _P, _L, _U, _U, _U, _U, _U, _U, _U, _P, _I]),
@@ -158,6 +160,14 @@ def spread2(comp, F, ds, X, nx, ny, n, mode=1):
     F, ds, X = _f64(F), _f64(ds), _f64(X)
     f = np.empty((ny, nx))
     lib().or_spread2(comp, _ptr(F), _ptr(ds), _ptr(X), X.shape[0], nx, ny, n, nx // n, ny // n, _ptr(f), mode)
+    return f
+
+
+def spread2_gather(comp, F, ds, X, nx, ny, n, mode=1):
+    """Spreading as the reference computes it: a kd-tree box gather per face."""
+    F, ds, X = _f64(F), _f64(ds), _f64(X)
+    f = np.empty((ny, nx))
+    lib().or_spread2_gather(comp, _ptr(F), _ptr(ds), _ptr(X), X.shape[0], nx, ny, n, nx // n, ny // n, _ptr(f), mode)
     return f
 
 

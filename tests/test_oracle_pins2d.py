@@ -208,3 +208,23 @@ def test_ns2_steps_converge_to_a_manufactured_solution(oracle):
     r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
     assert errs[2] < 5e-3, errs
     assert 3.4 < r1 < 4.6 and 3.4 < r2 < 4.6, (errs, r1, r2)
+
+
+@pytest.mark.parametrize("n", [16, 64])
+def test_kd_tree_gather_spreading_equals_the_scatter(oracle, n):
+    """The reference spreads by gathering, for every face, the markers in a
+    kd-tree box of half-width 2/n (dt/kdtree.h:39-146); the oracle's scatter
+    over each marker's window sums the same terms in another order.  Markers
+    at least 3h from the seams, as the reference's own tests place them (near
+    a seam the reference's gather uses periodic images while its window uses
+    the unwrapped coordinate)."""
+    rng = np.random.default_rng(70 + n)
+    m = 400
+    X = rng.uniform(3.0 / n, 1 - 3.0 / n, (m, 2))
+    X[:50] = 0.5 + rng.uniform(-0.5, 0.5, (50, 2)) / n          # a crowded cell
+    F, ds = rng.standard_normal(m), rng.uniform(0.5, 1.5, m) / n
+    for comp in (0, 1):
+        a = oracle.spread2(comp, F, ds, X, n, n, n, mode=1)
+        b = oracle.spread2_gather(comp, F, ds, X, n, n, n, mode=1)
+        assert np.array_equal(a != 0, b != 0)
+        assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(a)), comp
