@@ -119,6 +119,17 @@ __device__ inline void lap_row(int x, int y, int nx, int ny, double off, LapRow&
 	if (back != 0 && y == 0) put(back, x + nx * (ny - 1));
 }
 
+// fmod(x, p) for p a power of two (the unit-square width and depth):
+// x - p * trunc(x / p).  x / p and p * t are exact scalings and, when t != 0,
+// the difference is exact (Sterbenz), so the value is fmod's up to the sign
+// of a zero result -- which the window's fmod(fmod(q, W) + W, W) then adds to
+// W -- without fmod's remainder loop.
+__device__ inline double fmod_pow2(double x, unsigned p)
+{
+	const double pd = (double) p;
+	return x - pd * trunc(x * (1.0 / pd));
+}
+
 // ------------------------------------------------------------- IB window --
 // The 4x4 window of one MAC component around marker (cx, cy):
 // ib/eulerian_grid.h:64-93 through ib/mac_grid.h:49-73, restricted to the
@@ -128,20 +139,21 @@ __device__ inline void lap_row(int x, int y, int nx, int ny, double off, LapRow&
 __device__ inline void ib_window(int comp, double cx, double cy, unsigned n, unsigned W, unsigned D,
                                  unsigned nx, unsigned ny, int& ix0, int& iy0, double dx[4], double dy[4])
 {
+	const double h = 1.0 / n;      // n is a power of two: x / n == x * h exactly
 	double q0 = cx, q1 = cy;
-	if (comp == 0) q1 -= 0.5 / n; else q0 -= 0.5 / n;
-	q0 = fmod(fmod(q0, (double) W) + W, (double) W);
-	q1 = fmod(fmod(q1, (double) D) + D, (double) D);
+	if (comp == 0) q1 -= 0.5 * h; else q0 -= 0.5 * h;
+	q0 = fmod_pow2(fmod_pow2(q0, W) + W, W);
+	q1 = fmod_pow2(fmod_pow2(q1, D) + D, D);
 	double fx = floor(q0 * n), fy = floor(q1 * n);
 	// finite inputs give fx in [0, nx]; keep non-finite ones in bounds
 	if (!(fx >= 0.0 && fx <= (double) nx)) fx = 0.0;
 	if (!(fy >= 0.0 && fy <= (double) ny)) fy = 0.0;
-	ix0 = (int) (((size_t) fx + nx - 1) % nx);
-	iy0 = (int) (((size_t) fy + ny - 1) % ny);
+	ix0 = (int) (((unsigned) fx + nx - 1) & (nx - 1));
+	iy0 = (int) (((unsigned) fy + ny - 1) & (ny - 1));
 #pragma unroll
 	for (int d = 0; d < 4; ++d) {
-		double x = (fx + d - 1.0) / n, y = (fy + d - 1.0) / n;
-		if (comp == 0) y += 0.5 / n; else x += 0.5 / n;
+		double x = (fx + d - 1.0) * h, y = (fy + d - 1.0) * h;
+		if (comp == 0) y += 0.5 * h; else x += 0.5 * h;
 		dx[d] = delta(x - cx, n);
 		dy[d] = delta(y - cy, n);
 	}
@@ -152,22 +164,23 @@ __device__ inline void ib_window(int comp, double cx, double cy, unsigned n, uns
 __device__ inline double ib_window_factor(int comp, double cx, double cy, unsigned n, unsigned W, unsigned D,
                                           unsigned nx, unsigned ny, int d8, int& ix0, int& iy0)
 {
+	const double h = 1.0 / n;
 	double q0 = cx, q1 = cy;
-	if (comp == 0) q1 -= 0.5 / n; else q0 -= 0.5 / n;
-	q0 = fmod(fmod(q0, (double) W) + W, (double) W);
-	q1 = fmod(fmod(q1, (double) D) + D, (double) D);
+	if (comp == 0) q1 -= 0.5 * h; else q0 -= 0.5 * h;
+	q0 = fmod_pow2(fmod_pow2(q0, W) + W, W);
+	q1 = fmod_pow2(fmod_pow2(q1, D) + D, D);
 	double fx = floor(q0 * n), fy = floor(q1 * n);
 	if (!(fx >= 0.0 && fx <= (double) nx)) fx = 0.0;
 	if (!(fy >= 0.0 && fy <= (double) ny)) fy = 0.0;
-	ix0 = (int) (((size_t) fx + nx - 1) % nx);
-	iy0 = (int) (((size_t) fy + ny - 1) % ny);
+	ix0 = (int) (((unsigned) fx + nx - 1) & (nx - 1));
+	iy0 = (int) (((unsigned) fy + ny - 1) & (ny - 1));
 	if (d8 < 4) {
-		double x = (fx + d8 - 1.0) / n;
-		if (comp != 0) x += 0.5 / n;
+		double x = (fx + d8 - 1.0) * h;
+		if (comp != 0) x += 0.5 * h;
 		return delta(x - cx, n);
 	}
-	double y = (fy + (d8 - 4) - 1.0) / n;
-	if (comp == 0) y += 0.5 / n;
+	double y = (fy + (d8 - 4) - 1.0) * h;
+	if (comp == 0) y += 0.5 * h;
 	return delta(y - cy, n);
 }
 

@@ -429,9 +429,17 @@ __global__ __launch_bounds__(256) void k_spread_gather(int comp_mask, const BinR
 	for (int i = 0; i < SB; ++i) acc[i] = 0.0;
 	if (staged) merge_terms<true>(srec, lkey, lidx, head, end, X, Yj, (int) nx, ny, acc);
 	else merge_terms<false>(srec, lkey, lidx, head, end, X, Yj, (int) nx, ny, acc);
-	double* row = (comp == 0 ? fx : fy) + (size_t) nx * Yj + X;
-#pragma unroll
-	for (int i = 0; i < SB; ++i) row[i] = accumulate ? row[i] + acc[i] : acc[i];
+	// the lane's 4 cells as two 16-byte pairs (fields are 16-byte aligned and
+	// X is a multiple of 4)
+	double2* r2 = reinterpret_cast<double2*>((comp == 0 ? fx : fy) + (size_t) nx * Yj + X);
+	if (accumulate) {
+		const double2 a = r2[0], b = r2[1];
+		r2[0] = make_double2(a.x + acc[0], a.y + acc[1]);
+		r2[1] = make_double2(b.x + acc[2], b.y + acc[3]);
+	} else {
+		r2[0] = make_double2(acc[0], acc[1]);
+		r2[1] = make_double2(acc[2], acc[3]);
+	}
 }
 
 int launch_delta(Ctx* c, const double* r, long count, unsigned n, double* out)
