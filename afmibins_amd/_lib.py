@@ -46,9 +46,19 @@ SIGNATURES = {
     "afm_mg_prolong": (_I, [_P, _U, _P, _P]),
     "afm_mg_vcycle": (_I, [_P, _P, _I, _I]),
     "afm_mg_solve": (_I, [_P, _P, _D, _I, _I, _I, _I, _I, _IP, _DP]),
-    "afm_laplacian3_apply": (_I, [_P, _U, _P, _P]),
-    "afm_mg3_vcycle": (_I, [_P, _U, _P, _I, _I]),
-    "afm_mg3_solve": (_I, [_P, _U, _P, _D, _I, _I, _I, _I, _IP, _DP]),
+    "afm_laplacian3_apply": (_I, [_P, _U, _U, _P, _P]),
+    "afm_mg3_vcycle": (_I, [_P, _U, _U, _P, _I, _I, _D]),
+    "afm_mg3_restrict": (_I, [_P, _U, _U, _P, _P]),
+    "afm_mg3_prolong": (_I, [_P, _U, _U, _P, _P]),
+    "afm_mg3_smooth": (_I, [_P, _U, _U, _P, _P, _I]),
+    "afm_laplacian3_cg": (_I, [_P, _U, _U, _P, _P, _D, _I, _IP, _DP]),
+    "afm_helmholtz3_boundary": (_I, [_P, _U, _I, _D, _P, _P, _P]),
+    "afm_projection3_boundary": (_I, [_P, _U, _P, _P, _P, _P, _P, _P]),
+    "afm_csr_spmv": (_I, [_P, C.c_ulong, _P, _P, _P, _P, _P]),
+    "afm_csr_cg": (_I, [_P, C.c_ulong, _P, _P, _P, _P, _P, _D, _I, _IP, _DP]),
+    "afm_csr_smooth": (_I, [_P, C.c_ulong, _P, _P, _P, _P, _P, _I]),
+    "afm_axpy": (_I, [_P, C.c_ulong, _D, _P, _P]),
+    "afm_mg3_solve": (_I, [_P, _U, _U, _P, _D, _I, _I, _I, _I, _IP, _DP]),
     "afm_ns3_create": (_I, [_P, _U, _D, _D, _D, _D, _P, _P, _P, C.POINTER(_P)]),
     "afm_ns3_step": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "afm_ns3_get_p": (_I, [_P, _P]),

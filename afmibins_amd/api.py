@@ -162,35 +162,114 @@ class Context:
                    C.byref(rms))
         return cyc.value, rms.value
 
-    # -- the 3-D mode: n^3 cells, walls in y (shape (n, n, n) as (z, y, x)) --
-    def _f3(self, t, name):
+    # -- the 3-D mode: cubes of s = width * 2^levels cells per axis, walls in
+    # y (shape (s, s, s) as (z, y, x)); width 1 is geometry<L,1,1,1> --
+    def _f3(self, t, name, width=1):
         if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or t.device != self.device \
                 or not t.is_contiguous() or t.dim() != 3 or len(set(t.shape)) != 1:
-            raise TypeError(f"{name} must be a contiguous float64 (n, n, n) tensor on {self.device}")
-        n = t.shape[0]
-        if n < 2 or n & (n - 1):
-            raise ValueError(f"{name}: n = {n} is not a power of two >= 2")
-        return n.bit_length() - 1
+            raise TypeError(f"{name} must be a contiguous float64 (s, s, s) tensor on {self.device}")
+        s = t.shape[0]
+        if width < 1 or width & (width - 1) or s % width or (s // width) & (s // width - 1):
+            raise ValueError(f"{name}: {s} cells per axis is not width * 2^levels for width {width}")
+        return (s // width).bit_length() - 1
 
-    def laplacian3(self, x):
-        L = self._f3(x, "x")
+    def laplacian3(self, x, width=1):
+        L = self._f3(x, "x", width)
         y = torch.empty_like(x)
-        self._call("afm_laplacian3_apply", L, _ptr(x), _ptr(y))
+        self._call("afm_laplacian3_apply", L, width, _ptr(x), _ptr(y))
         return y
 
-    def mg3_vcycle(self, rhs, nu1=2, nu2=2):
-        """One V-cycle recurse(levels, rhs) of the 3-D solver, in place."""
-        L = self._f3(rhs, "rhs")
-        self._call("afm_mg3_vcycle", L, _ptr(rhs), nu1, nu2)
+    def mg3_vcycle(self, rhs, nu1=2, nu2=2, width=1, tol=0.0):
+        """One V-cycle recurse(levels, rhs) of the 3-D solver, in place (tol:
+        the coarsest level's CG tolerance when width > 1)."""
+        L = self._f3(rhs, "rhs", width)
+        self._call("afm_mg3_vcycle", L, width, _ptr(rhs), nu1, nu2, tol)
         return rhs
 
-    def mg3_solve(self, rhs, tol=1e-10, tol_mode=1, nu1=2, nu2=2, max_cycles=200):
+    def mg3_solve(self, rhs, tol=1e-10, tol_mode=1, nu1=2, nu2=2, max_cycles=200, width=1):
         """In place: rhs <- solution of the 3-D walled problem.  Returns
         (cycles, final RMS residual)."""
-        L = self._f3(rhs, "rhs")
+        L = self._f3(rhs, "rhs", width)
         cyc, rms = C.c_int(), C.c_double()
-        self._call("afm_mg3_solve", L, _ptr(rhs), tol, tol_mode, nu1, nu2, max_cycles, C.byref(cyc), C.byref(rms))
+        self._call("afm_mg3_solve", L, width, _ptr(rhs), tol, tol_mode, nu1, nu2, max_cycles, C.byref(cyc),
+                   C.byref(rms))
         return cyc.value, rms.value
+
+    def mg3_restrict(self, fine, width=1):
+        L = self._f3(fine, "fine", width)
+        s = fine.shape[0] // 2
+        coarse = torch.empty((s, s, s), dtype=torch.float64, device=self.device)
+        self._call("afm_mg3_restrict", L, width, _ptr(fine), _ptr(coarse))
+        return coarse
+
+    def mg3_prolong(self, coarse, width=1):
+        s = coarse.shape[0] * 2
+        fine = torch.empty((s, s, s), dtype=torch.float64, device=self.device)
+        L = self._f3(fine, "fine", width)
+        self._call("afm_mg3_prolong", L, width, _ptr(self._f(coarse, (s // 2,) * 3, "coarse")), _ptr(fine))
+        return fine
+
+    def mg3_smooth(self, x, b, sweeps=1, width=1):
+        L = self._f3(x, "x", width)
+        self._f(b, tuple(x.shape), "b")
+        self._call("afm_mg3_smooth", L, width, _ptr(x), _ptr(b), sweeps)
+        return x
+
+    def laplacian3_cg(self, b, x, tol=1e-10, max_iters=100000, width=1):
+        L = self._f3(x, "x", width)
+        self._f(b, tuple(x.shape), "b")
+        it, nrm = C.c_int(), C.c_double()
+        self._call("afm_laplacian3_cg", L, width, _ptr(b), _ptr(x), tol, max_iters, C.byref(it), C.byref(nrm))
+        return it.value, nrm.value
+
+    def helmholtz3_boundary(self, kind, scale, top, bottom):
+        """helmholtz_periodic (kind 1) / helmholtz_dirichlet (kind 3)
+        boundary(top, bottom): the wall terms of the right-hand side."""
+        n = top.shape[0]
+        self._f(top, (n, n), "top")
+        self._f(bottom, (n, n), "bottom")
+        out = torch.empty((n, n if kind == 1 else n - 1, n), dtype=torch.float64, device=self.device)
+        self._call("afm_helmholtz3_boundary", n.bit_length() - 1, kind, scale, _ptr(top), _ptr(bottom), _ptr(out))
+        return out
+
+    def projection3_boundary(self, phx, phz, ut, ub, wt, wb):
+        """projection::boundary: the wall vectors corrected in place."""
+        n = phx.shape[0]
+        self._f3(phx, "phx")
+        self._f3(phz, "phz")
+        for t, nm in ((ut, "ut"), (ub, "ub"), (wt, "wt"), (wb, "wb")):
+            self._f(t, (n, n), nm)
+        self._call("afm_projection3_boundary", n.bit_length() - 1, _ptr(phx), _ptr(phz), _ptr(ut), _ptr(ub),
+                   _ptr(wt), _ptr(wb))
+
+    # -- dt::matrix in device CSR form (int64 row starts and columns) --
+    def _csr(self, rs, col, val):
+        for t, nm, dt in ((rs, "row_start", torch.int64), (col, "column", torch.int64), (val, "values", torch.float64)):
+            if not isinstance(t, torch.Tensor) or t.dtype != dt or t.device != self.device or not t.is_contiguous():
+                raise TypeError(f"{nm} must be a contiguous {dt} tensor on {self.device}")
+        return rs.numel() - 1
+
+    def csr_spmv(self, rs, col, val, x):
+        rows = self._csr(rs, col, val)
+        y = torch.empty(rows, dtype=torch.float64, device=self.device)
+        self._call("afm_csr_spmv", rows, _ptr(rs), _ptr(col), _ptr(val), _ptr(x), _ptr(y))
+        return y
+
+    def csr_cg(self, rs, col, val, b, x, tol=1e-10, max_iters=100000):
+        rows = self._csr(rs, col, val)
+        it, nrm = C.c_int(), C.c_double()
+        self._call("afm_csr_cg", rows, _ptr(rs), _ptr(col), _ptr(val), _ptr(b), _ptr(x), tol, max_iters,
+                   C.byref(it), C.byref(nrm))
+        return it.value, nrm.value
+
+    def csr_smooth(self, rs, col, val, x, b, sweeps=1):
+        rows = self._csr(rs, col, val)
+        self._call("afm_csr_smooth", rows, _ptr(rs), _ptr(col), _ptr(val), _ptr(x), _ptr(b), sweeps)
+        return x
+
+    def axpy(self, a, x, y):
+        self._call("afm_axpy", x.numel(), a, _ptr(x), _ptr(y))
+        return y
 
     # -- 3-D wall-bounded Navier-Stokes operators: u, w, p (n, n, n), v
     # (n, n-1, n) on the interior y-faces, wall vectors (n, n), all [z, y, x]

@@ -298,37 +298,79 @@ int afm_mg_vcycle(afm_ctx* c, double* rhs, int nu1, int nu2)
 	return mg_vcycle(c, rhs, nu1, nu2);
 }
 
-// the 3-D mode: n = 2^levels per axis, 1 <= levels <= 9 (512^3)
-#define CHECK_MG3(c, levels, nu1, nu2)                                                                  \
+// the 3-D mode: geometry<levels, width, width, width>, width 2^levels cells
+// per axis; width a power of two, at most 512 cells per axis
+#define CHECK_MG3(c, levels, width, nu1, nu2)                                                           \
 	CHECK_ARG(c, "null context");                                                                       \
-	CHECK_ARG((levels) >= 1 && (levels) <= 9, "levels must be 1..9 (n = 2^levels cells per axis)");   \
+	CHECK_ARG((levels) <= 9 && (width) >= 1 && ((width) & ((width) - 1)) == 0,                         \
+	          "levels must be 0..9 and width a power of two");                                         \
+	CHECK_ARG(((size_t) (width) << (levels)) <= 512 && ((size_t) (width) << (levels)) >= 2,            \
+	          "the cube must have 2..512 cells per axis");                                             \
 	CHECK_ARG((nu1) >= 0 && (nu1) <= 16 && (nu2) >= 0 && (nu2) <= 16, "nu1, nu2 must be 0..16")
 
-int afm_laplacian3_apply(afm_ctx* c, unsigned levels, const double* x, double* y)
+int afm_laplacian3_apply(afm_ctx* c, unsigned levels, unsigned width, const double* x, double* y)
 {
-	CHECK_MG3(c, levels, 0, 0);
+	CHECK_MG3(c, levels, width, 0, 0);
 	CHECK_ARG(x && y && x != y, "null or aliased argument");
 	AFM_TRY(use(c));
-	return mg3_laplacian(c, levels, x, y);
+	return mg3_laplacian(c, levels, width, x, y);
 }
 
-int afm_mg3_vcycle(afm_ctx* c, unsigned levels, double* rhs, int nu1, int nu2)
+int afm_mg3_vcycle(afm_ctx* c, unsigned levels, unsigned width, double* rhs, int nu1, int nu2, double tol)
 {
-	CHECK_MG3(c, levels, nu1, nu2);
+	CHECK_MG3(c, levels, width, nu1, nu2);
 	CHECK_ARG(rhs, "null argument");
+	CHECK_ARG(tol >= 0, "tol must be >= 0");
 	AFM_TRY(use(c));
-	return mg3_vcycle(c, levels, rhs, nu1, nu2);
+	return mg3_vcycle(c, levels, width, rhs, nu1, nu2, tol);
 }
 
-int afm_mg3_solve(afm_ctx* c, unsigned levels, double* rhs, double tol, int tol_mode, int nu1, int nu2, int max_cycles,
-                  int* cycles, double* final_rms)
+int afm_mg3_solve(afm_ctx* c, unsigned levels, unsigned width, double* rhs, double tol, int tol_mode, int nu1,
+                  int nu2, int max_cycles, int* cycles, double* final_rms)
 {
-	CHECK_MG3(c, levels, nu1, nu2);
+	CHECK_MG3(c, levels, width, nu1, nu2);
 	CHECK_ARG(rhs, "null argument");
 	CHECK_ARG(tol_mode == 0 || tol_mode == 1, "tol_mode must be 0 or 1");
 	CHECK_ARG(tol >= 0 && max_cycles >= 0, "tol and max_cycles must be >= 0");
 	AFM_TRY(use(c));
-	return mg3_solve(c, levels, rhs, tol, tol_mode, nu1, nu2, max_cycles, cycles, final_rms);
+	return mg3_solve(c, levels, width, rhs, tol, tol_mode, nu1, nu2, max_cycles, cycles, final_rms);
+}
+
+int afm_mg3_restrict(afm_ctx* c, unsigned levels, unsigned width, const double* fine, double* coarse)
+{
+	CHECK_MG3(c, levels, width, 0, 0);
+	CHECK_ARG(levels >= 1, "restriction needs levels >= 1");
+	CHECK_ARG(fine && coarse && fine != coarse, "null or aliased argument");
+	AFM_TRY(use(c));
+	return mg3_restrict(c, levels, width, fine, coarse);
+}
+
+int afm_mg3_prolong(afm_ctx* c, unsigned levels, unsigned width, const double* coarse, double* fine)
+{
+	CHECK_MG3(c, levels, width, 0, 0);
+	CHECK_ARG(levels >= 1, "prolongation needs levels >= 1");
+	CHECK_ARG(fine && coarse && fine != coarse, "null or aliased argument");
+	AFM_TRY(use(c));
+	return mg3_prolong(c, levels, width, coarse, fine);
+}
+
+int afm_mg3_smooth(afm_ctx* c, unsigned levels, unsigned width, double* x, const double* b, int sweeps)
+{
+	CHECK_MG3(c, levels, width, 0, 0);
+	CHECK_ARG(x && b && x != b, "null or aliased argument");
+	CHECK_ARG(sweeps >= 0, "sweeps must be >= 0");
+	AFM_TRY(use(c));
+	return mg3_smooth(c, levels, width, x, b, sweeps);
+}
+
+int afm_laplacian3_cg(afm_ctx* c, unsigned levels, unsigned width, const double* b, double* x, double tol,
+                      int max_iters, int* iters, double* final_norm)
+{
+	CHECK_MG3(c, levels, width, 0, 0);
+	CHECK_ARG(x && b, "null argument");
+	CHECK_ARG(tol >= 0 && max_iters >= 0, "tol and max_iters must be >= 0");
+	AFM_TRY(use(c));
+	return mg3_cg(c, levels, width, b, x, tol, max_iters, iters, final_norm);
 }
 
 #define CHECK_NS3_LEVELS(levels) CHECK_ARG((levels) >= 1 && (levels) <= 8, "levels must be 1..8 (n = 2^levels)")
@@ -423,10 +465,63 @@ int afm_helmholtz3_cg(afm_ctx* c, unsigned levels, int kind, double scale, const
 	return ns3_helmholtz_cg(c, levels, kind, scale, b, x, tol, max_iters, iters, final_norm);
 }
 
+int afm_helmholtz3_boundary(afm_ctx* c, unsigned levels, int kind, double scale, const double* top,
+                            const double* bottom, double* out)
+{
+	CHECK_ARG(c && top && bottom && out, "null argument");
+	CHECK_NS3_LEVELS(levels);
+	CHECK_ARG(kind == 1 || kind == 3, "kind must be 1 (helmholtz_periodic) or 3 (helmholtz_dirichlet)");
+	AFM_TRY(use(c));
+	return ns3_helmholtz_boundary(c, levels, kind, scale, top, bottom, out);
+}
+
+int afm_projection3_boundary(afm_ctx* c, unsigned levels, const double* phx, const double* phz, double* ut,
+                             double* ub, double* wt, double* wb)
+{
+	CHECK_ARG(c && phx && phz && ut && ub && wt && wb, "null argument");
+	CHECK_NS3_LEVELS(levels);
+	AFM_TRY(use(c));
+	return ns3_projection_boundary(c, levels, phx, phz, ut, ub, wt, wb);
+}
+
+int afm_csr_spmv(afm_ctx* c, unsigned long rows, const unsigned long* row_start, const unsigned long* column,
+                 const double* values, const double* x, double* y)
+{
+	CHECK_ARG(c && (rows == 0 || (row_start && x && y && x != y)), "null or aliased argument");
+	AFM_TRY(use(c));
+	return csr_spmv(c, rows, row_start, column, values, x, y);
+}
+
+int afm_csr_cg(afm_ctx* c, unsigned long rows, const unsigned long* row_start, const unsigned long* column,
+               const double* values, const double* b, double* x, double tol, int max_iters, int* iters,
+               double* final_norm)
+{
+	CHECK_ARG(c && (rows == 0 || (row_start && b && x)), "null argument");
+	CHECK_ARG(tol >= 0 && max_iters >= 0, "tol and max_iters must be >= 0");
+	AFM_TRY(use(c));
+	return csr_cg(c, rows, row_start, column, values, b, x, tol, max_iters, iters, final_norm);
+}
+
+int afm_csr_smooth(afm_ctx* c, unsigned long rows, const unsigned long* row_start, const unsigned long* column,
+                   const double* values, double* x, const double* b, int sweeps)
+{
+	CHECK_ARG(c && (rows == 0 || (row_start && x && b && x != b)), "null or aliased argument");
+	CHECK_ARG(sweeps >= 0, "sweeps must be >= 0");
+	AFM_TRY(use(c));
+	return csr_smooth(c, rows, row_start, column, values, x, b, sweeps);
+}
+
+int afm_axpy(afm_ctx* c, unsigned long count, double a, const double* x, double* y)
+{
+	CHECK_ARG(c && (count == 0 || (x && y)), "null argument");
+	AFM_TRY(use(c));
+	return vec_axpy(c, count, a, x, y);
+}
+
 #define CHECK_IB3(c, levels, comp, m)                                                                 \
 	CHECK_ARG(c, "null context");                                                                     \
 	CHECK_NS3_LEVELS(levels);                                                                         \
-	CHECK_ARG((comp) >= 0 && (comp) <= 2, "comp must be 0 (u), 1 (v) or 2 (w)");                      \
+	CHECK_ARG((comp) >= 0 && (comp) <= 3, "comp must be 0 (u), 1 (v), 2 (w) or 3 (nodes)");           \
 	CHECK_ARG((m) >= 0 && (m) <= (1L << 24), "m must be 0..2^24 markers")
 
 int afm_interp3(afm_ctx* c, unsigned levels, int comp, const double* field, const double* X, long m, double* U)

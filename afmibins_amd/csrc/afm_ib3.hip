@@ -4,7 +4,8 @@
 // component, with the windows of ib/eulerian_grid.h:71-89 and
 // ib/mac_grid.h:60-73 (u: comp 0, w: comp 2; n^3 faces) and of the v grid
 // mac_grid<y> (ib/mac_grid.h:114-132; comp 1, the n*(n-1)*n interior
-// y-faces).  Taps the grid does not store -- the v window's wall faces and
+// y-faces), or the node grid eulerian_grid itself (comp 3: the window of
+// ib/eulerian_grid.h:64-93 unshifted, nodes at (i, j, k)/n, n^3 values).  Taps the grid does not store -- the v window's wall faces and
 // the u / w windows' ghost row above the top wall, which the reference
 // indexes out of range -- read nothing and receive nothing.
 //
@@ -33,7 +34,7 @@ struct Win3 {
 __device__ inline Win3 win3_base(int comp, const double c[3], unsigned n)
 {
 	double q[3] = {c[0], c[1], c[2]};
-	if (comp != 1) {
+	if (comp == 0 || comp == 2) {
 		q[(comp + 1) % 3] -= 0.5 / n;
 		q[(comp + 2) % 3] -= 0.5 / n;
 	}
@@ -65,8 +66,10 @@ __device__ inline unsigned win3_tap(int comp, const Win3& b, const double c[3], 
 		idx = ix + nx * (iy - 1 + (ny - 1) * iz);
 	} else {
 		idx = ix + nx * (iy - 1 + ny * iz);
-		p[(comp + 1) % 3] += 0.5 / n;
-		p[(comp + 2) % 3] += 0.5 / n;
+		if (comp != 3) {
+			p[(comp + 1) % 3] += 0.5 / n;
+			p[(comp + 2) % 3] += 0.5 / n;
+		}
 	}
 	w = delta(p[0] - c[0], n) * delta(p[1] - c[1], n) * delta(p[2] - c[2], n);
 	return (unsigned) idx;

@@ -157,14 +157,20 @@ int mg_smooth(Ctx* c, unsigned level, const double* x_in, const double* b, doubl
 int mg_restrict(Ctx* c, unsigned level, const double* fine, double* coarse);
 int mg_prolong(Ctx* c, unsigned level, const double* coarse, double* fine);
 int mg_vcycle(Ctx* c, double* rhs_inout, int nu1, int nu2);
-// The 3-D mode (afm_mg3.hip): n x n x n cells, n = 2^L, walls in y
-int mg3_laplacian(Ctx* c, unsigned L, const double* x, double* y);
-int mg3_vcycle(Ctx* c, unsigned L, double* rhs, int nu1, int nu2);
-int mg3_solve(Ctx* c, unsigned L, double* rhs, double tol, int tol_mode, int nu1, int nu2, int max_cycles,
-              int* cycles, double* final_rms);
+// The 3-D mode (afm_mg3.hip): cubes of W 2^L cells per side (W a power of
+// two), laplacian scale 2^L, walls in y
+int mg3_laplacian(Ctx* c, unsigned L, unsigned W, const double* x, double* y);
+int mg3_vcycle(Ctx* c, unsigned L, unsigned W, double* rhs, int nu1, int nu2, double tol);
+int mg3_solve(Ctx* c, unsigned L, unsigned W, double* rhs, double tol, int tol_mode, int nu1, int nu2,
+              int max_cycles, int* cycles, double* final_rms);
+int mg3_restrict(Ctx* c, unsigned L, unsigned W, const double* fine, double* coarse);
+int mg3_prolong(Ctx* c, unsigned L, unsigned W, const double* coarse, double* fine);
+int mg3_smooth(Ctx* c, unsigned L, unsigned W, double* x, const double* b, int sweeps);
+int mg3_cg(Ctx* c, unsigned L, unsigned W, const double* b, double* x, double tol, int max_iters, int* iters,
+           double* final_norm);
 // a persistent 3-D multigrid workspace (level buffers, iterate, residual)
 struct Mg3;
-int mg3_create(Ctx* c, unsigned L, int nu1, int nu2, Mg3** out);
+int mg3_create(Ctx* c, unsigned L, unsigned W, int nu1, int nu2, Mg3** out);
 int mg3_solve_with(Mg3* m, double* rhs, double tol, int tol_mode, int max_cycles, int* cycles, double* final_rms);
 void mg3_destroy(Mg3* m);
 // the 3-D Navier-Stokes solver (afm_ns3.hip)
@@ -182,6 +188,18 @@ int ns3_grad(Ctx* c, unsigned L, const double* p, double* px, double* py, double
 int ns3_advect(Ctx* c, unsigned L, const double* u, const double* v, const double* w, double* ua, double* va,
                double* wa);
 int ns3_helmholtz_apply(Ctx* c, unsigned L, int kind, double scale, const double* x, double* y);
+int ns3_helmholtz_boundary(Ctx* c, unsigned L, int kind, double scale, const double* top, const double* bottom,
+                           double* out);
+int ns3_projection_boundary(Ctx* c, unsigned L, const double* phx, const double* phz, double* ut, double* ub,
+                            double* wt, double* wb);
+// device CSR matrices and vectors (afm_csr.hip)
+int csr_spmv(Ctx* c, unsigned long rows, const unsigned long* rs, const unsigned long* col, const double* val,
+             const double* x, double* y);
+int csr_smooth(Ctx* c, unsigned long rows, const unsigned long* rs, const unsigned long* col, const double* val,
+               double* x, const double* b, int sweeps);
+int csr_cg(Ctx* c, unsigned long rows, const unsigned long* rs, const unsigned long* col, const double* val,
+           const double* b, double* x, double tol, int max_iters, int* iters, double* final_norm);
+int vec_axpy(Ctx* c, unsigned long N, double a, const double* x, double* y);
 int ns3_helmholtz_cg(Ctx* c, unsigned L, int kind, double scale, const double* b, double* x, double tol,
                      int max_iters, int* iters, double* final_norm);
 // the IB transfer on the 3-D MAC grid (afm_ib3.hip)

@@ -1,7 +1,10 @@
 // afm_mg3.hip -- the reference's native 3-D mode of the multigrid Poisson
-// solve: mg::solver<G> (mg/solver.h:35-271) on an n x n x n cell grid,
-// periodic in x and z, walls (homogeneous Neumann) in y -- geometry<L,1,1,1>
-// of dt/geometry.h, the geometry of the reference's own test/mg_test3.cc.
+// solve: mg::solver<G> (mg/solver.h:35-271) on a cube of s x s x s cells,
+// s = c 2^L, periodic in x and z, walls (homogeneous Neumann) in y --
+// geometry<L,c,c,c> of dt/geometry.h (c a power of two): <L,1,1,1> is the
+// geometry of the reference's test/mg_test3.cc, <1,8,8,8> that of
+// test/mg_test2.cc.  Level l has s_l = c 2^l cells per side and laplacian
+// scale n_l = 2^l.
 //
 // One thread per cell, global memory: this is the parity path for the
 // reference's own geometry, not the 2-D benchmark path, and it is not tiled.
@@ -17,8 +20,10 @@
 //                the wall rows, written with the reference's own index
 //                expression for those rows (it names ny where nx belongs;
 //                equal on cubes, which is why only cubes are accepted);
-//   level 0      mg/solver.h:264-271: mean removal and CG on the 1-cell grid,
-//                whose laplacian has no stored entry, so the result is 0.
+//   level 0      mg/solver.h:264-271: mean removal and CG (alg/cg.h:32-56,
+//                the right-hand side as the start) on the c^3 grid with
+//                n = 1; for c = 1 the laplacian has no stored entry, so the
+//                result is 0.
 // The solve (mg/solver.h:218-237): nested start, mean removal, V-cycles on
 // the residual until RMS(r) <= tol, with fixed-order GPU reductions (the
 // oracle sums sequentially, so solves agree to rounding, V-cycles exactly).
@@ -70,15 +75,16 @@ __device__ inline void cell3(size_t i, unsigned nx, unsigned ny, unsigned& x, un
 }
 
 // y = L x (dt/matrix.h:81-94: the row sum starts at 0, terms in column order)
-__global__ void k_lap3(const double* __restrict__ x, double* __restrict__ y, unsigned n)
+// on s^3 cells with laplacian scale n
+__global__ void k_lap3(const double* __restrict__ x, double* __restrict__ y, unsigned s, unsigned n)
 {
-	const size_t N = (size_t) n * n * n;
+	const size_t N = (size_t) s * s * s;
 	const size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
 	if (i >= N) return;
 	unsigned a, b, c;
-	cell3(i, n, n, a, b, c);
+	cell3(i, s, s, a, b, c);
 	Row7 r;
-	lap3_row(a, b, c, n, n, n, (double) n, r);
+	lap3_row(a, b, c, s, s, s, (double) n, r);
 	double v = 0.0;
 	for (int k = 0; k < r.cnt; ++k) v += r.a[k] * x[r.col[k]];
 	y[i] = v;
@@ -86,31 +92,31 @@ __global__ void k_lap3(const double* __restrict__ x, double* __restrict__ y, uns
 
 // r = b - L x
 __global__ void k_resid3(const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r,
-                         unsigned n)
+                         unsigned s, unsigned n)
 {
-	const size_t N = (size_t) n * n * n;
+	const size_t N = (size_t) s * s * s;
 	const size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
 	if (i >= N) return;
 	unsigned a, bb, c;
-	cell3(i, n, n, a, bb, c);
+	cell3(i, s, s, a, bb, c);
 	Row7 row;
-	lap3_row(a, bb, c, n, n, n, (double) n, row);
+	lap3_row(a, bb, c, s, s, s, (double) n, row);
 	double v = 0.0;
 	for (int k = 0; k < row.cnt; ++k) v += row.a[k] * x[row.col[k]];
 	r[i] = b[i] - v;
 }
 
 // rows of one colour ((x + y + z) & 1 == colour), alg/smooth.h:39-50
-__global__ void k_rb3(double* __restrict__ x, const double* __restrict__ b, unsigned n, unsigned colour)
+__global__ void k_rb3(double* __restrict__ x, const double* __restrict__ b, unsigned s, unsigned n, unsigned colour)
 {
-	const size_t N = (size_t) n * n * n;
+	const size_t N = (size_t) s * s * s;
 	const size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
 	if (i >= N) return;
 	unsigned a, bb, c;
-	cell3(i, n, n, a, bb, c);
+	cell3(i, s, s, a, bb, c);
 	if (((a + bb + c) & 1u) != colour) return;
 	Row7 row;
-	lap3_row(a, bb, c, n, n, n, (double) n, row);
+	lap3_row(a, bb, c, s, s, s, (double) n, row);
 	double value = 0.0, diagonal = 0.0;
 	for (int k = 0; k < row.cnt; ++k) {
 		if (row.col[k] == i) { diagonal = row.a[k]; continue; }
@@ -241,9 +247,38 @@ __global__ void k_sub3(double* __restrict__ x, const double* __restrict__ c, dou
 	if (i < N) x[i] -= (*c * scale) * 1.0;
 }
 
+// partial sums of a[i] * b[i]
+__global__ __launch_bounds__(256) void k_dot3(const double* __restrict__ a, const double* __restrict__ b, size_t N,
+                                              double* partials)
+{
+	double acc[1] = {0.0};
+	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) acc[0] += a[i] * b[i];
+	reduce_partial<1>(acc, partials);
+}
+
+// alg/cg.h:45-47: x += nu p; z *= nu; r -= z
+__global__ void k_cg3_step(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ p,
+                           double* __restrict__ z, double nu, size_t N)
+{
+	const size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+	if (i >= N) return;
+	x[i] += nu * p[i];
+	const double zi = z[i] * nu;
+	z[i] = zi;
+	r[i] -= zi;
+}
+
+// alg/cg.h:51-52: p *= mu; p += r
+__global__ void k_cg3_dir(double* __restrict__ p, const double* __restrict__ r, double mu, size_t N)
+{
+	const size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+	if (i >= N) return;
+	p[i] = p[i] * mu;
+	p[i] = p[i] + r[i];
+}
+
 // ------------------------------------------------------------- host side --
 static inline dim3 g3(size_t N) { return dim3((unsigned) ((N + 255) / 256)); }
-static inline size_t cells3(unsigned l) { return (size_t) 1 << (3 * l); }
 
 static unsigned red3(size_t N)
 {
@@ -257,10 +292,14 @@ static unsigned red3(size_t N)
 // copy (nested).  A level's rhs passed to recurse3 must not be its R.
 struct Mg3Ctx {
 	Ctx* c;
-	unsigned L;
+	unsigned L, W;           // levels, cube width c (cells per side at level 0)
 	int nu1, nu2;
+	double tol0 = 0.0;       // level 0's CG tolerance (the solve's absolute one)
 	std::vector<double*> X, B, R;
+	double* Z0 = nullptr;    // level 0's CG direction product
 	double* scal;
+	unsigned side(unsigned l) const { return W << l; }
+	size_t cells(unsigned l) const { const size_t s = side(l); return s * s * s; }
 };
 
 // The workspace of mg3_solve_with: the levels, plus the iterate, the
@@ -281,12 +320,67 @@ static int sum3(Ctx* c, const double* x, size_t N, int sq, double* out)
 	return 0;
 }
 
-static int smooth3(Ctx* c, double* x, const double* b, unsigned l)
+// one red-black sweep of L x = b on s^3 cells with scale n
+static int smooth3(Ctx* c, double* x, const double* b, unsigned s, unsigned n)
 {
-	const unsigned n = 1u << l;
+	const size_t N = (size_t) s * s * s;
 	for (unsigned colour = 0; colour < 2; ++colour)
-		hipLaunchKernelGGL(k_rb3, g3(cells3(l)), dim3(256), 0, c->stream, x, b, n, colour);
+		hipLaunchKernelGGL(k_rb3, g3(N), dim3(256), 0, c->stream, x, b, s, n, colour);
 	AFM_LAUNCH_CHECK();
+	return 0;
+}
+
+static int read_scalar(Ctx* c, const double* d, double* h)
+{
+	AFM_HIP(hipMemcpyAsync(h, d, 8, hipMemcpyDeviceToHost, c->stream));
+	AFM_HIP(hipStreamSynchronize(c->stream));
+	return 0;
+}
+
+static int dot3(Ctx* c, const double* a, const double* b, size_t N, double* dev, double* host)
+{
+	hipLaunchKernelGGL(k_dot3, dim3(red3(N)), dim3(256), 0, c->stream, a, b, N, c->partials);
+	hipLaunchKernelGGL(k_sum3_fin, dim3(1), dim3(256), 0, c->stream, red3(N), c->partials, dev);
+	AFM_LAUNCH_CHECK();
+	return read_scalar(c, dev, host);
+}
+
+// alg::conjugate_gradient (alg/cg.h:32-56) on the laplacian of s^3 cells with
+// scale n: x holds the start and receives the result; r, p, z are scratch of
+// s^3 doubles; stop when sqrt(r.r) <= tol.  Returns -4 past max_iters.
+static int cg3_lap(Ctx* c, unsigned s, unsigned n, double* x, const double* b, double* r, double* p, double* z,
+                   double* scal, double tol, int max_iters, int* iters, double* final_norm)
+{
+	const size_t N = (size_t) s * s * s;
+	hipLaunchKernelGGL(k_resid3, g3(N), dim3(256), 0, c->stream, x, b, r, s, n);
+	hipLaunchKernelGGL(k_copy3, g3(N), dim3(256), 0, c->stream, p, r, N);
+	AFM_LAUNCH_CHECK();
+	double n0 = 0.0, pz = 0.0;
+	AFM_TRY(dot3(c, r, r, N, scal, &n0));
+	int count = 0;
+	while (sqrt(n0) > tol) {
+		if (count >= max_iters) {
+			if (final_norm) *final_norm = sqrt(n0);
+			if (iters) *iters = count;
+			set_error("3-D CG did not converge within max_iters");
+			return -4;
+		}
+		hipLaunchKernelGGL(k_lap3, g3(N), dim3(256), 0, c->stream, p, z, s, n);
+		AFM_LAUNCH_CHECK();
+		AFM_TRY(dot3(c, p, z, N, scal, &pz));
+		const double nu = n0 / pz;
+		hipLaunchKernelGGL(k_cg3_step, g3(N), dim3(256), 0, c->stream, x, r, p, z, nu, N);
+		AFM_LAUNCH_CHECK();
+		double n1 = 0.0;
+		AFM_TRY(dot3(c, r, r, N, scal, &n1));
+		const double mu = n1 / n0;
+		n0 = n1;
+		hipLaunchKernelGGL(k_cg3_dir, g3(N), dim3(256), 0, c->stream, p, r, mu, N);
+		AFM_LAUNCH_CHECK();
+		++count;
+	}
+	if (iters) *iters = count;
+	if (final_norm) *final_norm = sqrt(n0);
 	return 0;
 }
 
@@ -294,28 +388,36 @@ static int smooth3(Ctx* c, double* x, const double* b, unsigned l)
 static int recurse3(Mg3Ctx& m, unsigned l, double* rhs)
 {
 	Ctx* c = m.c;
-	const size_t N = cells3(l);
+	const size_t N = m.cells(l);
 	if (l == 0) {
-		// mean removal of the 1-cell grid leaves 0; CG stops at once (no entry)
-		hipLaunchKernelGGL(k_fill3, dim3(1), dim3(64), 0, c->stream, rhs, 0.0, (size_t) 1);
+		if (m.W == 1) {
+			// mean removal of the 1-cell grid leaves 0; CG stops at once (no entry)
+			hipLaunchKernelGGL(k_fill3, dim3(1), dim3(64), 0, c->stream, rhs, 0.0, (size_t) 1);
+			AFM_LAUNCH_CHECK();
+			return 0;
+		}
+		// mg/solver.h:264-271: rhs -= mean; CG with rhs as start and right-hand side
+		AFM_TRY(sum3(c, rhs, N, 0, m.scal));
+		hipLaunchKernelGGL(k_sub3, g3(N), dim3(256), 0, c->stream, rhs, m.scal, 1.0 / (double) N, N);
+		hipLaunchKernelGGL(k_copy3, g3(N), dim3(256), 0, c->stream, m.B[0], rhs, N);
 		AFM_LAUNCH_CHECK();
-		return 0;
+		return cg3_lap(c, m.W, 1, rhs, m.B[0], m.R[0], m.X[0], m.Z0, m.scal + 2, m.tol0, 100000, nullptr, nullptr);
 	}
 	double* x = m.X[l];
 	double* rf = m.R[l];
 	hipLaunchKernelGGL(k_fill3, g3(N), dim3(256), 0, c->stream, x, 0.0, N);
-	for (int j = 0; j < m.nu1; ++j) AFM_TRY(smooth3(c, x, rhs, l));
-	hipLaunchKernelGGL(k_resid3, g3(N), dim3(256), 0, c->stream, x, rhs, rf, 1u << l);
-	hipLaunchKernelGGL(k_restrict3, g3(cells3(l - 1)), dim3(256), 0, c->stream, rf, m.B[l - 1], 1u << l);
+	for (int j = 0; j < m.nu1; ++j) AFM_TRY(smooth3(c, x, rhs, m.side(l), 1u << l));
+	hipLaunchKernelGGL(k_resid3, g3(N), dim3(256), 0, c->stream, x, rhs, rf, m.side(l), 1u << l);
+	hipLaunchKernelGGL(k_restrict3, g3(m.cells(l - 1)), dim3(256), 0, c->stream, rf, m.B[l - 1], m.side(l));
 	AFM_LAUNCH_CHECK();
 	AFM_TRY(recurse3(m, l - 1, m.B[l - 1]));
-	const unsigned nf = 1u << l, h = nf / 2;
+	const unsigned nf = m.side(l), h = nf / 2;
 	if (h >= 2)
 		hipLaunchKernelGGL(k_prolong3_interior, g3((size_t) h * (h - 1) * h), dim3(256), 0, c->stream, m.B[l - 1], rf, nf);
 	hipLaunchKernelGGL(k_prolong3_walls, g3((size_t) h * h), dim3(256), 0, c->stream, m.B[l - 1], rf, nf);
 	hipLaunchKernelGGL(k_axpy3, g3(N), dim3(256), 0, c->stream, x, rf, N);
 	AFM_LAUNCH_CHECK();
-	for (int j = 0; j < m.nu2; ++j) AFM_TRY(smooth3(c, x, rhs, l));
+	for (int j = 0; j < m.nu2; ++j) AFM_TRY(smooth3(c, x, rhs, m.side(l), 1u << l));
 	hipLaunchKernelGGL(k_copy3, g3(N), dim3(256), 0, c->stream, rhs, x, N);
 	AFM_LAUNCH_CHECK();
 	return 0;
@@ -326,35 +428,35 @@ static int nested3(Mg3Ctx& m, unsigned l, double* input)
 {
 	if (l == 0) return 0;
 	Ctx* c = m.c;
-	const size_t N = cells3(l);
+	const size_t N = m.cells(l);
 	double* copy = m.R[l];
 	hipLaunchKernelGGL(k_copy3, g3(N), dim3(256), 0, c->stream, copy, input, N);
-	hipLaunchKernelGGL(k_restrict3, g3(cells3(l - 1)), dim3(256), 0, c->stream, input, m.B[l - 1], 1u << l);
+	hipLaunchKernelGGL(k_restrict3, g3(m.cells(l - 1)), dim3(256), 0, c->stream, input, m.B[l - 1], m.side(l));
 	AFM_LAUNCH_CHECK();
 	AFM_TRY(nested3(m, l - 1, m.B[l - 1]));
 	AFM_TRY(recurse3(m, l - 1, m.B[l - 1]));
-	const unsigned nf = 1u << l, h = nf / 2;
+	const unsigned nf = m.side(l), h = nf / 2;
 	if (h >= 2)
 		hipLaunchKernelGGL(k_prolong3_interior, g3((size_t) h * (h - 1) * h), dim3(256), 0, c->stream, m.B[l - 1], input, nf);
 	hipLaunchKernelGGL(k_prolong3_walls, g3((size_t) h * h), dim3(256), 0, c->stream, m.B[l - 1], input, nf);
 	AFM_LAUNCH_CHECK();
-	for (int j = 0; j < m.nu2; ++j) AFM_TRY(smooth3(c, input, copy, l));
+	for (int j = 0; j < m.nu2; ++j) AFM_TRY(smooth3(c, input, copy, m.side(l), 1u << l));
 	return 0;
 }
 
-static int mg3_alloc(Ctx* c, unsigned L, int nu1, int nu2, Mg3Ctx& m, std::vector<void*>& own)
+static int mg3_alloc(Ctx* c, unsigned L, unsigned W, int nu1, int nu2, Mg3Ctx& m, std::vector<void*>& own)
 {
-	m.c = c; m.L = L; m.nu1 = nu1; m.nu2 = nu2;
+	m.c = c; m.L = L; m.W = W; m.nu1 = nu1; m.nu2 = nu2;
 	m.X.assign(L + 1, nullptr); m.B.assign(L + 1, nullptr); m.R.assign(L + 1, nullptr);
-	for (unsigned l = 0; l <= L; ++l) {
-		const size_t N = cells3(l);
-		for (double** p : {&m.X[l], &m.B[l], &m.R[l]}) {
-			if (hipMalloc((void**) p, N * 8) != hipSuccess) { set_error("out of device memory (3-D multigrid)"); return -3; }
-			own.push_back(*p);
-		}
-	}
-	if (hipMalloc((void**) &m.scal, 8 * 8) != hipSuccess) { set_error("out of device memory (3-D multigrid)"); return -3; }
-	own.push_back(m.scal);
+	auto get = [&](double** p, size_t count) -> int {
+		if (hipMalloc((void**) p, count * 8) != hipSuccess) { set_error("out of device memory (3-D multigrid)"); return -3; }
+		own.push_back(*p);
+		return 0;
+	};
+	for (unsigned l = 0; l <= L; ++l)
+		for (double** p : {&m.X[l], &m.B[l], &m.R[l]}) AFM_TRY(get(p, m.cells(l)));
+	AFM_TRY(get(&m.Z0, m.cells(0)));
+	AFM_TRY(get(&m.scal, 8));
 	return 0;
 }
 
@@ -364,35 +466,77 @@ static void mg3_free(Ctx* c, std::vector<void*>& own)
 	for (void* p : own) (void) hipFree(p);
 }
 
-int mg3_laplacian(Ctx* c, unsigned L, const double* x, double* y)
+int mg3_laplacian(Ctx* c, unsigned L, unsigned W, const double* x, double* y)
 {
-	hipLaunchKernelGGL(k_lap3, g3(cells3(L)), dim3(256), 0, c->stream, x, y, 1u << L);
+	const unsigned s = W << L;
+	hipLaunchKernelGGL(k_lap3, g3((size_t) s * s * s), dim3(256), 0, c->stream, x, y, s, 1u << L);
 	AFM_LAUNCH_CHECK();
 	return 0;
 }
 
-int mg3_vcycle(Ctx* c, unsigned L, double* rhs, int nu1, int nu2)
+// one recurse(L, rhs) (mg/solver.h:199-216); tol is level 0's CG tolerance
+int mg3_vcycle(Ctx* c, unsigned L, unsigned W, double* rhs, int nu1, int nu2, double tol)
 {
 	Mg3Ctx m;
 	std::vector<void*> own;
-	int rc = mg3_alloc(c, L, nu1, nu2, m, own);
+	int rc = mg3_alloc(c, L, W, nu1, nu2, m, own);
+	m.tol0 = tol;
 	if (!rc) rc = recurse3(m, L, rhs);
 	mg3_free(c, own);
 	return rc;
 }
 
-static int read_scalar(Ctx* c, const double* d, double* h)
+// mg::solver<G>::restriction (mg/solver.h:153-181): fine level L >= 1 to L-1
+int mg3_restrict(Ctx* c, unsigned L, unsigned W, const double* fine, double* coarse)
 {
-	AFM_HIP(hipMemcpyAsync(h, d, 8, hipMemcpyDeviceToHost, c->stream));
-	AFM_HIP(hipStreamSynchronize(c->stream));
+	const unsigned s = W << L, h = s / 2;
+	hipLaunchKernelGGL(k_restrict3, g3((size_t) h * h * h), dim3(256), 0, c->stream, fine, coarse, s);
+	AFM_LAUNCH_CHECK();
 	return 0;
 }
 
-int mg3_create(Ctx* c, unsigned L, int nu1, int nu2, Mg3** out)
+// mg::solver<G>::interpolation (mg/solver.h:71-151): level L-1 to L (every
+// fine row is written)
+int mg3_prolong(Ctx* c, unsigned L, unsigned W, const double* coarse, double* fine)
+{
+	const unsigned nf = W << L, h = nf / 2;
+	if (h >= 2)
+		hipLaunchKernelGGL(k_prolong3_interior, g3((size_t) h * (h - 1) * h), dim3(256), 0, c->stream, coarse, fine, nf);
+	hipLaunchKernelGGL(k_prolong3_walls, g3((size_t) h * h), dim3(256), 0, c->stream, coarse, fine, nf);
+	AFM_LAUNCH_CHECK();
+	return 0;
+}
+
+// red-black Gauss-Seidel sweeps of L x = b at level L (alg/smooth.h:35-51 per row)
+int mg3_smooth(Ctx* c, unsigned L, unsigned W, double* x, const double* b, int sweeps)
+{
+	for (int k = 0; k < sweeps; ++k) AFM_TRY(smooth3(c, x, b, W << L, 1u << L));
+	return 0;
+}
+
+// alg::conjugate_gradient on the level-L laplacian: x is the start and the result
+int mg3_cg(Ctx* c, unsigned L, unsigned W, const double* b, double* x, double tol, int max_iters, int* iters,
+           double* final_norm)
+{
+	const unsigned s = W << L;
+	const size_t N = (size_t) s * s * s;
+	std::vector<void*> own;
+	double *r = nullptr, *p = nullptr, *z = nullptr, *scal = nullptr;
+	int rc = 0;
+	for (double** q : {&r, &p, &z, &scal}) {
+		if (hipMalloc((void**) q, (q == &scal ? 8 : N) * 8) != hipSuccess) { set_error("out of device memory (3-D CG)"); rc = -3; break; }
+		own.push_back(*q);
+	}
+	if (!rc) rc = cg3_lap(c, s, 1u << L, x, b, r, p, z, scal, tol, max_iters, iters, final_norm);
+	mg3_free(c, own);
+	return rc;
+}
+
+int mg3_create(Ctx* c, unsigned L, unsigned W, int nu1, int nu2, Mg3** out)
 {
 	Mg3* g = new Mg3;
-	int rc = mg3_alloc(c, L, nu1, nu2, g->m, g->own);
-	const size_t N = cells3(L);
+	int rc = mg3_alloc(c, L, W, nu1, nu2, g->m, g->own);
+	const size_t N = g->m.cells(L);
 	for (double** p : {&g->xs, &g->res, &g->copyb}) {
 		if (rc) break;
 		if (hipMalloc((void**) p, N * 8) != hipSuccess) { set_error("out of device memory (3-D multigrid)"); rc = -3; break; }
@@ -411,11 +555,11 @@ void mg3_destroy(Mg3* g)
 }
 
 // mg::solver<G>::solve (mg/solver.h:218-237): rhs replaced by the solution
-int mg3_solve(Ctx* c, unsigned L, double* rhs, double tol, int tol_mode, int nu1, int nu2, int max_cycles,
-              int* cycles, double* final_rms)
+int mg3_solve(Ctx* c, unsigned L, unsigned W, double* rhs, double tol, int tol_mode, int nu1, int nu2,
+              int max_cycles, int* cycles, double* final_rms)
 {
 	Mg3* g = nullptr;
-	int rc = mg3_create(c, L, nu1, nu2, &g);
+	int rc = mg3_create(c, L, W, nu1, nu2, &g);
 	if (rc) return rc;
 	rc = mg3_solve_with(g, rhs, tol, tol_mode, max_cycles, cycles, final_rms);
 	mg3_destroy(g);
@@ -426,8 +570,8 @@ int mg3_solve_with(Mg3* g, double* rhs, double tol, int tol_mode, int max_cycles
 {
 	Mg3Ctx& m = g->m;
 	Ctx* c = m.c;
-	const unsigned L = m.L;
-	const size_t N = cells3(L);
+	const unsigned L = m.L, s = m.side(L), n = 1u << L;
+	const size_t N = m.cells(L);
 	double* sc = m.scal;
 	double* xs = g->xs;
 	double* res = g->res;
@@ -440,13 +584,14 @@ int mg3_solve_with(Mg3* g, double* rhs, double tol, int tol_mode, int max_cycles
 	AFM_TRY(sum3(c, copyb, N, 1, sc + 1));
 	AFM_TRY(read_scalar(c, sc + 1, &h_sq));
 	const double t = tol_mode == 1 ? tol * (sqrt(h_sq) / sqrt((double) N)) : tol;
+	m.tol0 = t;      // every level's solver holds the same tolerance (mg/solver.h:60-62)
 	// x = rhs; nested(L, x)
 	hipLaunchKernelGGL(k_copy3, g3(N), dim3(256), 0, c->stream, xs, rhs, N);
 	AFM_TRY(nested3(m, L, xs));
 	// rhs -= mean (c = dot(rhs, ones) / N, as the reference computes it)
 	AFM_TRY(sum3(c, rhs, N, 0, sc));
 	hipLaunchKernelGGL(k_sub3, g3(N), dim3(256), 0, c->stream, rhs, sc, 1.0 / (double) N, N);
-	hipLaunchKernelGGL(k_resid3, g3(N), dim3(256), 0, c->stream, xs, rhs, res, 1u << L);
+	hipLaunchKernelGGL(k_resid3, g3(N), dim3(256), 0, c->stream, xs, rhs, res, s, n);
 	AFM_TRY(sum3(c, res, N, 1, sc + 1));
 	AFM_TRY(read_scalar(c, sc + 1, &h_sq));
 	double r = sqrt(h_sq) / sqrt((double) N);
@@ -454,7 +599,7 @@ int mg3_solve_with(Mg3* g, double* rhs, double tol, int tol_mode, int max_cycles
 	while (r > t && count < max_cycles) {
 		AFM_TRY(recurse3(m, L, res));
 		hipLaunchKernelGGL(k_axpy3, g3(N), dim3(256), 0, c->stream, xs, res, N);
-		hipLaunchKernelGGL(k_resid3, g3(N), dim3(256), 0, c->stream, xs, rhs, res, 1u << L);
+		hipLaunchKernelGGL(k_resid3, g3(N), dim3(256), 0, c->stream, xs, rhs, res, s, n);
 		AFM_TRY(sum3(c, res, N, 1, sc + 1));
 		AFM_TRY(read_scalar(c, sc + 1, &h_sq));
 		r = sqrt(h_sq) / sqrt((double) N);

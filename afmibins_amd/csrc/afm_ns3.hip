@@ -324,6 +324,28 @@ __global__ void k_rhs3_face(int cn, const double* __restrict__ old, const double
 	}
 }
 
+// helmholtz_periodic::boundary (kind 1, ns/helmholtz.h:177-196: n^3 rows,
+// 8/3 s n^2 times the wall value in the first and last row) and
+// helmholtz_dirichlet::boundary (kind 3, ns/helmholtz.h:321-337: n (n-1) n
+// rows, s n^2 times it); every other row 0.  One thread per column (x, z),
+// bottom row written before the top one, as the reference loop does.
+__global__ void k_hbound3(int kind, double scale, const double* __restrict__ top, const double* __restrict__ bot,
+                          double* __restrict__ out, unsigned n)
+{
+	const size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+	if (i >= (size_t) n * n) return;
+	const unsigned x = (unsigned) (i % n), z = (unsigned) (i / n);
+	const unsigned ny = kind == 1 ? n : n - 1;
+	for (unsigned y = 0; y < ny; ++y) out[x + (size_t) n * (y + (size_t) ny * z)] = 0.0;
+	if (kind == 1) {
+		out[x + (size_t) n * ny * z] = 8. / 3. * scale * n * n * bot[i];
+		out[x + (size_t) n * (ny - 1 + (size_t) ny * z)] = 8. / 3. * scale * n * n * top[i];
+	} else {
+		out[x + (size_t) n * ny * z] = scale * n * n * bot[i];
+		out[x + (size_t) n * (ny - 1 + (size_t) ny * z)] = scale * n * n * top[i];
+	}
+}
+
 __global__ void k_sub3v(double* __restrict__ x, const double* __restrict__ y, size_t N)
 {
 	const size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
@@ -480,7 +502,7 @@ int ns3_create(Ctx* c, unsigned L, double mu, double rho, double dt, double tol,
 		{&s->rx, N}, {&s->ry, Nv}, {&s->rz, N}, {&s->cg.r, N}, {&s->cg.p, N}, {&s->cg.z, N}, {&s->cg.s, 4}};
 	for (auto& b : bufs)
 		if ((rc = ns3_alloc(s, b.p, b.count))) break;
-	if (!rc) rc = mg3_create(c, L, 2, 2, &s->mg);
+	if (!rc) rc = mg3_create(c, L, 1, 2, 2, &s->mg);
 	if (!rc && hipMemsetAsync(s->zero, 0, Nxz * 8, c->stream) != hipSuccess) { set_error("hipMemsetAsync failed"); rc = -2; }
 	if (!rc && hipMemsetAsync(s->phx, 0, N * 8, c->stream) != hipSuccess) { set_error("hipMemsetAsync failed"); rc = -2; }
 	if (!rc && hipMemsetAsync(s->phz, 0, N * 8, c->stream) != hipSuccess) { set_error("hipMemsetAsync failed"); rc = -2; }
@@ -603,6 +625,27 @@ int ns3_helmholtz_cg(Ctx* c, unsigned L, int kind, double scale, const double* b
 	(void) hipStreamSynchronize(c->stream);
 	for (void* p : t.own) (void) hipFree(p);
 	return rc;
+}
+
+// ns::helmholtz_*::boundary(top, bottom) into out (n^3 or n (n-1) n values)
+int ns3_helmholtz_boundary(Ctx* c, unsigned L, int kind, double scale, const double* top, const double* bottom,
+                           double* out)
+{
+	const unsigned n = 1u << L;
+	hipLaunchKernelGGL(k_hbound3, grid_of((size_t) n * n), dim3(256), 0, c->stream, kind, scale, top, bottom, out, n);
+	AFM_LAUNCH_CHECK();
+	return 0;
+}
+
+// projection::boundary (ns/projection.h:75-106) with the potential's
+// gradients phx, phz (n^3 each): the wall vectors are corrected in place
+int ns3_projection_boundary(Ctx* c, unsigned L, const double* phx, const double* phz, double* ut, double* ub,
+                            double* wt, double* wb)
+{
+	const unsigned n = 1u << L;
+	hipLaunchKernelGGL(k_proj_bound3, grid_of((size_t) n * n), dim3(256), 0, c->stream, phx, phz, ut, ub, wt, wb, n);
+	AFM_LAUNCH_CHECK();
+	return 0;
 }
 
 }  // namespace afm

@@ -169,7 +169,8 @@ def fourier_velocity_device(n, kmax, seed, device, rms=1.0):
     return u * (rms / s), v * (rms / s)
 
 
-# BASELINE configurations (sizes and physics) used by bench.py and the tests.
+# The benchmark and parity configurations (sizes and physics) used by bench.py
+# and the tests.
 CONFIGS = {
     "c1": dict(n=128, membranes=None, mu=0.01, rho=1.0, dt=1e-4, steps=200, tol=1e-10),
     "c2": dict(n=2048, membranes=(64, 256), seed=2, kmax=8, mu=1e-3, rho=1.0, dt=2e-5, steps=100, tol=1e-10),

@@ -337,7 +337,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              f"(the path is single-GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
+              f"(it times one GPU's share of the per-timestep path, a whole grid per GPU)", file=sys.stderr)
         sys.exit(2)
     import torch
     import afmibins_amd as A

@@ -121,22 +121,39 @@ int afm_mg_vcycle(afm_ctx* c, double* rhs_inout, int nu1, int nu2);
 int afm_mg_solve(afm_ctx* c, double* rhs_inout, double tol, int tol_mode, int nu1, int nu2, int precision,
                  int max_cycles, int* cycles, double* final_rms);
 
-/* The reference's native 3-D geometry for the multigrid solve: n x n x n
- * cells, n = 2^levels (1 <= levels <= 9), periodic in x and z, walls
- * (homogeneous Neumann) in y -- mg::solver<geometry<levels,1,1,1>>
- * (mg/solver.h:35-271, dt/geometry.h), the geometry of the reference's
- * test/mg_test3.cc.  Fields are x fastest, then y, then z.  The smoother is
+/* The reference's native 3-D geometry for the multigrid solve:
+ * geometry<levels, width, width, width> (dt/geometry.h:29-60): a cube of
+ * s = width * 2^levels cells per axis (width a power of two, 2 <= s <= 512),
+ * laplacian scale n = 2^levels, periodic in x and z, walls (homogeneous
+ * Neumann) in y -- mg::solver<G> (mg/solver.h:35-271); width 1 is the
+ * geometry of the reference's test/mg_test3.cc, geometry<1,8,8,8> that of
+ * test/mg_test2.cc.  Fields are x fastest, then y, then z.  The smoother is
  * red-black Gauss-Seidel with the reference's row arithmetic
- * (alg/smooth.h:35-51); restriction, prolongation, the nested start and the
- * solve loop are the reference's.  Device pointers on the context's device
- * and stream; level scratch is allocated per call (a parity path, not
- * tuned).  afm_laplacian3_apply: y = L x (mg/laplacian.h:46-153); mg3_vcycle:
- * one recurse(levels, rhs), rhs replaced by the result; mg3_solve: as
- * afm_mg_solve's precision 0 on this geometry. */
-int afm_laplacian3_apply(afm_ctx* c, unsigned levels, const double* x, double* y);
-int afm_mg3_vcycle(afm_ctx* c, unsigned levels, double* rhs_inout, int nu1, int nu2);
-int afm_mg3_solve(afm_ctx* c, unsigned levels, double* rhs_inout, double tol, int tol_mode, int nu1, int nu2,
-                  int max_cycles, int* cycles, double* final_rms);
+ * (alg/smooth.h:35-51); restriction, prolongation, the nested start, the
+ * solve loop and the coarsest level (mean removal and CG on width^3 cells,
+ * mg/solver.h:264-271) are the reference's.  Device pointers on the
+ * context's device and stream; level scratch is allocated per call (a parity
+ * path, not tuned).
+ *   laplacian3_apply: y = L x (mg/laplacian.h:46-153, dt/matrix.h:81-94);
+ *   mg3_vcycle: one recurse(levels, rhs), rhs replaced by the result (tol:
+ *     the coarsest level's CG tolerance, unused for width 1);
+ *   mg3_solve: as afm_mg_solve's precision 0 on this geometry;
+ *   mg3_restrict: mg::solver::restriction (mg/solver.h:153-181), level
+ *     `levels` (s^3 values) to levels - 1;
+ *   mg3_prolong: mg::solver::interpolation (mg/solver.h:71-151), levels - 1
+ *     to levels, every fine value written;
+ *   mg3_smooth: `sweeps` red-black sweeps of L x = b in place;
+ *   laplacian3_cg: alg::conjugate_gradient (alg/cg.h:32-56) on L: x is the
+ *     start and the result, stop when |r|_2 <= tol. */
+int afm_laplacian3_apply(afm_ctx* c, unsigned levels, unsigned width, const double* x, double* y);
+int afm_mg3_vcycle(afm_ctx* c, unsigned levels, unsigned width, double* rhs_inout, int nu1, int nu2, double tol);
+int afm_mg3_solve(afm_ctx* c, unsigned levels, unsigned width, double* rhs_inout, double tol, int tol_mode, int nu1,
+                  int nu2, int max_cycles, int* cycles, double* final_rms);
+int afm_mg3_restrict(afm_ctx* c, unsigned levels, unsigned width, const double* fine, double* coarse);
+int afm_mg3_prolong(afm_ctx* c, unsigned levels, unsigned width, const double* coarse, double* fine);
+int afm_mg3_smooth(afm_ctx* c, unsigned levels, unsigned width, double* x, const double* b, int sweeps);
+int afm_laplacian3_cg(afm_ctx* c, unsigned levels, unsigned width, const double* b, double* x, double tol,
+                      int max_iters, int* iters, double* final_norm);
 
 /* The reference's 3-D wall-bounded Navier-Stokes solver on the same geometry
  * (1 <= levels <= 8): ns::solver<geometry<levels,1,1,1>> (ns/solver.h:37-236)
@@ -173,9 +190,22 @@ int afm_advect3(afm_ctx* c, unsigned levels, const double* u, const double* v, c
 int afm_helmholtz3_apply(afm_ctx* c, unsigned levels, int kind, double scale, const double* x, double* y);
 int afm_helmholtz3_cg(afm_ctx* c, unsigned levels, int kind, double scale, const double* b, double* x, double tol,
                       int max_iters, int* iters, double* final_norm);
+/* helmholtz_periodic::boundary (kind 1, ns/helmholtz.h:177-196, n^3 values)
+ * and helmholtz_dirichlet::boundary (kind 3, ns/helmholtz.h:321-337,
+ * n*(n-1)*n values) of the wall vectors top, bottom (n*n) into out;
+ * projection::boundary (ns/projection.h:75-106) with the gradient
+ * components phx, phz (n^3) of the last projection's potential: ut, ub, wt,
+ * wb (n*n) are corrected in place. */
+int afm_helmholtz3_boundary(afm_ctx* c, unsigned levels, int kind, double scale, const double* top,
+                            const double* bottom, double* out);
+int afm_projection3_boundary(afm_ctx* c, unsigned levels, const double* phx, const double* phz, double* ut,
+                             double* ub, double* wt, double* wb);
+
 /* ib::grid::interpolate (ib/grid.h:68-84) on the same 3-D MAC grid, for
  * comp 0 (u), 1 (v: the mac_grid<y> window of ib/mac_grid.h:114-132 on the
- * n*(n-1)*n interior y-faces) or 2 (w); X holds m markers as (x, y, z).
+ * n*(n-1)*n interior y-faces), 2 (w), or 3 (the node grid eulerian_grid of
+ * ib/eulerian_grid.h:54-93, n^3 nodes at (i, j, k)/n); X holds m markers as
+ * (x, y, z).
  * interp3: U[k] = sum over the 64 taps of (w vol) field[idx], in the
  * reference's tap order.  spread3: f = sum over markers of (w ds_k) F_k per
  * face, summed per face in marker order (run-to-run identical, no float
@@ -185,6 +215,25 @@ int afm_helmholtz3_cg(afm_ctx* c, unsigned levels, int kind, double scale, const
 int afm_interp3(afm_ctx* c, unsigned levels, int comp, const double* field, const double* X, long m, double* U);
 int afm_spread3(afm_ctx* c, unsigned levels, int comp, const double* F, const double* ds, const double* X, long m,
                 double* f);
+
+/* A dt::matrix (dt/matrix.h:35-94) in CSR form in device memory -- rows + 1
+ * row starts, column indices in ascending order per row, values -- for
+ * operators that have no stencil kernel here (a matrix a program fills):
+ *   csr_spmv: y = A x, each row summed from 0 in column order
+ *     (dt/matrix.h:81-94);
+ *   csr_cg: alg::conjugate_gradient (alg/cg.h:32-56): x is the start and the
+ *     result, stop when |r|_2 <= tol, -4 past max_iters;
+ *   csr_smooth: `sweeps` lexicographic Gauss-Seidel sweeps (alg/smooth.h:
+ *     35-51) in the reference's row order, on one lane (not a fast path).
+ * axpy: y += a x on `count` doubles (the drop-in classes' vector updates). */
+int afm_csr_spmv(afm_ctx* c, unsigned long rows, const unsigned long* row_start, const unsigned long* column,
+                 const double* values, const double* x, double* y);
+int afm_csr_cg(afm_ctx* c, unsigned long rows, const unsigned long* row_start, const unsigned long* column,
+               const double* values, const double* b, double* x, double tol, int max_iters, int* iters,
+               double* final_norm);
+int afm_csr_smooth(afm_ctx* c, unsigned long rows, const unsigned long* row_start, const unsigned long* column,
+                   const double* values, double* x, const double* b, int sweeps);
+int afm_axpy(afm_ctx* c, unsigned long count, double a, const double* x, double* y);
 
 /* (I + scale L5) x (ns/helmholtz.h:61-175 for the periodic plane) and
  * alg::conjugate_gradient on it (alg/cg.h:32-56): x is the initial guess and

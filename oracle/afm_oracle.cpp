@@ -468,7 +468,7 @@ void window3(int comp, const double c[3], unsigned n, unsigned W, unsigned H, un
              unsigned nx, unsigned ny, unsigned nz, int mode, size_t idx[64], double w[64])
 {
 	double q[3] = {c[0], c[1], c[2]};
-	if (comp != 1) {
+	if (comp == 0 || comp == 2) {      // comp 3: the node grid, unshifted
 		q[(comp + 1) % 3] -= 0.5 / n;
 		q[(comp + 2) % 3] -= 0.5 / n;
 	}
@@ -499,8 +499,10 @@ void window3(int comp, const double c[3], unsigned n, unsigned W, unsigned H, un
 			idx[i] = ix + (size_t) nx * (iy - 1 + (size_t) (ny - 1) * iz);
 		} else {
 			idx[i] = ix + (size_t) nx * (iy - 1 + (size_t) ny * iz);
-			p[(comp + 1) % 3] += 0.5 / n;
-			p[(comp + 2) % 3] += 0.5 / n;
+			if (comp != 3) {
+				p[(comp + 1) % 3] += 0.5 / n;
+				p[(comp + 2) % 3] += 0.5 / n;
+			}
 		}
 		w[i] = delta(p[0] - c[0], n, mode) * delta(p[1] - c[1], n, mode) * delta(p[2] - c[2], n, mode);
 	}

@@ -79,6 +79,31 @@ void interp(const double* field, const double* X, double* U)
 	for (unsigned k = 0; k < M; ++k) U[k] = l[k];
 }
 
+// the node grid ib::eulerian_grid<G> (comp 3) in place of a MAC component
+template<class G>
+void interp_nodes(const double* field, const double* X, double* U)
+{
+	ib::eulerian_grid<G> e;
+	for (unsigned i = 0; i < e.size(); ++i) e[i] = field[i];
+	std::vector<dt::point> pts;
+	for (unsigned k = 0; k < M; ++k) pts.push_back(dt::point(X[3 * k], X[3 * k + 1], X[3 * k + 2], k));
+	Markers<G> l(pts, nullptr);
+	l.interpolate(e);
+	for (unsigned k = 0; k < M; ++k) U[k] = l[k];
+}
+
+template<class G>
+void spread_nodes(const double* F, const double* ds, const double* X, double* f)
+{
+	ib::eulerian_grid<G> e;
+	std::vector<dt::point> pts;
+	for (unsigned k = 0; k < M; ++k) pts.push_back(dt::point(X[3 * k], X[3 * k + 1], X[3 * k + 2], k));
+	Markers<G> l(pts, ds);
+	for (unsigned k = 0; k < M; ++k) l[k] = F[k];
+	e.interpolate(l);
+	for (unsigned i = 0; i < e.size(); ++i) f[i] = e[i];
+}
+
 template<class G, ib::mac::component C>
 void spread(const double* F, const double* ds, const double* X, double* f)
 {
@@ -257,9 +282,11 @@ int ref_interp(int geo, int comp, const double* field, const double* X, double* 
 {
 	using namespace ib::mac;
 	if (geo == 0) {
-		if (comp == 0) interp<G16, x>(field, X, U); else if (comp == 1) interp<G16, y>(field, X, U); else interp<G16, z>(field, X, U);
+		if (comp == 0) interp<G16, x>(field, X, U); else if (comp == 1) interp<G16, y>(field, X, U);
+		else if (comp == 2) interp<G16, z>(field, X, U); else interp_nodes<G16>(field, X, U);
 	} else if (geo == 1) {
-		if (comp == 0) interp<G32, x>(field, X, U); else if (comp == 1) interp<G32, y>(field, X, U); else interp<G32, z>(field, X, U);
+		if (comp == 0) interp<G32, x>(field, X, U); else if (comp == 1) interp<G32, y>(field, X, U);
+		else if (comp == 2) interp<G32, z>(field, X, U); else interp_nodes<G32>(field, X, U);
 	} else return -1;
 	return 0;
 }
@@ -269,9 +296,11 @@ int ref_spread(int geo, int comp, const double* F, const double* ds, const doubl
 {
 	using namespace ib::mac;
 	if (geo == 0) {
-		if (comp == 0) spread<G16, x>(F, ds, X, f); else if (comp == 1) spread<G16, y>(F, ds, X, f); else spread<G16, z>(F, ds, X, f);
+		if (comp == 0) spread<G16, x>(F, ds, X, f); else if (comp == 1) spread<G16, y>(F, ds, X, f);
+		else if (comp == 2) spread<G16, z>(F, ds, X, f); else spread_nodes<G16>(F, ds, X, f);
 	} else if (geo == 1) {
-		if (comp == 0) spread<G32, x>(F, ds, X, f); else if (comp == 1) spread<G32, y>(F, ds, X, f); else spread<G32, z>(F, ds, X, f);
+		if (comp == 0) spread<G32, x>(F, ds, X, f); else if (comp == 1) spread<G32, y>(F, ds, X, f);
+		else if (comp == 2) spread<G32, z>(F, ds, X, f); else spread_nodes<G32>(F, ds, X, f);
 	} else return -1;
 	return 0;
 }

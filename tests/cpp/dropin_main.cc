@@ -36,6 +36,82 @@ public:
 	double volume(unsigned) const { return radius * 2 * M_PI / np; }
 };
 
+// the reference's 3-D geometry (16^3, walls in y) and a cube of width 8 at
+// level 1 (test/mg_test2.cc) and the single coarse cell (test/restr_test.cc)
+typedef dt::geometry<4u, 1u, 1u, 1u> g3;
+typedef dt::geometry<1u, 8u, 8u, 8u> g8;
+typedef dt::geometry<1u, 1u, 1u, 1u> g1;
+
+// test/ib_test.cc:33-44 in 3-D
+class point3 : public ib::lagrangian_grid<g3, 1u> {
+public:
+	point3(double x, double y, double z) : ib::lagrangian_grid<g3, 1u>({dt::point(x, y, z, 0u)}) {}
+	double volume(unsigned) const { return 1.0 / (g3::n * g3::n * g3::n); }
+};
+
+// test/ib_test.cc:46-73 in 3-D: a ring in the plane z = z0
+class ring3 : public ib::lagrangian_grid<g3, 96u> {
+	double radius;
+	static std::vector<dt::point> pts(double x, double y, double z, double r)
+	{
+		std::vector<dt::point> p;
+		for (unsigned i = 0; i < 96; ++i) {
+			double a = 2 * pi * i / 96;
+			p.push_back(dt::point(x + r * std::cos(a), y + r * std::sin(a), z, i));
+		}
+		return p;
+	}
+public:
+	ring3(double x, double y, double z, double r) : ib::lagrangian_grid<g3, 96u>(pts(x, y, z, r)), radius(r) {}
+	double volume(unsigned) const { return radius * 2 * pi / 96; }
+};
+
+// test/mg_test2.cc:28-35 and test/restr_test.cc:28-36: the protected
+// operators reached through `using`
+template<class T>
+class open_solver : public mg::solver<T> {
+public:
+	using mg::solver<T>::L;
+	using mg::solver<T>::interpolation;
+	open_solver(double tol, unsigned nu1, unsigned nu2) : mg::solver<T>(tol, nu1, nu2) {}
+};
+template<class T>
+class restriction_solver : mg::solver<T> {
+public:
+	using mg::solver<T>::restriction;
+	restriction_solver(double tol, unsigned nu1, unsigned nu2) : mg::solver<T>(tol, nu1, nu2) {}
+};
+
+// a matrix a program fills itself: the 1-D Dirichlet second difference
+// (2, -1) on 200 unknowns, symmetric positive definite
+class tridiagonal : public dt::matrix<200> {
+public:
+	tridiagonal()
+	{
+		row_start.assign(1, 0);
+		for (std::size_t r = 0; r < 200; ++r) {
+			if (r > 0) { column.push_back(r - 1); values.push_back(-1.0); }
+			column.push_back(r); values.push_back(2.0);
+			if (r + 1 < 200) { column.push_back(r + 1); values.push_back(-1.0); }
+			row_start.push_back(values.size());
+		}
+	}
+};
+
+// 7-point stencil on an n-per-unit grid of nx x ny x nz cells, periodic in
+// x and z, walls in y folded into the diagonal (written out independently)
+template<unsigned NX, unsigned NY, unsigned NZ>
+static double stencil7(const dt::vector<NX * NY * NZ>& b, unsigned i, double n)
+{
+	const unsigned x = i % NX, y = (i / NX) % NY, z = i / (NX * NY);
+	auto at = [&](unsigned a, unsigned bb, unsigned cc) { return b[a + NX * (bb + NY * cc)]; };
+	double s = at((x + 1) % NX, y, z) + at((x + NX - 1) % NX, y, z) + at(x, y, (z + 1) % NZ) + at(x, y, (z + NZ - 1) % NZ);
+	int cnt = 4;
+	if (y > 0) { s += at(x, y - 1, z); ++cnt; }
+	if (y < NY - 1) { s += at(x, y + 1, z); ++cnt; }
+	return n * n * (s - cnt * b[i]);
+}
+
 int main()
 {
 	// ib_test: spread 1 from one marker, binarise, interpolate back.  The
@@ -110,7 +186,6 @@ int main()
 	// right-hand side; the residual is checked here with a 7-point stencil
 	// written out independently (wall neighbours fold into the diagonal)
 	{
-		typedef dt::geometry<4u, 1u, 1u, 1u> g3;
 		constexpr unsigned n = g3::n, M = g3::size;
 		dt::vector<M> b, rhs;
 		unsigned long long s = 12345;
@@ -142,7 +217,6 @@ int main()
 	// comment suggests, u0 = (H - y) y, no-slip walls, no force: the flow
 	// stays a function of y alone (v = w = 0) and decays by diffusion
 	{
-		typedef dt::geometry<4u, 1u, 1u, 1u> g3;
 		constexpr unsigned n = g3::n, nx = g3::nx, ny = g3::ny, nz = g3::nz;
 		dt::vector<nx * ny * nz> u0, w0, fx, fz;
 		dt::vector<nx * (ny - 1) * nz> v0, fy;
@@ -164,6 +238,101 @@ int main()
 			spread = std::fmax(spread, std::fabs(u0[i] - u0[nx * y]));
 		}
 		std::printf("ns3_vw_max %.3e\nns3_xz_spread %.3e\nns3_u_decay %.6e\n", vw, spread, before - u0[mid]);
+	}
+	// ib_test.cc in 3-D: spread 1 from a marker at the centre to the v-faces,
+	// binarise, interpolate back: the 4 x 4 x 4 window's partition of unity.
+	// The row at r = 2h in y has a zero delta weight: 4 * 3 * 4 taps.
+	{
+		ib::mac_grid<g3, ib::mac::y> egrid;
+		point3 lgrid(0.5, 0.5, 0.5);
+		lgrid[0] = 1.0;
+		egrid.interpolate(lgrid);
+		unsigned nz = 0;
+		for (auto& e : egrid) if (e != 0.0) { e = 1; ++nz; }
+		lgrid[0] = 0.0;
+		lgrid.interpolate(egrid);
+		std::printf("ib3_nonzero %u\nib3_value %.17g\n", nz, lgrid[0]);
+	}
+	// spreading to the nodes (eulerian_grid) and to the w-faces conserves
+	// force: sum_grid f h^3 = sum_k F_k ds_k; a unit field interpolates to 1
+	{
+		ring3 r(0.45, 0.5, 0.3, 0.2);
+		for (unsigned k = 0; k < 96; ++k) r[k] = std::cos(5.0 * k) + 0.25;
+		ib::eulerian_grid<g3> f;
+		ib::mac_grid<g3, ib::mac::z> fw;
+		f.interpolate(r);
+		fw.interpolate(r);
+		constexpr unsigned M = g3::nx * g3::ny * g3::nz;
+		double s = 0.0, sw = 0.0, t = 0.0;
+		for (unsigned i = 0; i < M; ++i) { s += f[i]; sw += fw[i]; }
+		for (unsigned k = 0; k < 96; ++k) t += r[k] * r.volume(k);
+		ib::eulerian_grid<g3> one(1.0);
+		r.interpolate(one);
+		double unity = 0.0;
+		for (unsigned k = 0; k < 96; ++k) unity = std::fmax(unity, std::fabs(r[k] - 1.0));
+		std::printf("spread3_node_error %.3e\nspread3_w_error %.3e\ninterp3_unity_error %.3e\n",
+		            std::fabs(s / M - t), std::fabs(sw / M - t), unity);
+	}
+	// mg_test2.cc: L * b on the width-8 cube at level 1 (h = 1/2, 16^3),
+	// against the stencil written out above; L * 1 = 0
+	{
+		constexpr unsigned M = g8::nx * g8::ny * g8::nz;
+		open_solver<g8> solver(1e-12, 2, 2);
+		dt::vector<M> b, ones(1.0);
+		unsigned long long st = 99;
+		for (unsigned i = 0; i < M; ++i) {
+			st = st * 6364136223846793005ull + 1442695040888963407ull;
+			b[i] = (double) (st >> 11) / 9007199254740992.0 - 0.5;
+		}
+		dt::vector<M> lb = solver.L * b, l1 = solver.L * ones;
+		double err = 0.0, zero = 0.0;
+		for (unsigned i = 0; i < M; ++i) {
+			err = std::fmax(err, std::fabs(lb[i] - stencil7<g8::nx, g8::ny, g8::nz>(b, i, g8::n)));
+			zero = std::fmax(zero, std::fabs(l1[i]));
+		}
+		std::printf("L8_error %.3e\nL8_ones %.3e\n", err, zero);
+	}
+	// restr_test.cc: one coarse cell of 2 x 2 x 2 fine cells; every fine
+	// cell's 64-point weights sum to 64/512, so coarse = mean(fine), added to
+	// what coarse holds.  Prolongation of one coarse value is that value.
+	{
+		restriction_solver<g1> rs(1e-16, 2, 2);
+		dt::vector<8> fine = {0.125, -0.5, 1.0, 0.25, -0.75, 2.0, 0.5, -1.0};
+		dt::vector<1> coarse;
+		rs.restriction(coarse, fine);
+		const double first = coarse[0];
+		rs.restriction(coarse, fine);
+		open_solver<g1> os(1e-16, 2, 2);
+		dt::vector<1> c1 = {0.75};
+		dt::vector<8> f1(-3.0);
+		os.interpolation(f1, c1);
+		double pmax = 0.0;
+		for (unsigned i = 0; i < 8; ++i) pmax = std::fmax(pmax, std::fabs(f1[i] - 0.75));
+		std::printf("restr_first %.17g\nrestr_second %.17g\nprolong_error %.3e\n", first, coarse[0], pmax);
+	}
+	// alg::conjugate_gradient and alg::smooth on a matrix the program fills:
+	// b = A x_true, solve from zero; one sweep against the reference's
+	// lexicographic update written out here
+	{
+		tridiagonal A;
+		dt::vector<200> xt, x, b;
+		for (unsigned i = 0; i < 200; ++i) xt[i] = std::sin(0.1 * i) + 0.01 * i;
+		b = A * xt;
+		double ax = 0.0;
+		for (unsigned i = 0; i < 200; ++i) {
+			const double want = 2 * xt[i] - (i > 0 ? xt[i - 1] : 0.0) - (i + 1 < 200 ? xt[i + 1] : 0.0);
+			ax = std::fmax(ax, std::fabs(b[i] - want));
+		}
+		const unsigned it = alg::conjugate_gradient(x, A, b, 1e-12);
+		double e = 0.0;
+		for (unsigned i = 0; i < 200; ++i) e = std::fmax(e, std::fabs(x[i] - xt[i]));
+		dt::vector<200> y, yh;
+		alg::smooth(y, A, b);
+		for (unsigned i = 0; i < 200; ++i)
+			yh[i] = (b[i] + (i > 0 ? yh[i - 1] : 0.0)) / 2.0;  // GS row i: (b_i + x_{i-1}) / 2, x_{i+1} still 0
+		double se = 0.0;
+		for (unsigned i = 0; i < 200; ++i) se = std::fmax(se, std::fabs(y[i] - yh[i]));
+		std::printf("csr_apply_error %.3e\ncsr_cg_iters %u\ncsr_cg_error %.3e\ncsr_smooth_error %.3e\n", ax, it, e, se);
 	}
 	std::printf("done 1\n");
 	return 0;

@@ -36,3 +36,23 @@ def test_dropin_programs():
     # the mid-channel value falls by 2 mu dt (the discrete laplacian of a
     # quadratic is exact away from the walls)
     assert abs(float(v["ns3_u_decay"]) - 2 * 5e-5) < 1e-9
+    # 3-D markers through ib::lagrangian_grid / mac_grid / eulerian_grid: the
+    # v-face window of a marker at the centre has one zero row (r = 2h in y)
+    assert int(v["ib3_nonzero"]) == 48
+    assert abs(float(v["ib3_value"]) - 1.0) < 1e-14
+    assert float(v["spread3_node_error"]) < 1e-13
+    assert float(v["spread3_w_error"]) < 1e-13
+    assert float(v["interp3_unity_error"]) < 1e-14
+    # the protected operators of mg::solver (test/mg_test2.cc, restr_test.cc)
+    assert float(v["L8_error"]) < 1e-13
+    assert float(v["L8_ones"]) == 0.0
+    # the 64 weights of a fine cell sum to 1/8 and every product is exact
+    # for these dyadic inputs: the mean 1.625 / 8, then added once more
+    assert float(v["restr_first"]) == 0.203125
+    assert float(v["restr_second"]) == 0.40625
+    assert float(v["prolong_error"]) == 0.0
+    # a matrix the program fills: y = A x, CG, one lexicographic sweep
+    assert float(v["csr_apply_error"]) == 0.0
+    assert 1 <= int(v["csr_cg_iters"]) <= 400
+    assert float(v["csr_cg_error"]) < 1e-7
+    assert float(v["csr_smooth_error"]) == 0.0

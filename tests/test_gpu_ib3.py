@@ -67,7 +67,7 @@ def test_ib3_empty_marker_sets_and_bad_input(ctx_of, dev):
     n = 8
     X = torch.empty((0, 3), dtype=torch.float64, device=dev)
     e = torch.empty(0, dtype=torch.float64, device=dev)
-    for comp in (0, 1, 2):
+    for comp in (0, 1, 2, 3):
         f = H(ctx.spread3(comp, e, e, X, n))
         assert f.shape == ((n, n - 1, n) if comp == 1 else (n, n, n)) and not f.any()
         shape = (n, n - 1, n) if comp == 1 else (n, n, n)
@@ -75,4 +75,4 @@ def test_ib3_empty_marker_sets_and_bad_input(ctx_of, dev):
     with pytest.raises(ValueError):
         ctx.interp3(1, torch.zeros((n, n, n), dtype=torch.float64, device=dev), X)
     with pytest.raises(AfmError):
-        ctx.spread3(3, e, e, X, n)
+        ctx.spread3(4, e, e, X, n)

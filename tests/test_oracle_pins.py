@@ -42,7 +42,7 @@ def test_delta_modes_differ_only_by_glibc_cos_rounding(oracle):
 
 
 @pytest.mark.parametrize("geo,L", [(0, 4), (1, 5)])
-@pytest.mark.parametrize("comp", [0, 1, 2])
+@pytest.mark.parametrize("comp", [0, 1, 2, 3])
 def test_interp3_matches_reference(oracle, ref, geo, L, comp):
     n = 2 ** L
     M = ref.ref_markers()
@@ -55,7 +55,7 @@ def test_interp3_matches_reference(oracle, ref, geo, L, comp):
     assert np.array_equal(U, oracle.interp3(comp, f, X, n, mode=0))
 
 
-@pytest.mark.parametrize("comp", [0, 1, 2])
+@pytest.mark.parametrize("comp", [0, 1, 2, 3])
 def test_spread3_matches_reference_up_to_summation_order(oracle, ref, comp):
     n, M = 16, ref.ref_markers()
     rng = np.random.default_rng(30 + comp)
