@@ -31,6 +31,7 @@ SIGNATURES = {
     "afm_own_stream": (_P, [_P]),
     "afm_update_timing": (_I, [_P, _I]),
     "afm_update_timing_read": (_I, [_P, _DP, C.POINTER(_L)]),
+    "afm_kernel_timing_read": (_I, [_P, _I, _DP, C.POINTER(_L)]),
     "afm_alloc": (_I, [_P, C.c_ulong, C.POINTER(_P)]),
     "afm_free": (_I, [_P, _P]),
     "afm_copy_to_device": (_I, [_P, _P, _P, C.c_ulong]),

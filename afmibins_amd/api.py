@@ -79,6 +79,15 @@ class Context:
         check(_lib.load().afm_update_timing_read(self._h, C.byref(avg), C.byref(n)), "afm_update_timing_read")
         return avg.value, n.value
 
+    def kernel_timing_read(self, kind):
+        """Timing of kind 0 (the outer update), 1 (the CG forms' last
+        post-smoothing) or 2 (the finest pre-smoothing + residual +
+        restriction): (average microseconds, launches timed)."""
+        avg, n = C.c_double(), C.c_long()
+        check(_lib.load().afm_kernel_timing_read(self._h, int(kind), C.byref(avg), C.byref(n)),
+              "afm_kernel_timing_read")
+        return avg.value, n.value
+
     def close(self):
         if self._h:
             torch.cuda.synchronize(self.device)

@@ -16,8 +16,9 @@ Ctx::~Ctx()
 	if (bin_list) (void) hipFree(bin_list);
 	if (srec) (void) hipFree(srec);
 	if (hctl) (void) hipHostFree(hctl);
-	for (hipEvent_t e : kt_ev)
-		if (e) (void) hipEventDestroy(e);
+	for (auto& kind : kt_ev)
+		for (hipEvent_t e : kind)
+			if (e) (void) hipEventDestroy(e);
 	if (own_stream) (void) hipStreamDestroy(own_stream);
 }
 
@@ -156,19 +157,29 @@ int afm_update_timing(afm_ctx* c, int enable)
 {
 	CHECK_ARG(c, "null context");
 	AFM_TRY(use(c));
-	if (enable && !c->kt_ev[0])
-		for (hipEvent_t& e : c->kt_ev) AFM_HIP(hipEventCreate(&e));
+	if (enable && !c->kt_ev[0][0])
+		for (auto& kind : c->kt_ev)
+			for (hipEvent_t& e : kind) AFM_HIP(hipEventCreate(&e));
 	c->kt_on = enable != 0;
-	c->kt_ms = 0.0;
-	c->kt_launches = 0;
+	for (int k = 0; k < Ctx::KT_KINDS; ++k) {
+		c->kt_ms[k] = 0.0;
+		c->kt_launches[k] = 0;
+	}
 	return 0;
 }
 
 int afm_update_timing_read(afm_ctx* c, double* avg_us, long* launches)
 {
 	CHECK_ARG(c && avg_us && launches, "null argument");
-	*launches = c->kt_launches;
-	*avg_us = c->kt_launches ? 1e3 * c->kt_ms / (double) c->kt_launches : 0.0;
+	return afm_kernel_timing_read(c, 0, avg_us, launches);
+}
+
+int afm_kernel_timing_read(afm_ctx* c, int kind, double* avg_us, long* launches)
+{
+	CHECK_ARG(c && avg_us && launches, "null argument");
+	CHECK_ARG(kind >= 0 && kind < Ctx::KT_KINDS, "kind must be 0, 1 or 2");
+	*launches = c->kt_launches[kind];
+	*avg_us = c->kt_launches[kind] ? 1e3 * c->kt_ms[kind] / (double) c->kt_launches[kind] : 0.0;
 	return 0;
 }
 

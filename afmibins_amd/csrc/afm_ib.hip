@@ -22,6 +22,12 @@ namespace afm {
 constexpr int SB = 4;             // spread bin edge (window base cells)
 constexpr int SCAN_BLOCKS = 1024; // at most this many scan chunks
 
+__global__ void k_zero_int(int* __restrict__ a, long count)
+{
+	const long i = blockIdx.x * (long) blockDim.x + threadIdx.x;
+	if (i < count) a[i] = 0;
+}
+
 __global__ void k_delta(const double* __restrict__ r, long count, unsigned n, double* __restrict__ out)
 {
 	long i = blockIdx.x * (long) blockDim.x + threadIdx.x;
@@ -150,50 +156,66 @@ __device__ inline int block_int_sum(int v, int* red)
 	return s;
 }
 
+// A scan chunk is 256 * items consecutive bins, visited as `items` rows of
+// 256 (thread t takes bin row * 256 + t, so a wave's loads and stores are
+// contiguous).
 __global__ __launch_bounds__(256) void k_spread_chunk(const int* __restrict__ counts, int* __restrict__ chunk,
                                                       int nbins, int items)
 {
 	__shared__ int red[256];
-	const long lo = ((long) blockIdx.x * 256 + threadIdx.x) * items;
+	const long lo = (long) blockIdx.x * 256 * items + threadIdx.x;
 	int s = 0;
 	for (int j = 0; j < items; ++j)
-		if (lo + j < nbins) s += counts[lo + j];
+		if (lo + 256L * j < nbins) s += counts[lo + 256L * j];
 	s = block_int_sum(s, red);
 	if (threadIdx.x == 0) chunk[blockIdx.x] = s;
 }
 
+// inclusive scan of one int per thread over a 256-thread block; `total` the
+// block's sum.  wsum: 4 ints of LDS, free again on return.
+__device__ inline int block_int_scan(int v, int* wsum, int& total)
+{
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+	for (int d = 1; d < 64; d <<= 1) {
+		const int t = __shfl_up(v, d, 64);
+		if (lane >= d) v += t;
+	}
+	if (lane == 63) wsum[w] = v;
+	__syncthreads();
+	int add = 0;
+	for (int q = 0; q < w; ++q) add += wsum[q];
+	total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+	__syncthreads();
+	return v + add;
+}
+
+// offs = exclusive scan of counts (offs[nbins] = the total); cursor and
+// counts cleared for the next spreading
 __global__ __launch_bounds__(256) void k_spread_scan(int* __restrict__ counts, const int* __restrict__ chunk,
                                                      int* __restrict__ offs, int* __restrict__ cursor, int nbins,
                                                      int items)
 {
 	__shared__ int red[256];
+	__shared__ int wsum[4];
 	const int c = blockIdx.x, tid = threadIdx.x;
 	int base = 0;
 	for (int q = tid; q < c; q += 256) base += chunk[q];
 	base = block_int_sum(base, red);
-	const long lo = ((long) c * 256 + tid) * items;
-	int s = 0;
-	for (int j = 0; j < items; ++j)
-		if (lo + j < nbins) s += counts[lo + j];
-	red[tid] = s;
-	__syncthreads();
-	for (int d = 1; d < 256; d <<= 1) {
-		const int t = tid >= d ? red[tid - d] : 0;
-		__syncthreads();
-		red[tid] += t;
-		__syncthreads();
-	}
-	int acc = base + red[tid] - s;
-	for (int j = 0; j < items; ++j)
-		if (lo + j < nbins) {
-			const int v = counts[lo + j];
-			offs[lo + j] = acc;
-			cursor[lo + j] = 0;
-			counts[lo + j] = 0;   // the next spreading's counts start at zero
-			acc += v;
+	const long lo = (long) c * 256 * items + tid;
+	for (int j = 0; j < items; ++j) {
+		const long i = lo + 256L * j;
+		const int v = i < nbins ? counts[i] : 0;
+		int total;
+		const int incl = block_int_scan(v, wsum, total);
+		if (i < nbins) {
+			offs[i] = base + incl - v;
+			cursor[i] = 0;
+			counts[i] = 0;   // the next spreading's counts start at zero
+			if (i == nbins - 1) offs[nbins] = base + incl;
 		}
-	// the lane holding the last bin closes the table
-	if (lo <= nbins - 1 && nbins - 1 < lo + items) offs[nbins] = acc;
+		base += total;
+	}
 }
 
 __global__ void k_spread_fill(int comp_mask, const MarkerRec* __restrict__ rec, long m, int nb1,
@@ -521,7 +543,10 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 		                   c->srec);
 		AFM_LAUNCH_CHECK();
 	} else {
-		AFM_HIP(hipMemsetAsync(offs, 0, sizeof(int) * (nb + 1), c->stream));
+		// a kernel, not a memset, so that a captured graph replays it
+		hipLaunchKernelGGL(k_zero_int, dim3((unsigned) ((nb + 1 + 255) / 256)), dim3(256), 0, c->stream, offs,
+		                   (long) nb + 1);
+		AFM_LAUNCH_CHECK();
 	}
 	const unsigned gblocks = (unsigned) (((c->sbx + GB - 1) / GB) * ((c->sby + GB - 1) / GB));
 	hipLaunchKernelGGL(k_spread_gather, dim3(gblocks, 2), dim3(256), 0, c->stream, comp_mask, c->srec, offs,

@@ -118,13 +118,15 @@ struct Ctx {
 	int sbx = 0, sby = 0;              // spread bins per axis (4x4 base cells each)
 	int last_mg_cycles = 0;            // predicted batch size for the next solve
 	int n_cu = 256;                    // compute units of the device (persistent grids)
-	// opt-in HIP-event timing of the finest-level post-smoothing + update
-	// launch of mg_solve (the largest single launch of a solve)
-	static constexpr int KT_SLOTS = 64;
+	// opt-in HIP-event timing of three finest-level launches of mg_solve:
+	// kind 0 the outer update (forms 0, 2: fused with the last post-smoothing;
+	// 1, 3: the CG update), kind 1 the last post-smoothing of the CG forms'
+	// preconditioner cycle, kind 2 the pre-smoothing + residual + restriction
+	static constexpr int KT_SLOTS = 64, KT_KINDS = 3;
 	bool kt_on = false;
-	hipEvent_t kt_ev[2 * KT_SLOTS] = {};
-	double kt_ms = 0.0;
-	long kt_launches = 0;
+	hipEvent_t kt_ev[KT_KINDS][2 * KT_SLOTS] = {};
+	double kt_ms[KT_KINDS] = {};
+	long kt_launches[KT_KINDS] = {};
 	int last_cg_iters = 0;
 	std::vector<void*> owned;
 	~Ctx();

@@ -389,6 +389,17 @@ template<> struct SmoothShape<1, 1> { static constexpr int NW = 8, RPW = 2; };
 template<> struct SmoothShape<2, 1> { static constexpr int NW = 8, RPW = 2; };
 template<> struct SmoothShape<1, 2> { static constexpr int NW = 6, RPW = 6; };
 template<> struct SmoothShape<2, 2> { static constexpr int NW = 4, RPW = 10; };
+// the flexible CG's finest post-smoothing (k_smooth_fcg): 512-thread blocks
+// of 8 rows per wave; its 80-odd VGPRs allow one 1024-thread block per CU
+// but two of these (2048^2: 720 blocks either way, 26.5 -> 22.0 us)
+#ifndef AFM_SHAPE_FCG_NW
+#define AFM_SHAPE_FCG_NW 8
+#endif
+#ifndef AFM_SHAPE_FCG_RPW
+#define AFM_SHAPE_FCG_RPW 8
+#endif
+template<> struct SmoothShape<1, 3> { static constexpr int NW = 6, RPW = 6; };
+template<> struct SmoothShape<2, 3> { static constexpr int NW = AFM_SHAPE_FCG_NW, RPW = AFM_SHAPE_FCG_RPW; };
 constexpr size_t SMALL_CELLS = (size_t) 512 * 512;
 
 // halo of a launch: 2 cells per sweep, 2 more for a residual epilogue
@@ -602,17 +613,17 @@ __device__ inline void reduce_partial_s(double (&v)[NV], double* partials, size_
 // the block's z.r_k, z.r_{k-1} and z.L z (at k = 0 rkm1 is b, and the
 // column unused).
 template<typename TE, int S>
-__global__ __launch_bounds__(64 * SmoothShape<S>::NW) void k_smooth_fcg(
+__global__ __launch_bounds__((64 * SmoothShape<S, 3>::NW)) void k_smooth_fcg(
 	const TE* __restrict__ epre, const TE* __restrict__ b, const TE* __restrict__ ec, TE* __restrict__ z,
 	const TE* __restrict__ rkm1, int nx, int ny, TE off, TE dg, TE inv, double off64, double dg64,
 	double* __restrict__ partials, unsigned stride)
 {
-	constexpr int NW = SmoothShape<S>::NW, RPW = SmoothShape<S>::RPW, H = smooth_halo<S, EPI_UPDATE>();
+	constexpr int NW = SmoothShape<S, 3>::NW, RPW = SmoothShape<S, 3>::RPW, H = smooth_halo<S, EPI_UPDATE>();
 	__shared__ TE bnd[bnd_size<NW>()];
 	__shared__ double bnd64[bnd_size<NW>()];
 	int X0, Y0;
 	double acc[3] = {0.0, 0.0, 0.0};
-	if (smooth_tile_origin<S, EPI_UPDATE>(nx, ny, X0, Y0)) {
+	if (smooth_tile_origin<S, EPI_UPDATE, 3>(nx, ny, X0, Y0)) {
 		Strip<TE, S, H, NW, RPW, true> st(X0, Y0, nx);
 		st.template load<M_PADD>(epre, b, ec, nx, ny);
 		st.sweeps(bnd, nx, ny, off, dg, inv);
@@ -1508,6 +1519,11 @@ static inline unsigned update_blocks(int nx, int ny, int sweeps)
 {
 	return sweeps == 1 ? smooth_blocks<1, EPI_UPDATE>(nx, ny) : smooth_blocks<2, EPI_UPDATE>(nx, ny);
 }
+// blocks of the flexible CG's post-smoothing launch (three partial sums each)
+static inline unsigned fcg_blocks(int nx, int ny, int sweeps)
+{
+	return sweeps == 1 ? smooth_blocks<1, EPI_UPDATE, 3>(nx, ny) : smooth_blocks<2, EPI_UPDATE, 3>(nx, ny);
+}
 
 // nu sweeps as launches of at most 2 sweeps; the first launch applies `mode`.
 template<typename T>
@@ -1577,7 +1593,7 @@ struct UpdateArgs {
 	T* res_out;             // next residual (the next V-cycle's right-hand side)
 	SolveCtl* ctl;
 	double* partials;       // one r^2 sum per block
-	int slot;               // timing event pair, or -1
+	int slot;               // timing event pair of the update (kind 0), or -1
 	const MGFold* fold;     // the previous cycle's stop test, or nullptr
 	// flexible CG: the last post-smoothing stores z and sums z.r_k, z.r_{k-1},
 	// z.L z instead of the update (xs, rhs, xs_out, res_out unused)
@@ -1586,7 +1602,26 @@ struct UpdateArgs {
 	T* z;
 	double* fpart;
 	unsigned fstride;
+	int tslot = -1;         // timing event pair of the finest smoothing launches (kinds 1, 2), or -1
 };
+
+// record one timing event of kind `kind`, slot `slot` (start 0, end 1)
+static inline int kt_record(Ctx* c, int kind, int slot, int end)
+{
+	if (slot < 0) return 0;
+	AFM_HIP(hipEventRecord(c->kt_ev[kind][2 * slot + end], c->stream));
+	return 0;
+}
+
+// add the elapsed time of kind `kind`, slot `slot` to the totals
+static inline int kt_accumulate(Ctx* c, int kind, int slot)
+{
+	float ms = 0.f;
+	AFM_HIP(hipEventElapsedTime(&ms, c->kt_ev[kind][2 * slot], c->kt_ev[kind][2 * slot + 1]));
+	c->kt_ms[kind] += ms;
+	++c->kt_launches[kind];
+	return 0;
+}
 
 // recurse(l, B_l) -> X_l (final in Xb); rhs of level l given explicitly.
 // With up (and l above the LDS levels), the last post-smoothing instead
@@ -1604,8 +1639,11 @@ static int vcycle(Ctx* c, unsigned l, const T* rhs_l, int nu1, int nu2, const in
 	for (unsigned k = l; k > lc; --k) {
 		const T* bk = (k == l) ? rhs_l : lvB<T>(c->lv[k]);
 		// pre-smoothing from zero, residual and restriction in one launch
+		const int ts = (k == l && up) ? up->tslot : -1;
+		AFM_TRY(kt_record(c, 2, ts, 0));
 		AFM_TRY(smooth_launch<T>(c, k, M_ZERO, nu1, nullptr, bk, nullptr, lvX<T>(c->lv[k], 0), done,
 		                         lvB<T>(c->lv[k - 1]), (k == l && up) ? up->fold : nullptr));
+		AFM_TRY(kt_record(c, 2, ts, 1));
 	}
 	AFM_TRY(coarse_launch<T>(c, 0, (int) lc, lvB<T>(c->lv[lc]), lvX<T>(c->lv[lc], 1), nu1, nu2, done));
 	for (unsigned k = lc + 1; k <= l; ++k) {
@@ -1621,21 +1659,23 @@ static int vcycle(Ctx* c, unsigned l, const T* rhs_l, int nu1, int nu2, const in
 			                   lvX<T>(c->lv[k - 1], 1), up->xs, up->rhs, up->xs_out, up->res_out, nx, ny, off, dg, \
 			                   inv, n2, -4.0 * n2, up->ctl, up->partials)
 #define AFM_SF(S_)                                                                                              \
-			hipLaunchKernelGGL((k_smooth_fcg<T, S_>), dim3(smooth_blocks<S_, EPI_UPDATE>(nx, ny)),                \
-			                   dim3(64 * SmoothShape<S_>::NW), 0, c->stream, lvX<T>(c->lv[k], 0), bk,              \
+			hipLaunchKernelGGL((k_smooth_fcg<T, S_>), dim3(smooth_blocks<S_, EPI_UPDATE, 3>(nx, ny)),             \
+			                   dim3(64 * SmoothShape<S_, 3>::NW), 0, c->stream, lvX<T>(c->lv[k], 0), bk,              \
 			                   lvX<T>(c->lv[k - 1], 1), up->z, up->rkm1, nx, ny, off, dg, inv, n2, -4.0 * n2,       \
 			                   up->fpart, up->fstride)
 			const int slot = up->slot;
-			if (slot >= 0) AFM_HIP(hipEventRecord(c->kt_ev[2 * slot], c->stream));
+			AFM_TRY(kt_record(c, 0, slot, 0));
 			if (up->fcg) {
+				AFM_TRY(kt_record(c, 1, up->tslot, 0));
 				if (nu2 == 1) AFM_SF(1);
 				else AFM_SF(2);
+				AFM_TRY(kt_record(c, 1, up->tslot, 1));
 			} else if (nu2 == 1) {
 				AFM_SU(1);
 			} else {
 				AFM_SU(2);
 			}
-			if (slot >= 0) AFM_HIP(hipEventRecord(c->kt_ev[2 * slot + 1], c->stream));
+			AFM_TRY(kt_record(c, 0, slot, 1));
 #undef AFM_SU
 #undef AFM_SF
 			AFM_LAUNCH_CHECK();
@@ -1739,6 +1779,8 @@ int mg_setup(Ctx* c)
 		size_t nb = MAX_RED_BLOCKS;
 		nb = std::max<size_t>(nb, update_blocks((int) c->nx, (int) c->ny, 1));
 		nb = std::max<size_t>(nb, update_blocks((int) c->nx, (int) c->ny, 2));
+		nb = std::max<size_t>(nb, fcg_blocks((int) c->nx, (int) c->ny, 1));
+		nb = std::max<size_t>(nb, fcg_blocks((int) c->nx, (int) c->ny, 2));
 		AFM_TRY(c->alloc((void**) &c->upartials, nb * 8));
 		// flexible CG: the dots launch's three columns, its search directions
 		// and the update launch's sums (alternating by iteration)
@@ -1809,7 +1851,7 @@ static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 	AFM_LAUNCH_CHECK();
 	const int* done = &dctl->done;
 	const bool fused = Lt > c->lc;     // else every cycle is one coarse-kernel launch
-	const unsigned fnblk = fused ? update_blocks(nx, ny, nu2) : rb;
+	const unsigned fnblk = fused ? fcg_blocks(nx, ny, nu2) : rb;
 	const unsigned fstride = fused ? c->fstride : (unsigned) MAX_RED_BLOCKS;
 	T* z = lvX<T>(c->lv[Lt], 1);
 	int enq = 0;
@@ -1830,6 +1872,7 @@ static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 				up.slot = -1;
 				up.fold = s > 0 ? &fold : nullptr;
 				up.fcg = 1;
+				up.tslot = (c->kt_on && s < Ctx::KT_SLOTS) ? s : -1;
 				up.rkm1 = rkm1;
 				up.z = z;
 				up.fpart = c->fpartials;
@@ -1844,7 +1887,7 @@ static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 			                   xs[(k + 1) & 1], rhs, r[(k + 1) & 1], dctl, c->fpartials, fstride, fnblk,
 			                   c->gpartials[k & 1], k};
 			const int slot = (c->kt_on && k >= 1 && s < Ctx::KT_SLOTS) ? s : -1;
-			if (slot >= 0) AFM_HIP(hipEventRecord(c->kt_ev[2 * slot], c->stream));
+			AFM_TRY(kt_record(c, 0, slot, 0));
 			const dim3 blk(tiled ? 64 * FU_NW : 256);
 #define AFM_FU(F_, Z_)                                                                                    \
 			do {                                                                                          \
@@ -1855,7 +1898,7 @@ static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 			else if (x0) AFM_FU(true, false);
 			else AFM_FU(true, true);
 #undef AFM_FU
-			if (slot >= 0) AFM_HIP(hipEventRecord(c->kt_ev[2 * slot + 1], c->stream));
+			AFM_TRY(kt_record(c, 0, slot, 1));
 			if (!fused)
 				hipLaunchKernelGGL(k_fcg_fin, dim3(1), dim3(256), 0, c->stream, k + 1, gb, (double) N, dctl,
 				                   c->gpartials[k & 1]);
@@ -1869,14 +1912,16 @@ static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 		}
 		AFM_TRY(read_ctl(c, 0));
 		if (c->kt_on) {
-			// update launches that ran (later ones exit at once)
+			// launches of iterations that ran (later ones exit at once, or
+			// run on scratch: the cycle of iteration k ran if k < count)
 			for (int s = 0; s < batch && s < Ctx::KT_SLOTS; ++s) {
 				const int k = enq0 + s;
-				if (k < 1 || k >= c->hctl[0].count) continue;
-				float ms = 0.f;
-				AFM_HIP(hipEventElapsedTime(&ms, c->kt_ev[2 * s], c->kt_ev[2 * s + 1]));
-				c->kt_ms += ms;
-				++c->kt_launches;
+				if (k >= c->hctl[0].count) continue;
+				if (k >= 1) AFM_TRY(kt_accumulate(c, 0, s));
+				if (fused) {
+					AFM_TRY(kt_accumulate(c, 1, s));
+					AFM_TRY(kt_accumulate(c, 2, s));
+				}
 			}
 		}
 		if (c->hctl[0].done || enq >= max_cycles) break;
@@ -1959,7 +2004,8 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 			const MGFold* fp = s > 0 ? &fold : nullptr;
 			if (precision == 0) {
 				if (fused) {
-					const UpdateArgs<double> up{cur, rhs, nxt, res64[rn], dctl, c->upartials, slot, fp};
+					UpdateArgs<double> up{cur, rhs, nxt, res64[rn], dctl, c->upartials, slot, fp};
+					up.tslot = slot;
 					AFM_TRY(vcycle<double>(c, T, res64[rc], nu1, nu2, done, &up));
 				} else {
 					AFM_TRY(vcycle<double>(c, T, res64[rc], nu1, nu2, done));
@@ -1968,7 +2014,8 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 				}
 			} else {
 				if (fused) {
-					const UpdateArgs<float> up{cur, rhs, nxt, res32[rn], dctl, c->upartials, slot, fp};
+					UpdateArgs<float> up{cur, rhs, nxt, res32[rn], dctl, c->upartials, slot, fp};
+					up.tslot = slot;
 					AFM_TRY(vcycle<float>(c, T, res32[rc], nu1, nu2, done, &up));
 				} else {
 					AFM_TRY(vcycle<float>(c, T, res32[rc], nu1, nu2, done));
@@ -1988,10 +2035,8 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 		if (c->kt_on && fused) {
 			// launches of cycles that ran (later ones exit at once)
 			for (int s = 0; s < batch && s < Ctx::KT_SLOTS && enq0 + s < c->hctl[0].count; ++s) {
-				float ms = 0.f;
-				AFM_HIP(hipEventElapsedTime(&ms, c->kt_ev[2 * s], c->kt_ev[2 * s + 1]));
-				c->kt_ms += ms;
-				++c->kt_launches;
+				AFM_TRY(kt_accumulate(c, 0, s));
+				AFM_TRY(kt_accumulate(c, 2, s));
 			}
 		}
 		if (c->hctl[0].done || enq >= max_cycles) break;

@@ -31,7 +31,10 @@ cycle's FP32 (4): 48 B per finest cell (56 B with FP64 cycles).  The FP64
 form's is the finest-level post-smoothing fused with the outer update
 (k_smooth_update): read e_pre and b (8 + 8), the coarse correction (8 per 4
 cells = 2), xs and rhs (8 + 8); write xs and r (8 + 8): 50 B (37 B with FP32
-cycles).  `cpu_baseline` is the oracle's restatement of the reference path
+cycles).  `roofline_smoothing` times the same way each form's finest-level
+pre-smoothing + residual + restriction launch and, for the CG forms, the
+preconditioner cycle's last post-smoothing (bytes per cell at SMOOTHING).
+`cpu_baseline` is the oracle's restatement of the reference path
 (kd-tree gather spreading, lexicographic multigrid, CSR CG, glibc cos) on one
 host core over a three-step sample of the same workload.
 """
@@ -52,6 +55,18 @@ FORMS = {
     "mixed": (1, "k_fcg_update<float>", 48.0, "k_fcg_update_f32_2048_pmc.txt"),
     "mixed_ir": (2, "k_smooth_update<float, float, 2>", 37.0, "k_smooth_update_f32_2048_pmc.txt"),
     "fp64_cg": (3, "k_fcg_update<double>", 56.0, None),
+}
+# the finest-level smoothing launches of each form (timed like the update):
+# the pre-smoothing from zero + residual + restriction reads b and writes x
+# and the coarse right-hand side (a quarter cell): 4 + 4 + 1 B per cell in
+# FP32, 18 in FP64; the CG forms' last post-smoothing reads e_pre, b, r_prev
+# and a quarter cell of the coarse correction and writes z: 17 B in FP32, 34
+# in FP64 (the other forms fuse it with the update above)
+SMOOTHING = {
+    "fp64": (("pre", "k_smooth<double,2,M_ZERO,RESTRICT>", 18.0),),
+    "mixed": (("pre", "k_smooth<float,2,M_ZERO,RESTRICT>", 9.0), ("post", "k_smooth_fcg<float,2>", 17.0)),
+    "mixed_ir": (("pre", "k_smooth<float,2,M_ZERO,RESTRICT>", 9.0),),
+    "fp64_cg": (("pre", "k_smooth<double,2,M_ZERO,RESTRICT>", 18.0), ("post", "k_smooth_fcg<double,2>", 34.0)),
 }
 DTYPE = {"fp64": "f64", "mixed": "f64 (f32 V-cycles preconditioning f64 CG)",
          "mixed_ir": "f64 (f32 V-cycles in f64 refinement)", "fp64_cg": "f64 (f64 V-cycles preconditioning f64 CG)"}
@@ -126,6 +141,8 @@ def run_form(A, ctx, precision, args, inputs, dev, timing_steps=0):
         ib.step(timing_steps)
         torch.cuda.synchronize()
         out["update_us"], out["update_launches"] = ctx.update_timing_read()
+        out["post_us"], out["post_launches"] = ctx.kernel_timing_read(1)
+        out["pre_us"], out["pre_launches"] = ctx.kernel_timing_read(2)
         ctx.update_timing(False)
     out["u_final"] = u.clone()
     ib.close(); ns.close()
@@ -405,6 +422,11 @@ def main():
         line["mixed_over_fp64"] = round(forms["mixed"]["steps_per_s"] / forms["fp64"]["steps_per_s"], 3)
         a, b = forms["mixed"]["u_final"], forms["fp64"]["u_final"]
         line["mixed_vs_fp64_u_rel_err"] = float((a - b).norm() / b.norm())
+    line["roofline_smoothing"] = {
+        name: {which: hbm_roofline(bpc * cells, forms[name][f"{which}_us"], f"{kern} ({n}x{n}, {bpc:.0f} B/cell)",
+                                   {"launches_timed": forms[name][f"{which}_launches"]})
+               for which, kern, bpc in SMOOTHING[name]}
+        for name in forms}
     for name in forms:
         if name != head_name:
             line[f"roofline_{name}_form"] = form_roof(name)

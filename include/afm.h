@@ -61,6 +61,13 @@ const char* afm_last_error(void);
  * of iterations after the first), counting only launches that ran.  afm_update_timing(c, 1) enables and resets; 0 disables. */
 int afm_update_timing(afm_ctx* c, int enable);
 int afm_update_timing_read(afm_ctx* c, double* avg_us, long* launches);
+/* The same timing for three finest-level launches of afm_mg_solve above the
+ * LDS levels (no reference counterpart): kind 0 the outer update (as
+ * afm_update_timing_read), kind 1 the last post-smoothing of the CG forms'
+ * preconditioner cycle (precision 1 and 3), kind 2 the pre-smoothing +
+ * residual + restriction launch; launches of cycles that ran.  Enabled and
+ * reset by afm_update_timing. */
+int afm_kernel_timing_read(afm_ctx* c, int kind, double* avg_us, long* launches);
 
 /* Device memory on the context's GPU for callers that hold no HIP headers
  * (the C++ drop-in classes of include/afm/).  Copies are synchronous. */
