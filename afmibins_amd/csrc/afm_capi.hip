@@ -116,6 +116,7 @@ int afm_ctx_create(const afm_geom* g, int device, afm_ctx** out)
 		// the bin counts start at zero once; each spreading's scan returns
 		// them to zero after reading them
 		if (hipMemsetAsync(c->bins, 0, sizeof(int) * nb, c->stream) != hipSuccess) { set_error("memset failed"); return fail(-3); }
+		if ((rc = spread_setup(c))) return fail(rc);
 	}
 	if ((rc = mg_setup(c))) return fail(rc);
 	if (hipStreamSynchronize(c->stream) != hipSuccess) { set_error("sync failed"); return fail(-3); }

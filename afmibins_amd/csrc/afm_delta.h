@@ -72,7 +72,7 @@ AFM_HD inline dd dd_add(dd a, dd b)
 }
 
 // (-1)^j / (2j)!  and  (-1)^j / (2j+1)!  as double-double.
-AFM_HD inline dd cos_coef(int j)
+AFM_HD constexpr dd cos_coef(int j)
 {
 	switch (j) {
 	case 0: return {0x1.0000000000000p+0, 0x0.0p+0};
@@ -93,7 +93,7 @@ AFM_HD inline dd cos_coef(int j)
 	}
 }
 
-AFM_HD inline dd sin_coef(int j)
+AFM_HD constexpr dd sin_coef(int j)
 {
 	switch (j) {
 	case 0: return {0x1.0000000000000p+0, 0x0.0p+0};
@@ -112,6 +112,18 @@ AFM_HD inline dd sin_coef(int j)
 	default: return {-0x1.d1ab1c2dccea3p-94, -0x1.054d0c78aea14p-149};
 	}
 }
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// The fast path's coefficients of one parity (0 cos, 1 sin), in the order the
+// series takes them: c10 .. c5 (double), then c4 .. c0 as (hi, lo).  On the
+// GPU a lane reads its parity's row: the lanes of a wave alternate between
+// the two series, and loads replace the per-lane selection of every constant
+// between two register pairs.  The values are cos_coef / sin_coef's.
+#define AFM_FC(c) {c(10).hi, c(9).hi, c(8).hi, c(7).hi, c(6).hi, c(5).hi, c(4).hi, c(4).lo, c(3).hi, c(3).lo, \
+                   c(2).hi, c(2).lo, c(1).hi, c(1).lo, c(0).hi, c(0).lo}
+static __constant__ double AFM_FAST_COEF[2][16] = {AFM_FC(cos_coef), AFM_FC(sin_coef)};
+#undef AFM_FC
+#endif
 
 // a + b without the careful low-order sum of dd_add; fine when the sum does
 // not cancel (every step of the series below adds a smaller term to a larger)
@@ -148,6 +160,15 @@ AFM_HD inline double cos_cr(double x)
 	int q = ((int) k) & 3;
 	const bool odd = (q & 1) != 0;
 	// fast: tail c5..c10 in double (Horner on z.hi), then c4..c0 in dd
+#if defined(__HIP_DEVICE_COMPILE__)
+	const double* cf = AFM_FAST_COEF[odd ? 1 : 0];
+	double pt = cf[0];
+#pragma unroll
+	for (int j = 1; j < 6; ++j) pt = fma(pt, z.hi, cf[j]);
+	dd p = {pt, 0.0};
+#pragma unroll
+	for (int j = 0; j < 5; ++j) p = dd_add_nc(dd_mul(p, z), dd{cf[6 + 2 * j], cf[7 + 2 * j]});
+#else
 	double pt = odd ? sin_coef(10).hi : cos_coef(10).hi;
 #pragma unroll
 	for (int j = 9; j >= 5; --j) pt = fma(pt, z.hi, odd ? sin_coef(j).hi : cos_coef(j).hi);
@@ -157,6 +178,7 @@ AFM_HD inline double cos_cr(double x)
 		const dd cj = odd ? sin_coef(j) : cos_coef(j);
 		p = dd_add_nc(dd_mul(p, z), cj);
 	}
+#endif
 	const dd one = {1.0, 0.0};
 	dd r = dd_mul(p, odd ? y : one);   // times 1 is exact
 	const double e = 0x1p-70 * fabs(r.hi);

@@ -21,6 +21,8 @@ namespace afm {
 
 constexpr int SB = 4;             // spread bin edge (window base cells)
 constexpr int SCAN_BLOCKS = 1024; // at most this many scan chunks
+constexpr int GCLS = 4;           // density classes of gather tiles (records reaching the tile:
+                                  // >= 1024, >= 128, >= 1, none), launched densest first
 
 __global__ void k_zero_int(int* __restrict__ a, long count)
 {
@@ -194,11 +196,13 @@ __device__ inline int block_int_scan(int v, int* wsum, int& total)
 // counts cleared for the next spreading
 __global__ __launch_bounds__(256) void k_spread_scan(int* __restrict__ counts, const int* __restrict__ chunk,
                                                      int* __restrict__ offs, int* __restrict__ cursor, int nbins,
-                                                     int items)
+                                                     int items, int* __restrict__ ccount)
 {
 	__shared__ int red[256];
 	__shared__ int wsum[4];
 	const int c = blockIdx.x, tid = threadIdx.x;
+	// the gather order's class counts of this spreading start at zero
+	if (ccount && c == 0 && tid < 2 * GCLS) ccount[tid] = 0;
 	int base = 0;
 	for (int q = tid; q < c; q += 256) base += chunk[q];
 	base = block_int_sum(base, red);
@@ -358,10 +362,62 @@ __device__ inline void merge_terms(const BinRec* __restrict__ srec, const int* l
 	}
 }
 
+// The gather's launch order: each tile (one thread) joins the list of its
+// density class (the records that can reach its cells: those of the 9 x 9
+// bins the gather stages); gather block i takes the i-th tile of the lists
+// in class order, so the long merges of dense tiles start first instead of
+// behind the sparse ones.  The order within a class depends on the atomics,
+// but no tile's result depends on when or by which block it is computed.
+// Slots are claimed per wave (ballot), then per block (LDS), so a class
+// takes one global atomic per block.
+__global__ __launch_bounds__(256) void k_gather_order(int comp_mask, const int* __restrict__ offs, int sbx, int sby,
+                                                      int nb1, int ntiles, int* __restrict__ ccount,
+                                                      int* __restrict__ order)
+{
+	__shared__ int bcount[GCLS], bbase[GCLS];
+	const int comp = (int) blockIdx.y;
+	if (!((comp_mask >> comp) & 1)) return;
+	const int tile = (int) (blockIdx.x * 256 + threadIdx.x);
+	if (threadIdx.x < GCLS) bcount[threadIdx.x] = 0;
+	int cls = -1;
+	if (tile < ntiles) {
+		const int* o = offs + comp * nb1;
+		const int bpr = (sbx + GB - 1) / GB;
+		const int bx0 = GB * (tile % bpr), by0 = GB * (tile / bpr);
+		int total = 0;
+#pragma unroll
+		for (int ly = 0; ly <= GB; ++ly) {
+			// the wrapped bin left of the tile, then its GB bins (contiguous)
+			const int gy = by0 - 1 + ly < 0 ? sby - 1 : by0 - 1 + ly;
+			const int bl = (bx0 == 0 ? sbx - 1 : bx0 - 1) + sbx * gy, bb = bx0 + sbx * gy;
+			total += (o[bl + 1] - o[bl]) + (o[bb + GB] - o[bb]);
+		}
+		cls = total >= 1024 ? 0 : total >= 128 ? 1 : total >= 1 ? 2 : 3;
+	}
+	__syncthreads();
+	const int lane = (int) (threadIdx.x & 63);
+	int slot = 0;
+	for (int c2 = 0; c2 < GCLS; ++c2) {
+		const unsigned long long mask = __ballot(cls == c2);
+		if (!mask) continue;
+		const int leader = __ffsll((long long) mask) - 1;
+		int base = 0;
+		if (lane == leader) base = atomicAdd(&bcount[c2], __popcll(mask));
+		base = __shfl(base, leader);
+		if (cls == c2) slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+	}
+	__syncthreads();
+	if (threadIdx.x < GCLS)
+		bbase[threadIdx.x] = bcount[threadIdx.x] ? atomicAdd(&ccount[comp * GCLS + threadIdx.x], bcount[threadIdx.x]) : 0;
+	__syncthreads();
+	if (cls >= 0) order[(size_t) (comp * GCLS + cls) * ntiles + bbase[cls] + slot] = tile;
+}
+
 __global__ __launch_bounds__(256) void k_spread_gather(int comp_mask, const BinRec* __restrict__ srec,
                                                        const int* __restrict__ offs, unsigned nx, int sbx, int sby,
                                                        int nb1, double* __restrict__ fx, double* __restrict__ fy,
-                                                       int accumulate)
+                                                       int accumulate, const int* __restrict__ ccount,
+                                                       const int* __restrict__ order)
 {
 	constexpr int NR = 2 * (GB + 1);   // srec ranges staged per block
 	__shared__ int lkey[GCAP], lidx[GCAP];
@@ -372,7 +428,15 @@ __global__ __launch_bounds__(256) void k_spread_gather(int comp_mask, const BinR
 	// consecutive blocks go round-robin to the 8 XCDs, which spreads the
 	// long merges of a dense cluster over all of them
 	const int bpr = (sbx + GB - 1) / GB;
-	const int bx0 = GB * ((int) blockIdx.x % bpr), by0 = GB * ((int) blockIdx.x / bpr);
+	int tile = (int) blockIdx.x;
+	if (order) {
+		// the tile of this block in the densest-first order (uniform loads)
+		const int nt = (int) gridDim.x;
+		int cls = 0;
+		while (cls < GCLS - 1 && tile >= ccount[comp * GCLS + cls]) tile -= ccount[comp * GCLS + cls++];
+		tile = order[(size_t) (comp * GCLS + cls) * nt + tile];
+	}
+	const int bx0 = GB * (tile % bpr), by0 = GB * (tile / bpr);
 	const int* o = offs + comp * nb1;
 	const int ny = SB * sby;
 	// lane: cell row j of bin (tx, ty) of the block.  Its four bins' ranges
@@ -417,6 +481,16 @@ __global__ __launch_bounds__(256) void k_spread_gather(int comp_mask, const BinR
 		}
 		__syncthreads();
 		const int total = rp[NR];
+		if (total == 0) {
+			// no record reaches the block's cells (the block decides as one,
+			// after its last barrier): zeros, or the field left as it is
+			if (live && !accumulate) {
+				double2* r2 = reinterpret_cast<double2*>((comp == 0 ? fx : fy) + (size_t) nx * (SB * by + j) + SB * bx);
+				r2[0] = make_double2(0.0, 0.0);
+				r2[1] = make_double2(0.0, 0.0);
+			}
+			return;
+		}
 		staged = total >= GMIN && total <= GCAP;
 		if (staged) {
 			for (int p = tid; p < total; p += 256) {
@@ -516,6 +590,13 @@ int ensure_markers(Ctx* c, long m)
 	return 0;
 }
 
+// the gather order's class counts and tile lists, both components
+int spread_setup(Ctx* c)
+{
+	const size_t nt = (size_t) ((c->sbx + GB - 1) / GB) * ((c->sby + GB - 1) / GB);
+	return c->alloc((void**) &c->gorder, sizeof(int) * 2 * GCLS * (1 + nt));
+}
+
 int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, const double* ds, const double* X,
                   const double* R, const double* k, long m, double* fx, double* fy, int accumulate)
 {
@@ -527,6 +608,13 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 	int* offs = counts + nb;           // nb + 1 entries
 	int* cursor = offs + nb + 1;
 	int* chunk = cursor + nb;          // nchunk entries
+	const unsigned gblocks = (unsigned) (((c->sbx + GB - 1) / GB) * ((c->sby + GB - 1) / GB));
+	// densest-first order where the tiles are staged (grids of >= 8 x 8 bins)
+	// and markers are many per tile: its launch costs about 7 us, which the
+	// reordering repays where clusters make long merges (262,144 markers on
+	// 16,384 tiles: gather 91 -> 70 us), not with a few markers per tile
+	// (16,384 on 4,096: 32 -> 27 us)
+	const bool ordered = m >= 8L * (long) gblocks && c->gorder && c->sbx >= GB && c->sby >= GB;
 	if (m > 0) {
 		AFM_TRY(ensure_markers(c, m));
 		dim3 g((m + 255) / 256, 2), g8((8 * m + 255) / 256, 2);
@@ -534,7 +622,8 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 		                   c->D, c->nx, c->ny, c->sbx, nb1, c->rec, counts);
 		AFM_LAUNCH_CHECK();
 		hipLaunchKernelGGL(k_spread_chunk, dim3(nchunk), dim3(256), 0, c->stream, counts, chunk, nb, items);
-		hipLaunchKernelGGL(k_spread_scan, dim3(nchunk), dim3(256), 0, c->stream, counts, chunk, offs, cursor, nb, items);
+		hipLaunchKernelGGL(k_spread_scan, dim3(nchunk), dim3(256), 0, c->stream, counts, chunk, offs, cursor, nb, items,
+		                   ordered ? c->gorder : nullptr);
 		AFM_LAUNCH_CHECK();
 		hipLaunchKernelGGL(k_spread_fill, g, dim3(256), 0, c->stream, comp_mask, c->rec, m, nb1, offs, cursor,
 		                   c->bin_list);
@@ -548,9 +637,14 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 		                   (long) nb + 1);
 		AFM_LAUNCH_CHECK();
 	}
-	const unsigned gblocks = (unsigned) (((c->sbx + GB - 1) / GB) * ((c->sby + GB - 1) / GB));
+	if (ordered) {
+		hipLaunchKernelGGL(k_gather_order, dim3((gblocks + 255) / 256, 2), dim3(256), 0, c->stream, comp_mask, offs,
+		                   c->sbx, c->sby, nb1, (int) gblocks, c->gorder, c->gorder + 2 * GCLS);
+		AFM_LAUNCH_CHECK();
+	}
 	hipLaunchKernelGGL(k_spread_gather, dim3(gblocks, 2), dim3(256), 0, c->stream, comp_mask, c->srec, offs,
-	                   c->nx, c->sbx, c->sby, nb1, fx, fy, accumulate);
+	                   c->nx, c->sbx, c->sby, nb1, fx, fy, accumulate, ordered ? c->gorder : nullptr,
+	                   ordered ? c->gorder + 2 * GCLS : nullptr);
 	AFM_LAUNCH_CHECK();
 	return 0;
 }

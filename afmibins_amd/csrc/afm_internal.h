@@ -112,6 +112,7 @@ struct Ctx {
 	// spreading
 	MarkerRec* rec = nullptr;          // [2 * mcap]
 	int* bins = nullptr;               // counts | offsets | cursors | chunk sums, 2 comps
+	int* gorder = nullptr;             // gather order: class counts | tile lists, 2 comps
 	int* bin_list = nullptr;           // [2 * mcap]
 	BinRec* srec = nullptr;            // [2 * mcap], records in bin order
 	long mcap = 0;
@@ -134,6 +135,7 @@ struct Ctx {
 };
 
 int ensure_markers(Ctx* c, long m);
+int spread_setup(Ctx* c);
 
 // -------------------------------------------------------- launchers ----
 // IB transfer (afm_ib.hip)
