@@ -159,6 +159,59 @@ __device__ inline void ib_window(int comp, double cx, double cy, unsigned n, uns
 	}
 }
 
+// Per-launch constants of the 2-D window, formed once on the host by the
+// same IEEE operations every lane would otherwise repeat (divisions and
+// conversions of uniform values cost a lane several instructions each).
+struct WinGeom {
+	double h, half_h, nd, Wd, invW, Dd, invD, nxd, nyd, vol;
+	unsigned n, nx, ny;
+};
+
+inline WinGeom win_geom(unsigned n, unsigned W, unsigned D, unsigned nx, unsigned ny)
+{
+	WinGeom g;
+	g.h = 1.0 / n;
+	g.half_h = 0.5 * g.h;
+	g.nd = (double) n;
+	g.Wd = (double) W;
+	g.invW = 1.0 / g.Wd;
+	g.Dd = (double) D;
+	g.invD = 1.0 / g.Dd;
+	g.nxd = (double) nx;
+	g.nyd = (double) ny;
+	g.vol = 1. / (n * n);
+	g.n = n;
+	g.nx = nx;
+	g.ny = ny;
+	return g;
+}
+
+// fmod_pow2 with the period and its reciprocal given
+__device__ inline double fmod_pow2g(double x, double pd, double inv) { return x - pd * trunc(x * inv); }
+
+// ib_window_factor below with the launch's constants (the same operations)
+__device__ inline double ib_window_factor(int comp, double cx, double cy, const WinGeom& g, int d8, int& ix0,
+                                          int& iy0)
+{
+	double q0 = cx, q1 = cy;
+	if (comp == 0) q1 -= g.half_h; else q0 -= g.half_h;
+	q0 = fmod_pow2g(fmod_pow2g(q0, g.Wd, g.invW) + g.Wd, g.Wd, g.invW);
+	q1 = fmod_pow2g(fmod_pow2g(q1, g.Dd, g.invD) + g.Dd, g.Dd, g.invD);
+	double fx = floor(q0 * g.nd), fy = floor(q1 * g.nd);
+	if (!(fx >= 0.0 && fx <= g.nxd)) fx = 0.0;
+	if (!(fy >= 0.0 && fy <= g.nyd)) fy = 0.0;
+	ix0 = (int) (((unsigned) fx + g.nx - 1) & (g.nx - 1));
+	iy0 = (int) (((unsigned) fy + g.ny - 1) & (g.ny - 1));
+	if (d8 < 4) {
+		double x = (fx + d8 - 1.0) * g.h;
+		if (comp != 0) x += g.half_h;
+		return delta(x - cx, g.n);
+	}
+	double y = (fy + (d8 - 4) - 1.0) * g.h;
+	if (comp == 0) y += g.half_h;
+	return delta(y - cy, g.n);
+}
+
 // One factor of ib_window: lane d8 < 4 returns dx[d8], lane 4 + d returns
 // dy[d]; every lane returns the base cell.  Same operations as ib_window.
 __device__ inline double ib_window_factor(int comp, double cx, double cy, unsigned n, unsigned W, unsigned D,

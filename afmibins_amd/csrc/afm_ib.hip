@@ -39,19 +39,20 @@ __global__ void k_delta(const double* __restrict__ r, long count, unsigned n, do
 // Eight lanes per marker: lane d evaluates one delta factor (d < 4: dx[d],
 // else dy[d - 4]) and the products of taps d and d + 8 (tap a + 4 b, x
 // fastest); lane 0 of the group adds the 16 products in tap order.
-__global__ void k_interp(int comp, const double* __restrict__ f, const double* __restrict__ X, long m,
-                         unsigned n, unsigned W, unsigned D, unsigned nx, unsigned ny, double* __restrict__ U)
+__global__ void k_interp(int comp, const double* __restrict__ f, const double* __restrict__ X, long m, WinGeom g,
+                         double* __restrict__ U)
 {
+	const unsigned nx = g.nx, ny = g.ny;
 	const long t = blockIdx.x * (long) blockDim.x + threadIdx.x;
 	const long k = t >> 3;
 	const int d8 = (int) (t & 7);
 	// every lane of the wave takes part in the shuffles: no early exit
 	const long kk = k < m ? k : m - 1;
 	int ix0, iy0;
-	const double fct = ib_window_factor(comp, X[2 * kk], X[2 * kk + 1], n, W, D, nx, ny, d8, ix0, iy0);
+	const double fct = ib_window_factor(comp, X[2 * kk], X[2 * kk + 1], g, d8, ix0, iy0);
 	const int a = d8 & 3, b = d8 >> 2;
 	const double wx = __shfl(fct, a, 8), wy0 = __shfl(fct, 4 + b, 8), wy1 = __shfl(fct, 6 + b, 8);
-	const double vol = 1. / (n * n);
+	const double vol = g.vol;
 	int ix = ix0 + a; if (ix >= (int) nx) ix -= nx;
 	int iy = iy0 + b; if (iy >= (int) ny) iy -= ny;
 	int iy2 = iy0 + b + 2; if (iy2 >= (int) ny) iy2 -= ny;
@@ -111,9 +112,8 @@ __global__ void k_ib_move_full(const double* __restrict__ uo, const double* __re
 // F[comp][k].
 __global__ void k_spread_prep(int comp_mask, const double* __restrict__ Fx, const double* __restrict__ Fy,
                               const double* __restrict__ ds, const double* __restrict__ X,
-                              const double* __restrict__ R, const double* __restrict__ kk, long m,
-                              unsigned n, unsigned W, unsigned D, unsigned nx, unsigned ny, int sbx, int nb1,
-                              MarkerRec* __restrict__ rec, int* __restrict__ counts)
+                              const double* __restrict__ R, const double* __restrict__ kk, long m, WinGeom g,
+                              int sbx, int nb1, MarkerRec* __restrict__ rec, int* __restrict__ counts)
 {
 	const long t = blockIdx.x * (long) blockDim.x + threadIdx.x;
 	const long k = t >> 3;
@@ -122,7 +122,7 @@ __global__ void k_spread_prep(int comp_mask, const double* __restrict__ Fx, cons
 	if (k >= m || !((comp_mask >> comp) & 1)) return;
 	const double cx = X[2 * k], cy = X[2 * k + 1];
 	int ix0, iy0;
-	const double f = ib_window_factor(comp, cx, cy, n, W, D, nx, ny, d8, ix0, iy0);
+	const double f = ib_window_factor(comp, cx, cy, g, d8, ix0, iy0);
 	MarkerRec& r = rec[comp * m + k];
 	if (d8 < 4) r.dx[d8] = f;
 	else r.dy[d8 - 4] = f;
@@ -549,8 +549,8 @@ int launch_delta(Ctx* c, const double* r, long count, unsigned n, double* out)
 int launch_interp(Ctx* c, int comp, const double* field, const double* X, long m, double* U)
 {
 	if (m <= 0) return 0;
-	hipLaunchKernelGGL(k_interp, dim3((8 * m + 255) / 256), dim3(256), 0, c->stream, comp, field, X, m, c->n, c->W,
-	                   c->D, c->nx, c->ny, U);
+	hipLaunchKernelGGL(k_interp, dim3((8 * m + 255) / 256), dim3(256), 0, c->stream, comp, field, X, m,
+	                   win_geom(c->n, c->W, c->D, c->nx, c->ny), U);
 	AFM_LAUNCH_CHECK();
 	return 0;
 }
@@ -618,8 +618,8 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 	if (m > 0) {
 		AFM_TRY(ensure_markers(c, m));
 		dim3 g((m + 255) / 256, 2), g8((8 * m + 255) / 256, 2);
-		hipLaunchKernelGGL(k_spread_prep, g8, dim3(256), 0, c->stream, comp_mask, Fx, Fy, ds, X, R, k, m, c->n, c->W,
-		                   c->D, c->nx, c->ny, c->sbx, nb1, c->rec, counts);
+		hipLaunchKernelGGL(k_spread_prep, g8, dim3(256), 0, c->stream, comp_mask, Fx, Fy, ds, X, R, k, m,
+		                   win_geom(c->n, c->W, c->D, c->nx, c->ny), c->sbx, nb1, c->rec, counts);
 		AFM_LAUNCH_CHECK();
 		hipLaunchKernelGGL(k_spread_chunk, dim3(nchunk), dim3(256), 0, c->stream, counts, chunk, nb, items);
 		hipLaunchKernelGGL(k_spread_scan, dim3(nchunk), dim3(256), 0, c->stream, counts, chunk, offs, cursor, nb, items,
