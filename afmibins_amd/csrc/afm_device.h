@@ -86,6 +86,28 @@ __device__ inline T stencil5(T off, T dg, T c, T e, T w, T nn, T s, bool we, boo
 	return v;
 }
 
+// stencil5 for power-of-two coefficients (the multigrid laplacian's n^2 and
+// -4 n^2): off * x is then exact, so fma(off, x, v) rounds exactly as
+// v + off * x does -- the same bits in one instruction instead of two
+__device__ inline float fma_p2(float a, float x, float v) { return __builtin_fmaf(a, x, v); }
+__device__ inline double fma_p2(double a, double x, double v) { return __builtin_fma(a, x, v); }
+
+template<typename T, bool DIAG>
+__device__ inline T stencil5p2(T off, T dg, T c, T e, T w, T nn, T s, bool we, bool ww, bool wn, bool ws)
+{
+	T v = T(0);
+	if (wn) v = fma_p2(off, nn, v);
+	if (!ws) v = fma_p2(off, s, v);
+	if (we) v = fma_p2(off, e, v);
+	if (!ww) v = fma_p2(off, w, v);
+	if (DIAG) v = fma_p2(dg, c, v);
+	if (!we) v = fma_p2(off, e, v);
+	if (ww) v = fma_p2(off, w, v);
+	if (!wn) v = fma_p2(off, nn, v);
+	if (ws) v = fma_p2(off, s, v);
+	return v;
+}
+
 // General row of the 2-D laplacian (any nx, ny >= 1), mg/laplacian.h:50-150
 // with ny(reference) == 1 and the reference's nx/nz == 1, 2 folding.
 struct LapRow {

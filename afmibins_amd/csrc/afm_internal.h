@@ -102,7 +102,7 @@ struct Ctx {
 	double *XSa = nullptr, *XSb = nullptr, *RES = nullptr, *RESb = nullptr;
 	float *RES32 = nullptr, *RES32b = nullptr;
 	double* upartials = nullptr;       // per-block sums of the multigrid update
-	double* P64[2] = {};               // flexible CG search directions (ping-pong)
+	double* P64[2] = {};               // flexible CG search directions (ping-pong; FP32 in the mixed form)
 	double* fpartials = nullptr;       // per-block z.r, z.r_prev, z.Lz [3 * fstride]
 	unsigned fstride = 0;
 	double* gpartials[2] = {};         // per-block r.r, p.r, p.Lp (3 columns), by iteration parity

@@ -134,9 +134,9 @@ struct Strip {
 	{
 		if (EDGE) {
 			const int px = wrap_once(gxu + o, nx), py = wrap_once(gyb + i, ny);
-			return stencil5<U, DIAG>(off, dg, c, E, W, N, Sv, px == nx - 1, px == 0, py == ny - 1, py == 0);
+			return stencil5p2<U, DIAG>(off, dg, c, E, W, N, Sv, px == nx - 1, px == 0, py == ny - 1, py == 0);
 		}
-		return stencil5<U, DIAG>(off, dg, c, E, W, N, Sv, false, false, false, false);
+		return stencil5p2<U, DIAG>(off, dg, c, E, W, N, Sv, false, false, false, false);
 	}
 
 	template<int MODE>
@@ -592,9 +592,13 @@ __global__ __launch_bounds__(64 * SmoothShape<S>::NW) void k_smooth_update(
 // p, x and r in one pass.  The residual is formed in FP64 (its sum of
 // squares, the stop test, p.r) and stored rounded to the cycle's type: the
 // cycle takes it so, and beta's and alpha's z.r terms use that same rounded
-// vector (products and sums in FP64).  A rounding there changes only the
-// step coefficients, by about 1e-7 relative; the search directions, the
-// solution and the tested residual stay FP64.  All sums are fixed-order.
+// vector (products and sums in FP64).  The search direction p is formed in
+// FP64 and stored rounded to the cycle's type too (the next iteration reads
+// it so; every tile forms its halo's p from the same stored values, so the
+// tiles agree): the recurrence for p.L p then holds to about 1e-7 relative.
+// These roundings change only the step coefficients, by about 1e-7
+// relative, and the search directions by FP32 rounding; the solution and
+// the tested residual stay FP64.  All sums are fixed-order.
 
 // block sums with an explicit column stride (the dots launch can have more
 // blocks than MAX_RED_BLOCKS)
@@ -651,7 +655,7 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const TE* __restrict__ z, cons
 		const size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;
 		const size_t in_ = gy == ny - 1 ? (size_t) gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;
 		const double zc = (double) z[i];
-		const double lz = stencil5<double, true>(off, dg, zc, (double) z[ie], (double) z[iw], (double) z[in_],
+		const double lz = stencil5p2<double, true>(off, dg, zc, (double) z[ie], (double) z[iw], (double) z[in_],
 		                                         (double) z[is], gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
 		acc[0] += zc * (double) rk[i];
 		acc[1] += zc * (double) rkm1[i];
@@ -663,8 +667,8 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const TE* __restrict__ z, cons
 template<typename TE>
 struct FcgArgs {
 	const TE* z;              // z_k
-	const double* p_old;      // p_{k-1}; nullptr at k == 0
-	double* p_new;            // p_k
+	const TE* p_old;          // p_{k-1}; nullptr at k == 0
+	TE* p_new;                // p_k, stored in the cycle's type
 	const double* x_old;      // x_k; nullptr when x_0 = 0 (k == 0 without a start)
 	double* x_new;            // x_{k+1}
 	const double* b;          // right-hand side, mean removed
@@ -768,20 +772,20 @@ __global__ __launch_bounds__(64 * FU_NW) void k_fcg_update(FcgArgs<TE> a, int nx
 		const int gyb = Y0 - FU_H + FU_RPW * w;
 		double pr[FU_RPW][2], xr[FU_RPW][2];
 		{
-			TE2 zv[FU_RPW];
-			double2 pv[FU_RPW], xv[FU_RPW];
+			TE2 zv[FU_RPW], pv[FU_RPW];
+			double2 xv[FU_RPW];
 #pragma unroll
 			for (int i = 0; i < FU_RPW; ++i) {
 				const size_t at = (size_t) nx * wrap_once(gyb + i, ny) + gx;
 				zv[i] = *(const TE2*) (a.z + at);
-				if (!FIRST) pv[i] = *(const double2*) (a.p_old + at);
+				if (!FIRST) pv[i] = *(const TE2*) (a.p_old + at);
 				if (!XZ) xv[i] = *(const double2*) (a.x_old + at);
 			}
 #pragma unroll
 			for (int i = 0; i < FU_RPW; ++i) {
 				const double z0 = (double) zv[i].x, z1 = (double) zv[i].y;
-				pr[i][0] = FIRST ? z0 : z0 + beta * pv[i].x;
-				pr[i][1] = FIRST ? z1 : z1 + beta * pv[i].y;
+				pr[i][0] = FIRST ? z0 : z0 + beta * (double) pv[i].x;
+				pr[i][1] = FIRST ? z1 : z1 + beta * (double) pv[i].y;
 				xr[i][0] = XZ ? alpha * pr[i][0] : xv[i].x + alpha * pr[i][0];
 				xr[i][1] = XZ ? alpha * pr[i][1] : xv[i].y + alpha * pr[i][1];
 			}
@@ -807,9 +811,9 @@ __global__ __launch_bounds__(64 * FU_NW) void k_fcg_update(FcgArgs<TE> a, int nx
 				const int px = wrap_once(gx + o, nx), py = wrap_once(gyb + i, ny);
 				const bool we = px == nx - 1, ww = px == 0, wn = py == ny - 1, ws = py == 0;
 				strip_nbrs<double, FU_RPW>(xr, i, o, xu[o], xd[o], E, W, Nn, S);
-				Lx[o] = stencil5<double, true>(off, dg, xr[i][o], E, W, Nn, S, we, ww, wn, ws);
+				Lx[o] = stencil5p2<double, true>(off, dg, xr[i][o], E, W, Nn, S, we, ww, wn, ws);
 				strip_nbrs<double, FU_RPW>(pr, i, o, pu[o], pd[o], E, W, Nn, S);
-				Lp[o] = stencil5<double, true>(off, dg, pr[i][o], E, W, Nn, S, we, ww, wn, ws);
+				Lp[o] = stencil5p2<double, true>(off, dg, pr[i][o], E, W, Nn, S, we, ww, wn, ws);
 			}
 			const int ly = FU_RPW * w + i;
 			if (!(oc && ly >= FU_H && ly < FU_LY - FU_H && gyb + i < ny)) continue;
@@ -823,10 +827,11 @@ __global__ __launch_bounds__(64 * FU_NW) void k_fcg_update(FcgArgs<TE> a, int nx
 				acc[1] += pr[i][o] * rv[o];
 				acc[2] += pr[i][o] * Lp[o];
 			}
-			double2 po, xo;
-			po.x = pr[i][0]; po.y = pr[i][1];
+			TE2 po;
+			double2 xo;
+			po.x = (TE) pr[i][0]; po.y = (TE) pr[i][1];
 			xo.x = xr[i][0]; xo.y = xr[i][1];
-			*(double2*) (a.p_new + at) = po;
+			*(TE2*) (a.p_new + at) = po;
 			*(double2*) (a.x_new + at) = xo;
 			TE2 ro;
 			ro.x = (TE) rv[0]; ro.y = (TE) rv[1];
@@ -855,7 +860,7 @@ __global__ __launch_bounds__(256) void k_fcg_update_1d(FcgArgs<TE> a, int nx, in
 		for (int t = 0; t < 5; ++t) zv[t] = (double) a.z[nb[t]];
 		if (!FIRST) {
 #pragma unroll
-			for (int t = 0; t < 5; ++t) pv[t] = a.p_old[nb[t]];
+			for (int t = 0; t < 5; ++t) pv[t] = (double) a.p_old[nb[t]];
 		}
 		if (!XZ) {
 #pragma unroll
@@ -869,10 +874,10 @@ __global__ __launch_bounds__(256) void k_fcg_update_1d(FcgArgs<TE> a, int nx, in
 			xv[t] = XZ ? alpha * p : xv[t] + alpha * p;
 		}
 		const bool we = gx == nx - 1, ww = gx == 0, wn = gy == ny - 1, ws = gy == 0;
-		const double Lx = stencil5<double, true>(off, dg, xv[0], xv[1], xv[2], xv[3], xv[4], we, ww, wn, ws);
-		const double Lp = stencil5<double, true>(off, dg, pv[0], pv[1], pv[2], pv[3], pv[4], we, ww, wn, ws);
+		const double Lx = stencil5p2<double, true>(off, dg, xv[0], xv[1], xv[2], xv[3], xv[4], we, ww, wn, ws);
+		const double Lp = stencil5p2<double, true>(off, dg, pv[0], pv[1], pv[2], pv[3], pv[4], we, ww, wn, ws);
 		const double r = bi - Lx;
-		a.p_new[i] = pv[0];
+		a.p_new[i] = (TE) pv[0];
 		a.x_new[i] = xv[0];
 		a.r[i] = (TE) r;
 		acc[0] += r * r;
@@ -960,7 +965,7 @@ __global__ __launch_bounds__(256) void k_resid_restr(const TS* __restrict__ x, c
 				gx += (gx < 0) ? nx : 0; gx -= (gx >= nx) ? nx : 0;
 				gy += (gy < 0) ? ny : 0; gy -= (gy >= ny) ? ny : 0;
 				const int li = (lx + 1) + LX * (ly + 1);
-				T Lx = stencil5<T, true>(off, dg, xs[li], xs[li + 1], xs[li - 1], xs[li + LX], xs[li - LX],
+				T Lx = stencil5p2<T, true>(off, dg, xs[li], xs[li + 1], xs[li - 1], xs[li + LX], xs[li - LX],
 				                         gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
 				rs[p] = bv[k] - Lx;
 			} else {
@@ -1023,7 +1028,7 @@ __global__ __launch_bounds__(256) void k_update_resid(const double* __restrict__
 		}
 		double r0 = rhs[i];
 		if (sub_mean) { r0 = r0 - mean * 1.0; rhs[i] = r0; acc[1] += r0 * r0; }
-		double Lx = stencil5<double, true>(off, dg, c, Ev, Wv, Nv, Sv, gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
+		double Lx = stencil5p2<double, true>(off, dg, c, Ev, Wv, Nv, Sv, gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
 		double r = r0 - Lx;
 		res[i] = (TR) r;
 		acc[0] += r * r;
@@ -1230,10 +1235,10 @@ __device__ void c_smooth(double* x, const double* b, bool zero, const double* xc
 						const int gy = k >> (L - 1), gx = ((k & (hx - 1)) << 1) + ((color + gy) & 1);
 						const CellOps o = cell_ops<L>(gx, gy);
 						double value = 0.0;
-						value += off * x[o.a0];
-						value += off * x[o.a1];
-						value += off * x[o.a2];
-						value += off * x[o.a3];
+						value = fma_p2(off, x[o.a0], value);   // off = n^2: exact products
+						value = fma_p2(off, x[o.a1], value);
+						value = fma_p2(off, x[o.a2], value);
+						value = fma_p2(off, x[o.a3], value);
 						x[o.i] = (b[o.i] - value) * inv;
 					}
 					csync<NT>();
@@ -1256,7 +1261,7 @@ __device__ void c_smooth(double* x, const double* b, bool zero, const double* xc
 							Nv = Nv + prolong_at(xc, gx, ny_, hx, hx);
 							Sv = Sv + prolong_at(xc, gx, sy, hx, hx);
 						}
-						value = stencil5<double, false>(off, dg, 0.0, E, Wv, Nv, Sv, gx == msk, gx == 0, gy == msk, gy == 0);
+						value = stencil5p2<double, false>(off, dg, 0.0, E, Wv, Nv, Sv, gx == msk, gx == 0, gy == msk, gy == 0);
 					}
 					x[i] = (b[i] - value) * inv;
 				}
@@ -1282,7 +1287,7 @@ __device__ void c_restrict(CoarseLDS& S, const double* x, const double* src, dou
 			double y;
 			if constexpr (L >= 2) {
 				const int ex = (gx + 1) & msk, wx = (gx - 1) & msk, ny_ = (gy + 1) & msk, sy = (gy - 1) & msk;
-				y = stencil5<double, true>(off, -4.0 * off, x[i], x[ex + (gy << L)], x[wx + (gy << L)],
+				y = stencil5p2<double, true>(off, -4.0 * off, x[i], x[ex + (gy << L)], x[wx + (gy << L)],
 				                           x[gx + (ny_ << L)], x[gx + (sy << L)], gx == msk, gx == 0, gy == msk, gy == 0);
 			} else {
 				// folded 2x2 rows in stored order: (0,0) d,xo,yo  (1,0) xo,d,yo
@@ -1883,7 +1888,7 @@ static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 				hipLaunchKernelGGL((k_fcg_dots<T>), dim3(rb), dim3(256), 0, c->stream, z, rk, rkm1, nx, ny, off, dg,
 				                   c->fpartials);
 			}
-			const FcgArgs<T> a{z, k ? c->P64[(k + 1) & 1] : nullptr, c->P64[k & 1], k ? xs[k & 1] : x0,
+			const FcgArgs<T> a{z, k ? (const T*) c->P64[(k + 1) & 1] : nullptr, (T*) c->P64[k & 1], k ? xs[k & 1] : x0,
 			                   xs[(k + 1) & 1], rhs, r[(k + 1) & 1], dctl, c->fpartials, fstride, fnblk,
 			                   c->gpartials[k & 1], k};
 			const int slot = (c->kt_on && k >= 1 && s < Ctx::KT_SLOTS) ? s : -1;
@@ -2065,7 +2070,7 @@ __global__ void k_lap_apply(const double* __restrict__ x, double* __restrict__ y
 	cell_xy(i, nx, gx, gy);
 	size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;
 	size_t in_ = gy == ny - 1 ? gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;
-	y[i] = stencil5<double, true>(off, dg, x[i], x[ie], x[iw], x[in_], x[is], gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
+	y[i] = stencil5p2<double, true>(off, dg, x[i], x[ie], x[iw], x[in_], x[is], gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
 }
 
 static int check_level(Ctx* c, unsigned level)

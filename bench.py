@@ -25,9 +25,10 @@ synchronisation.
 
 `roofline` is the headline form's largest FP64 launch, timed with HIP events
 on the stream it runs on during extra steps of the same workload: the CG
-update k_fcg_update (iterations after the first), which reads z (FP32, 4 B),
-p, x and b (FP64, 8 + 8 + 8) and writes p and x (8 + 8) and r rounded to the
-cycle's FP32 (4): 48 B per finest cell (56 B with FP64 cycles).  The FP64
+update k_fcg_update (iterations after the first), which reads z and p
+(stored in the cycle's FP32, 4 + 4 B), x and b (FP64, 8 + 8) and writes p
+(4), x (8) and r rounded to FP32 (4): 40 B per finest cell (56 B with FP64
+cycles).  The FP64
 form's is the finest-level post-smoothing fused with the outer update
 (k_smooth_update): read e_pre and b (8 + 8), the coarse correction (8 per 4
 cells = 2), xs and rhs (8 + 8); write xs and r (8 + 8): 50 B (37 B with FP32
@@ -52,7 +53,7 @@ HBM_PEAK_GBS = 8000.0
 # form: (precision code of afm_mg_solve, timed launch, its bytes per finest cell, counter summary)
 FORMS = {
     "fp64": (0, "k_smooth_update<double, double, 2>", 50.0, None),
-    "mixed": (1, "k_fcg_update<float>", 48.0, "k_fcg_update_f32_2048_pmc.txt"),
+    "mixed": (1, "k_fcg_update<float>", 40.0, "k_fcg_update_f32_2048_pmc.txt"),
     "mixed_ir": (2, "k_smooth_update<float, float, 2>", 37.0, "k_smooth_update_f32_2048_pmc.txt"),
     "fp64_cg": (3, "k_fcg_update<double>", 56.0, None),
 }
