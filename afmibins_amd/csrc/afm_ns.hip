@@ -220,6 +220,12 @@ __device__ inline bool cg_stop_test(CGArgs& a, int f, int it, double n1, double&
 // (alg/cg.h:39 and 51-52 of the previous iteration), stored; partial p'.(A p').
 // With fold, every block first does the previous iteration's stop test from
 // the update's partials (block 0 records it), instead of a separate launch.
+// TP: the type p is stored in -- double for the reference's iteration; float
+// in the mixed form, whose p' is formed in FP64 and stored rounded (every
+// thread forms its neighbours' p' from the same stored values, and x and r
+// are both updated with the stored p, so r stays b - A x; only the step
+// length sees the rounding, about 1e-7 relative).
+template<typename TP>
 __global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double off, double dg, int pin, int it,
                                               int fold, unsigned nblk)
 {
@@ -233,8 +239,8 @@ __global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double o
 		if (cg_stop_test(a, f, it, n1, mu, blockIdx.x == 0 && threadIdx.x == 0)) return;
 	}
 	const size_t N = (size_t) nx * ny;
-	const double* p = a.p[f][pin];
-	double* po = a.p[f][pin ^ 1];
+	const TP* p = (const TP*) a.p[f][pin];
+	TP* po = (TP*) a.p[f][pin ^ 1];
 	const double* r = a.r[f];
 	const bool first = it == 0;
 	double acc[1] = {0.0};
@@ -243,11 +249,11 @@ __global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double o
 		double pc, pe, pw, pn, ps;
 		if (first) { pc = r[i]; pe = r[ie]; pw = r[iw]; pn = r[in_]; ps = r[is]; }   // p = r
 		else {
-			pc = p[i] * mu + r[i]; pe = p[ie] * mu + r[ie]; pw = p[iw] * mu + r[iw];
-			pn = p[in_] * mu + r[in_]; ps = p[is] * mu + r[is];
+			pc = (double) p[i] * mu + r[i]; pe = (double) p[ie] * mu + r[ie]; pw = (double) p[iw] * mu + r[iw];
+			pn = (double) p[in_] * mu + r[in_]; ps = (double) p[is] * mu + r[is];
 		}
 		double z = stencil5<double, true>(off, dg, pc, pe, pw, pn, ps, we, ww, wn, ws);
-		po[i] = pc;
+		po[i] = (TP) pc;
 		acc[0] += pc * z;
 	}
 	reduce_partial<1>(acc, a.pp[f]);
@@ -256,6 +262,7 @@ __global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double o
 // Second half: nu = n0 / (p.z) from the p kernel's partials (every block,
 // same bits), z = A p (recomputed, same bits), x += p nu, r -= z nu;
 // partial r.r
+template<typename TP>
 __global__ __launch_bounds__(256) void k_cg_update(CGArgs a, int nx, int ny, double off, double dg, int pcur, int it,
                                                    unsigned nblk)
 {
@@ -267,12 +274,13 @@ __global__ __launch_bounds__(256) void k_cg_update(CGArgs a, int nx, int ny, dou
 	const size_t N = (size_t) nx * ny;
 	double* x = a.x[f];
 	double* r = a.r[f];
-	const double* p = a.p[f][pcur];
+	const TP* p = (const TP*) a.p[f][pcur];
 	double acc[1] = {0.0};
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
 		CG_NEIGHBOURS
-		const double pc = p[i];
-		double z = stencil5<double, true>(off, dg, pc, p[ie], p[iw], p[in_], p[is], we, ww, wn, ws);
+		const double pc = (double) p[i];
+		double z = stencil5<double, true>(off, dg, pc, (double) p[ie], (double) p[iw], (double) p[in_], (double) p[is],
+		                                  we, ww, wn, ws);
 		x[i] += pc * nu;
 		z *= nu;
 		double ri = r[i] - z;
@@ -532,7 +540,7 @@ struct AdvectInit {
 };
 
 int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* const b[2], double tol, int tol_mode,
-           int max_iters, int iters[2], double res[2], const AdvectInit* adv = nullptr)
+           int max_iters, int iters[2], double res[2], const AdvectInit* adv = nullptr, bool p32 = false)
 {
 	double off, dg;
 	helm_coef(c, scale, off, dg);
@@ -578,10 +586,17 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 		// before it; the batch's last update is finalised on its own
 		for (int s = 0; s < batch; ++s) {
 			const int pin = enq & 1;
-			hipLaunchKernelGGL(k_cg_p, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg, pin,
-			                   enq, s > 0 ? 1 : 0, rb);
-			hipLaunchKernelGGL(k_cg_update, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg,
-			                   pin ^ 1, enq, rb);
+			if (p32) {
+				hipLaunchKernelGGL(k_cg_p<float>, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off,
+				                   dg, pin, enq, s > 0 ? 1 : 0, rb);
+				hipLaunchKernelGGL(k_cg_update<float>, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny,
+				                   off, dg, pin ^ 1, enq, rb);
+			} else {
+				hipLaunchKernelGGL(k_cg_p<double>, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off,
+				                   dg, pin, enq, s > 0 ? 1 : 0, rb);
+				hipLaunchKernelGGL(k_cg_update<double>, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny,
+				                   off, dg, pin ^ 1, enq, rb);
+			}
 			AFM_LAUNCH_CHECK();
 			++enq;
 		}
@@ -698,7 +713,8 @@ int ns_step(NS* s, const double* fx, const double* fy)
 	const AdvectInit adv{fx, fy, s->uo, s->vo, s->dt, s->rho, hoff, hdg, s->half ? 0 : 1, neumann ? 1 : 0};
 	double* xs[2] = {s->u, s->v};
 	const double* bs[2] = {nullptr, nullptr};
-	AFM_TRY(op_cg2(c, s->cg, ss, xs, bs, s->tol, s->tol_mode, s->max_cg, s->last_cg, nullptr, &adv));
+	// the mixed form stores the CG's search directions in FP32
+	AFM_TRY(op_cg2(c, s->cg, ss, xs, bs, s->tol, s->tol_mode, s->max_cg, s->last_cg, nullptr, &adv, s->precision == 1));
 	AFM_TRY(ns_project(s, s->u, s->v, s->half ? 1 : 0));
 	s->half = !s->half;
 	return 0;
