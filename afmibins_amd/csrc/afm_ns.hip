@@ -220,12 +220,14 @@ __device__ inline bool cg_stop_test(CGArgs& a, int f, int it, double n1, double&
 // (alg/cg.h:39 and 51-52 of the previous iteration), stored; partial p'.(A p').
 // With fold, every block first does the previous iteration's stop test from
 // the update's partials (block 0 records it), instead of a separate launch.
-// TP: the type p is stored in -- double for the reference's iteration; float
-// in the mixed form, whose p' is formed in FP64 and stored rounded (every
-// thread forms its neighbours' p' from the same stored values, and x and r
-// are both updated with the stored p, so r stays b - A x; only the step
-// length sees the rounding, about 1e-7 relative).
-template<typename TP>
+// TP, TR: the types p and r are stored in -- double for the reference's
+// iteration; float in the mixed form, whose p' and r are formed in FP64 and
+// stored rounded (every thread forms its neighbours' p' from the same stored
+// values, and x and r are both updated with the stored p, so r stays b - A x
+// up to the roundings of r, about 1e-7 of the residuals before it; the step
+// length sees the rounding of p, about 1e-7 relative; r.r and the stop test
+// take each new residual before its rounding).
+template<typename TP, typename TR>
 __global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double off, double dg, int pin, int it,
                                               int fold, unsigned nblk)
 {
@@ -241,7 +243,7 @@ __global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double o
 	const size_t N = (size_t) nx * ny;
 	const TP* p = (const TP*) a.p[f][pin];
 	TP* po = (TP*) a.p[f][pin ^ 1];
-	const double* r = a.r[f];
+	const TR* r = (const TR*) a.r[f];
 	const bool first = it == 0;
 	double acc[1] = {0.0};
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
@@ -249,8 +251,9 @@ __global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double o
 		double pc, pe, pw, pn, ps;
 		if (first) { pc = r[i]; pe = r[ie]; pw = r[iw]; pn = r[in_]; ps = r[is]; }   // p = r
 		else {
-			pc = (double) p[i] * mu + r[i]; pe = (double) p[ie] * mu + r[ie]; pw = (double) p[iw] * mu + r[iw];
-			pn = (double) p[in_] * mu + r[in_]; ps = (double) p[is] * mu + r[is];
+			pc = (double) p[i] * mu + (double) r[i]; pe = (double) p[ie] * mu + (double) r[ie];
+			pw = (double) p[iw] * mu + (double) r[iw]; pn = (double) p[in_] * mu + (double) r[in_];
+			ps = (double) p[is] * mu + (double) r[is];
 		}
 		double z = stencil5<double, true>(off, dg, pc, pe, pw, pn, ps, we, ww, wn, ws);
 		po[i] = (TP) pc;
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double o
 // Second half: nu = n0 / (p.z) from the p kernel's partials (every block,
 // same bits), z = A p (recomputed, same bits), x += p nu, r -= z nu;
 // partial r.r
-template<typename TP>
+template<typename TP, typename TR>
 __global__ __launch_bounds__(256) void k_cg_update(CGArgs a, int nx, int ny, double off, double dg, int pcur, int it,
                                                    unsigned nblk)
 {
@@ -273,7 +276,7 @@ __global__ __launch_bounds__(256) void k_cg_update(CGArgs a, int nx, int ny, dou
 	const double nu = c->n0p[it & 1] / block_total(a.pp[f], nblk);
 	const size_t N = (size_t) nx * ny;
 	double* x = a.x[f];
-	double* r = a.r[f];
+	TR* r = (TR*) a.r[f];
 	const TP* p = (const TP*) a.p[f][pcur];
 	double acc[1] = {0.0};
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
@@ -283,8 +286,8 @@ __global__ __launch_bounds__(256) void k_cg_update(CGArgs a, int nx, int ny, dou
 		                                  we, ww, wn, ws);
 		x[i] += pc * nu;
 		z *= nu;
-		double ri = r[i] - z;
-		r[i] = ri;
+		double ri = (double) r[i] - z;
+		r[i] = (TR) ri;
 		acc[0] += ri * ri;
 	}
 	reduce_partial<1>(acc, a.pu[f]);
@@ -428,6 +431,7 @@ __global__ __launch_bounds__(256) void k_cg_start_x0(CGArgs a, const double* __r
 		x[i] = 3.0 * b[i] - 3.0 * cb[i] + helm_row_at(cb, i, nx, ny, N, off, dg);
 }
 
+template<typename TR>
 __global__ __launch_bounds__(256) void k_cg_start_r(CGArgs a, const double* __restrict__ b0,
                                                     const double* __restrict__ b1, int nx, int ny, double off,
                                                     double dg)
@@ -436,11 +440,11 @@ __global__ __launch_bounds__(256) void k_cg_start_r(CGArgs a, const double* __re
 	const double* b = f ? b1 : b0;
 	const size_t N = (size_t) nx * ny;
 	const double* x = a.x[f];
-	double* r = a.r[f];
+	TR* r = (TR*) a.r[f];
 	double acc[1] = {0.0};
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
 		const double ri = b[i] - helm_row_at(x, i, nx, ny, N, off, dg);
-		r[i] = ri;
+		r[i] = (TR) ri;
 		acc[0] += ri * ri;
 	}
 	reduce_partial<1>(acc, a.pu[f]);
@@ -557,6 +561,8 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 	}
 	a.tol_mode = tol_mode;
 	const unsigned rb = rblocks(c->N);
+	// FP32 r only where the Neumann start writes it (k_cg_start_r)
+	const bool r32 = p32 && adv && adv->neumann;
 	if (adv && adv->neumann) {
 		// b goes to the first p buffers and c = A b to the second: the first
 		// p update reads r and writes p[.][1], the second writes p[.][0]
@@ -567,8 +573,12 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 		                   w.p[1][1], (int) c->nx, (int) c->ny, off, dg);
 		hipLaunchKernelGGL(k_cg_start_x0, dim3(rb, 2), dim3(256), 0, c->stream, a, w.p[0][0], w.p[1][0], w.p[0][1],
 		                   w.p[1][1], (int) c->nx, (int) c->ny, off, dg);
-		hipLaunchKernelGGL(k_cg_start_r, dim3(rb, 2), dim3(256), 0, c->stream, a, w.p[0][0], w.p[1][0], (int) c->nx,
-		                   (int) c->ny, off, dg);
+		if (r32)
+			hipLaunchKernelGGL(k_cg_start_r<float>, dim3(rb, 2), dim3(256), 0, c->stream, a, w.p[0][0], w.p[1][0],
+			                   (int) c->nx, (int) c->ny, off, dg);
+		else
+			hipLaunchKernelGGL(k_cg_start_r<double>, dim3(rb, 2), dim3(256), 0, c->stream, a, w.p[0][0], w.p[1][0],
+			                   (int) c->nx, (int) c->ny, off, dg);
 	} else if (adv)
 		hipLaunchKernelGGL(k_advect_cg_init<false>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy, adv->uo,
 		                   adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff, adv->hdg,
@@ -586,16 +596,21 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 		// before it; the batch's last update is finalised on its own
 		for (int s = 0; s < batch; ++s) {
 			const int pin = enq & 1;
-			if (p32) {
-				hipLaunchKernelGGL(k_cg_p<float>, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off,
-				                   dg, pin, enq, s > 0 ? 1 : 0, rb);
-				hipLaunchKernelGGL(k_cg_update<float>, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny,
-				                   off, dg, pin ^ 1, enq, rb);
+			if (r32) {
+				hipLaunchKernelGGL((k_cg_p<float, float>), dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx,
+				                   (int) c->ny, off, dg, pin, enq, s > 0 ? 1 : 0, rb);
+				hipLaunchKernelGGL((k_cg_update<float, float>), dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx,
+				                   (int) c->ny, off, dg, pin ^ 1, enq, rb);
+			} else if (p32) {
+				hipLaunchKernelGGL((k_cg_p<float, double>), dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx,
+				                   (int) c->ny, off, dg, pin, enq, s > 0 ? 1 : 0, rb);
+				hipLaunchKernelGGL((k_cg_update<float, double>), dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx,
+				                   (int) c->ny, off, dg, pin ^ 1, enq, rb);
 			} else {
-				hipLaunchKernelGGL(k_cg_p<double>, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off,
-				                   dg, pin, enq, s > 0 ? 1 : 0, rb);
-				hipLaunchKernelGGL(k_cg_update<double>, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny,
-				                   off, dg, pin ^ 1, enq, rb);
+				hipLaunchKernelGGL((k_cg_p<double, double>), dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx,
+				                   (int) c->ny, off, dg, pin, enq, s > 0 ? 1 : 0, rb);
+				hipLaunchKernelGGL((k_cg_update<double, double>), dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx,
+				                   (int) c->ny, off, dg, pin ^ 1, enq, rb);
 			}
 			AFM_LAUNCH_CHECK();
 			++enq;
