@@ -323,7 +323,8 @@ __global__ __launch_bounds__(256) void k_sum_fin(unsigned nblk, SolveCtl* ctl, c
 // r = b - A u with u the initial guess (ns/solver.h:105-135 then
 // alg/cg.h:38-39), partial sums of r.r and b.b per field.  With STORE_B the
 // right-hand sides are stored to bu, bv instead and only b.b is summed (the
-// Neumann start, k_helm_pair / k_cg_start_x0 / k_cg_start_r, forms x0 and r).
+// Neumann start, k_cg_start_fused, forms x0 and r; the r.r partials are
+// written as zeros here).
 template<bool STORE_B>
 __global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* __restrict__ fx,
                                                         const double* __restrict__ fy, double* __restrict__ uo,
@@ -382,70 +383,171 @@ __global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* 
 	reduce_partial<2>(acc_v, a.pu[1]);
 }
 
-// (A b)_j for the periodic Helmholtz row of cell j (its own wrap flags, so
-// the value is the same bits in every thread that forms it)
-__device__ inline double helm_row_at(const double* __restrict__ b, size_t j, int nx, int ny, size_t N, double off,
-                                     double dg)
-{
-	int gx, gy;
-	cell_xy(j, nx, gx, gy);
-	const size_t ie = gx == nx - 1 ? j - (nx - 1) : j + 1, iw = gx == 0 ? j + (nx - 1) : j - 1;
-	const size_t in_ = gy == ny - 1 ? (size_t) gx : j + nx, is = gy == 0 ? j + N - nx : j - nx;
-	return stencil5<double, true>(off, dg, b[j], b[ie], b[iw], b[in_], b[is], gx == nx - 1, gx == 0, gy == ny - 1,
-	                              gy == 0);
-}
-
-// c = A b for both fields (the first pass of the Neumann start)
-__global__ __launch_bounds__(256) void k_helm_pair(const double* __restrict__ b0, const double* __restrict__ b1,
-                                                   double* __restrict__ c0, double* __restrict__ c1, int nx, int ny,
-                                                   double off, double dg)
-{
-	const int f = blockIdx.y;
-	const double* b = f ? b1 : b0;
-	double* c = f ? c1 : c0;
-	const size_t N = (size_t) nx * ny;
-	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256)
-		c[i] = helm_row_at(b, i, nx, ny, N, off, dg);
-}
-
 // The CG forms' start for the velocity solves (no reference counterpart: the
 // reference starts from the current velocity).  With A = I - sL and c = A b,
 // x0 = b + (I - A) b + (I - A)^2 b = 3b - 3c + A c, the first three terms of
 // the Neumann series of A^-1, so r0 = b - A x0 = (I - A)^3 b: on the step's
 // smooth right-hand sides a residual 4 orders below the one of the current
-// velocity (2-3 CG iterations instead of 4-5 at 2048^2).  Three 5-point
-// passes: c = A b (k_helm_pair), x0 (here, written to x), r = b - A x0
-// (k_cg_start_r, with the partial sums of r.r; b.b was summed by the
-// right-hand-side launch).
-__global__ __launch_bounds__(256) void k_cg_start_x0(CGArgs a, const double* __restrict__ b0,
-                                                     const double* __restrict__ b1, const double* __restrict__ c0,
-                                                     const double* __restrict__ c1, int nx, int ny, double off,
-                                                     double dg)
+// velocity (2-3 CG iterations instead of 4-5 at 2048^2).  b is stored by the
+// right-hand-side launch (which also sums b.b); one pass forms c, x0 and r
+// from it and sums r.r.
+// Three 5-point passes over b, c and x0 would read 88 B per cell and field
+// and write 20-24 B; this kernel reads b and writes x0 and r (20-24 B).
+// A block holds a tile of 128 columns x NW * RPW rows in registers: lane l
+// of wave w holds columns 2l, 2l + 1 of rows w * RPW .. w * RPW + RPW - 1.
+// c = A b is exact on the tile shrunk by one cell on every side, x0 by two
+// and r by three, so the tile writes its 122 x (NW * RPW - 6) inner cells;
+// tiles overlap by 6 cells in each direction.  East and west neighbours come
+// from the adjacent lane (DPP), north and south from the strip itself or,
+// for a wave's first and last rows, from the adjacent wave's boundary rows
+// exchanged through LDS.  Every value is formed from the reference's CSR row
+// of its own cell (stencil5 with that cell's wrap flags), so a cell's c and
+// x0 are the same bits in every tile that forms them.
+// Tile columns start at 122 tx - 4 (even: two-column loads stay aligned), so
+// tile 0 writes column n - 1 (as column -1) and the last tile stops at n - 2.
+constexpr int ST_OX = 122;   // columns written per tile
+#ifndef AFM_ST_NW
+#define AFM_ST_NW 8
+#endif
+#ifndef AFM_ST_RPW
+#define AFM_ST_RPW 8
+#endif
+constexpr int ST_NW = AFM_ST_NW, ST_RPW = AFM_ST_RPW;   // waves per block, rows per wave
+
+template<int NW, int RPW>
+struct StartTile {
+	static constexpr int LY = NW * RPW;   // rows held
+	static constexpr int OY = LY - 6;     // rows written
+};
+
+// xb[buf][wave + 1][first / last row][lane]: slots 0 and NW + 1 are never
+// written (the rows they stand for lie outside the tile's exact cells)
+template<int NW>
+using StartXch = double2[2][NW + 2][2][64];
+
+template<int NW, int RPW>
+__device__ inline void st_publish(StartXch<NW>& xb, int buf, const double (&v)[RPW][2], int w, int lane)
 {
-	const int f = blockIdx.y;
-	const double* b = f ? b1 : b0;
-	const double* cb = f ? c1 : c0;
-	const size_t N = (size_t) nx * ny;
-	double* x = a.x[f];
-	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256)
-		x[i] = 3.0 * b[i] - 3.0 * cb[i] + helm_row_at(cb, i, nx, ny, N, off, dg);
+	xb[buf][w + 1][0][lane] = make_double2(v[0][0], v[0][1]);
+	xb[buf][w + 1][1][lane] = make_double2(v[RPW - 1][0], v[RPW - 1][1]);
 }
 
-template<typename TR>
-__global__ __launch_bounds__(256) void k_cg_start_r(CGArgs a, const double* __restrict__ b0,
-                                                    const double* __restrict__ b1, int nx, int ny, double off,
-                                                    double dg)
+// A v at (row i, column o) of the lane's strip: E / W from the pair or the
+// adjacent lane, S / N from the strip or the adjacent waves' rows up / dn
+template<int RPW, bool EDGE>
+__device__ inline double st_apply(const double (&v)[RPW][2], int i, int o, double2 up, double2 dn, int px, int py,
+                                  int msk, double off, double dg)
 {
+	double E, W;
+	if (o == 0) { W = lane_shift<DPP_WAVE_SHR1>(v[i][1]); E = v[i][1]; }
+	else { W = v[i][0]; E = lane_shift<DPP_WAVE_SHL1>(v[i][0]); }
+	const double S = i > 0 ? v[i - 1][o] : (o ? up.y : up.x);
+	const double N = i < RPW - 1 ? v[i + 1][o] : (o ? dn.y : dn.x);
+	if (EDGE) return stencil5<double, true>(off, dg, v[i][o], E, W, N, S, px == msk, px == 0, py == msk, py == 0);
+	return stencil5<double, true>(off, dg, v[i][o], E, W, N, S, false, false, false, false);
+}
+
+// One tile: columns c0 .. c0 + 127 and rows r0 .. r0 + LY - 1 (unwrapped;
+// EDGE when some of them lie outside 1 .. n - 2, so that they wrap or carry
+// a wrap flag).  Adds the written cells' r^2 to acc.
+template<typename TR, int NW, int RPW, bool EDGE>
+__device__ inline void st_tile(StartXch<NW>& xb, int& buf, const double* __restrict__ b, double* __restrict__ x,
+                               TR* __restrict__ r, int n, int c0, int r0, double off, double dg, double& acc)
+{
+	constexpr int LY = StartTile<NW, RPW>::LY;
+	// the wave index (and with it the rows) is the same in every lane: held
+	// in scalar registers
+	const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), msk = n - 1;
+	const int cu = c0 + 2 * lane;                     // the pair's first column, unwrapped (even)
+	const int gc = EDGE ? (cu & msk) : cu;
+	int gy[RPW];
+#pragma unroll
+	for (int i = 0; i < RPW; ++i) gy[i] = EDGE ? ((r0 + RPW * w + i) & msk) : r0 + RPW * w + i;
+	double bv[RPW][2], cv[RPW][2], xv[RPW][2];
+#pragma unroll
+	for (int i = 0; i < RPW; ++i) {
+		const double2 t = *(const double2*) (b + (size_t) n * gy[i] + gc);
+		bv[i][0] = t.x; bv[i][1] = t.y;
+	}
+	// c = A b
+	st_publish<NW, RPW>(xb, buf, bv, w, lane);
+	__syncthreads();
+	double2 up = xb[buf][w][1][lane], dn = xb[buf][w + 2][0][lane];
+#pragma unroll
+	for (int i = 0; i < RPW; ++i)
+#pragma unroll
+		for (int o = 0; o < 2; ++o) cv[i][o] = st_apply<RPW, EDGE>(bv, i, o, up, dn, gc + o, gy[i], msk, off, dg);
+	// x0 = 3b - 3c + A c
+	buf ^= 1;
+	st_publish<NW, RPW>(xb, buf, cv, w, lane);
+	__syncthreads();
+	up = xb[buf][w][1][lane]; dn = xb[buf][w + 2][0][lane];
+#pragma unroll
+	for (int i = 0; i < RPW; ++i)
+#pragma unroll
+		for (int o = 0; o < 2; ++o)
+			xv[i][o] = 3.0 * bv[i][o] - 3.0 * cv[i][o] + st_apply<RPW, EDGE>(cv, i, o, up, dn, gc + o, gy[i], msk, off, dg);
+	// r = b - A x0, the written cells stored
+	buf ^= 1;
+	st_publish<NW, RPW>(xb, buf, xv, w, lane);
+	__syncthreads();
+	up = xb[buf][w][1][lane]; dn = xb[buf][w + 2][0][lane];
+	buf ^= 1;
+	bool colok[2];
+#pragma unroll
+	for (int o = 0; o < 2; ++o) {
+		const int lc = 2 * lane + o;
+		colok[o] = lc >= 3 && lc < 128 - 3 && (!EDGE || cu + o <= n - 2);
+	}
+#pragma unroll
+	for (int i = 0; i < RPW; ++i) {
+		const int ly = RPW * w + i;
+		if (!(ly >= 3 && ly < LY - 3 && (!EDGE || r0 + ly < n))) continue;
+		double ri[2];
+#pragma unroll
+		for (int o = 0; o < 2; ++o) ri[o] = bv[i][o] - st_apply<RPW, EDGE>(xv, i, o, up, dn, gc + o, gy[i], msk, off, dg);
+		const size_t row = (size_t) n * gy[i];
+		if (colok[0] && colok[1]) {
+			*(double2*) (x + row + gc) = make_double2(xv[i][0], xv[i][1]);
+			if (sizeof(TR) == 4) *(float2*) (r + row + gc) = make_float2((float) ri[0], (float) ri[1]);
+			else *(double2*) (r + row + gc) = make_double2(ri[0], ri[1]);
+		} else {
+#pragma unroll
+			for (int o = 0; o < 2; ++o)
+				if (colok[o]) {
+					x[row + gc + o] = xv[i][o];
+					r[row + gc + o] = (TR) ri[o];
+				}
+		}
+#pragma unroll
+		for (int o = 0; o < 2; ++o)
+			if (colok[o]) acc += ri[o] * ri[o];
+	}
+}
+
+// blockIdx.y = field; blocks walk tiles t = blockIdx.x, + gridDim.x, ...
+// (gridDim.x at most the right-hand-side launch's block count: the r.r
+// partials of blocks beyond it are the zeros that launch wrote)
+template<typename TR, int NW, int RPW>
+__global__ __launch_bounds__(64 * NW) void k_cg_start_fused(CGArgs a, const double* __restrict__ b0,
+                                                            const double* __restrict__ b1, int n, double off,
+                                                            double dg, int tiles_x, int ntiles)
+{
+	using ST = StartTile<NW, RPW>;
+	__shared__ StartXch<NW> xb;
 	const int f = blockIdx.y;
-	const double* b = f ? b1 : b0;
-	const size_t N = (size_t) nx * ny;
-	const double* x = a.x[f];
-	TR* r = (TR*) a.r[f];
+	if (!a.x[f]) return;
+	const double* __restrict__ b = f ? b1 : b0;
+	double* __restrict__ x = a.x[f];
+	TR* __restrict__ r = (TR*) a.r[f];
+	int buf = 0;
 	double acc[1] = {0.0};
-	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
-		const double ri = b[i] - helm_row_at(x, i, nx, ny, N, off, dg);
-		r[i] = (TR) ri;
-		acc[0] += ri * ri;
+	for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+		const int tx = t % tiles_x, ty = t / tiles_x;
+		const int c0 = ST_OX * tx - 4, r0 = ST::OY * ty - 3;
+		const bool edge = c0 < 1 || c0 + 127 > n - 2 || r0 < 1 || r0 + ST::LY - 1 > n - 2;
+		if (edge) st_tile<TR, NW, RPW, true>(xb, buf, b, x, r, n, c0, r0, off, dg, acc[0]);
+		else st_tile<TR, NW, RPW, false>(xb, buf, b, x, r, n, c0, r0, off, dg, acc[0]);
 	}
 	reduce_partial<1>(acc, a.pu[f]);
 }
@@ -540,7 +642,7 @@ struct AdvectInit {
 	double *uo, *vo;
 	double dt, rho, hoff, hdg;
 	int be;
-	int neumann;      // the CG forms' start x0 = 3b - 3Ab + A^2 b (k_cg_start_x0)
+	int neumann;      // the CG forms' start x0 = 3b - 3Ab + A^2 b (k_cg_start_fused)
 };
 
 int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* const b[2], double tol, int tol_mode,
@@ -561,24 +663,24 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 	}
 	a.tol_mode = tol_mode;
 	const unsigned rb = rblocks(c->N);
-	// FP32 r only where the Neumann start writes it (k_cg_start_r)
+	// FP32 r only where the Neumann start writes it (k_cg_start_fused)
 	const bool r32 = p32 && adv && adv->neumann;
 	if (adv && adv->neumann) {
-		// b goes to the first p buffers and c = A b to the second: the first
-		// p update reads r and writes p[.][1], the second writes p[.][0]
+		// b goes to the first p buffers (the first p update reads r and
+		// writes p[.][1], the second writes p[.][0])
 		hipLaunchKernelGGL(k_advect_cg_init<true>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy, adv->uo,
 		                   adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff, adv->hdg,
 		                   off, dg, tol, max_iters, w.p[0][0], w.p[1][0]);
-		hipLaunchKernelGGL(k_helm_pair, dim3(rb, 2), dim3(256), 0, c->stream, w.p[0][0], w.p[1][0], w.p[0][1],
-		                   w.p[1][1], (int) c->nx, (int) c->ny, off, dg);
-		hipLaunchKernelGGL(k_cg_start_x0, dim3(rb, 2), dim3(256), 0, c->stream, a, w.p[0][0], w.p[1][0], w.p[0][1],
-		                   w.p[1][1], (int) c->nx, (int) c->ny, off, dg);
+		using ST = StartTile<ST_NW, ST_RPW>;
+		const int n = (int) c->nx;
+		const int tiles_x = (n + ST_OX - 1) / ST_OX, ntiles = tiles_x * ((n + ST::OY - 1) / ST::OY);
+		const unsigned sb = (unsigned) ntiles < rb ? (unsigned) ntiles : rb;
 		if (r32)
-			hipLaunchKernelGGL(k_cg_start_r<float>, dim3(rb, 2), dim3(256), 0, c->stream, a, w.p[0][0], w.p[1][0],
-			                   (int) c->nx, (int) c->ny, off, dg);
+			hipLaunchKernelGGL((k_cg_start_fused<float, ST_NW, ST_RPW>), dim3(sb, 2), dim3(64 * ST_NW), 0, c->stream,
+			                   a, w.p[0][0], w.p[1][0], n, off, dg, tiles_x, ntiles);
 		else
-			hipLaunchKernelGGL(k_cg_start_r<double>, dim3(rb, 2), dim3(256), 0, c->stream, a, w.p[0][0], w.p[1][0],
-			                   (int) c->nx, (int) c->ny, off, dg);
+			hipLaunchKernelGGL((k_cg_start_fused<double, ST_NW, ST_RPW>), dim3(sb, 2), dim3(64 * ST_NW), 0, c->stream,
+			                   a, w.p[0][0], w.p[1][0], n, off, dg, tiles_x, ntiles);
 	} else if (adv)
 		hipLaunchKernelGGL(k_advect_cg_init<false>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy, adv->uo,
 		                   adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff, adv->hdg,
@@ -1160,8 +1262,13 @@ int op_cg_single(Ctx* c, double scale, const double* b, double* x, double tol, i
 		if (iters) *iters = it[0];
 		if (final_norm) *final_norm = res[0];
 	}
-	hipStreamSynchronize(c->stream);
+	// the buffers are freed only once the solve's kernels are done with them
+	const hipError_t se = hipStreamSynchronize(c->stream);
 	for (void* p : own) (void) hipFree(p);
+	if (rc == 0 && se != hipSuccess) {
+		set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(se));
+		return -3;
+	}
 	return rc;
 }
 
