@@ -52,16 +52,34 @@ __device__ inline void reduce_partial(double (&v)[NV], double* partials)
 	}
 }
 
+// p[t] + p[t + step] + p[t + 2 step] + ... over the indices below nblk, added
+// in that order from 0.  The loads of four terms are issued before their
+// adds (index clamped, term skipped): one memory round trip per four terms
+// instead of one per term, the same bits.
+__device__ inline double strided_sum(const double* __restrict__ p, unsigned t, unsigned step, unsigned nblk)
+{
+	double s = 0.0;
+	for (unsigned b = t; b < nblk; b += 4 * step) {
+		double v[4];
+#pragma unroll
+		for (int k = 0; k < 4; ++k) {
+			const unsigned q = b + k * step;
+			v[k] = p[q < nblk ? q : nblk - 1];
+		}
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			if (b + k * step < nblk) s += v[k];
+	}
+	return s;
+}
+
 // Phase 2: totals in thread 0 (call from every thread of a one-block kernel).
 template<int NV>
 __device__ inline void red_total(const double* partials, unsigned nblk, double (&tot)[NV])
 {
 	const int tid = threadIdx.x;
 #pragma unroll
-	for (int j = 0; j < NV; ++j) {
-		tot[j] = 0.0;
-		for (unsigned b = tid; b < nblk; b += blockDim.x) tot[j] += partials[(size_t) j * MAX_RED_BLOCKS + b];
-	}
+	for (int j = 0; j < NV; ++j) tot[j] = strided_sum(partials + (size_t) j * MAX_RED_BLOCKS, tid, blockDim.x, nblk);
 	block_sum<NV>(tot);
 }
 

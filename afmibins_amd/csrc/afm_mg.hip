@@ -465,9 +465,7 @@ __device__ inline void red_totals_256(const double* __restrict__ partials, size_
 	const int tid = (int) threadIdx.x;
 #pragma unroll
 	for (int j = 0; j < NV; ++j) {
-		double s = 0.0;
-		if (tid < 256)
-			for (unsigned b = tid; b < nblk; b += 256) s += partials[j * stride + b];
+		double s = tid < 256 ? strided_sum(partials + j * stride, tid, 256, nblk) : 0.0;
 		for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
 		if (tid < 256 && (tid & 63) == 0) wsum[j][tid >> 6] = s;
 	}
