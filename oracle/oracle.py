@@ -121,6 +121,8 @@ def ref():
             "ref_div_grad": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P]),
             "ref_ns3_run": (_I, [_I, _D, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P]),
             "ref_advect": (_I, [_P, _P, _P, _P, _P, _P]),
+            "ref_ib_test": (_I, [_D, _D, _D, _P, _P]),
+            "ref_restrict_g1": (_I, [_P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(R, name)

@@ -7,7 +7,8 @@
 // contains no reference code, only calls into it.  The reference sizes are
 // compile-time template parameters, so every entry point takes a geometry id:
 //   3-D (dt::geometry<L,W,H,D>):  0 -> <4,1,1,1> (16^3),  1 -> <5,1,1,1> (32^3),
-//                                  2 -> <1,8,8,8> (16^3, coarsest 8^3 by CG)
+//                                  2 -> <1,8,8,8> (16^3, coarsest 8^3 by CG),
+//                                  3 -> <1,1,1,1> (2^3; restriction only)
 //   2-D planes (nx x 1 x nz, the reference's operators with ny == 1):
 //                                 10 -> 16x1x16 (n 16), 11 -> 32x1x32 (n 32),
 //                                 12 -> 64x1x64 (n 64)
@@ -34,6 +35,7 @@ namespace {
 typedef dt::geometry<4u, 1u, 1u, 1u> G16;
 typedef dt::geometry<5u, 1u, 1u, 1u> G32;
 typedef dt::geometry<1u, 8u, 8u, 8u> G8;
+typedef dt::geometry<1u, 1u, 1u, 1u> G1;
 
 template<unsigned NX, unsigned NZ, unsigned N>
 struct Plane {
@@ -65,6 +67,15 @@ struct Markers : ib::lagrangian_grid<G, M> {
 	const double* ds;
 	Markers(std::vector<dt::point> p, const double* ds_) : ib::lagrangian_grid<G, M>(p), ds(ds_) {}
 	double volume(unsigned i) const { return ds[i]; }
+};
+
+// one marker of a given volume (test/ib_test.cc:33-43 defines such a grid)
+template<class G>
+struct OneMarker : ib::lagrangian_grid<G, 1u> {
+	double vol;
+	OneMarker(double x, double y, double z, double v)
+		: ib::lagrangian_grid<G, 1u>({dt::point(x, y, z, 0u)}), vol(v) {}
+	double volume(unsigned) const { return vol; }
 };
 
 template<class G, ib::mac::component C>
@@ -446,6 +457,36 @@ int ref_ns3_run(int geo, double mu, double rho, double dt, double tol, double* u
 	case 1: ns_run<G32>(mu, rho, dt, tol, u, v, w, ut, ub, wt, wb, fx, fy, fz, steps, p); return 0;
 	}
 	return -1;
+}
+
+// The computation of test/ib_test.cc:73-93 on geometry<5,1,1,1>: one marker
+// of volume 1/n^3 at (x, y, z) holding 1 is spread to mac_grid<G, y>; the
+// field is returned (nx (ny-1) nz entries), its non-zero entries are set to
+// 1 and interpolated back to the marker, whose value is returned in *out.
+int ref_ib_test(double x, double y, double z, double* field, double* out)
+{
+	constexpr unsigned n = G32::n;
+	ib::mac_grid<G32, ib::mac::y> egrid;
+	OneMarker<G32> lgrid(x, y, z, 1. / (n * n * n));
+	for (auto& e : lgrid) e = 1.0;
+	egrid.interpolate(lgrid);
+	for (unsigned i = 0; i < egrid.size(); ++i) field[i] = egrid[i];
+	for (auto& e : egrid) if (e != 0.0) e = 1;
+	for (auto& e : lgrid) e = 0.0;
+	lgrid.interpolate(egrid);
+	*out = lgrid[0];
+	return (int) egrid.size();
+}
+
+// mg::solver<geometry<1,1,1,1>>::restriction of 8 values into one (the
+// computation of test/restr_test.cc:38-50; the output starts at zero).
+int ref_restrict_g1(const double* fine, double* coarse)
+{
+	dt::vector<8> f; dt::vector<1> c;
+	for (unsigned i = 0; i < 8; ++i) f[i] = fine[i];
+	MGX<G1>::restriction(c, f);
+	coarse[0] = c[0];
+	return 0;
 }
 
 }  // extern "C"
