@@ -6,25 +6,37 @@
 //
 // The reference forms theta in x87 long double (a 64-bit-mantissa pi literal
 // times a double, ib/grid.h:31,54) and rounds it to double on the way into
-// cos().  GPUs have no 80-bit type, so theta is reproduced exactly with
-// integer arithmetic: the 117-bit product of r's significand and pi's 64-bit
-// significand is rounded to 64 bits, then to 53 bits, both to nearest-even.
+// cos().  GPUs have no 80-bit type.  pi_64 is split as A + B (A the double
+// nearest pi, B = pi_64 - A exact in 10 bits), so pi_64 * r is a sum of four
+// doubles formed exactly with fmas; its double rounding RN53(RN64(.)) differs
+// from a single rounding only when RN64 lands on a 53-bit midpoint, which the
+// sum's low part shows directly.  The few arguments whose low part sits
+// within 2^-32 ulp of that decision, or whose rounding is a power of two, take
+// the exact integer path (the 117-bit product rounded to 64, then 53 bits).
 // n is a power of two, so the scaling by n/2 is exact in either order.
 //
-// cos is correctly rounded: Cody-Waite reduction by pi/2 (four 50-bit
-// pieces, so k*P_i is exact for |k| < 8), then double-double Taylor series
-// for sin/cos on |y| <= pi/4 (truncation below 2^-105 relative), rounded
-// once.  glibc's cos is not correctly rounded (it differs in about 1e-3 of
+// cos is correctly rounded (Ziv): Cody-Waite reduction by pi/2 (four 50-bit
+// pieces, so k*P_i is exact for |k| < 8) to y, |y| <= pi/4; then y = j/64 + d,
+// |d| <= 1/128, and cos or sin of y is U cos d + V sin d with U, V from a
+// double-double table (afm_cos_table.h) and short series in d; if every value
+// within 2^-65 of the result rounds alike, that is the result, otherwise (a
+// few arguments in 10^4) the full double-double Taylor series on y decides.
+// glibc's cos is not correctly rounded (it differs in about 1e-3 of
 // arguments), so this is the build's definition of delta; every operation is
 // an explicit IEEE op or fma, so host and device produce the same bits.
 #pragma once
 #include <cstdint>
 #include <cmath>
 #include <cstring>
+#include "afm_cos_table.h"
 
 #ifndef AFM_HD
 #define AFM_HD __host__ __device__
 #endif
+// the fast paths are inlined into the kernels; the rarely taken exact paths
+// stay out of line, so a wave that needs one branches to it
+#define AFM_INLINE __attribute__((always_inline))
+#define AFM_OUTLINE __attribute__((noinline))
 
 namespace afm {
 
@@ -113,39 +125,29 @@ AFM_HD constexpr dd sin_coef(int j)
 	}
 }
 
+// cos(j/64), sin(j/64) as double-double, j = -51..51, per parity (see
+// afm_cos_table.h).  The device reads its copy from constant memory.
 #if defined(__HIP_DEVICE_COMPILE__)
-// The fast path's coefficients of one parity (0 cos, 1 sin), in the order the
-// series takes them: c10 .. c5 (double), then c4 .. c0 as (hi, lo).  On the
-// GPU a lane reads its parity's row: the lanes of a wave alternate between
-// the two series, and loads replace the per-lane selection of every constant
-// between two register pairs.  The values are cos_coef / sin_coef's.
-#define AFM_FC(c) {c(10).hi, c(9).hi, c(8).hi, c(7).hi, c(6).hi, c(5).hi, c(4).hi, c(4).lo, c(3).hi, c(3).lo, \
-                   c(2).hi, c(2).lo, c(1).hi, c(1).lo, c(0).hi, c(0).lo}
-static __constant__ double AFM_FAST_COEF[2][16] = {AFM_FC(cos_coef), AFM_FC(sin_coef)};
-#undef AFM_FC
+static __constant__ double AFM_COS_TAB[2][2 * AFM_COS_TAB_JMAX + 1][4] = AFM_COS_TAB_INIT;
+#else
+static const double AFM_COS_TAB[2][2 * AFM_COS_TAB_JMAX + 1][4] = AFM_COS_TAB_INIT;
 #endif
-
-// a + b without the careful low-order sum of dd_add; fine when the sum does
-// not cancel (every step of the series below adds a smaller term to a larger)
-AFM_HD inline dd dd_add_nc(dd a, dd b)
-{
-	dd s = two_sum(a.hi, b.hi);
-	double e = s.lo + (a.lo + b.lo);
-	return fast_two_sum(s.hi, e);
-}
 
 // Correctly rounded cos(x) for |x| < 8 (the delta needs |x| <= 3*pi/2).
 //
-// Ziv's strategy.  The fast evaluation takes the series to degree 10 in z =
-// y^2 with the first five coefficients in double-double and the other six in
-// double: its relative error is below 2^-75 (tail rounding < 2^-75.5,
-// truncation < 2^-77 for cos and < 2^-82 for sin on |y| <= pi/4).  If the
-// rounding of every value within 2^-70 of it is the same, that is the
-// correctly rounded result; otherwise (about one argument in 2^16) the full
-// degree-14 double-double series (2^-105) decides.  The cos and sin series
-// share one loop with the coefficients chosen per lane: the four taps of a
-// window are pi/2 apart in x, so neighbouring lanes alternate between them.
-AFM_HD inline double cos_cr(double x)
+// Fast path, for y = y.hi + y.lo reduced: j = rint(64 y.hi), d = y.hi - j/64
+// (exact, |d| <= 2^-7), and cos y (q even) or sin y (q odd) is
+//   R = U cos(d + y.lo) + V sin(d + y.lo)
+//     = U - U (z/2 + a) - U y.lo d + V d + V b + V y.lo (1 - z/2)
+// with d^2 = z + ze exactly, a = ze/2 - z^2/24 + z^3/720 - z^4/40320
+// (1 - cos d - z/2; truncation < 2^-91) and b = d z (-1/6 + z/120 -
+// z^2/5040) (sin d - d; truncation < 2^-81).  U + V d - U z/2 are summed
+// exactly (two_prod, two_sum), the rest in double.  Error, relative to R
+// (R >= 0.7 for cos, R >= max(|U|, |d|)/2 for sin): b's own rounding < 2^-67.4
+// (|b| < 2^-16.6 R), the fma that adds V b < 2^-69.6, all other terms below
+// 2^-70; so R is within 2^-66.5 R of the true value and the test below, with
+// a band of 2^-65 R, accepts only results that the exact value rounds to.
+AFM_HD AFM_INLINE inline double cos_fast(double x, bool& ok)
 {
 	const double P1 = 0x1.921fb54442d18p+0, P2 = 0x1.1a62633145c08p-54,
 	             P3 = -0x1.1f1976b7ed8f8p-106, P4 = -0x1.dd660ce2ff7d1p-157;
@@ -156,51 +158,40 @@ AFM_HD inline double cos_cr(double x)
 	dd s = two_sum(t, -(k * P2));
 	double ylo = (s.lo - k * P3) - k * P4;
 	dd y = fast_two_sum(s.hi, ylo);
-	dd z = dd_mul(y, y);
 	int q = ((int) k) & 3;
 	const bool odd = (q & 1) != 0;
-	// fast: tail c5..c10 in double (Horner on z.hi), then c4..c0 in dd
-#if defined(__HIP_DEVICE_COMPILE__)
-	const double* cf = AFM_FAST_COEF[odd ? 1 : 0];
-	double pt = cf[0];
-#pragma unroll
-	for (int j = 1; j < 6; ++j) pt = fma(pt, z.hi, cf[j]);
-	dd p = {pt, 0.0};
-#pragma unroll
-	for (int j = 0; j < 5; ++j) p = dd_add_nc(dd_mul(p, z), dd{cf[6 + 2 * j], cf[7 + 2 * j]});
-#else
-	double pt = odd ? sin_coef(10).hi : cos_coef(10).hi;
-#pragma unroll
-	for (int j = 9; j >= 5; --j) pt = fma(pt, z.hi, odd ? sin_coef(j).hi : cos_coef(j).hi);
-	dd p = {pt, 0.0};
-#pragma unroll
-	for (int j = 4; j >= 0; --j) {
-		const dd cj = odd ? sin_coef(j) : cos_coef(j);
-		p = dd_add_nc(dd_mul(p, z), cj);
-	}
-#endif
-	const dd one = {1.0, 0.0};
-	dd r = dd_mul(p, odd ? y : one);   // times 1 is exact
-	const double e = 0x1p-70 * fabs(r.hi);
-	if (!(r.hi + (r.lo + e) == r.hi && r.hi + (r.lo - e) == r.hi)) {
-		if (!odd) {
-			dd pc = cos_coef(14);
-			for (int j = 13; j >= 0; --j) pc = dd_add(dd_mul(pc, z), cos_coef(j));
-			r = pc;
-		} else {
-			dd ps = sin_coef(13);
-			for (int j = 12; j >= 0; --j) ps = dd_add(dd_mul(ps, z), sin_coef(j));
-			r = dd_mul(ps, y);
-		}
-	}
+	const double jf = rint(y.hi * 64.0);
+	const double d = fma(jf, -0x1p-6, y.hi);
+	// |jf| <= 51 for |x| < 8; the clamp keeps the read in the table for
+	// any input (NaN included)
+	const int ji = (int) fmin(fmax(jf, -(double) AFM_COS_TAB_JMAX), (double) AFM_COS_TAB_JMAX);
+	const double* T = AFM_COS_TAB[odd ? 1 : 0][ji + AFM_COS_TAB_JMAX];
+	const double Uh = T[0], Ul = T[1], Vh = T[2], Vl = T[3];
+	const double z = d * d, ze = fma(d, d, -z);
+	const double a = fma(z * z, fma(z, fma(z, -0x1.a01a01a01a01ap-16, 0x1.6c16c16c16c17p-10),
+	                                -0x1.5555555555555p-5), 0.5 * ze);
+	const double b = (d * z) * fma(z, fma(z, -0x1.a01a01a01a01ap-13, 0x1.1111111111111p-7),
+	                               -0x1.5555555555555p-3);
+	const dd p = two_prod(Vh, d);
+	const dd w = two_prod(0.5 * Uh, z);
+	// terms below 2^-52 R first, then the two larger ones by fma
+	double small = ((Ul + p.lo) - w.lo) + fma(Vl, d, fma(Vh, y.lo, -(Uh * y.lo) * d))
+	             - (Ul * z) * 0.5 - ((Vh * y.lo) * z) * 0.5;
+	small = fma(Vh, b, fma(-Uh, a, small));
+	const dd s1 = two_sum(Uh, p.hi);
+	const dd s2 = two_sum(s1.hi, -w.hi);
+	const dd r = fast_two_sum(s2.hi, (s1.lo + s2.lo) + small);
+	const double e = 0x1p-65 * fabs(r.hi);
+	// bitwise & (no short-circuit): the flag costs no branch
+	ok = ok & (r.hi + (r.lo + e) == r.hi) & (r.hi + (r.lo - e) == r.hi);
 	// cos(y + k pi/2): q=0 cos y, q=1 -sin y, q=2 -cos y, q=3 sin y
-	double v = r.hi;
-	return (q == 1 || q == 2) ? -v : v;
+	return (q == 1 || q == 2) ? -r.hi : r.hi;
 }
 
-// The degree-14 series alone (the definition the fast path reproduces);
-// kept for the tests that compare the two.
-AFM_HD inline double cos_cr_full(double x)
+// The degree-14 double-double series on the reduced argument (relative error
+// below 2^-105), rounded once: the definition the fast path reproduces, and
+// its fallback.
+AFM_HD AFM_INLINE inline double cos_cr_full(double x)
 {
 	const double P1 = 0x1.921fb54442d18p+0, P2 = 0x1.1a62633145c08p-54,
 	             P3 = -0x1.1f1976b7ed8f8p-106, P4 = -0x1.dd660ce2ff7d1p-157;
@@ -226,11 +217,25 @@ AFM_HD inline double cos_cr_full(double x)
 	return (q == 1 || q == 2) ? -v : v;
 }
 
+AFM_HD AFM_OUTLINE inline double cos_cr_slow(double x) { return cos_cr_full(x); }
+
+// Correctly rounded cos(x) for |x| < 8: the fast path, or the full series
+// where its rounding test fails.
+AFM_HD AFM_INLINE inline double cos_cr(double x)
+{
+	bool ok = true;
+	const double v = cos_fast(x, ok);
+	return ok ? v : cos_cr_slow(x);
+}
+
 AFM_HD inline uint64_t dbits(double d) { uint64_t b; memcpy(&b, &d, 8); return b; }
 
+AFM_HD inline double dfrom(uint64_t b) { double d; memcpy(&d, &b, 8); return d; }
+
 // theta = RN53(RN64(pi_64 * r)) * n / 2, exactly as the x87 sequence of
-// ib/grid.h:54 rounds it (r normal or zero; n a power of two).
-AFM_HD inline double delta_arg(double r, unsigned n)
+// ib/grid.h:54 rounds it (r normal or zero; n a power of two), in integer
+// arithmetic: the exact path of delta_arg.
+AFM_HD AFM_INLINE inline double delta_arg_exact(double r, unsigned n)
 {
 	if (r == 0.0) return r;
 	const uint64_t PM = 0xC90FDAA22168C235ull;   // pi = PM * 2^-62
@@ -270,13 +275,74 @@ AFM_HD inline double delta_arg(double r, unsigned n)
 	return neg ? -v : v;
 }
 
-AFM_HD inline double delta(double r, unsigned n)
+// The same value from the exact product pi_64 |r| = h + l + m + m2 (A r = h + l,
+// B r = m + m2, by fma).  With H = RN53 of that sum and T its remainder (to
+// 2^-100 ulp(H)), RN64 rounds onto the midpoint H + sign(T) ulp/2 exactly
+// when ||T| - ulp/2| <= ulp/2^12 (the midpoint is even on the 64-bit grid, so
+// both ends of that interval round to it); then RN53 ties to even.  Outside
+// that interval RN53(RN64(.)) = H.  Within 2^-32 ulp of its ends, for H a
+// power of two (the grid changes below it) and for tiny |r|, it clears ok
+// and the exact path decides.
+AFM_HD AFM_INLINE inline double delta_arg_fast(double r, unsigned n, bool& ok)
 {
-	double nr = n * r;
-	if (fabs(nr) >= 3.0)   // |(int)(n r)| > 2 without the out-of-range cast
-		return 0;
-	double c = cos_cr(delta_arg(r, n));
-	return 0.25 * n * (1 + c);
+	const double A = 0x1.921fb54442d18p+1, B = 0x1.1a8p-53;
+	const double ar = fabs(r);
+	const double h = A * ar, l = fma(A, ar, -h);
+	const double m = B * ar, m2 = fma(B, ar, -m);
+	const dd s = two_sum(l, m);
+	const dd x = fast_two_sum(h, s.hi);
+	const double T = x.lo + (s.lo + m2);
+	const uint64_t hb = dbits(x.hi);
+	const uint64_t eb = hb & 0x7FF0000000000000ull;
+	const double half = dfrom(eb - (53ull << 52));   // ulp(H) / 2
+	const double q64 = dfrom(eb - (64ull << 52));    // half the 64-bit ulp at H
+	const double D = fabs(fabs(T) - half);
+	double v = x.hi;
+	if (D < q64 * (1 - 0x1p-20) && (hb & 1ull))
+		v = x.hi + copysign(2 * half, T);
+	ok = ok & ((D < q64 * (1 - 0x1p-20)) | (D > q64 * (1 + 0x1p-20))) & ((hb & 0xFFFFFFFFFFFFFull) != 0)
+	     & (ar >= 0x1p-900);
+	v = v * (double) n;
+	v = v / 2;
+	return r < 0 ? -v : v;
+}
+
+AFM_HD AFM_OUTLINE inline double delta_arg_slow(double r, unsigned n) { return delta_arg_exact(r, n); }
+
+AFM_HD AFM_INLINE inline double delta_arg(double r, unsigned n)
+{
+	bool ok = true;
+	const double v = delta_arg_fast(r, n, ok);
+	return ok ? v : delta_arg_slow(r, n);
+}
+
+// delta by the fast paths; ok is cleared where delta_exact must decide.
+AFM_HD AFM_INLINE inline double delta_fast(double r, unsigned n, bool& ok)
+{
+	// |(int)(n r)| > 2 without the out-of-range cast: outside the support
+	const bool in = fabs(n * r) < 3.0;
+	bool fok = true;
+	const double arg = delta_arg_fast(r, n, fok);
+	const double c = cos_fast(in ? arg : 0.0, fok);
+	ok = ok & (fok | !in);
+	return in ? 0.25 * n * (1 + c) : 0.0;
+}
+
+// delta by the exact argument and the full series (the definition the fast
+// paths reproduce); delta_exact is the same out of line
+AFM_HD AFM_INLINE inline double delta_exact_inl(double r, unsigned n)
+{
+	if (fabs(n * r) >= 3.0) return 0;
+	return 0.25 * n * (1 + cos_cr_full(delta_arg_exact(r, n)));
+}
+
+AFM_HD AFM_OUTLINE inline double delta_exact(double r, unsigned n) { return delta_exact_inl(r, n); }
+
+AFM_HD AFM_INLINE inline double delta(double r, unsigned n)
+{
+	bool ok = true;
+	const double v = delta_fast(r, n, ok);
+	return ok ? v : delta_exact(r, n);
 }
 
 }  // namespace afm

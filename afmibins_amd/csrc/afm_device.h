@@ -171,6 +171,38 @@ __device__ inline double fmod_pow2(double x, unsigned p)
 }
 
 // ------------------------------------------------------------- IB window --
+// dx[d] = delta(rx[d]), dy[d] = delta(ry[d]): the fast paths of all eight
+// factors first (no branch between them, so their chains interleave), then
+// the exact path for the factors whose fast path could not decide (a wave
+// takes that branch when any of its lanes needs it).
+__device__ inline void window_deltas(const double rx[4], const double ry[4], unsigned n, double dx[4],
+                                     double dy[4])
+{
+	unsigned bad = 0;
+#pragma unroll
+	for (int d = 0; d < 4; ++d) {
+		bool okx = true, oky = true;
+		dx[d] = delta_fast(rx[d], n, okx);
+		dy[d] = delta_fast(ry[d], n, oky);
+		bad |= (okx ? 0u : 1u << d) | (oky ? 0u : 16u << d);
+	}
+	// one inlined copy of the exact path, in a loop over the factors that
+	// need it (register selects, not indexed arrays: nothing goes to scratch)
+	while (bad) {
+		const int i = __builtin_ctz(bad);
+		bad &= bad - 1;
+		double r = rx[0];
+#pragma unroll
+		for (int d = 1; d < 8; ++d) r = i == d ? (d < 4 ? rx[d] : ry[d - 4]) : r;
+		const double v = delta_exact_inl(r, n);
+#pragma unroll
+		for (int d = 0; d < 8; ++d) {
+			if (d < 4) dx[d] = i == d ? v : dx[d];
+			else dy[d - 4] = i == d ? v : dy[d - 4];
+		}
+	}
+}
+
 // The 4x4 window of one MAC component around marker (cx, cy):
 // ib/eulerian_grid.h:64-93 through ib/mac_grid.h:49-73, restricted to the
 // periodic (x, z) plane.  comp 0 (u, x-faces) is shifted by -h/2 in y before
@@ -190,13 +222,15 @@ __device__ inline void ib_window(int comp, double cx, double cy, unsigned n, uns
 	if (!(fy >= 0.0 && fy <= (double) ny)) fy = 0.0;
 	ix0 = (int) (((unsigned) fx + nx - 1) & (nx - 1));
 	iy0 = (int) (((unsigned) fy + ny - 1) & (ny - 1));
+	double rx[4], ry[4];
 #pragma unroll
 	for (int d = 0; d < 4; ++d) {
 		double x = (fx + d - 1.0) * h, y = (fy + d - 1.0) * h;
 		if (comp == 0) y += 0.5 * h; else x += 0.5 * h;
-		dx[d] = delta(x - cx, n);
-		dy[d] = delta(y - cy, n);
+		rx[d] = x - cx;
+		ry[d] = y - cy;
 	}
+	window_deltas(rx, ry, n, dx, dy);
 }
 
 // Per-launch constants of the 2-D window, formed once on the host by the
@@ -229,9 +263,9 @@ inline WinGeom win_geom(unsigned n, unsigned W, unsigned D, unsigned nx, unsigne
 // fmod_pow2 with the period and its reciprocal given
 __device__ inline double fmod_pow2g(double x, double pd, double inv) { return x - pd * trunc(x * inv); }
 
-// ib_window_factor below with the launch's constants (the same operations)
-__device__ inline double ib_window_factor(int comp, double cx, double cy, const WinGeom& g, int d8, int& ix0,
-                                          int& iy0)
+// ib_window with the launch's constants (the same operations)
+__device__ inline void ib_window_g(int comp, double cx, double cy, const WinGeom& g, int& ix0, int& iy0,
+                                   double dx[4], double dy[4])
 {
 	double q0 = cx, q1 = cy;
 	if (comp == 0) q1 -= g.half_h; else q0 -= g.half_h;
@@ -242,39 +276,15 @@ __device__ inline double ib_window_factor(int comp, double cx, double cy, const 
 	if (!(fy >= 0.0 && fy <= g.nyd)) fy = 0.0;
 	ix0 = (int) (((unsigned) fx + g.nx - 1) & (g.nx - 1));
 	iy0 = (int) (((unsigned) fy + g.ny - 1) & (g.ny - 1));
-	if (d8 < 4) {
-		double x = (fx + d8 - 1.0) * g.h;
-		if (comp != 0) x += g.half_h;
-		return delta(x - cx, g.n);
+	double rx[4], ry[4];
+#pragma unroll
+	for (int d = 0; d < 4; ++d) {
+		double x = (fx + d - 1.0) * g.h, y = (fy + d - 1.0) * g.h;
+		if (comp == 0) y += g.half_h; else x += g.half_h;
+		rx[d] = x - cx;
+		ry[d] = y - cy;
 	}
-	double y = (fy + (d8 - 4) - 1.0) * g.h;
-	if (comp == 0) y += g.half_h;
-	return delta(y - cy, g.n);
-}
-
-// One factor of ib_window: lane d8 < 4 returns dx[d8], lane 4 + d returns
-// dy[d]; every lane returns the base cell.  Same operations as ib_window.
-__device__ inline double ib_window_factor(int comp, double cx, double cy, unsigned n, unsigned W, unsigned D,
-                                          unsigned nx, unsigned ny, int d8, int& ix0, int& iy0)
-{
-	const double h = 1.0 / n;
-	double q0 = cx, q1 = cy;
-	if (comp == 0) q1 -= 0.5 * h; else q0 -= 0.5 * h;
-	q0 = fmod_pow2(fmod_pow2(q0, W) + W, W);
-	q1 = fmod_pow2(fmod_pow2(q1, D) + D, D);
-	double fx = floor(q0 * n), fy = floor(q1 * n);
-	if (!(fx >= 0.0 && fx <= (double) nx)) fx = 0.0;
-	if (!(fy >= 0.0 && fy <= (double) ny)) fy = 0.0;
-	ix0 = (int) (((unsigned) fx + nx - 1) & (nx - 1));
-	iy0 = (int) (((unsigned) fy + ny - 1) & (ny - 1));
-	if (d8 < 4) {
-		double x = (fx + d8 - 1.0) * h;
-		if (comp != 0) x += 0.5 * h;
-		return delta(x - cx, n);
-	}
-	double y = (fy + (d8 - 4) - 1.0) * h;
-	if (comp == 0) y += 0.5 * h;
-	return delta(y - cy, n);
+	window_deltas(rx, ry, g.n, dx, dy);
 }
 
 // sum_taps (w * vol) * field[idx], taps in the reference order (x fastest).

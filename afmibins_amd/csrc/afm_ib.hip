@@ -1,7 +1,7 @@
 // afm_ib.hip -- Lagrangian <-> Eulerian transfer on gfx950.
 //
-// Interpolation (ib/grid.h:68-84, marker targets): eight lanes per marker,
-// one delta factor each, 16 taps summed in the reference's order.  Same
+// Interpolation (ib/grid.h:68-84, marker targets): one lane per marker, its
+// eight delta factors, 16 taps summed in the reference's order.  Same
 // arithmetic as the reference, so the result is bit-exact against the CPU
 // restatement.
 //
@@ -36,35 +36,17 @@ __global__ void k_delta(const double* __restrict__ r, long count, unsigned n, do
 	if (i < count) out[i] = delta(r[i], n);
 }
 
-// Eight lanes per marker: lane d evaluates one delta factor (d < 4: dx[d],
-// else dy[d - 4]) and the products of taps d and d + 8 (tap a + 4 b, x
-// fastest); lane 0 of the group adds the 16 products in tap order.
+// One lane per marker: its window's eight delta factors, then the 16 taps
+// summed in the reference's order (tap a + 4 b, x fastest).
 __global__ void k_interp(int comp, const double* __restrict__ f, const double* __restrict__ X, long m, WinGeom g,
                          double* __restrict__ U)
 {
-	const unsigned nx = g.nx, ny = g.ny;
-	const long t = blockIdx.x * (long) blockDim.x + threadIdx.x;
-	const long k = t >> 3;
-	const int d8 = (int) (t & 7);
-	// every lane of the wave takes part in the shuffles: no early exit
-	const long kk = k < m ? k : m - 1;
+	const long k = blockIdx.x * (long) blockDim.x + threadIdx.x;
+	if (k >= m) return;
 	int ix0, iy0;
-	const double fct = ib_window_factor(comp, X[2 * kk], X[2 * kk + 1], g, d8, ix0, iy0);
-	const int a = d8 & 3, b = d8 >> 2;
-	const double wx = __shfl(fct, a, 8), wy0 = __shfl(fct, 4 + b, 8), wy1 = __shfl(fct, 6 + b, 8);
-	const double vol = g.vol;
-	int ix = ix0 + a; if (ix >= (int) nx) ix -= nx;
-	int iy = iy0 + b; if (iy >= (int) ny) iy -= ny;
-	int iy2 = iy0 + b + 2; if (iy2 >= (int) ny) iy2 -= ny;
-	const double w0 = wx * wy0, w1 = wx * wy1;
-	const double p0 = w0 * vol * f[ix + (size_t) nx * iy];
-	const double p1 = w1 * vol * f[ix + (size_t) nx * iy2];
-	double value = 0.0;
-#pragma unroll
-	for (int q = 0; q < 8; ++q) value += __shfl(p0, q, 8);
-#pragma unroll
-	for (int q = 0; q < 8; ++q) value += __shfl(p1, q, 8);
-	if (k < m && d8 == 0) U[k] = value;
+	double dx[4], dy[4];
+	ib_window_g(comp, X[2 * k], X[2 * k + 1], g, ix0, iy0, dx, dy);
+	U[k] = ib_gather(f, g.nx, g.ny, ix0, iy0, dx, dy, g.vol);
 }
 
 // X_h = X + half * (J_u(X) u, J_v(X) v).  The windows are the ones the
@@ -105,39 +87,37 @@ __global__ void k_ib_move_full(const double* __restrict__ uo, const double* __re
 	X[2 * k + 1] = X[2 * k + 1] + dt * (0.5 * (U0y + U1y));
 }
 
-// Per (marker, component): window, source value and bin, eight lanes per
-// pair: lane d < 4 evaluates the x factor dx[d], lane 4 + d the y factor
-// dy[d] (ib_window's operations, one factor each), lane 0 the rest.  With
-// R/k given, the source is the tether force F = -k (X - R); otherwise
+// Per (marker, component), one lane each: window, source value and bin.
+// With R/k given, the source is the tether force F = -k (X - R); otherwise
 // F[comp][k].
 __global__ void k_spread_prep(int comp_mask, const double* __restrict__ Fx, const double* __restrict__ Fy,
                               const double* __restrict__ ds, const double* __restrict__ X,
                               const double* __restrict__ R, const double* __restrict__ kk, long m, WinGeom g,
                               int sbx, int nb1, MarkerRec* __restrict__ rec, int* __restrict__ counts)
 {
-	const long t = blockIdx.x * (long) blockDim.x + threadIdx.x;
-	const long k = t >> 3;
-	const int d8 = (int) (t & 7);
+	const long k = blockIdx.x * (long) blockDim.x + threadIdx.x;
 	const int comp = blockIdx.y;
 	if (k >= m || !((comp_mask >> comp) & 1)) return;
 	const double cx = X[2 * k], cy = X[2 * k + 1];
 	int ix0, iy0;
-	const double f = ib_window_factor(comp, cx, cy, g, d8, ix0, iy0);
-	MarkerRec& r = rec[comp * m + k];
-	if (d8 < 4) r.dx[d8] = f;
-	else r.dy[d8 - 4] = f;
-	if (d8 == 0) {
-		double F;
-		if (R) F = comp == 0 ? -kk[k] * (cx - R[2 * k]) : -kk[k] * (cy - R[2 * k + 1]);
-		else F = comp == 0 ? Fx[k] : Fy[k];
-		r.ix0 = ix0;
-		r.iy0 = iy0;
-		r.ds = ds[k];
-		r.F = F;
-		r.tile = ix0 / SB + sbx * (iy0 / SB);
-		r.pad = 0;
-		atomicAdd(&counts[comp * nb1 + r.tile], 1);
+	double dx[4], dy[4];
+	ib_window_g(comp, cx, cy, g, ix0, iy0, dx, dy);
+	const double F = R ? (comp == 0 ? -kk[k] * (cx - R[2 * k]) : -kk[k] * (cy - R[2 * k + 1]))
+	                   : (comp == 0 ? Fx[k] : Fy[k]);
+	MarkerRec r;
+	r.ix0 = ix0;
+	r.iy0 = iy0;
+	r.tile = ix0 / SB + sbx * (iy0 / SB);
+	r.pad = 0;
+#pragma unroll
+	for (int d = 0; d < 4; ++d) {
+		r.dx[d] = dx[d];
+		r.dy[d] = dy[d];
 	}
+	r.ds = ds[k];
+	r.F = F;
+	rec[comp * m + k] = r;
+	atomicAdd(&counts[comp * nb1 + r.tile], 1);
 }
 
 // Exclusive scan of the bin counts in two launches: chunk sums (block c
@@ -549,7 +529,7 @@ int launch_delta(Ctx* c, const double* r, long count, unsigned n, double* out)
 int launch_interp(Ctx* c, int comp, const double* field, const double* X, long m, double* U)
 {
 	if (m <= 0) return 0;
-	hipLaunchKernelGGL(k_interp, dim3((8 * m + 255) / 256), dim3(256), 0, c->stream, comp, field, X, m,
+	hipLaunchKernelGGL(k_interp, dim3((m + 255) / 256), dim3(256), 0, c->stream, comp, field, X, m,
 	                   win_geom(c->n, c->W, c->D, c->nx, c->ny), U);
 	AFM_LAUNCH_CHECK();
 	return 0;
@@ -617,8 +597,8 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 	const bool ordered = m >= 8L * (long) gblocks && c->gorder && c->sbx >= GB && c->sby >= GB;
 	if (m > 0) {
 		AFM_TRY(ensure_markers(c, m));
-		dim3 g((m + 255) / 256, 2), g8((8 * m + 255) / 256, 2);
-		hipLaunchKernelGGL(k_spread_prep, g8, dim3(256), 0, c->stream, comp_mask, Fx, Fy, ds, X, R, k, m,
+		dim3 g((m + 255) / 256, 2);
+		hipLaunchKernelGGL(k_spread_prep, g, dim3(256), 0, c->stream, comp_mask, Fx, Fy, ds, X, R, k, m,
 		                   win_geom(c->n, c->W, c->D, c->nx, c->ny), c->sbx, nb1, c->rec, counts);
 		AFM_LAUNCH_CHECK();
 		hipLaunchKernelGGL(k_spread_chunk, dim3(nchunk), dim3(256), 0, c->stream, counts, chunk, nb, items);

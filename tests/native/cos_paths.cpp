@@ -1,6 +1,8 @@
-// Host build of afm_delta.h: the fast cos path (series + rounding test)
-// against the full double-double series and against cosq rounded to double.
-// Usage: cos_paths N seed -> prints "mismatch_full mismatch_quad slow_count".
+// Host build of afm_delta.h: the fast cos path (table + short series +
+// rounding test) against the full double-double series and against cosq
+// rounded to double; the fast delta argument (fma products, midpoint test)
+// against the exact integer one; the delta against its exact definition.
+// Usage: cos_paths N seed -> prints "mismatch_full mismatch_quad mismatch_arg".
 #define AFM_HD
 #include "afm_delta.h"
 #include <quadmath.h>
@@ -28,6 +30,20 @@ int main(int argc, char** argv)
 		if (a != cos_cr_full(x)) ++bad_full;
 		if (i % 8 == 0 && a != (double) cosq((__float128) x)) ++bad_quad;
 	}
-	printf("%ld %ld\n", bad_full, bad_quad);
+	// delta arguments: r uniform over the delta's support |n r| < 3 for
+	// n = 2 .. 2^13, a quarter of them tiny or near the support's ends
+	long bad_arg = 0;
+	for (long i = 0; i < N; ++i) {
+		const unsigned n = 1u << (1 + (int) (g() % 13));
+		double r = (3.0 / n) * ((double) (g() >> 11) * 0x1p-53) * ((g() & 1) ? 1 : -1);
+		if (i % 4 == 1) r = std::ldexp(r, -(int) (g() % 60));
+		if (i % 4 == 2) r = std::nextafter(3.0 / n, 0.0) * ((g() & 1) ? 1 : -1) - (double) (g() % 1000) * 0x1p-60;
+		if (delta_arg(r, n) != delta_arg_exact(r, n)) ++bad_arg;
+		// the whole delta (fast paths, flags, support test) against the exact
+		// definition, also outside the support (|n r| up to 4)
+		const double r2 = (4.0 / n) * ((double) (g() >> 11) * 0x1p-53) * ((g() & 1) ? 1 : -1);
+		if (delta(r, n) != delta_exact(r, n) || delta(r2, n) != delta_exact(r2, n)) ++bad_arg;
+	}
+	printf("%ld %ld %ld\n", bad_full, bad_quad, bad_arg);
 	return 0;
 }
