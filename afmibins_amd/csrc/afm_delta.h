@@ -147,7 +147,15 @@ static const double AFM_COS_TAB[2][2 * AFM_COS_TAB_JMAX + 1][4] = AFM_COS_TAB_IN
 // (|b| < 2^-16.6 R), the fma that adds V b < 2^-69.6, all other terms below
 // 2^-70; so R is within 2^-66.5 R of the true value and the test below, with
 // a band of 2^-65 R, accepts only results that the exact value rounds to.
-AFM_HD AFM_INLINE inline double cos_fast(double x, bool& ok)
+// The table-independent part of the fast path: reduction, table row, d and
+// the short series.  Split from cos_finish so that a lane evaluating several
+// factors issues all its table loads before it waits for any.
+struct CosPrep {
+	double ylo, d, z, ze, a, b;
+	int q, row;           // quadrant; table row (parity * rows + j + JMAX)
+};
+
+AFM_HD AFM_INLINE inline void cos_prep(double x, CosPrep& c)
 {
 	const double P1 = 0x1.921fb54442d18p+0, P2 = 0x1.1a62633145c08p-54,
 	             P3 = -0x1.1f1976b7ed8f8p-106, P4 = -0x1.dd660ce2ff7d1p-157;
@@ -158,26 +166,35 @@ AFM_HD AFM_INLINE inline double cos_fast(double x, bool& ok)
 	dd s = two_sum(t, -(k * P2));
 	double ylo = (s.lo - k * P3) - k * P4;
 	dd y = fast_two_sum(s.hi, ylo);
-	int q = ((int) k) & 3;
-	const bool odd = (q & 1) != 0;
+	c.q = ((int) k) & 3;
 	const double jf = rint(y.hi * 64.0);
-	const double d = fma(jf, -0x1p-6, y.hi);
+	c.d = fma(jf, -0x1p-6, y.hi);
 	// |jf| <= 51 for |x| < 8; the clamp keeps the read in the table for
 	// any input (NaN included)
 	const int ji = (int) fmin(fmax(jf, -(double) AFM_COS_TAB_JMAX), (double) AFM_COS_TAB_JMAX);
-	const double* T = AFM_COS_TAB[odd ? 1 : 0][ji + AFM_COS_TAB_JMAX];
-	const double Uh = T[0], Ul = T[1], Vh = T[2], Vl = T[3];
-	const double z = d * d, ze = fma(d, d, -z);
-	const double a = fma(z * z, fma(z, fma(z, -0x1.a01a01a01a01ap-16, 0x1.6c16c16c16c17p-10),
-	                                -0x1.5555555555555p-5), 0.5 * ze);
-	const double b = (d * z) * fma(z, fma(z, -0x1.a01a01a01a01ap-13, 0x1.1111111111111p-7),
-	                               -0x1.5555555555555p-3);
+	c.row = (c.q & 1) * (2 * AFM_COS_TAB_JMAX + 1) + ji + AFM_COS_TAB_JMAX;
+	c.ylo = y.lo;
+	const double d = c.d;
+	c.z = d * d;
+	c.ze = fma(d, d, -c.z);
+	const double z = c.z;
+	c.a = fma(z * z, fma(z, fma(z, -0x1.a01a01a01a01ap-16, 0x1.6c16c16c16c17p-10), -0x1.5555555555555p-5),
+	          0.5 * c.ze);
+	c.b = (d * z) * fma(z, fma(z, -0x1.a01a01a01a01ap-13, 0x1.1111111111111p-7), -0x1.5555555555555p-3);
+}
+
+AFM_HD AFM_INLINE inline const double* cos_row(const CosPrep& c) { return AFM_COS_TAB[0][c.row]; }
+
+// The rest of the fast path from the table row T = (U hi, lo, V hi, lo).
+AFM_HD AFM_INLINE inline double cos_finish(const CosPrep& c, double Uh, double Ul, double Vh, double Vl, bool& ok)
+{
+	const double d = c.d, z = c.z, ylo = c.ylo;
 	const dd p = two_prod(Vh, d);
 	const dd w = two_prod(0.5 * Uh, z);
 	// terms below 2^-52 R first, then the two larger ones by fma
-	double small = ((Ul + p.lo) - w.lo) + fma(Vl, d, fma(Vh, y.lo, -(Uh * y.lo) * d))
-	             - (Ul * z) * 0.5 - ((Vh * y.lo) * z) * 0.5;
-	small = fma(Vh, b, fma(-Uh, a, small));
+	double small = ((Ul + p.lo) - w.lo) + fma(Vl, d, fma(Vh, ylo, -(Uh * ylo) * d))
+	             - (Ul * z) * 0.5 - ((Vh * ylo) * z) * 0.5;
+	small = fma(Vh, c.b, fma(-Uh, c.a, small));
 	const dd s1 = two_sum(Uh, p.hi);
 	const dd s2 = two_sum(s1.hi, -w.hi);
 	const dd r = fast_two_sum(s2.hi, (s1.lo + s2.lo) + small);
@@ -185,7 +202,15 @@ AFM_HD AFM_INLINE inline double cos_fast(double x, bool& ok)
 	// bitwise & (no short-circuit): the flag costs no branch
 	ok = ok & (r.hi + (r.lo + e) == r.hi) & (r.hi + (r.lo - e) == r.hi);
 	// cos(y + k pi/2): q=0 cos y, q=1 -sin y, q=2 -cos y, q=3 sin y
-	return (q == 1 || q == 2) ? -r.hi : r.hi;
+	return (c.q == 1 || c.q == 2) ? -r.hi : r.hi;
+}
+
+AFM_HD AFM_INLINE inline double cos_fast(double x, bool& ok)
+{
+	CosPrep c;
+	cos_prep(x, c);
+	const double* T = cos_row(c);
+	return cos_finish(c, T[0], T[1], T[2], T[3], ok);
 }
 
 // The degree-14 double-double series on the reduced argument (relative error
@@ -326,6 +351,43 @@ AFM_HD AFM_INLINE inline double delta_fast(double r, unsigned n, bool& ok)
 	const double c = cos_fast(in ? arg : 0.0, fok);
 	ok = ok & (fok | !in);
 	return in ? 0.25 * n * (1 + c) : 0.0;
+}
+
+// delta_fast of N factors at once: all arguments and reductions, then all
+// table loads, then the rest; bit j of the result set where factor j needs
+// delta_exact
+template<int N>
+AFM_HD AFM_INLINE inline unsigned delta_fast_n(const double (&r)[N], unsigned n, double (&f)[N])
+{
+	CosPrep c[N];
+	bool ok[N], in[N];
+#pragma unroll
+	for (int j = 0; j < N; ++j) {
+		in[j] = fabs(n * r[j]) < 3.0;
+		ok[j] = true;
+		const double arg = delta_arg_fast(r[j], n, ok[j]);
+		cos_prep(in[j] ? arg : 0.0, c[j]);
+	}
+	double T[N][4];
+#pragma unroll
+	for (int j = 0; j < N; ++j) {
+		const double* t = cos_row(c[j]);
+#pragma unroll
+		for (int e = 0; e < 4; ++e) T[j][e] = t[e];
+	}
+#if defined(__HIP_DEVICE_COMPILE__)
+	// keep the scheduler from sinking each load to its first use: the loads
+	// are then in flight together and the lane waits once
+	__builtin_amdgcn_sched_barrier(0);
+#endif
+	unsigned bad = 0;
+#pragma unroll
+	for (int j = 0; j < N; ++j) {
+		const double cv = cos_finish(c[j], T[j][0], T[j][1], T[j][2], T[j][3], ok[j]);
+		f[j] = in[j] ? 0.25 * n * (1 + cv) : 0.0;
+		bad |= (ok[j] | !in[j]) ? 0u : 1u << j;
+	}
+	return bad;
 }
 
 // delta by the exact argument and the full series (the definition the fast

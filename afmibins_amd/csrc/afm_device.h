@@ -178,14 +178,15 @@ __device__ inline double fmod_pow2(double x, unsigned p)
 __device__ inline void window_deltas(const double rx[4], const double ry[4], unsigned n, double dx[4],
                                      double dy[4])
 {
-	unsigned bad = 0;
+	double r4[4], f4[4];
 #pragma unroll
-	for (int d = 0; d < 4; ++d) {
-		bool okx = true, oky = true;
-		dx[d] = delta_fast(rx[d], n, okx);
-		dy[d] = delta_fast(ry[d], n, oky);
-		bad |= (okx ? 0u : 1u << d) | (oky ? 0u : 16u << d);
-	}
+	for (int d = 0; d < 4; ++d) r4[d] = rx[d];
+	unsigned bad = delta_fast_n<4>(r4, n, f4);
+#pragma unroll
+	for (int d = 0; d < 4; ++d) { dx[d] = f4[d]; r4[d] = ry[d]; }
+	bad |= delta_fast_n<4>(r4, n, f4) << 4;
+#pragma unroll
+	for (int d = 0; d < 4; ++d) dy[d] = f4[d];
 	// one inlined copy of the exact path, in a loop over the factors that
 	// need it (register selects, not indexed arrays: nothing goes to scratch)
 	while (bad) {
@@ -262,6 +263,55 @@ inline WinGeom win_geom(unsigned n, unsigned W, unsigned D, unsigned nx, unsigne
 
 // fmod_pow2 with the period and its reciprocal given
 __device__ inline double fmod_pow2g(double x, double pd, double inv) { return x - pd * trunc(x * inv); }
+
+// f[d] = delta(r[d]), d = 0..3: as window_deltas, for one axis
+__device__ inline void axis_deltas(const double r[4], unsigned n, double f[4])
+{
+	double rr4[4], f4[4];
+#pragma unroll
+	for (int d = 0; d < 4; ++d) rr4[d] = r[d];
+	unsigned bad = delta_fast_n<4>(rr4, n, f4);
+#pragma unroll
+	for (int d = 0; d < 4; ++d) f[d] = f4[d];
+	while (bad) {
+		const int i = __builtin_ctz(bad);
+		bad &= bad - 1;
+		double rr = r[0];
+#pragma unroll
+		for (int d = 1; d < 4; ++d) rr = i == d ? r[d] : rr;
+		const double v = delta_exact_inl(rr, n);
+#pragma unroll
+		for (int d = 0; d < 4; ++d) f[d] = i == d ? v : f[d];
+	}
+}
+
+// One axis of ib_window with the launch's constants (the same operations):
+// p = 0 gives dx, p = 1 dy; both give the base cell.  A lane pair (one lane
+// per axis) forms a marker's window with half the work per lane.
+__device__ inline void ib_window_axis(int comp, double cx, double cy, const WinGeom& g, int p, int& ix0, int& iy0,
+                                      double f[4])
+{
+	double q0 = cx, q1 = cy;
+	if (comp == 0) q1 -= g.half_h; else q0 -= g.half_h;
+	q0 = fmod_pow2g(fmod_pow2g(q0, g.Wd, g.invW) + g.Wd, g.Wd, g.invW);
+	q1 = fmod_pow2g(fmod_pow2g(q1, g.Dd, g.invD) + g.Dd, g.Dd, g.invD);
+	double fx = floor(q0 * g.nd), fy = floor(q1 * g.nd);
+	if (!(fx >= 0.0 && fx <= g.nxd)) fx = 0.0;
+	if (!(fy >= 0.0 && fy <= g.nyd)) fy = 0.0;
+	ix0 = (int) (((unsigned) fx + g.nx - 1) & (g.nx - 1));
+	iy0 = (int) (((unsigned) fy + g.ny - 1) & (g.ny - 1));
+	const double base = p ? fy : fx, c = p ? cy : cx;
+	// comp 0 shifts the y taps by h/2, comp 1 the x taps
+	const bool shift = (p == 0) == (comp != 0);
+	double r[4];
+#pragma unroll
+	for (int d = 0; d < 4; ++d) {
+		double x = (base + d - 1.0) * g.h;
+		if (shift) x += g.half_h;
+		r[d] = x - c;
+	}
+	axis_deltas(r, g.n, f);
+}
 
 // ib_window with the launch's constants (the same operations)
 __device__ inline void ib_window_g(int comp, double cx, double cy, const WinGeom& g, int& ix0, int& iy0,

@@ -36,8 +36,10 @@ __global__ void k_delta(const double* __restrict__ r, long count, unsigned n, do
 	if (i < count) out[i] = delta(r[i], n);
 }
 
-// One lane per marker: its window's eight delta factors, then the 16 taps
-// summed in the reference's order (tap a + 4 b, x fastest).
+// One lane per marker: its window's eight delta factors (two batches of
+// four fast paths), then the 16 taps summed in the reference's order (tap
+// a + 4 b, x fastest).  (Two lanes per marker, one axis each, were slower:
+// 26.8 against 21.4 us at 262,144 markers.)
 __global__ void k_interp(int comp, const double* __restrict__ f, const double* __restrict__ X, long m, WinGeom g,
                          double* __restrict__ U)
 {
@@ -87,37 +89,41 @@ __global__ void k_ib_move_full(const double* __restrict__ uo, const double* __re
 	X[2 * k + 1] = X[2 * k + 1] + dt * (0.5 * (U0y + U1y));
 }
 
-// Per (marker, component), one lane each: window, source value and bin.
-// With R/k given, the source is the tether force F = -k (X - R); otherwise
-// F[comp][k].
+// Per (marker, component), two lanes: lane 0 the window's base cell, bin
+// and x factors, lane 1 the y factors and the source (the two halves of the
+// record).  With R/k given, the source is the tether force F = -k (X - R);
+// otherwise F[comp][k].
 __global__ void k_spread_prep(int comp_mask, const double* __restrict__ Fx, const double* __restrict__ Fy,
                               const double* __restrict__ ds, const double* __restrict__ X,
                               const double* __restrict__ R, const double* __restrict__ kk, long m, WinGeom g,
                               int sbx, int nb1, MarkerRec* __restrict__ rec, int* __restrict__ counts)
 {
-	const long k = blockIdx.x * (long) blockDim.x + threadIdx.x;
+	const long t = blockIdx.x * (long) blockDim.x + threadIdx.x;
+	const long k = t >> 1;
+	const int p = (int) (t & 1);
 	const int comp = blockIdx.y;
 	if (k >= m || !((comp_mask >> comp) & 1)) return;
 	const double cx = X[2 * k], cy = X[2 * k + 1];
 	int ix0, iy0;
-	double dx[4], dy[4];
-	ib_window_g(comp, cx, cy, g, ix0, iy0, dx, dy);
-	const double F = R ? (comp == 0 ? -kk[k] * (cx - R[2 * k]) : -kk[k] * (cy - R[2 * k + 1]))
-	                   : (comp == 0 ? Fx[k] : Fy[k]);
-	MarkerRec r;
-	r.ix0 = ix0;
-	r.iy0 = iy0;
-	r.tile = ix0 / SB + sbx * (iy0 / SB);
-	r.pad = 0;
+	double fct[4];
+	ib_window_axis(comp, cx, cy, g, p, ix0, iy0, fct);
+	MarkerRec& r = rec[comp * m + k];
+	if (p == 0) {
+		const int tile = ix0 / SB + sbx * (iy0 / SB);
+		r.ix0 = ix0;
+		r.iy0 = iy0;
+		r.tile = tile;
+		r.pad = 0;
 #pragma unroll
-	for (int d = 0; d < 4; ++d) {
-		r.dx[d] = dx[d];
-		r.dy[d] = dy[d];
+		for (int d = 0; d < 4; ++d) r.dx[d] = fct[d];
+		atomicAdd(&counts[comp * nb1 + tile], 1);
+	} else {
+#pragma unroll
+		for (int d = 0; d < 4; ++d) r.dy[d] = fct[d];
+		r.ds = ds[k];
+		r.F = R ? (comp == 0 ? -kk[k] * (cx - R[2 * k]) : -kk[k] * (cy - R[2 * k + 1]))
+		        : (comp == 0 ? Fx[k] : Fy[k]);
 	}
-	r.ds = ds[k];
-	r.F = F;
-	rec[comp * m + k] = r;
-	atomicAdd(&counts[comp * nb1 + r.tile], 1);
 }
 
 // Exclusive scan of the bin counts in two launches: chunk sums (block c
@@ -597,8 +603,8 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 	const bool ordered = m >= 8L * (long) gblocks && c->gorder && c->sbx >= GB && c->sby >= GB;
 	if (m > 0) {
 		AFM_TRY(ensure_markers(c, m));
-		dim3 g((m + 255) / 256, 2);
-		hipLaunchKernelGGL(k_spread_prep, g, dim3(256), 0, c->stream, comp_mask, Fx, Fy, ds, X, R, k, m,
+		dim3 g((m + 255) / 256, 2), g2((2 * m + 255) / 256, 2);
+		hipLaunchKernelGGL(k_spread_prep, g2, dim3(256), 0, c->stream, comp_mask, Fx, Fy, ds, X, R, k, m,
 		                   win_geom(c->n, c->W, c->D, c->nx, c->ny), c->sbx, nb1, c->rec, counts);
 		AFM_LAUNCH_CHECK();
 		hipLaunchKernelGGL(k_spread_chunk, dim3(nchunk), dim3(256), 0, c->stream, counts, chunk, nb, items);
