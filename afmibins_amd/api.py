@@ -455,14 +455,18 @@ class NSSolver:
         return p
 
     def get_start(self, kind):
-        """The CG forms' start potential of step kind 0 (backward-Euler half
-        step) or 1 (Crank-Nicolson full step), or None while there is none."""
+        """The CG forms' potentials: kind 0 / 1 the next start of the
+        backward-Euler half step / Crank-Nicolson full step (extrapolated
+        from the kind's last two potentials), kind 2 / 3 the kind's last
+        potential; None while unset."""
         phi = torch.empty_like(self.u)
         ok = C.c_int()
         self._lib("afm_ns_get_start", kind, _ptr(phi), C.byref(ok))
         return phi if ok.value else None
 
     def set_start(self, kind, phi):
+        """Set a potential of get_start (None: unset).  Setting kind 0 / 1
+        unsets kind 2 / 3: restore 0, 1 before 2, 3."""
         if phi is not None:
             self.ctx._f(phi, (self.ctx.ny, self.ctx.nx), "phi")
         self._lib("afm_ns_set_start", kind, _ptr(phi) if phi is not None else None)

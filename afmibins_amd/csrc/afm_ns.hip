@@ -69,8 +69,12 @@ __global__ void k_div(const double* __restrict__ u, const double* __restrict__ v
 // u -= n (p - p[x-1]),  v -= n (p - p[y-1])   (alg/grad.h:36-43, ns/projection.h:70-72)
 // u -= n (p - p_west), v -= n (p - p_south); with pcopy, p is also copied
 // there (the projection's potential kept for get_p)
+// u -= grad p, v likewise; pcopy (when given) receives p.  xnext (the CG
+// forms' next start of this step kind, when given) receives the linear
+// extrapolation 2 p - pcopy_old from the previous potential of the kind
+// (extrap), else p itself (2 p - p = p exactly).
 __global__ void k_grad_sub(const double* __restrict__ p, double* __restrict__ u, double* __restrict__ v, int nx, int ny,
-                           unsigned n, double* __restrict__ pcopy)
+                           unsigned n, double* __restrict__ pcopy, double* __restrict__ xnext, int extrap)
 {
 	const size_t N = (size_t) nx * ny;
 	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
@@ -82,7 +86,12 @@ __global__ void k_grad_sub(const double* __restrict__ p, double* __restrict__ u,
 	const double pz = n * (p[i] - p[izm]);
 	u[i] = u[i] - px;
 	v[i] = v[i] - pz;
-	if (pcopy) pcopy[i] = p[i];
+	if (pcopy) {
+		const double pv = p[i];
+		const double old = extrap ? pcopy[i] : pv;
+		pcopy[i] = pv;
+		if (xnext) xnext[i] = 2.0 * pv - old;
+	}
 }
 
 __global__ void k_helm_apply(const double* __restrict__ x, double* __restrict__ y, int nx, int ny, double off, double dg)
@@ -584,7 +593,7 @@ int op_div(Ctx* c, const double* u, const double* v, double* d)
 int op_grad_sub(Ctx* c, const double* p, double* u, double* v)
 {
 	hipLaunchKernelGGL(k_grad_sub, pgrid(c->N), dim3(256), 0, c->stream, p, u, v, (int) c->nx, (int) c->ny, c->n,
-	                   (double*) nullptr);
+	                   (double*) nullptr, (double*) nullptr, 0);
 	AFM_LAUNCH_CHECK();
 	return 0;
 }
@@ -749,11 +758,14 @@ struct NS {
 	double *u, *v;                         // caller-owned, aliased (ns/solver.h:54-56)
 	double *uo = nullptr, *vo = nullptr, *phi = nullptr, *fz = nullptr;
 	// The CG forms (precision 1, 3) start a step's projection from the
-	// potential of the previous step of the same kind (0: backward-Euler half
-	// step, 1: Crank-Nicolson full step), once there is one; the potential
-	// of every projection is kept where get_p reads it (phi_last).
-	double* start[2] = {nullptr, nullptr};
-	bool start_ok[2] = {false, false};
+	// potentials of the previous steps of the same kind (0: backward-Euler
+	// half step, 1: Crank-Nicolson full step): pot[kind] is the next start,
+	// the linear extrapolation 2 phi_n - phi_{n-1} of the last two potentials
+	// of the kind (phi_n alone after the first), pot[kind + 2] the last
+	// potential phi_n.  The potential of every projection is kept where get_p
+	// reads it (phi_last).
+	double* pot[4] = {nullptr, nullptr, nullptr, nullptr};
+	bool pot_ok[4] = {false, false, false, false};
 	const double* phi_last = nullptr;
 	CGWork cg;
 	bool half = false;
@@ -772,18 +784,21 @@ int ns_project(NS* s, double* u, double* v, int kind)
 	AFM_TRY(op_div(c, u, v, s->phi));
 	int cyc = 0;
 	const double* sol = nullptr;
-	double* keep = (kind >= 0 && s->start[kind]) ? s->start[kind] : s->phi;
-	const double* x0 = (keep != s->phi && s->start_ok[kind]) ? keep : nullptr;
+	const bool warm = kind >= 0 && s->pot[kind];
+	const double* x0 = (warm && s->pot_ok[kind]) ? s->pot[kind] : nullptr;
 	AFM_TRY(mg_solve(c, s->phi, s->tol, s->tol_mode, 2, 2, s->precision, s->max_cycles, &cyc, nullptr, &sol, x0));
 	s->last_cycles = cyc;
 	s->last_rms = c->hctl[0].res;
 	s->last_thr = c->hctl[0].tol;
 	// the gradient is taken where the solve left the potential, and the
-	// potential is stored (to phi, or as the kind's next start) in the same pass
+	// potential is stored (to phi, or as the kind's last potential, with the
+	// kind's next start extrapolated from it and the one before) in the same
+	// pass
+	double* keep = warm ? s->pot[kind + 2] : s->phi;
 	hipLaunchKernelGGL(k_grad_sub, pgrid(c->N), dim3(256), 0, c->stream, sol, u, v, (int) c->nx, (int) c->ny, c->n,
-	                   keep);
+	                   keep, warm ? s->pot[kind] : nullptr, warm && s->pot_ok[kind + 2] ? 1 : 0);
 	AFM_LAUNCH_CHECK();
-	if (keep != s->phi) s->start_ok[kind] = true;
+	if (warm) s->pot_ok[kind] = s->pot_ok[kind + 2] = true;
 	s->phi_last = keep;
 	return 0;
 }
@@ -802,8 +817,8 @@ int ns_create(Ctx* c, double mu, double rho, double dt, double tol, int tol_mode
 		return rc;
 	}
 	if (precision == 1 || precision == 3)
-		for (int k = 0; k < 2; ++k)
-			if ((rc = dalloc(o, (void**) &s->start[k], c->N * 8))) { delete s; return rc; }
+		for (int k = 0; k < 4; ++k)
+			if ((rc = dalloc(o, (void**) &s->pot[k], c->N * 8))) { delete s; return rc; }
 	s->phi_last = s->phi;
 	AFM_HIP(hipMemsetAsync(s->uo, 0, c->N * 8, c->stream));
 	AFM_HIP(hipMemsetAsync(s->vo, 0, c->N * 8, c->stream));
@@ -958,12 +973,12 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 	if (!c0 || !c1 || !part || !Xb || !ub || !vb) { set_error("out of device memory (adjoint)"); return fail(-3); }
 	if ((rc = adj_centroids(c, b->X, nm, msize, nullptr, c0, nullptr, nullptr))) return fail(rc);
 	// a checkpoint: u, v, X and, for the CG forms, the projections' start
-	// potentials (so that a segment's recomputation repeats the forward run
-	// bit for bit)
-	const bool warm = s->start[0] != nullptr;
-	const int K = warm ? 5 : 3;
+	// and last potentials (so that a segment's recomputation repeats the
+	// forward run bit for bit)
+	const bool warm = s->pot[0] != nullptr;
+	const int K = warm ? 7 : 3;
 	std::vector<double*> ck(want_grad ? K * nseg : 0);
-	std::vector<char> ck_ok(want_grad ? 2 * nseg : 0);
+	std::vector<char> ck_ok(want_grad ? 4 * nseg : 0);
 	for (size_t i = 0; i < ck.size(); ++i) {
 		ck[i] = A(i % K == 2 ? 2 * m : N);
 		if (!ck[i]) { set_error("out of device memory (checkpoints)"); return fail(-3); }
@@ -972,9 +987,9 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 		AFM_TRY(d2d(c, ck[K * sg], s->u, N));
 		AFM_TRY(d2d(c, ck[K * sg + 1], s->v, N));
 		AFM_TRY(d2d(c, ck[K * sg + 2], b->X, 2 * m));
-		for (int k = 0; warm && k < 2; ++k) {
-			AFM_TRY(d2d(c, ck[K * sg + 3 + k], s->start[k], N));
-			ck_ok[2 * sg + k] = s->start_ok[k];
+		for (int k = 0; warm && k < 4; ++k) {
+			AFM_TRY(d2d(c, ck[K * sg + 3 + k], s->pot[k], N));
+			ck_ok[4 * sg + k] = s->pot_ok[k];
 		}
 		return 0;
 	};
@@ -982,9 +997,9 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 		AFM_TRY(d2d(c, s->u, ck[K * sg], N));
 		AFM_TRY(d2d(c, s->v, ck[K * sg + 1], N));
 		AFM_TRY(d2d(c, b->X, ck[K * sg + 2], 2 * m));
-		for (int k = 0; warm && k < 2; ++k) {
-			AFM_TRY(d2d(c, s->start[k], ck[K * sg + 3 + k], N));
-			s->start_ok[k] = ck_ok[2 * sg + k] != 0;
+		for (int k = 0; warm && k < 4; ++k) {
+			AFM_TRY(d2d(c, s->pot[k], ck[K * sg + 3 + k], N));
+			s->pot_ok[k] = ck_ok[4 * sg + k] != 0;
 		}
 		return 0;
 	};
@@ -1008,8 +1023,12 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 	}
 	// final state, kept to restore after the sweep
 	double* fu = A(N); double* fv = A(N); double* fX = A(2 * m);
-	double* fs[2] = {warm ? A(N) : nullptr, warm ? A(N) : nullptr};
-	const bool fs_ok[2] = {s->start_ok[0], s->start_ok[1]};
+	double* fs[4] = {nullptr, nullptr, nullptr, nullptr};
+	bool fs_ok[4];
+	for (int k = 0; k < 4; ++k) {
+		if (warm) fs[k] = A(N);
+		fs_ok[k] = s->pot_ok[k];
+	}
 	std::vector<StepTape> tape(every);
 	for (auto& t : tape) { t.u = A(N); t.v = A(N); t.uh = A(N); t.vh = A(N); t.X = A(2 * m); t.Xh = A(2 * m); }
 	double* unb = A(N); double* vnb = A(N); double* uhb = A(N); double* vhb = A(N);
@@ -1020,7 +1039,8 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 	double* cx = A(m); double* cy = A(m); double* volv = A(m);
 	double* kb = gk ? gk : A(m); double* Rb = gR ? gR : A(2 * m);
 	if (!fu || !fv || !fX || !unb || !vnb || !uhb || !vhb || !us || !vs || !rbu || !rbv || !t1 || !t2 || !Xnb ||
-	    !Xhb || !Ux || !Uy || !Jx || !Jy || !cx || !cy || !volv || !kb || !Rb || (warm && (!fs[0] || !fs[1])) || oom) {
+	    !Xhb || !Ux || !Uy || !Jx || !Jy || !cx || !cy || !volv || !kb || !Rb ||
+	    (warm && (!fs[0] || !fs[1] || !fs[2] || !fs[3])) || oom) {
 		set_error("out of device memory (adjoint tape)");
 		return fail(-3);
 	}
@@ -1031,8 +1051,8 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 	if ((rc = d2d(c, fu, s->u, N)) || (rc = d2d(c, fv, s->v, N)) || (rc = d2d(c, fX, b->X, 2 * m)) ||
 	    (rc = d2d(c, fphi, s->phi, N)))
 		return fail(rc);
-	for (int k = 0; warm && k < 2; ++k)
-		if ((rc = d2d(c, fs[k], s->start[k], N))) return fail(rc);
+	for (int k = 0; warm && k < 4; ++k)
+		if ((rc = d2d(c, fs[k], s->pot[k], N))) return fail(rc);
 	const double vol = 1. / (c->n * c->n);
 	{
 		std::vector<double> hv(m, vol);
@@ -1119,11 +1139,11 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 	if ((rc = d2d(c, s->u, fu, N)) || (rc = d2d(c, s->v, fv, N)) || (rc = d2d(c, b->X, fX, 2 * m)) ||
 	    (rc = d2d(c, s->phi, fphi, N)))
 		return fail(rc);
-	for (int k = 0; warm && k < 2; ++k) {
-		if ((rc = d2d(c, s->start[k], fs[k], N))) return fail(rc);
-		s->start_ok[k] = fs_ok[k];
+	for (int k = 0; warm && k < 4; ++k) {
+		if ((rc = d2d(c, s->pot[k], fs[k], N))) return fail(rc);
+		s->pot_ok[k] = fs_ok[k];
 	}
-	s->phi_last = warm ? s->start[1] : s->phi;     // the last forward projection: a full step's
+	s->phi_last = warm ? s->pot[3] : s->phi;     // the last forward projection: a full step's
 	s->half = false;
 	return fail(0);
 }
@@ -1140,25 +1160,28 @@ void ib_free(IB* b) { (void) hipStreamSynchronize(b->ns->c->stream); delete b; }
 Ctx* ns_ctx(NS* s) { return s->c; }
 void ns_free(NS* s) { (void) hipStreamSynchronize(s->c->stream); delete s; }
 
-// the CG forms' start potential of a step kind (see NS): copy out, or in
+// the CG forms' potentials of a step kind (see NS): kind 0, 1 the next
+// start, 2, 3 the last potential; copy out, or in
 int ns_get_start(NS* s, int kind, double* phi, int* valid)
 {
-	if (kind < 0 || kind > 1) { set_error("kind must be 0 (half step) or 1 (full step)"); return -1; }
-	*valid = s->start[kind] && s->start_ok[kind];
-	if (*valid) AFM_HIP(hipMemcpyAsync(phi, s->start[kind], s->c->N * 8, hipMemcpyDeviceToDevice, s->c->stream));
+	if (kind < 0 || kind > 3) { set_error("kind must be 0 / 1 (next start of the half / full step) or 2 / 3 (their last potential)"); return -1; }
+	*valid = s->pot[kind] && s->pot_ok[kind];
+	if (*valid) AFM_HIP(hipMemcpyAsync(phi, s->pot[kind], s->c->N * 8, hipMemcpyDeviceToDevice, s->c->stream));
 	return 0;
 }
 
 int ns_set_start(NS* s, int kind, const double* phi)
 {
-	if (kind < 0 || kind > 1) { set_error("kind must be 0 (half step) or 1 (full step)"); return -1; }
-	if (!s->start[kind]) {
+	if (kind < 0 || kind > 3) { set_error("kind must be 0 / 1 (next start of the half / full step) or 2 / 3 (their last potential)"); return -1; }
+	if (!s->pot[kind]) {
 		if (!phi) return 0;
 		set_error("only the CG forms (precision 1, 3) start from a potential");
 		return -1;
 	}
-	s->start_ok[kind] = phi != nullptr;
-	if (phi) AFM_HIP(hipMemcpyAsync(s->start[kind], phi, s->c->N * 8, hipMemcpyDeviceToDevice, s->c->stream));
+	s->pot_ok[kind] = phi != nullptr;
+	// a start set alone has no last potential to extrapolate from
+	if (kind < 2) s->pot_ok[kind + 2] = false;
+	if (phi) AFM_HIP(hipMemcpyAsync(s->pot[kind], phi, s->c->N * 8, hipMemcpyDeviceToDevice, s->c->stream));
 	return 0;
 }
 

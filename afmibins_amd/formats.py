@@ -11,8 +11,8 @@ precision reproduces the reference's text exactly).
 A checkpoint holds what an IB run needs to continue: the velocity fields, the
 marker state (positions, tether points, stiffness, source volumes), the grid
 and fluid parameters and the step count, and -- for the CG forms of the
-projection (precision 1, 3) -- the two start potentials those solves begin
-from.  Between IB steps nothing else carries over (each step's
+projection (precision 1, 3) -- the potentials those solves start from (per
+step kind the next start and the last potential it is extrapolated from).  Between IB steps nothing else carries over (each step's
 backward-Euler half step takes its old velocity from u, v).  It is one
 ``.npz`` file, written and read with numpy alone and never unpickled; the
 parameters sit in a JSON string array inside it.
@@ -26,7 +26,7 @@ import numpy as np
 _NUM = r"[-+]?(?:nan|inf|(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?)"
 _VEC = re.compile(r"\[\s*((?:" + _NUM + r"\s*(?:,\s*" + _NUM + r"\s*)*)?)\]'")
 _PT = re.compile(r"(\d+)\s*\(\s*(" + _NUM + r")\s*,\s*(" + _NUM + r")\s*,\s*(" + _NUM + r")\s*\)")
-CHECKPOINT_VERSION = 2        # 2: optional start potentials (version 1 files are read too)
+CHECKPOINT_VERSION = 3        # 2: optional start potentials (2); 3: four (versions 1, 2 are read too)
 
 
 def _num(tok):
@@ -85,10 +85,12 @@ _ARRAYS = ("u", "v", "X", "R", "k", "ds")
 
 
 def save_checkpoint(path, u, v, X, R, k, ds, *, n, mu, rho, dt, tol, tol_mode=1, precision=0, step=0,
-                    starts=(None, None)):
+                    starts=(None, None, None, None)):
     """Write an IB state.  Arrays may be torch tensors (any device) or numpy
-    arrays; they are stored as float64.  starts: the solver's start
-    potentials (NSSolver.get_start(0), get_start(1)), None where unset."""
+    arrays; they are stored as float64.  starts: the solver's potentials
+    NSSolver.get_start(0) .. get_start(3) (fewer are padded with None),
+    None where unset."""
+    starts = tuple(starts) + (None,) * (4 - len(starts))
     def host(a):
         if hasattr(a, "detach"):
             a = a.detach().cpu().numpy()
@@ -118,8 +120,8 @@ def load_checkpoint(path):
         meta = json.loads(str(z["meta"][0]))
         if meta.get("format") != "afmibins-ib-checkpoint":
             raise ValueError(f"{path} is not an IB checkpoint")
-        if meta.get("version") not in (1, CHECKPOINT_VERSION):
-            raise ValueError(f"{path}: checkpoint version {meta.get('version')} (this build reads 1 and "
+        if meta.get("version") not in (1, 2, CHECKPOINT_VERSION):
+            raise ValueError(f"{path}: checkpoint version {meta.get('version')} (this build reads 1 to "
                              f"{CHECKPOINT_VERSION})")
         arrays = {name: np.array(z[name], dtype=np.float64) for name in _ARRAYS}
         for j, present in enumerate(meta.get("starts", [False, False])):
@@ -143,7 +145,8 @@ def restore(path, device="cuda:0"):
     # they are written back into the solver's (aliased) u, v
     u.copy_(t(a["u"]))
     v.copy_(t(a["v"]))
-    for j in range(2):
+    # the starts first: setting one unsets the last potential of its kind
+    for j in range(4):
         if f"start{j}" in a:
             ns.set_start(j, t(a[f"start{j}"]))
     ib = A.IBSolver(ns, t(a["X"]), t(a["R"]), t(a["k"]), t(a["ds"]))

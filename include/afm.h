@@ -288,13 +288,16 @@ int afm_ns_old(afm_ns* s, const double** u_old, const double** v_old);
 int afm_ns_last_residual(afm_ns* s, double* final_rms, double* threshold);
 /* The CG forms' start potentials (precision 1 and 3; no reference
  * counterpart: the reference's solve starts from its nested iteration).
- * Such a solver starts a step's projection from the potential of the
- * previous step of the same kind -- kind 0 the backward-Euler half step's,
- * 1 the Crank-Nicolson full step's -- once there is one.  get copies it to
- * phi (n x n on the device) and sets *valid (0: the next solve of that kind
- * starts from zero and phi is untouched); set copies phi in (NULL: start
- * from zero again).  A checkpoint of such a solver saves and restores both
- * to continue bit for bit. */
+ * Such a solver starts a step's projection from the potentials of the
+ * previous steps of the same kind -- the backward-Euler half step's or the
+ * Crank-Nicolson full step's: kind 0 / 1 is the next start of the half /
+ * full step (the linear extrapolation 2 phi_n - phi_{n-1} of the kind's
+ * last two potentials; phi_n alone after the first), kind 2 / 3 the kind's
+ * last potential phi_n.  get copies one to phi (n x n on the device) and
+ * sets *valid (0: unset, phi untouched); set copies phi in (NULL: unset).
+ * Setting kind 0 / 1 unsets kind 2 / 3 (a start alone has nothing to
+ * extrapolate from); set 2 / 3 after it.  A checkpoint of such a solver
+ * saves and restores all four to continue bit for bit. */
 int afm_ns_get_start(afm_ns* s, int kind, double* phi, int* valid);
 int afm_ns_set_start(afm_ns* s, int kind, const double* phi);
 int afm_ns_destroy(afm_ns* s);

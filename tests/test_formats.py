@@ -70,7 +70,13 @@ def test_checkpoint_round_trip_and_checks(tmp_path):
     s1 = rng.standard_normal((n, n))
     FM.save_checkpoint(p, **arrays, n=n, mu=1e-3, rho=1.0, dt=2e-5, tol=1e-10, precision=1, starts=(None, s1))
     meta, got = FM.load_checkpoint(p)
-    assert meta["starts"] == [False, True] and "start0" not in got and np.array_equal(got["start1"], s1)
+    assert meta["starts"] == [False, True, False, False] and "start0" not in got
+    assert np.array_equal(got["start1"], s1)
+    s3 = rng.standard_normal((n, n))
+    FM.save_checkpoint(p, **arrays, n=n, mu=1e-3, rho=1.0, dt=2e-5, tol=1e-10, precision=1,
+                       starts=(s1, s1, None, s3))
+    meta, got = FM.load_checkpoint(p)
+    assert meta["starts"] == [True, True, False, True] and np.array_equal(got["start3"], s3)
     with pytest.raises(ValueError):
         FM.save_checkpoint(p, **arrays, n=n, mu=1e-3, rho=1.0, dt=2e-5, tol=1e-10, starts=(s1[:3], None))
     with pytest.raises(ValueError):

@@ -1,6 +1,7 @@
 """An IB run saved to a checkpoint file and restored continues bit for bit
 as the uninterrupted run (the reference's FP64 iteration, and the mixed CG
-form, whose projections start from potentials the checkpoint carries)."""
+form, whose projections start from potentials the checkpoint carries: per
+step kind the next start and the last potential it is extrapolated from)."""
 import numpy as np
 import pytest
 import torch
@@ -27,7 +28,7 @@ def test_checkpoint_restart_continues_identically(tmp_path, precision):
         ib.step(3)
         path = str(tmp_path / "run.npz")
         FM.save_checkpoint(path, u, v, ib.markers(), R, k, ds, n=n, mu=mu, rho=rho, dt=dt, tol=tol, tol_mode=1,
-                           precision=precision, step=3, starts=(ns.get_start(0), ns.get_start(1)))
+                           precision=precision, step=3, starts=[ns.get_start(j) for j in range(4)])
         ib.step(2)
         ctx.sync()
         want = (u.cpu().numpy(), v.cpu().numpy(), ib.markers().cpu().numpy())

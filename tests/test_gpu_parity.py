@@ -389,6 +389,42 @@ def test_ib_step_is_run_to_run_reproducible(afm, dev):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision", [1, 3])
+def test_cg_forms_extrapolate_the_projection_start(afm, dev, precision):
+    """The CG forms start a step's projection from 2 phi_n - phi_{n-1}, the
+    last two potentials of the same step kind: the stored start equals that
+    combination of the stored last potentials (bit for bit), the first
+    projection of a kind starts from its potential alone, and setting a start
+    unsets the last potential it would extrapolate from."""
+    n = 128
+    ctx = afm.Context(n, 0)
+    u0, v0 = W.fourier_velocity(n, 4, seed=5, rms=0.2)
+    X, R, k, ds = W.membranes(4, 64, seed=3, center=(0.25, 0.75), axes=(0.05, 0.12))
+    u, v = T(u0, dev), T(v0, dev)
+    ns = afm.NSSolver(ctx, 1e-2, 1.0, 1e-4, 1e-10, u, v, precision=precision)
+    ib = afm.IBSolver(ns, T(X, dev), T(R, dev), T(k, dev), T(ds, dev))
+    try:
+        assert all(ns.get_start(j) is None for j in range(4))
+        ib.step(1)
+        last = [ns.get_start(2).clone(), ns.get_start(3).clone()]
+        for j in (0, 1):   # one potential per kind so far: the start is it
+            assert torch.equal(ns.get_start(j), last[j])
+        for _ in range(2):
+            ib.step(1)
+            for j in (0, 1):
+                now = ns.get_start(2 + j)
+                assert torch.equal(ns.get_start(j), 2 * now - last[j])
+                assert not torch.equal(now, last[j])
+                last[j] = now.clone()
+        # the projections still meet the tolerance from the extrapolated starts
+        res, thr = ns.last_residual()
+        assert res <= thr
+        ns.set_start(0, last[0])
+        assert ns.get_start(2) is None and ns.get_start(3) is not None
+    finally:
+        ib.close(); ns.close(); ctx.close()
+
+
 _REF_CASE = {}
 
 
