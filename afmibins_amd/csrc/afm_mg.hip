@@ -1135,6 +1135,7 @@ __global__ void k_copy_cvt(const T* __restrict__ src, double* __restrict__ dst, 
 // their own old value.
 struct CoarseLDS {
 	double X[5461], B[5461], TMP[4096];   // levels 0..6 stacked; TMP: one residual
+	const double* op3;                     // level 3's V-cycle as a matrix, or nullptr
 };
 
 constexpr int lvl_off(int l) { return ((1 << (2 * l)) - 1) / 3; }   // sum_{j<l} 4^j
@@ -1349,11 +1350,33 @@ __device__ inline void c_solve0(CoarseLDS& S)
 	csync<64>();
 }
 
+// X_3 = M B_3 with M the V-cycle of levels 3..0 (linear in B_3: smoothing
+// from zero, restriction, the coarsest solve's mean removal and prolongation
+// are), built from unit vectors by k_coarse_op3 with the same arithmetic;
+// only the order of the sums differs.  Wave 0: lane i forms row i, reading
+// column j of all rows as one contiguous 512-byte load.
+__device__ inline void c_apply_op3(CoarseLDS& S)
+{
+	const int i = (int) threadIdx.x;
+	const double* b = S.B + lvl_off(3);
+	double acc = 0.0;
+#pragma unroll 16
+	for (int j = 0; j < 64; ++j) acc = fma(S.op3[j * 64 + i], b[j], acc);
+	S.X[lvl_off(3) + i] = acc;
+	csync<64>();
+}
+
 // recurse(L, B_L) -> X_L (mg/solver.h:199-216).  L <= 4: wave 0 only;
 // L >= 5: every thread.
 template<int L>
 __device__ void c_recurse(CoarseLDS& S, int nu1, int nu2)
 {
+	if constexpr (L == 3) {
+		if (S.op3) {
+			c_apply_op3(S);
+			return;
+		}
+	}
 	if constexpr (L == 0) {
 		c_solve0(S);
 	} else {
@@ -1446,10 +1469,12 @@ __device__ void c_nested(CoarseLDS& S, int Lt, int nu1, int nu2)
 // mode 2: nested(Lt, in) -> out (the approximate solution itself).
 template<typename TI>
 __global__ __launch_bounds__(CTHREADS) void k_coarse(int mode, int Lt, const TI* __restrict__ in, TI* __restrict__ out,
-                                                     int nu1, int nu2, const int* __restrict__ done)
+                                                     int nu1, int nu2, const int* __restrict__ done,
+                                                     const double* __restrict__ op3)
 {
 	__shared__ CoarseLDS S;
 	if (done && *done) return;
+	if (threadIdx.x == 0) S.op3 = op3;
 	const int N = 1 << (2 * Lt), o = lvl_off(Lt);
 	for (int i = threadIdx.x; i < N; i += CTHREADS) S.B[o + i] = (double) in[i];
 	__syncthreads();
@@ -1460,6 +1485,19 @@ __global__ __launch_bounds__(CTHREADS) void k_coarse(int mode, int Lt, const TI*
 	} else {
 		for (int i = threadIdx.x; i < N; i += CTHREADS) out[i] = (TI) S.B[o + i];
 	}
+}
+
+// Column j of level 3's V-cycle matrix: recurse(3) on the unit vector e_j,
+// by the recursion c_apply_op3 stands in for.  One 64-thread block per column.
+__global__ __launch_bounds__(64) void k_coarse_op3(int nu1, int nu2, double* __restrict__ op3)
+{
+	__shared__ CoarseLDS S;
+	const int i = (int) threadIdx.x, j = (int) blockIdx.x;
+	if (i == 0) S.op3 = nullptr;
+	S.B[lvl_off(3) + i] = i == j ? 1.0 : 0.0;
+	csync<64>();
+	c_recurse<3>(S, nu1, nu2);
+	op3[j * 64 + i] = S.X[lvl_off(3) + i];
 }
 
 // ------------------------------------------------------------- host side --
@@ -1582,7 +1620,20 @@ template<> float* lvB<float>(LevelBuf& L) { return L.B32; }
 template<typename T>
 static int coarse_launch(Ctx* c, int mode, int Lt, const T* in, T* out, int nu1, int nu2, const int* done)
 {
-	hipLaunchKernelGGL((k_coarse<T>), dim3(1), dim3(CTHREADS), 0, c->stream, mode, Lt, in, out, nu1, nu2, done);
+	// the FP32-cycle forms (preconditioner and FP32 refinement) take levels
+	// 3..0 as one matrix product; the FP64 cycles keep the recursion, the
+	// reference's operation order
+	const double* op3 = nullptr;
+	if (std::is_same<T, float>::value && Lt >= 4) {
+		double*& m = c->op3[(nu1 - 1) * 2 + (nu2 - 1)];
+		if (!m) {
+			AFM_TRY(c->alloc((void**) &m, 64 * 64 * sizeof(double)));
+			hipLaunchKernelGGL(k_coarse_op3, dim3(64), dim3(64), 0, c->stream, nu1, nu2, m);
+			AFM_LAUNCH_CHECK();
+		}
+		op3 = m;
+	}
+	hipLaunchKernelGGL((k_coarse<T>), dim3(1), dim3(CTHREADS), 0, c->stream, mode, Lt, in, out, nu1, nu2, done, op3);
 	AFM_LAUNCH_CHECK();
 	return 0;
 }
