@@ -118,9 +118,9 @@ struct Ctx {
 	long mcap = 0;
 	int sbx = 0, sby = 0;              // spread bins per axis (4x4 base cells each)
 	int last_mg_cycles = 0;            // predicted batch size for the next solve
-	// the V-cycle of levels 3..0 as a 64 x 64 matrix, transposed (column-major
-	// rows: op3[j * 64 + i] = M[i][j]), one per (nu1, nu2), built on first use
-	// by the FP32-cycle forms' coarse kernel
+	// the V-cycle of levels 3..0 as a 64 x 64 matrix, stored by columns
+	// (op3[j * 64 + i] = M[i][j]), one per (nu1, nu2) at index 2 (nu1 - 1) +
+	// nu2 - 1, built by mg_setup for the FP32-cycle forms' coarse kernel
 	double* op3[4] = {};
 	int n_cu = 256;                    // compute units of the device (persistent grids)
 	// opt-in HIP-event timing of three finest-level launches of mg_solve:

@@ -1623,16 +1623,7 @@ static int coarse_launch(Ctx* c, int mode, int Lt, const T* in, T* out, int nu1,
 	// the FP32-cycle forms (preconditioner and FP32 refinement) take levels
 	// 3..0 as one matrix product; the FP64 cycles keep the recursion, the
 	// reference's operation order
-	const double* op3 = nullptr;
-	if (std::is_same<T, float>::value && Lt >= 4) {
-		double*& m = c->op3[(nu1 - 1) * 2 + (nu2 - 1)];
-		if (!m) {
-			AFM_TRY(c->alloc((void**) &m, 64 * 64 * sizeof(double)));
-			hipLaunchKernelGGL(k_coarse_op3, dim3(64), dim3(64), 0, c->stream, nu1, nu2, m);
-			AFM_LAUNCH_CHECK();
-		}
-		op3 = m;
-	}
+	const double* op3 = (std::is_same<T, float>::value && Lt >= 4) ? c->op3[(nu1 - 1) * 2 + (nu2 - 1)] : nullptr;
 	hipLaunchKernelGGL((k_coarse<T>), dim3(1), dim3(CTHREADS), 0, c->stream, mode, Lt, in, out, nu1, nu2, done, op3);
 	AFM_LAUNCH_CHECK();
 	return 0;
@@ -1822,6 +1813,14 @@ int mg_setup(Ctx* c)
 			AFM_TRY(c->alloc((void**) &L.Xa32, N * 4));
 		}
 	}
+	// the FP32-cycle forms' level-3 operators, one per (nu1, nu2), built here
+	// so that no solve allocates (solves may be captured in a graph)
+	if (c->levels >= 4)
+		for (int k = 0; k < 4; ++k) {
+			AFM_TRY(c->alloc((void**) &c->op3[k], 64 * 64 * sizeof(double)));
+			hipLaunchKernelGGL(k_coarse_op3, dim3(64), dim3(64), 0, c->stream, 1 + k / 2, 1 + k % 2, c->op3[k]);
+			AFM_LAUNCH_CHECK();
+		}
 	AFM_TRY(c->alloc((void**) &c->XSa, c->N * 8));
 	AFM_TRY(c->alloc((void**) &c->XSb, c->N * 8));
 	AFM_TRY(c->alloc((void**) &c->RES, c->N * 8));
