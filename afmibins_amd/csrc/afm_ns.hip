@@ -19,27 +19,30 @@
 namespace afm {
 
 // ---------------------------------------------------------------- advection
+// Cell indices are 32-bit (fields have fewer than 2^32 cells): the 64-bit
+// index arithmetic of the neighbour lookups was most of the kernel's work.
+// Each value is loaded once; every sum keeps the operand order of
+// uu(j) = (u[j] + u[xm j]) / 2 squared, ww(j) likewise in z, and
+// uw(j) = (u[zm j] + u[j]) / 2 * (w[xm j] + w[j]) / 2.
 __device__ inline void adv_at(const double* __restrict__ u, const double* __restrict__ w, int nx, int ny, unsigned n,
-                              size_t i, int x, int z, double& ua, double& wa)
+                              unsigned i, int x, int z, double& ua, double& wa)
 {
-	const size_t N = (size_t) nx * ny;
-	auto xm = [&](size_t j, int jx) { return jx == 0 ? j + (nx - 1) : j - 1; };
-	auto xp = [&](size_t j, int jx) { return jx == nx - 1 ? j - (nx - 1) : j + 1; };
-	auto zm = [&](size_t j, int jz) { return jz == 0 ? j + N - nx : j - nx; };
-	auto zp = [&](size_t j, int jz) { return jz == ny - 1 ? j - (N - nx) : j + nx; };
-	const int xP = x == nx - 1 ? 0 : x + 1, zP = z == ny - 1 ? 0 : z + 1;
-	const size_t ixp = xp(i, x), izp = zp(i, z);
-	auto uu = [&](size_t j, int jx) { double q = 0.5 * (u[j] + u[xm(j, jx)]); return q * q; };
-	auto ww = [&](size_t j, int jz) { double q = 0.5 * (w[j] + w[zm(j, jz)]); return q * q; };
-	auto uw = [&](size_t j, int jx, int jz) {
-		double q = 0.5 * (u[zm(j, jz)] + u[j]);
-		double r = 0.5 * (w[xm(j, jx)] + w[j]);
-		return q * r;
-	};
+	const unsigned N = (unsigned) nx * (unsigned) ny, unx = (unsigned) nx;
+	const unsigned ixm = x == 0 ? i + (unx - 1) : i - 1, ixp = x == nx - 1 ? i - (unx - 1) : i + 1;
+	const unsigned izm = z == 0 ? i + (N - unx) : i - unx, izp = z == ny - 1 ? i - (N - unx) : i + unx;
+	const unsigned ixp_zm = z == 0 ? ixp + (N - unx) : ixp - unx;   // zm(xp(i))
+	const unsigned izp_xm = x == 0 ? izp + (unx - 1) : izp - 1;     // xm(zp(i))
+	const double ui = u[i], uxm = u[ixm], uxp = u[ixp], uzm = u[izm], uzp = u[izp], uxpzm = u[ixp_zm];
+	const double wi = w[i], wxm = w[ixm], wxp = w[ixp], wzm = w[izm], wzp = w[izp], wzpxm = w[izp_xm];
+	auto sq = [](double q) { return q * q; };
+	const double uu_xp = sq(0.5 * (uxp + ui)), uu_i = sq(0.5 * (ui + uxm));
+	const double ww_zp = sq(0.5 * (wzp + wi)), ww_i = sq(0.5 * (wi + wzm));
+	const double uw_i = (0.5 * (uzm + ui)) * (0.5 * (wxm + wi));
+	const double uw_zp = (0.5 * (ui + uzp)) * (0.5 * (wzpxm + wzp));
+	const double uw_xp = (0.5 * (uxpzm + uxp)) * (0.5 * (wi + wxp));
 	const double dv = 0.0 - 0.0;   // alg/div.h:54 with zero top/bottom
-	const double uw_i = uw(i, x, z);
-	ua = n * ((uu(ixp, xP) - uu(i, x)) + dv + (uw(izp, x, zP) - uw_i));
-	wa = n * ((uw(ixp, xP, z) - uw_i) + dv + (ww(izp, zP) - ww(i, z)));
+	ua = n * ((uu_xp - uu_i) + dv + (uw_zp - uw_i));
+	wa = n * ((uw_xp - uw_i) + dv + (ww_zp - ww_i));
 }
 
 __global__ void k_advect(const double* __restrict__ u, const double* __restrict__ v, double* __restrict__ ua,
@@ -346,17 +349,17 @@ __global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* 
 		cg_ctl_start(a, 0, tol, max_iters);
 		cg_ctl_start(a, 1, tol, max_iters);
 	}
-	const size_t N = (size_t) nx * ny;
+	// 32-bit cell indices (fields have fewer than 2^32 cells)
+	const unsigned N = (unsigned) nx * (unsigned) ny, unx = (unsigned) nx;
 	const double* u = a.x[0];
 	const double* v = a.x[1];
 	double acc_u[2] = {0.0, 0.0}, acc_v[2] = {0.0, 0.0};
-	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
-		int x, z;
-	cell_xy(i, nx, x, z);
+	for (unsigned i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
+		const int x = (int) (i & (unx - 1)), z = (int) (i >> __builtin_ctz(unx));
 		double ua, va;
 		adv_at(u, v, nx, ny, n, i, x, z, ua, va);
-		const size_t ie = x == nx - 1 ? i - (nx - 1) : i + 1, iw = x == 0 ? i + (nx - 1) : i - 1;
-		const size_t in_ = z == ny - 1 ? (size_t) x : i + nx, is = z == 0 ? i + N - nx : i - nx;
+		const unsigned ie = x == nx - 1 ? i - (unx - 1) : i + 1, iw = x == 0 ? i + (unx - 1) : i - 1;
+		const unsigned in_ = z == ny - 1 ? (unsigned) x : i + unx, is = z == 0 ? i + (N - unx) : i - unx;
 		const bool we = x == nx - 1, ww = x == 0, wn = z == ny - 1, ws = z == 0;
 		const double u0 = u[i], v0 = v[i];
 		double rx, ry;
