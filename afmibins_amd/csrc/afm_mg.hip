@@ -1009,13 +1009,13 @@ __global__ __launch_bounds__(256) void k_update_resid(const double* __restrict__
 	// acc[1]: sum of (rhs - mean)^2 for a relative tolerance (with sub_mean:
 	// the same terms in the same order as a separate pass over rhs would add)
 	double acc[2] = {0.0, 0.0};
-	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
-		int gx, gy;
-	cell_xy(i, nx, gx, gy);
-		const size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1;
-		const size_t iw = gx == 0 ? i + (nx - 1) : i - 1;
-		const size_t in_ = gy == ny - 1 ? gx : i + nx;
-		const size_t is = gy == 0 ? i + N - nx : i - nx;
+	// 32-bit cell indices (fields have fewer than 2^32 cells)
+	for (unsigned i = blockIdx.x * 256 + threadIdx.x; i < (unsigned) N; i += gridDim.x * 256) {
+		const int gx = (int) (i & (unsigned) (nx - 1)), gy = (int) (i >> __builtin_ctz((unsigned) nx));
+		const unsigned ie = gx == nx - 1 ? i - (nx - 1) : i + 1;
+		const unsigned iw = gx == 0 ? i + (nx - 1) : i - 1;
+		const unsigned in_ = gy == ny - 1 ? (unsigned) gx : i + nx;
+		const unsigned is = gy == 0 ? (unsigned) (i + N - nx) : i - nx;
 		// xs == nullptr: the iterate is zero and e (below) is the whole start
 		double c = 0.0, Ev = 0.0, Wv = 0.0, Nv = 0.0, Sv = 0.0;
 		if (XS) { c = xs[i]; Ev = xs[ie]; Wv = xs[iw]; Nv = xs[in_]; Sv = xs[is]; }

@@ -127,10 +127,9 @@ struct CGArgs {
 };
 
 #define CG_NEIGHBOURS                                                                              \
-	int gx, gy;                                                                                    \
-	cell_xy(i, nx, gx, gy);                                                                        \
-	const size_t ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;    \
-	const size_t in_ = gy == ny - 1 ? (size_t) gx : i + nx, is = gy == 0 ? i + N - nx : i - nx;    \
+	const int gx = (int) (i & (unsigned) (nx - 1)), gy = (int) (i >> __builtin_ctz((unsigned) nx));    \
+	const unsigned ie = gx == nx - 1 ? i - (nx - 1) : i + 1, iw = gx == 0 ? i + (nx - 1) : i - 1;  \
+	const unsigned in_ = gy == ny - 1 ? (unsigned) gx : i + nx, is = gy == 0 ? i + N - nx : i - nx; \
 	const bool we = gx == nx - 1, ww = gx == 0, wn = gy == ny - 1, ws = gy == 0;
 
 // Start field f's control block (block 0 of the start launch; the stop test
@@ -157,7 +156,7 @@ __global__ __launch_bounds__(256) void k_cg_init(CGArgs a, int nx, int ny, doubl
 	const double* x = a.x[f];
 	const double* b = a.b[f];
 	double acc[2] = {0.0, 0.0};
-	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
+	for (unsigned i = blockIdx.x * 256 + threadIdx.x; i < (unsigned) N; i += gridDim.x * 256) {
 		CG_NEIGHBOURS
 		double Ax = stencil5<double, true>(off, dg, x[i], x[ie], x[iw], x[in_], x[is], we, ww, wn, ws);
 		double r = b[i] - Ax;
@@ -258,7 +257,7 @@ __global__ __launch_bounds__(256) void k_cg_p(CGArgs a, int nx, int ny, double o
 	const TR* r = (const TR*) a.r[f];
 	const bool first = it == 0;
 	double acc[1] = {0.0};
-	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
+	for (unsigned i = blockIdx.x * 256 + threadIdx.x; i < (unsigned) N; i += gridDim.x * 256) {
 		CG_NEIGHBOURS
 		double pc, pe, pw, pn, ps;
 		if (first) { pc = r[i]; pe = r[ie]; pw = r[iw]; pn = r[in_]; ps = r[is]; }   // p = r
@@ -291,7 +290,7 @@ __global__ __launch_bounds__(256) void k_cg_update(CGArgs a, int nx, int ny, dou
 	TR* r = (TR*) a.r[f];
 	const TP* p = (const TP*) a.p[f][pcur];
 	double acc[1] = {0.0};
-	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
+	for (unsigned i = blockIdx.x * 256 + threadIdx.x; i < (unsigned) N; i += gridDim.x * 256) {
 		CG_NEIGHBOURS
 		const double pc = (double) p[i];
 		double z = stencil5<double, true>(off, dg, pc, (double) p[ie], (double) p[iw], (double) p[in_], (double) p[is],
