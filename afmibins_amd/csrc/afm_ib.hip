@@ -23,6 +23,9 @@ constexpr int SB = 4;             // spread bin edge (window base cells)
 constexpr int SCAN_BLOCKS = 1024; // at most this many scan chunks
 constexpr int GCLS = 4;           // density classes of gather tiles (records reaching the tile:
                                   // >= 1024, >= 128, >= 1, none), launched densest first
+constexpr int GB = 8;        // gather block edge, bins
+constexpr int GCAP = 2048;   // records staged per block
+constexpr int GMIN = 64;     // fewer: short lists, not worth the copy
 
 __global__ void k_zero_int(int* __restrict__ a, long count)
 {
@@ -96,7 +99,8 @@ __global__ void k_ib_move_full(const double* __restrict__ uo, const double* __re
 __global__ void k_spread_prep(int comp_mask, const double* __restrict__ Fx, const double* __restrict__ Fy,
                               const double* __restrict__ ds, const double* __restrict__ X,
                               const double* __restrict__ R, const double* __restrict__ kk, long m, WinGeom g,
-                              int sbx, int nb1, MarkerRec* __restrict__ rec, int* __restrict__ counts)
+                              int sbx, int sby, int nb1, MarkerRec* __restrict__ rec, int* __restrict__ counts,
+                              int* __restrict__ tflag, int tpr, int ntile)
 {
 	const long t = blockIdx.x * (long) blockDim.x + threadIdx.x;
 	const long k = t >> 1;
@@ -117,6 +121,15 @@ __global__ void k_spread_prep(int comp_mask, const double* __restrict__ Fx, cons
 #pragma unroll
 		for (int d = 0; d < 4; ++d) r.dx[d] = fct[d];
 		atomicAdd(&counts[comp * nb1 + tile], 1);
+		// the gather tiles whose cells the window reaches: those of bins
+		// (bx, by), (bx + 1, by), (bx, by + 1), (bx + 1, by + 1), periodic
+		const int bx = ix0 / SB, by = iy0 / SB;
+		const int bx1 = bx + 1 == sbx ? 0 : bx + 1, by1 = by + 1 == sby ? 0 : by + 1;
+		int* tf = tflag + comp * ntile;
+		tf[bx / GB + tpr * (by / GB)] = 1;
+		tf[bx1 / GB + tpr * (by / GB)] = 1;
+		tf[bx / GB + tpr * (by1 / GB)] = 1;
+		tf[bx1 / GB + tpr * (by1 / GB)] = 1;
 	} else {
 #pragma unroll
 		for (int d = 0; d < 4; ++d) r.dy[d] = fct[d];
@@ -259,9 +272,6 @@ __global__ void k_spread_rank(int comp_mask, const int* __restrict__ offs, const
 // factors are loaded from global memory one merge step ahead of their use.
 // Blocks with fewer than GMIN or more than GCAP records, and grids of fewer
 // than 8 x 8 bins, merge from global memory instead (same order, same sums).
-constexpr int GB = 8;        // gather block edge, bins
-constexpr int GCAP = 2048;   // records staged per block
-constexpr int GMIN = 64;     // fewer: short lists, not worth the copy
 
 // a record's values in registers (named fields, no arrays: the factor
 // selection below must stay a choice among registers)
@@ -403,7 +413,7 @@ __global__ __launch_bounds__(256) void k_spread_gather(int comp_mask, const BinR
                                                        const int* __restrict__ offs, unsigned nx, int sbx, int sby,
                                                        int nb1, double* __restrict__ fx, double* __restrict__ fy,
                                                        int accumulate, const int* __restrict__ ccount,
-                                                       const int* __restrict__ order)
+                                                       const int* __restrict__ order, int* __restrict__ tflag)
 {
 	constexpr int NR = 2 * (GB + 1);   // srec ranges staged per block
 	__shared__ int lkey[GCAP], lidx[GCAP];
@@ -423,6 +433,26 @@ __global__ __launch_bounds__(256) void k_spread_gather(int comp_mask, const BinR
 		tile = order[(size_t) (comp * GCLS + cls) * nt + tile];
 	}
 	const int bx0 = GB * (tile % bpr), by0 = GB * (tile / bpr);
+	{
+		// a tile no record reaches (flag unset by the window kernel) stores
+		// zeros, or leaves the field as it is, without reading the bins
+		int* tf = tflag + (size_t) comp * gridDim.x + tile;
+		if (!*tf) {
+			if (!accumulate) {
+				const int q0 = tid >> 2, bxz = bx0 + (q0 & (GB - 1)), byz = by0 + q0 / GB;
+				if (bxz < sbx && byz < sby) {
+					double2* r2 = reinterpret_cast<double2*>((comp == 0 ? fx : fy) + (size_t) nx * (SB * byz + (tid & 3))
+					                                         + SB * bxz);
+					r2[0] = make_double2(0.0, 0.0);
+					r2[1] = make_double2(0.0, 0.0);
+				}
+			}
+			return;
+		}
+		// every thread has read the flag before it is cleared for the next call
+		__syncthreads();
+		if (tid == 0) *tf = 0;
+	}
 	const int* o = offs + comp * nb1;
 	const int ny = SB * sby;
 	// lane: cell row j of bin (tx, ty) of the block.  Its four bins' ranges
@@ -580,7 +610,11 @@ int ensure_markers(Ctx* c, long m)
 int spread_setup(Ctx* c)
 {
 	const size_t nt = (size_t) ((c->sbx + GB - 1) / GB) * ((c->sby + GB - 1) / GB);
-	return c->alloc((void**) &c->gorder, sizeof(int) * 2 * GCLS * (1 + nt));
+	AFM_TRY(c->alloc((void**) &c->gorder, sizeof(int) * 2 * GCLS * (1 + nt)));
+	AFM_TRY(c->alloc((void**) &c->tflag, sizeof(int) * 2 * nt));
+	// the flags start unset once; each gather clears the ones it read
+	AFM_HIP(hipMemsetAsync(c->tflag, 0, sizeof(int) * 2 * nt, c->stream));
+	return 0;
 }
 
 int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, const double* ds, const double* X,
@@ -605,7 +639,8 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 		AFM_TRY(ensure_markers(c, m));
 		dim3 g((m + 255) / 256, 2), g2((2 * m + 255) / 256, 2);
 		hipLaunchKernelGGL(k_spread_prep, g2, dim3(256), 0, c->stream, comp_mask, Fx, Fy, ds, X, R, k, m,
-		                   win_geom(c->n, c->W, c->D, c->nx, c->ny), c->sbx, nb1, c->rec, counts);
+		                   win_geom(c->n, c->W, c->D, c->nx, c->ny), c->sbx, c->sby, nb1, c->rec, counts, c->tflag,
+		                   (c->sbx + GB - 1) / GB, (int) gblocks);
 		AFM_LAUNCH_CHECK();
 		hipLaunchKernelGGL(k_spread_chunk, dim3(nchunk), dim3(256), 0, c->stream, counts, chunk, nb, items);
 		hipLaunchKernelGGL(k_spread_scan, dim3(nchunk), dim3(256), 0, c->stream, counts, chunk, offs, cursor, nb, items,
@@ -630,7 +665,7 @@ int launch_spread(Ctx* c, int comp_mask, const double* Fx, const double* Fy, con
 	}
 	hipLaunchKernelGGL(k_spread_gather, dim3(gblocks, 2), dim3(256), 0, c->stream, comp_mask, c->srec, offs,
 	                   c->nx, c->sbx, c->sby, nb1, fx, fy, accumulate, ordered ? c->gorder : nullptr,
-	                   ordered ? c->gorder + 2 * GCLS : nullptr);
+	                   ordered ? c->gorder + 2 * GCLS : nullptr, c->tflag);
 	AFM_LAUNCH_CHECK();
 	return 0;
 }

@@ -113,6 +113,8 @@ struct Ctx {
 	MarkerRec* rec = nullptr;          // [2 * mcap]
 	int* bins = nullptr;               // counts | offsets | cursors | chunk sums, 2 comps
 	int* gorder = nullptr;             // gather order: class counts | tile lists, 2 comps
+	int* tflag = nullptr;              // per gather tile and comp: a record reaches it (set by the
+	                                   // window kernel, cleared by the gather that reads it)
 	int* bin_list = nullptr;           // [2 * mcap]
 	BinRec* srec = nullptr;            // [2 * mcap], records in bin order
 	long mcap = 0;
