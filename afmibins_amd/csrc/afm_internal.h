@@ -134,7 +134,7 @@ struct Ctx {
 	hipEvent_t kt_ev[KT_KINDS][2 * KT_SLOTS] = {};
 	double kt_ms[KT_KINDS] = {};
 	long kt_launches[KT_KINDS] = {};
-	int last_cg_iters = 0;
+	int last_cg_iters = -1;   // the last Helmholtz CG's iterations (-1: none yet)
 	std::vector<void*> owned;
 	~Ctx();
 	int alloc(void** p, size_t bytes);

@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_sum_fin(unsigned nblk, SolveCtl* ctl, c
 // r = b - A u with u the initial guess (ns/solver.h:105-135 then
 // alg/cg.h:38-39), partial sums of r.r and b.b per field.  With STORE_B the
 // right-hand sides are stored to bu, bv instead and only b.b is summed (the
-// Neumann start, k_cg_start_fused, forms x0 and r; the r.r partials are
+// polynomial start, k_cg_start_fused, forms x0 and r; the r.r partials are
 // written as zeros here).
 template<bool STORE_B>
 __global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* __restrict__ fx,
@@ -395,30 +395,47 @@ __global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* 
 }
 
 // The CG forms' start for the velocity solves (no reference counterpart: the
-// reference starts from the current velocity).  With A = I - sL and c = A b,
-// x0 = b + (I - A) b + (I - A)^2 b = 3b - 3c + A c, the first three terms of
-// the Neumann series of A^-1, so r0 = b - A x0 = (I - A)^3 b: on the step's
-// smooth right-hand sides a residual 4 orders below the one of the current
-// velocity (2-3 CG iterations instead of 4-5 at 2048^2).  b is stored by the
-// right-hand-side launch (which also sums b.b); one pass forms c, x0 and r
-// from it and sums r.r.
-// Three 5-point passes over b, c and x0 would read 88 B per cell and field
-// and write 20-24 B; this kernel reads b and writes x0 and r (20-24 B).
-// A block holds a tile of 128 columns x NW * RPW rows in registers: lane l
-// of wave w holds columns 2l, 2l + 1 of rows w * RPW .. w * RPW + RPW - 1.
-// c = A b is exact on the tile shrunk by one cell on every side, x0 by two
-// and r by three, so the tile writes its 122 x (NW * RPW - 6) inner cells;
-// tiles overlap by 6 cells in each direction.  East and west neighbours come
-// from the adjacent lane (DPP), north and south from the strip itself or,
-// for a wave's first and last rows, from the adjacent wave's boundary rows
-// exchanged through LDS.  Every value is formed from the reference's CSR row
-// of its own cell (stencil5 with that cell's wrap flags), so a cell's c and
-// x0 are the same bits in every tile that forms them.
-// Tile columns start at 122 tx - 4 (even: two-column loads stay aligned), so
-// tile 0 writes column n - 1 (as column -1) and the last tile stops at n - 2.
-constexpr int ST_OX = 122;   // columns written per tile
+// reference starts from the current velocity).  A = I - sL has its spectrum
+// in [1, lmax], lmax = 1 + 8|s|n^2 (the periodic 5-point Laplacian's
+// eigenvalues lie in [-8n^2, 0]).  The start is a fixed polynomial in A
+// applied to b, evaluated in six applications of A:
+//   two Neumann steps from x = b: r1 = b - A b, x2 = b + r1,
+//   r2 = r1 - A r1 = (I - A)^2 b;
+//   four Chebyshev steps on [1, lmax] from (x2, r2) (Saad, Iterative Methods
+//   for Sparse Linear Systems, 2nd ed., Algorithm 12.1): d0 = r2 / theta,
+//   x += d_k, r -= A d_k, d_{k+1} = ca_k d_k + cb_k r
+// so that r0 = (I - A)^2 C4(A) b with |C4| <= 1 / T4((lmax + 1) / (lmax - 1))
+// on the spectrum.  The Neumann factors remove the smooth part of the step's
+// right-hand side (eigenvalues near 1), the Chebyshev factor the rough part
+// the spread forces bring in.  At 2048^2 (8|s|n^2 = 0.34) the start's residual
+// is about 1e-11 |b|, below the solves' tolerance: the CG takes no iteration
+// (2-3 after the three-term Neumann start x0 = 3b - 3Ab + A^2 b, 4-5 from the
+// current velocity).  Used only while 8|s|n^2 < 0.5 (ns_step).
+// b is stored by the right-hand-side launch (which also sums b.b); one pass
+// forms x0 and r from it and sums r.r.  Six 5-point passes would read about
+// 200 B per cell and field; this kernel reads b and writes x0 and r
+// (20-24 B).
+// A block holds a tile of 128 columns x NW * RPW rows in registers (96 rows:
+// 12 waves of 8): lane l of wave w holds columns 2l, 2l + 1 of rows
+// w * RPW .. w * RPW + RPW - 1.  The k-th application's result is exact on
+// the tile shrunk by k cells on every side, so the tile writes its
+// 116 x (NW * RPW - 12) inner cells; tiles overlap by 12 cells in each
+// direction.  East and west neighbours come from the adjacent lane (DPP),
+// north and south from the strip itself or, for a wave's first and last
+// rows, from the adjacent wave's boundary rows exchanged through LDS.  Every
+// cell takes the same operations (A v = dg v + off ((E + W) + (N + S)), fused
+// multiply-adds, no wrap-dependent order), so a cell's values are the same
+// bits in every tile that forms them.  164 VGPRs: three waves per SIMD, one
+// block per CU.  (Measured at 2048^2, both fields: 8-wave blocks 56.8 us,
+// 6-wave 67.0 us, 12-wave 54.0 us; with the reference's CSR-order stencil
+// and its wrap flags instead of the fused form, 75.9 us.)
+// Tile columns start at 116 tx - 6 (even: two-column loads stay aligned):
+// tile 0 writes columns 0 .. 115, the last tile stops at n - 1.
+constexpr int ST_HALO = 6;               // applications of A per start
+constexpr int ST_CHEB = 4;               // of which Chebyshev steps
+constexpr int ST_OX = 128 - 2 * ST_HALO; // columns written per tile
 #ifndef AFM_ST_NW
-#define AFM_ST_NW 8
+#define AFM_ST_NW 12
 #endif
 #ifndef AFM_ST_RPW
 #define AFM_ST_RPW 8
@@ -427,8 +444,18 @@ constexpr int ST_NW = AFM_ST_NW, ST_RPW = AFM_ST_RPW;   // waves per block, rows
 
 template<int NW, int RPW>
 struct StartTile {
-	static constexpr int LY = NW * RPW;   // rows held
-	static constexpr int OY = LY - 6;     // rows written
+	static constexpr int LY = NW * RPW;            // rows held
+	static constexpr int OY = LY - 2 * ST_HALO;    // rows written
+};
+
+// The Chebyshev steps' coefficients for the spectrum [1, lmax] (host side,
+// start_poly): theta = (lmax + 1) / 2, delta = (lmax - 1) / 2,
+// rho_0 = delta / theta, rho_{k+1} = delta / (2 theta - delta rho_k),
+// ca_k = rho_{k+1} rho_k, cb_k = 2 / (2 theta - delta rho_k) (the algorithm's
+// 2 rho_{k+1} / delta, finite at delta = 0).
+struct StartPoly {
+	double inv_theta;
+	double ca[ST_CHEB - 1], cb[ST_CHEB - 1];
 };
 
 // xb[buf][wave + 1][first / last row][lane]: slots 0 and NW + 1 are never
@@ -443,27 +470,49 @@ __device__ inline void st_publish(StartXch<NW>& xb, int buf, const double (&v)[R
 	xb[buf][w + 1][1][lane] = make_double2(v[RPW - 1][0], v[RPW - 1][1]);
 }
 
-// A v at (row i, column o) of the lane's strip: E / W from the pair or the
-// adjacent lane, S / N from the strip or the adjacent waves' rows up / dn
-template<int RPW, bool EDGE>
-__device__ inline double st_apply(const double (&v)[RPW][2], int i, int o, double2 up, double2 dn, int px, int py,
-                                  int msk, double off, double dg)
+// The sum of v's four neighbours at (row i, column o) of the lane's strip:
+// E / W from the pair or the adjacent lane, S / N from the strip or the
+// adjacent waves' rows up / dn.  (E + W) + (N + S) in every cell: the start
+// needs no parity with the reference's CSR rows (the CG's stop test decides
+// what it returns), only the same operations wherever tiles overlap.
+template<int RPW>
+__device__ inline double st_nsum(const double (&v)[RPW][2], int i, int o, double2 up, double2 dn)
 {
 	double E, W;
 	if (o == 0) { W = lane_shift<DPP_WAVE_SHR1>(v[i][1]); E = v[i][1]; }
 	else { W = v[i][0]; E = lane_shift<DPP_WAVE_SHL1>(v[i][0]); }
 	const double S = i > 0 ? v[i - 1][o] : (o ? up.y : up.x);
 	const double N = i < RPW - 1 ? v[i + 1][o] : (o ? dn.y : dn.x);
-	if (EDGE) return stencil5<double, true>(off, dg, v[i][o], E, W, N, S, px == msk, px == 0, py == msk, py == 0);
-	return stencil5<double, true>(off, dg, v[i][o], E, W, N, S, false, false, false, false);
+	return (E + W) + (N + S);
+}
+
+// y - A v with A v = dg v + off s, s the neighbour sum of v
+__device__ inline double st_sub_apply(double y, double v, double s, double off, double dg)
+{
+	return __builtin_fma(-off, s, __builtin_fma(-dg, v, y));
+}
+
+// Marks v as formed here (an empty asm statement that takes and returns it
+// in registers): without it the compiler sinks the x += d additions to the
+// stores and holds every step's d in registers until then
+__device__ inline void st_pin(double& v) { __asm__ volatile("" : "+v"(v)); }
+
+// Publishes v's first and last rows and waits for the block (one LDS round;
+// the two buffers alternate, so a wave that runs ahead writes the other one)
+template<int NW, int RPW>
+__device__ inline void st_post(StartXch<NW>& xb, int buf, const double (&v)[RPW][2], int w, int lane)
+{
+	st_publish<NW, RPW>(xb, buf, v, w, lane);
+	__syncthreads();
 }
 
 // One tile: columns c0 .. c0 + 127 and rows r0 .. r0 + LY - 1 (unwrapped;
-// EDGE when some of them lie outside 1 .. n - 2, so that they wrap or carry
-// a wrap flag).  Adds the written cells' r^2 to acc.
+// EDGE when some of them lie outside 0 .. n - 1, so that their loads and
+// stores wrap).  Adds the written cells' r^2 to acc.
 template<typename TR, int NW, int RPW, bool EDGE>
 __device__ inline void st_tile(StartXch<NW>& xb, int& buf, const double* __restrict__ b, double* __restrict__ x,
-                               TR* __restrict__ r, int n, int c0, int r0, double off, double dg, double& acc)
+                               TR* __restrict__ r, int n, int c0, int r0, double off, double dg, const StartPoly& P,
+                               double& acc)
 {
 	constexpr int LY = StartTile<NW, RPW>::LY;
 	// the wave index (and with it the rows) is the same in every lane: held
@@ -474,65 +523,92 @@ __device__ inline void st_tile(StartXch<NW>& xb, int& buf, const double* __restr
 	int gy[RPW];
 #pragma unroll
 	for (int i = 0; i < RPW; ++i) gy[i] = EDGE ? ((r0 + RPW * w + i) & msk) : r0 + RPW * w + i;
-	double bv[RPW][2], cv[RPW][2], xv[RPW][2];
+	double xv[RPW][2], rv[RPW][2], dv[RPW][2];
+	double2 up, dn;
 #pragma unroll
 	for (int i = 0; i < RPW; ++i) {
 		const double2 t = *(const double2*) (b + (size_t) n * gy[i] + gc);
-		bv[i][0] = t.x; bv[i][1] = t.y;
+		xv[i][0] = t.x; xv[i][1] = t.y;
 	}
-	// c = A b
-	st_publish<NW, RPW>(xb, buf, bv, w, lane);
-	__syncthreads();
-	double2 up = xb[buf][w][1][lane], dn = xb[buf][w + 2][0][lane];
-#pragma unroll
-	for (int i = 0; i < RPW; ++i)
-#pragma unroll
-		for (int o = 0; o < 2; ++o) cv[i][o] = st_apply<RPW, EDGE>(bv, i, o, up, dn, gc + o, gy[i], msk, off, dg);
-	// x0 = 3b - 3c + A c
-	buf ^= 1;
-	st_publish<NW, RPW>(xb, buf, cv, w, lane);
-	__syncthreads();
+	// r1 = b - A b (x = b)
+	st_post<NW, RPW>(xb, buf, xv, w, lane);
 	up = xb[buf][w][1][lane]; dn = xb[buf][w + 2][0][lane];
+	buf ^= 1;
 #pragma unroll
 	for (int i = 0; i < RPW; ++i)
 #pragma unroll
 		for (int o = 0; o < 2; ++o)
-			xv[i][o] = 3.0 * bv[i][o] - 3.0 * cv[i][o] + st_apply<RPW, EDGE>(cv, i, o, up, dn, gc + o, gy[i], msk, off, dg);
-	// r = b - A x0, the written cells stored
-	buf ^= 1;
-	st_publish<NW, RPW>(xb, buf, xv, w, lane);
-	__syncthreads();
+			rv[i][o] = st_sub_apply(xv[i][o], xv[i][o], st_nsum<RPW>(xv, i, o, up, dn), off, dg);
+	// x2 = b + r1, r2 = r1 - A r1
+	st_post<NW, RPW>(xb, buf, rv, w, lane);
 	up = xb[buf][w][1][lane]; dn = xb[buf][w + 2][0][lane];
 	buf ^= 1;
+#pragma unroll
+	for (int i = 0; i < RPW; ++i)
+#pragma unroll
+		for (int o = 0; o < 2; ++o)
+			dv[i][o] = st_sub_apply(rv[i][o], rv[i][o], st_nsum<RPW>(rv, i, o, up, dn), off, dg);
+#pragma unroll
+	for (int i = 0; i < RPW; ++i)
+#pragma unroll
+		for (int o = 0; o < 2; ++o) {
+			xv[i][o] += rv[i][o];
+			st_pin(xv[i][o]);
+			rv[i][o] = dv[i][o];
+		}
+	// the Chebyshev steps: d_k, then x += d_k, r -= A d_k.  The strip's first
+	// and last rows of d_k are formed before the LDS round, the others while
+	// its reads are in flight.
+#pragma unroll
+	for (int k = 0; k < ST_CHEB; ++k) {
+		const double ca = k ? P.ca[k - 1] : 0.0, cb = k ? P.cb[k - 1] : P.inv_theta;
+#pragma unroll
+		for (int i = 0; i < RPW; i += RPW - 1)
+#pragma unroll
+			for (int o = 0; o < 2; ++o) dv[i][o] = k ? __builtin_fma(ca, dv[i][o], cb * rv[i][o]) : cb * rv[i][o];
+		st_post<NW, RPW>(xb, buf, dv, w, lane);
+		up = xb[buf][w][1][lane]; dn = xb[buf][w + 2][0][lane];
+		buf ^= 1;
+#pragma unroll
+		for (int i = 1; i < RPW - 1; ++i)
+#pragma unroll
+			for (int o = 0; o < 2; ++o) dv[i][o] = k ? __builtin_fma(ca, dv[i][o], cb * rv[i][o]) : cb * rv[i][o];
+#pragma unroll
+		for (int i = 0; i < RPW; ++i)
+#pragma unroll
+			for (int o = 0; o < 2; ++o) {
+				rv[i][o] = st_sub_apply(rv[i][o], dv[i][o], st_nsum<RPW>(dv, i, o, up, dn), off, dg);
+				xv[i][o] += dv[i][o];
+				st_pin(xv[i][o]);
+			}
+	}
+	// the written cells stored
 	bool colok[2];
 #pragma unroll
 	for (int o = 0; o < 2; ++o) {
 		const int lc = 2 * lane + o;
-		colok[o] = lc >= 3 && lc < 128 - 3 && (!EDGE || cu + o <= n - 2);
+		colok[o] = lc >= ST_HALO && lc < 128 - ST_HALO && (!EDGE || cu + o <= n - 1);
 	}
 #pragma unroll
 	for (int i = 0; i < RPW; ++i) {
 		const int ly = RPW * w + i;
-		if (!(ly >= 3 && ly < LY - 3 && (!EDGE || r0 + ly < n))) continue;
-		double ri[2];
-#pragma unroll
-		for (int o = 0; o < 2; ++o) ri[o] = bv[i][o] - st_apply<RPW, EDGE>(xv, i, o, up, dn, gc + o, gy[i], msk, off, dg);
+		if (!(ly >= ST_HALO && ly < LY - ST_HALO && (!EDGE || r0 + ly < n))) continue;
 		const size_t row = (size_t) n * gy[i];
 		if (colok[0] && colok[1]) {
 			*(double2*) (x + row + gc) = make_double2(xv[i][0], xv[i][1]);
-			if (sizeof(TR) == 4) *(float2*) (r + row + gc) = make_float2((float) ri[0], (float) ri[1]);
-			else *(double2*) (r + row + gc) = make_double2(ri[0], ri[1]);
+			if (sizeof(TR) == 4) *(float2*) (r + row + gc) = make_float2((float) rv[i][0], (float) rv[i][1]);
+			else *(double2*) (r + row + gc) = make_double2(rv[i][0], rv[i][1]);
 		} else {
 #pragma unroll
 			for (int o = 0; o < 2; ++o)
 				if (colok[o]) {
 					x[row + gc + o] = xv[i][o];
-					r[row + gc + o] = (TR) ri[o];
+					r[row + gc + o] = (TR) rv[i][o];
 				}
 		}
 #pragma unroll
 		for (int o = 0; o < 2; ++o)
-			if (colok[o]) acc += ri[o] * ri[o];
+			if (colok[o]) acc += rv[i][o] * rv[i][o];
 	}
 }
 
@@ -542,7 +618,7 @@ __device__ inline void st_tile(StartXch<NW>& xb, int& buf, const double* __restr
 template<typename TR, int NW, int RPW>
 __global__ __launch_bounds__(64 * NW) void k_cg_start_fused(CGArgs a, const double* __restrict__ b0,
                                                             const double* __restrict__ b1, int n, double off,
-                                                            double dg, int tiles_x, int ntiles)
+                                                            double dg, StartPoly P, int tiles_x, int ntiles)
 {
 	using ST = StartTile<NW, RPW>;
 	__shared__ StartXch<NW> xb;
@@ -555,10 +631,10 @@ __global__ __launch_bounds__(64 * NW) void k_cg_start_fused(CGArgs a, const doub
 	double acc[1] = {0.0};
 	for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
 		const int tx = t % tiles_x, ty = t / tiles_x;
-		const int c0 = ST_OX * tx - 4, r0 = ST::OY * ty - 3;
-		const bool edge = c0 < 1 || c0 + 127 > n - 2 || r0 < 1 || r0 + ST::LY - 1 > n - 2;
-		if (edge) st_tile<TR, NW, RPW, true>(xb, buf, b, x, r, n, c0, r0, off, dg, acc[0]);
-		else st_tile<TR, NW, RPW, false>(xb, buf, b, x, r, n, c0, r0, off, dg, acc[0]);
+		const int c0 = ST_OX * tx - ST_HALO, r0 = ST::OY * ty - ST_HALO;
+		const bool edge = c0 < 0 || c0 + 127 > n - 1 || r0 < 0 || r0 + ST::LY - 1 > n - 1;
+		if (edge) st_tile<TR, NW, RPW, true>(xb, buf, b, x, r, n, c0, r0, off, dg, P, acc[0]);
+		else st_tile<TR, NW, RPW, false>(xb, buf, b, x, r, n, c0, r0, off, dg, P, acc[0]);
 	}
 	reduce_partial<1>(acc, a.pu[f]);
 }
@@ -653,8 +729,25 @@ struct AdvectInit {
 	double *uo, *vo;
 	double dt, rho, hoff, hdg;
 	int be;
-	int neumann;      // the CG forms' start x0 = 3b - 3Ab + A^2 b (k_cg_start_fused)
+	int neumann;      // the CG forms' polynomial start (k_cg_start_fused)
 };
+
+// The start's Chebyshev coefficients (StartPoly) for the spectrum of
+// A = I + (off / n^2) L, [1, 1 - 8 off] (off < 0; [1 - 8 off, 1] for off > 0)
+static StartPoly start_poly(double off)
+{
+	StartPoly P{};
+	const double theta = 1.0 - 4.0 * off, delta = 4.0 * std::fabs(off);
+	P.inv_theta = 1.0 / theta;
+	double rho = delta / theta;
+	for (int k = 0; k < ST_CHEB - 1; ++k) {
+		const double den = 2.0 * theta - delta * rho, rn = delta / den;
+		P.ca[k] = rn * rho;
+		P.cb[k] = 2.0 / den;
+		rho = rn;
+	}
+	return P;
+}
 
 int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* const b[2], double tol, int tol_mode,
            int max_iters, int iters[2], double res[2], const AdvectInit* adv = nullptr, bool p32 = false)
@@ -674,7 +767,7 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 	}
 	a.tol_mode = tol_mode;
 	const unsigned rb = rblocks(c->N);
-	// FP32 r only where the Neumann start writes it (k_cg_start_fused)
+	// FP32 r only where the polynomial start writes it (k_cg_start_fused)
 	const bool r32 = p32 && adv && adv->neumann;
 	if (adv && adv->neumann) {
 		// b goes to the first p buffers (the first p update reads r and
@@ -686,12 +779,13 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 		const int n = (int) c->nx;
 		const int tiles_x = (n + ST_OX - 1) / ST_OX, ntiles = tiles_x * ((n + ST::OY - 1) / ST::OY);
 		const unsigned sb = (unsigned) ntiles < rb ? (unsigned) ntiles : rb;
+		const StartPoly P = start_poly(off);
 		if (r32)
 			hipLaunchKernelGGL((k_cg_start_fused<float, ST_NW, ST_RPW>), dim3(sb, 2), dim3(64 * ST_NW), 0, c->stream,
-			                   a, w.p[0][0], w.p[1][0], n, off, dg, tiles_x, ntiles);
+			                   a, w.p[0][0], w.p[1][0], n, off, dg, P, tiles_x, ntiles);
 		else
 			hipLaunchKernelGGL((k_cg_start_fused<double, ST_NW, ST_RPW>), dim3(sb, 2), dim3(64 * ST_NW), 0, c->stream,
-			                   a, w.p[0][0], w.p[1][0], n, off, dg, tiles_x, ntiles);
+			                   a, w.p[0][0], w.p[1][0], n, off, dg, P, tiles_x, ntiles);
 	} else if (adv)
 		hipLaunchKernelGGL(k_advect_cg_init<false>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy, adv->uo,
 		                   adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff, adv->hdg,
@@ -702,7 +796,9 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 	hipLaunchKernelGGL(k_cg_init_fin, dim3(2), dim3(256), 0, c->stream, a, rb);
 	AFM_LAUNCH_CHECK();
 	int enq = 0;
-	int batch = c->last_cg_iters > 0 ? c->last_cg_iters : 4;
+	// as many iterations as the previous solve took (none when its start met
+	// the tolerance), 4 for a context's first solve
+	int batch = c->last_cg_iters >= 0 ? c->last_cg_iters : 4;
 	for (;;) {
 		if (batch > max_iters - enq) batch = max_iters - enq;
 		// two launches per iteration: each folds the finaliser of the one
@@ -728,8 +824,10 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 			AFM_LAUNCH_CHECK();
 			++enq;
 		}
-		hipLaunchKernelGGL(k_cg_update_fin, dim3(2), dim3(256), 0, c->stream, a, enq, rb);
-		AFM_LAUNCH_CHECK();
+		if (batch > 0) {
+			hipLaunchKernelGGL(k_cg_update_fin, dim3(2), dim3(256), 0, c->stream, a, enq, rb);
+			AFM_LAUNCH_CHECK();
+		}
 		AFM_HIP(hipMemcpyAsync(&c->hctl[1], &c->ctl[1], 2 * sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
 		AFM_HIP(hipStreamSynchronize(c->stream));
 		bool all = c->hctl[1].done && c->hctl[2].done;
@@ -839,10 +937,10 @@ int ns_step(NS* s, const double* fx, const double* fy)
 	double hoff, hdg;
 	helm_coef(c, sm, hoff, hdg);
 	// advection and the step's right-hand side are formed inside the CG start
-	// The Neumann start x0 = 3b - 3Ab + A^2 b leaves r0 = (I - A)^3 b; A = I - ss L
-	// has its spectrum in [1, 1 + 8|ss|n^2], so the start only helps while
-	// 8|ss|n^2 is well below 1.  Beyond that the CG forms start from the
-	// current velocity, as the FP64 form always does.
+	// The polynomial start leaves r0 = (I - A)^2 C4(A) b; A = I - ss L has
+	// its spectrum in [1, 1 + 8|ss|n^2], where the Neumann factor (I - A)^2
+	// only damps while 8|ss|n^2 is well below 1.  Beyond that the CG forms
+	// start from the current velocity, as the FP64 form always does.
 	const bool neumann = (s->precision == 1 || s->precision == 3) && 8.0 * std::fabs(ss) * c->n * c->n < 0.5;
 	const AdvectInit adv{fx, fy, s->uo, s->vo, s->dt, s->rho, hoff, hdg, s->half ? 0 : 1, neumann ? 1 : 0};
 	double* xs[2] = {s->u, s->v};

@@ -14,7 +14,7 @@
 * Stress (4096^2, 262,144 clustered markers): spreading and interpolation
   bit-exact to the oracle and run-to-run identical; the mixed-form adjoint
   over 50 steps against central differences.
-* The CG forms' Neumann start off when 8|s|n^2 is not small; the IB step
+* The CG forms' polynomial start off when 8|s|n^2 is not small; the IB step
   refuses a half-stepped solver; get_p after a gradient run.
 
 Each test prints its measured errors."""
@@ -292,9 +292,10 @@ def test_stress_mixed_adjoint(afm, dev):
 
 # ----------------------------------------------------- host-side checks
 def test_cg_forms_keep_the_reference_start_when_the_operator_is_stiff(afm, oracle, dev):
-    """With 8|s|n^2 >= 0.5 (here s n^2 = 2.6) the Neumann start would grow
-    the residual as (8|s|n^2)^3: the CG forms then start from the current
-    velocity, as the FP64 form does, and take the same CG iterations."""
+    """With 8|s|n^2 >= 0.5 (here s n^2 = 2.6) the polynomial start's Neumann
+    factor (I - A)^2 would grow the residual as (8|s|n^2)^2: the CG forms
+    then start from the current velocity, as the FP64 form does, and take
+    the same CG iterations."""
     n = 256
     u0, v0 = W.fourier_velocity(n, 4, seed=9)
     rng = np.random.default_rng(9)

@@ -316,11 +316,13 @@ def test_ns_steps_match_oracle(afm, oracle, dev, n):
 
 
 @pytest.mark.parametrize("n", [256, 1024])
-def test_cg_forms_neumann_start(afm, oracle, dev, n):
-    """The CG forms start the velocity solves from x0 = 3b - 3Ab + A^2 b: the
-    same steps as the FP64 form (which starts from the current velocity, as
-    the reference does) within the solves' tolerance, with no more CG
-    iterations, and the oracle's steps within the same bound."""
+def test_cg_forms_polynomial_start(afm, oracle, dev, n):
+    """The CG forms start the velocity solves from a fixed polynomial in A
+    applied to b (two Neumann steps, then four Chebyshev steps on A's
+    spectrum [1, 1 + 8|s|n^2]): the same steps as the FP64 form (which starts
+    from the current velocity, as the reference does) within the solves'
+    tolerance, with fewer CG iterations, and the oracle's steps within the
+    same bound."""
     u0, v0 = W.fourier_velocity(n, 8, seed=n + 1)
     rng = np.random.default_rng(n)
     fx, fy = rng.standard_normal((n, n)), rng.standard_normal((n, n))
