@@ -124,6 +124,11 @@ struct Ctx {
 	// (op3[j * 64 + i] = M[i][j]), one per (nu1, nu2) at index 2 (nu1 - 1) +
 	// nu2 - 1, built by mg_setup for the FP32-cycle forms' coarse kernel
 	double* op3[4] = {};
+	// level 6's V(2, 2) cycle (levels 6..0) as a 4096 x 4096 matrix, stored by
+	// rows in FP16, row i scaled by 1 / op6_inv[i], built by mg_setup for the
+	// FP32-cycle forms (k_coarse_mv6)
+	_Float16* op6 = nullptr;
+	float* op6_inv = nullptr;
 	int n_cu = 256;                    // compute units of the device (persistent grids)
 	// opt-in HIP-event timing of three finest-level launches of mg_solve:
 	// kind 0 the outer update (forms 0, 2: fused with the last post-smoothing;

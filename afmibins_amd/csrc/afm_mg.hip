@@ -1500,6 +1500,98 @@ __global__ __launch_bounds__(64) void k_coarse_op3(int nu1, int nu2, double* __r
 	op3[j * 64 + i] = S.X[lvl_off(3) + i];
 }
 
+// Level 6 of the FP32 cycles (V(2, 2)) as one 4096 x 4096 matrix: the
+// recursion of levels 6..0 (k_coarse, mode 0) is a fixed linear map of the
+// level's right-hand side, the same for every grid of 64^2 cells or more
+// (the level's spacing is fixed), so it is formed once per context and
+// applied as a matrix-vector product by the whole chip instead of by one
+// workgroup.  Column j: recurse(6) on e_j, with level 3's matrix as the
+// cycles use it (one block per column, FP32, by rows: m[i * 4096 + j] =
+// M[i][j]); then each row is stored in FP16 scaled by a power of two that
+// puts its largest entry in [2^14, 2^15) (33.5 MB: half the bytes per
+// product, and half the Infinity Cache the finest level's vectors share
+// with it).  Rounding the entries to 11 bits perturbs the coarse correction
+// by about 5e-4 relative, far below what the V-cycle leaves (its
+// convergence factor is about 0.03).
+constexpr int C6N = 1 << (2 * CMAXL);    // level 6's cells
+
+__global__ __launch_bounds__(CTHREADS) void k_coarse_op6(int nu1, int nu2, const double* __restrict__ op3,
+                                                         float* __restrict__ m)
+{
+	__shared__ CoarseLDS S;
+	const int j = (int) blockIdx.x, o = lvl_off(CMAXL);
+	if (threadIdx.x == 0) S.op3 = op3;
+	for (int i = threadIdx.x; i < C6N; i += CTHREADS) S.B[o + i] = i == j ? 1.0 : 0.0;
+	__syncthreads();
+	c_recurse<CMAXL>(S, nu1, nu2);
+	for (int i = threadIdx.x; i < C6N; i += CTHREADS) m[(size_t) i * C6N + j] = (float) S.X[o + i];
+}
+
+// Row i of m to FP16 times 2^e_i (largest magnitude to [2^14, 2^15); a zero
+// row keeps e_i = 0); inv[i] = 2^-e_i.  One 256-thread block per row.
+__global__ __launch_bounds__(256) void k_coarse_op6_pack(const float* __restrict__ m, _Float16* __restrict__ op6,
+                                                         float* __restrict__ inv)
+{
+	__shared__ float smax[256];
+	const size_t row = (size_t) blockIdx.x * C6N;
+	float mx = 0.f;
+	for (int j = threadIdx.x; j < C6N; j += 256) mx = fmaxf(mx, fabsf(m[row + j]));
+	smax[threadIdx.x] = mx;
+	__syncthreads();
+	for (int h = 128; h >= 1; h >>= 1) {
+		if ((int) threadIdx.x < h) smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + h]);
+		__syncthreads();
+	}
+	int e = 0;
+	if (smax[0] > 0.f) {
+		int ex;
+		(void) frexpf(smax[0], &ex);     // smax = f 2^ex, f in [0.5, 1)
+		e = 15 - ex;                      // smax 2^e in [2^14, 2^15)
+	}
+	const float sc = ldexpf(1.f, e);
+	for (int j = threadIdx.x; j < C6N; j += 256) op6[row + j] = (_Float16) (m[row + j] * sc);
+	if (threadIdx.x == 0) inv[blockIdx.x] = ldexpf(1.f, -e);
+}
+
+// out = M6 in: one wave per row (16 rows per 1024-thread block), the row
+// read with 16-byte loads (eight FP16 entries per lane and load, 1 KB per
+// wave), in staged in LDS.  Each lane sums its 64 products (FP32) in order,
+// the wave adds the lanes' sums in a fixed butterfly and scales by the row's
+// 2^-e: the same bits on every call.
+typedef _Float16 afm_half8 __attribute__((ext_vector_type(8)));
+
+__global__ __launch_bounds__(1024) void k_coarse_mv6(const _Float16* __restrict__ op6, const float* __restrict__ inv,
+                                                     const float* __restrict__ in, float* __restrict__ out,
+                                                     const int* __restrict__ done)
+{
+	__shared__ float4 sb[C6N / 4];
+	if (done && *done) return;
+	sb[threadIdx.x] = ((const float4*) in)[threadIdx.x];
+	__syncthreads();
+	const int lane = threadIdx.x & 63, row = blockIdx.x * 16 + (threadIdx.x >> 6);
+	const afm_half8* __restrict__ m = (const afm_half8*) (op6 + (size_t) row * C6N);
+	afm_half8 v[8];
+#pragma unroll
+	for (int k = 0; k < 8; ++k) v[k] = m[k * 64 + lane];
+	float acc = 0.f;
+#pragma unroll
+	for (int k = 0; k < 8; ++k) {
+		const float4 b0 = sb[(k * 64 + lane) * 2], b1 = sb[(k * 64 + lane) * 2 + 1];
+		acc = __builtin_fmaf((float) v[k][0], b0.x, acc);
+		acc = __builtin_fmaf((float) v[k][1], b0.y, acc);
+		acc = __builtin_fmaf((float) v[k][2], b0.z, acc);
+		acc = __builtin_fmaf((float) v[k][3], b0.w, acc);
+		acc = __builtin_fmaf((float) v[k][4], b1.x, acc);
+		acc = __builtin_fmaf((float) v[k][5], b1.y, acc);
+		acc = __builtin_fmaf((float) v[k][6], b1.z, acc);
+		acc = __builtin_fmaf((float) v[k][7], b1.w, acc);
+	}
+#pragma unroll
+	for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+	if (lane == 0) out[row] = acc * inv[row];
+}
+static_assert(C6N == 4096, "k_coarse_mv6 maps 1024 threads x 4 values onto level 6");
+
 // ------------------------------------------------------------- host side --
 static inline unsigned red_blocks(size_t N)
 {
@@ -1623,6 +1715,12 @@ static int coarse_launch(Ctx* c, int mode, int Lt, const T* in, T* out, int nu1,
 	// the FP32-cycle forms (preconditioner and FP32 refinement) take levels
 	// 3..0 as one matrix product; the FP64 cycles keep the recursion, the
 	// reference's operation order
+	if (std::is_same<T, float>::value && mode == 0 && Lt == CMAXL && c->op6 && nu1 == 2 && nu2 == 2) {
+		hipLaunchKernelGGL(k_coarse_mv6, dim3(C6N / 16), dim3(1024), 0, c->stream, c->op6, c->op6_inv,
+		                   (const float*) in, (float*) out, done);
+		AFM_LAUNCH_CHECK();
+		return 0;
+	}
 	const double* op3 = (std::is_same<T, float>::value && Lt >= 4) ? c->op3[(nu1 - 1) * 2 + (nu2 - 1)] : nullptr;
 	hipLaunchKernelGGL((k_coarse<T>), dim3(1), dim3(CTHREADS), 0, c->stream, mode, Lt, in, out, nu1, nu2, done, op3);
 	AFM_LAUNCH_CHECK();
@@ -1821,6 +1919,20 @@ int mg_setup(Ctx* c)
 			hipLaunchKernelGGL(k_coarse_op3, dim3(64), dim3(64), 0, c->stream, 1 + k / 2, 1 + k % 2, c->op3[k]);
 			AFM_LAUNCH_CHECK();
 		}
+	// and level 6's matrix for V(2, 2) (after level 3's, which it uses)
+	if (c->levels >= (unsigned) CMAXL) {
+		AFM_TRY(c->alloc((void**) &c->op6, (size_t) C6N * C6N * sizeof(_Float16)));
+		AFM_TRY(c->alloc((void**) &c->op6_inv, (size_t) C6N * sizeof(float)));
+		float* m = nullptr;
+		AFM_HIP(hipMalloc((void**) &m, (size_t) C6N * C6N * sizeof(float)));
+		hipLaunchKernelGGL(k_coarse_op6, dim3(C6N), dim3(CTHREADS), 0, c->stream, 2, 2, c->op3[3], m);
+		hipLaunchKernelGGL(k_coarse_op6_pack, dim3(C6N), dim3(256), 0, c->stream, m, c->op6, c->op6_inv);
+		const hipError_t le = hipGetLastError();
+		const hipError_t se = hipStreamSynchronize(c->stream);
+		(void) hipFree(m);
+		AFM_HIP(le);
+		AFM_HIP(se);
+	}
 	AFM_TRY(c->alloc((void**) &c->XSa, c->N * 8));
 	AFM_TRY(c->alloc((void**) &c->XSb, c->N * 8));
 	AFM_TRY(c->alloc((void**) &c->RES, c->N * 8));
