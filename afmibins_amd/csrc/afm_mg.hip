@@ -401,6 +401,14 @@ template<> struct SmoothShape<2, 2> { static constexpr int NW = 4, RPW = 10; };
 template<> struct SmoothShape<1, 3> { static constexpr int NW = 6, RPW = 6; };
 template<> struct SmoothShape<2, 3> { static constexpr int NW = AFM_SHAPE_FCG_NW, RPW = AFM_SHAPE_FCG_RPW; };
 constexpr size_t SMALL_CELLS = (size_t) 512 * 512;
+// the prolongation + post-smoothing launch at levels above SMALL_CELLS up to
+// MID_CELLS (1024^2 at the benchmark's 2048^2): 256-thread blocks of 6 rows
+// per wave, 576 blocks at 1024^2 instead of 171 (8.1 -> 6.6 us per launch;
+// the pre-smoothing launch there keeps the 16-wave tiles: 7.5 against
+// 7.6-8.6 us for the shorter ones)
+template<> struct SmoothShape<1, 5> { static constexpr int NW = 4, RPW = 6; };
+template<> struct SmoothShape<2, 5> { static constexpr int NW = 4, RPW = 6; };
+constexpr size_t MID_CELLS = (size_t) 1024 * 1024;
 
 // halo of a launch: 2 cells per sweep, 2 more for a residual epilogue
 template<int S, int EPI> constexpr int smooth_halo() { return 2 * S + (EPI != EPI_STORE ? 2 : 0); }
@@ -1619,12 +1627,18 @@ static int smooth_launch(Ctx* c, unsigned l, int mode, int sweeps, const T* xin,
 		if (small) AFM_SM2(S_, M_, E_, 1);          \
 		else AFM_SM2(S_, M_, E_, 0);                \
 	} while (0)
+#define AFM_SMP(S_)                                         \
+	do {                                                    \
+		if (small) AFM_SM2(S_, M_PADD, EPI_STORE, 1);       \
+		else if (mid) AFM_SM2(S_, M_PADD, EPI_STORE, 5);    \
+		else AFM_SM2(S_, M_PADD, EPI_STORE, 0);             \
+	} while (0)
 #define AFM_SMN(S_)                                         \
 	do {                                                    \
 		if (small) AFM_SM2(S_, M_NORMAL, EPI_STORE, 1);     \
 		else AFM_SM2(S_, M_NORMAL, EPI_STORE, 2);           \
 	} while (0)
-	const bool small = (size_t) nx * ny <= SMALL_CELLS;
+	const bool small = (size_t) nx * ny <= SMALL_CELLS, mid = !small && (size_t) nx * ny <= MID_CELLS;
 	if (rc) {
 		if (mode != M_ZERO) { set_error("fused restriction needs a zero start"); return -1; }
 		if (sweeps == 1) AFM_SM(1, M_ZERO, EPI_RESTRICT);
@@ -1632,15 +1646,16 @@ static int smooth_launch(Ctx* c, unsigned l, int mode, int sweeps, const T* xin,
 		else { set_error("smoother supports 1 or 2 sweeps per launch"); return -1; }
 	} else if (sweeps == 1) {
 		switch (mode) { case M_NORMAL: AFM_SMN(1); break; case M_ZERO: AFM_SM(1, M_ZERO, EPI_STORE); break;
-		                case M_PADD: AFM_SM(1, M_PADD, EPI_STORE); break; default: AFM_SM(1, M_PSET, EPI_STORE); }
+		                case M_PADD: AFM_SMP(1); break; default: AFM_SM(1, M_PSET, EPI_STORE); }
 	} else if (sweeps == 2) {
 		switch (mode) { case M_NORMAL: AFM_SMN(2); break; case M_ZERO: AFM_SM(2, M_ZERO, EPI_STORE); break;
-		                case M_PADD: AFM_SM(2, M_PADD, EPI_STORE); break; default: AFM_SM(2, M_PSET, EPI_STORE); }
+		                case M_PADD: AFM_SMP(2); break; default: AFM_SM(2, M_PSET, EPI_STORE); }
 	} else {
 		set_error("smoother supports 1 or 2 sweeps per launch");
 		return -1;
 	}
 #undef AFM_SM
+#undef AFM_SMP
 #undef AFM_SMN
 #undef AFM_SM2
 	AFM_LAUNCH_CHECK();
