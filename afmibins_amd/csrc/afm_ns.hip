@@ -24,16 +24,13 @@ namespace afm {
 // Each value is loaded once; every sum keeps the operand order of
 // uu(j) = (u[j] + u[xm j]) / 2 squared, ww(j) likewise in z, and
 // uw(j) = (u[zm j] + u[j]) / 2 * (w[xm j] + w[j]) / 2.
-__device__ inline void adv_at(const double* __restrict__ u, const double* __restrict__ w, int nx, int ny, unsigned n,
-                              unsigned i, int x, int z, double& ua, double& wa)
+// The advection terms of one cell from its neighbours' values: u at the
+// cell, x - 1, x + 1, z - 1, z + 1 and (x + 1, z - 1); w at the cell, x - 1,
+// x + 1, z - 1, z + 1 and (x - 1, z + 1).
+__device__ inline void adv_vals(double ui, double uxm, double uxp, double uzm, double uzp, double uxpzm, double wi,
+                                double wxm, double wxp, double wzm, double wzp, double wzpxm, unsigned n, double& ua,
+                                double& wa)
 {
-	const unsigned N = (unsigned) nx * (unsigned) ny, unx = (unsigned) nx;
-	const unsigned ixm = x == 0 ? i + (unx - 1) : i - 1, ixp = x == nx - 1 ? i - (unx - 1) : i + 1;
-	const unsigned izm = z == 0 ? i + (N - unx) : i - unx, izp = z == ny - 1 ? i - (N - unx) : i + unx;
-	const unsigned ixp_zm = z == 0 ? ixp + (N - unx) : ixp - unx;   // zm(xp(i))
-	const unsigned izp_xm = x == 0 ? izp + (unx - 1) : izp - 1;     // xm(zp(i))
-	const double ui = u[i], uxm = u[ixm], uxp = u[ixp], uzm = u[izm], uzp = u[izp], uxpzm = u[ixp_zm];
-	const double wi = w[i], wxm = w[ixm], wxp = w[ixp], wzm = w[izm], wzp = w[izp], wzpxm = w[izp_xm];
 	auto sq = [](double q) { return q * q; };
 	const double uu_xp = sq(0.5 * (uxp + ui)), uu_i = sq(0.5 * (ui + uxm));
 	const double ww_zp = sq(0.5 * (wzp + wi)), ww_i = sq(0.5 * (wi + wzm));
@@ -43,6 +40,18 @@ __device__ inline void adv_at(const double* __restrict__ u, const double* __rest
 	const double dv = 0.0 - 0.0;   // alg/div.h:54 with zero top/bottom
 	ua = n * ((uu_xp - uu_i) + dv + (uw_zp - uw_i));
 	wa = n * ((uw_xp - uw_i) + dv + (ww_zp - ww_i));
+}
+
+__device__ inline void adv_at(const double* __restrict__ u, const double* __restrict__ w, int nx, int ny, unsigned n,
+                              unsigned i, int x, int z, double& ua, double& wa)
+{
+	const unsigned N = (unsigned) nx * (unsigned) ny, unx = (unsigned) nx;
+	const unsigned ixm = x == 0 ? i + (unx - 1) : i - 1, ixp = x == nx - 1 ? i - (unx - 1) : i + 1;
+	const unsigned izm = z == 0 ? i + (N - unx) : i - unx, izp = z == ny - 1 ? i - (N - unx) : i + unx;
+	const unsigned ixp_zm = z == 0 ? ixp + (N - unx) : ixp - unx;   // zm(xp(i))
+	const unsigned izp_xm = x == 0 ? izp + (unx - 1) : izp - 1;     // xm(zp(i))
+	adv_vals(u[i], u[ixm], u[ixp], u[izm], u[izp], u[ixp_zm], w[i], w[ixm], w[ixp], w[izm], w[izp], w[izp_xm], n, ua,
+	         wa);
 }
 
 __global__ void k_advect(const double* __restrict__ u, const double* __restrict__ v, double* __restrict__ ua,
@@ -389,6 +398,104 @@ __global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* 
 		acc_u[1] += rx * rx;
 		acc_v[0] += r_v * r_v;
 		acc_v[1] += ry * ry;
+	}
+	reduce_partial<2>(acc_u, a.pu[0]);
+	reduce_partial<2>(acc_v, a.pu[1]);
+}
+
+// k_advect_cg_init on cell pairs: thread k forms cells 2k and 2k + 1 of a
+// row (nx even) from 16-byte loads of the pair in rows z - 1, z and z + 1
+// plus the four single neighbours outside the pair, each value read once,
+// and stores the pair's results as 16-byte vectors: half the memory
+// instructions and index arithmetic of one cell per thread.  Every cell
+// takes the arithmetic of adv_vals / stencil5 with its own wrap flags, so
+// each cell's b and r are the same bits as k_advect_cg_init's; the r.r and
+// b.b partials add the pair's terms in cell order.
+template<bool STORE_B>
+__global__ __launch_bounds__(256) void k_advect_cg_init_pairs(CGArgs a, const double* __restrict__ fx,
+                                                              const double* __restrict__ fy, double* __restrict__ uo,
+                                                              double* __restrict__ vo, int nx, int ny, unsigned n,
+                                                              double dt, double rho, int be, double hoff, double hdg,
+                                                              double off, double dg, double tol, int max_iters,
+                                                              double* __restrict__ bu, double* __restrict__ bv)
+{
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		cg_ctl_start(a, 0, tol, max_iters);
+		cg_ctl_start(a, 1, tol, max_iters);
+	}
+	const unsigned N = (unsigned) nx * (unsigned) ny, unx = (unsigned) nx, P = N / 2;
+	const double* __restrict__ u = a.x[0];
+	const double* __restrict__ v = a.x[1];
+	double acc_u[2] = {0.0, 0.0}, acc_v[2] = {0.0, 0.0};
+	for (unsigned k = blockIdx.x * 256 + threadIdx.x; k < P; k += gridDim.x * 256) {
+		const unsigned i = 2 * k;
+		const int x = (int) (i & (unx - 1)), z = (int) (i >> __builtin_ctz(unx));
+		const unsigned rz = i - (unsigned) x;                                 // row z
+		const unsigned rm = z == 0 ? rz + (N - unx) : rz - unx;               // row z - 1
+		const unsigned rp = z == ny - 1 ? rz - (N - unx) : rz + unx;          // row z + 1
+		const unsigned xm = x == 0 ? unx - 1 : (unsigned) x - 1;              // column x - 1
+		const unsigned x2 = x + 2 == nx ? 0u : (unsigned) x + 2;              // column x + 2
+		const double2 U = *(const double2*) (u + i), Um = *(const double2*) (u + rm + x),
+		              Up = *(const double2*) (u + rp + x);
+		const double2 W = *(const double2*) (v + i), Wm = *(const double2*) (v + rm + x),
+		              Wp = *(const double2*) (v + rp + x);
+		const double uxm0 = u[rz + xm], uxp1 = u[rz + x2], umx2 = u[rm + x2];
+		const double wxm0 = v[rz + xm], wxp1 = v[rz + x2], wpxm = v[rp + xm];
+		double ua[2], va[2];
+		adv_vals(U.x, uxm0, U.y, Um.x, Up.x, Um.y, W.x, wxm0, W.y, Wm.x, Wp.x, wpxm, n, ua[0], va[0]);
+		adv_vals(U.y, U.x, uxp1, Um.y, Up.y, umx2, W.y, W.x, wxp1, Wm.y, Wp.y, Wp.x, n, ua[1], va[1]);
+		const double2 F = *(const double2*) (fx + i), G = *(const double2*) (fy + i);
+		const bool wn = z == ny - 1, ws = z == 0;
+		// wrap flags of the pair's cells: x = 0 only for the first, x = nx - 1
+		// only for the second
+		const bool we1 = x + 1 == nx - 1, ww0 = x == 0;
+		double rx[2], ry[2];
+		if (be) {
+			*(double2*) (uo + i) = U;
+			*(double2*) (vo + i) = W;
+			rx[0] = U.x + (dt / 2) * (1 / rho * F.x - ua[0]);
+			rx[1] = U.y + (dt / 2) * (1 / rho * F.y - ua[1]);
+			ry[0] = W.x + (dt / 2) * (1 / rho * G.x - va[0]);
+			ry[1] = W.y + (dt / 2) * (1 / rho * G.y - va[1]);
+		} else {
+			const double2 O = *(const double2*) (uo + i), Om = *(const double2*) (uo + rm + x),
+			              Op = *(const double2*) (uo + rp + x);
+			const double2 Q = *(const double2*) (vo + i), Qm = *(const double2*) (vo + rm + x),
+			              Qp = *(const double2*) (vo + rp + x);
+			const double oxm0 = uo[rz + xm], oxp1 = uo[rz + x2], qxm0 = vo[rz + xm], qxp1 = vo[rz + x2];
+			const double hu0 = stencil5<double, true>(hoff, hdg, O.x, O.y, oxm0, Op.x, Om.x, false, ww0, wn, ws);
+			const double hu1 = stencil5<double, true>(hoff, hdg, O.y, oxp1, O.x, Op.y, Om.y, we1, false, wn, ws);
+			const double hv0 = stencil5<double, true>(hoff, hdg, Q.x, Q.y, qxm0, Qp.x, Qm.x, false, ww0, wn, ws);
+			const double hv1 = stencil5<double, true>(hoff, hdg, Q.y, qxp1, Q.x, Qp.y, Qm.y, we1, false, wn, ws);
+			rx[0] = hu0 + dt * (1 / rho * F.x - ua[0]);
+			rx[1] = hu1 + dt * (1 / rho * F.y - ua[1]);
+			ry[0] = hv0 + dt * (1 / rho * G.x - va[0]);
+			ry[1] = hv1 + dt * (1 / rho * G.y - va[1]);
+		}
+		if (STORE_B) {
+			*(double2*) (bu + i) = make_double2(rx[0], rx[1]);
+			*(double2*) (bv + i) = make_double2(ry[0], ry[1]);
+#pragma unroll
+			for (int o = 0; o < 2; ++o) {
+				acc_u[1] += rx[o] * rx[o];
+				acc_v[1] += ry[o] * ry[o];
+			}
+			continue;
+		}
+		const double Au0 = stencil5<double, true>(off, dg, U.x, U.y, uxm0, Up.x, Um.x, false, ww0, wn, ws);
+		const double Au1 = stencil5<double, true>(off, dg, U.y, uxp1, U.x, Up.y, Um.y, we1, false, wn, ws);
+		const double Av0 = stencil5<double, true>(off, dg, W.x, W.y, wxm0, Wp.x, Wm.x, false, ww0, wn, ws);
+		const double Av1 = stencil5<double, true>(off, dg, W.y, wxp1, W.x, Wp.y, Wm.y, we1, false, wn, ws);
+		const double r_u[2] = {rx[0] - Au0, rx[1] - Au1}, r_v[2] = {ry[0] - Av0, ry[1] - Av1};
+		*(double2*) (a.r[0] + i) = make_double2(r_u[0], r_u[1]);
+		*(double2*) (a.r[1] + i) = make_double2(r_v[0], r_v[1]);
+#pragma unroll
+		for (int o = 0; o < 2; ++o) {
+			acc_u[0] += r_u[o] * r_u[o];
+			acc_u[1] += rx[o] * rx[o];
+			acc_v[0] += r_v[o] * r_v[o];
+			acc_v[1] += ry[o] * ry[o];
+		}
 	}
 	reduce_partial<2>(acc_u, a.pu[0]);
 	reduce_partial<2>(acc_v, a.pu[1]);
@@ -767,14 +874,22 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 	}
 	a.tol_mode = tol_mode;
 	const unsigned rb = rblocks(c->N);
+	// cell pairs along x for the right-hand side (even row lengths: every
+	// power of two from 2)
+	const bool pairs = c->nx % 2 == 0;
 	// FP32 r only where the polynomial start writes it (k_cg_start_fused)
 	const bool r32 = p32 && adv && adv->neumann;
 	if (adv && adv->neumann) {
 		// b goes to the first p buffers (the first p update reads r and
 		// writes p[.][1], the second writes p[.][0])
-		hipLaunchKernelGGL(k_advect_cg_init<true>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy, adv->uo,
-		                   adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff, adv->hdg,
-		                   off, dg, tol, max_iters, w.p[0][0], w.p[1][0]);
+		if (pairs)
+			hipLaunchKernelGGL(k_advect_cg_init_pairs<true>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy,
+			                   adv->uo, adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be,
+			                   adv->hoff, adv->hdg, off, dg, tol, max_iters, w.p[0][0], w.p[1][0]);
+		else
+			hipLaunchKernelGGL(k_advect_cg_init<true>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy, adv->uo,
+			                   adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff,
+			                   adv->hdg, off, dg, tol, max_iters, w.p[0][0], w.p[1][0]);
 		using ST = StartTile<ST_NW, ST_RPW>;
 		const int n = (int) c->nx;
 		const int tiles_x = (n + ST_OX - 1) / ST_OX, ntiles = tiles_x * ((n + ST::OY - 1) / ST::OY);
@@ -787,9 +902,14 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 			hipLaunchKernelGGL((k_cg_start_fused<double, ST_NW, ST_RPW>), dim3(sb, 2), dim3(64 * ST_NW), 0, c->stream,
 			                   a, w.p[0][0], w.p[1][0], n, off, dg, P, tiles_x, ntiles);
 	} else if (adv)
-		hipLaunchKernelGGL(k_advect_cg_init<false>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy, adv->uo,
-		                   adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff, adv->hdg,
-		                   off, dg, tol, max_iters, nullptr, nullptr);
+		if (pairs)
+			hipLaunchKernelGGL(k_advect_cg_init_pairs<false>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy,
+			                   adv->uo, adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be,
+			                   adv->hoff, adv->hdg, off, dg, tol, max_iters, nullptr, nullptr);
+		else
+			hipLaunchKernelGGL(k_advect_cg_init<false>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy, adv->uo,
+			                   adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be, adv->hoff,
+			                   adv->hdg, off, dg, tol, max_iters, nullptr, nullptr);
 	else
 		hipLaunchKernelGGL(k_cg_init, dim3(rb, 2), dim3(256), 0, c->stream, a, (int) c->nx, (int) c->ny, off, dg, tol,
 		                   max_iters);
