@@ -1046,15 +1046,96 @@ __global__ __launch_bounds__(256) void k_update_resid(const double* __restrict__
 	}
 }
 
+// k_update_resid on cell pairs along x (nx even): thread k takes cells 2k
+// and 2k + 1 of a row, with 16-byte loads of the pair in rows z - 1, z,
+// z + 1 and single loads of its two outer neighbours, each value read
+// once, and 16-byte stores.  Every cell takes the arithmetic of
+// k_update_resid (the same bits of xs_out, rhs and res); the partial sums
+// add the pair's terms in cell order.
+template<typename TE, typename TR, bool XS, bool E>
+__global__ __launch_bounds__(256) void k_update_resid_pairs(const double* __restrict__ xs, const TE* __restrict__ e,
+                                                            double* __restrict__ rhs, double* __restrict__ xs_out,
+                                                            TR* __restrict__ res, int nx, int ny, double off,
+                                                            double dg, int sub_mean, int first,
+                                                            SolveCtl* __restrict__ ctl, double* __restrict__ partials)
+{
+	using TE2 = typename Pair<TE>::type;
+	using TR2 = typename Pair<TR>::type;
+	if (!first && ctl->done) return;
+	const unsigned N = (unsigned) nx * (unsigned) ny, unx = (unsigned) nx, P = N / 2;
+	const double mean = ctl->mean;
+	double acc[2] = {0.0, 0.0};
+	for (unsigned k = blockIdx.x * 256 + threadIdx.x; k < P; k += gridDim.x * 256) {
+		const unsigned i = 2 * k;
+		const int x = (int) (i & (unx - 1)), z = (int) (i >> __builtin_ctz(unx));
+		const unsigned rz = i - (unsigned) x;
+		const unsigned rm = z == 0 ? rz + (N - unx) : rz - unx;
+		const unsigned rp = z == ny - 1 ? rz - (N - unx) : rz + unx;
+		const unsigned xm = x == 0 ? unx - 1 : (unsigned) x - 1;
+		const unsigned x2 = x + 2 == nx ? 0u : (unsigned) x + 2;
+		// per cell o: centre, E, W, N (z + 1), S (z - 1)
+		double c[2] = {0.0, 0.0}, Ev[2] = {0.0, 0.0}, Wv[2] = {0.0, 0.0}, Nv[2] = {0.0, 0.0}, Sv[2] = {0.0, 0.0};
+		if (XS) {
+			const double2 X = *(const double2*) (xs + i), Xm = *(const double2*) (xs + rm + x),
+			              Xp = *(const double2*) (xs + rp + x);
+			const double xw = xs[rz + xm], xe = xs[rz + x2];
+			c[0] = X.x; Ev[0] = X.y; Wv[0] = xw; Nv[0] = Xp.x; Sv[0] = Xm.x;
+			c[1] = X.y; Ev[1] = xe; Wv[1] = X.x; Nv[1] = Xp.y; Sv[1] = Xm.y;
+		}
+		if (E) {
+			const TE2 Y = *(const TE2*) (e + i), Ym = *(const TE2*) (e + rm + x), Yp = *(const TE2*) (e + rp + x);
+			const double yw = (double) e[rz + xm], ye = (double) e[rz + x2];
+			c[0] = c[0] + (double) Y.x; Ev[0] = Ev[0] + (double) Y.y; Wv[0] = Wv[0] + yw;
+			Nv[0] = Nv[0] + (double) Yp.x; Sv[0] = Sv[0] + (double) Ym.x;
+			c[1] = c[1] + (double) Y.y; Ev[1] = Ev[1] + ye; Wv[1] = Wv[1] + (double) Y.x;
+			Nv[1] = Nv[1] + (double) Yp.y; Sv[1] = Sv[1] + (double) Ym.y;
+			*(double2*) (xs_out + i) = make_double2(c[0], c[1]);
+		}
+		const double2 B = *(const double2*) (rhs + i);
+		double r0[2] = {B.x, B.y};
+		if (sub_mean) {
+			r0[0] = r0[0] - mean * 1.0;
+			r0[1] = r0[1] - mean * 1.0;
+			*(double2*) (rhs + i) = make_double2(r0[0], r0[1]);
+			acc[1] += r0[0] * r0[0];
+			acc[1] += r0[1] * r0[1];
+		}
+		const bool wn = z == ny - 1, ws = z == 0;
+		const double Lx0 = stencil5p2<double, true>(off, dg, c[0], Ev[0], Wv[0], Nv[0], Sv[0], false, x == 0, wn, ws);
+		const double Lx1 = stencil5p2<double, true>(off, dg, c[1], Ev[1], Wv[1], Nv[1], Sv[1], x + 1 == nx - 1, false,
+		                                            wn, ws);
+		const double r[2] = {r0[0] - Lx0, r0[1] - Lx1};
+		TR2 ro;
+		ro.x = (TR) r[0]; ro.y = (TR) r[1];
+		*(TR2*) (res + i) = ro;
+		acc[0] += r[0] * r[0];
+		acc[0] += r[1] * r[1];
+	}
+	if (sub_mean) reduce_partial<2>(acc, partials);
+	else {
+		double a1[1] = {acc[0]};
+		reduce_partial<1>(a1, partials);
+	}
+}
+
 template<typename TE, typename TR>
 static void update_resid_launch(Ctx* c, unsigned grid, const double* xs, const TE* e, double* rhs, double* xs_out,
                                 TR* res, double off, double dg, int sub_mean, int first, SolveCtl* ctl,
                                 double* partials)
 {
 	const int nx = (int) c->nx, ny = (int) c->ny;
+	// cell pairs along x when the rows have even length (every power of two
+	// from 2)
+	const bool pairs = nx % 2 == 0;
 #define AFM_UR(X_, E_)                                                                                           \
-	hipLaunchKernelGGL((k_update_resid<TE, TR, X_, E_>), dim3(grid), dim3(256), 0, c->stream, xs, e, rhs, xs_out, res, \
-	                   nx, ny, off, dg, sub_mean, first, ctl, partials)
+	do {                                                                                                         \
+		if (pairs)                                                                                               \
+			hipLaunchKernelGGL((k_update_resid_pairs<TE, TR, X_, E_>), dim3(grid), dim3(256), 0, c->stream, xs, e, \
+			                   rhs, xs_out, res, nx, ny, off, dg, sub_mean, first, ctl, partials);                \
+		else                                                                                                     \
+			hipLaunchKernelGGL((k_update_resid<TE, TR, X_, E_>), dim3(grid), dim3(256), 0, c->stream, xs, e, rhs,  \
+			                   xs_out, res, nx, ny, off, dg, sub_mean, first, ctl, partials);                     \
+	} while (0)
 	if (xs && e) AFM_UR(true, true);
 	else if (xs) AFM_UR(true, false);
 	else if (e) AFM_UR(false, true);
