@@ -742,18 +742,20 @@ __device__ inline bool fcg_coefficients(const FcgArgs<TE>& a, double& beta, doub
 
 // The update over tiles of 128 x 32 cells in registers, like the smoother:
 // lane k of wave w holds columns 2k, 2k+1 of rows 4w .. 4w+3 of a region
-// whose outer ring of 2 cells is halo; p_k and x_{k+1} are formed on the
-// whole region, E/W neighbours by DPP, N/S through LDS boundary rows, and
-// r_{k+1} = b - L x_{k+1} on the inner 124 x 28 cells.  Every cell of the
+// whose outer 2 columns and 1 row on each side are halo (a lane holds a
+// column pair); p_k and x_{k+1} are formed on the whole region, E/W
+// neighbours by DPP, N/S through LDS boundary rows, and r_{k+1} = b - L
+// x_{k+1} on the inner 124 x 30 cells (1.10x the algorithmic reads of z, p
+// and x; a 2-row halo, 124 x 28, read 1.18x: 37.5 -> 37.2 us).  Every cell of the
 // grid is written by one tile; a halo cell's p and x are formed from the
 // same inputs by the same operations, so its value equals the stored one.
 // Two 512-thread blocks per CU walk tiles blockIdx.x, + gridDim.x, ... (one
 // coefficient prologue per block; the partial sums per block follow that
 // fixed order): while one block computes, the other's loads are in flight.
-// At 2048^2 the 1258 short tiles spread over the blocks more evenly than
-// 595 taller ones, which outweighs their 7% more halo reads.
-constexpr int FU_NW = 8, FU_RPW = 4, FU_H = 2, FU_LY = FU_NW * FU_RPW;
-constexpr int FU_TXO = 128 - 2 * FU_H, FU_TYO = FU_LY - 2 * FU_H;
+// At 2048^2 the 1173 short tiles spread over the blocks more evenly than
+// taller ones, which outweighs their more halo reads.
+constexpr int FU_NW = 8, FU_RPW = 4, FU_H = 2, FU_HY = 1, FU_LY = FU_NW * FU_RPW;
+constexpr int FU_TXO = 128 - 2 * FU_H, FU_TYO = FU_LY - 2 * FU_HY;
 
 static inline unsigned fcg_update_blocks(int nx, int ny)
 {
@@ -775,7 +777,7 @@ __global__ __launch_bounds__(64 * FU_NW) void k_fcg_update(FcgArgs<TE> a, int nx
 		const unsigned tile = xcd_tile(t, ntiles);
 		const int X0 = (int) (tile % ntx) * FU_TXO, Y0 = (int) (tile / ntx) * FU_TYO;
 		const int gxu = X0 - FU_H + 2 * lane, gx = wrap_once(gxu, nx);
-		const int gyb = Y0 - FU_H + FU_RPW * w;
+		const int gyb = Y0 - FU_HY + FU_RPW * w;
 		double pr[FU_RPW][2], xr[FU_RPW][2];
 		{
 			TE2 zv[FU_RPW], pv[FU_RPW];
@@ -822,7 +824,7 @@ __global__ __launch_bounds__(64 * FU_NW) void k_fcg_update(FcgArgs<TE> a, int nx
 				Lp[o] = stencil5p2<double, true>(off, dg, pr[i][o], E, W, Nn, S, we, ww, wn, ws);
 			}
 			const int ly = FU_RPW * w + i;
-			if (!(oc && ly >= FU_H && ly < FU_LY - FU_H && gyb + i < ny)) continue;
+			if (!(oc && ly >= FU_HY && ly < FU_LY - FU_HY && gyb + i < ny)) continue;
 			const size_t at = (size_t) nx * (gyb + i) + gx;
 			const double2 bv = *(const double2*) (a.b + at);
 			double rv[2];
