@@ -403,103 +403,144 @@ __global__ __launch_bounds__(256) void k_advect_cg_init(CGArgs a, const double* 
 	reduce_partial<2>(acc_v, a.pu[1]);
 }
 
-// k_advect_cg_init on cell pairs: thread k forms cells 2k and 2k + 1 of a
-// row (nx even) from 16-byte loads of the pair in rows z - 1, z and z + 1
-// plus the four single neighbours outside the pair, each value read once,
-// and stores the pair's results as 16-byte vectors: half the memory
-// instructions and index arithmetic of one cell per thread.  Every cell
-// takes the arithmetic of adv_vals / stencil5 with its own wrap flags, so
-// each cell's b and r are the same bits as k_advect_cg_init's; the r.r and
-// b.b partials add the pair's terms in cell order.
-template<bool STORE_B>
-__global__ __launch_bounds__(256) void k_advect_cg_init_pairs(CGArgs a, const double* __restrict__ fx,
-                                                              const double* __restrict__ fy, double* __restrict__ uo,
-                                                              double* __restrict__ vo, int nx, int ny, unsigned n,
-                                                              double dt, double rho, int be, double hoff, double hdg,
-                                                              double off, double dg, double tol, int max_iters,
-                                                              double* __restrict__ bu, double* __restrict__ bv)
+// k_advect_cg_init on runs of C cells along x (nx a multiple of C): thread
+// k forms cells Ck .. Ck + C - 1 of a row from 16-byte loads of the run in
+// rows z - 1, z and z + 1 plus the single neighbours outside it, each value
+// read once, and stores the run's results as 16-byte vectors: a fraction of
+// the memory instructions and index arithmetic of one cell per thread.
+// Every cell takes the arithmetic of adv_vals / stencil5 with its own wrap
+// flags, so each cell's b and r are the same bits as k_advect_cg_init's; the
+// r.r and b.b partials add the run's terms in cell order.
+constexpr int ADV_BS = 256;   // threads per block of the right-hand-side launch
+
+template<bool STORE_B, int C>
+__global__ __launch_bounds__(ADV_BS) void k_advect_cg_init_runs(CGArgs a, const double* __restrict__ fx,
+                                                             const double* __restrict__ fy, double* __restrict__ uo,
+                                                             double* __restrict__ vo, int nx, int ny, unsigned n,
+                                                             double dt, double rho, int be, double hoff, double hdg,
+                                                             double off, double dg, double tol, int max_iters,
+                                                             double* __restrict__ bu, double* __restrict__ bv)
 {
+	static_assert(C % 2 == 0, "runs of whole 16-byte vectors");
 	if (blockIdx.x == 0 && threadIdx.x == 0) {
 		cg_ctl_start(a, 0, tol, max_iters);
 		cg_ctl_start(a, 1, tol, max_iters);
 	}
-	const unsigned N = (unsigned) nx * (unsigned) ny, unx = (unsigned) nx, P = N / 2;
+	const unsigned N = (unsigned) nx * (unsigned) ny, unx = (unsigned) nx, P = N / C;
 	const double* __restrict__ u = a.x[0];
 	const double* __restrict__ v = a.x[1];
 	double acc_u[2] = {0.0, 0.0}, acc_v[2] = {0.0, 0.0};
-	for (unsigned k = blockIdx.x * 256 + threadIdx.x; k < P; k += gridDim.x * 256) {
-		const unsigned i = 2 * k;
+	// the run's values with one extra slot each side: [0] = x - 1, [C + 1] = x + C
+	auto load_run = [&](const double* __restrict__ f, unsigned row, int x, double (&r)[C]) {
+#pragma unroll
+		for (int j = 0; j < C; j += 2) {
+			const double2 t = *(const double2*) (f + row + x + j);
+			r[j] = t.x; r[j + 1] = t.y;
+		}
+	};
+	// consecutive block indices on each XCD (blocks are dealt to the 8 XCDs
+	// in turn): a run's neighbour rows are mostly read by the same XCD, from
+	// its own L2 (72.0 -> 69.5 us at 2048^2)
+	const unsigned lb = xcd_tile(blockIdx.x, gridDim.x);
+	for (unsigned k = lb * ADV_BS + threadIdx.x; k < P; k += gridDim.x * ADV_BS) {
+		const unsigned i = C * k;
 		const int x = (int) (i & (unx - 1)), z = (int) (i >> __builtin_ctz(unx));
 		const unsigned rz = i - (unsigned) x;                                 // row z
 		const unsigned rm = z == 0 ? rz + (N - unx) : rz - unx;               // row z - 1
 		const unsigned rp = z == ny - 1 ? rz - (N - unx) : rz + unx;          // row z + 1
 		const unsigned xm = x == 0 ? unx - 1 : (unsigned) x - 1;              // column x - 1
-		const unsigned x2 = x + 2 == nx ? 0u : (unsigned) x + 2;              // column x + 2
-		const double2 U = *(const double2*) (u + i), Um = *(const double2*) (u + rm + x),
-		              Up = *(const double2*) (u + rp + x);
-		const double2 W = *(const double2*) (v + i), Wm = *(const double2*) (v + rm + x),
-		              Wp = *(const double2*) (v + rp + x);
-		const double uxm0 = u[rz + xm], uxp1 = u[rz + x2], umx2 = u[rm + x2];
-		const double wxm0 = v[rz + xm], wxp1 = v[rz + x2], wpxm = v[rp + xm];
-		double ua[2], va[2];
-		adv_vals(U.x, uxm0, U.y, Um.x, Up.x, Um.y, W.x, wxm0, W.y, Wm.x, Wp.x, wpxm, n, ua[0], va[0]);
-		adv_vals(U.y, U.x, uxp1, Um.y, Up.y, umx2, W.y, W.x, wxp1, Wm.y, Wp.y, Wp.x, n, ua[1], va[1]);
-		const double2 F = *(const double2*) (fx + i), G = *(const double2*) (fy + i);
+		const unsigned xc = x + C == nx ? 0u : (unsigned) x + C;              // column x + C
+		double U[C], Um[C], Up[C], W[C], Wm[C], Wp[C];
+		load_run(u, rz, x, U); load_run(u, rm, x, Um); load_run(u, rp, x, Up);
+		load_run(v, rz, x, W); load_run(v, rm, x, Wm); load_run(v, rp, x, Wp);
+		const double uW = u[rz + xm], uE = u[rz + xc], umE = u[rm + xc];
+		const double wW = v[rz + xm], wE = v[rz + xc], wpW = v[rp + xm];
+		double ua[C], va[C];
+#pragma unroll
+		for (int j = 0; j < C; ++j) {
+			const double uxm = j ? U[j - 1] : uW, uxp = j < C - 1 ? U[j + 1] : uE;
+			const double uxpzm = j < C - 1 ? Um[j + 1] : umE;
+			const double wxm = j ? W[j - 1] : wW, wxp = j < C - 1 ? W[j + 1] : wE;
+			const double wzpxm = j ? Wp[j - 1] : wpW;
+			adv_vals(U[j], uxm, uxp, Um[j], Up[j], uxpzm, W[j], wxm, wxp, Wm[j], Wp[j], wzpxm, n, ua[j], va[j]);
+		}
+		double F[C], G[C];
+		load_run(fx, rz, x, F); load_run(fy, rz, x, G);
 		const bool wn = z == ny - 1, ws = z == 0;
-		// wrap flags of the pair's cells: x = 0 only for the first, x = nx - 1
-		// only for the second
-		const bool we1 = x + 1 == nx - 1, ww0 = x == 0;
-		double rx[2], ry[2];
+		double rx[C], ry[C];
 		if (be) {
-			*(double2*) (uo + i) = U;
-			*(double2*) (vo + i) = W;
-			rx[0] = U.x + (dt / 2) * (1 / rho * F.x - ua[0]);
-			rx[1] = U.y + (dt / 2) * (1 / rho * F.y - ua[1]);
-			ry[0] = W.x + (dt / 2) * (1 / rho * G.x - va[0]);
-			ry[1] = W.y + (dt / 2) * (1 / rho * G.y - va[1]);
+#pragma unroll
+			for (int j = 0; j < C; j += 2) {
+				*(double2*) (uo + i + j) = make_double2(U[j], U[j + 1]);
+				*(double2*) (vo + i + j) = make_double2(W[j], W[j + 1]);
+			}
+#pragma unroll
+			for (int j = 0; j < C; ++j) {
+				rx[j] = U[j] + (dt / 2) * (1 / rho * F[j] - ua[j]);
+				ry[j] = W[j] + (dt / 2) * (1 / rho * G[j] - va[j]);
+			}
 		} else {
-			const double2 O = *(const double2*) (uo + i), Om = *(const double2*) (uo + rm + x),
-			              Op = *(const double2*) (uo + rp + x);
-			const double2 Q = *(const double2*) (vo + i), Qm = *(const double2*) (vo + rm + x),
-			              Qp = *(const double2*) (vo + rp + x);
-			const double oxm0 = uo[rz + xm], oxp1 = uo[rz + x2], qxm0 = vo[rz + xm], qxp1 = vo[rz + x2];
-			const double hu0 = stencil5<double, true>(hoff, hdg, O.x, O.y, oxm0, Op.x, Om.x, false, ww0, wn, ws);
-			const double hu1 = stencil5<double, true>(hoff, hdg, O.y, oxp1, O.x, Op.y, Om.y, we1, false, wn, ws);
-			const double hv0 = stencil5<double, true>(hoff, hdg, Q.x, Q.y, qxm0, Qp.x, Qm.x, false, ww0, wn, ws);
-			const double hv1 = stencil5<double, true>(hoff, hdg, Q.y, qxp1, Q.x, Qp.y, Qm.y, we1, false, wn, ws);
-			rx[0] = hu0 + dt * (1 / rho * F.x - ua[0]);
-			rx[1] = hu1 + dt * (1 / rho * F.y - ua[1]);
-			ry[0] = hv0 + dt * (1 / rho * G.x - va[0]);
-			ry[1] = hv1 + dt * (1 / rho * G.y - va[1]);
+			double O[C], Om[C], Op[C], Q[C], Qm[C], Qp[C];
+			load_run(uo, rz, x, O); load_run(uo, rm, x, Om); load_run(uo, rp, x, Op);
+			load_run(vo, rz, x, Q); load_run(vo, rm, x, Qm); load_run(vo, rp, x, Qp);
+			const double oW = uo[rz + xm], oE = uo[rz + xc], qW = vo[rz + xm], qE = vo[rz + xc];
+#pragma unroll
+			for (int j = 0; j < C; ++j) {
+				// wrap flags: x = 0 only for the run's first cell, x = nx - 1 only for its last
+				const bool we = j == C - 1 && x + C == nx, ww = j == 0 && x == 0;
+				const double hu = stencil5<double, true>(hoff, hdg, O[j], j < C - 1 ? O[j + 1] : oE,
+				                                         j ? O[j - 1] : oW, Op[j], Om[j], we, ww, wn, ws);
+				const double hv = stencil5<double, true>(hoff, hdg, Q[j], j < C - 1 ? Q[j + 1] : qE,
+				                                         j ? Q[j - 1] : qW, Qp[j], Qm[j], we, ww, wn, ws);
+				rx[j] = hu + dt * (1 / rho * F[j] - ua[j]);
+				ry[j] = hv + dt * (1 / rho * G[j] - va[j]);
+			}
 		}
 		if (STORE_B) {
-			*(double2*) (bu + i) = make_double2(rx[0], rx[1]);
-			*(double2*) (bv + i) = make_double2(ry[0], ry[1]);
 #pragma unroll
-			for (int o = 0; o < 2; ++o) {
-				acc_u[1] += rx[o] * rx[o];
-				acc_v[1] += ry[o] * ry[o];
+			for (int j = 0; j < C; j += 2) {
+				*(double2*) (bu + i + j) = make_double2(rx[j], rx[j + 1]);
+				*(double2*) (bv + i + j) = make_double2(ry[j], ry[j + 1]);
+			}
+#pragma unroll
+			for (int j = 0; j < C; ++j) {
+				acc_u[1] += rx[j] * rx[j];
+				acc_v[1] += ry[j] * ry[j];
 			}
 			continue;
 		}
-		const double Au0 = stencil5<double, true>(off, dg, U.x, U.y, uxm0, Up.x, Um.x, false, ww0, wn, ws);
-		const double Au1 = stencil5<double, true>(off, dg, U.y, uxp1, U.x, Up.y, Um.y, we1, false, wn, ws);
-		const double Av0 = stencil5<double, true>(off, dg, W.x, W.y, wxm0, Wp.x, Wm.x, false, ww0, wn, ws);
-		const double Av1 = stencil5<double, true>(off, dg, W.y, wxp1, W.x, Wp.y, Wm.y, we1, false, wn, ws);
-		const double r_u[2] = {rx[0] - Au0, rx[1] - Au1}, r_v[2] = {ry[0] - Av0, ry[1] - Av1};
-		*(double2*) (a.r[0] + i) = make_double2(r_u[0], r_u[1]);
-		*(double2*) (a.r[1] + i) = make_double2(r_v[0], r_v[1]);
+		double r_u[C], r_v[C];
 #pragma unroll
-		for (int o = 0; o < 2; ++o) {
-			acc_u[0] += r_u[o] * r_u[o];
-			acc_u[1] += rx[o] * rx[o];
-			acc_v[0] += r_v[o] * r_v[o];
-			acc_v[1] += ry[o] * ry[o];
+		for (int j = 0; j < C; ++j) {
+			const bool we = j == C - 1 && x + C == nx, ww = j == 0 && x == 0;
+			const double Au = stencil5<double, true>(off, dg, U[j], j < C - 1 ? U[j + 1] : uE, j ? U[j - 1] : uW,
+			                                         Up[j], Um[j], we, ww, wn, ws);
+			const double Av = stencil5<double, true>(off, dg, W[j], j < C - 1 ? W[j + 1] : wE, j ? W[j - 1] : wW,
+			                                         Wp[j], Wm[j], we, ww, wn, ws);
+			r_u[j] = rx[j] - Au;
+			r_v[j] = ry[j] - Av;
+		}
+#pragma unroll
+		for (int j = 0; j < C; j += 2) {
+			*(double2*) (a.r[0] + i + j) = make_double2(r_u[j], r_u[j + 1]);
+			*(double2*) (a.r[1] + i + j) = make_double2(r_v[j], r_v[j + 1]);
+		}
+#pragma unroll
+		for (int j = 0; j < C; ++j) {
+			acc_u[0] += r_u[j] * r_u[j];
+			acc_u[1] += rx[j] * rx[j];
+			acc_v[0] += r_v[j] * r_v[j];
+			acc_v[1] += ry[j] * ry[j];
 		}
 	}
 	reduce_partial<2>(acc_u, a.pu[0]);
 	reduce_partial<2>(acc_v, a.pu[1]);
 }
+
+// cells per thread of the right-hand-side launch: pairs (2048^2, both
+// fields: 71.8 us; runs of 4: 82.8, of 8: 108.6, one cell per thread: 80.3;
+// 512- and 1024-thread blocks of pairs: 73.2 and 77.6)
+constexpr int ADV_RUN = 2;
 
 // The CG forms' start for the velocity solves (no reference counterpart: the
 // reference starts from the current velocity).  A = I - sL has its spectrum
@@ -874,16 +915,16 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 	}
 	a.tol_mode = tol_mode;
 	const unsigned rb = rblocks(c->N);
-	// cell pairs along x for the right-hand side (even row lengths: every
-	// power of two from 2)
-	const bool pairs = c->nx % 2 == 0;
+	// runs of ADV_RUN cells along x for the right-hand side (row lengths a
+	// multiple of the run: every power of two from ADV_RUN)
+	const bool pairs = c->nx % ADV_RUN == 0;
 	// FP32 r only where the polynomial start writes it (k_cg_start_fused)
 	const bool r32 = p32 && adv && adv->neumann;
 	if (adv && adv->neumann) {
 		// b goes to the first p buffers (the first p update reads r and
 		// writes p[.][1], the second writes p[.][0])
 		if (pairs)
-			hipLaunchKernelGGL(k_advect_cg_init_pairs<true>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy,
+			hipLaunchKernelGGL((k_advect_cg_init_runs<true, ADV_RUN>), dim3(rb), dim3(ADV_BS), 0, c->stream, a, adv->fx, adv->fy,
 			                   adv->uo, adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be,
 			                   adv->hoff, adv->hdg, off, dg, tol, max_iters, w.p[0][0], w.p[1][0]);
 		else
@@ -903,7 +944,7 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 			                   a, w.p[0][0], w.p[1][0], n, off, dg, P, tiles_x, ntiles);
 	} else if (adv)
 		if (pairs)
-			hipLaunchKernelGGL(k_advect_cg_init_pairs<false>, dim3(rb), dim3(256), 0, c->stream, a, adv->fx, adv->fy,
+			hipLaunchKernelGGL((k_advect_cg_init_runs<false, ADV_RUN>), dim3(rb), dim3(ADV_BS), 0, c->stream, a, adv->fx, adv->fy,
 			                   adv->uo, adv->vo, (int) c->nx, (int) c->ny, c->n, adv->dt, adv->rho, adv->be,
 			                   adv->hoff, adv->hdg, off, dg, tol, max_iters, nullptr, nullptr);
 		else
