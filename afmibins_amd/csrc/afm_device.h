@@ -408,4 +408,15 @@ __device__ inline unsigned xcd_tile(unsigned bid, unsigned nblk)
 	return (bid % X) * (nblk / X) + bid / X;
 }
 
+// The same for any block count: XCD x receives blocks x, x + 8, ... (there
+// are (nblk - x + 7) / 8 of them) and gives them the consecutive indices
+// after those of XCDs 0 .. x - 1.  A bijection of 0 .. nblk - 1.
+__device__ inline unsigned xcd_remap(unsigned bid, unsigned nblk)
+{
+	const unsigned X = 8, x = bid % X, q = nblk / X, r = nblk % X;
+	// XCDs below r receive q + 1 blocks, the others q
+	const unsigned before = x * q + (x < r ? x : r);
+	return before + bid / X;
+}
+
 }  // namespace afm

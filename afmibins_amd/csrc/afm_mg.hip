@@ -1067,7 +1067,8 @@ __global__ __launch_bounds__(256) void k_update_resid_pairs(const double* __rest
 	const unsigned N = (unsigned) nx * (unsigned) ny, unx = (unsigned) nx, P = N / 2;
 	const double mean = ctl->mean;
 	double acc[2] = {0.0, 0.0};
-	for (unsigned k = blockIdx.x * 256 + threadIdx.x; k < P; k += gridDim.x * 256) {
+	// consecutive blocks per XCD (26.9 -> 25.3 us at 2048^2)
+	for (unsigned k = xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x; k < P; k += gridDim.x * 256) {
 		const unsigned i = 2 * k;
 		const int x = (int) (i & (unx - 1)), z = (int) (i >> __builtin_ctz(unx));
 		const unsigned rz = i - (unsigned) x;

@@ -69,7 +69,9 @@ __global__ void k_div(const double* __restrict__ u, const double* __restrict__ v
                       int ny, unsigned n)
 {
 	const size_t N = (size_t) nx * ny;
-	size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x;
+	// consecutive blocks per XCD: a row's z + 1 neighbours come from the same
+	// L2 (17.3 -> 16.7 us at 2048^2; the gradient update lost 8% so)
+	size_t i = xcd_remap(blockIdx.x, gridDim.x) * (size_t) blockDim.x + threadIdx.x;
 	if (i >= N) return;
 	int x, z;
 	cell_xy(i, nx, x, z);
