@@ -187,8 +187,12 @@ struct Strip {
 		}
 	}
 
-	// 2S red-black half-sweeps; bnd: LDS boundary-row exchange
-	__device__ void sweeps(T* bnd, int nx, int ny, T off, T dg, T inv)
+	// 2S red-black half-sweeps; bnd: LDS boundary-row exchange.  With SUMS,
+	// s64[i] receives off times the FP64 sum of the four neighbours the last
+	// half-sweep (colour 1) read for row i's colour-1 cell (its final
+	// colour-0 neighbours), for fcg_dots_edges.
+	template<bool SUMS = false>
+	__device__ void sweeps(T* bnd, int nx, int ny, T off, T dg, T inv, double off64 = 0.0, double* s64 = nullptr)
 	{
 		bnd_publish<T, RPW>(bnd, xr, w, lane);
 		__syncthreads();
@@ -205,6 +209,8 @@ struct Strip {
 				strip_nbrs<T, RPW>(xr, i, o, up, dn, Ea, We, Nv, Sv);
 				const T v = stencil<false, T>(i, o, off, dg, T(0), Ea, We, Nv, Sv, nx, ny);
 				xr[i][o] = (br[i][o] - v) * inv;
+				if (SUMS && ph == 2 * S - 1)
+					s64[i] = off64 * (((double) Ea + (double) We) + ((double) Nv + (double) Sv));
 			}
 			if (ph < 2 * S - 1) {
 				bnd_at(bnd, w + 1, 0, lane, c) = xr[0][c];
@@ -337,39 +343,35 @@ struct Strip {
 	// residual.  Stores z on the output tile and adds this lane's z.r_k (r_k
 	// as the cycle took it: b, in registers), z.r_{k-1} (rkm1, rounded the
 	// same way) and z.(L z), products and sums in FP64 (rows, then columns).
-	__device__ void fcg_dots(T* __restrict__ z, const T* __restrict__ rkm1, double* bx, int nx, int ny, double off,
-	                         double dg, double (&acc)[3]) const
+	// Stores z and sums z.r_k, z.r_{k-1} and z.Lz without exchanging z's
+	// boundary rows: every grid edge joins a colour-0 and a colour-1 cell, so
+	// z.Lz = sum dg z^2 + 2 sum over colour-1
+	// cells of z times off (sum of its neighbours), whose sums the last
+	// half-sweep kept (sweeps<true>); each edge is counted once, by the tile
+	// that writes its colour-1 cell.  Products and sums in FP64.
+	__device__ void fcg_dots_edges(T* __restrict__ z, const T* __restrict__ rkm1, int nx, int ny, double dg,
+	                               const double* s64, double (&acc)[3]) const
 	{
-		double zz[RPW][2];
-#pragma unroll
-		for (int i = 0; i < RPW; ++i) { zz[i][0] = (double) xr[i][0]; zz[i][1] = (double) xr[i][1]; }
-		bnd_publish<double, RPW>(bx, zz, w, lane);
-		__syncthreads();
-		double up[2], dn[2];
-#pragma unroll
-		for (int o = 0; o < 2; ++o) { up[o] = bnd_at(bx, w, 1, lane, o); dn[o] = bnd_at(bx, w + 2, 0, lane, o); }
 		const bool oc = out_col(nx);
 #pragma unroll
 		for (int i = 0; i < RPW; ++i) {
-			double lz[2];
-#pragma unroll
-			for (int o = 0; o < 2; ++o) {
-				double Ea, We, Nv, Sv;
-				strip_nbrs<double, RPW>(zz, i, o, up[o], dn[o], Ea, We, Nv, Sv);
-				lz[o] = stencil<true, double>(i, o, off, dg, zz[i][o], Ea, We, Nv, Sv, nx, ny);
-			}
 			if (oc && out_row(i, ny)) {
 				const size_t at = (size_t) nx * (gyb + i) + gx;
 				T2 zo;
 				zo.x = xr[i][0]; zo.y = xr[i][1];
 				*(T2*) (z + at) = zo;
 				const T2 q = *(const T2*) (rkm1 + at);
-				acc[0] += zz[i][0] * (double) br[i][0];
-				acc[0] += zz[i][1] * (double) br[i][1];
-				acc[1] += zz[i][0] * (double) q.x;
-				acc[1] += zz[i][1] * (double) q.y;
-				acc[2] += zz[i][0] * lz[0];
-				acc[2] += zz[i][1] * lz[1];
+				const double z0 = (double) xr[i][0], z1 = (double) xr[i][1];
+				// row i's colour-1 cell is column (1 + i) & 1 (rows and columns of a
+				// strip start even)
+				const double zb = ((1 + i) & 1) ? z1 : z0;
+				acc[0] += z0 * (double) br[i][0];
+				acc[0] += z1 * (double) br[i][1];
+				acc[1] += z0 * (double) q.x;
+				acc[1] += z1 * (double) q.y;
+				acc[2] += dg * z0 * z0;
+				acc[2] += dg * z1 * z1;
+				acc[2] += 2.0 * zb * s64[i];
 			}
 		}
 	}
@@ -630,19 +632,19 @@ __global__ __launch_bounds__((64 * SmoothShape<S, 3>::NW)) void k_smooth_fcg(
 {
 	constexpr int NW = SmoothShape<S, 3>::NW, RPW = SmoothShape<S, 3>::RPW, H = smooth_halo<S, EPI_UPDATE>();
 	__shared__ TE bnd[bnd_size<NW>()];
-	__shared__ double bnd64[bnd_size<NW>()];
 	int X0, Y0;
 	double acc[3] = {0.0, 0.0, 0.0};
+	double s64[RPW];   // the colour-1 cells' neighbour sums (fcg_dots_edges)
 	if (smooth_tile_origin<S, EPI_UPDATE, 3>(nx, ny, X0, Y0)) {
 		Strip<TE, S, H, NW, RPW, true> st(X0, Y0, nx);
 		st.template load<M_PADD>(epre, b, ec, nx, ny);
-		st.sweeps(bnd, nx, ny, off, dg, inv);
-		st.fcg_dots(z, rkm1, bnd64, nx, ny, off64, dg64, acc);
+		st.template sweeps<true>(bnd, nx, ny, off, dg, inv, off64, s64);
+		st.fcg_dots_edges(z, rkm1, nx, ny, dg64, s64, acc);
 	} else {
 		Strip<TE, S, H, NW, RPW, false> st(X0, Y0, nx);
 		st.template load<M_PADD>(epre, b, ec, nx, ny);
-		st.sweeps(bnd, nx, ny, off, dg, inv);
-		st.fcg_dots(z, rkm1, bnd64, nx, ny, off64, dg64, acc);
+		st.template sweeps<true>(bnd, nx, ny, off, dg, inv, off64, s64);
+		st.fcg_dots_edges(z, rkm1, nx, ny, dg64, s64, acc);
 	}
 	reduce_partial_s<3>(acc, partials, stride);
 }
