@@ -165,8 +165,11 @@ int mg_setup(Ctx* c);
 // points at the solution inside the context (valid until its next solve).
 // x0: the CG forms' (precision 1, 3) initial guess (nullptr: zero); the
 // reference's iteration always starts from its nested iteration.
+// sums_ready: the caller's launch has already summed rhs into the partials
+// (grid red_blocks(N)) and started the control block, as k_mean does.
 int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, int precision, int max_cycles,
-             int* cycles, double* final_rms, const double** sol = nullptr, const double* x0 = nullptr);
+             int* cycles, double* final_rms, const double** sol = nullptr, const double* x0 = nullptr,
+             bool sums_ready = false);
 int mg_apply_laplacian(Ctx* c, unsigned level, const double* x, double* y);
 int mg_smooth(Ctx* c, unsigned level, const double* x_in, const double* b, double* x_out, int sweeps, int zero_init);
 int mg_restrict(Ctx* c, unsigned level, const double* fine, double* coarse);

@@ -2204,7 +2204,7 @@ static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 }
 
 int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, int precision, int max_cycles,
-             int* cycles, double* final_rms, const double** sol, const double* x0)
+             int* cycles, double* final_rms, const double** sol, const double* x0, bool sums_ready)
 {
 	if (nu1 < 1 || nu1 > 2 || nu2 < 1 || nu2 > 2) { set_error("nu1, nu2 must be 1 or 2 (one fused launch per smoothing step)"); return -1; }
 	if (precision < 0 || precision > 3) {
@@ -2221,8 +2221,9 @@ int mg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int nu2, in
 	// (mg/solver.h:224 does the mean after the nested start; the value is the
 	// same because rhs is not touched there); for a relative tolerance the RMS
 	// of rhs - mean is summed by the start's residual launch below
-	hipLaunchKernelGGL(k_mean, dim3(rb), dim3(256), 0, c->stream, rhs, N, 0, dctl, c->partials,
-	                   tol_mode == 1 ? 0.0 : tol, max_cycles);
+	if (!sums_ready)
+		hipLaunchKernelGGL(k_mean, dim3(rb), dim3(256), 0, c->stream, rhs, N, 0, dctl, c->partials,
+		                   tol_mode == 1 ? 0.0 : tol, max_cycles);
 	hipLaunchKernelGGL(k_mean_fin, dim3(1), dim3(256), 0, c->stream, 0, tol, rb, N, dctl, c->partials);
 	AFM_LAUNCH_CHECK();
 	if (precision == 1) return fcg_solve<float>(c, rhs, tol, tol_mode, nu1, nu2, max_cycles, cycles, final_rms, sol, x0);

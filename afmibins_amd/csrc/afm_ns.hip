@@ -80,6 +80,45 @@ __global__ void k_div(const double* __restrict__ u, const double* __restrict__ v
 	d[i] = n * ((u[ixp] - u[i]) + dv + (v[izp] - v[i]));
 }
 
+// k_div fused with the first launch of the projection's solve (mg_solve's
+// k_mean, pass 0): the divergence on cell pairs along x (grid-stride over
+// the reduction grid, consecutive blocks per XCD), each cell by k_div's
+// formula (the same bits), and per block the sum of d * 1 in cell order for
+// the mean; block 0 starts the solve's control block as k_mean does.
+__global__ __launch_bounds__(256) void k_div_sum(const double* __restrict__ u, const double* __restrict__ v,
+                                                 double* __restrict__ d, int nx, int ny, unsigned n,
+                                                 SolveCtl* __restrict__ ctl, double* __restrict__ partials,
+                                                 double tol, int max_iter)
+{
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		ctl->tol = tol;
+		ctl->max_iter = max_iter;
+		ctl->count = 0;
+		ctl->done = 0;
+		ctl->err = 0;
+		ctl->res = 0.0;
+	}
+	const unsigned N = (unsigned) nx * (unsigned) ny, unx = (unsigned) nx, P = N / 2;
+	double acc[1] = {0.0};
+	for (unsigned k = xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x; k < P; k += gridDim.x * 256) {
+		const unsigned i = 2 * k;
+		const int x = (int) (i & (unx - 1)), z = (int) (i >> __builtin_ctz(unx));
+		const unsigned rz = i - (unsigned) x;
+		const unsigned rp = z == ny - 1 ? rz - (N - unx) : rz + unx;
+		const unsigned x2 = x + 2 == nx ? 0u : (unsigned) x + 2;
+		const double2 U = *(const double2*) (u + i), V = *(const double2*) (v + i),
+		              Vp = *(const double2*) (v + rp + x);
+		const double uE = u[rz + x2];
+		const double dv = 0.0 - 0.0;
+		const double d0 = n * ((U.y - U.x) + dv + (Vp.x - V.x));
+		const double d1 = n * ((uE - U.y) + dv + (Vp.y - V.y));
+		*(double2*) (d + i) = make_double2(d0, d1);
+		acc[0] += d0 * 1.0;
+		acc[0] += d1 * 1.0;
+	}
+	reduce_partial<1>(acc, partials);
+}
+
 // u -= n (p - p[x-1]),  v -= n (p - p[y-1])   (alg/grad.h:36-43, ns/projection.h:70-72)
 // u -= n (p - p_west), v -= n (p - p_south); with pcopy, p is also copied
 // there (the projection's potential kept for get_p)
@@ -1044,12 +1083,21 @@ struct NS {
 int ns_project(NS* s, double* u, double* v, int kind)
 {
 	Ctx* c = s->c;
-	AFM_TRY(op_div(c, u, v, s->phi));
+	// the divergence and its sum for the mean in one pass (even row lengths)
+	bool sums = false;
+	if (c->nx % 2 == 0) {
+		hipLaunchKernelGGL(k_div_sum, dim3(rblocks(c->N)), dim3(256), 0, c->stream, u, v, s->phi, (int) c->nx,
+		                   (int) c->ny, c->n, &c->ctl[0], c->partials, s->tol_mode == 1 ? 0.0 : s->tol, s->max_cycles);
+		AFM_LAUNCH_CHECK();
+		sums = true;
+	} else
+		AFM_TRY(op_div(c, u, v, s->phi));
 	int cyc = 0;
 	const double* sol = nullptr;
 	const bool warm = kind >= 0 && s->pot[kind];
 	const double* x0 = (warm && s->pot_ok[kind]) ? s->pot[kind] : nullptr;
-	AFM_TRY(mg_solve(c, s->phi, s->tol, s->tol_mode, 2, 2, s->precision, s->max_cycles, &cyc, nullptr, &sol, x0));
+	AFM_TRY(mg_solve(c, s->phi, s->tol, s->tol_mode, 2, 2, s->precision, s->max_cycles, &cyc, nullptr, &sol, x0,
+	                 sums));
 	s->last_cycles = cyc;
 	s->last_rms = c->hctl[0].res;
 	s->last_thr = c->hctl[0].tol;
