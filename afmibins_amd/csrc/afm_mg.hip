@@ -771,6 +771,8 @@ __global__ __launch_bounds__(64 * FU_NW) void k_fcg_update(FcgArgs<TE> a, int nx
 	__shared__ double bp[bnd_size<FU_NW>()], bx[bnd_size<FU_NW>()];
 	double beta, alpha;
 	if (fcg_coefficients<TE, FIRST>(a, beta, alpha)) return;
+	// b is the right-hand side with its mean (k_mean); r = (b - mean) - L x
+	const double bmean = a.ctl->mean;
 	const int lane = (int) threadIdx.x & 63, w = (int) threadIdx.x >> 6;
 	const int ntx = (nx + FU_TXO - 1) / FU_TXO;
 	const unsigned ntiles = (unsigned) (ntx * ((ny + FU_TYO - 1) / FU_TYO));
@@ -832,7 +834,7 @@ __global__ __launch_bounds__(64 * FU_NW) void k_fcg_update(FcgArgs<TE> a, int nx
 			double rv[2];
 #pragma unroll
 			for (int o = 0; o < 2; ++o) {
-				rv[o] = (o ? bv.y : bv.x) - Lx[o];
+				rv[o] = ((o ? bv.y : bv.x) - bmean * 1.0) - Lx[o];
 				acc[0] += rv[o] * rv[o];
 				acc[1] += pr[i][o] * rv[o];
 				acc[2] += pr[i][o] * Lp[o];
@@ -858,6 +860,8 @@ __global__ __launch_bounds__(256) void k_fcg_update_1d(FcgArgs<TE> a, int nx, in
 {
 	double beta, alpha;
 	if (fcg_coefficients<TE, FIRST>(a, beta, alpha)) return;
+	// b is the right-hand side with its mean (k_mean); r = (b - mean) - L x
+	const double bmean = a.ctl->mean;
 	const size_t N = (size_t) nx * ny;
 	double acc[3] = {0.0, 0.0, 0.0};
 	for (size_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t) gridDim.x * 256) {
@@ -876,7 +880,7 @@ __global__ __launch_bounds__(256) void k_fcg_update_1d(FcgArgs<TE> a, int nx, in
 #pragma unroll
 			for (int t = 0; t < 5; ++t) xv[t] = a.x_old[nb[t]];
 		}
-		const double bi = a.b[i];
+		const double bi = a.b[i] - bmean * 1.0;
 #pragma unroll
 		for (int t = 0; t < 5; ++t) {
 			const double p = FIRST ? zv[t] : zv[t] + beta * pv[t];
@@ -1004,8 +1008,9 @@ __global__ __launch_bounds__(256) void k_resid_restr(const TS* __restrict__ x, c
 
 // ------------------------------------------------------- update + residual --
 // xs_new = xs + e (e may be null), r = rhs' - L xs_new, where rhs' = rhs -
-// mean when `sub_mean` (written back to rhs).  Deterministic RMS of r and the
-// stop test (mg/solver.h:226-233) in the last block.  XS / E: xs / e given
+// mean when `sub_mean` (written back to rhs when it is 1, not when 2).
+// Deterministic RMS of r and the stop test (mg/solver.h:226-233) in the
+// last block.  XS / E: xs / e given
 // (template flags, so a cell's loads issue together; update_resid_launch
 // picks the variant from the pointers).
 template<typename TE, typename TR, bool XS, bool E>
@@ -1037,7 +1042,11 @@ __global__ __launch_bounds__(256) void k_update_resid(const double* __restrict__
 			xs_out[i] = c;
 		}
 		double r0 = rhs[i];
-		if (sub_mean) { r0 = r0 - mean * 1.0; rhs[i] = r0; acc[1] += r0 * r0; }
+		if (sub_mean) {
+			r0 = r0 - mean * 1.0;
+			if (sub_mean == 1) rhs[i] = r0;
+			acc[1] += r0 * r0;
+		}
 		double Lx = stencil5p2<double, true>(off, dg, c, Ev, Wv, Nv, Sv, gx == nx - 1, gx == 0, gy == ny - 1, gy == 0);
 		double r = r0 - Lx;
 		res[i] = (TR) r;
@@ -1101,7 +1110,7 @@ __global__ __launch_bounds__(256) void k_update_resid_pairs(const double* __rest
 		if (sub_mean) {
 			r0[0] = r0[0] - mean * 1.0;
 			r0[1] = r0[1] - mean * 1.0;
-			*(double2*) (rhs + i) = make_double2(r0[0], r0[1]);
+			if (sub_mean == 1) *(double2*) (rhs + i) = make_double2(r0[0], r0[1]);
 			acc[1] += r0[0] * r0[0];
 			acc[1] += r0[1] * r0[1];
 		}
@@ -2107,11 +2116,12 @@ static int fcg_solve(Ctx* c, double* rhs, double tol, int tol_mode, int nu1, int
 	T* r[2];
 	if (F32) { r[0] = (T*) c->RES32; r[1] = (T*) c->RES32b; }
 	else { r[0] = (T*) c->RES; r[1] = (T*) c->RESb; }
-	// r_0 = rhs - mean - L x_0, with rhs - mean written back to rhs
+	// r_0 = rhs - mean - L x_0; rhs keeps its mean (the updates subtract it,
+	// with the same bits, instead of reading rhs - mean back)
 	// (mg/solver.h:224-225) and r_0, rounded, as the first cycle's right-hand
 	// side; the relative tolerance
 	update_resid_launch<T, T>(c, rb, x0, (const T*) nullptr, rhs, (double*) nullptr,
-	                          r[0], off, dg, 1, 1, dctl, c->partials);
+	                          r[0], off, dg, 2, 1, dctl, c->partials);
 	hipLaunchKernelGGL(k_update_fin, dim3(1), dim3(256), 0, c->stream, tol_mode == 1 ? 2 : 1, rb, N, dctl, c->partials,
 	                   tol);
 	AFM_LAUNCH_CHECK();
