@@ -207,14 +207,16 @@ def test_mg_solve_matches_oracle(ctx_of, oracle, dev, n):
 
 
 @pytest.mark.parametrize("precision", [1, 2, 3])
-@pytest.mark.parametrize("n", [32, 128, 1024])
+@pytest.mark.parametrize("n", [32, 64, 128, 1024])
 def test_mixed_precision_solve(ctx_of, oracle, dev, n, precision):
     """Forms 1-3 of afm_mg_solve against the reference iteration in FP64
     (form 0) at the same relative tolerance: the test holds for the true
     residual of the returned solution (recomputed here with the oracle's
     SpMV), the solutions agree far below the tolerance up to the periodic
     problem's free constant, and the CG forms need no more V-cycles.  n = 32
-    runs every cycle in the coarse kernel (the unfused CG path)."""
+    runs every cycle in the coarse kernel (the unfused CG path); at n = 64
+    the FP32 cycle of forms 1 and 2 is level 6's FP16 matrix alone, from
+    n = 128 on it is the matrix below the grid kernels."""
     ctx = ctx_of(n)
     rhs = W.fourier_rhs(n, 128, min(32, n // 4), seed=3 * n)
     a, b = T(rhs, dev), T(rhs, dev)
