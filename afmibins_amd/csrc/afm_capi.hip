@@ -19,6 +19,7 @@ Ctx::~Ctx()
 	for (auto& kind : kt_ev)
 		for (hipEvent_t e : kind)
 			if (e) (void) hipEventDestroy(e);
+	if (cg_read_ev) (void) hipEventDestroy(cg_read_ev);
 	if (own_stream) (void) hipStreamDestroy(own_stream);
 }
 
@@ -34,7 +35,7 @@ struct NS;
 struct IB;
 int ns_create(Ctx*, double, double, double, double, int, int, double*, double*, NS**);
 int ns_step(NS*, const double*, const double*);
-int ns_project(NS*, double*, double*, int);
+int ns_project(NS*, double*, double*, int, bool div_ready);
 int ns_get_start(NS*, int, double*, int*);
 int ns_set_start(NS*, int, const double*);
 int ns_get_p(NS*, double*);
@@ -104,6 +105,7 @@ int afm_ctx_create(const afm_geom* g, int device, afm_ctx** out)
 	auto fail = [&](int r) { delete c; return r; };
 	if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) { set_error("hipStreamCreate failed"); return fail(-3); }
 	c->stream = c->own_stream;
+	if (hipEventCreateWithFlags(&c->cg_read_ev, hipEventDisableTiming) != hipSuccess) { set_error("hipEventCreate failed"); return fail(-3); }
 	if ((rc = c->alloc((void**) &c->ctl, sizeof(SolveCtl) * NCTL))) return fail(rc);
 	if (hipHostMalloc((void**) &c->hctl, sizeof(SolveCtl) * NCTL, hipHostMallocDefault) != hipSuccess) { set_error("hipHostMalloc failed"); return fail(-3); }
 	if ((rc = c->alloc((void**) &c->partials, sizeof(double) * MAX_RED_BLOCKS * 8))) return fail(rc);
@@ -635,7 +637,7 @@ int afm_ns_project(afm_ns* s, double* u, double* v)
 {
 	CHECK_ARG(s && u && v, "null argument");
 	AFM_TRY(use(ns_ctx(s->s)));
-	return ns_project(s->s, u, v, -1);
+	return ns_project(s->s, u, v, -1, false);
 }
 
 int afm_ns_get_p(afm_ns* s, double* p)

@@ -140,6 +140,7 @@ struct Ctx {
 	double kt_ms[KT_KINDS] = {};
 	long kt_launches[KT_KINDS] = {};
 	int last_cg_iters = -1;   // the last Helmholtz CG's iterations (-1: none yet)
+	hipEvent_t cg_read_ev = nullptr;   // recorded after a CG's control-block read-back (op_cg2)
 	std::vector<void*> owned;
 	~Ctx();
 	int alloc(void** p, size_t bytes);

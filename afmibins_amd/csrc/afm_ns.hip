@@ -938,9 +938,17 @@ static StartPoly start_poly(double off)
 	return P;
 }
 
+// spec (optional): enqueues work that needs the converged solution, right
+// after the first read-back of the control blocks is enqueued; the host then
+// waits for that read-back only, so the work runs while the host decides.
+// *spec_ok tells whether the solve had converged at that first test (the
+// work is then valid; otherwise the CG went on and the work must be redone).
 int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* const b[2], double tol, int tol_mode,
-           int max_iters, int iters[2], double res[2], const AdvectInit* adv = nullptr, bool p32 = false)
+           int max_iters, int iters[2], double res[2], const AdvectInit* adv = nullptr, bool p32 = false,
+           int (*spec)(void*) = nullptr, void* spec_arg = nullptr, bool* spec_ok = nullptr)
 {
+	if (spec_ok) *spec_ok = false;
+	bool first_read = true;
 	double off, dg;
 	helm_coef(c, scale, off, dg);
 	CGArgs a{};
@@ -1031,8 +1039,18 @@ int op_cg2(Ctx* c, CGWork& w, double scale, double* const x[2], const double* co
 			AFM_LAUNCH_CHECK();
 		}
 		AFM_HIP(hipMemcpyAsync(&c->hctl[1], &c->ctl[1], 2 * sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
-		AFM_HIP(hipStreamSynchronize(c->stream));
-		bool all = c->hctl[1].done && c->hctl[2].done;
+		bool all;
+		if (spec && first_read) {
+			AFM_HIP(hipEventRecord(c->cg_read_ev, c->stream));
+			AFM_TRY(spec(spec_arg));
+			AFM_HIP(hipEventSynchronize(c->cg_read_ev));
+			all = c->hctl[1].done && c->hctl[2].done;
+			if (spec_ok) *spec_ok = all;
+		} else {
+			AFM_HIP(hipStreamSynchronize(c->stream));
+			all = c->hctl[1].done && c->hctl[2].done;
+		}
+		first_read = false;
 		if (all || enq >= max_iters) break;
 		batch = 2;
 	}
@@ -1080,18 +1098,33 @@ struct NS {
 
 // kind: the step kind whose start potential the solve uses and replaces
 // (0, 1), or -1 for a projection outside the time steps
-int ns_project(NS* s, double* u, double* v, int kind)
+// The projection's divergence and its sum for the mean in one pass (even row
+// lengths; returns 1 then), else the divergence alone (0)
+static int ns_div(NS* s, const double* u, const double* v)
 {
 	Ctx* c = s->c;
-	// the divergence and its sum for the mean in one pass (even row lengths)
-	bool sums = false;
 	if (c->nx % 2 == 0) {
 		hipLaunchKernelGGL(k_div_sum, dim3(rblocks(c->N)), dim3(256), 0, c->stream, u, v, s->phi, (int) c->nx,
 		                   (int) c->ny, c->n, &c->ctl[0], c->partials, s->tol_mode == 1 ? 0.0 : s->tol, s->max_cycles);
 		AFM_LAUNCH_CHECK();
-		sums = true;
-	} else
-		AFM_TRY(op_div(c, u, v, s->phi));
+		return 1;
+	}
+	AFM_TRY(op_div(c, u, v, s->phi));
+	return 0;
+}
+
+// div_ready: ns_div(s, u, v) has been enqueued since u, v last changed
+// (ns_step's velocity solve, which enqueues it while the host waits)
+int ns_project(NS* s, double* u, double* v, int kind, bool div_ready)
+{
+	Ctx* c = s->c;
+	bool sums = false;
+	if (div_ready) sums = c->nx % 2 == 0;
+	else {
+		const int r = ns_div(s, u, v);
+		if (r < 0) return r;
+		sums = r == 1;
+	}
 	int cyc = 0;
 	const double* sol = nullptr;
 	const bool warm = kind >= 0 && s->pot[kind];
@@ -1133,7 +1166,7 @@ int ns_create(Ctx* c, double mu, double rho, double dt, double tol, int tol_mode
 	s->phi_last = s->phi;
 	AFM_HIP(hipMemsetAsync(s->uo, 0, c->N * 8, c->stream));
 	AFM_HIP(hipMemsetAsync(s->vo, 0, c->N * 8, c->stream));
-	rc = ns_project(s, s->u, s->v, -1);     // ns/solver.h:71
+	rc = ns_project(s, s->u, s->v, -1, false);     // ns/solver.h:71
 	if (rc) { delete s; return rc; }
 	*out = s;
 	return 0;
@@ -1156,9 +1189,19 @@ int ns_step(NS* s, const double* fx, const double* fy)
 	const AdvectInit adv{fx, fy, s->uo, s->vo, s->dt, s->rho, hoff, hdg, s->half ? 0 : 1, neumann ? 1 : 0};
 	double* xs[2] = {s->u, s->v};
 	const double* bs[2] = {nullptr, nullptr};
-	// the mixed form stores the CG's search directions in FP32
-	AFM_TRY(op_cg2(c, s->cg, ss, xs, bs, s->tol, s->tol_mode, s->max_cg, s->last_cg, nullptr, &adv, s->precision == 1));
-	AFM_TRY(ns_project(s, s->u, s->v, s->half ? 1 : 0));
+	// the mixed form stores the CG's search directions in FP32; the
+	// projection's divergence is enqueued behind the CG's first read-back
+	// (valid when the solve had then converged: the start met the
+	// tolerance, or the predicted iterations sufficed)
+	auto spec = [](void* p) -> int {
+		NS* ns = (NS*) p;
+		const int r = ns_div(ns, ns->u, ns->v);
+		return r < 0 ? r : 0;
+	};
+	bool div_ready = false;
+	AFM_TRY(op_cg2(c, s->cg, ss, xs, bs, s->tol, s->tol_mode, s->max_cg, s->last_cg, nullptr, &adv, s->precision == 1,
+	               spec, s, &div_ready));
+	AFM_TRY(ns_project(s, s->u, s->v, s->half ? 1 : 0, div_ready));
 	s->half = !s->half;
 	return 0;
 }
@@ -1404,7 +1447,7 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 			AFM_TRY(adj_interp_dX(c, 1, vnx, tp.Xh, Uy, m, Xhb));
 			// step 6: Crank-Nicolson full step
 			AFM_TRY(d2d(c, us, ub, N)); AFM_TRY(d2d(c, vs, vb, N));
-			AFM_TRY(ns_project(s, us, vs, -1));
+			AFM_TRY(ns_project(s, us, vs, -1, false));
 			AFM_TRY(solve_H(rbu, rbv, us, vs));
 			AFM_TRY(op_helm_apply(c, sm, rbu, t1)); AFM_TRY(adj_axpy(c, unb, t1, 1.0, N));
 			AFM_TRY(op_helm_apply(c, sm, rbv, t2)); AFM_TRY(adj_axpy(c, vnb, t2, 1.0, N));
@@ -1428,7 +1471,7 @@ int ib_grad(IB* b, int steps, int every, long msize, double* J_out, double* gk, 
 			AFM_TRY(adj_interp_dX(c, 0, tp.uh, tp.X, Ux, m, Xnb));
 			AFM_TRY(adj_interp_dX(c, 1, tp.vh, tp.X, Uy, m, Xnb));
 			// step 3: backward-Euler half step
-			AFM_TRY(ns_project(s, uhb, vhb, -1));
+			AFM_TRY(ns_project(s, uhb, vhb, -1, false));
 			AFM_TRY(solve_H(rbu, rbv, uhb, vhb));
 			AFM_TRY(adj_axpy(c, unb, rbu, 1.0, N));
 			AFM_TRY(adj_axpy(c, vnb, rbv, 1.0, N));
